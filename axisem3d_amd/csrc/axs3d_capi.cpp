@@ -1,0 +1,204 @@
+// axs3d_capi.cpp -- extern "C" entry points declared in include/axs3d.h.
+#include <exception>
+#include <memory>
+#include <string>
+
+#include "axs3d.h"
+#include "axs3d_domain.hpp"
+
+struct axs3d_ctx {
+    std::unique_ptr<axs3d::DomainBase> dom;
+    std::string err;
+    int device = 0;
+};
+
+namespace {
+
+template <typename F>
+int guarded(axs3d_ctx *ctx, F &&f) {
+    if (!ctx) return 2;
+    try {
+        (void)hipSetDevice(ctx->device);
+        f(*ctx->dom);
+        ctx->err.clear();
+        return 0;
+    } catch (const std::exception &e) {
+        ctx->err = e.what();
+        return 1;
+    } catch (...) {
+        ctx->err = "axs3d || Unknown exception.";
+        return 1;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int axs3d_ctx_create(int device, int precision, axs3d_ctx **out) {
+    if (!out) return 2;
+    *out = nullptr;
+    try {
+        auto ctx = std::make_unique<axs3d_ctx>();
+        ctx->device = device;
+        ctx->dom = axs3d::make_domain(device, precision);
+        *out = ctx.release();
+        return 0;
+    } catch (...) {
+        return 1;
+    }
+}
+
+int axs3d_ctx_destroy(axs3d_ctx *ctx) {
+    if (!ctx) return 0;
+    (void)hipSetDevice(ctx->device);
+    delete ctx;
+    return 0;
+}
+
+const char *axs3d_last_error(const axs3d_ctx *ctx) {
+    return ctx ? ctx->err.c_str() : "axs3d || Null context.";
+}
+
+int axs3d_precision_of(const axs3d_ctx *ctx) {
+    return ctx ? ctx->dom->precision() : -1;
+}
+
+int axs3d_set_gmat(axs3d_ctx *ctx, const double *g_gll, const double *g_glj) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) { d.setGMat(g_gll, g_glj); });
+}
+
+int axs3d_set_solid_points(axs3d_ctx *ctx, int n, const int32_t *nr,
+                           const double *mass) {
+    return guarded(ctx,
+                   [&](axs3d::DomainBase &d) { d.setSolidPoints(n, nr, mass); });
+}
+
+int axs3d_set_solid_elements(axs3d_ctx *ctx, int n, const int32_t *points,
+                             const uint8_t *axial, const double *gq) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) {
+        d.setSolidElements(n, points, axial, gq);
+    });
+}
+
+int axs3d_set_solid_iso1d(axs3d_ctx *ctx, const double *lam_mu) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) { d.setSolidIso1D(lam_mu); });
+}
+
+int axs3d_set_axial_points(axs3d_ctx *ctx, int n, const int32_t *points) {
+    return guarded(ctx,
+                   [&](axs3d::DomainBase &d) { d.setAxialPoints(n, points); });
+}
+
+int axs3d_set_moment_source(axs3d_ctx *ctx, int elem, int nu1,
+                            const double *pattern, int nsteps,
+                            const double *stf) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) {
+        d.setMomentSource(elem, nu1, pattern, nsteps, stf);
+    });
+}
+
+int axs3d_set_receivers(axs3d_ctx *ctx, int n, const int32_t *elem,
+                        const double *weights, const double *phi,
+                        int max_samples) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) {
+        d.setReceivers(n, elem, weights, phi, max_samples);
+    });
+}
+
+int axs3d_set_dt(axs3d_ctx *ctx, double dt) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) { d.setDt(dt); });
+}
+
+int axs3d_finalize(axs3d_ctx *ctx) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) { d.finalize(); });
+}
+
+int axs3d_apply_sources(axs3d_ctx *ctx, int tstep) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) { d.applySources(tstep); });
+}
+
+int axs3d_compute_stiffness(axs3d_ctx *ctx) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) { d.computeStiffness(); });
+}
+
+int axs3d_apply_bc_before(axs3d_ctx *ctx) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) {
+        d.applyBC_BeforeAssemblingStiff();
+    });
+}
+
+int axs3d_record(axs3d_ctx *ctx, int isample) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) { d.recordOutput(isample); });
+}
+
+int axs3d_apply_bc_after(axs3d_ctx *ctx) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) {
+        d.applyBC_AfterAssemblingStiff();
+    });
+}
+
+int axs3d_stiff_to_accel(axs3d_ctx *ctx) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) { d.computeStiffToAccel(); });
+}
+
+int axs3d_newmark(axs3d_ctx *ctx) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) { d.newmark(); });
+}
+
+int axs3d_check_stability(axs3d_ctx *ctx, int tstep) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) { d.checkStability(tstep); });
+}
+
+int axs3d_run(axs3d_ctx *ctx, int step_begin, int nsteps) {
+    return guarded(ctx,
+                   [&](axs3d::DomainBase &d) { d.solve(step_begin, nsteps); });
+}
+
+int axs3d_set_record_origin(axs3d_ctx *ctx, int record_step0) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) {
+        d.setRecordOrigin(record_step0);
+    });
+}
+
+int axs3d_read_seismograms(axs3d_ctx *ctx, double *out, int nsamples) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) {
+        d.readSeismograms(out, nsamples);
+    });
+}
+
+int axs3d_num_modes(const axs3d_ctx *ctx, int64_t *nmodes) {
+    if (!ctx || !nmodes) return 2;
+    *nmodes = ctx->dom->numModes();
+    return 0;
+}
+
+int axs3d_get_field(axs3d_ctx *ctx, int which, double *out) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) { d.getField(which, out); });
+}
+
+int axs3d_set_field(axs3d_ctx *ctx, int which, const double *in) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) { d.setField(which, in); });
+}
+
+int axs3d_synchronize(axs3d_ctx *ctx) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) { d.synchronize(); });
+}
+
+int axs3d_profile_enable(axs3d_ctx *ctx, int enable) {
+    return guarded(ctx,
+                   [&](axs3d::DomainBase &d) { d.profileEnable(enable != 0); });
+}
+
+int axs3d_profile_read(axs3d_ctx *ctx, int kernel, double *total_ms,
+                       int64_t *count) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) {
+        d.profileRead(kernel, total_ms, count);
+    });
+}
+
+int axs3d_profile_reset(axs3d_ctx *ctx) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) { d.profileReset(); });
+}
+
+}  // extern "C"

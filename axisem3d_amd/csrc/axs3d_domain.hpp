@@ -1,0 +1,166 @@
+// axs3d_domain.hpp -- host C++ façade: Domain and NewmarkTimeScheme.
+//
+// Keeps the reference's phase structure (src = SOLVER/src):
+//   Domain::applySources / computeStiffness / applyBC_* / recordOutput /
+//   computeStiffToAccel / checkStability   src/core/domain/Domain.cpp:285-612
+//   NewmarkTimeScheme::solve / update      src/core/time/NewmarkTimeScheme.cpp:18-131,
+//                                          NewmarkTimeScheme.hpp:48-71
+// but the points and elements live in device memory as structure-of-arrays
+// instead of shared_ptr objects, and each phase is one kernel launch on the
+// domain's stream.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "axs3d.h"
+#include "axs3d_common.hpp"
+
+namespace axs3d {
+
+// "Class::method || detail" errors, as the reference throws them.
+struct Error : public std::runtime_error {
+    explicit Error(const std::string &m) : std::runtime_error(m) {}
+};
+
+void hip_check(hipError_t e, const char *where);
+
+// Owning device allocation.
+class DeviceBuffer {
+public:
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer &) = delete;
+    DeviceBuffer &operator=(const DeviceBuffer &) = delete;
+    ~DeviceBuffer() { reset(); }
+    void alloc(size_t bytes);
+    void upload(const void *src, size_t bytes, hipStream_t st);
+    void zero(hipStream_t st);
+    void reset();
+    template <typename T>
+    T *as() const { return static_cast<T *>(ptr_); }
+    size_t bytes() const { return bytes_; }
+
+private:
+    void *ptr_ = nullptr;
+    size_t bytes_ = 0;
+};
+
+class DomainBase {
+public:
+    virtual ~DomainBase();
+    // set-up
+    void setGMat(const double *g_gll, const double *g_glj);
+    void setSolidPoints(int n, const int32_t *nr, const double *mass);
+    void setSolidElements(int n, const int32_t *points, const uint8_t *axial,
+                          const double *gq);
+    void setSolidIso1D(const double *lam_mu);
+    void setAxialPoints(int n, const int32_t *points);
+    void setMomentSource(int elem, int nu1, const double *pattern, int nsteps,
+                         const double *stf);
+    void setReceivers(int n, const int32_t *elem, const double *w,
+                      const double *phi, int max_samples);
+    void setDt(double dt) { mDt = dt; }
+    void finalize();
+    bool finalized() const { return mFinalized; }
+
+    // time-loop phases (Domain.cpp:285-345)
+    virtual void applySources(int tstep) = 0;
+    virtual void computeStiffness() = 0;
+    void applyBC_BeforeAssemblingStiff() {}
+    void recordOutput(int isample);
+    virtual void applyBC_AfterAssemblingStiff() = 0;
+    virtual void computeStiffToAccel() = 0;
+    virtual void newmark() = 0;
+    virtual void fusedPointUpdate() = 0;
+    void checkStability(int tstep);
+
+    // bulk
+    void solve(int step_begin, int nsteps);
+    void setRecordOrigin(int step0) { mRecordStep0 = step0; }
+    void readSeismograms(double *out, int nsamples);
+
+    // debug
+    int64_t numModes() const { return mNumModes; }
+    virtual void getField(int which, double *out) = 0;
+    virtual void setField(int which, const double *in) = 0;
+    void synchronize();
+
+    // measurement
+    void profileEnable(bool on) { mProfile = on; }
+    void profileRead(int kernel, double *ms, int64_t *count);
+    void profileReset();
+
+    hipStream_t stream() const { return mStream; }
+    int precision() const { return mPrecision; }
+
+protected:
+    explicit DomainBase(int device, int precision);
+    void requireFinalized(const char *where) const;
+    // profiling brackets around one launch of `kernel`
+    void tick(int kernel);
+    void tock(int kernel);
+
+    int mDevice = 0;
+    int mPrecision = 0;
+    hipStream_t mStream = nullptr;
+    bool mFinalized = false;
+    double mDt = 0.;
+    int mRecordStep0 = 0;
+
+    // host staging until finalize
+    std::vector<double> hGll, hGlj;
+    std::vector<int32_t> hPointNr;
+    std::vector<double> hMass;
+    std::vector<int32_t> hElemPoints;
+    std::vector<uint8_t> hElemAxial;
+    std::vector<double> hElemGQ;
+    std::vector<double> hElemIso;
+    std::vector<int32_t> hAxial;
+    int hSrcElem = -1, hSrcNu1 = 0;
+    std::vector<double> hSrcPattern, hStf;
+    std::vector<int32_t> hRecElem;
+    std::vector<double> hRecW, hRecPhi;
+    int hMaxSamples = 0;
+
+    // sizes
+    int mNumPoints = 0, mNumElems = 0, mNumEM = 0, mNumRec = 0;
+    int64_t mNumModes = 0;
+    int mSrcNu1c = 0;
+    int mStfSteps = 0;
+
+    // device: points
+    DeviceBuffer dPointOff, dPointNu1, dPointNr, dPointAxial, dInvMass,
+        dModePoint;
+    // device: fields (FP64) and stiffness (Real)
+    DeviceBuffer dDispl, dVeloc, dAccel, dStiff;
+    // device: elements
+    DeviceBuffer dElemPoints, dElemAxial, dElemNu1, dElemOps, dElemMat,
+        dEMList, dGTab;
+    // device: source and receivers
+    DeviceBuffer dSrcPattern, dStf, dRecElem, dRecW, dRecPhi, dSeismo;
+    DeviceBuffer dFlag;
+
+    virtual void uploadReal() = 0;   // precision-specific operand upload
+
+    // profiling
+    bool mProfile = false;
+    struct Pending {
+        int kernel;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> mPending;
+    std::vector<hipEvent_t> mEventPool;
+    hipEvent_t mOpenA[AXS3D_NUM_KERNELS] = {};
+    double mProfMs[AXS3D_NUM_KERNELS] = {};
+    int64_t mProfCount[AXS3D_NUM_KERNELS] = {};
+    hipEvent_t takeEvent();
+    void drainPending();
+};
+
+std::unique_ptr<DomainBase> make_domain(int device, int precision);
+
+}  // namespace axs3d

@@ -1,0 +1,530 @@
+"""Preloop: synthetic meshes and the exact operands of the stiffness path.
+
+The reference builds its operands once, in FP64, from an Exodus mesh
+(``SOLVER/src/preloop``).  This module builds the same operands from
+seeded synthetic structured meshes (Cartesian boxes and spherical sectors),
+following the reference formulas:
+
+* element mapping and Jacobian: ``MappingLinear.hpp:23-58``,
+  ``MappingSpherical.hpp:35-86`` (curved outer edge = edge 2);
+* inverse-Jacobian entries and 1/s: ``Quad.hpp:157-184``;
+* integral factor with the axial L'Hospital rule: ``Quad.cpp:404-435``;
+* 1D material interpolated bilinearly from the element corners:
+  ``Material.cpp:26-60`` and ``spectrals.hpp:330-370``;
+* isotropic moduli lambda = rho (vp^2 - 2 vs^2), mu = rho vs^2:
+  ``Material.cpp:566-582``; mass = rho * IF assembled on points:
+  ``Material.cpp:308-346``, ``Quad.cpp:100-121``;
+* dt = courant * min_e hmin / vmax, rounded to 4 significant figures:
+  ``Quad.cpp:271-349``, ``SE_Model.cpp:178-204``;
+* nodal Nr -> GLL Nr (round, lucky numbers) -> element Nr = max of its
+  points: ``Quad.cpp:50-63``, ``Element.cpp:28-35``.
+
+Element point index: ``ipnt = ipol * 5 + jpol`` with xi along ipol.  On an
+axial element xi = -1 (ipol = 0) lies on the axis and uses GLJ nodes.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import Callable, Optional
+
+import numpy as np
+
+from . import spectral
+
+NPED = spectral.NPED
+NPEM = spectral.NPEM
+
+
+def round_half_away(x):
+    """std::round semantics (Eigen ``.round()``): halves away from zero."""
+    x = np.asarray(x, dtype=np.float64)
+    return np.sign(x) * np.floor(np.abs(x) + 0.5)
+
+
+# ---------------------------------------------------------------- geometry
+@dataclasses.dataclass
+class StructuredMesh:
+    """nx elements along xi (s or theta) by nz along eta (z or r).
+
+    ``node_a`` (nx+1) lateral node coordinates (s for Cartesian, theta in
+    radians for spherical); ``node_b`` (nz+1) vertical node coordinates
+    (z for Cartesian, r for spherical).  Element (ix, iz) has corner nodes
+    0:(ix,iz) 1:(ix+1,iz) 2:(ix+1,iz+1) 3:(ix,iz+1), i.e. xi along ix and
+    eta along iz, matching ``gNodeIPnt = {0, 20, 24, 4}``.
+    """
+    node_a: np.ndarray
+    node_b: np.ndarray
+    spherical: bool = False
+    fluid_rows: Optional[np.ndarray] = None   # bool per iz, True = fluid
+
+    @property
+    def nx(self):
+        return len(self.node_a) - 1
+
+    @property
+    def nz(self):
+        return len(self.node_b) - 1
+
+    @property
+    def nelem(self):
+        return self.nx * self.nz
+
+    def element_ix_iz(self):
+        e = np.arange(self.nelem)
+        return e % self.nx, e // self.nx
+
+    def is_axial(self):
+        ix, _ = self.element_ix_iz()
+        return (ix == 0) & (abs(self.node_a[0]) < 1e-12)
+
+    def nodal_sz(self):
+        """(E, 2, 4) nodal (s, z) of each element."""
+        ix, iz = self.element_ix_iz()
+        a0, a1 = self.node_a[ix], self.node_a[ix + 1]
+        b0, b1 = self.node_b[iz], self.node_b[iz + 1]
+        aa = np.stack([a0, a1, a1, a0], axis=1)
+        bb = np.stack([b0, b0, b1, b1], axis=1)
+        if self.spherical:
+            s = bb * np.sin(aa)
+            z = bb * np.cos(aa)
+        else:
+            s, z = aa, bb
+        return np.stack([s, z], axis=1)
+
+    def map_points(self, xieta, axial):
+        """Mapping and Jacobian at reference coords.
+
+        xieta: (2, 25).  Returns sz (E, 2, 25) and J (E, 25, 2, 2) with
+        J = d(s,z)/d(xi,eta).
+        """
+        ix, iz = self.element_ix_iz()
+        xi = xieta[0][None, :]
+        eta = xieta[1][None, :]
+        if not self.spherical:
+            # MappingLinear: bilinear shape functions on the 4 nodes
+            nsz = self.nodal_sz()
+            xip, xim, etp, etm = 1 + xi, 1 - xi, 1 + eta, 1 - eta
+            shp = np.stack([xim * etm, xip * etm, xip * etp, xim * etp],
+                           axis=1) * 0.25  # (1,4,25)
+            sz = np.einsum('eij,xjp->eip', nsz, shp)
+            d_xi = np.stack([-etm, etm, etp, -etp], axis=1) * 0.25
+            d_eta = np.stack([-xim, -xip, xip, xim], axis=1) * 0.25
+            J = np.empty((self.nelem, NPEM, 2, 2))
+            dxi = np.einsum('eij,xjp->eip', nsz, d_xi)
+            deta = np.einsum('eij,xjp->eip', nsz, d_eta)
+            J[:, :, 0, 0] = dxi[:, 0]
+            J[:, :, 1, 0] = dxi[:, 1]
+            J[:, :, 0, 1] = deta[:, 0]
+            J[:, :, 1, 1] = deta[:, 1]
+            return sz, J
+        # MappingSpherical with curved outer edge 2 (identity rotation):
+        # t0 = t3 = theta_ix, t1 = t2 = theta_ix+1, r0 = r_iz, r2 = r_iz+1
+        t0 = self.node_a[ix][:, None]
+        t1 = self.node_a[ix + 1][:, None]
+        t2, t3 = t1, t0
+        r0 = self.node_b[iz][:, None]
+        r2 = self.node_b[iz + 1][:, None]
+        t01 = ((1. - xi) * t0 + (1. + xi) * t1) * .5
+        t32 = ((1. - xi) * t3 + (1. + xi) * t2) * .5
+        r0m = (1. - eta) * r0 * .5
+        r2p = (1. + eta) * r2 * .5
+        sz = np.stack([r0m * np.sin(t01) + r2p * np.sin(t32),
+                       r0m * np.cos(t01) + r2p * np.cos(t32)], axis=1)
+        t01_xi = (t1 - t0) * .5
+        t32_xi = (t2 - t3) * .5
+        r0m_eta = -r0 * .5
+        r2p_eta = r2 * .5
+        J = np.empty((self.nelem, NPEM, 2, 2))
+        J[:, :, 0, 0] = r0m * np.cos(t01) * t01_xi + r2p * np.cos(t32) * t32_xi
+        J[:, :, 1, 0] = -(r0m * np.sin(t01) * t01_xi +
+                          r2p * np.sin(t32) * t32_xi)
+        J[:, :, 0, 1] = r0m_eta * np.sin(t01) + r2p_eta * np.sin(t32)
+        J[:, :, 1, 1] = r0m_eta * np.cos(t01) + r2p_eta * np.cos(t32)
+        return sz, J
+
+    def element_points(self):
+        """(E, 25) global point index on the structured GLL grid.
+
+        Point id = gz * (4 nx + 1) + gs with gs = 4 ix + ipol and
+        gz = 4 iz + jpol.
+        """
+        ix, iz = self.element_ix_iz()
+        ipol = np.repeat(np.arange(NPED), NPED)
+        jpol = np.tile(np.arange(NPED), NPED)
+        gs = 4 * ix[:, None] + ipol[None, :]
+        gz = 4 * iz[:, None] + jpol[None, :]
+        return (gz * (4 * self.nx + 1) + gs).astype(np.int64)
+
+    @property
+    def npoint_grid(self):
+        return (4 * self.nx + 1) * (4 * self.nz + 1)
+
+    def edge_points(self, edge):
+        """Element point indices on an edge (``vicinity.hpp:52-56``)."""
+        return {0: [0, 5, 10, 15, 20], 1: [20, 21, 22, 23, 24],
+                2: [24, 19, 14, 9, 4], 3: [4, 3, 2, 1, 0]}[edge]
+
+
+# ------------------------------------------------------------ GradQuad ops
+def gradquad_operands(mesh: StructuredMesh):
+    """Per-element GradQuad operands, each (E, 25) float64.
+
+    Returns dict with dsdxii, dsdeta, dzdxii, dzdeta, inv_s, ifact, sz, J.
+    Names follow ``Quad::createGradient`` (``Quad.hpp:166-178``): they are
+    inverse-Jacobian entries named after the Jacobian entries.
+    """
+    axial = mesh.is_axial()
+    E = mesh.nelem
+    out = {k: np.empty((E, NPEM)) for k in
+           ("dsdxii", "dsdeta", "dzdxii", "dzdeta", "inv_s", "ifact")}
+    sz_all = np.empty((E, 2, NPEM))
+    J_all = np.empty((E, NPEM, 2, 2))
+    for ax in (False, True):
+        sel = np.nonzero(axial == ax)[0]
+        if len(sel) == 0:
+            continue
+        xieta = spectral.xieta_element(ax)
+        w = spectral.weights_element(ax)
+        sz, J = mesh.map_points(xieta, ax)
+        sz, J = sz[sel], J[sel]
+        det = J[..., 0, 0] * J[..., 1, 1] - J[..., 0, 1] * J[..., 1, 0]
+        out["dsdxii"][sel] = J[..., 0, 0] / det
+        out["dsdeta"][sel] = -J[..., 0, 1] / det
+        out["dzdxii"][sel] = -J[..., 1, 0] / det
+        out["dzdeta"][sel] = J[..., 1, 1] / det
+        s = sz[:, 0, :]
+        ifact = w[None, :] * det
+        if ax:
+            on_axis = (np.arange(NPEM) // NPED) == 0
+            inv_s = np.where(on_axis[None, :], 0.0,
+                             1.0 / np.where(on_axis[None, :], 1.0, s))
+            one_xi = np.where(on_axis, 1.0, 1.0 + xieta[0])[None, :]
+            ifact = np.where(on_axis[None, :], ifact * J[..., 0, 0],
+                             ifact * s / one_xi)
+        else:
+            inv_s = 1.0 / s
+            ifact = ifact * s
+        out["inv_s"][sel] = inv_s
+        out["ifact"][sel] = ifact
+        sz_all[sel] = sz
+        J_all[sel] = J
+    out["sz"] = sz_all
+    out["J"] = J_all
+    return out
+
+
+def interpolate_gll(nodal, axial):
+    """Bilinear interpolation of nodal values (E, 4) to the 25 GLL points
+    (``spectrals::interpolateGLL``, ``spectrals.hpp:330-370``)."""
+    nodal = np.asarray(nodal, dtype=np.float64)
+    out = np.empty((nodal.shape[0], NPEM))
+    for ax in (False, True):
+        sel = np.nonzero(axial == ax)[0]
+        if len(sel) == 0:
+            continue
+        xi, eta = spectral.xieta_element(ax)
+        lsf = np.stack([(1 - xi) * (1 - eta), (1 + xi) * (1 - eta),
+                        (1 + xi) * (1 + eta), (1 - xi) * (1 + eta)]) / 4.0
+        out[sel] = nodal[sel] @ lsf
+    return out
+
+
+def compute_dt(sz, vmax, courant, factor=None):
+    """dt = courant * min_e (hmin_e / vmax_e) * factor_e, then rounded to
+    4 significant figures (``Quad.cpp:271-349``, ``SE_Model.cpp:198-203``).
+
+    sz: (E, 2, 25) point coords; vmax: (E,) max velocity per element.
+    """
+    ipol = np.repeat(np.arange(NPED), NPED)
+    jpol = np.tile(np.arange(NPED), NPED)
+    hmin = np.full(sz.shape[0], np.inf)
+    for p0 in range(NPEM - 1):
+        for p1 in range(p0 + 1, NPEM):
+            if abs(ipol[p1] - ipol[p0]) <= 1 and abs(jpol[p1] - jpol[p0]) <= 1:
+                d = np.hypot(sz[:, 0, p1] - sz[:, 0, p0],
+                             sz[:, 1, p1] - sz[:, 1, p0])
+                hmin = np.minimum(hmin, d)
+    dt_e = hmin / vmax
+    if factor is not None:
+        dt_e = dt_e * factor
+    dt = float(np.min(dt_e) * courant)
+    return float("%.3e" % dt)
+
+
+# ------------------------------------------------------------------ source
+def ricker_stf(t, half_duration, decay_factor, time_shift=0.0):
+    """GaussianSTF, derivative order 2 (``GaussianSTF.hpp:35-58``), FP64."""
+    f = decay_factor / half_duration
+    tt = np.asarray(t, dtype=np.float64) - time_shift
+    ft = f * tt
+    return (1.0 / math.sqrt(math.pi)) * f ** 3 * np.exp(-ft * ft) * (
+        4.0 * ft * ft - 2.0)
+
+
+def moment_pattern_axial(mt6, inplane, nu1_elem):
+    """Fourier pattern of an axial moment-tensor source in an axial element.
+
+    mt6 = [Mzz, Mss, Mpp, Mzs, Mzp, Msp] (``Mechanism.cpp:131-145``);
+    inplane: (25,) interpolation weights.  Returns complex (nu1, 6, 25) with
+    Voigt order [ss, pp, zz, pz, sz, sp] (``Mechanism.cpp:160-218``).
+    """
+    nu1 = min(nu1_elem, 3)
+    mzz, mss, mpp, mzs, mzp, msp = [float(v) for v in mt6]
+    pat = np.zeros((nu1, 6, NPEM), dtype=np.complex128)
+    row0 = slice(0, NPED)   # ipol == 0 points, on the axis
+    w = inplane[row0]
+    pat[0, 0, row0] = (mss * w + mpp * w) / 2.0
+    pat[0, 1, row0] = (mss * w + mpp * w) / 2.0
+    pat[0, 2, row0] = mzz * w
+    if nu1 >= 2:
+        pat[1, 4, row0] = (mzs * w - 1j * mzp * w) / 2.0
+        pat[1, 3, row0] = 1j * pat[1, 4, row0]
+    if nu1 >= 3:
+        pat[2, 0, row0] = (mss * w - mpp * w - 2.0j * msp * w) / 4.0
+        pat[2, 1, row0] = -pat[2, 0, row0]
+        pat[2, 5, row0] = 1j * pat[2, 0, row0]
+    pat /= 2.0 * math.pi
+    return pat
+
+
+def moment_pattern_offaxis(mt6_zsp, inplane, nu1_elem, phi):
+    """Pattern of a non-axial source (``Mechanism.cpp:146-160``); mt6_zsp is
+    already rotated into (z, s, phi)."""
+    mzz, mss, mpp, mzs, mzp, msp = [float(v) for v in mt6_zsp]
+    pat = np.zeros((nu1_elem, 6, NPEM), dtype=np.complex128)
+    for a in range(nu1_elem):
+        e = np.exp(-1j * a * phi)
+        for k, m in enumerate([mss, mpp, mzz, mzp, mzs, msp]):
+            pat[a, k] = e * m * inplane
+    return pat
+
+
+# --------------------------------------------------------------- the spec
+@dataclasses.dataclass
+class DomainSpec:
+    """All operands of the hot path, FP64, in the layout the C ABI takes."""
+    # points
+    solid_nr: np.ndarray                 # int32 (Us,)
+    solid_mass: np.ndarray               # float64 (Us,)
+    solid_sz: np.ndarray                 # float64 (Us, 2)
+    # solid elements
+    elem_points: np.ndarray              # int32 (Es, 25) -> solid point
+    elem_axial: np.ndarray               # uint8 (Es,)
+    elem_gq: np.ndarray                  # float64 (Es, 6, 25)
+    elem_nr: np.ndarray                  # int32 (Es,)
+    # material, isotropic 1D: (Es, 2, 25) [lambda, mu]
+    elem_iso: np.ndarray
+    # boundaries
+    axial_points: np.ndarray             # int32 solid points on the axis
+    # source
+    src_elem: int = -1
+    src_pattern: Optional[np.ndarray] = None   # complex (nu1, 6, 25)
+    stf: Optional[np.ndarray] = None           # float64 (nsteps,)
+    # receivers
+    rec_elem: Optional[np.ndarray] = None      # int32 (R,)
+    rec_weights: Optional[np.ndarray] = None   # float64 (R, 25)
+    rec_phi: Optional[np.ndarray] = None       # float64 (R,)
+    # time
+    t0: float = 0.0
+    dt: float = 0.0
+    nsteps: int = 0
+    name: str = ""
+
+    @property
+    def n_solid_points(self):
+        return len(self.solid_nr)
+
+    @property
+    def n_solid_elements(self):
+        return self.elem_points.shape[0]
+
+    def n_updates_per_step(self):
+        """N_upd = sum over points of (Nr/2 + 1)."""
+        return int(np.sum(self.solid_nr // 2 + 1))
+
+
+def _compact_points(elem_points_grid, npoint_grid):
+    used = np.zeros(npoint_grid, dtype=bool)
+    used[elem_points_grid.reshape(-1)] = True
+    remap = -np.ones(npoint_grid, dtype=np.int64)
+    remap[used] = np.arange(int(used.sum()))
+    return remap, used
+
+
+def build_domain(mesh: StructuredMesh,
+                 rho_vp_vs: Callable,
+                 nodal_nr: Callable,
+                 courant: float = 0.6,
+                 lucky: bool = True):
+    """Points, solid elements, 1D isotropic material, mass, dt.
+
+    rho_vp_vs(r) -> (rho, vp, vs) arrays evaluated at the element corners'
+    radial coordinate (z for Cartesian, |(s,z)| for spherical);
+    nodal_nr(s, z) -> Nr at element corners.
+    """
+    axial = mesh.is_axial()
+    E = mesh.nelem
+    gq = gradquad_operands(mesh)
+    nsz = mesh.nodal_sz()
+    rcorner = (np.hypot(nsz[:, 0], nsz[:, 1]) if mesh.spherical
+               else nsz[:, 1])
+    rho_n, vp_n, vs_n = rho_vp_vs(rcorner)
+    rho = interpolate_gll(rho_n, axial)
+    vp = interpolate_gll(vp_n, axial)
+    vs = interpolate_gll(vs_n, axial)
+    lam = rho * vp ** 2 - 2.0 * (rho * vs ** 2)
+    mu = rho * vs ** 2
+
+    # points
+    ep_grid = mesh.element_points()
+    remap, used = _compact_points(ep_grid, mesh.npoint_grid)
+    elem_points = remap[ep_grid].astype(np.int32)
+    U = int(used.sum())
+    mass = np.zeros(U)
+    np.add.at(mass, elem_points.reshape(-1), (rho * gq["ifact"]).reshape(-1))
+    sz_pts = np.zeros((U, 2))
+    sz_pts[elem_points.reshape(-1)] = gq["sz"].transpose(0, 2, 1).reshape(-1, 2)
+
+    # Nr: nodal -> GLL (round half away) -> lucky -> element max
+    nr_nodal = nodal_nr(nsz[:, 0], nsz[:, 1])
+    nr_gll = round_half_away(interpolate_gll(nr_nodal, axial)).astype(np.int64)
+    if lucky:
+        uniq = np.unique(nr_gll)
+        lut = {int(n): spectral.next_lucky_number(int(n)) for n in uniq}
+        nr_gll = np.vectorize(lambda n: lut[int(n)])(nr_gll)
+    point_nr = np.zeros(U, dtype=np.int64)
+    point_nr[elem_points.reshape(-1)] = nr_gll.reshape(-1)
+    elem_nr = point_nr[elem_points].max(axis=1)
+
+    # axial points: ipol == 0 points of axial elements
+    ax_e = np.nonzero(axial)[0]
+    axial_points = np.unique(elem_points[ax_e][:, :NPED].reshape(-1))
+
+    # dt
+    vmax = vp.max(axis=1)
+    dt = compute_dt(gq["sz"], vmax, courant)
+
+    elem_gq = np.stack([gq["dsdxii"], gq["dsdeta"], gq["dzdxii"],
+                        gq["dzdeta"], gq["inv_s"], gq["ifact"]], axis=1)
+    spec = DomainSpec(
+        solid_nr=point_nr.astype(np.int32), solid_mass=mass, solid_sz=sz_pts,
+        elem_points=elem_points, elem_axial=axial.astype(np.uint8),
+        elem_gq=np.ascontiguousarray(elem_gq),
+        elem_nr=elem_nr.astype(np.int32),
+        elem_iso=np.ascontiguousarray(np.stack([lam, mu], axis=1)),
+        axial_points=axial_points.astype(np.int32), dt=dt)
+    spec._mesh = mesh
+    spec._gq = gq
+    return spec
+
+
+def locate(mesh: StructuredMesh, s, z):
+    """Element and (xi, eta) of a point (s, z); first element in index order
+    whose closed cell contains it."""
+    if mesh.spherical:
+        a = math.atan2(s, z)
+        b = math.hypot(s, z)
+    else:
+        a, b = s, z
+    na, nb = mesh.node_a, mesh.node_b
+    tol = 1e-9 * max(abs(na[-1] - na[0]), abs(nb[-1] - nb[0]))
+    ix = int(np.clip(np.searchsorted(na, a - tol, side="right") - 1,
+                     0, mesh.nx - 1))
+    iz = int(np.clip(np.searchsorted(nb, b - tol, side="right") - 1,
+                     0, mesh.nz - 1))
+    xi = 2.0 * (a - na[ix]) / (na[ix + 1] - na[ix]) - 1.0
+    eta = 2.0 * (b - nb[iz]) / (nb[iz + 1] - nb[iz]) - 1.0
+    xi = float(np.clip(xi, -1.0, 1.0))
+    eta = float(np.clip(eta, -1.0, 1.0))
+    return iz * mesh.nx + ix, xi, eta
+
+
+def inplane_factor(mesh, elem, xi, eta):
+    """Lagrange weights of (xi, eta) at the 25 element points
+    (``SE_Model::computeInplaneFactor``, ``SE_Model.cpp:604-618``)."""
+    axial = bool(mesh.is_axial()[elem])
+    wx = spectral.interp_lagrange(xi, spectral.positions(axial))
+    we = spectral.interp_lagrange(eta, spectral.positions(False))
+    return np.outer(wx, we).reshape(-1)
+
+
+def time_axis(t0, dt, nsteps):
+    """Times of the steps as the solver accumulates them: t = t0, then
+    t += dt after every step (``NewmarkTimeScheme.cpp:41,107``)."""
+    out = np.empty(nsteps)
+    t = float(t0)
+    for n in range(nsteps):
+        out[n] = t
+        t += dt
+    return out
+
+
+def add_axial_moment_source(spec: DomainSpec, depth_z: float, mt6,
+                            half_duration, decay_factor, nsteps,
+                            time_shift=0.0):
+    """Axial moment source at (s=0, z) with a Ricker STF; sets the time axis
+    t0 = floor(t_start / dt) dt (``main.cpp:574-602``); hdur >= 5 dt
+    (``STF.cpp:32``)."""
+    mesh = spec._mesh
+    elem, xi, eta = locate(mesh, 0.0, depth_z)
+    w = inplane_factor(mesh, elem, xi, eta)
+    nu1 = int(spec.elem_nr[elem]) // 2 + 1
+    spec.src_elem = int(elem)
+    spec.src_pattern = moment_pattern_axial(mt6, w, nu1)
+    hdur = max(half_duration, spec.dt * 5.0)
+    t_start = -hdur * 2.5 + time_shift
+    step0 = int(math.floor(t_start / spec.dt))
+    spec.t0 = step0 * spec.dt
+    spec.nsteps = int(nsteps)
+    spec.stf = ricker_stf(time_axis(spec.t0, spec.dt, nsteps), hdur,
+                          decay_factor, time_shift)
+    spec._stf_params = (hdur, decay_factor, time_shift)
+    return spec
+
+
+def add_receivers(spec: DomainSpec, s_list, z_list, phi_list):
+    mesh = spec._mesh
+    elems, ws = [], []
+    for s, z in zip(s_list, z_list):
+        e, xi, eta = locate(mesh, s, z)
+        elems.append(e)
+        ws.append(inplane_factor(mesh, e, xi, eta))
+    spec.rec_elem = np.array(elems, dtype=np.int32)
+    spec.rec_weights = np.array(ws, dtype=np.float64)
+    spec.rec_phi = np.asarray(phi_list, dtype=np.float64)
+    return spec
+
+
+# ----------------------------------------------------------- workloads
+def cartesian_homogeneous(nx=80, nz=100, width=80e3, depth=80e3, nu=2,
+                          vp=6000.0, vs=3464.0, rho=2600.0, nsteps=3000,
+                          src_depth=10e3, n_receivers=10, seed=0,
+                          nodal_nr=None, m0=1e16):
+    """Cartesian box, homogeneous isotropic solid, axial moment source with
+    a 2 Hz Ricker, surface receivers (the 8000-element nu=2 workload)."""
+    mesh = StructuredMesh(np.linspace(0.0, width, nx + 1),
+                          np.linspace(0.0, depth, nz + 1))
+    nr = 2 * nu + 1
+
+    def material(r):
+        one = np.ones_like(r)
+        return rho * one, vp * one, vs * one
+
+    if nodal_nr is None:
+        nodal_nr = lambda s, z: np.full(s.shape, nr)
+    spec = build_domain(mesh, material, nodal_nr)
+    rng = np.random.default_rng(seed)
+    mt = rng.standard_normal(6)
+    mt *= m0 / np.linalg.norm(mt)
+    f = 2.0 * math.pi * 1.0     # exp(-f^2 t^2) with f = pi * 2 Hz
+    decay = 1.628
+    add_axial_moment_source(spec, depth - src_depth, mt, decay / f, decay,
+                            nsteps)
+    rng1 = np.random.default_rng(seed + 1)
+    s_rec = np.linspace(width / n_receivers, width, n_receivers)
+    phi = rng1.uniform(0.0, 2.0 * math.pi, n_receivers)
+    add_receivers(spec, s_rec, np.full(n_receivers, depth), phi)
+    spec.name = f"cartesian_iso1d_E{mesh.nelem}_nu{nu}"
+    spec._mt = mt
+    return spec

@@ -1,0 +1,168 @@
+"""Host driver of the HIP path: uploads a ``DomainSpec`` through the C ABI
+and runs the Newmark loop on the device.
+
+The phase methods keep the names of the reference's ``Domain`` and
+``NewmarkTimeScheme`` (``src/core/domain/Domain.cpp:285-345``,
+``src/core/time/NewmarkTimeScheme.cpp:18-131``) so stage-level parity tests
+read like the reference's loop.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import capi, spectral
+
+PRECISIONS = {"fp64": capi.AXS3D_FP64, "fp32": capi.AXS3D_FP32_FP64}
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+class GpuSolver:
+    def __init__(self, spec, precision="fp64", device=0, max_samples=None):
+        self.lib = capi.load()
+        self.spec = spec
+        self.precision = precision
+        ctx = C.c_void_p()
+        rc = self.lib.axs3d_ctx_create(device, PRECISIONS[precision],
+                                       C.byref(ctx))
+        if rc != 0 or not ctx.value:
+            raise capi.Axs3dError("axs3d_ctx_create failed (no HIP device "
+                                  "or invalid precision)")
+        self.ctx = ctx
+        self._upload(max_samples)
+        n = C.c_int64()
+        self._chk(self.lib.axs3d_num_modes(self.ctx, C.byref(n)))
+        self.nmodes = n.value
+        self.step = 0
+
+    # ------------------------------------------------------------------
+    def _chk(self, rc, what=""):
+        capi.check(self.ctx, rc, what)
+
+    def _upload(self, max_samples):
+        L, s, ctx = self.lib, self.spec, self.ctx
+        t = spectral.tables()
+        g_gll, g_glj = _c(t[4], np.float64), _c(t[5], np.float64)
+        self._chk(L.axs3d_set_gmat(ctx, g_gll.ctypes.data,
+                                   g_glj.ctypes.data), "set_gmat")
+        nr = _c(s.solid_nr, np.int32)
+        mass = _c(s.solid_mass, np.float64)
+        self._chk(L.axs3d_set_solid_points(ctx, len(nr), nr.ctypes.data,
+                                           mass.ctypes.data), "set_points")
+        ep = _c(s.elem_points, np.int32)
+        ax = _c(s.elem_axial, np.uint8)
+        gq = _c(s.elem_gq, np.float64)
+        self._chk(L.axs3d_set_solid_elements(ctx, ep.shape[0], ep.ctypes.data,
+                                             ax.ctypes.data, gq.ctypes.data),
+                  "set_elements")
+        iso = _c(s.elem_iso, np.float64)
+        self._chk(L.axs3d_set_solid_iso1d(ctx, iso.ctypes.data), "set_iso1d")
+        axp = _c(s.axial_points, np.int32)
+        self._chk(L.axs3d_set_axial_points(ctx, len(axp), axp.ctypes.data),
+                  "set_axial")
+        if s.src_elem >= 0 and s.src_pattern is not None:
+            pat = _c(s.src_pattern, np.complex128)
+            stf = _c(s.stf, np.float64)
+            self._chk(L.axs3d_set_moment_source(
+                ctx, int(s.src_elem), pat.shape[0], pat.ctypes.data,
+                len(stf), stf.ctypes.data), "set_source")
+        self.nrec = 0 if s.rec_elem is None else len(s.rec_elem)
+        if self.nrec:
+            ms = int(max_samples if max_samples is not None else
+                     max(s.nsteps, 1))
+            self.max_samples = ms
+            re = _c(s.rec_elem, np.int32)
+            rw = _c(s.rec_weights, np.float64)
+            rp = _c(s.rec_phi, np.float64)
+            self._chk(L.axs3d_set_receivers(ctx, self.nrec, re.ctypes.data,
+                                            rw.ctypes.data, rp.ctypes.data,
+                                            ms), "set_receivers")
+        self._chk(L.axs3d_set_dt(ctx, float(s.dt)), "set_dt")
+        self._chk(L.axs3d_finalize(ctx), "finalize")
+
+    def close(self):
+        if getattr(self, "ctx", None) is not None and self.ctx.value:
+            self.lib.axs3d_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------- bulk loop
+    def run(self, nsteps):
+        self._chk(self.lib.axs3d_run(self.ctx, self.step, int(nsteps)), "run")
+        self.step += nsteps
+
+    def seismograms(self, nsamples=None):
+        n = self.step if nsamples is None else nsamples
+        out = np.zeros((n, max(self.nrec, 1), 3))
+        if self.nrec and n:
+            self._chk(self.lib.axs3d_read_seismograms(self.ctx,
+                                                      out.ctypes.data, n))
+        return out[:, :self.nrec]
+
+    # ------------------------------------------------------- phases
+    def applySources(self, tstep):
+        self._chk(self.lib.axs3d_apply_sources(self.ctx, tstep))
+
+    def computeStiffness(self):
+        self._chk(self.lib.axs3d_compute_stiffness(self.ctx))
+
+    def applyBC_BeforeAssemblingStiff(self):
+        self._chk(self.lib.axs3d_apply_bc_before(self.ctx))
+
+    def recordOutput(self, isample):
+        self._chk(self.lib.axs3d_record(self.ctx, isample))
+
+    def applyBC_AfterAssemblingStiff(self):
+        self._chk(self.lib.axs3d_apply_bc_after(self.ctx))
+
+    def computeStiffToAccel(self):
+        self._chk(self.lib.axs3d_stiff_to_accel(self.ctx))
+
+    def newmark(self):
+        self._chk(self.lib.axs3d_newmark(self.ctx))
+
+    def checkStability(self, tstep=0):
+        self._chk(self.lib.axs3d_check_stability(self.ctx, tstep))
+
+    def synchronize(self):
+        self._chk(self.lib.axs3d_synchronize(self.ctx))
+
+    # ------------------------------------------------------- fields
+    _FIELDS = {"displ": capi.FIELD_DISPL, "veloc": capi.FIELD_VELOC,
+               "accel": capi.FIELD_ACCEL, "stiff": capi.FIELD_STIFF}
+
+    def get_field(self, name):
+        out = np.zeros((self.nmodes, 3), dtype=np.complex128)
+        self._chk(self.lib.axs3d_get_field(self.ctx, self._FIELDS[name],
+                                           out.ctypes.data))
+        return out
+
+    def set_field(self, name, values):
+        v = _c(values, np.complex128)
+        if v.shape != (self.nmodes, 3):
+            raise ValueError("field shape mismatch")
+        self._chk(self.lib.axs3d_set_field(self.ctx, self._FIELDS[name],
+                                           v.ctypes.data))
+
+    # ------------------------------------------------------- profiling
+    def profile(self, enable=True):
+        self._chk(self.lib.axs3d_profile_enable(self.ctx, int(enable)))
+
+    def profile_reset(self):
+        self._chk(self.lib.axs3d_profile_reset(self.ctx))
+
+    def profile_read(self, kernel):
+        ms = C.c_double()
+        cnt = C.c_int64()
+        self._chk(self.lib.axs3d_profile_read(self.ctx, kernel, C.byref(ms),
+                                              C.byref(cnt)))
+        return ms.value, cnt.value

@@ -1,0 +1,175 @@
+/*
+ * axs3d.h -- C ABI of the MI355X-native AxiSEM3D stiffness path.
+ *
+ * The reference has no FFI: its boundary is the C++ virtual interface of
+ * Domain / Element / Point / TimeScheme (src = SOLVER/src):
+ *   TimeScheme::solve()                     src/core/time/TimeScheme.hpp:66
+ *   Domain phase methods                    src/core/domain/Domain.hpp:158-207
+ *   SolidElement::computeStiff()            src/core/element/SolidElement.cpp:213-222
+ *   Elastic::strainToStress_FR/_CD          src/core/element/material/elastic/Elastic.hpp:63-71
+ *   Mass::computeAccel                      src/core/point/mass/Mass.hpp:26-30
+ *   NewmarkTimeScheme::update               src/core/time/NewmarkTimeScheme.hpp:48-71
+ * Each entry point below names the reference function it replaces.  The
+ * operands are the ones the reference preloop hands to those classes
+ * (SE_Model::release, src/preloop/se_model/SE_Model.cpp:207-243), passed
+ * as plain host arrays; the library copies them to device memory.
+ *
+ * Conventions
+ *  - Every call returns 0 on success, non-zero on error; the message, in
+ *    the reference's "Class::method || detail" format, is available from
+ *    axs3d_last_error().
+ *  - One host thread per context.  A context owns its HIP stream and all
+ *    its device memory; host arrays may be freed after each call returns.
+ *  - Complex arrays are interleaved (re, im) doubles.
+ *  - Point fields are [mode][3] complex per point, points concatenated in
+ *    index order, mode count Nr/2+1 per point (SolidPoint::Fields,
+ *    src/core/point/SolidPoint.hpp:115-121).
+ *  - Element operands are [element][k][25], ipnt = ipol * 5 + jpol.
+ *  - Precision: AXS3D_FP64 computes everything in FP64 (the reference
+ *    built with -Ddouble=true); AXS3D_FP32_FP64 computes gradient,
+ *    stress and quadrature in FP32 and keeps displacement, velocity,
+ *    acceleration, mass division, source-time function and receivers in
+ *    FP64.
+ */
+#ifndef AXS3D_H
+#define AXS3D_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct axs3d_ctx axs3d_ctx;
+
+enum axs3d_precision { AXS3D_FP64 = 0, AXS3D_FP32_FP64 = 1 };
+
+enum axs3d_field {
+    AXS3D_FIELD_DISPL = 0,
+    AXS3D_FIELD_VELOC = 1,
+    AXS3D_FIELD_ACCEL = 2,
+    AXS3D_FIELD_STIFF = 3
+};
+
+enum axs3d_kernel_id {
+    AXS3D_KERNEL_STIFFNESS = 0,   /* all element stiffness launches      */
+    AXS3D_KERNEL_POINT = 1,       /* fused BC/mass/Newmark point update  */
+    AXS3D_KERNEL_SOURCE = 2,
+    AXS3D_KERNEL_RECORD = 3,
+    AXS3D_NUM_KERNELS = 4
+};
+
+/* Context.  Replaces the construction of Domain + NewmarkTimeScheme in
+ * main() (src/main.cpp:503-548). */
+int axs3d_ctx_create(int device, int precision, axs3d_ctx **out);
+int axs3d_ctx_destroy(axs3d_ctx *ctx);
+const char *axs3d_last_error(const axs3d_ctx *ctx);
+int axs3d_precision_of(const axs3d_ctx *ctx);
+
+/* GLL and GLJ derivative matrices, G[i*5+j] = l_i'(x_j).
+ * Replaces GradientQuadrature::setGMat (GradientQuadrature.hpp:510-516). */
+int axs3d_set_gmat(axs3d_ctx *ctx, const double *g_gll, const double *g_glj);
+
+/* Solid GLL points: Nr per point and assembled 1D mass.  Replaces
+ * GLLPoint::release for solid points (src/preloop/se_model/gll/
+ * GLLPoint.cpp:40-77) with Mass1D (src/core/point/mass/Mass1D.hpp:19-22). */
+int axs3d_set_solid_points(axs3d_ctx *ctx, int n, const int32_t *nr,
+                           const double *mass);
+
+/* Solid elements: point indices [n][25], axial flags and the GradQuad
+ * operands gq[n][6][25] = dsdxii, dsdeta, dzdxii, dzdeta, inv_s, ifact
+ * (inverse-Jacobian entries under the reference's Jacobian-entry names,
+ * Quad::createGradient, src/preloop/se_model/quad/Quad.hpp:157-184, and
+ * the integral factor, Quad.cpp:404-435).  Replaces the GradientQuadrature
+ * constructor (GradientQuadrature.hpp:122-137) and SolidElement's point
+ * set (src/core/element/Element.cpp:28-43). */
+int axs3d_set_solid_elements(axs3d_ctx *ctx, int n, const int32_t *points,
+                             const uint8_t *axial, const double *gq);
+
+/* 1D isotropic solid material, lam_mu[n][2][25] = lambda, mu.  Replaces
+ * the Isotropic 1D constructor (src/core/element/material/elastic/
+ * anisotropy/Isotropic.hpp:24-34) built by Material::createIsotropic
+ * (src/preloop/se_model/quad/physical/Material.cpp:566-582). */
+int axs3d_set_solid_iso1d(axs3d_ctx *ctx, const double *lam_mu);
+
+/* Solid points on the axis.  Replaces AxialBoundary::addPoint
+ * (src/core/boundary/axial/AxialBoundary.hpp). */
+int axs3d_set_axial_points(axs3d_ctx *ctx, int n, const int32_t *points);
+
+/* Moment-tensor source: element, Fourier pattern [nu1][6][25] complex and
+ * the source-time function sampled at every step (FP64).  Replaces
+ * SolidMoment (src/core/source/elem_src/solid/SolidMoment.cpp:15-33) with
+ * its GaussianSTF (src/core/source/elem_src/stf/GaussianSTF.hpp:35-58). */
+int axs3d_set_moment_source(axs3d_ctx *ctx, int elem, int nu1,
+                            const double *pattern, int nsteps,
+                            const double *stf);
+
+/* Receivers: element, in-plane weights [n][25], azimuth phi.  Replaces
+ * StationSolid::setElement (src/core/output/station-wise/station/
+ * StationSolid.cpp:18-27, Station.cpp:13-38).  max_samples bounds the
+ * device seismogram buffer (one sample per recorded step). */
+int axs3d_set_receivers(axs3d_ctx *ctx, int n, const int32_t *elem,
+                        const double *weights, const double *phi,
+                        int max_samples);
+
+/* Time step.  Replaces TimeScheme::setTime (src/core/time/TimeScheme.hpp)
+ * as used by NewmarkTimeScheme::update's dt constants
+ * (NewmarkTimeScheme.hpp:65-67). */
+int axs3d_set_dt(axs3d_ctx *ctx, double dt);
+
+/* Ends set-up: checks sizes and compatibility (the reference's
+ * checkCompatibility calls, e.g. src/core/element/common/Property.hpp:53-80)
+ * and builds the device work lists. */
+int axs3d_finalize(axs3d_ctx *ctx);
+
+/* ---------------- per-phase entry points (Domain, Domain.cpp:285-345) -- */
+/* Domain::applySources (Domain.cpp:285-295). */
+int axs3d_apply_sources(axs3d_ctx *ctx, int tstep);
+/* Domain::computeStiffness (Domain.cpp:298-308). */
+int axs3d_compute_stiffness(axs3d_ctx *ctx);
+/* Domain::applyBC_BeforeAssemblingStiff (Domain.cpp:313-317). */
+int axs3d_apply_bc_before(axs3d_ctx *ctx);
+/* Domain::recordOutput for station displacement (Domain.cpp:385-409),
+ * stored as sample `isample`. */
+int axs3d_record(axs3d_ctx *ctx, int isample);
+/* Domain::applyBC_AfterAssemblingStiff (Domain.cpp:322-326). */
+int axs3d_apply_bc_after(axs3d_ctx *ctx);
+/* Domain::computeStiffToAccel (Domain.cpp:335-345): Nyquist masking and
+ * mass division, in place in the stiffness field. */
+int axs3d_stiff_to_accel(axs3d_ctx *ctx);
+/* NewmarkTimeScheme::update on all points (NewmarkTimeScheme.hpp:48-71);
+ * expects the stiffness field to hold the acceleration. */
+int axs3d_newmark(axs3d_ctx *ctx);
+/* Domain::checkStability (Domain.cpp:574-612). */
+int axs3d_check_stability(axs3d_ctx *ctx, int tstep);
+
+/* ---------------- bulk ---------------------------------------------- */
+/* NewmarkTimeScheme::solve (src/core/time/NewmarkTimeScheme.cpp:18-131)
+ * over steps [step_begin, step_begin + nsteps), recording every step into
+ * the seismogram buffer at sample (tstep - record_step0).  The
+ * BC-after/mass/Newmark phases run as one fused kernel. */
+int axs3d_run(axs3d_ctx *ctx, int step_begin, int nsteps);
+int axs3d_set_record_origin(axs3d_ctx *ctx, int record_step0);
+/* Copies samples [0, nsamples) of the seismograms, [sample][rec][3]. */
+int axs3d_read_seismograms(axs3d_ctx *ctx, double *out, int nsamples);
+
+/* ---------------- debug hooks ---------------------------------------- */
+int axs3d_num_modes(const axs3d_ctx *ctx, int64_t *nmodes);
+int axs3d_get_field(axs3d_ctx *ctx, int which, double *out);
+int axs3d_set_field(axs3d_ctx *ctx, int which, const double *in);
+int axs3d_synchronize(axs3d_ctx *ctx);
+
+/* ---------------- measurement --------------------------------------- */
+/* When enabled, every launch of the kernels in axs3d_kernel_id is
+ * bracketed by HIP events on the context's stream; read returns the
+ * summed milliseconds and the launch count since the last reset. */
+int axs3d_profile_enable(axs3d_ctx *ctx, int enable);
+int axs3d_profile_read(axs3d_ctx *ctx, int kernel, double *total_ms,
+                       int64_t *count);
+int axs3d_profile_reset(axs3d_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AXS3D_H */

@@ -1,0 +1,232 @@
+"""ctypes wrapper of the FP64 CPU oracle (``oracle/axs3d_oracle.c``).
+
+TEST INFRASTRUCTURE ONLY: imported by ``tests/``, ``__graft_entry__.smoke``
+and the ``cpu_baseline`` leg of ``bench.py`` as the checker.  The product
+path (``axisem3d_amd``) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+
+def build(force=False):
+    """Compile the oracle with its Makefile (gcc)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.check_call(["make", "-s", "-C", _HERE])
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = C.CDLL(_LIB_PATH)
+        _setup(_lib)
+    return _lib
+
+
+_p = C.c_void_p
+
+
+class OracleDomain(C.Structure):
+    _fields_ = [
+        ("n_points", C.c_int), ("point_nr", _p), ("point_mass", _p),
+        ("offsets", _p),
+        ("n_elem", C.c_int), ("elem_points", _p), ("elem_nr", _p),
+        ("elem_axial", _p), ("elem_gq", _p), ("elem_iso", _p),
+        ("n_axial", C.c_int), ("axial_points", _p),
+        ("src_elem", C.c_int), ("src_nu1", C.c_int), ("src_pattern", _p),
+        ("stf", _p),
+        ("n_rec", C.c_int), ("rec_elem", _p), ("rec_weights", _p),
+        ("rec_phi", _p),
+        ("dt", C.c_double),
+    ]
+
+
+def _setup(L):
+    L.oracle_set_gmat.argtypes = [_p, _p]
+    for name in ("oracle_grad6", "oracle_quad6", "oracle_quad6_noint"):
+        getattr(L, name).argtypes = [_p, C.c_int, C.c_int, _p, _p]
+    L.oracle_iso_stress.argtypes = [_p, _p, _p, _p]
+    L.oracle_solid_element_stiff.argtypes = [C.POINTER(OracleDomain), C.c_int,
+                                             _p, _p]
+    L.oracle_run.argtypes = [C.POINTER(OracleDomain), C.c_int, C.c_int,
+                             _p, _p, _p, _p, _p]
+    L.oracle_run.restype = C.c_int
+    L.oracle_record.argtypes = [C.POINTER(OracleDomain), _p, _p]
+    L.oracle_apply_moment_source.argtypes = [C.POINTER(OracleDomain),
+                                             C.c_double, _p]
+    L.oracle_axial_solid.argtypes = [C.POINTER(OracleDomain), _p]
+    L.oracle_stiff_to_accel.argtypes = [C.POINTER(OracleDomain), _p]
+    L.oracle_newmark.argtypes = [C.c_int64, C.c_double, _p, _p, _p, _p]
+    from axisem3d_amd import spectral
+    t = spectral.tables()
+    g_gll = np.ascontiguousarray(t[4])
+    g_glj = np.ascontiguousarray(t[5])
+    L.oracle_set_gmat(g_gll.ctypes.data, g_glj.ctypes.data)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+# ------------------------------------------------------------ stage level
+def grad6(gq_e, axial, alpha, u):
+    """u complex (3, 25) -> strain complex (6, 25)."""
+    u = np.ascontiguousarray(u, dtype=np.complex128)
+    gq_e = np.ascontiguousarray(gq_e, dtype=np.float64)
+    out = np.zeros((6, 25), dtype=np.complex128)
+    lib().oracle_grad6(gq_e.ctypes.data, int(axial), int(alpha),
+                       u.ctypes.data, out.ctypes.data)
+    return out
+
+
+def quad6(gq_e, axial, alpha, sig, integrate=True):
+    sig = np.ascontiguousarray(sig, dtype=np.complex128)
+    gq_e = np.ascontiguousarray(gq_e, dtype=np.float64)
+    out = np.zeros((3, 25), dtype=np.complex128)
+    fn = lib().oracle_quad6 if integrate else lib().oracle_quad6_noint
+    fn(gq_e.ctypes.data, int(axial), int(alpha), sig.ctypes.data,
+       out.ctypes.data)
+    return out
+
+
+def iso_stress(lam, mu, eps):
+    eps = np.ascontiguousarray(eps, dtype=np.complex128)
+    lam = np.ascontiguousarray(lam, dtype=np.float64)
+    mu = np.ascontiguousarray(mu, dtype=np.float64)
+    out = np.zeros((6, 25), dtype=np.complex128)
+    lib().oracle_iso_stress(lam.ctypes.data, mu.ctypes.data, eps.ctypes.data,
+                            out.ctypes.data)
+    return out
+
+
+# ------------------------------------------------------------ domain level
+class Oracle:
+    """FP64 restatement of the time loop on a ``DomainSpec``."""
+
+    def __init__(self, spec):
+        self.spec = spec
+        self._keep = []
+        nr = np.ascontiguousarray(spec.solid_nr, dtype=np.int32)
+        self.offsets = np.zeros(len(nr) + 1, dtype=np.int64)
+        self.offsets[1:] = np.cumsum(nr // 2 + 1)
+        self.nmodes = int(self.offsets[-1])
+        d = OracleDomain()
+        d.n_points = len(nr)
+        d.point_nr = self._k(nr)
+        d.point_mass = self._k(np.ascontiguousarray(spec.solid_mass,
+                                                    dtype=np.float64))
+        d.offsets = self._k(self.offsets)
+        d.n_elem = spec.n_solid_elements
+        d.elem_points = self._k(np.ascontiguousarray(spec.elem_points,
+                                                     dtype=np.int32))
+        d.elem_nr = self._k(np.ascontiguousarray(spec.elem_nr,
+                                                 dtype=np.int32))
+        d.elem_axial = self._k(np.ascontiguousarray(spec.elem_axial,
+                                                    dtype=np.uint8))
+        d.elem_gq = self._k(np.ascontiguousarray(spec.elem_gq,
+                                                 dtype=np.float64))
+        d.elem_iso = self._k(np.ascontiguousarray(spec.elem_iso,
+                                                  dtype=np.float64))
+        ax = np.ascontiguousarray(spec.axial_points, dtype=np.int32)
+        d.n_axial = len(ax)
+        d.axial_points = self._k(ax)
+        if spec.src_elem >= 0 and spec.src_pattern is not None:
+            d.src_elem = int(spec.src_elem)
+            d.src_nu1 = spec.src_pattern.shape[0]
+            d.src_pattern = self._k(np.ascontiguousarray(
+                spec.src_pattern, dtype=np.complex128))
+            d.stf = self._k(np.ascontiguousarray(spec.stf, dtype=np.float64))
+        else:
+            d.src_elem = -1
+        if spec.rec_elem is not None:
+            d.n_rec = len(spec.rec_elem)
+            d.rec_elem = self._k(np.ascontiguousarray(spec.rec_elem,
+                                                      dtype=np.int32))
+            d.rec_weights = self._k(np.ascontiguousarray(spec.rec_weights,
+                                                         dtype=np.float64))
+            d.rec_phi = self._k(np.ascontiguousarray(spec.rec_phi,
+                                                     dtype=np.float64))
+        d.dt = float(spec.dt)
+        self.d = d
+        z = lambda: np.zeros((self.nmodes, 3), dtype=np.complex128)
+        self.displ, self.veloc, self.accel, self.stiff = z(), z(), z(), z()
+        self.step = 0
+
+    def _k(self, a):
+        self._keep.append(a)
+        return a.ctypes.data
+
+    def run(self, nsteps, record=True):
+        nrec = self.d.n_rec
+        seis = np.zeros((nsteps, max(nrec, 1), 3)) if record else None
+        rc = lib().oracle_run(C.byref(self.d), self.step, nsteps,
+                              self.displ.ctypes.data, self.veloc.ctypes.data,
+                              self.accel.ctypes.data, self.stiff.ctypes.data,
+                              _ptr(seis))
+        if rc != 0:
+            raise RuntimeError("Domain::checkStability || Simulation has "
+                               "blown up (oracle).")
+        self.step += nsteps
+        return seis[:, :nrec] if record else None
+
+    def element_stiff(self, e, displ):
+        """Stiffness of one element (subtracted into a zero array)."""
+        displ = np.ascontiguousarray(displ, dtype=np.complex128)
+        out = np.zeros_like(displ)
+        lib().oracle_solid_element_stiff(C.byref(self.d), int(e),
+                                         displ.ctypes.data, out.ctypes.data)
+        return out
+
+    def stiffness(self, displ):
+        """Assembled stiffness of all solid elements."""
+        displ = np.ascontiguousarray(displ, dtype=np.complex128)
+        out = np.zeros_like(displ)
+        for e in range(self.d.n_elem):
+            lib().oracle_solid_element_stiff(C.byref(self.d), e,
+                                             displ.ctypes.data,
+                                             out.ctypes.data)
+        return out
+
+    # ------------------------------------------------- phase level
+    def apply_source(self, stiff, tstep):
+        stiff = np.ascontiguousarray(stiff, dtype=np.complex128).copy()
+        lib().oracle_apply_moment_source(C.byref(self.d),
+                                         float(self.spec.stf[tstep]),
+                                         stiff.ctypes.data)
+        return stiff
+
+    def axial(self, stiff):
+        stiff = np.ascontiguousarray(stiff, dtype=np.complex128).copy()
+        lib().oracle_axial_solid(C.byref(self.d), stiff.ctypes.data)
+        return stiff
+
+    def stiff_to_accel(self, stiff):
+        stiff = np.ascontiguousarray(stiff, dtype=np.complex128).copy()
+        lib().oracle_stiff_to_accel(C.byref(self.d), stiff.ctypes.data)
+        return stiff
+
+    def newmark(self, stiff, displ, veloc, accel):
+        """Returns updated (stiff, displ, veloc, accel) copies."""
+        arrs = [np.ascontiguousarray(a, dtype=np.complex128).copy()
+                for a in (stiff, displ, veloc, accel)]
+        lib().oracle_newmark(self.nmodes, float(self.d.dt),
+                             *[a.ctypes.data for a in arrs])
+        return arrs
+
+    def record(self, displ):
+        displ = np.ascontiguousarray(displ, dtype=np.complex128)
+        out = np.zeros((max(self.d.n_rec, 1), 3))
+        lib().oracle_record(C.byref(self.d), displ.ctypes.data,
+                            out.ctypes.data)
+        return out[:self.d.n_rec]

@@ -1,0 +1,69 @@
+/*
+ * axs3d_oracle.h -- FP64 CPU restatement of the AxiSEM3D stiffness path.
+ * TEST INFRASTRUCTURE ONLY (the checker and the CPU baseline); the product
+ * path never includes, links or loads it.
+ */
+#ifndef AXS3D_ORACLE_H
+#define AXS3D_ORACLE_H
+
+#include <complex.h>
+#include <stdint.h>
+
+typedef struct {
+    /* solid points */
+    int n_points;
+    const int32_t *point_nr;      /* Nr per point                        */
+    const double *point_mass;     /* assembled 1D mass per point         */
+    const int64_t *offsets;       /* prefix sum of Nr/2+1, n_points + 1  */
+    /* solid elements */
+    int n_elem;
+    const int32_t *elem_points;   /* [e][25]                             */
+    const int32_t *elem_nr;       /* element Nr = max of its points      */
+    const uint8_t *elem_axial;
+    const double *elem_gq;        /* [e][6][25]                          */
+    const double *elem_iso;       /* [e][2][25] lambda, mu               */
+    /* axis */
+    int n_axial;
+    const int32_t *axial_points;
+    /* moment source */
+    int src_elem;                 /* -1: none                            */
+    int src_nu1;
+    const double complex *src_pattern;  /* [nu1][6][25]                  */
+    const double *stf;                  /* per step                      */
+    /* receivers */
+    int n_rec;
+    const int32_t *rec_elem;
+    const double *rec_weights;    /* [r][25]                             */
+    const double *rec_phi;
+    /* time */
+    double dt;
+} oracle_domain;
+
+void oracle_set_gmat(const double *g_gll, const double *g_glj);
+void oracle_grad6(const double *gq_e, int axial, int alpha,
+                  const double complex *u, double complex *e);
+void oracle_quad6(const double *gq_e, int axial, int alpha,
+                  const double complex *sij, double complex *fi);
+void oracle_quad6_noint(const double *gq_e, int axial, int alpha,
+                        const double complex *sij, double complex *fi);
+void oracle_iso_stress(const double *lam, const double *mu,
+                       const double complex *e, double complex *s);
+void oracle_solid_element_stiff(const oracle_domain *d, int e,
+                                const double complex *displ,
+                                double complex *stiff);
+void oracle_apply_moment_source(const oracle_domain *d, double stf_value,
+                                double complex *stiff);
+void oracle_axial_solid(const oracle_domain *d, double complex *stiff);
+void oracle_stiff_to_accel(const oracle_domain *d, double complex *stiff);
+void oracle_newmark(int64_t nmodes, double dt, double complex *stiff,
+                    double complex *displ, double complex *veloc,
+                    double complex *accel);
+void oracle_record(const oracle_domain *d, const double complex *displ,
+                   double *out);
+int oracle_run(const oracle_domain *d, int step_begin, int nsteps,
+               double complex *displ, double complex *veloc,
+               double complex *accel, double complex *stiff, double *seismo);
+int64_t *oracle_make_offsets(int n, const int32_t *nr);
+void oracle_free(void *p);
+
+#endif
