@@ -16,7 +16,8 @@ AXS3D_FP32_FP64 = 1
 
 FIELD_DISPL, FIELD_VELOC, FIELD_ACCEL, FIELD_STIFF = 0, 1, 2, 3
 KERNEL_STIFFNESS, KERNEL_POINT, KERNEL_SOURCE, KERNEL_RECORD = 0, 1, 2, 3
-NUM_KERNELS = 4
+KERNEL_STIFFNESS_3D = 4
+NUM_KERNELS = 5
 
 # every symbol declared in include/axs3d.h, with (restype, argtypes)
 _p = C.c_void_p
@@ -32,6 +33,10 @@ SIGNATURES = {
     "axs3d_set_solid_elements": (_i, [_p, _i, _p, _p, _p]),
     "axs3d_set_solid_iso1d": (_i, [_p, _p]),
     "axs3d_set_axial_points": (_i, [_p, _i, _p]),
+    "axs3d_set_element_kinds": (_i, [_p, _p]),
+    "axs3d_set_solid_iso3d": (_i, [_p, _i, _i, _p]),
+    "axs3d_set_attenuation": (_i, [_p, _i, _i, _p, _p, _p]),
+    "axs3d_set_attenuation3d": (_i, [_p, _i, _i, _p]),
     "axs3d_set_moment_source": (_i, [_p, _i, _i, _p, _i, _p]),
     "axs3d_set_receivers": (_i, [_p, _i, _p, _p, _p, _i]),
     "axs3d_set_dt": (_i, [_p, _d]),

@@ -85,6 +85,32 @@ int axs3d_set_solid_iso1d(axs3d_ctx *ctx, const double *lam_mu) {
     return guarded(ctx, [&](axs3d::DomainBase &d) { d.setSolidIso1D(lam_mu); });
 }
 
+int axs3d_set_element_kinds(axs3d_ctx *ctx, const uint8_t *kind) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) { d.setElementKinds(kind); });
+}
+
+int axs3d_set_solid_iso3d(axs3d_ctx *ctx, int e_begin, int e_end,
+                          const double *lam_mu) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) {
+        d.setSolidIso3D(e_begin, e_end, lam_mu);
+    });
+}
+
+int axs3d_set_attenuation(axs3d_ctx *ctx, int nsls, int cg4,
+                          const double *alpha, const double *beta,
+                          const double *gamma) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) {
+        d.setAttenuation(nsls, cg4, alpha, beta, gamma);
+    });
+}
+
+int axs3d_set_attenuation3d(axs3d_ctx *ctx, int e_begin, int e_end,
+                            const double *dlam_dmu) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) {
+        d.setAttenuation3D(e_begin, e_end, dlam_dmu);
+    });
+}
+
 int axs3d_set_axial_points(axs3d_ctx *ctx, int n, const int32_t *points) {
     return guarded(ctx,
                    [&](axs3d::DomainBase &d) { d.setAxialPoints(n, points); });

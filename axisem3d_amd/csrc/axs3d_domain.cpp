@@ -105,6 +105,102 @@ void DomainBase::setSolidIso1D(const double *lam_mu) {
     hElemIso.assign(lam_mu, lam_mu + (size_t)mNumElems * 2 * NPEM);
 }
 
+void DomainBase::setElementKinds(const uint8_t *kind) {
+    if (mNumElems <= 0)
+        throw Error("Domain::setElementKinds || Set elements first.");
+    if (!hSliceOff.empty())
+        throw Error("Domain::setElementKinds || Kinds must be set before "
+                    "3D properties.");
+    hElemKind.assign(kind, kind + mNumElems);
+    for (uint8_t k : hElemKind)
+        if (k > 1)
+            throw Error("Domain::setElementKinds || Unknown element kind.");
+}
+
+// Element Nr (Element::pointSet, Element.cpp:28-35) and the slice layout of
+// 3D properties: the Nr slices of each 3D element, in element order.
+void DomainBase::layout3D() {
+    if (!hSliceOff.empty()) return;
+    if (mNumPoints <= 0 || mNumElems <= 0)
+        throw Error("Domain::layout3D || Set points and elements first.");
+    if (hElemKind.empty()) hElemKind.assign(mNumElems, 0);
+    hElemNr.assign(mNumElems, 0);
+    hSliceOff.assign(mNumElems, -1);
+    int64_t sl = 0;
+    mNum3D = 0;
+    for (int e = 0; e < mNumElems; e++) {
+        int nr = 0;
+        for (int k = 0; k < NPEM; k++) {
+            int p = hElemPoints[(size_t)e * NPEM + k];
+            if (p < 0 || p >= mNumPoints)
+                throw Error("Domain::layout3D || Element point out of range.");
+            nr = std::max(nr, hPointNr[p]);
+        }
+        hElemNr[e] = nr;
+        if (hElemKind[e] == 1) {
+            hSliceOff[e] = sl;
+            sl += nr;
+            mNum3D++;
+        }
+    }
+    mNumSlices = sl;
+    hProp3DSet.assign(mNumElems, 0);
+    hAtt3DSet.assign(mNumElems, 0);
+}
+
+static void slice_range(const std::vector<uint8_t> &kind,
+                        const std::vector<int64_t> &off,
+                        const std::vector<int32_t> &nr, int e0, int e1,
+                        int64_t &s0, int64_t &ns) {
+    s0 = -1;
+    ns = 0;
+    for (int e = e0; e < e1; e++) {
+        if (kind[e] != 1) continue;
+        if (s0 < 0) s0 = off[e];
+        ns += nr[e];
+    }
+}
+
+void DomainBase::setSolidIso3D(int e_begin, int e_end, const double *lam_mu) {
+    layout3D();
+    if (e_begin < 0 || e_end > mNumElems || e_begin > e_end)
+        throw Error("Domain::setSolidIso3D || Element range out of bounds.");
+    int64_t s0, ns;
+    slice_range(hElemKind, hSliceOff, hElemNr, e_begin, e_end, s0, ns);
+    if (ns == 0) return;
+    uploadProp3D(s0, ns, lam_mu);
+    for (int e = e_begin; e < e_end; e++)
+        if (hElemKind[e] == 1) hProp3DSet[e] = 1;
+}
+
+void DomainBase::setAttenuation(int nsls, int cg4, const double *alpha,
+                                const double *beta, const double *gamma) {
+    if (nsls < 0 || nsls > 16)
+        throw Error("Domain::setAttenuation || Invalid number of SLS.");
+    mNsls = nsls;
+    mAttCG4 = cg4 ? 1 : 0;
+    hAttABG.clear();
+    hAttABG.insert(hAttABG.end(), alpha, alpha + nsls);
+    hAttABG.insert(hAttABG.end(), beta, beta + nsls);
+    hAttABG.insert(hAttABG.end(), gamma, gamma + nsls);
+}
+
+void DomainBase::setAttenuation3D(int e_begin, int e_end,
+                                  const double *dlam_dmu) {
+    layout3D();
+    if (mNsls <= 0)
+        throw Error("Domain::setAttenuation3D || Set the SLS first.");
+    if (e_begin < 0 || e_end > mNumElems || e_begin > e_end)
+        throw Error("Domain::setAttenuation3D || Element range out of "
+                    "bounds.");
+    int64_t s0, ns;
+    slice_range(hElemKind, hSliceOff, hElemNr, e_begin, e_end, s0, ns);
+    if (ns == 0) return;
+    uploadAtt3D(s0, ns, dlam_dmu);
+    for (int e = e_begin; e < e_end; e++)
+        if (hElemKind[e] == 1) hAtt3DSet[e] = 1;
+}
+
 void DomainBase::setAxialPoints(int n, const int32_t *points) {
     hAxial.assign(points, points + std::max(n, 0));
 }
@@ -133,9 +229,21 @@ void DomainBase::finalize() {
         throw Error("Domain::finalize || G matrices are not set.");
     if (mNumPoints <= 0 || mNumElems <= 0)
         throw Error("Domain::finalize || Points or elements are not set.");
-    if (hElemIso.size() != (size_t)mNumElems * 2 * NPEM)
-        throw Error("Domain::finalize || Material is not set.");
     if (!(mDt > 0.)) throw Error("Domain::finalize || dt is not set.");
+    layout3D();
+    const bool has1D = mNum3D < mNumElems;
+    if (has1D && hElemIso.size() != (size_t)mNumElems * 2 * NPEM)
+        throw Error("Domain::finalize || Material is not set.");
+    for (int e = 0; e < mNumElems; e++) {
+        if (hElemKind[e] != 1) continue;
+        if (!hProp3DSet[e])
+            throw Error("Domain::finalize || 3D properties are not set.");
+        if (mNsls > 0 && !hAtt3DSet[e])
+            throw Error("Domain::finalize || 3D attenuation is not set.");
+    }
+    if (mNsls > 0 && has1D)
+        throw Error("Domain::finalize || Attenuation in Fourier-path (1D) "
+                    "elements is not supported by this build.");
     // points: mode offsets and the mode -> point map
     std::vector<uint32_t> off(mNumPoints + 1, 0);
     std::vector<int32_t> nu1(mNumPoints);
@@ -170,9 +278,74 @@ void DomainBase::finalize() {
             nrmax = std::max(nrmax, hPointNr[p]);
         }
         eNu1[e] = nrmax / 2 + 1;
-        for (int a = 0; a < eNu1[e]; a++) em.push_back({e, a});
+        if (hElemKind[e] != 1)
+            for (int a = 0; a < eNu1[e]; a++) em.push_back({e, a});
     }
     mNumEM = (int)em.size();
+    // 3D elements, heaviest (largest Nr) first for the work queue
+    std::vector<int32_t> list3d;
+    mNu1Max3D = 0;
+    for (int e = 0; e < mNumElems; e++)
+        if (hElemKind[e] == 1) {
+            list3d.push_back(e);
+            mNu1Max3D = std::max(mNu1Max3D, eNu1[e]);
+        }
+    std::stable_sort(list3d.begin(), list3d.end(), [&](int a, int b) {
+        return hElemNr[a] > hElemNr[b];
+    });
+    // FFT twiddle tables exp(-2 pi i t / Nr) for every 3D Nr; the LDS FFT
+    // takes radices 2..13 and three packed series of Nr per point
+    std::vector<int64_t> twOff;
+    std::vector<double> twTab;
+    if (!list3d.empty()) {
+        int nrmax = hElemNr[list3d.front()];
+        if (3 * nrmax > CARD_FFT_CAP)
+            throw Error("Domain::finalize || Nr of a 3D element exceeds the "
+                        "cardinal kernel's FFT capacity (832).");
+        twOff.assign(nrmax + 1, -1);
+        for (int32_t e : list3d) {
+            int nr = hElemNr[e];
+            if (twOff[nr] >= 0) continue;
+            int m = nr;
+            for (int r : {2, 3, 5, 7, 11, 13})
+                while (m % r == 0) m /= r;
+            if (m != 1)
+                throw Error("Domain::finalize || Nr of a 3D element has a "
+                            "prime factor above 13 (use FFT lucky numbers).");
+            twOff[nr] = (int64_t)twTab.size() / 2;
+            for (int t = 0; t < nr; t++) {
+                double ang = -2. * M_PI * (double)t / (double)nr;
+                twTab.push_back(std::cos(ang));
+                twTab.push_back(std::sin(ang));
+            }
+        }
+    }
+    // element-local force buffer [e][ipnt][alpha]: offsets in triples
+    std::vector<uint32_t> ebOff(mNumElems);
+    int64_t ebTotal = 0;
+    for (int e = 0; e < mNumElems; e++) {
+        ebOff[e] = (uint32_t)ebTotal;
+        ebTotal += (int64_t)NPEM * eNu1[e];
+    }
+    if (ebTotal >= (int64_t)UINT32_MAX / 2)
+        throw Error("Domain::finalize || Too many element modes.");
+    mNumEBTriples = ebTotal;
+    // point -> (element, ipnt) adjacency in ascending element order, the
+    // order in which Domain::computeStiffness visits the elements; four
+    // slots per point (conforming quad meshes)
+    std::vector<uint32_t> adj((size_t)mNumPoints * 4, 0xffffffffu);
+    {
+        std::vector<uint8_t> fill(mNumPoints, 0);
+        for (int e = 0; e < mNumElems; e++)
+            for (int k = 0; k < NPEM; k++) {
+                int p = hElemPoints[(size_t)e * NPEM + k];
+                if (fill[p] >= 4)
+                    throw Error("Domain::finalize || A point is shared by "
+                                "more than four elements.");
+                adj[(size_t)p * 4 + fill[p]++] =
+                    ebOff[e] + (uint32_t)k * eNu1[e];
+            }
+    }
     // source
     if (hSrcElem >= 0) {
         if (hSrcElem >= mNumElems)
@@ -203,6 +376,18 @@ void DomainBase::finalize() {
     up(dElemAxial, hElemAxial.data(), hElemAxial.size());
     up(dElemNu1, eNu1.data(), eNu1.size() * 4);
     up(dEMList, em.data(), em.size() * sizeof(EMItem));
+    up(dElemEBOff, ebOff.data(), ebOff.size() * 4);
+    up(dAdj, adj.data(), adj.size() * 4);
+    up(dElemNr, hElemNr.data(), hElemNr.size() * 4);
+    up(dSliceOff, hSliceOff.data(), hSliceOff.size() * 8);
+    if (!list3d.empty()) {
+        up(dList3D, list3d.data(), list3d.size() * 4);
+        up(dTwOff, twOff.data(), twOff.size() * 8);
+        dWork.alloc(sizeof(int));
+        dWork.zero(st);
+    }
+    hTwTab.swap(twTab);
+    if (mNsls > 0) up(dAttABG, hAttABG.data(), hAttABG.size() * 8);
     const size_t fbytes = (size_t)mNumModes * 6 * sizeof(double);
     dDispl.alloc(fbytes);
     dVeloc.alloc(fbytes);
@@ -262,14 +447,12 @@ void DomainBase::checkStability(int tstep) {
 void DomainBase::solve(int step_begin, int nsteps) {
     requireFinalized("NewmarkTimeScheme::solve");
     for (int tstep = step_begin; tstep < step_begin + nsteps; tstep++) {
-        applySources(tstep);
-        computeStiffness();
-        applyBC_BeforeAssemblingStiff();
+        int isample = -1;
         if (mNumRec > 0) {
-            int isample = tstep - mRecordStep0;
-            if (isample >= 0 && isample < hMaxSamples) recordOutput(isample);
+            isample = tstep - mRecordStep0;
+            if (isample < 0 || isample >= hMaxSamples) isample = -1;
         }
-        fusedPointUpdate();
+        step(tstep, isample);
     }
     checkStability(step_begin + nsteps);
 }
@@ -354,41 +537,103 @@ class Domain final : public DomainBase {
 public:
     Domain(int device, int precision) : DomainBase(device, precision) {}
 
+    StiffArgs<Real> stiffArgs() const {
+        StiffArgs<Real> a;
+        a.n_em = mNumEM;
+        a.em_list = dEMList.as<EMItem>();
+        a.elem_points = dElemPoints.as<int32_t>();
+        a.elem_axial = dElemAxial.as<uint8_t>();
+        a.elem_ops = dElemOps.as<Real>();
+        a.elem_mat = dElemMat.as<Real>();
+        a.point_off = dPointOff.as<uint32_t>();
+        a.point_nu1 = dPointNu1.as<int32_t>();
+        a.displ = dDispl.as<double>();
+        a.elem_eb_off = dElemEBOff.as<uint32_t>();
+        a.elem_nu1 = dElemNu1.as<int32_t>();
+        a.eb = dEB.as<Real>();
+        a.gtab = dGTab.as<Real>();
+        return a;
+    }
+    CardArgs<Real> cardArgs() const {
+        CardArgs<Real> c;
+        c.s = stiffArgs();
+        c.n3d = mNum3D;
+        c.list3d = dList3D.as<int32_t>();
+        c.work_counter = dWork.as<int>();
+        c.elem_nr = dElemNr.as<int32_t>();
+        c.slice_off = dSliceOff.as<int64_t>();
+        c.prop3d = dProp3D.as<Real>();
+        c.twiddle = dTwiddle.as<cpx<Real>>();
+        c.tw_off = dTwOff.as<int64_t>();
+        c.scratch = dCardScratch.as<Real>();
+        c.nu1max = mNu1Max3D;
+        c.nsls = mNsls;
+        c.npa = mAttCG4 ? 4 : NPEM;
+        c.abg = dAttABG.as<double>();
+        c.att3d = dAtt3D.as<Real>();
+        c.memvar = dMemVar.as<double>();
+        c.dstress = dDStress.as<Real>();
+        return c;
+    }
+    PointGeom pointGeom() const {
+        PointGeom g;
+        g.n_modes = mNumModes;
+        g.mode_point = dModePoint.as<int32_t>();
+        g.point_off = dPointOff.as<uint32_t>();
+        g.point_nr = dPointNr.as<int32_t>();
+        g.point_axial = dPointAxial.as<uint8_t>();
+        g.inv_mass = dInvMass.as<double>();
+        g.adj = dAdj.as<uint32_t>();
+        return g;
+    }
+
+    // Domain::applySources: the source force is subtracted from the point
+    // stiffness (stage form).
     void applySources(int tstep) override {
         requireFinalized("Domain::applySources");
+        sourceKernel(tstep, false);
+    }
+
+    void sourceKernel(int tstep, bool toElem) {
         if (hSrcElem < 0) return;
         if (tstep < 0 || tstep >= mStfSteps)
             throw Error("Domain::applySources || Step outside the STF.");
         tick(AXS3D_KERNEL_SOURCE);
-        launch_moment_source<Real>(
-            mStream, hSrcElem, mSrcNu1c, dSrcPattern.as<Real>(),
-            dStf.as<double>(), tstep, dElemPoints.as<int32_t>(),
-            dElemAxial.as<uint8_t>(), dElemOps.as<Real>(),
-            dPointOff.as<uint32_t>(), dPointNu1.as<int32_t>(),
-            dStiff.as<Real>(), dGTab.as<Real>());
+        launch_moment_source<Real>(mStream, toElem, stiffArgs(), hSrcElem,
+                                   mSrcNu1c, dSrcPattern.as<Real>(),
+                                   dStf.as<double>(), tstep,
+                                   dStiff.as<Real>());
         tock(AXS3D_KERNEL_SOURCE);
     }
 
+    void elementKernel() {
+        if (mNumEM > 0) {
+            tick(AXS3D_KERNEL_STIFFNESS);
+            launch_solid_stiff_fr<Real>(mStream, stiffArgs());
+            tock(AXS3D_KERNEL_STIFFNESS);
+        }
+        if (mNum3D > 0) {
+            tick(AXS3D_KERNEL_STIFFNESS_3D);
+            launch_solid_stiff_cd<Real>(mStream, cardArgs(), mCardSlots);
+            tock(AXS3D_KERNEL_STIFFNESS_3D);
+        }
+    }
+
+    // Domain::computeStiffness (stage form): element kernel, then the
+    // points subtract the element-local forces.
     void computeStiffness() override {
         requireFinalized("Domain::computeStiffness");
-        tick(AXS3D_KERNEL_STIFFNESS);
-        launch_solid_stiff_fr<Real>(
-            mStream, mNumEM, dEMList.as<EMItem>(), dElemPoints.as<int32_t>(),
-            dElemAxial.as<uint8_t>(), dElemOps.as<Real>(),
-            dElemMat.as<Real>(), dPointOff.as<uint32_t>(),
-            dPointNu1.as<int32_t>(), dDispl.as<double>(), dStiff.as<Real>(),
-            dGTab.as<Real>());
-        tock(AXS3D_KERNEL_STIFFNESS);
+        elementKernel();
+        launch_assemble<Real>(mStream, pointGeom(), dEB.as<Real>(),
+                              dStiff.as<Real>());
     }
 
     void point(int phases) {
         tick(AXS3D_KERNEL_POINT);
-        launch_point<Real>(mStream, phases, mNumModes,
-                           dModePoint.as<int32_t>(), dPointOff.as<uint32_t>(),
-                           dPointNr.as<int32_t>(), dPointAxial.as<uint8_t>(),
-                           dInvMass.as<double>(), dStiff.as<Real>(),
-                           dDispl.as<double>(), dVeloc.as<double>(),
-                           dAccel.as<double>(), mDt, dFlag.as<int>());
+        launch_point<Real>(mStream, phases, pointGeom(), dEB.as<Real>(),
+                           dStiff.as<Real>(), dDispl.as<double>(),
+                           dVeloc.as<double>(), dAccel.as<double>(), mDt,
+                           dFlag.as<int>());
         tock(AXS3D_KERNEL_POINT);
     }
     void applyBC_AfterAssemblingStiff() override {
@@ -403,8 +648,16 @@ public:
         requireFinalized("NewmarkTimeScheme::update");
         point(PHASE_MARCH);
     }
-    void fusedPointUpdate() override {
-        point(PHASE_AXIAL | PHASE_MASS | PHASE_MARCH);
+
+    // One Newmark step in its bulk form: the element kernel writes the
+    // element-local forces, the source adds into its element's, the
+    // receivers sample u(t), and one point kernel gathers, masks, divides
+    // by the mass and marches.
+    void step(int tstep, int isample) override {
+        elementKernel();
+        sourceKernel(tstep, true);
+        if (isample >= 0) recordOutput(isample);
+        point(PHASE_GATHER | PHASE_AXIAL | PHASE_MASS | PHASE_MARCH);
     }
 
     void getField(int which, double *out) override {
@@ -452,6 +705,33 @@ private:
         }
     }
 
+    // 3D properties, [slice][2][25] lambda, mu, cast to Real by chunks.
+    void uploadProp3D(int64_t s0, int64_t ns, const double *src) override {
+        if (dProp3D.bytes() == 0)
+            dProp3D.alloc((size_t)mNumSlices * 2 * NPEM * sizeof(Real));
+        uploadCast(dProp3D, (size_t)s0 * 2 * NPEM, (size_t)ns * 2 * NPEM, src);
+    }
+    // 3D attenuation, [slice][2][npa] dlambda, dmu.
+    void uploadAtt3D(int64_t s0, int64_t ns, const double *src) override {
+        const int npa = mAttCG4 ? 4 : NPEM;
+        if (dAtt3D.bytes() == 0)
+            dAtt3D.alloc((size_t)mNumSlices * 2 * npa * sizeof(Real));
+        uploadCast(dAtt3D, (size_t)s0 * 2 * npa, (size_t)ns * 2 * npa, src);
+    }
+    void uploadCast(DeviceBuffer &b, size_t first, size_t n,
+                    const double *src) {
+        const size_t chunk = (size_t)1 << 24;
+        std::vector<Real> tmp(std::min(n, chunk));
+        for (size_t k = 0; k < n; k += chunk) {
+            const size_t m = std::min(chunk, n - k);
+            for (size_t q = 0; q < m; q++) tmp[q] = (Real)src[k + q];
+            hip_check(hipMemcpyAsync(b.as<Real>() + first + k, tmp.data(),
+                                     m * sizeof(Real), hipMemcpyHostToDevice,
+                                     mStream), "Domain::upload3D");
+            hip_check(hipStreamSynchronize(mStream), "Domain::upload3D");
+        }
+    }
+
     // Operands cast to Real after their FP64 products, as the reference's
     // GradientQuadrature constructor does (GradientQuadrature.hpp:127-137).
     void uploadReal() override {
@@ -471,6 +751,7 @@ private:
         dElemOps.alloc(ops.size() * sizeof(Real));
         dElemOps.upload(ops.data(), ops.size() * sizeof(Real), st);
         std::vector<Real> mat(hElemIso.begin(), hElemIso.end());
+        if (mat.empty()) mat.assign((size_t)mNumElems * 2 * NPEM, (Real)0);
         dElemMat.alloc(mat.size() * sizeof(Real));
         dElemMat.upload(mat.data(), mat.size() * sizeof(Real), st);
         std::vector<Real> gt(2 * NPEM);
@@ -482,6 +763,29 @@ private:
         dGTab.upload(gt.data(), gt.size() * sizeof(Real), st);
         dStiff.alloc((size_t)mNumModes * 6 * sizeof(Real));
         dStiff.zero(st);
+        dEB.alloc((size_t)mNumEBTriples * 6 * sizeof(Real));
+        dEB.zero(st);
+        if (mNum3D > 0) {
+            std::vector<Real> tw(hTwTab.begin(), hTwTab.end());
+            dTwiddle.alloc(tw.size() * sizeof(Real));
+            dTwiddle.upload(tw.data(), tw.size() * sizeof(Real), st);
+            int ncu = 0;
+            hip_check(hipDeviceGetAttribute(
+                          &ncu, hipDeviceAttributeMultiprocessorCount,
+                          mDevice), "Domain::finalize");
+            const int per_cu = card_blocks_per_cu<Real>(mNsls > 0);
+            mCardSlots = std::max(1, std::min(mNum3D, per_cu * ncu));
+            dCardScratch.alloc((size_t)mCardSlots * 6 * NPEM * mNu1Max3D *
+                               2 * sizeof(Real));
+            if (mNsls > 0) {
+                const int npa = mAttCG4 ? 4 : NPEM;
+                dMemVar.alloc((size_t)mNumSlices * mNsls * 6 * npa *
+                              sizeof(double));
+                dMemVar.zero(st);
+                dDStress.alloc((size_t)mNumSlices * 6 * npa * sizeof(Real));
+                dDStress.zero(st);
+            }
+        }
         if (hSrcElem >= 0) {
             std::vector<Real> pat(hSrcPattern.begin(), hSrcPattern.end());
             dSrcPattern.alloc(pat.size() * sizeof(Real));

@@ -58,6 +58,11 @@ public:
     void setSolidElements(int n, const int32_t *points, const uint8_t *axial,
                           const double *gq);
     void setSolidIso1D(const double *lam_mu);
+    void setElementKinds(const uint8_t *kind);
+    void setSolidIso3D(int e_begin, int e_end, const double *lam_mu);
+    void setAttenuation(int nsls, int cg4, const double *alpha,
+                        const double *beta, const double *gamma);
+    void setAttenuation3D(int e_begin, int e_end, const double *dlam_dmu);
     void setAxialPoints(int n, const int32_t *points);
     void setMomentSource(int elem, int nu1, const double *pattern, int nsteps,
                          const double *stf);
@@ -75,7 +80,8 @@ public:
     virtual void applyBC_AfterAssemblingStiff() = 0;
     virtual void computeStiffToAccel() = 0;
     virtual void newmark() = 0;
-    virtual void fusedPointUpdate() = 0;
+    // one bulk Newmark step (sources, stiffness, receivers, points)
+    virtual void step(int tstep, int isample) = 0;
     void checkStability(int tstep);
 
     // bulk
@@ -129,6 +135,19 @@ protected:
     // sizes
     int mNumPoints = 0, mNumElems = 0, mNumEM = 0, mNumRec = 0;
     int64_t mNumModes = 0;
+    int64_t mNumEBTriples = 0;
+    // 3D (cardinal) elements
+    std::vector<uint8_t> hElemKind;
+    std::vector<int32_t> hElemNr;       // element Nr (max over its points)
+    std::vector<int64_t> hSliceOff;     // first slice of each 3D element
+    int64_t mNumSlices = 0;
+    int mNum3D = 0, mNu1Max3D = 0, mCardSlots = 0;
+    std::vector<uint8_t> hProp3DSet, hAtt3DSet;
+    // attenuation
+    int mNsls = 0, mAttCG4 = 0;
+    std::vector<double> hAttABG;
+    std::vector<double> hTwTab;         // interleaved twiddles, FP64
+    void layout3D();                    // element Nr and slice offsets
     int mSrcNu1c = 0;
     int mStfSteps = 0;
 
@@ -139,7 +158,14 @@ protected:
     DeviceBuffer dDispl, dVeloc, dAccel, dStiff;
     // device: elements
     DeviceBuffer dElemPoints, dElemAxial, dElemNu1, dElemOps, dElemMat,
-        dEMList, dGTab;
+        dEMList, dGTab, dElemEBOff, dEB, dAdj;
+    // device: cardinal path
+    DeviceBuffer dElemNr, dSliceOff, dList3D, dWork, dProp3D, dTwiddle,
+        dTwOff, dCardScratch, dAttABG, dAtt3D, dMemVar, dDStress;
+    virtual void uploadProp3D(int64_t slice0, int64_t nslices,
+                              const double *src) = 0;
+    virtual void uploadAtt3D(int64_t slice0, int64_t nslices,
+                             const double *src) = 0;
     // device: source and receivers
     DeviceBuffer dSrcPattern, dStf, dRecElem, dRecW, dRecPhi, dSeismo;
     DeviceBuffer dFlag;

@@ -1,0 +1,201 @@
+// axs3d_element.hpp -- per-GLL-point gradient and quadrature device code
+// shared by the Fourier-path and cardinal-path element kernels.
+//
+// One thread owns one (element, Fourier order alpha, GLL point); the 25
+// threads of an element-mode exchange the displacement (gradient) and the
+// quadrature pre-products X, Y through LDS.  Reference (src = SOLVER/src):
+//   computeGrad6   src/core/element/grad/GradientQuadrature.hpp:207-231,405-459
+//   computeQuad6   GradientQuadrature.hpp:294-324,463-500
+#pragma once
+
+#include "axs3d_common.hpp"
+
+namespace axs3d {
+
+constexpr int EMB = 10;          // element-modes per 256-thread block
+constexpr int STIFF_BLOCK = 256;
+
+template <typename Real>
+struct PointOps {                // inverse-Jacobian entries and 1/s
+    Real dsdxii, dsdeta, dzdxii, dzdeta, invs;
+};
+
+// Loads the plain (gradient) and integral-factor-scaled (quadrature)
+// operators of point ipnt of element e, device layout [e][ELEM_OPS][25].
+template <typename Real>
+__device__ __forceinline__ void load_point_ops(const Real *__restrict__ ops_all,
+                                               int e, int ipnt,
+                                               PointOps<Real> &g,
+                                               PointOps<Real> &q) {
+    const Real *ops = ops_all + (size_t)e * ELEM_OPS * NPEM + ipnt;
+    g.dsdxii = ops[OP_DSDXII * NPEM];
+    g.dsdeta = ops[OP_DSDETA * NPEM];
+    g.dzdxii = ops[OP_DZDXII * NPEM];
+    g.dzdeta = ops[OP_DZDETA * NPEM];
+    g.invs = ops[OP_INVS * NPEM];
+    q.dsdxii = ops[OP_DSDXII_IF * NPEM];
+    q.dsdeta = ops[OP_DSDETA_IF * NPEM];
+    q.dzdxii = ops[OP_DZDXII_IF * NPEM];
+    q.dzdeta = ops[OP_DZDETA_IF * NPEM];
+    q.invs = ops[OP_INVS_IF * NPEM];
+}
+
+// The four 5-vectors of G this point needs: GT_xi(i,.), G(.,j) for the
+// gradient and G_xi(i,.), GT(.,j) for the quadrature.  sG = [G_GLL, G_GLJ].
+template <typename Real>
+struct PointG {
+    Real gxi_col[NPED], g_col[NPED], gxi_row[NPED], g_row[NPED];
+};
+
+template <typename Real>
+__device__ __forceinline__ void load_point_g(const Real (*sG)[NPEM],
+                                             bool axial, int i, int j,
+                                             PointG<Real> &G) {
+    const Real *Gxi = sG[axial ? 1 : 0];
+#pragma unroll
+    for (int k = 0; k < NPED; k++) {
+        G.gxi_col[k] = Gxi[k * NPED + i];   // GT_xi(i,k) = G_xi(k,i)
+        G.g_col[k] = sG[0][k * NPED + j];   // G(k,j)
+        G.gxi_row[k] = Gxi[i * NPED + k];   // G_xi(i,k)
+        G.g_row[k] = sG[0][j * NPED + k];   // GT(k,j) = G(j,k)
+    }
+}
+
+// computeGrad6 at point (i, j) of one element-mode whose displacement
+// tile sU[3][25] is in LDS; u = this point's own displacement.  Voigt
+// strain [ss, pp, zz, pz, sz, sp] with engineering shears.
+template <typename Real>
+__device__ __forceinline__ void grad6_point(const cpx<Real> (*sU)[NPEM],
+                                            const cpx<Real> (&u)[3],
+                                            Real ralpha, bool axial_row0,
+                                            int i, int j,
+                                            const PointG<Real> &G,
+                                            const PointOps<Real> &g,
+                                            cpx<Real> (&eps)[6]) {
+    // higher-order terms (computeGrad_dSdZ, GradientQuadrature.hpp:405-418)
+    cpx<Real> dS[3], dZ[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        cpx<Real> x = {0, 0}, y = {0, 0};
+#pragma unroll
+        for (int k = 0; k < NPED; k++) {
+            x = cfma(sU[c][k * NPED + j], G.gxi_col[k], x);
+            y = cfma(sU[c][i * NPED + k], G.g_col[k], y);
+        }
+        dS[c] = cadd(cmul(x, g.dzdeta), cmul(y, g.dzdxii));
+        dZ[c] = cadd(cmul(x, g.dsdeta), cmul(y, g.dsdxii));
+    }
+    eps[0] = dS[0];
+    eps[4] = cadd(dZ[0], dS[2]);
+    eps[5] = dS[1];
+    eps[3] = dZ[1];
+    eps[2] = dZ[2];
+    // lower-order terms: (i alpha u0 - u1, i alpha u1 + u0, i alpha u2) / s
+    if (!axial_row0) {
+        const cpx<Real> a0 = csub(ialpha(ralpha, u[0]), u[1]);
+        const cpx<Real> a1 = cadd(ialpha(ralpha, u[1]), u[0]);
+        const cpx<Real> a2 = ialpha(ralpha, u[2]);
+        eps[5] = cfma(a0, g.invs, eps[5]);
+        eps[1] = cmul(a1, g.invs);
+        eps[3] = cfma(a2, g.invs, eps[3]);
+    } else {
+        // axis: d/ds of the lower-order numerators (L'Hospital),
+        // computeGrad_1overS_setTo/addTo row 0, :426-437
+        cpx<Real> xa[3] = {{0, 0}, {0, 0}, {0, 0}};
+        cpx<Real> ya[3] = {{0, 0}, {0, 0}, {0, 0}};
+#pragma unroll
+        for (int k = 0; k < NPED; k++) {
+            const cpx<Real> uc0 = sU[0][k * NPED + j];
+            const cpx<Real> uc1 = sU[1][k * NPED + j];
+            const cpx<Real> uc2 = sU[2][k * NPED + j];
+            const cpx<Real> ur0 = sU[0][k];
+            const cpx<Real> ur1 = sU[1][k];
+            const cpx<Real> ur2 = sU[2][k];
+            xa[0] = cfma(csub(ialpha(ralpha, uc0), uc1), G.gxi_col[k], xa[0]);
+            xa[1] = cfma(cadd(ialpha(ralpha, uc1), uc0), G.gxi_col[k], xa[1]);
+            xa[2] = cfma(ialpha(ralpha, uc2), G.gxi_col[k], xa[2]);
+            ya[0] = cfma(csub(ialpha(ralpha, ur0), ur1), G.g_col[k], ya[0]);
+            ya[1] = cfma(cadd(ialpha(ralpha, ur1), ur0), G.g_col[k], ya[1]);
+            ya[2] = cfma(ialpha(ralpha, ur2), G.g_col[k], ya[2]);
+        }
+        eps[5] = cadd(eps[5], cadd(cmul(xa[0], g.dzdeta), cmul(ya[0], g.dzdxii)));
+        eps[1] = cadd(cmul(xa[1], g.dzdeta), cmul(ya[1], g.dzdxii));
+        eps[3] = cadd(eps[3], cadd(cmul(xa[2], g.dzdeta), cmul(ya[2], g.dzdxii)));
+    }
+}
+
+// Isotropic Hooke's law (Isotropic::strainToStress, src/core/element/
+// material/elastic/anisotropy/Isotropic.hpp:155-172).
+template <typename Real>
+__device__ __forceinline__ void iso_hooke(const cpx<Real> (&eps)[6], Real lam,
+                                          Real mu, cpx<Real> (&sig)[6]) {
+    const Real mu2 = mu * (Real)2;
+    const cpx<Real> tr = cmul(cadd(cadd(eps[0], eps[1]), eps[2]), lam);
+    sig[0] = cfma(eps[0], mu2, tr);
+    sig[1] = cfma(eps[1], mu2, tr);
+    sig[2] = cfma(eps[2], mu2, tr);
+    sig[3] = cmul(eps[3], mu);
+    sig[4] = cmul(eps[4], mu);
+    sig[5] = cmul(eps[5], mu);
+}
+
+// X = dFdS*dzdeta + dFdZ*dsdeta, Y = dFdS*dzdxii + dFdZ*dsdxii
+// (computeQuad_dSdZ, GradientQuadrature.hpp:471-474).
+template <typename Real>
+__device__ __forceinline__ void quad_xy(cpx<Real> dfds, cpx<Real> dfdz,
+                                        const PointOps<Real> &q, cpx<Real> &X,
+                                        cpx<Real> &Y) {
+    X = cadd(cmul(dfds, q.dzdeta), cmul(dfdz, q.dsdeta));
+    Y = cadd(cmul(dfds, q.dzdxii), cmul(dfdz, q.dsdxii));
+}
+
+// Quadrature at this point from its six stress components: forms X, Y
+// (with the axial L'Hospital corrections of computeQuad_1overS_addTo,
+// :493-497, folded into row 0 of X and Y) and the local 1/s term, exchanges
+// X, Y through LDS and contracts with G.  Contains one __syncthreads: every
+// thread of the block must call it.
+template <typename Real>
+__device__ __forceinline__ void quad6_point(
+    const cpx<Real> (&sig)[6], Real alpha, bool axial_row0,
+    const PointOps<Real> &q, cpx<Real> (*sX)[NPEM], cpx<Real> (*sY)[NPEM],
+    int ipnt, int i, int j, const PointG<Real> &G, bool active,
+    cpx<Real> (&F)[3]) {
+    cpx<Real> L[3];
+    if (active) {
+        cpx<Real> X[3], Y[3];
+        // (dFdS, dFdZ) per output component: (0,4), (5,3), (4,2)
+        quad_xy(sig[0], sig[4], q, X[0], Y[0]);
+        quad_xy(sig[5], sig[3], q, X[1], Y[1]);
+        quad_xy(sig[4], sig[2], q, X[2], Y[2]);
+        // lower-order terms (GradientQuadrature.hpp:311-318)
+        cpx<Real> b[3];
+        b[0] = csub(ialpha(alpha, sig[5]), sig[1]);
+        b[1] = cadd(ialpha(alpha, sig[1]), sig[5]);
+        b[2] = ialpha(alpha, sig[3]);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            L[c] = cmul(b[c], -q.invs);
+            if (axial_row0) {
+                X[c] = csub(X[c], cmul(b[c], q.dzdeta));
+                Y[c] = csub(Y[c], cmul(b[c], q.dzdxii));
+            }
+            sX[c][ipnt] = X[c];
+            sY[c][ipnt] = Y[c];
+        }
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            cpx<Real> a = {0, 0}, bsum = {0, 0};
+#pragma unroll
+            for (int k = 0; k < NPED; k++) {
+                a = cfma(sX[c][k * NPED + j], G.gxi_row[k], a);
+                bsum = cfma(sY[c][i * NPED + k], G.g_row[k], bsum);
+            }
+            F[c] = cadd(cadd(a, bsum), L[c]);
+        }
+    }
+}
+
+}  // namespace axs3d

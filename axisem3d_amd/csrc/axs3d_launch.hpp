@@ -9,30 +9,94 @@
 namespace axs3d {
 
 // point-update phase bits
-enum PointPhase { PHASE_AXIAL = 1, PHASE_MASS = 2, PHASE_MARCH = 4 };
+enum PointPhase {
+    PHASE_AXIAL = 1,
+    PHASE_MASS = 2,
+    PHASE_MARCH = 4,
+    PHASE_GATHER = 8
+};
+
+// Device operands of the solid element kernels.
+template <typename Real>
+struct StiffArgs {
+    int n_em;
+    const EMItem *em_list;
+    const int32_t *elem_points;
+    const uint8_t *elem_axial;
+    const Real *elem_ops;
+    const Real *elem_mat;
+    const uint32_t *point_off;
+    const int32_t *point_nu1;
+    const double *displ;
+    const uint32_t *elem_eb_off;   // element-local force offset, triples
+    const int32_t *elem_nu1;
+    Real *eb;                      // element-local forces
+    const Real *gtab;
+};
+
+// Complex FFT capacity of one LDS buffer of the cardinal kernel: three
+// packed series of length Nr per GLL point, so Nr <= 832.
+constexpr int CARD_FFT_CAP = 2496;
+
+// Device operands of the cardinal-path (3D material) element kernel.
+template <typename Real>
+struct CardArgs {
+    StiffArgs<Real> s;
+    int n3d;                       // number of 3D elements
+    const int32_t *list3d;         // 3D elements, heaviest first
+    int *work_counter;
+    const int32_t *elem_nr;
+    const int64_t *slice_off;      // first slice of each element
+    const Real *prop3d;            // [slice][2][25] lambda, mu
+    const cpx<Real> *twiddle;      // exp(-2 pi i t / Nr), per Nr
+    const int64_t *tw_off;         // offset of each Nr's table
+    Real *scratch;                 // [slot][150][nu1max] complex
+    int nu1max;
+    // attenuation (nsls = 0: none)
+    int nsls;
+    int npa;                       // 4 (CG4) or 25 (FULL)
+    const double *abg;             // [3][nsls] alpha, beta, gamma
+    const Real *att3d;             // [slice][2][npa] dlambda, dmu
+    double *memvar;                // [slice][nsls][6][npa]
+    Real *dstress;                 // [slice][6][npa]
+};
+
+// Device point description used by the gather and point kernels.
+struct PointGeom {
+    int64_t n_modes;
+    const int32_t *mode_point;
+    const uint32_t *point_off;
+    const int32_t *point_nr;
+    const uint8_t *point_axial;
+    const double *inv_mass;
+    const uint32_t *adj;           // [point][4] element-local offsets,
+                                   // ascending element, padded with ~0u
+};
 
 template <typename Real>
-void launch_solid_stiff_fr(hipStream_t st, int n_em, const EMItem *em_list,
-                           const int32_t *elem_points,
-                           const uint8_t *elem_axial, const Real *elem_ops,
-                           const Real *elem_mat, const uint32_t *point_off,
-                           const int32_t *point_nu1, const double *displ,
-                           Real *stiff, const Real *gtab);
+void launch_solid_stiff_fr(hipStream_t st, const StiffArgs<Real> &a);
 
 template <typename Real>
-void launch_moment_source(hipStream_t st, int elem, int nu1c,
+void launch_solid_stiff_cd(hipStream_t st, const CardArgs<Real> &a,
+                           int n_slots);
+
+template <typename Real>
+int card_blocks_per_cu(bool att);
+
+template <typename Real>
+void launch_moment_source(hipStream_t st, bool to_elem,
+                          const StiffArgs<Real> &a, int elem, int nu1c,
                           const Real *pattern, const double *stf, int tstep,
-                          const int32_t *elem_points,
-                          const uint8_t *elem_axial, const Real *elem_ops,
-                          const uint32_t *point_off, const int32_t *point_nu1,
-                          Real *stiff, const Real *gtab);
+                          Real *stiff);
 
 template <typename Real>
-void launch_point(hipStream_t st, int phases, int64_t n_modes,
-                  const int32_t *mode_point, const uint32_t *point_off,
-                  const int32_t *point_nr, const uint8_t *point_axial,
-                  const double *inv_mass, Real *stiff, double *displ,
-                  double *veloc, double *accel, double dt, int *flag);
+void launch_assemble(hipStream_t st, const PointGeom &g, const Real *eb,
+                     Real *stiff);
+
+template <typename Real>
+void launch_point(hipStream_t st, int phases, const PointGeom &g,
+                  const Real *eb, Real *stiff, double *displ, double *veloc,
+                  double *accel, double dt, int *flag);
 
 void launch_record(hipStream_t st, int nrec, const int32_t *rec_elem,
                    const double *rec_w, const double *rec_phi,

@@ -369,7 +369,11 @@ def build_domain(mesh: StructuredMesh,
     nsz = mesh.nodal_sz()
     rcorner = (np.hypot(nsz[:, 0], nsz[:, 1]) if mesh.spherical
                else nsz[:, 1])
-    rho_n, vp_n, vs_n = rho_vp_vs(rcorner)
+    try:
+        rho_n, vp_n, vs_n = rho_vp_vs(rcorner, rcorner.mean(axis=1,
+                                                            keepdims=True))
+    except TypeError:
+        rho_n, vp_n, vs_n = rho_vp_vs(rcorner)
     rho = interpolate_gll(rho_n, axial)
     vp = interpolate_gll(vp_n, axial)
     vs = interpolate_gll(vs_n, axial)
@@ -416,6 +420,8 @@ def build_domain(mesh: StructuredMesh,
         axial_points=axial_points.astype(np.int32), dt=dt)
     spec._mesh = mesh
     spec._gq = gq
+    spec._vp1d = vp
+    spec._courant = courant
     return spec
 
 
@@ -526,5 +532,264 @@ def cartesian_homogeneous(nx=80, nz=100, width=80e3, depth=80e3, nu=2,
     phi = rng1.uniform(0.0, 2.0 * math.pi, n_receivers)
     add_receivers(spec, s_rec, np.full(n_receivers, depth), phi)
     spec.name = f"cartesian_iso1d_E{mesh.nelem}_nu{nu}"
+    spec._mt = mt
+    return spec
+
+
+# ------------------------------------------------------------- 3D media
+class RandomField:
+    """Seeded Gaussian random field with Gaussian covariance
+    exp(-|r|^2 / (2 l^2)) by random Fourier features:
+    d(x) = amp sqrt(2/K) sum_k cos(w_k . x + b_k), w_k ~ N(0, I / l^2),
+    clipped to [-clip, clip]."""
+
+    def __init__(self, seed, corr_len=50e3, nfeat=16, amp=0.015, clip=0.03):
+        rng = np.random.default_rng(seed)
+        self.w = rng.standard_normal((nfeat, 3)) / corr_len
+        self.b = rng.uniform(0.0, 2.0 * math.pi, nfeat)
+        self.amp = float(amp)
+        self.clip = float(clip)
+
+    @property
+    def features(self):
+        """[K][4] = wx, wy, wz, b (the layout the C++ generator takes)."""
+        return np.ascontiguousarray(np.concatenate([self.w, self.b[:, None]],
+                                                   axis=1))
+
+    def __call__(self, x, y, z):
+        ph = (x[..., None] * self.w[:, 0] + y[..., None] * self.w[:, 1] +
+              z[..., None] * self.w[:, 2] + self.b)
+        d = self.amp * math.sqrt(2.0 / len(self.b)) * np.cos(ph).sum(-1)
+        return np.clip(d, -self.clip, self.clip)
+
+
+def att_sls_fit(nsls, f_min, f_max):
+    """Relaxation frequencies w_l (log-spaced over the band) and weights
+    y_l fitting Q^-1(omega) = sum_l y_l omega w_l / (omega^2 + w_l^2) = 1
+    over the band in least squares.  (The reference reads w_l, y_l from
+    its external mesher; any such pair defines the SLS model the solver
+    then integrates, AttBuilder.cpp:25-36.)"""
+    w = 2 * math.pi * np.logspace(math.log10(f_min), math.log10(f_max), nsls)
+    om = 2 * math.pi * np.logspace(math.log10(f_min), math.log10(f_max), 100)
+    A = om[:, None] * w[None, :] / (om[:, None] ** 2 + w[None, :] ** 2)
+    y = np.linalg.lstsq(A, np.ones_like(om), rcond=None)[0]
+    return w, y
+
+
+def att_alpha_beta_gamma(w, y, dt):
+    """AttBuilder::setAlphaBetaGamma (AttBuilder.cpp:183-198)."""
+    yun = y / y.sum()
+    wdt = w * dt
+    alpha = np.exp(-wdt)
+    beta = yun * ((1.0 - alpha) / wdt - alpha)
+    gamma = yun * ((alpha - 1.0) / wdt + 1.0)
+    return alpha, beta, gamma
+
+
+def att_constants(w, y, f_min, f_max, f_ref):
+    """Constants of AttBuilder::createAttenuation (AttBuilder.cpp:107-112):
+    ysum, w10f, fact."""
+    ysum = float(y.sum())
+    w0 = f_ref * 2 * math.pi
+    w1 = math.sqrt(f_min * f_max) * 2 * math.pi
+    w10f = 2.0 * math.log(w1 / w0) / math.pi
+    fact = float(np.sum(y * w ** 2 / (w ** 2 + w1 * w1)) / ysum)
+    return ysum, w10f, fact
+
+
+CG4_POINTS = (6, 8, 16, 18)   # (1,1), (1,3), (3,1), (3,3)
+
+
+def cg4_weights(ifact):
+    """Quad::computeWeightsCG4 (Quad.cpp:487-511); ifact (E, 25)."""
+    f = ifact.reshape(-1, 5, 5)
+    w = np.empty((f.shape[0], 4))
+    w[:, 0] = (f[:, 0, 0] + f[:, 0, 1] + f[:, 1, 0] + f[:, 1, 1] +
+               0.50 * (f[:, 0, 2] + f[:, 1, 2] + f[:, 2, 0] + f[:, 2, 1]) +
+               0.25 * f[:, 2, 2]) / f[:, 1, 1]
+    w[:, 1] = (f[:, 0, 3] + f[:, 0, 4] + f[:, 1, 3] + f[:, 1, 4] +
+               0.50 * (f[:, 0, 2] + f[:, 1, 2] + f[:, 2, 3] + f[:, 2, 4]) +
+               0.25 * f[:, 2, 2]) / f[:, 1, 3]
+    w[:, 2] = (f[:, 3, 0] + f[:, 3, 1] + f[:, 4, 0] + f[:, 4, 1] +
+               0.50 * (f[:, 2, 0] + f[:, 2, 1] + f[:, 3, 2] + f[:, 4, 2]) +
+               0.25 * f[:, 2, 2]) / f[:, 3, 1]
+    w[:, 3] = (f[:, 3, 3] + f[:, 3, 4] + f[:, 4, 3] + f[:, 4, 4] +
+               0.50 * (f[:, 2, 3] + f[:, 2, 4] + f[:, 3, 2] + f[:, 4, 2]) +
+               0.25 * f[:, 2, 2]) / f[:, 3, 3]
+    return w
+
+
+def att_isotropic(lam, mu, qkp, qmu, consts, cg4, wcg4=None):
+    """Unrelaxed-modulus correction and memory-variable moduli of an
+    isotropic medium (Material::createIsotropic with an AttBuilder,
+    Material.cpp:585-598, and AttBuilder::createAttenuation,
+    AttBuilder.cpp:103-178).  Arrays (..., 25) broadcast together; wcg4
+    (..., 4).  Returns (lam, mu, dlam, dmu) with dlam/dmu (..., 4) for CG4
+    (weighted) or (..., 25) for FULL."""
+    ysum, w10f, fact = consts
+    kp = lam + (2.0 / 3.0) * mu
+    kp_na = 1.0 + w10f / qkp
+    dkp = kp_na / (qkp / ysum + (1.0 - fact))
+    kp_at = kp_na + dkp * fact
+    dkp = dkp * kp
+    mu_na = 1.0 + w10f / qmu
+    dmu = mu_na / (qmu / ysum + (1.0 - fact))
+    mu_at = mu_na + dmu * fact
+    dmu = dmu * mu
+    dlm = dkp - (2.0 / 3.0) * dmu
+    if cg4:
+        idx = list(CG4_POINTS)
+        dlm4 = wcg4 * dlm[..., idx]
+        dmu4 = wcg4 * dmu[..., idx]
+        kp = kp * kp_na
+        mu = mu * mu_na
+        kp[..., idx] *= 1.0 + wcg4 * (kp_at[..., idx] / kp_na[..., idx] - 1.0)
+        mu[..., idx] *= 1.0 + wcg4 * (mu_at[..., idx] / mu_na[..., idx] - 1.0)
+        out_d = (dlm4, dmu4)
+    else:
+        kp = kp * kp_at
+        mu = mu * mu_at
+        out_d = (dlm, dmu)
+    lam_c = kp - (2.0 / 3.0) * mu
+    return lam_c, mu, out_d[0], out_d[1]
+
+
+def make_3d_isotropic(spec, field, att=None):
+    """Turn every solid element into a 3D isotropic element: vp and vs of
+    the 1D model scaled by (1 + d(x, y, z)) with d the random field at the
+    GLL points of slice phi_k = 2 pi k / Nr (Model3D.cpp:32,87), rho kept
+    1D.  att: None or dict(nsls, cg4, f_min, f_max, f_ref, qkp, qmu)
+    with qkp, qmu (E, 25).  Numpy version for small domains; the C++
+    generator (synth.py) computes the same arrays for large ones."""
+    E = spec.n_solid_elements
+    nr = spec.elem_nr.astype(np.int64)
+    off = np.zeros(E + 1, dtype=np.int64)
+    off[1:] = np.cumsum(nr)
+    lam1, mu1 = spec.elem_iso[:, 0], spec.elem_iso[:, 1]
+    sz = spec._gq["sz"]
+    prop = np.empty((off[-1], 2, NPEM))
+    if att is not None:
+        w, y = att_sls_fit(att["nsls"], att["f_min"], att["f_max"])
+        consts = att_constants(w, y, att["f_min"], att["f_max"],
+                               att["f_ref"])
+        npa = 4 if att["cg4"] else NPEM
+        d3d = np.empty((off[-1], 2, npa))
+        wcg4 = cg4_weights(spec._gq["ifact"])
+    vmax = np.zeros(E)
+    for e in range(E):
+        phi = 2 * math.pi * np.arange(nr[e]) / nr[e]
+        s, z = sz[e, 0], sz[e, 1]
+        x = s[None, :] * np.cos(phi)[:, None]
+        yy = s[None, :] * np.sin(phi)[:, None]
+        zz = np.broadcast_to(z[None, :], x.shape)
+        sc = (1.0 + field(x, yy, zz)) ** 2
+        lam = lam1[e][None, :] * sc
+        mu = mu1[e][None, :] * sc
+        vmax[e] = (spec._vp1d[e][None, :] * np.sqrt(sc)).max()
+        if att is not None:
+            qk = np.broadcast_to(att["qkp"][e][None, :], lam.shape)
+            qm = np.broadcast_to(att["qmu"][e][None, :], lam.shape)
+            lam, mu, dl, dm = att_isotropic(lam.copy(), mu.copy(), qk, qm,
+                                            consts, att["cg4"],
+                                            wcg4[e][None, :])
+            d3d[off[e]:off[e + 1], 0] = dl
+            d3d[off[e]:off[e + 1], 1] = dm
+        prop[off[e]:off[e + 1], 0] = lam
+        prop[off[e]:off[e + 1], 1] = mu
+    spec.elem_kind = np.ones(E, dtype=np.uint8)
+    spec.elem_slice_off = off[:-1].copy()
+    spec.prop3d = prop
+    # dt from the 3D velocities (max over slices, Quad.cpp:306-339)
+    spec.dt = compute_dt(sz, vmax, spec._courant)
+    if att is not None:
+        spec.att = {"nsls": att["nsls"], "cg4": int(att["cg4"]),
+                    "w": w, "y": y, "d3d": d3d, "d1d": None,
+                    "f_min": att["f_min"], "f_max": att["f_max"],
+                    "f_ref": att["f_ref"]}
+        a, b, g = att_alpha_beta_gamma(w, y, spec.dt)
+        spec.att.update(alpha=a, beta=b, gamma=g)
+    return spec
+
+
+def layered_profile(seed, r_top, r_bot, nlayer=6, vs=(3200.0, 4800.0),
+                    vpvs=(1.73, 1.80), rho=(2600.0, 3600.0)):
+    """Seeded 1D model of `nlayer` equal-thickness layers, linear within
+    each layer: vs and rho increase with depth (sorted draws), vp/vs drawn
+    per layer.  Returns (interfaces, f) with f(r, r_center) ->
+    (rho, vp, vs) evaluated in the layer holding r_center."""
+    rng = np.random.default_rng(seed)
+    vs_v = np.sort(rng.uniform(vs[0], vs[1], 2 * nlayer))
+    rho_v = np.sort(rng.uniform(rho[0], rho[1], 2 * nlayer))
+    k = rng.uniform(vpvs[0], vpvs[1], nlayer)
+    ifc = np.linspace(r_top, r_bot, nlayer + 1)
+
+    def f(r, rc):
+        li = np.clip(((r_top - rc) / (r_top - r_bot) * nlayer).astype(int),
+                     0, nlayer - 1)
+        t = (ifc[li] - r) / (ifc[li] - ifc[li + 1])
+        vsr = vs_v[2 * li] + t * (vs_v[2 * li + 1] - vs_v[2 * li])
+        rhr = rho_v[2 * li] + t * (rho_v[2 * li + 1] - rho_v[2 * li])
+        return rhr, k[li] * vsr, vsr
+    return ifc, f
+
+
+def spherical_regional(ntheta=556, nrad=216, theta_max_deg=30.0,
+                       r_outer=6371e3, depth=1200e3, nu_min=10, nu_max=400,
+                       seed=11, attenuation=True, cg4=True, nsls=5,
+                       nsteps=10000, n_receivers=100, src_depth=50e3,
+                       f_peak=0.2, field=None, corr_len=50e3):
+    """Spherical sector (theta in [0, theta_max] from the axis, r in
+    [r_outer - depth, r_outer]) with a seeded 6-layer 1D model, per-point
+    Fourier order nu from nu_min (on the axis) to nu_max (at the sector
+    edge), a +-3% 3D random field on vp and vs with correlation length
+    corr_len, and 5-SLS attenuation with Q_mu ~ U[80, 600] per layer
+    (Q_kappa = 57823).  Axial moment-tensor source with a Ricker STF and
+    surface receivers."""
+    th = np.linspace(0.0, math.radians(theta_max_deg), ntheta + 1)
+    rr = np.linspace(r_outer - depth, r_outer, nrad + 1)
+    mesh = StructuredMesh(th, rr, spherical=True)
+    r_bot = r_outer - depth
+    ifc, prof = layered_profile(seed, r_outer, r_bot)
+    s_max = r_outer * math.sin(math.radians(theta_max_deg))
+
+    def nodal_nr(s, z):
+        nu = nu_min + (nu_max - nu_min) * np.clip(s / s_max, 0.0, 1.0)
+        return 2.0 * nu + 1.0
+
+    spec = build_domain(mesh, prof, nodal_nr)
+    if field is None:
+        field = RandomField(seed + 1, corr_len=corr_len)
+    att = None
+    if attenuation:
+        rng = np.random.default_rng(seed + 2)
+        qmu_layer = rng.uniform(80.0, 600.0, 6)
+        rc = np.hypot(spec._gq["sz"][:, 0], spec._gq["sz"][:, 1]).mean(1)
+        li = np.clip(((r_outer - rc) / depth * 6).astype(int), 0, 5)
+        qmu = np.repeat(qmu_layer[li][:, None], NPEM, axis=1)
+        att = dict(nsls=nsls, cg4=cg4, f_min=0.005, f_max=0.5, f_ref=1.0,
+                   qkp=np.full_like(qmu, 57823.0), qmu=qmu)
+    spec._field = field
+    spec._att_in = att
+    spec._build_3d = lambda: make_3d_isotropic(spec, field, att)
+    return spec
+
+
+def finish_regional(spec, nsteps, src_depth=50e3, f_peak=0.2,
+                    n_receivers=100, seed=11, theta_max_deg=30.0,
+                    r_outer=6371e3):
+    """Source (axial moment tensor, Ricker) and surface receivers of the
+    spherical workload, after its 3D model fixed dt."""
+    rng = np.random.default_rng(seed + 3)
+    mt = rng.standard_normal(6)
+    mt *= 1e16 / np.linalg.norm(mt)
+    f = math.pi * f_peak
+    decay = 1.628
+    add_axial_moment_source(spec, r_outer - src_depth, mt, decay / f, decay,
+                            nsteps)
+    th = np.linspace(math.radians(theta_max_deg) / n_receivers,
+                     math.radians(theta_max_deg), n_receivers)
+    phi = np.random.default_rng(seed + 4).uniform(0, 2 * math.pi,
+                                                  n_receivers)
+    add_receivers(spec, r_outer * np.sin(th), r_outer * np.cos(th), phi)
     spec._mt = mt
     return spec

@@ -59,8 +59,35 @@ class GpuSolver:
         self._chk(L.axs3d_set_solid_elements(ctx, ep.shape[0], ep.ctypes.data,
                                              ax.ctypes.data, gq.ctypes.data),
                   "set_elements")
-        iso = _c(s.elem_iso, np.float64)
-        self._chk(L.axs3d_set_solid_iso1d(ctx, iso.ctypes.data), "set_iso1d")
+        kind = getattr(s, "elem_kind", None)
+        if kind is None or np.any(np.asarray(kind) == 0):
+            iso = _c(s.elem_iso, np.float64)
+            self._chk(L.axs3d_set_solid_iso1d(ctx, iso.ctypes.data),
+                      "set_iso1d")
+        if kind is not None:
+            kd = _c(kind, np.uint8)
+            self._chk(L.axs3d_set_element_kinds(ctx, kd.ctypes.data),
+                      "set_kinds")
+            att = getattr(s, "att", None)
+            if att is not None:
+                a, b, g = (_c(att[k], np.float64)
+                           for k in ("alpha", "beta", "gamma"))
+                self._chk(L.axs3d_set_attenuation(
+                    ctx, int(att["nsls"]), int(att["cg4"]), a.ctypes.data,
+                    b.ctypes.data, g.ctypes.data), "set_attenuation")
+            if getattr(s, "prop3d", None) is not None:
+                p3 = _c(s.prop3d, np.float64)
+                self._chk(L.axs3d_set_solid_iso3d(ctx, 0, ep.shape[0],
+                                                  p3.ctypes.data),
+                          "set_iso3d")
+                if att is not None:
+                    d3 = _c(att["d3d"], np.float64)
+                    self._chk(L.axs3d_set_attenuation3d(
+                        ctx, 0, ep.shape[0], d3.ctypes.data),
+                        "set_attenuation3d")
+            elif getattr(s, "_upload_3d", None) is not None:
+                # large domains stream their 3D operands in chunks
+                s._upload_3d(self)
         axp = _c(s.axial_points, np.int32)
         self._chk(L.axs3d_set_axial_points(ctx, len(axp), axp.ctypes.data),
                   "set_axial")

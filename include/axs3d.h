@@ -52,11 +52,12 @@ enum axs3d_field {
 };
 
 enum axs3d_kernel_id {
-    AXS3D_KERNEL_STIFFNESS = 0,   /* all element stiffness launches      */
+    AXS3D_KERNEL_STIFFNESS = 0,   /* Fourier-path (1D) element stiffness */
     AXS3D_KERNEL_POINT = 1,       /* fused BC/mass/Newmark point update  */
     AXS3D_KERNEL_SOURCE = 2,
     AXS3D_KERNEL_RECORD = 3,
-    AXS3D_NUM_KERNELS = 4
+    AXS3D_KERNEL_STIFFNESS_3D = 4,/* cardinal-path (3D) element stiffness */
+    AXS3D_NUM_KERNELS = 5
 };
 
 /* Context.  Replaces the construction of Domain + NewmarkTimeScheme in
@@ -91,6 +92,38 @@ int axs3d_set_solid_elements(axs3d_ctx *ctx, int n, const int32_t *points,
  * anisotropy/Isotropic.hpp:24-34) built by Material::createIsotropic
  * (src/preloop/se_model/quad/physical/Material.cpp:566-582). */
 int axs3d_set_solid_iso1d(axs3d_ctx *ctx, const double *lam_mu);
+
+/* Element kinds, one per solid element: 0 = 1D material (Fourier path),
+ * 1 = 3D isotropic material (cardinal path: FFT to Nr slices, stress per
+ * slice, FFT back).  Replaces the Fourier/cardinal branch of
+ * SolidElement::displToStiff (src/core/element/SolidElement.cpp:160-198)
+ * selected by Elastic::is1D (Elastic.hpp:37).  Set before 3D properties. */
+int axs3d_set_element_kinds(axs3d_ctx *ctx, const uint8_t *kind);
+
+/* 3D isotropic material of the 3D elements in [e_begin, e_end): for each
+ * such element in order, Nr slices (Nr = element Nr, max over its points)
+ * of [2][25] = lambda, mu at phi_k = 2 pi k / Nr.  May be called in
+ * chunks.  Replaces the Isotropic 3D constructor (Isotropic.hpp:36-46,
+ * Property<N> 3D data Nr x 25, src/core/element/common/Property.hpp:39-42). */
+int axs3d_set_solid_iso3d(axs3d_ctx *ctx, int e_begin, int e_end,
+                          const double *lam_mu);
+
+/* Standard linear solids: alpha, beta, gamma per SLS
+ * (AttBuilder::setAlphaBetaGamma, src/preloop/se_model/quad/physical/
+ * AttBuilder.cpp:183-198 -> Attenuation::setAlphaBetaGamma,
+ * src/core/element/material/elastic/attenuation/Attenuation.hpp:144-150);
+ * cg4 = 1 selects AttenuationCG4 (4 points), 0 AttenuationFull. */
+int axs3d_set_attenuation(axs3d_ctx *ctx, int nsls, int cg4,
+                          const double *alpha, const double *beta,
+                          const double *gamma);
+
+/* dlambda, dmu of the attenuation of the 3D elements in [e_begin, e_end):
+ * for each such element, Nr slices of [2][npa], npa = 4 (CG4, points
+ * (1,1),(1,3),(3,1),(3,3), already scaled by the CG4 weights) or 25.
+ * Replaces the AttenuationCG4/Full 3D constructors built by
+ * AttBuilder::createAttenuation (AttBuilder.cpp:103-178). */
+int axs3d_set_attenuation3d(axs3d_ctx *ctx, int e_begin, int e_end,
+                            const double *dlam_dmu);
 
 /* Solid points on the axis.  Replaces AxialBoundary::addPoint
  * (src/core/boundary/axial/AxialBoundary.hpp). */

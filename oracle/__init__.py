@@ -43,6 +43,9 @@ class OracleDomain(C.Structure):
         ("offsets", _p),
         ("n_elem", C.c_int), ("elem_points", _p), ("elem_nr", _p),
         ("elem_axial", _p), ("elem_gq", _p), ("elem_iso", _p),
+        ("elem_kind", _p), ("elem_slice_off", _p), ("prop3d", _p),
+        ("nsls", C.c_int), ("att_cg4", C.c_int), ("att_abg", _p),
+        ("att1d", _p), ("att3d", _p), ("att_state", _p),
         ("n_axial", C.c_int), ("axial_points", _p),
         ("src_elem", C.c_int), ("src_nu1", C.c_int), ("src_pattern", _p),
         ("stf", _p),
@@ -68,6 +71,13 @@ def _setup(L):
     L.oracle_axial_solid.argtypes = [C.POINTER(OracleDomain), _p]
     L.oracle_stiff_to_accel.argtypes = [C.POINTER(OracleDomain), _p]
     L.oracle_newmark.argtypes = [C.c_int64, C.c_double, _p, _p, _p, _p]
+    L.oracle_r2c.argtypes = [_p, C.c_int, _p]
+    L.oracle_c2r.argtypes = [_p, C.c_int, _p]
+    L.oracle_att_alloc.argtypes = [C.POINTER(OracleDomain)]
+    L.oracle_att_alloc.restype = _p
+    L.oracle_att_free.argtypes = [_p, C.c_int]
+    L.oracle_att_apply_cd.argtypes = [C.POINTER(OracleDomain), C.c_int,
+                                      C.c_int, _p, _p]
     from axisem3d_amd import spectral
     t = spectral.tables()
     g_gll = np.ascontiguousarray(t[4])
@@ -107,6 +117,23 @@ def iso_stress(lam, mu, eps):
     out = np.zeros((6, 25), dtype=np.complex128)
     lib().oracle_iso_stress(lam.ctypes.data, mu.ctypes.data, eps.ctypes.data,
                             out.ctypes.data)
+    return out
+
+
+def r2c(x):
+    """computeR2C of one real series (reference FFT conventions)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros(len(x) // 2 + 1, dtype=np.complex128)
+    lib().oracle_r2c(x.ctypes.data, len(x), out.ctypes.data)
+    return out
+
+
+def c2r(c, nr):
+    """computeC2R of one Hermitian series of nr // 2 + 1 coefficients."""
+    c = np.ascontiguousarray(c, dtype=np.complex128)
+    assert len(c) == nr // 2 + 1
+    out = np.zeros(nr)
+    lib().oracle_c2r(c.ctypes.data, nr, out.ctypes.data)
     return out
 
 
@@ -157,8 +184,32 @@ class Oracle:
                                                          dtype=np.float64))
             d.rec_phi = self._k(np.ascontiguousarray(spec.rec_phi,
                                                      dtype=np.float64))
+        kind = getattr(spec, "elem_kind", None)
+        if kind is not None:
+            d.elem_kind = self._k(np.ascontiguousarray(kind, dtype=np.uint8))
+            d.elem_slice_off = self._k(np.ascontiguousarray(
+                spec.elem_slice_off, dtype=np.int64))
+            d.prop3d = self._k(np.ascontiguousarray(spec.prop3d,
+                                                    dtype=np.float64))
+        att = getattr(spec, "att", None)
+        if att is not None:
+            d.nsls = int(att["nsls"])
+            d.att_cg4 = int(att["cg4"])
+            d.att_abg = self._k(np.ascontiguousarray(
+                np.concatenate([att["alpha"], att["beta"], att["gamma"]]),
+                dtype=np.float64))
+            if att.get("d1d") is not None:
+                d.att1d = self._k(np.ascontiguousarray(att["d1d"],
+                                                       dtype=np.float64))
+            if att.get("d3d") is not None:
+                d.att3d = self._k(np.ascontiguousarray(att["d3d"],
+                                                       dtype=np.float64))
         d.dt = float(spec.dt)
         self.d = d
+        self._att_state = None
+        if d.nsls > 0:
+            self._att_state = lib().oracle_att_alloc(C.byref(d))
+            d.att_state = self._att_state
         z = lambda: np.zeros((self.nmodes, 3), dtype=np.complex128)
         self.displ, self.veloc, self.accel, self.stiff = z(), z(), z(), z()
         self.step = 0
@@ -166,6 +217,14 @@ class Oracle:
     def _k(self, a):
         self._keep.append(a)
         return a.ctypes.data
+
+    def __del__(self):
+        try:
+            if self._att_state:
+                lib().oracle_att_free(self._att_state, self.d.n_elem)
+                self._att_state = None
+        except Exception:
+            pass
 
     def run(self, nsteps, record=True):
         nrec = self.d.n_rec
@@ -230,3 +289,11 @@ class Oracle:
         lib().oracle_record(C.byref(self.d), displ.ctypes.data,
                             out.ctypes.data)
         return out[:self.d.n_rec]
+
+    def att_apply_cd(self, e, slice_, strain, stress):
+        """Attenuation of one slice (strain, stress: (6, 25) real)."""
+        strain = np.ascontiguousarray(strain, dtype=np.float64)
+        stress = np.ascontiguousarray(stress, dtype=np.float64).copy()
+        lib().oracle_att_apply_cd(C.byref(self.d), int(e), int(slice_),
+                                  strain.ctypes.data, stress.ctypes.data)
+        return stress

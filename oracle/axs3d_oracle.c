@@ -19,6 +19,9 @@
  */
 #include <complex.h>
 #include <math.h>
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -223,6 +226,195 @@ void oracle_iso_stress(const double *lam, const double *mu, const cplx *e,
     }
 }
 
+
+/* ------------------------------------------------------------------ */
+/* FFT (SolverFFTW conventions, src/core/fft/SolverFFTW.hpp:101-168)   */
+/* ------------------------------------------------------------------ */
+static int smallest_prime_factor(int n) {
+    for (int p = 2; p * p <= n; p++)
+        if (n % p == 0) return p;
+    return n;
+}
+
+/* out[k] = sum_j in[j*is] W^{jk}, W = exp(sign 2 pi i / n) read from the
+ * table of the top-level length N: W_n^x = tw[(x * L) % N], L = N / n.
+ * Mixed radix, decimation in time on the smallest prime factor. */
+static void fft_rec(const cplx *in, cplx *out, int n, int is,
+                    const cplx *tw, int nt) {
+    if (n == 1) {
+        out[0] = in[0];
+        return;
+    }
+    int p = smallest_prime_factor(n), m = n / p, L = nt / n;
+    for (int r = 0; r < p; r++)
+        fft_rec(in + (int64_t)r * is, out + (int64_t)r * m, m, is * p, tw, nt);
+    cplx tloc[64];
+    cplx *t = p <= 64 ? tloc : (cplx *)malloc(sizeof(cplx) * p);
+    for (int k = 0; k < m; k++) {
+        for (int r = 0; r < p; r++)
+            t[r] = out[r * m + k] * tw[((int64_t)r * k * L) % nt];
+        for (int q = 0; q < p; q++) {
+            cplx acc = 0;
+            for (int r = 0; r < p; r++)
+                acc += t[r] * tw[((int64_t)((r * q) % p) * m * L) % nt];
+            out[q * m + k] = acc;
+        }
+    }
+    if (t != tloc) free(t);
+}
+
+static void cfft(const cplx *in, cplx *out, int n, int sign) {
+    cplx *tw = (cplx *)malloc(sizeof(cplx) * n);
+    for (int j = 0; j < n; j++)
+        tw[j] = cexp(sign * 2. * M_PI * I * (double)j / (double)n);
+    fft_rec(in, out, n, 1, tw, n);
+    free(tw);
+}
+
+/* computeR2C: X_k = (1/N) sum_n x_n e^{-2 pi i k n / N}, k <= N/2, Nyquist
+ * (N even) halved (SolverFFTW.hpp:101-116). */
+void oracle_r2c(const double *r, int nr, cplx *c) {
+    cplx *a = (cplx *)malloc(sizeof(cplx) * nr * 2);
+    for (int k = 0; k < nr; k++) a[k] = r[k] * (1. / nr);
+    cfft(a, a + nr, nr, -1);
+    int nc = nr / 2 + 1;
+    for (int k = 0; k < nc; k++) c[k] = a[nr + k];
+    if (nr % 2 == 0) c[nc - 1] *= .5;
+    free(a);
+}
+
+/* computeC2R: Nyquist (N even) doubled, then FFTW's unnormalised c2r on the
+ * Hermitian extension, which takes only the real parts of X_0 and X_{N/2}
+ * (SolverFFTW.hpp:138-153). */
+void oracle_c2r(const cplx *c, int nr, double *r) {
+    cplx *a = (cplx *)malloc(sizeof(cplx) * nr * 2);
+    int nc = nr / 2 + 1;
+    for (int k = 0; k < nr; k++) a[k] = 0;
+    a[0] = creal(c[0]);
+    for (int k = 1; k < nc; k++) {
+        if (nr % 2 == 0 && k == nc - 1) {
+            a[k] = 2. * creal(c[k]);
+        } else {
+            a[k] = c[k];
+            a[nr - k] = conj(c[k]);
+        }
+    }
+    cfft(a, a + nr, nr, +1);
+    for (int k = 0; k < nr; k++) r[k] = creal(a[nr + k]);
+    free(a);
+}
+
+/* ------------------------------------------------------------------ */
+/* Attenuation (src/core/element/material/elastic/attenuation)        */
+/* ------------------------------------------------------------------ */
+/* CG4 points (1,1),(1,3),(3,1),(3,3) (AttenuationCG4.hpp:203-212,
+ * AttBuilder.cpp:141-147). */
+static const int kCG4[4] = {6, 8, 16, 18};
+
+typedef struct {
+    int n;          /* rows: nu1 (FR) or Nr (CD) */
+    int fr;         /* 1: complex Fourier rows   */
+    double *dstress;      /* [n][6][npa] x (fr ? 2 : 1) */
+    double *memvar;       /* [nsls][n][6][npa] x (fr ? 2 : 1) */
+} att_elem;
+
+static int att_npa(const oracle_domain *d) { return d->att_cg4 ? 4 : 25; }
+
+static int elem_is3d(const oracle_domain *d, int e) {
+    return d->elem_kind && d->elem_kind[e] == 1;
+}
+
+void *oracle_att_alloc(const oracle_domain *d) {
+    if (d->nsls <= 0) return NULL;
+    att_elem *st = (att_elem *)calloc(d->n_elem, sizeof(att_elem));
+    int npa = att_npa(d);
+    for (int e = 0; e < d->n_elem; e++) {
+        int nr = d->elem_nr[e];
+        st[e].fr = !elem_is3d(d, e);
+        st[e].n = st[e].fr ? nr / 2 + 1 : nr;
+        int w = st[e].fr ? 2 : 1;
+        size_t sz = (size_t)st[e].n * 6 * npa * w;
+        st[e].dstress = (double *)calloc(sz, sizeof(double));
+        st[e].memvar = (double *)calloc(sz * d->nsls, sizeof(double));
+    }
+    return st;
+}
+
+void oracle_att_free(void *state, int n_elem) {
+    att_elem *st = (att_elem *)state;
+    if (!st) return;
+    for (int e = 0; e < n_elem; e++) {
+        free(st[e].dstress);
+        free(st[e].memvar);
+    }
+    free(st);
+}
+
+/* AttenuationFull/CG4::apply on one row x (one Fourier order or one slice):
+ * stress -= sum_l m_l; m_l = alpha_l m_l + beta_l dS; dS = iso(dlam,dmu)(e);
+ * m_l += gamma_l dS  (AttenuationFull.hpp:192-207, AttenuationCG4.hpp:312-330,
+ * Attenuation.hpp:92-140).  T = cplx (FR) or double (CD) via macro. */
+#define ATT_ROW(T, strain, stress, dS, mv, dl, dm)                           \
+    do {                                                                    \
+        const double *al = d->att_abg, *be = al + d->nsls,                  \
+                     *ga = be + d->nsls;                                    \
+        int npa = att_npa(d);                                               \
+        for (int l = 0; l < d->nsls; l++)                                   \
+            for (int dd = 0; dd < 6; dd++)                                  \
+                for (int q = 0; q < npa; q++) {                             \
+                    int pt = d->att_cg4 ? kCG4[q] : q;                      \
+                    stress[dd * N + pt] -= mv[(l * rows + x) * 6 * npa +    \
+                                              dd * npa + q];                \
+                }                                                           \
+        for (int l = 0; l < d->nsls; l++)                                   \
+            for (int k = 0; k < 6 * npa; k++) {                             \
+                T *m = &mv[(l * rows + x) * 6 * npa + k];                   \
+                *m = al[l] * *m + be[l] * dS[x * 6 * npa + k];              \
+            }                                                               \
+        for (int q = 0; q < npa; q++) {                                     \
+            int pt = d->att_cg4 ? kCG4[q] : q;                              \
+            T tr = (strain[0 * N + pt] + strain[1 * N + pt] +               \
+                    strain[2 * N + pt]) * dl[q];                            \
+            double dm2 = dm[q] * 2.;                                        \
+            T *o = &dS[x * 6 * npa];                                        \
+            o[0 * npa + q] = tr + strain[0 * N + pt] * dm2;                 \
+            o[1 * npa + q] = tr + strain[1 * N + pt] * dm2;                 \
+            o[2 * npa + q] = tr + strain[2 * N + pt] * dm2;                 \
+            o[3 * npa + q] = strain[3 * N + pt] * dm[q];                    \
+            o[4 * npa + q] = strain[4 * N + pt] * dm[q];                    \
+            o[5 * npa + q] = strain[5 * N + pt] * dm[q];                    \
+        }                                                                   \
+        for (int l = 0; l < d->nsls; l++)                                   \
+            for (int k = 0; k < 6 * npa; k++)                               \
+                mv[(l * rows + x) * 6 * npa + k] += ga[l] * dS[x * 6 * npa + k]; \
+    } while (0)
+
+static void att_fr(const oracle_domain *d, int e, int alpha, const cplx *strain,
+                   cplx *stress) {
+    att_elem *st = &((att_elem *)d->att_state)[e];
+    int npa = att_npa(d), rows = st->n, x = alpha;
+    cplx *dS = (cplx *)st->dstress, *mv = (cplx *)st->memvar;
+    const double *dl = d->att1d + (int64_t)e * 2 * npa, *dm = dl + npa;
+    ATT_ROW(cplx, strain, stress, dS, mv, dl, dm);
+}
+
+static void att_cd(const oracle_domain *d, int e, int slice,
+                   const double *strain, double *stress) {
+    att_elem *st = &((att_elem *)d->att_state)[e];
+    int npa = att_npa(d), rows = st->n, x = slice;
+    double *dS = st->dstress, *mv = st->memvar;
+    const double *dl = d->att3d +
+                       (d->elem_slice_off[e] + slice) * 2 * (int64_t)npa;
+    const double *dm = dl + npa;
+    ATT_ROW(double, strain, stress, dS, mv, dl, dm);
+}
+
+/* Test hook: attenuation of one slice of a 3D element (cardinal path). */
+void oracle_att_apply_cd(const oracle_domain *d, int e, int slice,
+                         const double *strain, double *stress) {
+    att_cd(d, e, slice, strain, stress);
+}
+
 /* ------------------------------------------------------------------ */
 /* Domain / time loop                                                  */
 /* ------------------------------------------------------------------ */
@@ -233,20 +425,26 @@ static int64_t *make_offsets(int n, const int32_t *nr) {
     return off;
 }
 
-/* SolidElement::computeStiff (SolidElement.cpp:213-222) for the Fourier
- * path of a 1D isotropic element, no PRT:
+/* SolidElement::computeStiff (SolidElement.cpp:213-222), no PRT:
  * collectDisplFromPoints (SolidElement.cpp:102-109, zero-padded,
- * SolidPoint.hpp:82-98) -> computeGrad6 -> strainToStress_FR per mode ->
- * computeQuad6 -> addStiffToPoints (subtract, SolidPoint.hpp:101-109). */
+ * SolidPoint.hpp:82-98) -> displToStiff (SolidElement.cpp:160-198) ->
+ * addStiffToPoints (subtract, SolidPoint.hpp:101-109).
+ * 1D material (Fourier path): computeGrad6 -> strainToStress_FR per mode
+ * (+ attenuation per mode) -> computeQuad6.
+ * 3D isotropic (cardinal path): computeGrad6 -> gFFT_N6.computeC2R ->
+ * Isotropic::strainToStress<_3D_CD> per slice (+ attenuation per slice) ->
+ * gFFT_N6.computeR2C -> computeQuad6. */
 void oracle_solid_element_stiff(const oracle_domain *d, int e,
                                 const cplx *displ, cplx *stiff) {
     const int32_t *pts = d->elem_points + (int64_t)e * N;
-    int nu1 = d->elem_nr[e] / 2 + 1;
+    int nr = d->elem_nr[e];
+    int nu1 = nr / 2 + 1;
     const double *gq_e = d->elem_gq + (int64_t)e * 6 * N;
-    const double *lam = d->elem_iso + (int64_t)e * 2 * N;
-    const double *mu = lam + N;
     int axial = d->elem_axial[e];
-    cplx u[3 * N], eps[6 * N], sig[6 * N], f[3 * N];
+    int is3d = elem_is3d(d, e);
+    cplx *eps = (cplx *)malloc(sizeof(cplx) * nu1 * 6 * N);
+    cplx *sig = (cplx *)malloc(sizeof(cplx) * nu1 * 6 * N);
+    cplx u[3 * N], f[3 * N];
     for (int alpha = 0; alpha < nu1; alpha++) {
         for (int p = 0; p < N; p++) {
             int ip = pts[p];
@@ -255,9 +453,58 @@ void oracle_solid_element_stiff(const oracle_domain *d, int e,
                 u[c * N + p] = alpha < pnu1
                     ? displ[(d->offsets[ip] + alpha) * 3 + c] : 0.;
         }
-        oracle_grad6(gq_e, axial, alpha, u, eps);
-        oracle_iso_stress(lam, mu, eps, sig);
-        oracle_quad6(gq_e, axial, alpha, sig, f);
+        oracle_grad6(gq_e, axial, alpha, u, eps + (int64_t)alpha * 6 * N);
+    }
+    if (!is3d) {
+        const double *lam = d->elem_iso + (int64_t)e * 2 * N;
+        const double *mu = lam + N;
+        for (int alpha = 0; alpha < nu1; alpha++) {
+            oracle_iso_stress(lam, mu, eps + (int64_t)alpha * 6 * N,
+                              sig + (int64_t)alpha * 6 * N);
+            if (d->nsls > 0 && d->att1d)
+                att_fr(d, e, alpha, eps + (int64_t)alpha * 6 * N,
+                       sig + (int64_t)alpha * 6 * N);
+        }
+    } else {
+        double *ecd = (double *)malloc(sizeof(double) * nr * 6 * N);
+        double *scd = (double *)malloc(sizeof(double) * nr * 6 * N);
+        cplx *ser = (cplx *)malloc(sizeof(cplx) * nu1);
+        double *rser = (double *)malloc(sizeof(double) * nr);
+        for (int col = 0; col < 6 * N; col++) {
+            for (int a = 0; a < nu1; a++) ser[a] = eps[(int64_t)a * 6 * N + col];
+            oracle_c2r(ser, nr, rser);
+            for (int k = 0; k < nr; k++) ecd[(int64_t)k * 6 * N + col] = rser[k];
+        }
+        for (int k = 0; k < nr; k++) {
+            const double *lam = d->prop3d + (d->elem_slice_off[e] + k) * 2 * N;
+            const double *mu = lam + N;
+            const double *ek = ecd + (int64_t)k * 6 * N;
+            double *sk = scd + (int64_t)k * 6 * N;
+            for (int p = 0; p < N; p++) {
+                double tr = (ek[0 * N + p] + ek[1 * N + p] + ek[2 * N + p]) *
+                            lam[p];
+                double mu2 = mu[p] * 2.;
+                sk[0 * N + p] = tr + ek[0 * N + p] * mu2;
+                sk[1 * N + p] = tr + ek[1 * N + p] * mu2;
+                sk[2 * N + p] = tr + ek[2 * N + p] * mu2;
+                sk[3 * N + p] = ek[3 * N + p] * mu[p];
+                sk[4 * N + p] = ek[4 * N + p] * mu[p];
+                sk[5 * N + p] = ek[5 * N + p] * mu[p];
+            }
+            if (d->nsls > 0 && d->att3d) att_cd(d, e, k, ek, sk);
+        }
+        for (int col = 0; col < 6 * N; col++) {
+            for (int k = 0; k < nr; k++) rser[k] = scd[(int64_t)k * 6 * N + col];
+            oracle_r2c(rser, nr, ser);
+            for (int a = 0; a < nu1; a++) sig[(int64_t)a * 6 * N + col] = ser[a];
+        }
+        free(ecd);
+        free(scd);
+        free(ser);
+        free(rser);
+    }
+    for (int alpha = 0; alpha < nu1; alpha++) {
+        oracle_quad6(gq_e, axial, alpha, sig + (int64_t)alpha * 6 * N, f);
         for (int p = 0; p < N; p++) {
             int ip = pts[p];
             int pnu1 = d->point_nr[ip] / 2 + 1;
@@ -266,6 +513,8 @@ void oracle_solid_element_stiff(const oracle_domain *d, int e,
                     stiff[(d->offsets[ip] + alpha) * 3 + c] -= f[c * N + p];
         }
     }
+    free(eps);
+    free(sig);
 }
 
 /* SolidMoment::apply + SolidElement::addMomentSource (SolidMoment.cpp:29-33,

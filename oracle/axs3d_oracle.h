@@ -21,7 +21,18 @@ typedef struct {
     const int32_t *elem_nr;       /* element Nr = max of its points      */
     const uint8_t *elem_axial;
     const double *elem_gq;        /* [e][6][25]                          */
-    const double *elem_iso;       /* [e][2][25] lambda, mu               */
+    const double *elem_iso;       /* [e][2][25] lambda, mu (1D elements) */
+    const uint8_t *elem_kind;     /* NULL or per element: 0 1D (Fourier
+                                     path), 1 3D isotropic (cardinal)    */
+    const int64_t *elem_slice_off;/* 3D: first slice of element e        */
+    const double *prop3d;         /* 3D: [slice][2][25] lambda, mu       */
+    /* attenuation (standard linear solids) */
+    int nsls;                     /* 0: none                             */
+    int att_cg4;                  /* 1: CG4 (4 points), 0: FULL (25)     */
+    const double *att_abg;        /* [3][nsls] alpha, beta, gamma        */
+    const double *att1d;          /* [e][2][npa] dlambda, dmu (1D)       */
+    const double *att3d;          /* [slice][2][npa] dlambda, dmu (3D)   */
+    void *att_state;              /* owned, see oracle_att_alloc          */
     /* axis */
     int n_axial;
     const int32_t *axial_points;
@@ -64,6 +75,12 @@ int oracle_run(const oracle_domain *d, int step_begin, int nsteps,
                double complex *displ, double complex *veloc,
                double complex *accel, double complex *stiff, double *seismo);
 int64_t *oracle_make_offsets(int n, const int32_t *nr);
+void oracle_r2c(const double *r, int nr, double complex *c);
+void oracle_c2r(const double complex *c, int nr, double *r);
+void *oracle_att_alloc(const oracle_domain *d);
+void oracle_att_free(void *state, int n_elem);
+void oracle_att_apply_cd(const oracle_domain *d, int e, int slice,
+                         const double *strain, double *stress);
 void oracle_free(void *p);
 
 #endif

@@ -1,0 +1,266 @@
+"""Pins the FP64 oracle (oracle/axs3d_oracle.c) without the reference:
+golden tables, closed forms, identities and textbook special cases, each
+chosen so that a plausible slip (a dropped term, a wrong sign or index, a
+transposed operand) fails.  CPU only."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from axisem3d_amd import preloop, spectral
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+# --------------------------------------------------------------- tables
+def test_tables_equal_reference_literals():
+    ref = json.load(open(os.path.join(HERE, "golden", "spectral_npol4.json")))
+    xg, xj, wg, wj, gg, gj = spectral.tables()
+    for name, arr in (("pGLL", xg), ("pGLJ", xj), ("wGLL", wg),
+                      ("wGLJ", wj), ("gGLL", gg.ravel()), ("gGLJ", gj.ravel())):
+        assert np.array_equal(np.array(ref[name]), arr), name
+
+
+def test_tables_closed_form_identities():
+    xg, xj, wg, wj, gg, gj = spectral.tables(exact=True)
+    for x, G, w, b in ((xg, gg, wg, 0), (xj, gj, wj, 1)):
+        # derivative of sum_i l_i = 1 vanishes
+        assert np.abs(G.sum(axis=0)).max() < 1e-12
+        # G differentiates every polynomial of degree <= 4 exactly
+        for deg in range(5):
+            f = x ** deg
+            df = deg * x ** (deg - 1) if deg else np.zeros_like(x)
+            assert np.abs(f @ G - df).max() < 1e-12
+        # quadrature integrates x^k (1+x)^b exactly up to its order
+        for k in range(2 * 4 - 1 - b):
+            exact = np.polynomial.polynomial.polyint(
+                np.polynomial.polynomial.polymul([0] * k + [1],
+                                                 [1, 1] if b else [1]))
+            val = np.polynomial.polynomial.polyval(1, exact) - \
+                np.polynomial.polynomial.polyval(-1, exact)
+            assert abs(np.sum(w * x ** k) - val) < 1e-12
+
+
+def test_lucky_numbers_equal_reference_table():
+    ref = json.load(open(os.path.join(HERE, "golden", "spectral_npol4.json")))
+    assert list(spectral._lucky_numbers()) == ref["lucky"]
+    assert spectral.next_lucky_number(17) == 18
+    assert spectral.next_lucky_number(143) == 144     # 11*13 is not lucky
+    assert spectral.next_lucky_number(121) == 125     # 11^2 is not lucky
+    assert spectral.next_lucky_number(100001) == 100001
+
+
+# --------------------------------------------------------------- gradient
+def _element(axial, s0=1000.0, ds=800.0, z0=2000.0, dz=600.0):
+    """One rectangular element; axial elements start on the axis."""
+    s0 = 0.0 if axial else s0
+    mesh = preloop.StructuredMesh(np.array([s0, s0 + ds]),
+                                  np.array([z0, z0 + dz]))
+    gq = preloop.gradquad_operands(mesh)
+    ops = np.stack([gq["dsdxii"], gq["dsdeta"], gq["dzdxii"], gq["dzdeta"],
+                    gq["inv_s"], gq["ifact"]], axis=1)[0]
+    return ops, gq["sz"][0]
+
+
+def _poly(rng, s, z, deg=4):
+    c = rng.standard_normal((deg + 1, deg + 1)) + 1j * rng.standard_normal(
+        (deg + 1, deg + 1))
+    sc, zc = 1e-3, 1e-3
+
+    def f(s, z):
+        return sum(c[a, b] * (s * sc) ** a * (z * zc) ** b
+                   for a in range(deg + 1) for b in range(deg + 1 - a))
+
+    def fs(s, z):
+        return sum(c[a, b] * a * sc * (s * sc) ** (a - 1) * (z * zc) ** b
+                   for a in range(1, deg + 1) for b in range(deg + 1 - a))
+
+    def fz(s, z):
+        return sum(c[a, b] * b * zc * (s * sc) ** a * (z * zc) ** (b - 1)
+                   for a in range(deg + 1) for b in range(1, deg + 1 - a))
+    return f, fs, fz
+
+
+@pytest.mark.parametrize("alpha", [0, 1, 2, 5])
+def test_grad6_closed_form_off_axis(alpha):
+    ops, sz = _element(False)
+    s, z = sz
+    rng = np.random.default_rng(alpha)
+    fields = [_poly(rng, s, z) for _ in range(3)]
+    u = np.array([f[0](s, z) for f in fields])
+    us, up, uz = u
+    d = [(f[1](s, z), f[2](s, z)) for f in fields]
+    ia = 1j * alpha
+    expect = np.array([
+        d[0][0],                               # ss
+        (ia * up + us) / s,                    # pp
+        d[2][1],                               # zz
+        d[1][1] + ia * uz / s,                 # pz (engineering)
+        d[0][1] + d[2][0],                     # sz (engineering)
+        d[1][0] + (ia * us - up) / s,          # sp (engineering)
+    ])
+    got = oracle.grad6(ops, 0, alpha, u)
+    assert np.abs(got - expect).max() < 1e-9 * np.abs(expect).max()
+
+
+@pytest.mark.parametrize("alpha", [0, 1, 2, 3])
+def test_grad6_closed_form_on_axis(alpha):
+    """Axis-regular fields: the lower-order numerators vanish on s = 0 and
+    the L'Hospital limit is the s-derivative of the numerator."""
+    ops, sz = _element(True)
+    s, z = sz
+    rng = np.random.default_rng(10 + alpha)
+    g = [_poly(rng, s, z, deg=3) for _ in range(3)]
+    # u = s * g (vanishes on the axis) except mode-1 s/phi pairs
+    us = s * g[0][0](s, z)
+    up = s * g[1][0](s, z)
+    uz = s * g[2][0](s, z) if alpha > 0 else g[2][0](s, z)
+    dus_s = g[0][0](s, z) + s * g[0][1](s, z)
+    dup_s = g[1][0](s, z) + s * g[1][1](s, z)
+    duz_s = (g[2][0](s, z) + s * g[2][1](s, z)) if alpha > 0 else g[2][1](s, z)
+    dus_z = s * g[0][2](s, z)
+    dup_z = s * g[1][2](s, z)
+    duz_z = s * g[2][2](s, z) if alpha > 0 else g[2][2](s, z)
+    ia = 1j * alpha
+    on = s == 0.0
+    sd = np.where(on, 1.0, s)
+    pp = np.where(on, ia * dup_s + dus_s, (ia * up + us) / sd)
+    pz = dup_z + np.where(on, ia * duz_s, ia * uz / sd)
+    sp = dup_s + np.where(on, ia * dus_s - dup_s, (ia * us - up) / sd)
+    expect = np.array([dus_s, pp, duz_z, pz, dus_z + duz_s, sp])
+    got = oracle.grad6(ops, 1, alpha, np.array([us, up, uz]))
+    assert np.abs(got - expect).max() < 1e-9 * np.abs(expect).max()
+
+
+@pytest.mark.parametrize("axial", [False, True])
+def test_rigid_body_null_space(axial):
+    ops, sz = _element(axial)
+    s, z = sz
+    one, zero = np.ones(25), np.zeros(25)
+    cases = [(0, [zero, zero, one]),              # translation along z
+             (0, [zero, s, zero]),                # rotation about z
+             (1, [0.5 * one, 0.5j * one, zero])]  # translation along x
+    for alpha, u in cases:
+        e = oracle.grad6(ops, int(axial), alpha, np.array(u, complex))
+        # the cancelling terms are O(1); the 12-decimal G tables leave
+        # O(1e-12) residue
+        assert np.abs(e).max() < 1e-10
+
+
+@pytest.mark.parametrize("axial", [False, True])
+@pytest.mark.parametrize("alpha", [0, 1, 3])
+def test_quad_is_integral_weighted_transpose_of_grad(axial, alpha):
+    """<Quad(sig), v> = <sig, IF * Grad_{-alpha}(v)>: the quadrature is the
+    transpose of the gradient weighted by the integral factor (test
+    functions carry e^{-i alpha phi})."""
+    ops, _ = _element(axial)
+    rng = np.random.default_rng(alpha + 7 * axial)
+    sig = rng.standard_normal((6, 25)) + 1j * rng.standard_normal((6, 25))
+    v = rng.standard_normal((3, 25)) + 1j * rng.standard_normal((3, 25))
+    lhs = np.sum(oracle.quad6(ops, int(axial), alpha, sig) * v)
+    rhs = np.sum(ops[5] * sig * oracle.grad6(ops, int(axial), -alpha, v))
+    assert abs(lhs - rhs) < 1e-10 * abs(rhs)
+    ops1 = ops.copy()
+    ops1[5] = 1.0
+    lhs = np.sum(oracle.quad6(ops, int(axial), alpha, sig, integrate=False)
+                 * v)
+    rhs = np.sum(sig * oracle.grad6(ops1, int(axial), -alpha, v))
+    assert abs(lhs - rhs) < 1e-10 * abs(rhs)
+
+
+def test_isotropic_hooke_matches_voigt_matrix():
+    rng = np.random.default_rng(1)
+    lam = rng.uniform(1, 2, 25)
+    mu = rng.uniform(1, 2, 25)
+    eps = rng.standard_normal((6, 25)) + 1j * rng.standard_normal((6, 25))
+    got = oracle.iso_stress(lam, mu, eps)
+    for p in range(25):
+        C = np.zeros((6, 6))
+        C[:3, :3] = lam[p]
+        C[[0, 1, 2], [0, 1, 2]] += 2 * mu[p]
+        C[[3, 4, 5], [3, 4, 5]] = mu[p]      # engineering shears
+        assert np.allclose(got[:, p], C @ eps[:, p], rtol=1e-14, atol=0)
+
+
+# --------------------------------------------------------------- FFT
+@pytest.mark.parametrize("nr", list(range(1, 17)) + [21, 26, 39, 45, 77, 91])
+def test_fft_conventions_against_dft(nr):
+    """R2C = (1/N) DFT with the Nyquist row halved; C2R doubles it and
+    returns the samples: checked against a brute-force DFT."""
+    rng = np.random.default_rng(nr)
+    x = rng.standard_normal(nr)
+    n = np.arange(nr)
+    k = np.arange(nr // 2 + 1)
+    dft = np.exp(-2j * np.pi * np.outer(k, n) / nr) @ x / nr
+    if nr % 2 == 0:
+        dft[-1] *= 0.5
+    c = oracle.r2c(x)
+    assert np.abs(c - dft).max() < 1e-13
+    assert np.abs(oracle.c2r(c, nr) - x).max() < 1e-12
+    # C2R ignores Im of the mean and of the Nyquist row
+    c2 = c.copy()
+    c2[0] += 0.7j
+    if nr % 2 == 0:
+        c2[-1] += 0.3j
+    assert np.abs(oracle.c2r(c2, nr) - x).max() < 1e-12
+
+
+# --------------------------------------------------------------- Newmark
+def test_newmark_harmonic_oscillator():
+    """Newmark with gamma = 1/2, beta = 0 on u'' = -w^2 u is the central
+    difference scheme: u_n = cos(n theta), cos(theta) = 1 - (w dt)^2 / 2."""
+    w, dt, n = 2.0, 0.05, 400
+    u = np.array([[1.0 + 0j, 0, 0]])
+    a = -w * w * u
+    v = -0.5 * dt * (a + a)        # so that the velocity at t = 0 is zero
+    out = []
+    for _ in range(n):
+        out.append(u[0, 0].real)
+        s = -w * w * u
+        s, u, v, a = _newmark(s, u, v, a, dt)
+    theta = math.acos(1 - (w * dt) ** 2 / 2)
+    ref = np.cos(theta * np.arange(n))
+    assert np.abs(np.array(out) - ref).max() < 1e-12
+
+
+def _newmark(s, u, v, a, dt):
+    spec = preloop.cartesian_homogeneous(nx=1, nz=1, width=1e3, depth=1e3,
+                                         nu=0, nsteps=1, n_receivers=1)
+    o = oracle.Oracle(spec)
+    o.d.dt = dt
+    o.nmodes = 1
+    return o.newmark(s, u, v, a)
+
+
+# --------------------------------------------------------------- receivers
+def test_receiver_synthesis_inverts_r2c():
+    """A receiver sitting on one point recovers f(phi_k) from the
+    Fourier coefficients r2c(f): pins c_0 = 1, c_alpha = 2 and the Nyquist
+    convention against the FFT."""
+    for nu in (2, 3):
+        spec = preloop.cartesian_homogeneous(nx=2, nz=2, width=2e3,
+                                             depth=2e3, nu=nu, nsteps=1,
+                                             n_receivers=1)
+        nr = int(spec.solid_nr[0])
+        e = 3
+        pts = spec.elem_points[e]
+        w = np.zeros(25)
+        w[12] = 1.0
+        spec.rec_elem = np.array([e], np.int32)
+        spec.rec_weights = w[None, :]
+        rng = np.random.default_rng(nu)
+        phis = 2 * np.pi * np.arange(nr) / nr
+        o = oracle.Oracle(spec)
+        f = rng.standard_normal((nr, 3))
+        coef = np.array([oracle.r2c(f[:, c]) for c in range(3)]).T
+        for k, phi in enumerate(phis):
+            spec.rec_phi = np.array([phi])
+            o2 = oracle.Oracle(spec)
+            displ = np.zeros((o2.nmodes, 3), complex)
+            off = o2.offsets[pts[12]]
+            displ[off:off + nr // 2 + 1] = coef
+            got = o2.record(displ)[0]
+            assert np.abs(got - f[k]).max() < 1e-12
