@@ -352,11 +352,45 @@ def _compact_points(elem_points_grid, npoint_grid):
     return remap, used
 
 
+def inplane_nr_bound(mesh: StructuredMesh):
+    """Upper bound of the nodal Nr by the in-plane resolution, per element
+    corner (E, 4): round(2 pi s / h) with h the mean edge length of the
+    quads sharing the node divided by npol, and 5 on the axis
+    (ExodusMesh::formNrAtNodes with bound_Nr_by_inplane,
+    src/preloop/exodus/ExodusMesh.cpp:663-713)."""
+    nsz = mesh.nodal_sz()                                   # (E, 2, 4)
+    edges = np.zeros(mesh.nelem)
+    for a, b in ((0, 1), (1, 2), (2, 3), (3, 0)):
+        edges += np.hypot(nsz[:, 0, a] - nsz[:, 0, b],
+                          nsz[:, 1, a] - nsz[:, 1, b])
+    ix, iz = mesh.element_ix_iz()
+    na, nb = mesh.nx + 1, mesh.nz + 1
+    corner = [(0, 0), (1, 0), (1, 1), (0, 1)]
+    tot = np.zeros((nb, na))
+    cnt = np.zeros((nb, na))
+    for c, (da, db) in enumerate(corner):
+        np.add.at(tot, (iz + db, ix + da), edges)
+        np.add.at(cnt, (iz + db, ix + da), 1.0)
+    spacing = tot / (cnt * 4 * spectral.NPOL)
+    out = np.empty((mesh.nelem, 4))
+    for c, (da, db) in enumerate(corner):
+        h = spacing[iz + db, ix + da]
+        s = nsz[:, 0, c]
+        up = np.maximum(round_half_away(2.0 * math.pi * s / h), 1.0)
+        tol = 1e-6 * max(abs(mesh.node_a[-1] - mesh.node_a[0]),
+                         abs(mesh.node_b[-1] - mesh.node_b[0]))
+        if mesh.spherical:
+            tol = 1e-6 * float(np.max(mesh.node_b))
+        out[:, c] = np.where(s < tol, 5.0, up)
+    return out
+
+
 def build_domain(mesh: StructuredMesh,
                  rho_vp_vs: Callable,
                  nodal_nr: Callable,
                  courant: float = 0.6,
-                 lucky: bool = True):
+                 lucky: bool = True,
+                 bound_nr: bool = True):
     """Points, solid elements, 1D isotropic material, mass, dt.
 
     rho_vp_vs(r) -> (rho, vp, vs) arrays evaluated at the element corners'
@@ -390,8 +424,11 @@ def build_domain(mesh: StructuredMesh,
     sz_pts = np.zeros((U, 2))
     sz_pts[elem_points.reshape(-1)] = gq["sz"].transpose(0, 2, 1).reshape(-1, 2)
 
-    # Nr: nodal -> GLL (round half away) -> lucky -> element max
+    # Nr: nodal (bounded by the in-plane resolution) -> GLL (round half
+    # away) -> lucky -> element max
     nr_nodal = nodal_nr(nsz[:, 0], nsz[:, 1])
+    if bound_nr:
+        nr_nodal = np.minimum(nr_nodal, inplane_nr_bound(mesh))
     nr_gll = round_half_away(interpolate_gll(nr_nodal, axial)).astype(np.int64)
     if lucky:
         uniq = np.unique(nr_gll)
@@ -737,14 +774,19 @@ def spherical_regional(ntheta=556, nrad=216, theta_max_deg=30.0,
                        r_outer=6371e3, depth=1200e3, nu_min=10, nu_max=400,
                        seed=11, attenuation=True, cg4=True, nsls=5,
                        nsteps=10000, n_receivers=100, src_depth=50e3,
-                       f_peak=0.2, field=None, corr_len=50e3):
+                       f_peak=0.2, field=None, corr_len=50e3, courant=0.5,
+                       bound_nr=True):
     """Spherical sector (theta in [0, theta_max] from the axis, r in
-    [r_outer - depth, r_outer]) with a seeded 6-layer 1D model, per-point
-    Fourier order nu from nu_min (on the axis) to nu_max (at the sector
-    edge), a +-3% 3D random field on vp and vs with correlation length
+    [r_outer - depth, r_outer]) with a seeded 6-layer 1D model, per-node
+    Fourier order nu from nu_min (next to the axis, bounded there by the
+    in-plane resolution) to nu_max (at the sector edge), a +-3% 3D random field on vp and vs with correlation length
     corr_len, and 5-SLS attenuation with Q_mu ~ U[80, 600] per layer
     (Q_kappa = 57823).  Axial moment-tensor source with a Ricker STF and
-    surface receivers."""
+    surface receivers.  Courant number 0.5: the GLJ points next to the axis
+    carry orders alpha whose azimuthal stiffness (~ vp alpha / s) adds to
+    the in-plane one, and at the template's 0.6 the axial elements exceed
+    the Newmark stability limit for alpha ~ 4 (measured by power iteration
+    of M^-1 K_alpha), as the reference solver would with the same input."""
     th = np.linspace(0.0, math.radians(theta_max_deg), ntheta + 1)
     rr = np.linspace(r_outer - depth, r_outer, nrad + 1)
     mesh = StructuredMesh(th, rr, spherical=True)
@@ -753,10 +795,13 @@ def spherical_regional(ntheta=556, nrad=216, theta_max_deg=30.0,
     s_max = r_outer * math.sin(math.radians(theta_max_deg))
 
     def nodal_nr(s, z):
+        # nu grows with the distance from the axis; the in-plane bound of
+        # build_domain caps it near the axis (Nr = 5 on it)
         nu = nu_min + (nu_max - nu_min) * np.clip(s / s_max, 0.0, 1.0)
         return 2.0 * nu + 1.0
 
-    spec = build_domain(mesh, prof, nodal_nr)
+    spec = build_domain(mesh, prof, nodal_nr, courant=courant,
+                        bound_nr=bound_nr)
     if field is None:
         field = RandomField(seed + 1, corr_len=corr_len)
     att = None
@@ -793,3 +838,37 @@ def finish_regional(spec, nsteps, src_depth=50e3, f_peak=0.2,
     add_receivers(spec, r_outer * np.sin(th), r_outer * np.cos(th), phi)
     spec._mt = mt
     return spec
+
+
+def subdomain(spec, elems, prop3d=None, d3d=None):
+    """A DomainSpec holding only the given solid elements (and their
+    points, renumbered), for timing a bounded sample of a large workload.
+    3D operands of the sampled elements may be passed in element order
+    ([sum Nr][2][25] and [sum Nr][2][npa])."""
+    elems = np.asarray(elems, dtype=np.int64)
+    ep = spec.elem_points[elems]
+    uniq, inv = np.unique(ep.reshape(-1), return_inverse=True)
+    sub = DomainSpec(
+        solid_nr=spec.solid_nr[uniq].copy(),
+        solid_mass=spec.solid_mass[uniq].copy(),
+        solid_sz=spec.solid_sz[uniq].copy(),
+        elem_points=inv.reshape(ep.shape).astype(np.int32),
+        elem_axial=spec.elem_axial[elems].copy(),
+        elem_gq=spec.elem_gq[elems].copy(),
+        elem_nr=spec.elem_nr[elems].copy(),
+        elem_iso=spec.elem_iso[elems].copy(),
+        axial_points=np.nonzero(np.isin(uniq, spec.axial_points))[0].astype(
+            np.int32),
+        dt=spec.dt, name=spec.name + "_sample")
+    if getattr(spec, "elem_kind", None) is not None:
+        sub.elem_kind = spec.elem_kind[elems].copy()
+        nr = sub.elem_nr.astype(np.int64)
+        off = np.zeros(len(elems) + 1, dtype=np.int64)
+        off[1:] = np.cumsum(nr)
+        sub.elem_slice_off = off[:-1]
+        sub.prop3d = prop3d
+        att = getattr(spec, "att", None)
+        if att is not None:
+            sub.att = dict(att)
+            sub.att["d3d"] = d3d
+    return sub

@@ -248,46 +248,69 @@ static void fft_rec(const cplx *in, cplx *out, int n, int is,
     int p = smallest_prime_factor(n), m = n / p, L = nt / n;
     for (int r = 0; r < p; r++)
         fft_rec(in + (int64_t)r * is, out + (int64_t)r * m, m, is * p, tw, nt);
-    cplx tloc[64];
+    cplx tloc[64], wloc[64];
     cplx *t = p <= 64 ? tloc : (cplx *)malloc(sizeof(cplx) * p);
+    cplx *wp = p <= 64 ? wloc : (cplx *)malloc(sizeof(cplx) * p);
+    for (int x = 0; x < p; x++) wp[x] = tw[(int64_t)x * m * L];
     for (int k = 0; k < m; k++) {
+        /* r k L < p m L = nt: no reduction needed */
         for (int r = 0; r < p; r++)
-            t[r] = out[r * m + k] * tw[((int64_t)r * k * L) % nt];
+            t[r] = out[r * m + k] * tw[(int64_t)r * k * L];
         for (int q = 0; q < p; q++) {
-            cplx acc = 0;
-            for (int r = 0; r < p; r++)
-                acc += t[r] * tw[((int64_t)((r * q) % p) * m * L) % nt];
+            cplx acc = t[0];
+            int x = 0;
+            for (int r = 1; r < p; r++) {
+                x += q;
+                if (x >= p) x -= p;
+                acc += t[r] * wp[x];
+            }
             out[q * m + k] = acc;
         }
     }
+    if (wp != wloc) free(wp);
     if (t != tloc) free(t);
 }
 
-static void cfft(const cplx *in, cplx *out, int n, int sign) {
+/* Twiddle tables exp(sign 2 pi i j / n), cached per n (n < 4096). */
+static cplx *tw_cache[2][4096];
+
+static const cplx *twiddles(int n, int sign) {
+    int sl = sign > 0;
+    if (n < 4096 && tw_cache[sl][n]) return tw_cache[sl][n];
     cplx *tw = (cplx *)malloc(sizeof(cplx) * n);
     for (int j = 0; j < n; j++)
         tw[j] = cexp(sign * 2. * M_PI * I * (double)j / (double)n);
+    if (n < 4096) tw_cache[sl][n] = tw;
+    return tw;
+}
+
+/* out = sum_j in[j] exp(sign 2 pi i jk/n); sign = +1 uses the conjugate
+ * table (inverse transform). */
+static void cfft(const cplx *in, cplx *out, int n, int sign) {
+    const cplx *tw = twiddles(n, sign);
     fft_rec(in, out, n, 1, tw, n);
-    free(tw);
+    if (n >= 4096) free((void *)tw);
 }
 
 /* computeR2C: X_k = (1/N) sum_n x_n e^{-2 pi i k n / N}, k <= N/2, Nyquist
  * (N even) halved (SolverFFTW.hpp:101-116). */
 void oracle_r2c(const double *r, int nr, cplx *c) {
-    cplx *a = (cplx *)malloc(sizeof(cplx) * nr * 2);
+    cplx loc[4096];
+    cplx *a = nr <= 2048 ? loc : (cplx *)malloc(sizeof(cplx) * nr * 2);
     for (int k = 0; k < nr; k++) a[k] = r[k] * (1. / nr);
     cfft(a, a + nr, nr, -1);
     int nc = nr / 2 + 1;
     for (int k = 0; k < nc; k++) c[k] = a[nr + k];
     if (nr % 2 == 0) c[nc - 1] *= .5;
-    free(a);
+    if (a != loc) free(a);
 }
 
 /* computeC2R: Nyquist (N even) doubled, then FFTW's unnormalised c2r on the
  * Hermitian extension, which takes only the real parts of X_0 and X_{N/2}
  * (SolverFFTW.hpp:138-153). */
 void oracle_c2r(const cplx *c, int nr, double *r) {
-    cplx *a = (cplx *)malloc(sizeof(cplx) * nr * 2);
+    cplx loc[4096];
+    cplx *a = nr <= 2048 ? loc : (cplx *)malloc(sizeof(cplx) * nr * 2);
     int nc = nr / 2 + 1;
     for (int k = 0; k < nr; k++) a[k] = 0;
     a[0] = creal(c[0]);
@@ -301,7 +324,7 @@ void oracle_c2r(const cplx *c, int nr, double *r) {
     }
     cfft(a, a + nr, nr, +1);
     for (int k = 0; k < nr; k++) r[k] = creal(a[nr + k]);
-    free(a);
+    if (a != loc) free(a);
 }
 
 /* ------------------------------------------------------------------ */

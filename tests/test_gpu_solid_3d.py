@@ -69,3 +69,54 @@ def test_cardinal_seismograms_match_oracle(prec, tol):
     err = _rel(got, ref)
     print(f"3D+attenuation {prec}: relative seismogram L2 = {err:.3e}")
     assert err < tol
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-11), ("fp32", 2e-5)])
+def test_cardinal_large_nr_matches_oracle(prec, tol):
+    """Elements with Nr from ~600 to ~810 (FFT radix chains of 2, 3, 4, 5,
+    7, 11, 13), 3D model and CG4 attenuation: stage-level stiffness."""
+    spec = preloop.spherical_regional(
+        ntheta=3, nrad=2, theta_max_deg=30.0, depth=12e3, nu_min=300,
+        nu_max=400, nsteps=10, n_receivers=2, corr_len=50e3, bound_nr=False)
+    spec._build_3d()
+    preloop.finish_regional(spec, 10, src_depth=5e3, n_receivers=2)
+    assert spec.elem_nr.max() > 600
+    o = oracle.Oracle(spec)
+    g = _solver(spec, prec)
+    rng = np.random.default_rng(9)
+    u = rng.standard_normal((o.nmodes, 3)) + 1j * rng.standard_normal(
+        (o.nmodes, 3))
+    g.set_field("displ", u)
+    g.set_field("stiff", np.zeros_like(u))
+    g.computeStiffness()
+    err = _rel(g.get_field("stiff"), o.stiffness(u))
+    print(f"large Nr {sorted(set(spec.elem_nr.tolist()))} {prec}: {err:.2e}")
+    assert err < tol
+
+
+def test_regional_reduced_fp32_vs_fp64_stable():
+    """The regional workload at reduced width (nu 10..400 over the full
+    30 degrees, 60 x 24 elements): 400 steps are stable and the FP32
+    stiffness path stays within the north-star bar of the FP64 path at
+    the near-source monitors and the surface receivers."""
+    nsteps = 400
+    spec = preloop.spherical_regional(ntheta=60, nrad=24, nsteps=nsteps,
+                                      n_receivers=10, f_peak=0.05,
+                                      courant=0.4)
+    spec._build_3d()
+    preloop.finish_regional(spec, nsteps, src_depth=50e3, f_peak=0.05,
+                            n_receivers=10)
+    s64 = _run(spec, "fp64", nsteps)
+    s32 = _run(spec, "fp32", nsteps)
+    assert np.isfinite(s64).all() and np.abs(s64).max() > 0
+    err = _rel(s32, s64)
+    print(f"regional 60x24: FP32 vs FP64 relative L2 = {err:.3e}")
+    assert err < 1e-4
+
+
+def _run(spec, prec, nsteps):
+    g = _solver(spec, prec)
+    g.run(nsteps)
+    out = g.seismograms()
+    g.close()
+    return out
