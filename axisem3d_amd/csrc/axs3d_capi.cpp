@@ -223,6 +223,14 @@ int axs3d_profile_read(axs3d_ctx *ctx, int kernel, double *total_ms,
     });
 }
 
+int axs3d_autotune_info(const axs3d_ctx *ctx, int *cardinal_occupancy,
+                        int *cardinal_blocks, double *trial_ms3) {
+    if (!ctx || !cardinal_occupancy || !cardinal_blocks || !trial_ms3)
+        return 2;
+    ctx->dom->autotuneInfo(cardinal_occupancy, cardinal_blocks, trial_ms3);
+    return 0;
+}
+
 int axs3d_profile_reset(axs3d_ctx *ctx) {
     return guarded(ctx, [&](axs3d::DomainBase &d) { d.profileReset(); });
 }

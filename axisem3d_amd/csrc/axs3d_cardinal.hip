@@ -45,17 +45,21 @@ __device__ __forceinline__ int next_radix(int m) {
     return m;
 }
 
-__device__ __forceinline__ void sincos_pi(float x, float *s, float *c) {
-    sincospif(x, s, c);
-}
-__device__ __forceinline__ void sincos_pi(double x, double *s, double *c) {
-    sincospi(x, s, c);
-}
-
 // One Stockham stage of radix R over `nbatch` contiguous series of length
 // n: out[(j/Ns)*Ns*R + j%Ns + q*Ns] = sum_r W_R^{rq} W_n^{r (j%Ns) n/(Ns R)}
 // in[j + r n/R] for j < n/R (Govindaraju et al., SC 2008).  sign = -1
-// forward, +1 inverse; tw[t] = exp(-2 pi i t / n).
+// forward, +1 inverse; tw[t] = exp(-2 pi i t / n).  Radices 2, 3, 4, 5
+// use closed-form butterflies; 7, 11 and 13 form each output from the
+// table (W_R^x = W_n^{x n/R}) and store it at once, keeping register use
+// at 2R + O(1) values.
+template <typename Real>
+__device__ __forceinline__ cpx<Real> twid(const cpx<Real> *__restrict__ tw,
+                                          int t, int sign) {
+    cpx<Real> w = tw[t];
+    if (sign > 0) w.im = -w.im;
+    return w;
+}
+
 template <typename Real, int R>
 __device__ __forceinline__ void fft_stage(const cpx<Real> *__restrict__ in,
                                           cpx<Real> *__restrict__ out, int n,
@@ -63,61 +67,83 @@ __device__ __forceinline__ void fft_stage(const cpx<Real> *__restrict__ in,
                                           const cpx<Real> *__restrict__ tw) {
     const int nb = n / R;
     const int step = n / (Ns * R);
-    Real wc[R], ws[R];
-#pragma unroll
-    for (int x = 0; x < R; x++) {
-        Real sv, cv;
-        sincos_pi((Real)(2 * x) / (Real)R, &sv, &cv);
-        wc[x] = cv;
-        ws[x] = sign < 0 ? -sv : sv;
-    }
     const int total = nb * nbatch;
+    const Real sg = (Real)sign;
     for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
         const int h = idx / nb;
         const int j = idx - h * nb;
         const cpx<Real> *a = in + (size_t)h * n;
         cpx<Real> *b = out + (size_t)h * n;
         const int k = j % Ns;
-        cpx<Real> v[R];
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-            cpx<Real> x = a[j + r * nb];
-            if (r > 0 && k > 0) {
-                cpx<Real> w = tw[r * k * step];
-                if (sign > 0) w.im = -w.im;
-                x = cmulc(x, w);
+        const int idxD = (j / Ns) * Ns * R + k;
+        if (R >= 7) {
+            // y_q = sum_r x_r W_n^{r (k step + q n/R)}: stage twiddle and
+            // DFT kernel from one table index, reduced incrementally
+            // (k step + q n/R < n, so one subtraction keeps t < n)
+            const int wstep = n / R;
+#pragma unroll 1
+            for (int q = 0; q < R; q++) {
+                const int base = k * step + q * wstep;
+                cpx<Real> acc = a[j];
+                int t = 0;
+#pragma unroll 1
+                for (int r = 1; r < R; r++) {
+                    t += base;
+                    if (t >= n) t -= n;
+                    acc = cadd(acc, cmulc(a[j + r * nb], twid(tw, t, sign)));
+                }
+                b[idxD + q * Ns] = acc;
             }
+            continue;
+        }
+        cpx<Real> v[R < 7 ? R : 1];
+#pragma unroll
+        for (int r = 0; r < (R < 7 ? R : 1); r++) {
+            cpx<Real> x = a[j + r * nb];
+            if (r > 0 && k > 0) x = cmulc(x, twid(tw, r * k * step, sign));
             v[r] = x;
         }
-        cpx<Real> y[R];
         if (R == 2) {
-            y[0] = cadd(v[0], v[1]);
-            y[1] = csub(v[0], v[1]);
+            b[idxD] = cadd(v[0], v[1]);
+            b[idxD + Ns] = csub(v[0], v[1]);
         } else if (R == 4) {
             const cpx<Real> s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
             const cpx<Real> s13 = cadd(v[1], v[3]), d13 = csub(v[1], v[3]);
-            // W_4 = -i (forward) or +i (inverse)
-            const cpx<Real> rd13 = sign < 0 ? cpx<Real>{d13.im, -d13.re}
-                                            : cpx<Real>{-d13.im, d13.re};
-            y[0] = cadd(s02, s13);
-            y[2] = csub(s02, s13);
-            y[1] = cadd(d02, rd13);
-            y[3] = csub(d02, rd13);
-        } else {
-#pragma unroll
-            for (int q = 0; q < R; q++) {
-                cpx<Real> acc = v[0];
-#pragma unroll
-                for (int r = 1; r < R; r++) {
-                    const int x = (r * q) % R;
-                    acc = cadd(acc, cmulc(v[r], cpx<Real>{wc[x], ws[x]}));
-                }
-                y[q] = acc;
-            }
+            // (-i) d13 forward, (+i) d13 inverse
+            const cpx<Real> rd13 = {-sg * d13.im, sg * d13.re};
+            b[idxD] = cadd(s02, s13);
+            b[idxD + 2 * Ns] = csub(s02, s13);
+            b[idxD + Ns] = cadd(d02, rd13);
+            b[idxD + 3 * Ns] = csub(d02, rd13);
+        } else if (R == 3) {
+            const Real c1 = (Real)-0.5;
+            const Real s1 = sg * (Real)0.86602540378443864676;
+            const cpx<Real> t = cadd(v[1], v[2]);
+            const cpx<Real> d = csub(v[1], v[2]);
+            const cpx<Real> m = cfma(t, c1, v[0]);
+            const cpx<Real> id = {-s1 * d.im, s1 * d.re};
+            b[idxD] = cadd(v[0], t);
+            b[idxD + Ns] = cadd(m, id);
+            b[idxD + 2 * Ns] = csub(m, id);
+        } else if (R == 5) {
+            const Real c1 = (Real)0.30901699437494742410;   // cos 2pi/5
+            const Real c2 = (Real)-0.80901699437494742410;  // cos 4pi/5
+            const Real s1 = sg * (Real)0.95105651629515357212;
+            const Real s2 = sg * (Real)0.58778525229247312917;
+            const cpx<Real> t1 = cadd(v[1], v[4]), d1 = csub(v[1], v[4]);
+            const cpx<Real> t2 = cadd(v[2], v[3]), d2 = csub(v[2], v[3]);
+            const cpx<Real> m1 = cadd(v[0], cadd(cmul(t1, c1), cmul(t2, c2)));
+            const cpx<Real> m2 = cadd(v[0], cadd(cmul(t1, c2), cmul(t2, c1)));
+            // i (s1 d1 + s2 d2) and i (s2 d1 - s1 d2)
+            const cpx<Real> e1 = cadd(cmul(d1, s1), cmul(d2, s2));
+            const cpx<Real> e2 = csub(cmul(d1, s2), cmul(d2, s1));
+            const cpx<Real> ie1 = {-e1.im, e1.re}, ie2 = {-e2.im, e2.re};
+            b[idxD] = cadd(v[0], cadd(t1, t2));
+            b[idxD + Ns] = cadd(m1, ie1);
+            b[idxD + 4 * Ns] = csub(m1, ie1);
+            b[idxD + 2 * Ns] = cadd(m2, ie2);
+            b[idxD + 3 * Ns] = csub(m2, ie2);
         }
-        const int idxD = (j / Ns) * Ns * R + k;
-#pragma unroll
-        for (int q = 0; q < R; q++) b[idxD + q * Ns] = y[q];
     }
 }
 
@@ -163,8 +189,10 @@ struct CardLds {
     cpx<Real> sY[EMB][3][NPEM];
 };
 
-template <typename Real, bool ATT>
-__global__ __launch_bounds__(STIFF_BLOCK) void k_solid_stiff_cd(
+// OCC = minimum waves per SIMD the register allocation must allow (2, 3
+// or 4 blocks of 256 threads per CU); picked per workload by measurement.
+template <typename Real, bool ATT, int OCC>
+__global__ __launch_bounds__(STIFF_BLOCK, OCC) void k_solid_stiff_cd(
     CardArgs<Real> a) {
     constexpr size_t fft_bytes = 2 * (size_t)CARD_FFT_CAP * sizeof(cpx<Real>);
     constexpr size_t ac_bytes = sizeof(CardLds<Real>);
@@ -200,19 +228,20 @@ __global__ __launch_bounds__(STIFF_BLOCK) void k_solid_stiff_cd(
         const bool axial = s.elem_axial[e] != 0;
         const bool axial_row0 = axial && (i == 0);
 
-        PointG<Real> G;
-        load_point_g<Real>(sG, axial, i, j, G);
-        PointOps<Real> g{}, q{};
-        int p = 0, nu1p = 0;
+        int nu1p = 0;
         uint32_t off = 0;
         if (lane_ok) {
-            load_point_ops(s.elem_ops, e, ipnt, g, q);
-            p = s.elem_points[e * NPEM + ipnt];
+            const int p = s.elem_points[e * NPEM + ipnt];
             nu1p = s.point_nu1[p];
             off = s.point_off[p];
         }
 
         // ---------------- A. gradient -> strain (Fourier) in scratch
+        {
+        PointG<Real> G;
+        load_point_g<Real>(sG, axial, i, j, G);
+        PointOps<Real> g{}, q{};
+        if (lane_ok) load_point_ops(s.elem_ops, e, ipnt, g, q);
         for (int a0 = 0; a0 < nu1; a0 += EMB) {
             const int alpha = a0 + eml;
             const bool active = lane_ok && alpha < nu1;
@@ -237,6 +266,7 @@ __global__ __launch_bounds__(STIFF_BLOCK) void k_solid_stiff_cd(
                     S[(size_t)(d * NPEM + ipnt) * nu1max + alpha] = eps[d];
             }
             __syncthreads();
+        }
         }
 
         // ---------------- B. FFT -> stress (+ attenuation) -> FFT
@@ -371,6 +401,11 @@ __global__ __launch_bounds__(STIFF_BLOCK) void k_solid_stiff_cd(
         }
 
         // ---------------- C. quadrature -> element-local forces
+        {
+        PointG<Real> G;
+        load_point_g<Real>(sG, axial, i, j, G);
+        PointOps<Real> g{}, q{};
+        if (lane_ok) load_point_ops(s.elem_ops, e, ipnt, g, q);
         for (int a0 = 0; a0 < nu1; a0 += EMB) {
             const int alpha = a0 + eml;
             const bool active = lane_ok && alpha < nu1;
@@ -395,6 +430,7 @@ __global__ __launch_bounds__(STIFF_BLOCK) void k_solid_stiff_cd(
             }
             __syncthreads();
         }
+        }
     }
 }
 
@@ -402,37 +438,62 @@ __global__ void k_reset_counter(int *c) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0;
 }
 
+template <typename Real, bool ATT>
+static void launch_cd_att(hipStream_t st, const CardArgs<Real> &a, int n_slots,
+                          int occ) {
+    switch (occ) {
+        case 4:
+            hipLaunchKernelGGL((k_solid_stiff_cd<Real, ATT, 4>),
+                               dim3(n_slots), dim3(STIFF_BLOCK), 0, st, a);
+            break;
+        case 3:
+            hipLaunchKernelGGL((k_solid_stiff_cd<Real, ATT, 3>),
+                               dim3(n_slots), dim3(STIFF_BLOCK), 0, st, a);
+            break;
+        default:
+            hipLaunchKernelGGL((k_solid_stiff_cd<Real, ATT, 2>),
+                               dim3(n_slots), dim3(STIFF_BLOCK), 0, st, a);
+            break;
+    }
+}
+
 template <typename Real>
 void launch_solid_stiff_cd(hipStream_t st, const CardArgs<Real> &a,
-                           int n_slots) {
+                           int n_slots, int occ) {
     if (a.n3d <= 0) return;
     hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st,
                        a.work_counter);
     if (a.nsls > 0)
-        hipLaunchKernelGGL((k_solid_stiff_cd<Real, true>), dim3(n_slots),
-                           dim3(STIFF_BLOCK), 0, st, a);
+        launch_cd_att<Real, true>(st, a, n_slots, occ);
     else
-        hipLaunchKernelGGL((k_solid_stiff_cd<Real, false>), dim3(n_slots),
-                           dim3(STIFF_BLOCK), 0, st, a);
+        launch_cd_att<Real, false>(st, a, n_slots, occ);
 }
 
-template <typename Real>
-int card_blocks_per_cu(bool att) {
+template <typename Real, bool ATT, int OCC>
+static int blocks_per_cu_v() {
     int n = 0;
-    if (att)
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &n, k_solid_stiff_cd<Real, true>, STIFF_BLOCK, 0);
-    else
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &n, k_solid_stiff_cd<Real, false>, STIFF_BLOCK, 0);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &n, k_solid_stiff_cd<Real, ATT, OCC>, STIFF_BLOCK, 0);
     return n > 0 ? n : 1;
 }
 
-template int card_blocks_per_cu<float>(bool);
-template int card_blocks_per_cu<double>(bool);
+template <typename Real>
+int card_blocks_per_cu(bool att, int occ) {
+    if (att)
+        return occ == 4 ? blocks_per_cu_v<Real, true, 4>()
+             : occ == 3 ? blocks_per_cu_v<Real, true, 3>()
+                        : blocks_per_cu_v<Real, true, 2>();
+    return occ == 4 ? blocks_per_cu_v<Real, false, 4>()
+         : occ == 3 ? blocks_per_cu_v<Real, false, 3>()
+                    : blocks_per_cu_v<Real, false, 2>();
+}
+
+template int card_blocks_per_cu<float>(bool, int);
+template int card_blocks_per_cu<double>(bool, int);
 template void launch_solid_stiff_cd<float>(hipStream_t,
-                                           const CardArgs<float> &, int);
+                                           const CardArgs<float> &, int, int);
 template void launch_solid_stiff_cd<double>(hipStream_t,
-                                            const CardArgs<double> &, int);
+                                            const CardArgs<double> &, int,
+                                            int);
 
 }  // namespace axs3d

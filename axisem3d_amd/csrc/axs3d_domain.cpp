@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <sstream>
 
@@ -408,6 +409,7 @@ void DomainBase::finalize() {
     uploadReal();
     hip_check(hipStreamSynchronize(st), "Domain::finalize");
     mFinalized = true;
+    autotune();
 }
 
 void DomainBase::recordOutput(int isample) {
@@ -614,7 +616,8 @@ public:
         }
         if (mNum3D > 0) {
             tick(AXS3D_KERNEL_STIFFNESS_3D);
-            launch_solid_stiff_cd<Real>(mStream, cardArgs(), mCardSlots);
+            launch_solid_stiff_cd<Real>(mStream, cardArgs(), mCardSlots,
+                                        mCardOcc);
             tock(AXS3D_KERNEL_STIFFNESS_3D);
         }
     }
@@ -705,6 +708,41 @@ private:
         }
     }
 
+    // Times the cardinal kernel's occupancy variants on this workload and
+    // keeps the fastest.  Runs right after finalize, on zero fields: the
+    // element forces come out zero and the attenuation state stays zero,
+    // so the trials leave no trace.  AXS3D_AUTOTUNE=0 keeps variant 2.
+    void autotune() override {
+        if (mNum3D == 0) return;
+        const char *env = std::getenv("AXS3D_AUTOTUNE");
+        if (env && env[0] == '0') return;
+        hipEvent_t a, b;
+        hip_check(hipEventCreate(&a), "Domain::autotune");
+        hip_check(hipEventCreate(&b), "Domain::autotune");
+        double best = 1e300;
+        int bestOcc = 2;
+        for (int occ = 2; occ <= 4; occ++) {
+            mCardOcc = occ;
+            mCardSlots = mCardSlotsFor[occ];
+            launch_solid_stiff_cd<Real>(mStream, cardArgs(), mCardSlots, occ);
+            hip_check(hipEventRecord(a, mStream), "Domain::autotune");
+            launch_solid_stiff_cd<Real>(mStream, cardArgs(), mCardSlots, occ);
+            hip_check(hipEventRecord(b, mStream), "Domain::autotune");
+            hip_check(hipEventSynchronize(b), "Domain::autotune");
+            float ms = 0.f;
+            hip_check(hipEventElapsedTime(&ms, a, b), "Domain::autotune");
+            mCardTuneMs[occ] = ms;
+            if (ms < best) {
+                best = ms;
+                bestOcc = occ;
+            }
+        }
+        mCardOcc = bestOcc;
+        mCardSlots = mCardSlotsFor[bestOcc];
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+    }
+
     // 3D properties, [slice][2][25] lambda, mu, cast to Real by chunks.
     void uploadProp3D(int64_t s0, int64_t ns, const double *src) override {
         if (dProp3D.bytes() == 0)
@@ -773,9 +811,16 @@ private:
             hip_check(hipDeviceGetAttribute(
                           &ncu, hipDeviceAttributeMultiprocessorCount,
                           mDevice), "Domain::finalize");
-            const int per_cu = card_blocks_per_cu<Real>(mNsls > 0);
-            mCardSlots = std::max(1, std::min(mNum3D, per_cu * ncu));
-            dCardScratch.alloc((size_t)mCardSlots * 6 * NPEM * mNu1Max3D *
+            int maxSlots = 1;
+            for (int occ = 2; occ <= 4; occ++) {
+                const int per_cu = card_blocks_per_cu<Real>(mNsls > 0, occ);
+                mCardSlotsFor[occ] = std::max(1, std::min(mNum3D,
+                                                          per_cu * ncu));
+                maxSlots = std::max(maxSlots, mCardSlotsFor[occ]);
+            }
+            mCardOcc = 2;
+            mCardSlots = mCardSlotsFor[2];
+            dCardScratch.alloc((size_t)maxSlots * 6 * NPEM * mNu1Max3D *
                                2 * sizeof(Real));
             if (mNsls > 0) {
                 const int npa = mAttCG4 ? 4 : NPEM;

@@ -101,6 +101,11 @@ public:
     void profileReset();
 
     hipStream_t stream() const { return mStream; }
+    void autotuneInfo(int *occ, int *slots, double *ms) const {
+        *occ = mCardOcc;
+        *slots = mCardSlots;
+        for (int k = 0; k < 3; k++) ms[k] = mCardTuneMs[k + 2];
+    }
     int precision() const { return mPrecision; }
 
 protected:
@@ -142,6 +147,9 @@ protected:
     std::vector<int64_t> hSliceOff;     // first slice of each 3D element
     int64_t mNumSlices = 0;
     int mNum3D = 0, mNu1Max3D = 0, mCardSlots = 0;
+    int mCardOcc = 2;                   // chosen occupancy variant
+    int mCardSlotsFor[5] = {0, 0, 0, 0, 0};
+    double mCardTuneMs[5] = {0, 0, 0, 0, 0};
     std::vector<uint8_t> hProp3DSet, hAtt3DSet;
     // attenuation
     int mNsls = 0, mAttCG4 = 0;
@@ -171,6 +179,7 @@ protected:
     DeviceBuffer dFlag;
 
     virtual void uploadReal() = 0;   // precision-specific operand upload
+    virtual void autotune() = 0;     // pick kernel variants by timing
 
     // profiling
     bool mProfile = false;

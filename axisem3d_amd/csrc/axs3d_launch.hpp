@@ -76,12 +76,13 @@ struct PointGeom {
 template <typename Real>
 void launch_solid_stiff_fr(hipStream_t st, const StiffArgs<Real> &a);
 
+// occ: 2, 3 or 4 = minimum waves per SIMD of the register allocation
 template <typename Real>
 void launch_solid_stiff_cd(hipStream_t st, const CardArgs<Real> &a,
-                           int n_slots);
+                           int n_slots, int occ);
 
 template <typename Real>
-int card_blocks_per_cu(bool att);
+int card_blocks_per_cu(bool att, int occ);
 
 template <typename Real>
 void launch_moment_source(hipStream_t st, bool to_elem,

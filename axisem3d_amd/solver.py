@@ -193,3 +193,12 @@ class GpuSolver:
         self._chk(self.lib.axs3d_profile_read(self.ctx, kernel, C.byref(ms),
                                               C.byref(cnt)))
         return ms.value, cnt.value
+
+    def autotune_info(self):
+        occ, blocks = C.c_int(), C.c_int()
+        ms = (C.c_double * 3)()
+        self._chk(self.lib.axs3d_autotune_info(self.ctx, C.byref(occ),
+                                               C.byref(blocks), ms))
+        return {"cardinal_occupancy": occ.value,
+                "cardinal_blocks": blocks.value,
+                "trial_ms": {"2": ms[0], "3": ms[1], "4": ms[2]}}

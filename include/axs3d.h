@@ -201,6 +201,13 @@ int axs3d_profile_read(axs3d_ctx *ctx, int kernel, double *total_ms,
                        int64_t *count);
 int axs3d_profile_reset(axs3d_ctx *ctx);
 
+/* Kernel variants chosen at axs3d_finalize by timing them on the
+ * workload (no reference counterpart; the reference has no GPU): the
+ * cardinal kernel's occupancy (minimum waves per SIMD, 2-4), its number
+ * of persistent blocks, and the trial times (ms) of variants 2, 3, 4. */
+int axs3d_autotune_info(const axs3d_ctx *ctx, int *cardinal_occupancy,
+                        int *cardinal_blocks, double *trial_ms3);
+
 #ifdef __cplusplus
 }
 #endif
