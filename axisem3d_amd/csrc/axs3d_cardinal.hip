@@ -12,16 +12,25 @@
 // halved; C2R with the Nyquist row doubled and the imaginary parts of the
 // mean and Nyquist rows ignored).
 //
-// Design: a persistent grid, one block per scratch slot, pulls elements
-// (heaviest first) from a global work counter.  For each element:
-//  A. gradient: 250 threads = 10 Fourier orders x 25 GLL points per pass;
-//     strain written to the slot's scratch, series-major [6*25][nu1].
-//  B. per batch of GLL points: the 6 strain series of a point are packed
-//     two per complex array (x + i y), inverse-transformed with an LDS
-//     mixed-radix Stockham FFT (radices 4, 2, 3, 5, 7, 11, 13), turned into
-//     stress slice by slice (3D lambda, mu; SLS memory variables in FP64),
-//     forward-transformed and unpacked back into the scratch.
-//  C. quadrature from the scratch, element-local forces to the buffer the
+// Design: a persistent grid of blocks of five wave64s.  A block takes one
+// element at a time from a global work counter (elements sorted by Nr,
+// heaviest first) and keeps its strain, then its stress, in the Fourier
+// domain in its own scratch slot, series-major [6*25][nu1max] complex.
+// Inside an element every phase is split between the waves, which
+// synchronise only among their own lanes; the block meets at two barriers
+// per element (gradient done, stress done) and one for the work counter.
+//  A. gradient: each wave takes two Fourier orders at a time (lanes 0-24
+//     order a, lanes 25-49 order a+1, one lane per GLL point), exchanging
+//     the displacement through a per-wave LDS tile.
+//  B. five GLL points per wave.  The six strain series of a point are
+//     packed two per complex array (x + i y) and inverse-transformed in
+//     place in the wave's LDS buffer by a mixed-radix Stockham FFT (radices
+//     4, 2, 3, 5, 7, 11, 13; a stage reads all its inputs into registers,
+//     then writes its outputs; plans and twiddles are precomputed per Nr).
+//     The first two arrays wait in registers while the third is
+//     transformed; then stress slice by slice (3D lambda, mu; SLS memory
+//     variables), forward transforms and unpack into the scratch.
+//  C. quadrature, split like A, element-local forces to the buffer the
 //     point kernel gathers.
 #include "axs3d_common.hpp"
 #include "axs3d_element.hpp"
@@ -29,150 +38,200 @@
 
 namespace axs3d {
 
+constexpr int CD_WAVES = 5;                        // waves per block
+constexpr int CD_THREADS = CD_WAVES * 64;
+constexpr int CD_MP = (CARD_NR_CAP + 63) / 64;     // slices per lane, max
+
 template <typename T>
 __device__ __forceinline__ cpx<T> cmulc(cpx<T> a, cpx<T> b) {
     return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
 }
 
-__device__ __forceinline__ int next_radix(int m) {
-    if (m % 4 == 0) return 4;
-    if (m % 2 == 0) return 2;
-    if (m % 3 == 0) return 3;
-    if (m % 5 == 0) return 5;
-    if (m % 7 == 0) return 7;
-    if (m % 11 == 0) return 11;
-    if (m % 13 == 0) return 13;
-    return m;
-}
-
-// One Stockham stage of radix R over `nbatch` contiguous series of length
-// n: out[(j/Ns)*Ns*R + j%Ns + q*Ns] = sum_r W_R^{rq} W_n^{r (j%Ns) n/(Ns R)}
-// in[j + r n/R] for j < n/R (Govindaraju et al., SC 2008).  sign = -1
-// forward, +1 inverse; tw[t] = exp(-2 pi i t / n).  Radices 2, 3, 4, 5
-// use closed-form butterflies; 7, 11 and 13 form each output from the
-// table (W_R^x = W_n^{x n/R}) and store it at once, keeping register use
-// at 2R + O(1) values.
 template <typename Real>
-__device__ __forceinline__ cpx<Real> twid(const cpx<Real> *__restrict__ tw,
-                                          int t, int sign) {
+__device__ __forceinline__ cpx<Real> twid(const cpx<Real> *tw, int t,
+                                          bool inv) {
     cpx<Real> w = tw[t];
-    if (sign > 0) w.im = -w.im;
+    if (inv) w.im = -w.im;
     return w;
 }
 
+// One in-place Stockham stage of radix R on the series buf[0..n) held by
+// one wave: out[(b/Ns) Ns R + b%Ns + q Ns] = sum_r W_R^{rq} W_n^{r (b%Ns)
+// step} in[b + r n/R], b < n/R, step = n/(Ns R) (Govindaraju et al., SC
+// 2008).  Forward: W = exp(-2 pi i / .); inverse (inv): conjugate.  tw[t] =
+// exp(-2 pi i t / n) in LDS.  All lanes read all their inputs before any
+// lane writes, so input and output share the buffer.  b % Ns comes from
+// the magic multiplier mg = ceil(2^32 / Ns), exact for b Ns < 2^32.
 template <typename Real, int R>
-__device__ __forceinline__ void fft_stage(const cpx<Real> *__restrict__ in,
-                                          cpx<Real> *__restrict__ out, int n,
-                                          int nbatch, int Ns, int sign,
-                                          const cpx<Real> *__restrict__ tw) {
+__device__ __forceinline__ void wstage(cpx<Real> *buf, int n, int Ns,
+                                       uint32_t mg, int step, bool inv,
+                                       const cpx<Real> *tw, int lane) {
+    constexpr int M = (CARD_NR_CAP / R + 63) / 64;
     const int nb = n / R;
-    const int step = n / (Ns * R);
-    const int total = nb * nbatch;
-    const Real sg = (Real)sign;
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-        const int h = idx / nb;
-        const int j = idx - h * nb;
-        const cpx<Real> *a = in + (size_t)h * n;
-        cpx<Real> *b = out + (size_t)h * n;
-        const int k = j % Ns;
-        const int idxD = (j / Ns) * Ns * R + k;
-        if (R >= 7) {
-            // y_q = sum_r x_r W_n^{r (k step + q n/R)}: stage twiddle and
-            // DFT kernel from one table index, reduced incrementally
-            // (k step + q n/R < n, so one subtraction keeps t < n)
-            const int wstep = n / R;
-#pragma unroll 1
-            for (int q = 0; q < R; q++) {
-                const int base = k * step + q * wstep;
-                cpx<Real> acc = a[j];
-                int t = 0;
-#pragma unroll 1
-                for (int r = 1; r < R; r++) {
-                    t += base;
-                    if (t >= n) t -= n;
-                    acc = cadd(acc, cmulc(a[j + r * nb], twid(tw, t, sign)));
-                }
-                b[idxD + q * Ns] = acc;
-            }
-            continue;
-        }
-        cpx<Real> v[R < 7 ? R : 1];
+    cpx<Real> v[M][R];
 #pragma unroll
-        for (int r = 0; r < (R < 7 ? R : 1); r++) {
-            cpx<Real> x = a[j + r * nb];
-            if (r > 0 && k > 0) x = cmulc(x, twid(tw, r * k * step, sign));
-            v[r] = x;
+    for (int m = 0; m < M; m++) {
+        const int b = lane + 64 * m;
+        if (b < nb) {
+#pragma unroll
+            for (int r = 0; r < R; r++) v[m][r] = buf[b + r * nb];
         }
-        if (R == 2) {
-            b[idxD] = cadd(v[0], v[1]);
-            b[idxD + Ns] = csub(v[0], v[1]);
-        } else if (R == 4) {
-            const cpx<Real> s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
-            const cpx<Real> s13 = cadd(v[1], v[3]), d13 = csub(v[1], v[3]);
-            // (-i) d13 forward, (+i) d13 inverse
-            const cpx<Real> rd13 = {-sg * d13.im, sg * d13.re};
-            b[idxD] = cadd(s02, s13);
-            b[idxD + 2 * Ns] = csub(s02, s13);
-            b[idxD + Ns] = cadd(d02, rd13);
-            b[idxD + 3 * Ns] = csub(d02, rd13);
-        } else if (R == 3) {
-            const Real c1 = (Real)-0.5;
-            const Real s1 = sg * (Real)0.86602540378443864676;
-            const cpx<Real> t = cadd(v[1], v[2]);
-            const cpx<Real> d = csub(v[1], v[2]);
-            const cpx<Real> m = cfma(t, c1, v[0]);
-            const cpx<Real> id = {-s1 * d.im, s1 * d.re};
-            b[idxD] = cadd(v[0], t);
-            b[idxD + Ns] = cadd(m, id);
-            b[idxD + 2 * Ns] = csub(m, id);
-        } else if (R == 5) {
-            const Real c1 = (Real)0.30901699437494742410;   // cos 2pi/5
-            const Real c2 = (Real)-0.80901699437494742410;  // cos 4pi/5
-            const Real s1 = sg * (Real)0.95105651629515357212;
-            const Real s2 = sg * (Real)0.58778525229247312917;
-            const cpx<Real> t1 = cadd(v[1], v[4]), d1 = csub(v[1], v[4]);
-            const cpx<Real> t2 = cadd(v[2], v[3]), d2 = csub(v[2], v[3]);
-            const cpx<Real> m1 = cadd(v[0], cadd(cmul(t1, c1), cmul(t2, c2)));
-            const cpx<Real> m2 = cadd(v[0], cadd(cmul(t1, c2), cmul(t2, c1)));
-            // i (s1 d1 + s2 d2) and i (s2 d1 - s1 d2)
-            const cpx<Real> e1 = cadd(cmul(d1, s1), cmul(d2, s2));
-            const cpx<Real> e2 = csub(cmul(d1, s2), cmul(d2, s1));
-            const cpx<Real> ie1 = {-e1.im, e1.re}, ie2 = {-e2.im, e2.re};
-            b[idxD] = cadd(v[0], cadd(t1, t2));
-            b[idxD + Ns] = cadd(m1, ie1);
-            b[idxD + 4 * Ns] = csub(m1, ie1);
-            b[idxD + 2 * Ns] = cadd(m2, ie2);
-            b[idxD + 3 * Ns] = csub(m2, ie2);
+    }
+    wave_sync();
+    const Real sg = inv ? (Real)1 : (Real)-1;
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+        const int b = lane + 64 * m;
+        if (b < nb) {
+            const int k =
+                Ns == 1 ? 0 : b - (int)__umulhi((uint32_t)b, mg) * Ns;
+            if (k > 0) {
+#pragma unroll
+                for (int r = 1; r < R; r++)
+                    v[m][r] = cmulc(v[m][r], twid(tw, r * k * step, inv));
+            }
+            cpx<Real> *o = buf + (b - k) * R + k;
+            const cpx<Real> *x = v[m];
+            if constexpr (R == 2) {
+                o[0] = cadd(x[0], x[1]);
+                o[Ns] = csub(x[0], x[1]);
+            } else if constexpr (R == 4) {
+                const cpx<Real> s02 = cadd(x[0], x[2]), d02 = csub(x[0], x[2]);
+                const cpx<Real> s13 = cadd(x[1], x[3]), d13 = csub(x[1], x[3]);
+                // (-i) d13 forward, (+i) d13 inverse
+                const cpx<Real> rd13 = {-sg * d13.im, sg * d13.re};
+                o[0] = cadd(s02, s13);
+                o[2 * Ns] = csub(s02, s13);
+                o[Ns] = cadd(d02, rd13);
+                o[3 * Ns] = csub(d02, rd13);
+            } else if constexpr (R == 3) {
+                const Real c1 = (Real)-0.5;
+                const Real s1 = sg * (Real)0.86602540378443864676;
+                const cpx<Real> tt = cadd(x[1], x[2]);
+                const cpx<Real> d = csub(x[1], x[2]);
+                const cpx<Real> mm = cfma(tt, c1, x[0]);
+                const cpx<Real> id = {-s1 * d.im, s1 * d.re};
+                o[0] = cadd(x[0], tt);
+                o[Ns] = cadd(mm, id);
+                o[2 * Ns] = csub(mm, id);
+            } else if constexpr (R == 5) {
+                const Real c1 = (Real)0.30901699437494742410;   // cos 2pi/5
+                const Real c2 = (Real)-0.80901699437494742410;  // cos 4pi/5
+                const Real s1 = sg * (Real)0.95105651629515357212;
+                const Real s2 = sg * (Real)0.58778525229247312917;
+                const cpx<Real> t1 = cadd(x[1], x[4]), d1 = csub(x[1], x[4]);
+                const cpx<Real> t2 = cadd(x[2], x[3]), d2 = csub(x[2], x[3]);
+                const cpx<Real> m1 =
+                    cadd(x[0], cadd(cmul(t1, c1), cmul(t2, c2)));
+                const cpx<Real> m2 =
+                    cadd(x[0], cadd(cmul(t1, c2), cmul(t2, c1)));
+                // i (s1 d1 + s2 d2) and i (s2 d1 - s1 d2)
+                const cpx<Real> e1 = cadd(cmul(d1, s1), cmul(d2, s2));
+                const cpx<Real> e2 = csub(cmul(d1, s2), cmul(d2, s1));
+                const cpx<Real> ie1 = {-e1.im, e1.re}, ie2 = {-e2.im, e2.re};
+                o[0] = cadd(x[0], cadd(t1, t2));
+                o[Ns] = cadd(m1, ie1);
+                o[4 * Ns] = csub(m1, ie1);
+                o[2 * Ns] = cadd(m2, ie2);
+                o[3 * Ns] = csub(m2, ie2);
+            } else {
+                // radices 7, 11, 13: direct DFT, one output at a time, the
+                // kernel W_R^{rq} = tw[(rq mod R) n/R] read from LDS
+#pragma unroll 1
+                for (int q = 0; q < R; q++) {
+                    cpx<Real> acc = x[0];
+                    int rq = 0;
+#pragma unroll
+                    for (int r = 1; r < R; r++) {
+                        rq += q;
+                        if (rq >= R) rq -= R;
+                        acc = cadd(acc, cmulc(x[r], twid(tw, rq * nb, inv)));
+                    }
+                    o[q * Ns] = acc;
+                }
+            }
+        }
+    }
+    wave_sync();
+}
+
+// Whole FFT of buf[0..n) by one wave, stages from the plan.
+template <typename Real>
+__device__ __forceinline__ void wfft(cpx<Real> *buf, int n,
+                                     const uint32_t *__restrict__ plan,
+                                     bool inv, const cpx<Real> *tw, int lane) {
+    const int nst = (int)plan[0];
+    for (int s = 0; s < nst; s++) {
+        const uint32_t w0 = plan[1 + 3 * s];
+        const uint32_t mg = plan[2 + 3 * s];
+        const int step = (int)plan[3 + 3 * s];
+        const int R = (int)(w0 & 255u), Ns = (int)(w0 >> 8);
+        switch (R) {
+            case 4: wstage<Real, 4>(buf, n, Ns, mg, step, inv, tw, lane); break;
+            case 2: wstage<Real, 2>(buf, n, Ns, mg, step, inv, tw, lane); break;
+            case 3: wstage<Real, 3>(buf, n, Ns, mg, step, inv, tw, lane); break;
+            case 5: wstage<Real, 5>(buf, n, Ns, mg, step, inv, tw, lane); break;
+            case 7: wstage<Real, 7>(buf, n, Ns, mg, step, inv, tw, lane); break;
+            case 11: wstage<Real, 11>(buf, n, Ns, mg, step, inv, tw, lane); break;
+            default: wstage<Real, 13>(buf, n, Ns, mg, step, inv, tw, lane); break;
         }
     }
 }
 
-// Batched FFT of `nbatch` series of length n held in LDS buffer a (b is the
-// ping-pong partner).  Returns the buffer holding the result.  Block-wide:
-// every thread must call it.
+// C2R input of the strain pair (X, Y) = rows (2jp, 2jp+1) of point p:
+// buf[k] = Xfull[k] + i Yfull[k], with Xfull the Hermitian extension
+// under SolverFFTW's C2R rules (Nyquist doubled, Im of mean and Nyquist
+// ignored).
 template <typename Real>
-__device__ cpx<Real> *fft_batch(cpx<Real> *a, cpx<Real> *b, int n, int nbatch,
-                                int sign, const cpx<Real> *__restrict__ tw) {
-    int Ns = 1, rem = n;
-    while (rem > 1) {
-        const int R = next_radix(rem);
-        switch (R) {
-            case 4: fft_stage<Real, 4>(a, b, n, nbatch, Ns, sign, tw); break;
-            case 2: fft_stage<Real, 2>(a, b, n, nbatch, Ns, sign, tw); break;
-            case 3: fft_stage<Real, 3>(a, b, n, nbatch, Ns, sign, tw); break;
-            case 5: fft_stage<Real, 5>(a, b, n, nbatch, Ns, sign, tw); break;
-            case 7: fft_stage<Real, 7>(a, b, n, nbatch, Ns, sign, tw); break;
-            case 11: fft_stage<Real, 11>(a, b, n, nbatch, Ns, sign, tw); break;
-            default: fft_stage<Real, 13>(a, b, n, nbatch, Ns, sign, tw); break;
+__device__ __forceinline__ void pack_pair(cpx<Real> *buf,
+                                          const cpx<Real> *__restrict__ X,
+                                          const cpx<Real> *__restrict__ Y,
+                                          int nr, int nu1, bool even,
+                                          int lane) {
+#pragma unroll 2
+    for (int k = lane; k < nr; k += 64) {
+        {
+            cpx<Real> xf, yf;
+            if (k == 0) {
+                xf = {X[0].re, 0};
+                yf = {Y[0].re, 0};
+            } else if (k < nu1) {
+                xf = X[k];
+                yf = Y[k];
+                if (even && k == nu1 - 1) {
+                    xf = {(Real)2 * xf.re, 0};
+                    yf = {(Real)2 * yf.re, 0};
+                }
+            } else {
+                const cpx<Real> xs = X[nr - k], ys = Y[nr - k];
+                xf = {xs.re, -xs.im};
+                yf = {ys.re, -ys.im};
+            }
+            buf[k] = {xf.re - yf.im, xf.im + yf.re};
         }
-        __syncthreads();
-        cpx<Real> *t = a;
-        a = b;
-        b = t;
-        Ns *= R;
-        rem /= R;
     }
-    return a;
+    wave_sync();
+}
+
+// R2C output: X_k = (Z_k + conj Z_{N-k}) / 2N, Y_k = (Z_k - conj Z_{N-k})
+// / 2iN, Nyquist row halved; written to rows (2jp, 2jp+1).
+template <typename Real>
+__device__ __forceinline__ void unpack_pair(const cpx<Real> *buf,
+                                            cpx<Real> *__restrict__ X,
+                                            cpx<Real> *__restrict__ Y, int nr,
+                                            int nu1, bool even, int lane) {
+    const Real h0 = (Real)0.5 / (Real)nr;
+#pragma unroll 2
+    for (int k = lane; k < nu1; k += 64) {
+        {
+            const cpx<Real> zk = buf[k];
+            const cpx<Real> zn = buf[k == 0 ? 0 : nr - k];
+            const Real h = (even && k == nu1 - 1) ? h0 * (Real)0.5 : h0;
+            X[k] = {(zk.re + zn.re) * h, (zk.im - zn.im) * h};
+            Y[k] = {(zk.im + zn.im) * h, (zn.re - zk.re) * h};
+        }
+    }
+    wave_sync();
 }
 
 // CG4 attenuation points (1,1),(1,3),(3,1),(3,3) (AttBuilder.cpp:141-147).
@@ -181,256 +240,315 @@ __device__ __forceinline__ int att_slot(int p, int npa) {
     return p == 6 ? 0 : p == 8 ? 1 : p == 16 ? 2 : p == 18 ? 3 : -1;
 }
 
+// Per-slice material of one GLL point: pointers at slice 0 of element e;
+// component c of slice k sits at [c * Nr + k].
 template <typename Real>
-struct CardLds {
-    // phases A and C
-    cpx<Real> sU[EMB][3][NPEM];
-    cpx<Real> sX[EMB][3][NPEM];
-    cpx<Real> sY[EMB][3][NPEM];
+struct SliceMat {
+    const Real *lam, *mu;        // 3D lambda, mu of this point
+    const Real *dl, *dm;         // memory moduli of its attenuation slot
+    Real *mv;                    // memory variables [nsls][6] of the slot
+    Real *ds;                    // memory stress [6] of the slot
+    size_t cs;                   // component step of mv, ds, dl/dm: npa Nr
 };
 
-// OCC = minimum waves per SIMD the register allocation must allow (2, 3
-// or 4 blocks of 256 threads per CU); picked per workload by measurement.
-template <typename Real, bool ATT, int OCC>
-__global__ __launch_bounds__(STIFF_BLOCK, OCC) void k_solid_stiff_cd(
-    CardArgs<Real> a) {
-    constexpr size_t fft_bytes = 2 * (size_t)CARD_FFT_CAP * sizeof(cpx<Real>);
-    constexpr size_t ac_bytes = sizeof(CardLds<Real>);
-    constexpr size_t lds_bytes = fft_bytes > ac_bytes ? fft_bytes : ac_bytes;
-    __shared__ __align__(16) unsigned char smem[lds_bytes];
-    __shared__ Real sG[2][NPEM];
-    __shared__ int sWork;
-    CardLds<Real> &L = *reinterpret_cast<CardLds<Real> *>(smem);
-    cpx<Real> *buf0 = reinterpret_cast<cpx<Real> *>(smem);
-    cpx<Real> *buf1 = buf0 + CARD_FFT_CAP;
+// Stress component d of one slice (Isotropic::strainToStress<_3D_CD>,
+// Isotropic.hpp:155-172: sigma_d = lambda tr + 2 mu eps_d for d < 3, mu
+// eps_d for the engineering shears), with the SLS memory variables of
+// this component when the point carries them (Attenuation::updateMem*,
+// AttenuationFull/CG4::apply: sigma -= sum_l m_l; m_l = alpha_l m_l +
+// beta_l ds_old + gamma_l ds_new; ds_old = ds_new).  Components are
+// independent apart from the trace, so they can be formed in any order.
+template <typename Real, bool ATT>
+__device__ __forceinline__ Real comp_stress(const CardArgs<Real> &a,
+                                            const SliceMat<Real> &M, bool att,
+                                            int d, int k, Real eps, Real tr,
+                                            Real lam, Real mu) {
+    Real sig = d < 3 ? tr * lam + eps * (mu * (Real)2) : eps * mu;
+    if (!ATT || !att) return sig;
+    const Real dl = M.dl[k], dm = M.dm[k];
+    const Real dsn = d < 3 ? tr * dl + eps * (dm * (Real)2) : eps * dm;
+    Real *ds = M.ds + (size_t)d * M.cs + k;
+    const Real dso = *ds;
+    const int nsls = a.nsls;
+    Real *mv = M.mv + (size_t)d * M.cs + k;
+    const size_t ls = 6 * M.cs;
+    for (int l = 0; l < nsls; l++) sig -= mv[l * ls];
+    for (int l = 0; l < nsls; l++) {
+        const Real al = (Real)a.abg[l];
+        const Real be = (Real)a.abg[nsls + l];
+        const Real ga = (Real)a.abg[2 * nsls + l];
+        Real &m = mv[l * ls];
+        m = al * m + be * dso + ga * dsn;
+    }
+    *ds = dsn;
+    return sig;
+}
 
-    const StiffArgs<Real> &s = a.s;
-    const int t = threadIdx.x;
-    if (t < 2 * NPEM) sG[t / NPEM][t % NPEM] = s.gtab[t];
-    const int nu1max = a.nu1max;
-    cpx<Real> *S = reinterpret_cast<cpx<Real> *>(a.scratch) +
-                   (size_t)blockIdx.x * 6 * NPEM * nu1max;
-    // phase A/C thread map: 10 orders x 25 points, order fastest
-    const bool lane_ok = t < EMB * NPEM;
-    const int eml = t % EMB;
-    const int ipnt = lane_ok ? t / EMB : 0;
-    const int i = ipnt / NPED, j = ipnt - (ipnt / NPED) * NPED;
+// Per-wave LDS tiles of phases A and C (two element-modes), aliased with
+// the wave's FFT buffers.  Consecutive passes alternate between two
+// copies (first index): the addresses then change from pass to pass and
+// the compiler recomputes them instead of holding dozens of them in
+// registers across the loop.
+template <typename Real>
+struct WaveTiles {
+    cpx<Real> U[2][2][3][NPEM];
+    cpx<Real> X[2][2][3][NPEM];
+    cpx<Real> Y[2][2][3][NPEM];
+};
 
-    for (;;) {
-        if (t == 0) sWork = atomicAdd(a.work_counter, 1);
-        __syncthreads();
-        const int w = sWork;
-        __syncthreads();
-        if (w >= a.n3d) break;
-        const int e = a.list3d[w];
-        const int nu1 = s.elem_nu1[e];
-        const int nr = a.elem_nr[e];
-        const bool axial = s.elem_axial[e] != 0;
-        const bool axial_row0 = axial && (i == 0);
+// Two complex series of Nr per wave (FFT work buffers).
+template <typename Real>
+struct CdLds {
+    cpx<Real> tw[CARD_NR_CAP];
+    cpx<Real> wbuf[CD_WAVES][2][CARD_NR_CAP];
+    Real G[2][NPEM];
+    int work;
+};
+static_assert(sizeof(WaveTiles<float>) <= 2 * sizeof(cpx<float>) * CARD_NR_CAP,
+              "phase A/C tiles must fit the wave buffers");
+static_assert(sizeof(WaveTiles<double>) <=
+                  2 * sizeof(cpx<double>) * CARD_NR_CAP,
+              "phase A/C tiles must fit the wave buffers");
 
-        int nu1p = 0;
-        uint32_t off = 0;
-        if (lane_ok) {
-            const int p = s.elem_points[e * NPEM + ipnt];
-            nu1p = s.point_nu1[p];
-            off = s.point_off[p];
+// Lane map of phases A and C: two Fourier orders x 25 GLL points per wave
+// (lanes 0-24 order a0, 25-49 order a0+1), lanes 50-63 idle.
+struct AcLane {
+    bool ok;
+    int hh, ipnt, i, j;
+    __device__ explicit AcLane(int lane) {
+        ok = lane < 2 * NPEM;
+        hh = ok ? lane / NPEM : 0;
+        ipnt = ok ? lane - hh * NPEM : 0;
+        i = ipnt / NPED;
+        j = ipnt - i * NPED;
+    }
+};
+
+
+// A: gradient of element e, Fourier orders a0 = 2 wv, 2 wv + 2 CD_WAVES, ...
+// into the scratch rows [d*25 + ipnt][alpha].
+template <typename Real>
+__device__ __forceinline__ void cd_phase_grad(const StiffArgs<Real> *s,
+                                           cpx<Real> *S, int nu1max,
+                                           WaveTiles<Real> *T,
+                                           const Real (*Gtab)[NPEM], int e,
+                                           int nu1, int lane, int wv) {
+    const AcLane ln(lane);
+    const bool axial = s->elem_axial[e] != 0;
+    const bool axial_row0 = axial && ln.i == 0;
+    int nu1p = 0;
+    uint32_t off = 0;
+    PointOps<Real> g{}, q{};
+    if (ln.ok) {
+        const int p = s->elem_points[e * NPEM + ln.ipnt];
+        nu1p = s->point_nu1[p];
+        off = s->point_off[p];
+        load_point_ops(s->elem_ops, e, ln.ipnt, g, q);
+    }
+    PointG<Real> G;
+    load_point_g<Real>(Gtab, axial, ln.i, ln.j, G);
+    int par = 0;
+#pragma unroll 1
+    for (int a0 = 2 * wv; a0 < nu1; a0 += 2 * CD_WAVES, par ^= 1) {
+        const int alpha = a0 + ln.hh;
+        const bool active = ln.ok && alpha < nu1;
+        cpx<Real> u[3] = {{0, 0}, {0, 0}, {0, 0}};
+        if (active) {
+            if (alpha < nu1p) {
+                const double *src = s->displ + (size_t)(off + alpha) * 6;
+#pragma unroll
+                for (int c = 0; c < 3; c++)
+                    u[c] = {(Real)src[2 * c], (Real)src[2 * c + 1]};
+            }
+#pragma unroll
+            for (int c = 0; c < 3; c++) T->U[par][ln.hh][c][ln.ipnt] = u[c];
         }
-
-        // ---------------- A. gradient -> strain (Fourier) in scratch
-        {
-        PointG<Real> G;
-        load_point_g<Real>(sG, axial, i, j, G);
-        PointOps<Real> g{}, q{};
-        if (lane_ok) load_point_ops(s.elem_ops, e, ipnt, g, q);
-        for (int a0 = 0; a0 < nu1; a0 += EMB) {
-            const int alpha = a0 + eml;
-            const bool active = lane_ok && alpha < nu1;
-            cpx<Real> u[3] = {{0, 0}, {0, 0}, {0, 0}};
-            if (active) {
-                if (alpha < nu1p) {
-                    const double *src = s.displ + (size_t)(off + alpha) * 6;
+        wave_sync();
+        if (active) {
+            cpx<Real> eps[6];
+            grad6_point<Real>(T->U[par][ln.hh], u, (Real)alpha, axial_row0, ln.i,
+                              ln.j, G, g, eps);
 #pragma unroll
-                    for (int c = 0; c < 3; c++)
-                        u[c] = {(Real)src[2 * c], (Real)src[2 * c + 1]};
-                }
-#pragma unroll
-                for (int c = 0; c < 3; c++) L.sU[eml][c][ipnt] = u[c];
-            }
-            __syncthreads();
-            if (active) {
-                cpx<Real> eps[6];
-                grad6_point<Real>(L.sU[eml], u, (Real)alpha, axial_row0, i, j,
-                                  G, g, eps);
-#pragma unroll
-                for (int d = 0; d < 6; d++)
-                    S[(size_t)(d * NPEM + ipnt) * nu1max + alpha] = eps[d];
-            }
-            __syncthreads();
+            for (int d = 0; d < 6; d++)
+                S[(size_t)(d * NPEM + ln.ipnt) * nu1max + alpha] = eps[d];
         }
-        }
+        wave_sync();
+    }
+}
 
-        // ---------------- B. FFT -> stress (+ attenuation) -> FFT
-        const cpx<Real> *tw = a.twiddle + a.tw_off[nr];
-        const int np = min(NPEM, CARD_FFT_CAP / (3 * nr));
-        const int64_t sl0 = a.slice_off[e];
-        const Real inv_nr = (Real)1 / (Real)nr;
-        const bool even = (nr & 1) == 0;
-        for (int p0 = 0; p0 < NPEM; p0 += np) {
-            const int npc = min(np, NPEM - p0);
-            // pack: z = Xfull + i Yfull for dim pairs (0,1), (2,3), (4,5)
-            for (int idx = t; idx < npc * 3 * nr; idx += blockDim.x) {
-                const int qq = idx / (3 * nr);
-                const int rem = idx - qq * 3 * nr;
-                const int jp = rem / nr;
-                const int k = rem - jp * nr;
-                const int pp = p0 + qq;
-                const cpx<Real> *X = S + (size_t)((2 * jp) * NPEM + pp) * nu1max;
-                const cpx<Real> *Y = X + (size_t)NPEM * nu1max;
-                cpx<Real> xf, yf;
-                if (k == 0) {
-                    xf = {X[0].re, 0};
-                    yf = {Y[0].re, 0};
-                } else if (k < nu1) {
-                    if (even && k == nu1 - 1) {
-                        xf = {(Real)2 * X[k].re, 0};
-                        yf = {(Real)2 * Y[k].re, 0};
-                    } else {
-                        xf = X[k];
-                        yf = Y[k];
-                    }
-                } else {
-                    const cpx<Real> xs = X[nr - k], ys = Y[nr - k];
-                    xf = {xs.re, -xs.im};
-                    yf = {ys.re, -ys.im};
-                }
-                buf0[idx] = {xf.re - yf.im, xf.im + yf.re};
-            }
-            __syncthreads();
-            cpx<Real> *res = fft_batch<Real>(buf0, buf1, nr, npc * 3, +1, tw);
-            cpx<Real> *oth = res == buf0 ? buf1 : buf0;
-            // stress slice by slice
-            for (int idx = t; idx < npc * nr; idx += blockDim.x) {
-                const int qq = idx / nr;
-                const int k = idx - qq * nr;
-                const int pp = p0 + qq;
-                cpx<Real> *z = res + (size_t)qq * 3 * nr + k;
-                Real eps[6], sig[6];
+// B: the six series of GLL point pp in the wave's two LDS buffers: pair
+// (2,3) into buffer 0, where sigma_pz is formed and eps_zz waits; pair
+// (0,1) into buffer 1, which completes the trace for sigma_ss, sigma_pp
+// and sigma_zz; both go back; then the shear pair (4,5).
+template <typename Real, bool ATT>
+__device__ __forceinline__ void cd_phase_point(const CardArgs<Real> *a,
+                                               cpx<Real> *S, cpx<Real> *b0,
+                                               cpx<Real> *b1,
+                                               const cpx<Real> *tw, int e,
+                                               int nr, int nu1, int pp,
+                                               int lane) {
+    const int nu1max = a->nu1max;
+    const uint32_t *plan = a->plan + (size_t)nr * CARD_PLAN_W;
+    const int64_t sl0 = a->slice_off[e];
+    const bool even = (nr & 1) == 0;
+    cpx<Real> *row[6];
 #pragma unroll
-                for (int jp = 0; jp < 3; jp++) {
-                    eps[2 * jp] = z[jp * nr].re;
-                    eps[2 * jp + 1] = z[jp * nr].im;
-                }
-                const Real *lm = a.prop3d + ((size_t)(sl0 + k) * 2) * NPEM;
-                const Real lam = lm[pp], mu = lm[NPEM + pp];
-                const Real mu2 = mu * (Real)2;
-                const Real tr = (eps[0] + eps[1] + eps[2]) * lam;
-                sig[0] = tr + eps[0] * mu2;
-                sig[1] = tr + eps[1] * mu2;
-                sig[2] = tr + eps[2] * mu2;
-                sig[3] = eps[3] * mu;
-                sig[4] = eps[4] * mu;
-                sig[5] = eps[5] * mu;
-                if (ATT) {
-                    const int ip4 = att_slot(pp, a.npa);
-                    if (ip4 >= 0) {
-                        const int npa = a.npa, nsls = a.nsls;
-                        const size_t sl = (size_t)(sl0 + k);
-                        double *mv = a.memvar + sl * nsls * 6 * npa + ip4;
-                        Real *ds = a.dstress + sl * 6 * npa + ip4;
-                        const Real *dlm = a.att3d + sl * 2 * npa + ip4;
-                        const Real dl = dlm[0], dm = dlm[npa];
-                        // stress -= sum of memory variables
-                        double msum[6] = {0, 0, 0, 0, 0, 0};
-                        for (int l = 0; l < nsls; l++)
-#pragma unroll
-                            for (int d = 0; d < 6; d++)
-                                msum[d] += mv[(l * 6 + d) * npa];
-#pragma unroll
-                        for (int d = 0; d < 6; d++)
-                            sig[d] = (Real)((double)sig[d] - msum[d]);
-                        // new memory stress from this strain
-                        const Real dm2 = dm * (Real)2;
-                        const Real dtr = (eps[0] + eps[1] + eps[2]) * dl;
-                        Real dsn[6];
-                        dsn[0] = dtr + eps[0] * dm2;
-                        dsn[1] = dtr + eps[1] * dm2;
-                        dsn[2] = dtr + eps[2] * dm2;
-                        dsn[3] = eps[3] * dm;
-                        dsn[4] = eps[4] * dm;
-                        dsn[5] = eps[5] * dm;
-                        for (int l = 0; l < nsls; l++) {
-                            const double al = a.abg[l];
-                            const double be = a.abg[nsls + l];
-                            const double ga = a.abg[2 * nsls + l];
-#pragma unroll
-                            for (int d = 0; d < 6; d++) {
-                                double m = mv[(l * 6 + d) * npa];
-                                m = al * m + be * (double)ds[d * npa];
-                                m += ga * (double)dsn[d];
-                                mv[(l * 6 + d) * npa] = m;
-                            }
-                        }
-#pragma unroll
-                        for (int d = 0; d < 6; d++) ds[d * npa] = dsn[d];
-                    }
-                }
-#pragma unroll
-                for (int jp = 0; jp < 3; jp++)
-                    z[jp * nr] = {sig[2 * jp], sig[2 * jp + 1]};
-            }
-            __syncthreads();
-            cpx<Real> *fres = fft_batch<Real>(res, oth, nr, npc * 3, -1, tw);
-            // unpack X_k = (Z_k + conj Z_{N-k}) / 2, Y_k = (Z_k - conj Z_{N-k}) / 2i
-            for (int idx = t; idx < npc * 3 * nu1; idx += blockDim.x) {
-                const int qq = idx / (3 * nu1);
-                const int rem = idx - qq * 3 * nu1;
-                const int jp = rem / nu1;
-                const int k = rem - jp * nu1;
-                const int pp = p0 + qq;
-                const cpx<Real> *z = fres + (size_t)(qq * 3 + jp) * nr;
-                const cpx<Real> zk = z[k];
-                const cpx<Real> zn = z[k == 0 ? 0 : nr - k];
-                Real h = (Real)0.5 * inv_nr;
-                if (even && k == nu1 - 1) h *= (Real)0.5;
-                const cpx<Real> X = {(zk.re + zn.re) * h, (zk.im - zn.im) * h};
-                const cpx<Real> Y = {(zk.im + zn.im) * h, (zn.re - zk.re) * h};
-                S[(size_t)((2 * jp) * NPEM + pp) * nu1max + k] = X;
-                S[(size_t)((2 * jp + 1) * NPEM + pp) * nu1max + k] = Y;
-            }
-            __syncthreads();
-        }
+    for (int d = 0; d < 6; d++) row[d] = S + (size_t)(d * NPEM + pp) * nu1max;
+    SliceMat<Real> M;
+    M.lam = a->prop3d + (size_t)sl0 * 2 * NPEM + (size_t)pp * nr;
+    M.mu = M.lam + (size_t)NPEM * nr;
+    const int ip4 = ATT ? att_slot(pp, a->npa) : -1;
+    const bool att = ATT && ip4 >= 0;
+    if (ATT) {
+        const int npa = a->npa;
+        M.cs = (size_t)npa * nr;
+        M.dl = a->att3d + (size_t)sl0 * 2 * npa + (size_t)ip4 * nr;
+        M.dm = M.dl + M.cs;
+        M.mv = a->memvar + (size_t)sl0 * a->nsls * 6 * npa + (size_t)ip4 * nr;
+        M.ds = a->dstress + (size_t)sl0 * 6 * npa + (size_t)ip4 * nr;
+    }
+    // (eps_zz, eps_pz) -> (eps_zz, sigma_pz) in b0
+    pack_pair<Real>(b0, row[2], row[3], nr, nu1, even, lane);
+    wfft<Real>(b0, nr, plan, true, tw, lane);
+#pragma unroll 2
+    for (int k = lane; k < nr; k += 64) {
+        const cpx<Real> z = b0[k];
+        b0[k] = {z.re, comp_stress<Real, ATT>(*a, M, att, 3, k, z.im, 0, 0,
+                                              M.mu[k])};
+    }
+    // (eps_ss, eps_pp) in b1; trace; sigma_ss, sigma_pp to b1, sigma_zz to b0
+    pack_pair<Real>(b1, row[0], row[1], nr, nu1, even, lane);
+    wfft<Real>(b1, nr, plan, true, tw, lane);
+#pragma unroll 2
+    for (int k = lane; k < nr; k += 64) {
+        const cpx<Real> z = b1[k];
+        const cpx<Real> y = b0[k];
+        const Real lam = M.lam[k], mu = M.mu[k];
+        const Real tr = z.re + z.im + y.re;
+        const Real s0 = comp_stress<Real, ATT>(*a, M, att, 0, k, z.re, tr,
+                                               lam, mu);
+        const Real s1 = comp_stress<Real, ATT>(*a, M, att, 1, k, z.im, tr,
+                                               lam, mu);
+        const Real s2 = comp_stress<Real, ATT>(*a, M, att, 2, k, y.re, tr,
+                                               lam, mu);
+        b1[k] = {s0, s1};
+        b0[k] = {s2, y.im};
+    }
+    wave_sync();
+    wfft<Real>(b1, nr, plan, false, tw, lane);
+    unpack_pair<Real>(b1, row[0], row[1], nr, nu1, even, lane);
+    wfft<Real>(b0, nr, plan, false, tw, lane);
+    unpack_pair<Real>(b0, row[2], row[3], nr, nu1, even, lane);
+    // shears (eps_sz, eps_sp) in b0
+    pack_pair<Real>(b0, row[4], row[5], nr, nu1, even, lane);
+    wfft<Real>(b0, nr, plan, true, tw, lane);
+#pragma unroll 2
+    for (int k = lane; k < nr; k += 64) {
+        const cpx<Real> z = b0[k];
+        const Real mu = M.mu[k];
+        const Real s4 = comp_stress<Real, ATT>(*a, M, att, 4, k, z.re, 0, 0,
+                                               mu);
+        const Real s5 = comp_stress<Real, ATT>(*a, M, att, 5, k, z.im, 0, 0,
+                                               mu);
+        b0[k] = {s4, s5};
+    }
+    wave_sync();
+    wfft<Real>(b0, nr, plan, false, tw, lane);
+    unpack_pair<Real>(b0, row[4], row[5], nr, nu1, even, lane);
+}
 
-        // ---------------- C. quadrature -> element-local forces
-        {
-        PointG<Real> G;
-        load_point_g<Real>(sG, axial, i, j, G);
-        PointOps<Real> g{}, q{};
-        if (lane_ok) load_point_ops(s.elem_ops, e, ipnt, g, q);
-        for (int a0 = 0; a0 < nu1; a0 += EMB) {
-            const int alpha = a0 + eml;
-            const bool active = lane_ok && alpha < nu1;
+// C: quadrature of element e from the scratch stress rows into the
+// element-local force buffer.
+template <typename Real>
+__device__ __forceinline__ void cd_phase_quad(const StiffArgs<Real> *s,
+                                           const cpx<Real> *S, int nu1max,
+                                           WaveTiles<Real> *T,
+                                           const Real (*Gtab)[NPEM], int e,
+                                           int nu1, int lane, int wv) {
+    const AcLane ln(lane);
+    const bool axial = s->elem_axial[e] != 0;
+    const bool axial_row0 = axial && ln.i == 0;
+    int nu1p = 0;
+    PointOps<Real> g{}, q{};
+    if (ln.ok) {
+        const int p = s->elem_points[e * NPEM + ln.ipnt];
+        nu1p = s->point_nu1[p];
+        load_point_ops(s->elem_ops, e, ln.ipnt, g, q);
+    }
+    PointG<Real> G;
+    load_point_g<Real>(Gtab, axial, ln.i, ln.j, G);
+    Real *eb = s->eb + (size_t)s->elem_eb_off[e] * 6;
+    int par = 0;
+#pragma unroll 1
+    for (int a0 = 2 * wv; a0 < nu1; a0 += 2 * CD_WAVES, par ^= 1) {
+        const int alpha = a0 + ln.hh;
+        const bool active = ln.ok && alpha < nu1;
+        cpx<Real> Lt[3];
+        if (active) {
             cpx<Real> sig[6];
-            if (active) {
 #pragma unroll
-                for (int d = 0; d < 6; d++)
-                    sig[d] = S[(size_t)(d * NPEM + ipnt) * nu1max + alpha];
-            }
+            for (int d = 0; d < 6; d++)
+                sig[d] = S[(size_t)(d * NPEM + ln.ipnt) * nu1max + alpha];
+            quad6_prep<Real>(sig, (Real)alpha, axial_row0, q,
+                             T->X[par][ln.hh], T->Y[par][ln.hh], ln.ipnt, Lt);
+        }
+        wave_sync();
+        if (active) {
             cpx<Real> F[3];
-            quad6_point<Real>(sig, (Real)alpha, axial_row0, q,
-                              L.sX[active ? eml : 0], L.sY[active ? eml : 0],
-                              ipnt, i, j, G, active, F);
-            if (active && alpha < nu1p) {
-                Real *dst = s.eb + ((size_t)s.elem_eb_off[e] +
-                                    (size_t)ipnt * nu1 + alpha) * 6;
+            quad6_finish<Real>(T->X[par][ln.hh], T->Y[par][ln.hh], ln.i, ln.j,
+                               G, Lt, F);
+            if (alpha < nu1p) {
+                Real *dst = eb + ((size_t)ln.ipnt * nu1 + alpha) * 6;
 #pragma unroll
                 for (int c = 0; c < 3; c++) {
                     dst[2 * c] = F[c].re;
                     dst[2 * c + 1] = F[c].im;
                 }
             }
-            __syncthreads();
         }
+        wave_sync();
+    }
+}
+
+// OCC = minimum waves per SIMD the register allocation must allow; picked
+// per workload by measurement.
+template <typename Real, bool ATT, int OCC>
+__global__ __launch_bounds__(CD_THREADS, OCC) void k_solid_stiff_cd(
+    CardArgs<Real> a) {
+    __shared__ CdLds<Real> L;
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    // the wave index is uniform across the wave: keep it (and all that is
+    // derived from it) in scalar registers
+    const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
+    if (t < 2 * NPEM) L.G[t / NPEM][t % NPEM] = a.s.gtab[t];
+    const int nu1max = a.nu1max;
+    cpx<Real> *S = reinterpret_cast<cpx<Real> *>(a.scratch) +
+                   (size_t)blockIdx.x * 6 * NPEM * nu1max;
+    cpx<Real> *b0 = L.wbuf[wv][0], *b1 = L.wbuf[wv][1];
+    WaveTiles<Real> *T = reinterpret_cast<WaveTiles<Real> *>(b0);
+    for (;;) {
+        if (t == 0) L.work = atomicAdd(a.work_counter, 1);
+        __syncthreads();
+        const int w = L.work;
+        if (w >= a.n3d) break;
+        const int e = a.list3d[w];
+        const int nu1 = a.s.elem_nu1[e];
+        const int nr = a.elem_nr[e];
+        {
+            const cpx<Real> *twg = a.twiddle + a.tw_off[nr];
+            for (int x = t; x < nr; x += CD_THREADS) L.tw[x] = twg[x];
         }
+        cd_phase_grad<Real>(&a.s, S, nu1max, T, L.G, e, nu1, lane, wv);
+        __syncthreads();
+        for (int pp = wv; pp < NPEM; pp += CD_WAVES)
+            cd_phase_point<Real, ATT>(&a, S, b0, b1, L.tw, e, nr, nu1, pp,
+                                      lane);
+        __syncthreads();
+        cd_phase_quad<Real>(&a.s, S, nu1max, T, L.G, e, nu1, lane, wv);
+        // every wave is done with this element's scratch and with the
+        // work slot before the next element is fetched
+        __syncthreads();
     }
 }
 
@@ -444,15 +562,15 @@ static void launch_cd_att(hipStream_t st, const CardArgs<Real> &a, int n_slots,
     switch (occ) {
         case 4:
             hipLaunchKernelGGL((k_solid_stiff_cd<Real, ATT, 4>),
-                               dim3(n_slots), dim3(STIFF_BLOCK), 0, st, a);
+                               dim3(n_slots), dim3(CD_THREADS), 0, st, a);
             break;
         case 3:
             hipLaunchKernelGGL((k_solid_stiff_cd<Real, ATT, 3>),
-                               dim3(n_slots), dim3(STIFF_BLOCK), 0, st, a);
+                               dim3(n_slots), dim3(CD_THREADS), 0, st, a);
             break;
         default:
             hipLaunchKernelGGL((k_solid_stiff_cd<Real, ATT, 2>),
-                               dim3(n_slots), dim3(STIFF_BLOCK), 0, st, a);
+                               dim3(n_slots), dim3(CD_THREADS), 0, st, a);
             break;
     }
 }
@@ -472,20 +590,23 @@ void launch_solid_stiff_cd(hipStream_t st, const CardArgs<Real> &a,
 template <typename Real, bool ATT, int OCC>
 static int blocks_per_cu_v() {
     int n = 0;
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &n, k_solid_stiff_cd<Real, ATT, OCC>, STIFF_BLOCK, 0);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &n, k_solid_stiff_cd<Real, ATT, OCC>, CD_THREADS, 0) !=
+        hipSuccess)
+        return 1;
     return n > 0 ? n : 1;
 }
 
 template <typename Real>
 int card_blocks_per_cu(bool att, int occ) {
-    if (att)
-        return occ == 4 ? blocks_per_cu_v<Real, true, 4>()
-             : occ == 3 ? blocks_per_cu_v<Real, true, 3>()
-                        : blocks_per_cu_v<Real, true, 2>();
-    return occ == 4 ? blocks_per_cu_v<Real, false, 4>()
-         : occ == 3 ? blocks_per_cu_v<Real, false, 3>()
-                    : blocks_per_cu_v<Real, false, 2>();
+    if (att) {
+        if (occ == 4) return blocks_per_cu_v<Real, true, 4>();
+        if (occ == 3) return blocks_per_cu_v<Real, true, 3>();
+        return blocks_per_cu_v<Real, true, 2>();
+    }
+    if (occ == 4) return blocks_per_cu_v<Real, false, 4>();
+    if (occ == 3) return blocks_per_cu_v<Real, false, 3>();
+    return blocks_per_cu_v<Real, false, 2>();
 }
 
 template int card_blocks_per_cu<float>(bool, int);

@@ -37,6 +37,16 @@ __device__ __forceinline__ cpx<T> ialpha(T alpha, cpx<T> a) {
     return {-alpha * a.im, alpha * a.re};
 }
 
+// Orders the LDS accesses of one wave: every lane's accesses before the
+// call complete before any lane's accesses after it (LDS executes a
+// wave's instructions in order; the fences keep the compiler from moving
+// accesses across).  For data exchanged only within one wave.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Element-mode work item: element index and Fourier order alpha.
 struct EMItem {
     int32_t elem;

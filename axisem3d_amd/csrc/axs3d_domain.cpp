@@ -169,7 +169,7 @@ void DomainBase::setSolidIso3D(int e_begin, int e_end, const double *lam_mu) {
     int64_t s0, ns;
     slice_range(hElemKind, hSliceOff, hElemNr, e_begin, e_end, s0, ns);
     if (ns == 0) return;
-    uploadProp3D(s0, ns, lam_mu);
+    uploadSlices(dProp3D, 2 * NPEM, e_begin, e_end, lam_mu);
     for (int e = e_begin; e < e_end; e++)
         if (hElemKind[e] == 1) hProp3DSet[e] = 1;
 }
@@ -197,7 +197,7 @@ void DomainBase::setAttenuation3D(int e_begin, int e_end,
     int64_t s0, ns;
     slice_range(hElemKind, hSliceOff, hElemNr, e_begin, e_end, s0, ns);
     if (ns == 0) return;
-    uploadAtt3D(s0, ns, dlam_dmu);
+    uploadSlices(dAtt3D, 2 * (mAttCG4 ? 4 : NPEM), e_begin, e_end, dlam_dmu);
     for (int e = e_begin; e < e_end; e++)
         if (hElemKind[e] == 1) hAtt3DSet[e] = 1;
 }
@@ -294,25 +294,47 @@ void DomainBase::finalize() {
     std::stable_sort(list3d.begin(), list3d.end(), [&](int a, int b) {
         return hElemNr[a] > hElemNr[b];
     });
-    // FFT twiddle tables exp(-2 pi i t / Nr) for every 3D Nr; the LDS FFT
-    // takes radices 2..13 and three packed series of Nr per point
+    // FFT twiddle tables exp(-2 pi i t / Nr) and stage plans for every 3D
+    // Nr; the FFT takes radices 4, 2, 3, 5, 7, 11, 13 in that order of
+    // preference
     std::vector<int64_t> twOff;
     std::vector<double> twTab;
+    std::vector<uint32_t> plan;
     if (!list3d.empty()) {
         int nrmax = hElemNr[list3d.front()];
-        if (3 * nrmax > CARD_FFT_CAP)
+        if (nrmax > CARD_NR_CAP)
             throw Error("Domain::finalize || Nr of a 3D element exceeds the "
                         "cardinal kernel's FFT capacity (832).");
         twOff.assign(nrmax + 1, -1);
+        plan.assign((size_t)(nrmax + 1) * CARD_PLAN_W, 0u);
         for (int32_t e : list3d) {
             int nr = hElemNr[e];
             if (twOff[nr] >= 0) continue;
-            int m = nr;
-            for (int r : {2, 3, 5, 7, 11, 13})
-                while (m % r == 0) m /= r;
-            if (m != 1)
-                throw Error("Domain::finalize || Nr of a 3D element has a "
-                            "prime factor above 13 (use FFT lucky numbers).");
+            uint32_t *pl = plan.data() + (size_t)nr * CARD_PLAN_W;
+            int rem = nr, ns = 1, nst = 0;
+            while (rem > 1) {
+                int r = 0;
+                for (int c : {4, 2, 3, 5, 7, 11, 13})
+                    if (rem % c == 0) {
+                        r = c;
+                        break;
+                    }
+                if (r == 0)
+                    throw Error("Domain::finalize || Nr of a 3D element has "
+                                "a prime factor above 13 (use FFT lucky "
+                                "numbers).");
+                if (nst == CARD_MAX_STAGES)
+                    throw Error("Domain::finalize || Too many FFT stages.");
+                pl[1 + 3 * nst] = (uint32_t)r | ((uint32_t)ns << 8);
+                pl[2 + 3 * nst] =
+                    ns == 1 ? 0u
+                            : (uint32_t)((((uint64_t)1 << 32) + ns - 1) / ns);
+                pl[3 + 3 * nst] = (uint32_t)(nr / (ns * r));
+                nst++;
+                ns *= r;
+                rem /= r;
+            }
+            pl[0] = (uint32_t)nst;
             twOff[nr] = (int64_t)twTab.size() / 2;
             for (int t = 0; t < nr; t++) {
                 double ang = -2. * M_PI * (double)t / (double)nr;
@@ -384,6 +406,7 @@ void DomainBase::finalize() {
     if (!list3d.empty()) {
         up(dList3D, list3d.data(), list3d.size() * 4);
         up(dTwOff, twOff.data(), twOff.size() * 8);
+        up(dPlan, plan.data(), plan.size() * 4);
         dWork.alloc(sizeof(int));
         dWork.zero(st);
     }
@@ -567,13 +590,14 @@ public:
         c.prop3d = dProp3D.as<Real>();
         c.twiddle = dTwiddle.as<cpx<Real>>();
         c.tw_off = dTwOff.as<int64_t>();
+        c.plan = dPlan.as<uint32_t>();
         c.scratch = dCardScratch.as<Real>();
         c.nu1max = mNu1Max3D;
         c.nsls = mNsls;
         c.npa = mAttCG4 ? 4 : NPEM;
         c.abg = dAttABG.as<double>();
         c.att3d = dAtt3D.as<Real>();
-        c.memvar = dMemVar.as<double>();
+        c.memvar = dMemVar.as<Real>();
         c.dstress = dDStress.as<Real>();
         return c;
     }
@@ -743,31 +767,40 @@ private:
         (void)hipEventDestroy(b);
     }
 
-    // 3D properties, [slice][2][25] lambda, mu, cast to Real by chunks.
-    void uploadProp3D(int64_t s0, int64_t ns, const double *src) override {
-        if (dProp3D.bytes() == 0)
-            dProp3D.alloc((size_t)mNumSlices * 2 * NPEM * sizeof(Real));
-        uploadCast(dProp3D, (size_t)s0 * 2 * NPEM, (size_t)ns * 2 * NPEM, src);
-    }
-    // 3D attenuation, [slice][2][npa] dlambda, dmu.
-    void uploadAtt3D(int64_t s0, int64_t ns, const double *src) override {
-        const int npa = mAttCG4 ? 4 : NPEM;
-        if (dAtt3D.bytes() == 0)
-            dAtt3D.alloc((size_t)mNumSlices * 2 * npa * sizeof(Real));
-        uploadCast(dAtt3D, (size_t)s0 * 2 * npa, (size_t)ns * 2 * npa, src);
-    }
-    void uploadCast(DeviceBuffer &b, size_t first, size_t n,
-                    const double *src) {
+    // 3D operands, host [slice][width] per element -> device [width][Nr]
+    // per element, cast to Real; uploaded in chunks of whole elements.
+    void uploadSlices(DeviceBuffer &b, int width, int e_begin, int e_end,
+                      const double *src) override {
+        if (b.bytes() == 0)
+            b.alloc((size_t)mNumSlices * width * sizeof(Real));
         const size_t chunk = (size_t)1 << 24;
-        std::vector<Real> tmp(std::min(n, chunk));
-        for (size_t k = 0; k < n; k += chunk) {
-            const size_t m = std::min(chunk, n - k);
-            for (size_t q = 0; q < m; q++) tmp[q] = (Real)src[k + q];
-            hip_check(hipMemcpyAsync(b.as<Real>() + first + k, tmp.data(),
-                                     m * sizeof(Real), hipMemcpyHostToDevice,
-                                     mStream), "Domain::upload3D");
+        std::vector<Real> tmp;
+        tmp.reserve(chunk + (size_t)width * 1024);
+        int64_t first = -1;
+        auto flush = [&]() {
+            if (tmp.empty()) return;
+            hip_check(hipMemcpyAsync(b.as<Real>() + (size_t)first * width,
+                                     tmp.data(), tmp.size() * sizeof(Real),
+                                     hipMemcpyHostToDevice, mStream),
+                      "Domain::upload3D");
             hip_check(hipStreamSynchronize(mStream), "Domain::upload3D");
+            tmp.clear();
+            first = -1;
+        };
+        for (int e = e_begin; e < e_end; e++) {
+            if (hElemKind[e] != 1) continue;
+            const int nr = hElemNr[e];
+            if (first < 0) first = hSliceOff[e];
+            const size_t base = tmp.size();
+            tmp.resize(base + (size_t)nr * width);
+            for (int k = 0; k < nr; k++)
+                for (int c = 0; c < width; c++)
+                    tmp[base + (size_t)c * nr + k] =
+                        (Real)src[(size_t)k * width + c];
+            src += (size_t)nr * width;
+            if (tmp.size() >= chunk) flush();
         }
+        flush();
     }
 
     // Operands cast to Real after their FP64 products, as the reference's
@@ -825,7 +858,7 @@ private:
             if (mNsls > 0) {
                 const int npa = mAttCG4 ? 4 : NPEM;
                 dMemVar.alloc((size_t)mNumSlices * mNsls * 6 * npa *
-                              sizeof(double));
+                              sizeof(Real));
                 dMemVar.zero(st);
                 dDStress.alloc((size_t)mNumSlices * 6 * npa * sizeof(Real));
                 dDStress.zero(st);

@@ -169,11 +169,12 @@ protected:
         dEMList, dGTab, dElemEBOff, dEB, dAdj;
     // device: cardinal path
     DeviceBuffer dElemNr, dSliceOff, dList3D, dWork, dProp3D, dTwiddle,
-        dTwOff, dCardScratch, dAttABG, dAtt3D, dMemVar, dDStress;
-    virtual void uploadProp3D(int64_t slice0, int64_t nslices,
-                              const double *src) = 0;
-    virtual void uploadAtt3D(int64_t slice0, int64_t nslices,
-                             const double *src) = 0;
+        dTwOff, dPlan, dCardScratch, dAttABG, dAtt3D, dMemVar, dDStress;
+    // 3D operands of elements [e_begin, e_end), host layout [slice][width]
+    // in element order; stored per element as [width][Nr] so that
+    // consecutive slices are consecutive in memory.
+    virtual void uploadSlices(DeviceBuffer &b, int width, int e_begin,
+                              int e_end, const double *src) = 0;
     // device: source and receivers
     DeviceBuffer dSrcPattern, dStf, dRecElem, dRecW, dRecPhi, dSeismo;
     DeviceBuffer dFlag;

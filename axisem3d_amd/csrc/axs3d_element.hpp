@@ -149,11 +149,60 @@ __device__ __forceinline__ void quad_xy(cpx<Real> dfds, cpx<Real> dfdz,
     Y = cadd(cmul(dfds, q.dzdxii), cmul(dfdz, q.dsdxii));
 }
 
-// Quadrature at this point from its six stress components: forms X, Y
-// (with the axial L'Hospital corrections of computeQuad_1overS_addTo,
-// :493-497, folded into row 0 of X and Y) and the local 1/s term, exchanges
-// X, Y through LDS and contracts with G.  Contains one __syncthreads: every
-// thread of the block must call it.
+// Quadrature at this point, first half: forms X, Y from the six stress
+// components (with the axial L'Hospital corrections of
+// computeQuad_1overS_addTo, :493-497, folded into row 0 of X and Y), puts
+// them in the LDS tiles sX, sY, and returns the local 1/s term L
+// (GradientQuadrature.hpp:311-318).
+template <typename Real>
+__device__ __forceinline__ void quad6_prep(
+    const cpx<Real> (&sig)[6], Real alpha, bool axial_row0,
+    const PointOps<Real> &q, cpx<Real> (*sX)[NPEM], cpx<Real> (*sY)[NPEM],
+    int ipnt, cpx<Real> (&L)[3]) {
+    cpx<Real> X[3], Y[3];
+    // (dFdS, dFdZ) per output component: (0,4), (5,3), (4,2)
+    quad_xy(sig[0], sig[4], q, X[0], Y[0]);
+    quad_xy(sig[5], sig[3], q, X[1], Y[1]);
+    quad_xy(sig[4], sig[2], q, X[2], Y[2]);
+    cpx<Real> b[3];
+    b[0] = csub(ialpha(alpha, sig[5]), sig[1]);
+    b[1] = cadd(ialpha(alpha, sig[1]), sig[5]);
+    b[2] = ialpha(alpha, sig[3]);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        L[c] = cmul(b[c], -q.invs);
+        if (axial_row0) {
+            X[c] = csub(X[c], cmul(b[c], q.dzdeta));
+            Y[c] = csub(Y[c], cmul(b[c], q.dzdxii));
+        }
+        sX[c][ipnt] = X[c];
+        sY[c][ipnt] = Y[c];
+    }
+}
+
+// Second half, once the element-mode's X, Y tiles are complete:
+// F = G_xi X + Y G^T + L at point (i, j).
+template <typename Real>
+__device__ __forceinline__ void quad6_finish(const cpx<Real> (*sX)[NPEM],
+                                             const cpx<Real> (*sY)[NPEM],
+                                             int i, int j,
+                                             const PointG<Real> &G,
+                                             const cpx<Real> (&L)[3],
+                                             cpx<Real> (&F)[3]) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        cpx<Real> a = {0, 0}, bsum = {0, 0};
+#pragma unroll
+        for (int k = 0; k < NPED; k++) {
+            a = cfma(sX[c][k * NPED + j], G.gxi_row[k], a);
+            bsum = cfma(sY[c][i * NPED + k], G.g_row[k], bsum);
+        }
+        F[c] = cadd(cadd(a, bsum), L[c]);
+    }
+}
+
+// Both halves with a block barrier between them: every thread of the block
+// must call it.
 template <typename Real>
 __device__ __forceinline__ void quad6_point(
     const cpx<Real> (&sig)[6], Real alpha, bool axial_row0,
@@ -161,41 +210,9 @@ __device__ __forceinline__ void quad6_point(
     int ipnt, int i, int j, const PointG<Real> &G, bool active,
     cpx<Real> (&F)[3]) {
     cpx<Real> L[3];
-    if (active) {
-        cpx<Real> X[3], Y[3];
-        // (dFdS, dFdZ) per output component: (0,4), (5,3), (4,2)
-        quad_xy(sig[0], sig[4], q, X[0], Y[0]);
-        quad_xy(sig[5], sig[3], q, X[1], Y[1]);
-        quad_xy(sig[4], sig[2], q, X[2], Y[2]);
-        // lower-order terms (GradientQuadrature.hpp:311-318)
-        cpx<Real> b[3];
-        b[0] = csub(ialpha(alpha, sig[5]), sig[1]);
-        b[1] = cadd(ialpha(alpha, sig[1]), sig[5]);
-        b[2] = ialpha(alpha, sig[3]);
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            L[c] = cmul(b[c], -q.invs);
-            if (axial_row0) {
-                X[c] = csub(X[c], cmul(b[c], q.dzdeta));
-                Y[c] = csub(Y[c], cmul(b[c], q.dzdxii));
-            }
-            sX[c][ipnt] = X[c];
-            sY[c][ipnt] = Y[c];
-        }
-    }
+    if (active) quad6_prep(sig, alpha, axial_row0, q, sX, sY, ipnt, L);
     __syncthreads();
-    if (active) {
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            cpx<Real> a = {0, 0}, bsum = {0, 0};
-#pragma unroll
-            for (int k = 0; k < NPED; k++) {
-                a = cfma(sX[c][k * NPED + j], G.gxi_row[k], a);
-                bsum = cfma(sY[c][i * NPED + k], G.g_row[k], bsum);
-            }
-            F[c] = cadd(cadd(a, bsum), L[c]);
-        }
-    }
+    if (active) quad6_finish<Real>(sX, sY, i, j, G, L, F);
 }
 
 }  // namespace axs3d

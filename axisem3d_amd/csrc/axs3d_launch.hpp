@@ -34,9 +34,13 @@ struct StiffArgs {
     const Real *gtab;
 };
 
-// Complex FFT capacity of one LDS buffer of the cardinal kernel: three
-// packed series of length Nr per GLL point, so Nr <= 832.
-constexpr int CARD_FFT_CAP = 2496;
+// Largest element Nr of the cardinal kernel (one complex series of Nr per
+// wave in LDS, two more in registers).
+constexpr int CARD_NR_CAP = 832;
+// FFT plan per Nr: word 0 = number of stages; per stage s, words 1+3s..:
+// radix | (Ns << 8), ceil(2^32 / Ns) (0 for Ns = 1), Nr / (Ns * radix).
+constexpr int CARD_MAX_STAGES = 12;
+constexpr int CARD_PLAN_W = 1 + 3 * CARD_MAX_STAGES;
 
 // Device operands of the cardinal-path (3D material) element kernel.
 template <typename Real>
@@ -47,18 +51,21 @@ struct CardArgs {
     int *work_counter;
     const int32_t *elem_nr;
     const int64_t *slice_off;      // first slice of each element
-    const Real *prop3d;            // [slice][2][25] lambda, mu
+    // Per-slice operands are stored per element as [component][Nr], at
+    // element offset slice_off * (components per slice).
+    const Real *prop3d;            // [2][25] lambda, mu
     const cpx<Real> *twiddle;      // exp(-2 pi i t / Nr), per Nr
     const int64_t *tw_off;         // offset of each Nr's table
+    const uint32_t *plan;          // [Nr][CARD_PLAN_W] FFT stages
     Real *scratch;                 // [slot][150][nu1max] complex
     int nu1max;
     // attenuation (nsls = 0: none)
     int nsls;
     int npa;                       // 4 (CG4) or 25 (FULL)
     const double *abg;             // [3][nsls] alpha, beta, gamma
-    const Real *att3d;             // [slice][2][npa] dlambda, dmu
-    double *memvar;                // [slice][nsls][6][npa]
-    Real *dstress;                 // [slice][6][npa]
+    const Real *att3d;             // [2][npa] dlambda, dmu
+    Real *memvar;                  // [nsls][6][npa]
+    Real *dstress;                 // [6][npa]
 };
 
 // Device point description used by the gather and point kernels.
@@ -76,7 +83,8 @@ struct PointGeom {
 template <typename Real>
 void launch_solid_stiff_fr(hipStream_t st, const StiffArgs<Real> &a);
 
-// occ: 2, 3 or 4 = minimum waves per SIMD of the register allocation
+// occ: 2, 3 or 4 = minimum waves per SIMD of the register allocation;
+// n_slots persistent blocks, each with its scratch slot
 template <typename Real>
 void launch_solid_stiff_cd(hipStream_t st, const CardArgs<Real> &a,
                            int n_slots, int occ);

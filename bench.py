@@ -49,7 +49,27 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    metavar="MIB",
+                    help="only run a plain device copy of MIB MiB three "
+                         "times (reference reading for the FETCH_SIZE and "
+                         "WRITE_SIZE counters) and exit")
     return ap.parse_args()
+
+
+def copy_calibration(mib):
+    """A plain copy kernel of known size: the HBM counters of one launch
+    should read `mib` MiB fetched and `mib` MiB written."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_MiB": mib, "launches": 3}),
+          flush=True)
 
 
 def build_workload(name, nsteps):
@@ -95,7 +115,7 @@ def bytes_per_step(spec, real):
     complex x3) and one assembled stiffness write (Real complex x3) per
     point-mode, GradQuad operands (10 rows), connectivity, material (1D:
     lambda, mu per point; 3D: per slice), attenuation memory variables
-    (FP64, read+write) and memory stress (Real, read+write) per slice.
+    (Real, read+write) and memory stress (Real, read+write) per slice.
     point: stiffness read, u, v, a read+written (FP64), per-point data."""
     nupd = spec.n_updates_per_step()
     E, U = spec.n_solid_elements, spec.n_solid_points
@@ -108,7 +128,7 @@ def bytes_per_step(spec, real):
     stiff += int((~is3d).sum()) * 2 * 25 * real + slices * 2 * 25 * real
     if att is not None:
         npa = 4 if att["cg4"] else 25
-        stiff += slices * (att["nsls"] * 6 * npa * 8 * 2 +
+        stiff += slices * (att["nsls"] * 6 * npa * real * 2 +
                            6 * npa * real * 2 + 2 * npa * real)
     point = nupd * (6 * real + 6 * 48 + 4) + U * (4 + 4 + 4 + 1 + 8 + 16)
     return {"stiffness": int(stiff), "point": int(point)}
@@ -119,8 +139,9 @@ def time_gpu(spec, prec, steps, warmup):
     from axisem3d_amd.solver import GpuSolver
     t0 = time.perf_counter()
     g = GpuSolver(spec, prec, max_samples=steps + warmup)
+    tune = g.autotune_info()
     log(f"{prec}: set-up {time.perf_counter() - t0:.1f} s, "
-        f"{g.nmodes} point-modes")
+        f"{g.nmodes} point-modes, autotune {tune}")
     g.run(warmup)
     g.synchronize()
     g.profile(True)
@@ -134,6 +155,7 @@ def time_gpu(spec, prec, steps, warmup):
     seis = g.seismograms(steps + warmup)
     g.close()
     log(f"{prec}: {steps} steps in {t1 - t0:.3f} s")
+    prof["autotune"] = tune
     return (t1 - t0), prof, seis
 
 
@@ -200,6 +222,9 @@ def main():
               "not supported (every GPU box here has one GPU).",
               file=sys.stderr)
         sys.exit(2)
+    if a.copy_calibration > 0:
+        copy_calibration(a.copy_calibration)
+        return
     from axisem3d_amd import capi
     nsteps = a.steps + a.warmup
     t0 = time.perf_counter()
@@ -240,6 +265,9 @@ def main():
         kernels[names[k]] = {"avg_us": avg * 1e3, "launches": cnt,
                              "bytes_per_launch": byts[k],
                              "achieved_GBs": byts[k] / (avg * 1e-3) / 1e9}
+    tune32 = prof32.pop("autotune")
+    if out_64 is not None:
+        prof64.pop("autotune")
     dom = max((k for k in prof32 if prof32[k][1] > 0),
               key=lambda k: prof32[k][0])
     avg_ms = prof32[dom][0] / prof32[dom][1]
@@ -263,6 +291,7 @@ def main():
                           "frac": step_bytes / (t32 / a.steps) / 1e9 /
                           HBM_PEAK_GBS},
         "kernels": kernels,
+        "autotune": tune32,
     }
     if not a.no_cpu:
         t0 = time.perf_counter()

@@ -264,3 +264,71 @@ def test_receiver_synthesis_inverts_r2c():
             displ[off:off + nr // 2 + 1] = coef
             got = o2.record(displ)[0]
             assert np.abs(got - f[k]).max() < 1e-12
+
+
+# --------------------------------------------------------------- attenuation
+@pytest.mark.parametrize("cg4", [True, False])
+def test_sls_memory_variables_follow_the_relaxation_ode(cg4):
+    """Each SLS memory variable solves m' = -w (m - y~ dsig(t)), y~ =
+    y / sum(y), with dsig = iso(dlambda, dmu)(strain).  For a strain ramp
+    eps(t) = E t the exact solution is m(t) = y~ D (t - (1 - e^{-w t}) / w)
+    with D = iso(dlambda, dmu)(E), and the (alpha, beta, gamma) recursion
+    integrates piecewise-linear input exactly.  The stress returned at step
+    n has the memory of step n-1 subtracted; points outside the CG4 set
+    carry no memory.  A swapped beta/gamma, the new dsig used in place of
+    the old one, or a memory added instead of subtracted fails."""
+    spec = preloop.spherical_regional(
+        ntheta=2, nrad=2, theta_max_deg=1.0, depth=20e3, nu_min=2, nu_max=3,
+        nsteps=1, n_receivers=1, attenuation=True, cg4=cg4, corr_len=10e3)
+    spec._build_3d()
+    preloop.finish_regional(spec, 1, src_depth=10e3, f_peak=0.5,
+                            n_receivers=1, theta_max_deg=1.0)
+    o = oracle.Oracle(spec)
+    att = spec.att
+    w, y = att["w"], att["y"]
+    dt = spec.dt
+    e, sl = 1, 1
+    npa = 4 if cg4 else 25
+    pts = list(preloop.CG4_POINTS) if cg4 else list(range(25))
+    s0 = int(spec.elem_slice_off[e]) + sl
+    dl = att["d3d"][s0, 0]
+    dm = att["d3d"][s0, 1]
+    rng = np.random.default_rng(3)
+    E = rng.standard_normal((6, 25)) * 1e-6
+    D = np.zeros((6, npa))
+    for q, p in enumerate(pts):
+        tr = (E[0, p] + E[1, p] + E[2, p]) * dl[q]
+        D[:3, q] = tr + 2 * dm[q] * E[:3, p]
+        D[3:, q] = dm[q] * E[3:, p]
+    yt = y / y.sum()
+    for n in range(0, 400):
+        got = o.att_apply_cd(e, sl, E * (n * dt), np.zeros((6, 25)))
+        t = (n - 1) * dt
+        if n == 0:
+            msum = 0.0
+        else:
+            msum = sum(yt[l] * (t - (1 - math.exp(-w[l] * t)) / w[l])
+                       for l in range(len(w)))
+        want = np.zeros((6, 25))
+        for q, p in enumerate(pts):
+            want[:, p] = -D[:, q] * msum
+        scale = np.abs(D).max() * max(msum, 1e-30)
+        assert np.abs(got - want).max() <= 1e-10 * scale + 1e-300, n
+
+
+def test_unrelaxed_correction_vanishes_without_attenuation():
+    """Q -> infinity: the memory moduli vanish and the elastic moduli are
+    returned unchanged (AttBuilder::createAttenuation limits)."""
+    rng = np.random.default_rng(5)
+    lam = rng.uniform(3e10, 5e10, (3, 25))
+    mu = rng.uniform(2e10, 4e10, (3, 25))
+    w, y = preloop.att_sls_fit(5, 0.005, 0.5)
+    consts = preloop.att_constants(w, y, 0.005, 0.5, 1.0)
+    q = np.full((3, 25), 1e15)
+    wcg4 = rng.uniform(1, 4, (3, 4))
+    for cg4 in (True, False):
+        lc, mc, dl, dm = preloop.att_isotropic(lam, mu, q, q, consts, cg4,
+                                               wcg4)
+        assert np.allclose(lc, lam, rtol=1e-12)
+        assert np.allclose(mc, mu, rtol=1e-12)
+        assert np.abs(dl).max() < 1e-3 and np.abs(dm).max() < 1e-3
