@@ -111,6 +111,12 @@ int axs3d_set_attenuation3d(axs3d_ctx *ctx, int e_begin, int e_end,
     });
 }
 
+int axs3d_set_attenuation1d(axs3d_ctx *ctx, const double *dlam_dmu) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) {
+        d.setAttenuation1D(dlam_dmu);
+    });
+}
+
 int axs3d_set_axial_points(axs3d_ctx *ctx, int n, const int32_t *points) {
     return guarded(ctx,
                    [&](axs3d::DomainBase &d) { d.setAxialPoints(n, points); });

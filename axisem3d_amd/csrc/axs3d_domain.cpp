@@ -202,6 +202,15 @@ void DomainBase::setAttenuation3D(int e_begin, int e_end,
         if (hElemKind[e] == 1) hAtt3DSet[e] = 1;
 }
 
+void DomainBase::setAttenuation1D(const double *dlam_dmu) {
+    if (mNsls <= 0)
+        throw Error("Domain::setAttenuation1D || Set the SLS first.");
+    if (mNumElems <= 0)
+        throw Error("Domain::setAttenuation1D || Set elements first.");
+    hAtt1D.assign(dlam_dmu,
+                  dlam_dmu + (size_t)mNumElems * 2 * (mAttCG4 ? 4 : NPEM));
+}
+
 void DomainBase::setAxialPoints(int n, const int32_t *points) {
     hAxial.assign(points, points + std::max(n, 0));
 }
@@ -242,9 +251,9 @@ void DomainBase::finalize() {
         if (mNsls > 0 && !hAtt3DSet[e])
             throw Error("Domain::finalize || 3D attenuation is not set.");
     }
-    if (mNsls > 0 && has1D)
-        throw Error("Domain::finalize || Attenuation in Fourier-path (1D) "
-                    "elements is not supported by this build.");
+    if (mNsls > 0 && has1D &&
+        hAtt1D.size() != (size_t)mNumElems * 2 * (mAttCG4 ? 4 : NPEM))
+        throw Error("Domain::finalize || 1D attenuation is not set.");
     // points: mode offsets and the mode -> point map
     std::vector<uint32_t> off(mNumPoints + 1, 0);
     std::vector<int32_t> nu1(mNumPoints);
@@ -577,6 +586,12 @@ public:
         a.elem_nu1 = dElemNu1.as<int32_t>();
         a.eb = dEB.as<Real>();
         a.gtab = dGTab.as<Real>();
+        a.nsls = (mNsls > 0 && mNumEM > 0) ? mNsls : 0;
+        a.npa = mAttCG4 ? 4 : NPEM;
+        a.abg = dAttABG.as<double>();
+        a.att1d = dAtt1D.as<Real>();
+        a.mv_fr = dMemVarFR.as<Real>();
+        a.ds_fr = dDStressFR.as<Real>();
         return a;
     }
     CardArgs<Real> cardArgs() const {
@@ -836,6 +851,17 @@ private:
         dStiff.zero(st);
         dEB.alloc((size_t)mNumEBTriples * 6 * sizeof(Real));
         dEB.zero(st);
+        if (mNsls > 0 && mNumEM > 0) {
+            const int npa = mAttCG4 ? 4 : NPEM;
+            std::vector<Real> a1(hAtt1D.begin(), hAtt1D.end());
+            dAtt1D.alloc(a1.size() * sizeof(Real));
+            dAtt1D.upload(a1.data(), a1.size() * sizeof(Real), st);
+            dMemVarFR.alloc((size_t)mNumEM * mNsls * 6 * npa * 2 *
+                            sizeof(Real));
+            dMemVarFR.zero(st);
+            dDStressFR.alloc((size_t)mNumEM * 6 * npa * 2 * sizeof(Real));
+            dDStressFR.zero(st);
+        }
         if (mNum3D > 0) {
             std::vector<Real> tw(hTwTab.begin(), hTwTab.end());
             dTwiddle.alloc(tw.size() * sizeof(Real));

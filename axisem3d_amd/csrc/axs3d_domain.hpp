@@ -63,6 +63,7 @@ public:
     void setAttenuation(int nsls, int cg4, const double *alpha,
                         const double *beta, const double *gamma);
     void setAttenuation3D(int e_begin, int e_end, const double *dlam_dmu);
+    void setAttenuation1D(const double *dlam_dmu);
     void setAxialPoints(int n, const int32_t *points);
     void setMomentSource(int elem, int nu1, const double *pattern, int nsteps,
                          const double *stf);
@@ -154,6 +155,7 @@ protected:
     // attenuation
     int mNsls = 0, mAttCG4 = 0;
     std::vector<double> hAttABG;
+    std::vector<double> hAtt1D;         // [e][2][npa] dlambda, dmu
     std::vector<double> hTwTab;         // interleaved twiddles, FP64
     void layout3D();                    // element Nr and slice offsets
     int mSrcNu1c = 0;
@@ -169,7 +171,8 @@ protected:
         dEMList, dGTab, dElemEBOff, dEB, dAdj;
     // device: cardinal path
     DeviceBuffer dElemNr, dSliceOff, dList3D, dWork, dProp3D, dTwiddle,
-        dTwOff, dPlan, dCardScratch, dAttABG, dAtt3D, dMemVar, dDStress;
+        dTwOff, dPlan, dCardScratch, dAttABG, dAtt3D, dMemVar, dDStress,
+        dAtt1D, dMemVarFR, dDStressFR;
     // 3D operands of elements [e_begin, e_end), host layout [slice][width]
     // in element order; stored per element as [width][Nr] so that
     // consecutive slices are consecutive in memory.

@@ -29,28 +29,26 @@
 
 namespace axs3d {
 
-template <typename Real>
+// CG4 attenuation points (1,1),(1,3),(3,1),(3,3) (AttBuilder.cpp:141-147).
+__device__ __forceinline__ int att_slot_fr(int p, int npa) {
+    if (npa == NPEM) return p;
+    return p == 6 ? 0 : p == 8 ? 1 : p == 16 ? 2 : p == 18 ? 3 : -1;
+}
+
+template <typename Real, bool ATT>
 __global__ __launch_bounds__(STIFF_BLOCK) void k_solid_stiff_fr(
-    int n_em, const EMItem *__restrict__ em_list,
-    const int32_t *__restrict__ elem_points,
-    const uint8_t *__restrict__ elem_axial,
-    const Real *__restrict__ elem_ops, const Real *__restrict__ elem_mat,
-    const uint32_t *__restrict__ point_off,
-    const int32_t *__restrict__ point_nu1, const double *__restrict__ displ,
-    const uint32_t *__restrict__ elem_eb_off,
-    const int32_t *__restrict__ elem_nu1, Real *__restrict__ eb,
-    const Real *__restrict__ gtab) {
+    StiffArgs<Real> a) {
     __shared__ cpx<Real> sU[EMB][3][NPEM];
     __shared__ cpx<Real> sX[EMB][3][NPEM];
     __shared__ cpx<Real> sY[EMB][3][NPEM];
     __shared__ Real sG[2][NPEM];
 
     const int t = threadIdx.x;
-    if (t < 2 * NPEM) sG[t / NPEM][t % NPEM] = gtab[t];
+    if (t < 2 * NPEM) sG[t / NPEM][t % NPEM] = a.gtab[t];
     const int eml = t / NPEM;
     const int ipnt = t - eml * NPEM;
     const int em = blockIdx.x * EMB + eml;
-    const bool active = (eml < EMB) && (em < n_em);
+    const bool active = (eml < EMB) && (em < a.n_em);
     const int i = ipnt / NPED, j = ipnt - (ipnt / NPED) * NPED;
     const int emx = active ? eml : 0;
 
@@ -59,15 +57,15 @@ __global__ __launch_bounds__(STIFF_BLOCK) void k_solid_stiff_fr(
     bool axial = false;
     cpx<Real> u[3] = {{0, 0}, {0, 0}, {0, 0}};
     if (active) {
-        const EMItem it = em_list[em];
+        const EMItem it = a.em_list[em];
         e = it.elem;
         alpha = it.alpha;
-        const int p = elem_points[e * NPEM + ipnt];
-        nu1p = point_nu1[p];
-        off = point_off[p];
-        axial = elem_axial[e] != 0;
+        const int p = a.elem_points[e * NPEM + ipnt];
+        nu1p = a.point_nu1[p];
+        off = a.point_off[p];
+        axial = a.elem_axial[e] != 0;
         if (alpha < nu1p) {
-            const double *src = displ + (size_t)(off + alpha) * 6;
+            const double *src = a.displ + (size_t)(off + alpha) * 6;
 #pragma unroll
             for (int c = 0; c < 3; c++)
                 u[c] = {(Real)src[2 * c], (Real)src[2 * c + 1]};
@@ -84,18 +82,56 @@ __global__ __launch_bounds__(STIFF_BLOCK) void k_solid_stiff_fr(
     const Real ralpha = (Real)alpha;
     const bool axial_row0 = axial && (i == 0);
     if (active) {
-        load_point_ops(elem_ops, e, ipnt, g, q);
+        load_point_ops(a.elem_ops, e, ipnt, g, q);
         cpx<Real> eps[6];
         grad6_point<Real>(sU[eml], u, ralpha, axial_row0, i, j, G, g, eps);
-        const Real *mat = elem_mat + (size_t)e * 2 * NPEM + ipnt;
+        const Real *mat = a.elem_mat + (size_t)e * 2 * NPEM + ipnt;
         iso_hooke<Real>(eps, mat[0], mat[NPEM], sig);
+        if (ATT) {
+            // AttenuationFull/CG4::apply per Fourier order with 1D moduli
+            // (AttenuationCG4.hpp:312-330, Attenuation.hpp:92-140):
+            // sigma -= sum_l m_l; m_l = alpha_l m_l + beta_l ds_old;
+            // ds = iso(dlambda, dmu)(eps); m_l += gamma_l ds
+            const int npa = a.npa, nsls = a.nsls;
+            const int qa = att_slot_fr(ipnt, npa);
+            if (qa >= 0) {
+                const Real *dlm = a.att1d + (size_t)e * 2 * npa + qa;
+                cpx<Real> dsn[6];
+                iso_hooke<Real>(eps, dlm[0], dlm[npa], dsn);
+                cpx<Real> *mv = reinterpret_cast<cpx<Real> *>(a.mv_fr) +
+                                (size_t)em * nsls * 6 * npa + qa;
+                cpx<Real> *ds = reinterpret_cast<cpx<Real> *>(a.ds_fr) +
+                                (size_t)em * 6 * npa + qa;
+                for (int l = 0; l < nsls; l++)
+#pragma unroll
+                    for (int d = 0; d < 6; d++)
+                        sig[d] = csub(sig[d], mv[(l * 6 + d) * npa]);
+                cpx<Real> dso[6];
+#pragma unroll
+                for (int d = 0; d < 6; d++) dso[d] = ds[d * npa];
+                for (int l = 0; l < nsls; l++) {
+                    const Real al = (Real)a.abg[l];
+                    const Real be = (Real)a.abg[nsls + l];
+                    const Real ga = (Real)a.abg[2 * nsls + l];
+#pragma unroll
+                    for (int d = 0; d < 6; d++) {
+                        cpx<Real> &m = mv[(l * 6 + d) * npa];
+                        cpx<Real> x = cmul(m, al);
+                        x = cfma(dso[d], be, x);
+                        m = cfma(dsn[d], ga, x);
+                    }
+                }
+#pragma unroll
+                for (int d = 0; d < 6; d++) ds[d * npa] = dsn[d];
+            }
+        }
     }
     cpx<Real> F[3];
     quad6_point<Real>(sig, ralpha, axial_row0, q, sX[emx], sY[emx], ipnt, i,
                       j, G, active, F);
     if (active && alpha < nu1p) {
-        Real *dst = eb + ((size_t)elem_eb_off[e] +
-                          (size_t)ipnt * elem_nu1[e] + alpha) * 6;
+        Real *dst = a.eb + ((size_t)a.elem_eb_off[e] +
+                            (size_t)ipnt * a.elem_nu1[e] + alpha) * 6;
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             dst[2 * c] = F[c].re;
@@ -379,10 +415,12 @@ template <typename Real>
 void launch_solid_stiff_fr(hipStream_t st, const StiffArgs<Real> &a) {
     if (a.n_em <= 0) return;
     const int grid = (a.n_em + EMB - 1) / EMB;
-    hipLaunchKernelGGL(k_solid_stiff_fr<Real>, dim3(grid), dim3(STIFF_BLOCK),
-                       0, st, a.n_em, a.em_list, a.elem_points, a.elem_axial,
-                       a.elem_ops, a.elem_mat, a.point_off, a.point_nu1,
-                       a.displ, a.elem_eb_off, a.elem_nu1, a.eb, a.gtab);
+    if (a.nsls > 0)
+        hipLaunchKernelGGL((k_solid_stiff_fr<Real, true>), dim3(grid),
+                           dim3(STIFF_BLOCK), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_solid_stiff_fr<Real, false>), dim3(grid),
+                           dim3(STIFF_BLOCK), 0, st, a);
 }
 
 template <typename Real>

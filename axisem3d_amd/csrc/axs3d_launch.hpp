@@ -32,6 +32,14 @@ struct StiffArgs {
     const int32_t *elem_nu1;
     Real *eb;                      // element-local forces
     const Real *gtab;
+    // attenuation of the Fourier-path (1D-material) elements, per element
+    // mode em of em_list (nsls = 0: none)
+    int nsls;
+    int npa;                       // 4 (CG4) or 25 (FULL)
+    const double *abg;             // [3][nsls] alpha, beta, gamma
+    const Real *att1d;             // [e][2][npa] dlambda, dmu
+    Real *mv_fr;                   // [em][nsls][6][npa] complex
+    Real *ds_fr;                   // [em][6][npa] complex
 };
 
 // Largest element Nr of the cardinal kernel (one complex series of Nr per

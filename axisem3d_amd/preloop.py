@@ -691,18 +691,25 @@ def att_isotropic(lam, mu, qkp, qmu, consts, cg4, wcg4=None):
     return lam_c, mu, out_d[0], out_d[1]
 
 
-def make_3d_isotropic(spec, field, att=None):
-    """Turn every solid element into a 3D isotropic element: vp and vs of
-    the 1D model scaled by (1 + d(x, y, z)) with d the random field at the
-    GLL points of slice phi_k = 2 pi k / Nr (Model3D.cpp:32,87), rho kept
-    1D.  att: None or dict(nsls, cg4, f_min, f_max, f_ref, qkp, qmu)
-    with qkp, qmu (E, 25).  Numpy version for small domains; the C++
-    generator (synth.py) computes the same arrays for large ones."""
+def make_3d_isotropic(spec, field, att=None, mask3d=None):
+    """Turn the solid elements selected by mask3d (all by default) into 3D
+    isotropic elements: vp and vs of the 1D model scaled by (1 + d(x, y,
+    z)) with d the random field at the GLL points of slice phi_k = 2 pi k /
+    Nr (Model3D.cpp:32,87), rho kept 1D.  The other elements keep their 1D
+    material (Fourier path).  att: None or dict(nsls, cg4, f_min, f_max,
+    f_ref, qkp, qmu) with qkp, qmu (E, 25); 1D elements then get the 1D
+    attenuation (unrelaxed moduli and dlambda, dmu per element).  Numpy
+    version for small domains; the C++ generator (synth.py) computes the
+    same 3D arrays for large ones."""
     E = spec.n_solid_elements
-    nr = spec.elem_nr.astype(np.int64)
+    if mask3d is None:
+        mask3d = np.ones(E, dtype=bool)
+    mask3d = np.asarray(mask3d, dtype=bool)
+    nr = np.where(mask3d, spec.elem_nr.astype(np.int64), 0)
     off = np.zeros(E + 1, dtype=np.int64)
     off[1:] = np.cumsum(nr)
-    lam1, mu1 = spec.elem_iso[:, 0], spec.elem_iso[:, 1]
+    iso = spec.elem_iso.copy()
+    lam1, mu1 = iso[:, 0], iso[:, 1]
     sz = spec._gq["sz"]
     prop = np.empty((off[-1], 2, NPEM))
     if att is not None:
@@ -711,9 +718,21 @@ def make_3d_isotropic(spec, field, att=None):
                                att["f_ref"])
         npa = 4 if att["cg4"] else NPEM
         d3d = np.empty((off[-1], 2, npa))
+        d1d = np.zeros((E, 2, npa))
         wcg4 = cg4_weights(spec._gq["ifact"])
     vmax = np.zeros(E)
     for e in range(E):
+        if not mask3d[e]:
+            vmax[e] = spec._vp1d[e].max()
+            if att is not None:
+                lc, mc, dl, dm = att_isotropic(
+                    lam1[e].copy(), mu1[e].copy(), att["qkp"][e],
+                    att["qmu"][e], consts, att["cg4"], wcg4[e])
+                spec.elem_iso[e, 0] = lc
+                spec.elem_iso[e, 1] = mc
+                d1d[e, 0] = dl
+                d1d[e, 1] = dm
+            continue
         phi = 2 * math.pi * np.arange(nr[e]) / nr[e]
         s, z = sz[e, 0], sz[e, 1]
         x = s[None, :] * np.cos(phi)[:, None]
@@ -733,14 +752,15 @@ def make_3d_isotropic(spec, field, att=None):
             d3d[off[e]:off[e + 1], 1] = dm
         prop[off[e]:off[e + 1], 0] = lam
         prop[off[e]:off[e + 1], 1] = mu
-    spec.elem_kind = np.ones(E, dtype=np.uint8)
+    spec.elem_kind = mask3d.astype(np.uint8)
     spec.elem_slice_off = off[:-1].copy()
     spec.prop3d = prop
     # dt from the 3D velocities (max over slices, Quad.cpp:306-339)
     spec.dt = compute_dt(sz, vmax, spec._courant)
     if att is not None:
         spec.att = {"nsls": att["nsls"], "cg4": int(att["cg4"]),
-                    "w": w, "y": y, "d3d": d3d, "d1d": None,
+                    "w": w, "y": y, "d3d": d3d,
+                    "d1d": d1d if not mask3d.all() else None,
                     "f_min": att["f_min"], "f_max": att["f_max"],
                     "f_ref": att["f_ref"]}
         a, b, g = att_alpha_beta_gamma(w, y, spec.dt)
@@ -815,7 +835,8 @@ def spherical_regional(ntheta=556, nrad=216, theta_max_deg=30.0,
                    qkp=np.full_like(qmu, 57823.0), qmu=qmu)
     spec._field = field
     spec._att_in = att
-    spec._build_3d = lambda: make_3d_isotropic(spec, field, att)
+    spec._build_3d = lambda mask3d=None: make_3d_isotropic(spec, field, att,
+                                                           mask3d)
     return spec
 
 

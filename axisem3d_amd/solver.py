@@ -64,23 +64,28 @@ class GpuSolver:
             iso = _c(s.elem_iso, np.float64)
             self._chk(L.axs3d_set_solid_iso1d(ctx, iso.ctypes.data),
                       "set_iso1d")
+        att = getattr(s, "att", None)
         if kind is not None:
             kd = _c(kind, np.uint8)
             self._chk(L.axs3d_set_element_kinds(ctx, kd.ctypes.data),
                       "set_kinds")
-            att = getattr(s, "att", None)
-            if att is not None:
-                a, b, g = (_c(att[k], np.float64)
-                           for k in ("alpha", "beta", "gamma"))
-                self._chk(L.axs3d_set_attenuation(
-                    ctx, int(att["nsls"]), int(att["cg4"]), a.ctypes.data,
-                    b.ctypes.data, g.ctypes.data), "set_attenuation")
+        if att is not None:
+            a, b, g = (_c(att[k], np.float64)
+                       for k in ("alpha", "beta", "gamma"))
+            self._chk(L.axs3d_set_attenuation(
+                ctx, int(att["nsls"]), int(att["cg4"]), a.ctypes.data,
+                b.ctypes.data, g.ctypes.data), "set_attenuation")
+            if att.get("d1d") is not None:
+                d1 = _c(att["d1d"], np.float64)
+                self._chk(L.axs3d_set_attenuation1d(ctx, d1.ctypes.data),
+                          "set_attenuation1d")
+        if kind is not None:
             if getattr(s, "prop3d", None) is not None:
                 p3 = _c(s.prop3d, np.float64)
                 self._chk(L.axs3d_set_solid_iso3d(ctx, 0, ep.shape[0],
                                                   p3.ctypes.data),
                           "set_iso3d")
-                if att is not None:
+                if att is not None and att.get("d3d") is not None:
                     d3 = _c(att["d3d"], np.float64)
                     self._chk(L.axs3d_set_attenuation3d(
                         ctx, 0, ep.shape[0], d3.ctypes.data),

@@ -125,6 +125,14 @@ int axs3d_set_attenuation(axs3d_ctx *ctx, int nsls, int cg4,
 int axs3d_set_attenuation3d(axs3d_ctx *ctx, int e_begin, int e_end,
                             const double *dlam_dmu);
 
+/* dlambda, dmu of the attenuation of the 1D (Fourier-path) elements:
+ * [n_elem][2][npa] in element order (entries of 3D elements are ignored),
+ * npa as above, CG4 values already scaled by the CG4 weights.  Replaces
+ * the AttenuationCG4/Full 1D constructors built by
+ * AttBuilder::createAttenuation (AttBuilder.cpp:103-178;
+ * AttenuationCG4.hpp:40-47). */
+int axs3d_set_attenuation1d(axs3d_ctx *ctx, const double *dlam_dmu);
+
 /* Solid points on the axis.  Replaces AxialBoundary::addPoint
  * (src/core/boundary/axial/AxialBoundary.hpp). */
 int axs3d_set_axial_points(axs3d_ctx *ctx, int n, const int32_t *points);

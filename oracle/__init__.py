@@ -52,6 +52,8 @@ class OracleDomain(C.Structure):
         ("n_rec", C.c_int), ("rec_elem", _p), ("rec_weights", _p),
         ("rec_phi", _p),
         ("dt", C.c_double),
+        ("elem_mtype", _p), ("mat1d", _p), ("mat1d_off", _p),
+        ("prop_off", _p), ("elem_theta", _p),
     ]
 
 
@@ -78,6 +80,11 @@ def _setup(L):
     L.oracle_att_free.argtypes = [_p, C.c_int]
     L.oracle_att_apply_cd.argtypes = [C.POINTER(OracleDomain), C.c_int,
                                       C.c_int, _p, _p]
+    L.oracle_att_apply_fr.argtypes = [C.POINTER(OracleDomain), C.c_int,
+                                      C.c_int, _p, _p]
+    L.oracle_spz_to_rtz6.argtypes = [_p, _p]
+    L.oracle_material_stress.argtypes = [C.c_int, _p, _p, _p]
+    L.oracle_rtz_to_spz6.argtypes = [_p, _p]
     from axisem3d_amd import spectral
     t = spectral.tables()
     g_gll = np.ascontiguousarray(t[4])
@@ -118,6 +125,34 @@ def iso_stress(lam, mu, eps):
     lib().oracle_iso_stress(lam.ctypes.data, mu.ctypes.data, eps.ctypes.data,
                             out.ctypes.data)
     return out
+
+
+def material_stress(mt, par, eps):
+    """Stress (6, 25) of material type mt (0 isotropic [lambda, mu], 1 TI
+    [A, C, F, L, N], 2 anisotropic [21 Cij]); par (np, 25)."""
+    eps = np.ascontiguousarray(eps, dtype=np.complex128)
+    par = np.ascontiguousarray(par, dtype=np.float64)
+    out = np.zeros((6, 25), dtype=np.complex128)
+    lib().oracle_material_stress(int(mt), par.ctypes.data, eps.ctypes.data,
+                                 out.ctypes.data)
+    return out
+
+
+def spz_to_rtz6(theta, eps):
+    """Voigt strain (engineering shears) (6, 25) from (s, phi, z) to
+    (R, T, Z) at colatitudes theta (25,)."""
+    e = np.ascontiguousarray(eps, dtype=np.complex128).copy()
+    th = np.ascontiguousarray(theta, dtype=np.float64)
+    lib().oracle_spz_to_rtz6(th.ctypes.data, e.ctypes.data)
+    return e
+
+
+def rtz_to_spz6(theta, sig):
+    """Voigt stress (6, 25) from (R, T, Z) back to (s, phi, z)."""
+    s = np.ascontiguousarray(sig, dtype=np.complex128).copy()
+    th = np.ascontiguousarray(theta, dtype=np.float64)
+    lib().oracle_rtz_to_spz6(th.ctypes.data, s.ctypes.data)
+    return s
 
 
 def r2c(x):
@@ -191,6 +226,20 @@ class Oracle:
                 spec.elem_slice_off, dtype=np.int64))
             d.prop3d = self._k(np.ascontiguousarray(spec.prop3d,
                                                     dtype=np.float64))
+        mtype = getattr(spec, "elem_mtype", None)
+        if mtype is not None:
+            d.elem_mtype = self._k(np.ascontiguousarray(mtype, dtype=np.uint8))
+            d.mat1d = self._k(np.ascontiguousarray(spec.mat1d,
+                                                   dtype=np.float64))
+            d.mat1d_off = self._k(np.ascontiguousarray(spec.mat1d_off,
+                                                       dtype=np.int64))
+            if getattr(spec, "prop_off", None) is not None:
+                d.prop_off = self._k(np.ascontiguousarray(spec.prop_off,
+                                                          dtype=np.int64))
+        theta = getattr(spec, "elem_theta", None)
+        if theta is not None:
+            d.elem_theta = self._k(np.ascontiguousarray(theta,
+                                                        dtype=np.float64))
         att = getattr(spec, "att", None)
         if att is not None:
             d.nsls = int(att["nsls"])
@@ -289,6 +338,15 @@ class Oracle:
         lib().oracle_record(C.byref(self.d), displ.ctypes.data,
                             out.ctypes.data)
         return out[:self.d.n_rec]
+
+    def att_apply_fr(self, e, alpha, strain, stress):
+        """Attenuation of one Fourier order of a 1D element (strain,
+        stress: (6, 25) complex)."""
+        strain = np.ascontiguousarray(strain, dtype=np.complex128)
+        stress = np.ascontiguousarray(stress, dtype=np.complex128).copy()
+        lib().oracle_att_apply_fr(C.byref(self.d), int(e), int(alpha),
+                                  strain.ctypes.data, stress.ctypes.data)
+        return stress
 
     def att_apply_cd(self, e, slice_, strain, stress):
         """Attenuation of one slice (strain, stress: (6, 25) real)."""

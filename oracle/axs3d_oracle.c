@@ -227,6 +227,145 @@ void oracle_iso_stress(const double *lam, const double *mu, const cplx *e,
 }
 
 
+/* TransverselyIsotropic::strainToStress, src/core/element/material/
+ * elastic/anisotropy/TransverselyIsotropic.hpp:161-180, RTZ Voigt with
+ * engineering shears; par = [A, C, F, L, N][25]; T = cplx (Fourier) or
+ * double (cardinal).  Blocks 3 and 4 of the stress hold e0 + e1 and
+ * A (e0 + e1) + F e2 first, as in the reference. */
+#define TI_STRESS(e, s, par, p)                                              \
+    do {                                                                     \
+        const double A_ = (par)[0 * N + (p)], C_ = (par)[1 * N + (p)];       \
+        const double F_ = (par)[2 * N + (p)], L_ = (par)[3 * N + (p)];       \
+        const double N_ = (par)[4 * N + (p)], mN2_ = N_ * -2.;               \
+        (s)[3 * N + (p)] = (e)[0 * N + (p)] + (e)[1 * N + (p)];              \
+        (s)[4 * N + (p)] = (s)[3 * N + (p)] * A_ + (e)[2 * N + (p)] * F_;    \
+        (s)[0 * N + (p)] = (s)[4 * N + (p)] + (e)[1 * N + (p)] * mN2_;       \
+        (s)[1 * N + (p)] = (s)[4 * N + (p)] + (e)[0 * N + (p)] * mN2_;       \
+        (s)[2 * N + (p)] = (s)[3 * N + (p)] * F_ + (e)[2 * N + (p)] * C_;    \
+        (s)[3 * N + (p)] = (e)[3 * N + (p)] * L_;                            \
+        (s)[4 * N + (p)] = (e)[4 * N + (p)] * L_;                            \
+        (s)[5 * N + (p)] = (e)[5 * N + (p)] * N_;                            \
+    } while (0)
+
+/* Anisotropic::strainToStress, src/core/element/material/elastic/
+ * anisotropy/Anisotropic.hpp:203-274: s_i = sum_j C_ij e_j in order
+ * j = 0..5; par = the 21 upper-triangle Cij (C11 C12 .. C16 C22 .. C66),
+ * each [25]. */
+static const int kCijIndex[6][6] = {
+    {0, 1, 2, 3, 4, 5},      {1, 6, 7, 8, 9, 10},    {2, 7, 11, 12, 13, 14},
+    {3, 8, 12, 15, 16, 17},  {4, 9, 13, 16, 18, 19}, {5, 10, 14, 17, 19, 20}};
+#define ANISO_STRESS(T, e, s, par, p)                                        \
+    do {                                                                     \
+        for (int i_ = 0; i_ < 6; i_++) {                                     \
+            T acc_ = (e)[0 * N + (p)] * (par)[kCijIndex[i_][0] * N + (p)];   \
+            for (int j_ = 1; j_ < 6; j_++)                                   \
+                acc_ += (e)[j_ * N + (p)] * (par)[kCijIndex[i_][j_] * N + (p)]; \
+            (s)[i_ * N + (p)] = acc_;                                        \
+        }                                                                    \
+    } while (0)
+
+/* Stress of one Fourier order (FR) or one slice (CD) for material type
+ * mt (0 isotropic [lambda, mu], 1 TI [A, C, F, L, N], 2 anisotropic [21
+ * Cij]); par = that type's parameters, each [25]. */
+static void stress_fr(int mt, const double *par, const cplx *e, cplx *s) {
+    if (mt == 0) {
+        oracle_iso_stress(par, par + N, e, s);
+    } else if (mt == 1) {
+        for (int p = 0; p < N; p++) TI_STRESS(e, s, par, p);
+    } else {
+        for (int p = 0; p < N; p++) ANISO_STRESS(cplx, e, s, par, p);
+    }
+}
+
+static void stress_cd(int mt, const double *par, const double *e, double *s) {
+    if (mt == 0) {
+        const double *lam = par, *mu = par + N;
+        for (int p = 0; p < N; p++) {
+            double tr = (e[0 * N + p] + e[1 * N + p] + e[2 * N + p]) * lam[p];
+            double mu2 = mu[p] * 2.;
+            s[0 * N + p] = tr + e[0 * N + p] * mu2;
+            s[1 * N + p] = tr + e[1 * N + p] * mu2;
+            s[2 * N + p] = tr + e[2 * N + p] * mu2;
+            s[3 * N + p] = e[3 * N + p] * mu[p];
+            s[4 * N + p] = e[4 * N + p] * mu[p];
+            s[5 * N + p] = e[5 * N + p] * mu[p];
+        }
+    } else if (mt == 1) {
+        for (int p = 0; p < N; p++) TI_STRESS(e, s, par, p);
+    } else {
+        for (int p = 0; p < N; p++) ANISO_STRESS(double, e, s, par, p);
+    }
+}
+
+int oracle_mat_nparams(int mt) { return mt == 0 ? 2 : (mt == 1 ? 5 : 21); }
+
+/* Test hook: stress of one Fourier order for material type mt. */
+void oracle_material_stress(int mt, const double *par, const cplx *e,
+                            cplx *s) {
+    stress_fr(mt, par, e, s);
+}
+
+/* CoordTransformSpherical::transformSPZ_RTZ6 (strain, engineering shears)
+ * and transformRTZ_SPZ6 (stress), src/core/element/crd/
+ * CoordTransformSpherical.hpp:186-249, per GLL point with its colatitude
+ * theta (Element::createCoordTransform, src/core/element/Element.cpp:
+ * 133-151). */
+void oracle_spz_to_rtz6(const double *theta, cplx *e) {
+    for (int p = 0; p < N; p++) {
+        double s1 = sin(theta[p]), c1 = cos(theta[p]);
+        double s2 = sin(2. * theta[p]), c2 = cos(2. * theta[p]);
+        cplx s02 = e[0 * N + p] + e[2 * N + p];
+        cplx d02 = e[0 * N + p] - e[2 * N + p];
+        e[0 * N + p] = (s02 + d02 * c2 - e[4 * N + p] * s2) * .5;
+        e[2 * N + p] = s02 - e[0 * N + p];
+        e[4 * N + p] = e[4 * N + p] * c2 + d02 * s2;
+        cplx e3 = e[3 * N + p];
+        e[3 * N + p] = e3 * c1 + e[5 * N + p] * s1;
+        e[5 * N + p] = e[5 * N + p] * c1 - e3 * s1;
+    }
+}
+
+void oracle_rtz_to_spz6(const double *theta, cplx *s) {
+    for (int p = 0; p < N; p++) {
+        double s1 = sin(theta[p]), c1 = cos(theta[p]);
+        double s2 = sin(2. * theta[p]), c2 = cos(2. * theta[p]);
+        cplx s02 = s[0 * N + p] + s[2 * N + p];
+        cplx d02 = s[0 * N + p] - s[2 * N + p];
+        s[0 * N + p] = (s02 + d02 * c2) * .5 + s[4 * N + p] * s2;
+        s[2 * N + p] = s02 - s[0 * N + p];
+        s[4 * N + p] = s[4 * N + p] * c2 - d02 * s2 * .5;
+        cplx s3 = s[3 * N + p];
+        s[3 * N + p] = s3 * c1 - s[5 * N + p] * s1;
+        s[5 * N + p] = s[5 * N + p] * c1 + s3 * s1;
+    }
+}
+
+/* Material type of element e (Material::createElastic, src/preloop/
+ * se_model/quad/physical/Material.cpp:366-379). */
+static int elem_mt(const oracle_domain *d, int e) {
+    return d->elem_mtype ? d->elem_mtype[e] : 0;
+}
+
+/* Elastic::inRTZ (TI and anisotropic) in a spherical mesh. */
+static int elem_rtz(const oracle_domain *d, int e) {
+    return d->elem_theta && elem_mt(d, e) != 0;
+}
+
+/* 1D parameters of element e. */
+static const double *elem_par1d(const oracle_domain *d, int e) {
+    if (d->elem_mtype) return d->mat1d + d->mat1d_off[e];
+    return d->elem_iso + (int64_t)e * 2 * N;
+}
+
+/* 3D parameters of slice k of element e, [np][25]. */
+static const double *elem_par3d(const oracle_domain *d, int e, int k) {
+    int mt = elem_mt(d, e);
+    if (d->elem_mtype)
+        return d->prop3d + d->prop_off[e] +
+               (int64_t)k * oracle_mat_nparams(mt) * N;
+    return d->prop3d + (d->elem_slice_off[e] + k) * 2 * N;
+}
+
 /* ------------------------------------------------------------------ */
 /* FFT (SolverFFTW conventions, src/core/fft/SolverFFTW.hpp:101-168)   */
 /* ------------------------------------------------------------------ */
@@ -432,6 +571,13 @@ static void att_cd(const oracle_domain *d, int e, int slice,
     ATT_ROW(double, strain, stress, dS, mv, dl, dm);
 }
 
+/* Test hook: attenuation of one Fourier order of a 1D element (Fourier
+ * path); strain, stress complex [6][25]. */
+void oracle_att_apply_fr(const oracle_domain *d, int e, int alpha,
+                         const cplx *strain, cplx *stress) {
+    att_fr(d, e, alpha, strain, stress);
+}
+
 /* Test hook: attenuation of one slice of a 3D element (cardinal path). */
 void oracle_att_apply_cd(const oracle_domain *d, int e, int slice,
                          const double *strain, double *stress) {
@@ -454,9 +600,11 @@ static int64_t *make_offsets(int n, const int32_t *nr) {
  * addStiffToPoints (subtract, SolidPoint.hpp:101-109).
  * 1D material (Fourier path): computeGrad6 -> strainToStress_FR per mode
  * (+ attenuation per mode) -> computeQuad6.
- * 3D isotropic (cardinal path): computeGrad6 -> gFFT_N6.computeC2R ->
- * Isotropic::strainToStress<_3D_CD> per slice (+ attenuation per slice) ->
- * gFFT_N6.computeR2C -> computeQuad6. */
+ * 3D material (cardinal path): computeGrad6 -> gFFT_N6.computeC2R ->
+ * strainToStress<_3D_CD> per slice (+ attenuation per slice) ->
+ * gFFT_N6.computeR2C -> computeQuad6.  TI and anisotropic media work in
+ * RTZ on spherical meshes: transformSPZ_RTZ6 after the gradient and
+ * transformRTZ_SPZ6 before the quadrature (SolidElement.cpp:160-196). */
 void oracle_solid_element_stiff(const oracle_domain *d, int e,
                                 const cplx *displ, cplx *stiff) {
     const int32_t *pts = d->elem_points + (int64_t)e * N;
@@ -465,6 +613,7 @@ void oracle_solid_element_stiff(const oracle_domain *d, int e,
     const double *gq_e = d->elem_gq + (int64_t)e * 6 * N;
     int axial = d->elem_axial[e];
     int is3d = elem_is3d(d, e);
+    int mt = elem_mt(d, e), rtz = elem_rtz(d, e);
     cplx *eps = (cplx *)malloc(sizeof(cplx) * nu1 * 6 * N);
     cplx *sig = (cplx *)malloc(sizeof(cplx) * nu1 * 6 * N);
     cplx u[3 * N], f[3 * N];
@@ -477,13 +626,15 @@ void oracle_solid_element_stiff(const oracle_domain *d, int e,
                     ? displ[(d->offsets[ip] + alpha) * 3 + c] : 0.;
         }
         oracle_grad6(gq_e, axial, alpha, u, eps + (int64_t)alpha * 6 * N);
+        if (rtz)
+            oracle_spz_to_rtz6(d->elem_theta + (int64_t)e * N,
+                               eps + (int64_t)alpha * 6 * N);
     }
     if (!is3d) {
-        const double *lam = d->elem_iso + (int64_t)e * 2 * N;
-        const double *mu = lam + N;
+        const double *par = elem_par1d(d, e);
         for (int alpha = 0; alpha < nu1; alpha++) {
-            oracle_iso_stress(lam, mu, eps + (int64_t)alpha * 6 * N,
-                              sig + (int64_t)alpha * 6 * N);
+            stress_fr(mt, par, eps + (int64_t)alpha * 6 * N,
+                      sig + (int64_t)alpha * 6 * N);
             if (d->nsls > 0 && d->att1d)
                 att_fr(d, e, alpha, eps + (int64_t)alpha * 6 * N,
                        sig + (int64_t)alpha * 6 * N);
@@ -499,21 +650,9 @@ void oracle_solid_element_stiff(const oracle_domain *d, int e,
             for (int k = 0; k < nr; k++) ecd[(int64_t)k * 6 * N + col] = rser[k];
         }
         for (int k = 0; k < nr; k++) {
-            const double *lam = d->prop3d + (d->elem_slice_off[e] + k) * 2 * N;
-            const double *mu = lam + N;
             const double *ek = ecd + (int64_t)k * 6 * N;
             double *sk = scd + (int64_t)k * 6 * N;
-            for (int p = 0; p < N; p++) {
-                double tr = (ek[0 * N + p] + ek[1 * N + p] + ek[2 * N + p]) *
-                            lam[p];
-                double mu2 = mu[p] * 2.;
-                sk[0 * N + p] = tr + ek[0 * N + p] * mu2;
-                sk[1 * N + p] = tr + ek[1 * N + p] * mu2;
-                sk[2 * N + p] = tr + ek[2 * N + p] * mu2;
-                sk[3 * N + p] = ek[3 * N + p] * mu[p];
-                sk[4 * N + p] = ek[4 * N + p] * mu[p];
-                sk[5 * N + p] = ek[5 * N + p] * mu[p];
-            }
+            stress_cd(mt, elem_par3d(d, e, k), ek, sk);
             if (d->nsls > 0 && d->att3d) att_cd(d, e, k, ek, sk);
         }
         for (int col = 0; col < 6 * N; col++) {
@@ -527,6 +666,9 @@ void oracle_solid_element_stiff(const oracle_domain *d, int e,
         free(rser);
     }
     for (int alpha = 0; alpha < nu1; alpha++) {
+        if (rtz)
+            oracle_rtz_to_spz6(d->elem_theta + (int64_t)e * N,
+                               sig + (int64_t)alpha * 6 * N);
         oracle_quad6(gq_e, axial, alpha, sig + (int64_t)alpha * 6 * N, f);
         for (int p = 0; p < N; p++) {
             int ip = pts[p];

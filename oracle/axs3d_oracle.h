@@ -48,6 +48,16 @@ typedef struct {
     const double *rec_phi;
     /* time */
     double dt;
+    /* material types (NULL: all isotropic from elem_iso / prop3d by
+     * elem_slice_off); with types, parameters are packed per element:
+     * 1D mat1d + mat1d_off[e] = [np][25], 3D prop3d + prop_off[e] =
+     * [Nr][np][25], np = 2, 5, 21 for isotropic, TI, anisotropic */
+    const uint8_t *elem_mtype;
+    const double *mat1d;
+    const int64_t *mat1d_off;
+    const int64_t *prop_off;
+    const double *elem_theta;     /* [e][25] colatitude (spherical), or
+                                     NULL (Cartesian)                    */
 } oracle_domain;
 
 void oracle_set_gmat(const double *g_gll, const double *g_glj);
@@ -59,6 +69,11 @@ void oracle_quad6_noint(const double *gq_e, int axial, int alpha,
                         const double complex *sij, double complex *fi);
 void oracle_iso_stress(const double *lam, const double *mu,
                        const double complex *e, double complex *s);
+int oracle_mat_nparams(int mt);
+void oracle_material_stress(int mt, const double *par,
+                            const double complex *e, double complex *s);
+void oracle_spz_to_rtz6(const double *theta, double complex *e);
+void oracle_rtz_to_spz6(const double *theta, double complex *s);
 void oracle_solid_element_stiff(const oracle_domain *d, int e,
                                 const double complex *displ,
                                 double complex *stiff);
@@ -79,6 +94,9 @@ void oracle_r2c(const double *r, int nr, double complex *c);
 void oracle_c2r(const double complex *c, int nr, double *r);
 void *oracle_att_alloc(const oracle_domain *d);
 void oracle_att_free(void *state, int n_elem);
+void oracle_att_apply_fr(const oracle_domain *d, int e, int alpha,
+                         const double complex *strain,
+                         double complex *stress);
 void oracle_att_apply_cd(const oracle_domain *d, int e, int slice,
                          const double *strain, double *stress);
 void oracle_free(void *p);

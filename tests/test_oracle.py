@@ -267,49 +267,66 @@ def test_receiver_synthesis_inverts_r2c():
 
 
 # --------------------------------------------------------------- attenuation
+def _att_spec(cg4, mask3d):
+    spec = preloop.spherical_regional(
+        ntheta=2, nrad=2, theta_max_deg=1.0, depth=20e3, nu_min=2, nu_max=3,
+        nsteps=1, n_receivers=1, attenuation=True, cg4=cg4, corr_len=10e3)
+    spec._build_3d(None if mask3d is None else np.asarray(mask3d))
+    preloop.finish_regional(spec, 1, src_depth=10e3, f_peak=0.5,
+                            n_receivers=1, theta_max_deg=1.0)
+    return spec
+
+
 @pytest.mark.parametrize("cg4", [True, False])
-def test_sls_memory_variables_follow_the_relaxation_ode(cg4):
+@pytest.mark.parametrize("path", ["cardinal", "fourier"])
+def test_sls_memory_variables_follow_the_relaxation_ode(cg4, path):
     """Each SLS memory variable solves m' = -w (m - y~ dsig(t)), y~ =
     y / sum(y), with dsig = iso(dlambda, dmu)(strain).  For a strain ramp
     eps(t) = E t the exact solution is m(t) = y~ D (t - (1 - e^{-w t}) / w)
     with D = iso(dlambda, dmu)(E), and the (alpha, beta, gamma) recursion
     integrates piecewise-linear input exactly.  The stress returned at step
     n has the memory of step n-1 subtracted; points outside the CG4 set
-    carry no memory.  A swapped beta/gamma, the new dsig used in place of
-    the old one, or a memory added instead of subtracted fails."""
-    spec = preloop.spherical_regional(
-        ntheta=2, nrad=2, theta_max_deg=1.0, depth=20e3, nu_min=2, nu_max=3,
-        nsteps=1, n_receivers=1, attenuation=True, cg4=cg4, corr_len=10e3)
-    spec._build_3d()
-    preloop.finish_regional(spec, 1, src_depth=10e3, f_peak=0.5,
-                            n_receivers=1, theta_max_deg=1.0)
+    carry no memory.  Cardinal path: one slice of a 3D element, real
+    strain; Fourier path: one order of a 1D element, complex strain.  A
+    swapped beta/gamma, the new dsig used in place of the old one, or a
+    memory added instead of subtracted fails."""
+    e = 1
+    spec = _att_spec(cg4, None if path == "cardinal" else [False] * 4)
     o = oracle.Oracle(spec)
     att = spec.att
     w, y = att["w"], att["y"]
     dt = spec.dt
-    e, sl = 1, 1
     npa = 4 if cg4 else 25
     pts = list(preloop.CG4_POINTS) if cg4 else list(range(25))
-    s0 = int(spec.elem_slice_off[e]) + sl
-    dl = att["d3d"][s0, 0]
-    dm = att["d3d"][s0, 1]
     rng = np.random.default_rng(3)
-    E = rng.standard_normal((6, 25)) * 1e-6
-    D = np.zeros((6, npa))
+    if path == "cardinal":
+        sl = 1
+        s0 = int(spec.elem_slice_off[e]) + sl
+        dl, dm = att["d3d"][s0, 0], att["d3d"][s0, 1]
+        E = rng.standard_normal((6, 25)) * 1e-6
+        apply = lambda eps: o.att_apply_cd(e, sl, eps, np.zeros((6, 25)))
+    else:
+        alpha = 1
+        dl, dm = att["d1d"][e, 0], att["d1d"][e, 1]
+        E = (rng.standard_normal((6, 25)) +
+             1j * rng.standard_normal((6, 25))) * 1e-6
+        apply = lambda eps: o.att_apply_fr(e, alpha, eps,
+                                           np.zeros((6, 25), complex))
+    D = np.zeros((6, npa), dtype=E.dtype)
     for q, p in enumerate(pts):
         tr = (E[0, p] + E[1, p] + E[2, p]) * dl[q]
         D[:3, q] = tr + 2 * dm[q] * E[:3, p]
         D[3:, q] = dm[q] * E[3:, p]
     yt = y / y.sum()
     for n in range(0, 400):
-        got = o.att_apply_cd(e, sl, E * (n * dt), np.zeros((6, 25)))
+        got = apply(E * (n * dt))
         t = (n - 1) * dt
         if n == 0:
             msum = 0.0
         else:
             msum = sum(yt[l] * (t - (1 - math.exp(-w[l] * t)) / w[l])
                        for l in range(len(w)))
-        want = np.zeros((6, 25))
+        want = np.zeros((6, 25), dtype=E.dtype)
         for q, p in enumerate(pts):
             want[:, p] = -D[:, q] * msum
         scale = np.abs(D).max() * max(msum, 1e-30)
@@ -332,3 +349,131 @@ def test_unrelaxed_correction_vanishes_without_attenuation():
         assert np.allclose(lc, lam, rtol=1e-12)
         assert np.allclose(mc, mu, rtol=1e-12)
         assert np.abs(dl).max() < 1e-3 and np.abs(dm).max() < 1e-3
+
+
+# --------------------------------------------------------------- media
+def _voigt_to_tensor(v, engineering):
+    f = 0.5 if engineering else 1.0
+    t = np.empty((3, 3) + v.shape[1:], dtype=v.dtype)
+    t[0, 0], t[1, 1], t[2, 2] = v[0], v[1], v[2]
+    t[1, 2] = t[2, 1] = f * v[3]
+    t[0, 2] = t[2, 0] = f * v[4]
+    t[0, 1] = t[1, 0] = f * v[5]
+    return t
+
+
+def _tensor_to_voigt(t, engineering):
+    f = 2.0 if engineering else 1.0
+    return np.array([t[0, 0], t[1, 1], t[2, 2], f * t[1, 2], f * t[0, 2],
+                     f * t[0, 1]])
+
+
+def test_rtz_rotation_is_the_tensor_rotation():
+    """(s, phi, z) -> (R, T, Z) at colatitude theta: e_R = cos e_s - sin
+    e_z, e_T = e_phi, e_Z = sin e_s + cos e_z (the vector rule of
+    CoordTransformSpherical).  Strain (engineering shears) maps as
+    Q eps Q^T, stress back as Q^T sig Q; the work sig : eps is invariant."""
+    rng = np.random.default_rng(11)
+    theta = rng.uniform(0, np.pi, 25)
+    eps = rng.standard_normal((6, 25)) + 1j * rng.standard_normal((6, 25))
+    sig = rng.standard_normal((6, 25)) + 1j * rng.standard_normal((6, 25))
+    got_e = oracle.spz_to_rtz6(theta, eps)
+    got_s = oracle.rtz_to_spz6(theta, sig)
+    for p in range(25):
+        c, s = np.cos(theta[p]), np.sin(theta[p])
+        Q = np.array([[c, 0, -s], [0, 1, 0], [s, 0, c]])
+        te = _voigt_to_tensor(eps[:, p], True)
+        want_e = _tensor_to_voigt(Q @ te @ Q.T, True)
+        ts = _voigt_to_tensor(sig[:, p], False)
+        want_s = _tensor_to_voigt(Q.T @ ts @ Q, False)
+        assert np.allclose(got_e[:, p], want_e, rtol=0, atol=1e-13)
+        assert np.allclose(got_s[:, p], want_s, rtol=0, atol=1e-13)
+    # work invariance: sig_spz . eps_spz == sig_rtz . eps_rtz
+    back = oracle.rtz_to_spz6(theta, sig)
+    w1 = np.sum(back * eps, axis=0)
+    w2 = np.sum(sig * got_e, axis=0)
+    assert np.allclose(w1, w2, rtol=1e-12, atol=0)
+
+
+def _iso_params(rng):
+    lam = rng.uniform(1, 2, 25)
+    mu = rng.uniform(1, 2, 25)
+    return lam, mu
+
+
+def test_ti_and_anisotropic_reduce_to_isotropic():
+    rng = np.random.default_rng(12)
+    lam, mu = _iso_params(rng)
+    eps = rng.standard_normal((6, 25)) + 1j * rng.standard_normal((6, 25))
+    want = oracle.material_stress(0, np.array([lam, mu]), eps)
+    ti = np.array([lam + 2 * mu, lam + 2 * mu, lam, mu, mu])
+    assert np.allclose(oracle.material_stress(1, ti, eps), want,
+                       rtol=1e-14, atol=0)
+    c = np.zeros((21, 25))
+    for k, v in ((0, lam + 2 * mu), (6, lam + 2 * mu), (11, lam + 2 * mu),
+                 (1, lam), (2, lam), (7, lam), (15, mu), (18, mu), (20, mu)):
+        c[k] = v
+    assert np.allclose(oracle.material_stress(2, c, eps), want,
+                       rtol=1e-14, atol=0)
+
+
+def _bond_z(psi):
+    """Bond matrices of a rotation by psi about the Z axis acting on Voigt
+    strain (engineering shears) and stress."""
+    c, s = np.cos(psi), np.sin(psi)
+    Q = np.array([[c, s, 0], [-s, c, 0], [0, 0, 1.0]])
+    Me = np.zeros((6, 6))
+    Ms = np.zeros((6, 6))
+    for k in range(6):
+        v = np.zeros(6)
+        v[k] = 1
+        Me[:, k] = _tensor_to_voigt(Q @ _voigt_to_tensor(v, True) @ Q.T, True)
+        Ms[:, k] = _tensor_to_voigt(Q @ _voigt_to_tensor(v, False) @ Q.T,
+                                    False)
+    return Me, Ms
+
+
+def test_ti_is_invariant_about_its_symmetry_axis():
+    """A TI medium with symmetry axis Z: rotating the strain about Z rotates
+    the stress the same way (sig(Me eps) = Ms sig(eps)) for any angle; a
+    misplaced modulus (N on the TZ shear, F on the wrong normal) breaks
+    it."""
+    rng = np.random.default_rng(13)
+    A, C, F, L, Nn = rng.uniform(1, 2, (5, 25))
+    par = np.array([A, C, F, L, Nn])
+    eps = rng.standard_normal((6, 25)) + 1j * rng.standard_normal((6, 25))
+    for psi in (0.3, 1.1, 2.5):
+        Me, Ms = _bond_z(psi)
+        lhs = oracle.material_stress(1, par, Me @ eps)
+        rhs = Ms @ oracle.material_stress(1, par, eps)
+        assert np.allclose(lhs, rhs, rtol=1e-12, atol=1e-12)
+    # and it is not isotropic: rotation about another axis breaks it
+    Me, Ms = _bond_z(0.7)
+    P = np.array([[0, 0, 1], [1, 0, 0], [0, 1, 0.0]])   # axis permutation
+    Mp = np.zeros((6, 6))
+    for k in range(6):
+        v = np.zeros(6)
+        v[k] = 1
+        Mp[:, k] = _tensor_to_voigt(P @ _voigt_to_tensor(v, True) @ P.T, True)
+    lhs = oracle.material_stress(1, par, Mp @ eps)
+    assert not np.allclose(lhs, oracle.material_stress(1, par, eps))
+
+
+def test_anisotropic_is_the_symmetric_voigt_matrix():
+    """21 Cij in the reference's order (C11 C12 .. C16 C22 .. C26 C33 ..
+    C66) act as the symmetric 6x6 Voigt matrix; the work eps' sig(eps) is
+    a symmetric bilinear form (eps2 . sig(eps1) = eps1 . sig(eps2))."""
+    rng = np.random.default_rng(14)
+    c21 = rng.standard_normal((21, 25))
+    eps1 = rng.standard_normal((6, 25)) + 1j * rng.standard_normal((6, 25))
+    eps2 = rng.standard_normal((6, 25)) + 1j * rng.standard_normal((6, 25))
+    s1 = oracle.material_stress(2, c21, eps1)
+    s2 = oracle.material_stress(2, c21, eps2)
+    assert np.allclose(np.sum(eps2 * s1, 0), np.sum(eps1 * s2, 0),
+                       rtol=1e-12, atol=1e-12)
+    iu = np.triu_indices(6)
+    for p in range(25):
+        M = np.zeros((6, 6))
+        M[iu] = c21[:, p]
+        M = M + M.T - np.diag(np.diag(M))
+        assert np.allclose(s1[:, p], M @ eps1[:, p], rtol=1e-13, atol=1e-13)
