@@ -38,6 +38,10 @@ SIGNATURES = {
     "axs3d_set_attenuation": (_i, [_p, _i, _i, _p, _p, _p]),
     "axs3d_set_attenuation3d": (_i, [_p, _i, _i, _p]),
     "axs3d_set_attenuation1d": (_i, [_p, _p]),
+    "axs3d_set_material_types": (_i, [_p, _p]),
+    "axs3d_set_solid_material1d": (_i, [_p, _p]),
+    "axs3d_set_solid_material3d": (_i, [_p, _i, _i, _p]),
+    "axs3d_set_rtz_theta": (_i, [_p, _p]),
     "axs3d_set_moment_source": (_i, [_p, _i, _i, _p, _i, _p]),
     "axs3d_set_receivers": (_i, [_p, _i, _p, _p, _p, _i]),
     "axs3d_set_dt": (_i, [_p, _d]),

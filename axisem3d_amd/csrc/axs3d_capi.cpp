@@ -111,6 +111,29 @@ int axs3d_set_attenuation3d(axs3d_ctx *ctx, int e_begin, int e_end,
     });
 }
 
+int axs3d_set_material_types(axs3d_ctx *ctx, const uint8_t *mtype) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) {
+        d.setMaterialTypes(mtype);
+    });
+}
+
+int axs3d_set_solid_material1d(axs3d_ctx *ctx, const double *params) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) {
+        d.setSolidMaterial1D(params);
+    });
+}
+
+int axs3d_set_solid_material3d(axs3d_ctx *ctx, int e_begin, int e_end,
+                               const double *params) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) {
+        d.setSolidMaterial3D(e_begin, e_end, params);
+    });
+}
+
+int axs3d_set_rtz_theta(axs3d_ctx *ctx, const double *theta) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) { d.setRtzTheta(theta); });
+}
+
 int axs3d_set_attenuation1d(axs3d_ctx *ctx, const double *dlam_dmu) {
     return guarded(ctx, [&](axs3d::DomainBase &d) {
         d.setAttenuation1D(dlam_dmu);

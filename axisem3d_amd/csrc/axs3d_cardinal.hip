@@ -241,30 +241,31 @@ __device__ __forceinline__ int att_slot(int p, int npa) {
 }
 
 // Per-slice material of one GLL point: pointers at slice 0 of element e;
-// component c of slice k sits at [c * Nr + k].
+// parameter / component c of slice k sits at [c * Nr + k] (material) or
+// [c * npa * Nr + k] (attenuation).
 template <typename Real>
 struct SliceMat {
-    const Real *lam, *mu;        // 3D lambda, mu of this point
+    const Real *par;             // this point's parameter 0; parameter c at
+    size_t ps;                   // par + c * ps (ps = 25 Nr)
     const Real *dl, *dm;         // memory moduli of its attenuation slot
     Real *mv;                    // memory variables [nsls][6] of the slot
     Real *ds;                    // memory stress [6] of the slot
     size_t cs;                   // component step of mv, ds, dl/dm: npa Nr
+    __device__ Real p(int c, int k) const { return par[c * ps + k]; }
 };
 
-// Stress component d of one slice (Isotropic::strainToStress<_3D_CD>,
-// Isotropic.hpp:155-172: sigma_d = lambda tr + 2 mu eps_d for d < 3, mu
-// eps_d for the engineering shears), with the SLS memory variables of
-// this component when the point carries them (Attenuation::updateMem*,
-// AttenuationFull/CG4::apply: sigma -= sum_l m_l; m_l = alpha_l m_l +
-// beta_l ds_old + gamma_l ds_new; ds_old = ds_new).  Components are
-// independent apart from the trace, so they can be formed in any order.
+// SLS memory of stress component d at slice k, when the point carries it
+// (Attenuation::updateMem*, AttenuationFull/CG4::apply): sigma -= sum_l
+// m_l; m_l = alpha_l m_l + beta_l ds_old + gamma_l ds_new; ds_old =
+// ds_new, with ds = isotropic(dlambda, dmu)(eps) on the strain in the
+// frame the stress is formed in.  Components are independent apart from
+// the trace tr = eps_0 + eps_1 + eps_2.
 template <typename Real, bool ATT>
-__device__ __forceinline__ Real comp_stress(const CardArgs<Real> &a,
-                                            const SliceMat<Real> &M, bool att,
-                                            int d, int k, Real eps, Real tr,
-                                            Real lam, Real mu) {
-    Real sig = d < 3 ? tr * lam + eps * (mu * (Real)2) : eps * mu;
-    if (!ATT || !att) return sig;
+__device__ __forceinline__ void att_comp(const CardArgs<Real> &a,
+                                         const SliceMat<Real> &M, bool att,
+                                         int d, int k, Real eps, Real tr,
+                                         Real &sig) {
+    if (!ATT || !att) return;
     const Real dl = M.dl[k], dm = M.dm[k];
     const Real dsn = d < 3 ? tr * dl + eps * (dm * (Real)2) : eps * dm;
     Real *ds = M.ds + (size_t)d * M.cs + k;
@@ -281,7 +282,6 @@ __device__ __forceinline__ Real comp_stress(const CardArgs<Real> &a,
         m = al * m + be * dso + ga * dsn;
     }
     *ds = dsn;
-    return sig;
 }
 
 // Per-wave LDS tiles of phases A and C (two element-modes), aliased with
@@ -325,9 +325,23 @@ struct AcLane {
 };
 
 
+// sin, cos of theta and 2 theta of GLL point ipnt of element e for the
+// RTZ rotation of TI and anisotropic media on spherical meshes
+// (SolidElement.cpp:177-193), or false.
+template <typename Real, int MT>
+__device__ __forceinline__ bool rtz_of(const Real *rot, int e, int ipnt,
+                                       Real (&r)[4]) {
+    if (MT == 0 || rot == nullptr) return false;
+    const Real *q = rot + (size_t)e * 4 * NPEM + ipnt;
+#pragma unroll
+    for (int c = 0; c < 4; c++) r[c] = q[c * NPEM];
+    return true;
+}
+
 // A: gradient of element e, Fourier orders a0 = 2 wv, 2 wv + 2 CD_WAVES, ...
-// into the scratch rows [d*25 + ipnt][alpha].
-template <typename Real>
+// into the scratch rows [d*25 + ipnt][alpha], rotated to RTZ where the
+// medium works there.
+template <typename Real, int MT>
 __device__ __forceinline__ void cd_phase_grad(const StiffArgs<Real> *s,
                                            cpx<Real> *S, int nu1max,
                                            WaveTiles<Real> *T,
@@ -347,6 +361,8 @@ __device__ __forceinline__ void cd_phase_grad(const StiffArgs<Real> *s,
     }
     PointG<Real> G;
     load_point_g<Real>(Gtab, axial, ln.i, ln.j, G);
+    Real rr[4];
+    const bool rtz = rtz_of<Real, MT>(s->rot, e, ln.ipnt, rr);
     int par = 0;
 #pragma unroll 1
     for (int a0 = 2 * wv; a0 < nu1; a0 += 2 * CD_WAVES, par ^= 1) {
@@ -368,6 +384,9 @@ __device__ __forceinline__ void cd_phase_grad(const StiffArgs<Real> *s,
             cpx<Real> eps[6];
             grad6_point<Real>(T->U[par][ln.hh], u, (Real)alpha, axial_row0, ln.i,
                               ln.j, G, g, eps);
+            if (rtz)
+                rot_spz_rtz6<cpx<Real>, Real>(eps, rr[0], rr[1], rr[2],
+                                              rr[3]);
 #pragma unroll
             for (int d = 0; d < 6; d++)
                 S[(size_t)(d * NPEM + ln.ipnt) * nu1max + alpha] = eps[d];
@@ -376,11 +395,17 @@ __device__ __forceinline__ void cd_phase_grad(const StiffArgs<Real> *s,
     }
 }
 
-// B: the six series of GLL point pp in the wave's two LDS buffers: pair
-// (2,3) into buffer 0, where sigma_pz is formed and eps_zz waits; pair
-// (0,1) into buffer 1, which completes the trace for sigma_ss, sigma_pp
-// and sigma_zz; both go back; then the shear pair (4,5).
-template <typename Real, bool ATT>
+// B: the six series of GLL point pp in the wave's two LDS buffers.
+// Isotropic and TI media (MT 0, 1): pair (2,3) into buffer 0, where the
+// shear stress 3 is formed and eps_2 waits; pair (0,1) into buffer 1,
+// which completes the normal stresses; both go back; then the shear pair
+// (4,5).  Anisotropic media (MT 2) couple all six: pair (0,1) waits in
+// the point's own scratch rows 0-1 (free once packed) while pairs (2,3)
+// and (4,5) are in the buffers.  Isotropic::strainToStress
+// (Isotropic.hpp:155-172), TransverselyIsotropic::strainToStress
+// (TransverselyIsotropic.hpp:161-180), Anisotropic::strainToStress
+// (Anisotropic.hpp:203-274).
+template <typename Real, bool ATT, int MT>
 __device__ __forceinline__ void cd_phase_point(const CardArgs<Real> *a,
                                                cpx<Real> *S, cpx<Real> *b0,
                                                cpx<Real> *b1,
@@ -395,8 +420,8 @@ __device__ __forceinline__ void cd_phase_point(const CardArgs<Real> *a,
 #pragma unroll
     for (int d = 0; d < 6; d++) row[d] = S + (size_t)(d * NPEM + pp) * nu1max;
     SliceMat<Real> M;
-    M.lam = a->prop3d + (size_t)sl0 * 2 * NPEM + (size_t)pp * nr;
-    M.mu = M.lam + (size_t)NPEM * nr;
+    M.ps = (size_t)NPEM * nr;
+    M.par = a->prop3d + a->prop_off[e] + (size_t)pp * nr;
     const int ip4 = ATT ? att_slot(pp, a->npa) : -1;
     const bool att = ATT && ip4 >= 0;
     if (ATT) {
@@ -407,59 +432,125 @@ __device__ __forceinline__ void cd_phase_point(const CardArgs<Real> *a,
         M.mv = a->memvar + (size_t)sl0 * a->nsls * 6 * npa + (size_t)ip4 * nr;
         M.ds = a->dstress + (size_t)sl0 * 6 * npa + (size_t)ip4 * nr;
     }
-    // (eps_zz, eps_pz) -> (eps_zz, sigma_pz) in b0
-    pack_pair<Real>(b0, row[2], row[3], nr, nu1, even, lane);
-    wfft<Real>(b0, nr, plan, true, tw, lane);
+    if constexpr (MT != 2) {
+        // (eps_2, eps_3) -> (eps_2, sigma_3) in b0
+        pack_pair<Real>(b0, row[2], row[3], nr, nu1, even, lane);
+        wfft<Real>(b0, nr, plan, true, tw, lane);
 #pragma unroll 2
-    for (int k = lane; k < nr; k += 64) {
-        const cpx<Real> z = b0[k];
-        b0[k] = {z.re, comp_stress<Real, ATT>(*a, M, att, 3, k, z.im, 0, 0,
-                                              M.mu[k])};
-    }
-    // (eps_ss, eps_pp) in b1; trace; sigma_ss, sigma_pp to b1, sigma_zz to b0
-    pack_pair<Real>(b1, row[0], row[1], nr, nu1, even, lane);
-    wfft<Real>(b1, nr, plan, true, tw, lane);
+        for (int k = lane; k < nr; k += 64) {
+            const cpx<Real> z = b0[k];
+            // sigma_3 = mu eps_3 (isotropic) or L eps_3 (TI)
+            Real s3 = z.im * M.p(MT == 0 ? 1 : 3, k);
+            att_comp<Real, ATT>(*a, M, att, 3, k, z.im, 0, s3);
+            b0[k] = {z.re, s3};
+        }
+        // (eps_0, eps_1) in b1: sigma_0, sigma_1 to b1, sigma_2 to b0
+        pack_pair<Real>(b1, row[0], row[1], nr, nu1, even, lane);
+        wfft<Real>(b1, nr, plan, true, tw, lane);
 #pragma unroll 2
-    for (int k = lane; k < nr; k += 64) {
-        const cpx<Real> z = b1[k];
-        const cpx<Real> y = b0[k];
-        const Real lam = M.lam[k], mu = M.mu[k];
-        const Real tr = z.re + z.im + y.re;
-        const Real s0 = comp_stress<Real, ATT>(*a, M, att, 0, k, z.re, tr,
-                                               lam, mu);
-        const Real s1 = comp_stress<Real, ATT>(*a, M, att, 1, k, z.im, tr,
-                                               lam, mu);
-        const Real s2 = comp_stress<Real, ATT>(*a, M, att, 2, k, y.re, tr,
-                                               lam, mu);
-        b1[k] = {s0, s1};
-        b0[k] = {s2, y.im};
-    }
-    wave_sync();
-    wfft<Real>(b1, nr, plan, false, tw, lane);
-    unpack_pair<Real>(b1, row[0], row[1], nr, nu1, even, lane);
-    wfft<Real>(b0, nr, plan, false, tw, lane);
-    unpack_pair<Real>(b0, row[2], row[3], nr, nu1, even, lane);
-    // shears (eps_sz, eps_sp) in b0
-    pack_pair<Real>(b0, row[4], row[5], nr, nu1, even, lane);
-    wfft<Real>(b0, nr, plan, true, tw, lane);
+        for (int k = lane; k < nr; k += 64) {
+            const cpx<Real> z = b1[k];
+            const cpx<Real> y = b0[k];
+            const Real e0 = z.re, e1 = z.im, e2 = y.re;
+            Real s0, s1, s2;
+            if constexpr (MT == 0) {
+                const Real lam = M.p(0, k), mu = M.p(1, k);
+                const Real mu2 = mu * (Real)2;
+                const Real tl = (e0 + e1 + e2) * lam;
+                s0 = tl + e0 * mu2;
+                s1 = tl + e1 * mu2;
+                s2 = tl + e2 * mu2;
+            } else {
+                const Real A = M.p(0, k), C = M.p(1, k), F = M.p(2, k);
+                const Real mN2 = M.p(4, k) * (Real)-2;
+                const Real t3 = e0 + e1;
+                const Real t4 = t3 * A + e2 * F;
+                s0 = t4 + e1 * mN2;
+                s1 = t4 + e0 * mN2;
+                s2 = t3 * F + e2 * C;
+            }
+            if (ATT && att) {
+                const Real tr = e0 + e1 + e2;
+                att_comp<Real, ATT>(*a, M, att, 0, k, e0, tr, s0);
+                att_comp<Real, ATT>(*a, M, att, 1, k, e1, tr, s1);
+                att_comp<Real, ATT>(*a, M, att, 2, k, e2, tr, s2);
+            }
+            b1[k] = {s0, s1};
+            b0[k] = {s2, y.im};
+        }
+        wave_sync();
+        wfft<Real>(b1, nr, plan, false, tw, lane);
+        unpack_pair<Real>(b1, row[0], row[1], nr, nu1, even, lane);
+        wfft<Real>(b0, nr, plan, false, tw, lane);
+        unpack_pair<Real>(b0, row[2], row[3], nr, nu1, even, lane);
+        // shears (eps_4, eps_5) in b0
+        pack_pair<Real>(b0, row[4], row[5], nr, nu1, even, lane);
+        wfft<Real>(b0, nr, plan, true, tw, lane);
 #pragma unroll 2
-    for (int k = lane; k < nr; k += 64) {
-        const cpx<Real> z = b0[k];
-        const Real mu = M.mu[k];
-        const Real s4 = comp_stress<Real, ATT>(*a, M, att, 4, k, z.re, 0, 0,
-                                               mu);
-        const Real s5 = comp_stress<Real, ATT>(*a, M, att, 5, k, z.im, 0, 0,
-                                               mu);
-        b0[k] = {s4, s5};
+        for (int k = lane; k < nr; k += 64) {
+            const cpx<Real> z = b0[k];
+            Real s4, s5;
+            if constexpr (MT == 0) {
+                const Real mu = M.p(1, k);
+                s4 = z.re * mu;
+                s5 = z.im * mu;
+            } else {
+                s4 = z.re * M.p(3, k);
+                s5 = z.im * M.p(4, k);
+            }
+            att_comp<Real, ATT>(*a, M, att, 4, k, z.re, 0, s4);
+            att_comp<Real, ATT>(*a, M, att, 5, k, z.im, 0, s5);
+            b0[k] = {s4, s5};
+        }
+        wave_sync();
+        wfft<Real>(b0, nr, plan, false, tw, lane);
+        unpack_pair<Real>(b0, row[4], row[5], nr, nu1, even, lane);
+    } else {
+        // (eps_0, eps_1) waits in rows 0-1: slice k at row0[k] (k < nu1)
+        // or row1[k - nu1] (nr - nu1 < nu1 <= nu1max)
+        cpx<Real> *r0 = row[0], *r1 = row[1] - nu1;
+        pack_pair<Real>(b0, row[0], row[1], nr, nu1, even, lane);
+        wfft<Real>(b0, nr, plan, true, tw, lane);
+        for (int k = lane; k < nr; k += 64) (k < nu1 ? r0 : r1)[k] = b0[k];
+        wave_sync();
+        pack_pair<Real>(b1, row[2], row[3], nr, nu1, even, lane);
+        wfft<Real>(b1, nr, plan, true, tw, lane);
+        pack_pair<Real>(b0, row[4], row[5], nr, nu1, even, lane);
+        wfft<Real>(b0, nr, plan, true, tw, lane);
+#pragma unroll 1
+        for (int k = lane; k < nr; k += 64) {
+            cpx<Real> *pk = (k < nu1 ? r0 : r1) + k;
+            const cpx<Real> z01 = *pk, z23 = b1[k], z45 = b0[k];
+            const Real ev[6] = {z01.re, z01.im, z23.re, z23.im, z45.re,
+                                z45.im};
+            Real sv[6];
+            aniso_hooke<Real, Real>(ev, [&](int c) { return M.p(c, k); },
+                                    sv);
+            if (ATT && att) {
+                const Real tr = ev[0] + ev[1] + ev[2];
+#pragma unroll
+                for (int d = 0; d < 6; d++)
+                    att_comp<Real, ATT>(*a, M, att, d, k, ev[d], tr, sv[d]);
+            }
+            *pk = {sv[0], sv[1]};
+            b1[k] = {sv[2], sv[3]};
+            b0[k] = {sv[4], sv[5]};
+        }
+        wave_sync();
+        wfft<Real>(b0, nr, plan, false, tw, lane);
+        unpack_pair<Real>(b0, row[4], row[5], nr, nu1, even, lane);
+        wfft<Real>(b1, nr, plan, false, tw, lane);
+        unpack_pair<Real>(b1, row[2], row[3], nr, nu1, even, lane);
+        for (int k = lane; k < nr; k += 64) b0[k] = (k < nu1 ? r0 : r1)[k];
+        wave_sync();
+        wfft<Real>(b0, nr, plan, false, tw, lane);
+        unpack_pair<Real>(b0, row[0], row[1], nr, nu1, even, lane);
     }
-    wave_sync();
-    wfft<Real>(b0, nr, plan, false, tw, lane);
-    unpack_pair<Real>(b0, row[4], row[5], nr, nu1, even, lane);
 }
 
-// C: quadrature of element e from the scratch stress rows into the
-// element-local force buffer.
-template <typename Real>
+// C: quadrature of element e from the scratch stress rows (rotated back
+// from RTZ where needed) into the element-local force buffer.
+template <typename Real, int MT>
 __device__ __forceinline__ void cd_phase_quad(const StiffArgs<Real> *s,
                                            const cpx<Real> *S, int nu1max,
                                            WaveTiles<Real> *T,
@@ -478,6 +569,8 @@ __device__ __forceinline__ void cd_phase_quad(const StiffArgs<Real> *s,
     PointG<Real> G;
     load_point_g<Real>(Gtab, axial, ln.i, ln.j, G);
     Real *eb = s->eb + (size_t)s->elem_eb_off[e] * 6;
+    Real rr[4];
+    const bool rtz = rtz_of<Real, MT>(s->rot, e, ln.ipnt, rr);
     int par = 0;
 #pragma unroll 1
     for (int a0 = 2 * wv; a0 < nu1; a0 += 2 * CD_WAVES, par ^= 1) {
@@ -489,6 +582,9 @@ __device__ __forceinline__ void cd_phase_quad(const StiffArgs<Real> *s,
 #pragma unroll
             for (int d = 0; d < 6; d++)
                 sig[d] = S[(size_t)(d * NPEM + ln.ipnt) * nu1max + alpha];
+            if (rtz)
+                rot_rtz_spz6<cpx<Real>, Real>(sig, rr[0], rr[1], rr[2],
+                                              rr[3]);
             quad6_prep<Real>(sig, (Real)alpha, axial_row0, q,
                              T->X[par][ln.hh], T->Y[par][ln.hh], ln.ipnt, Lt);
         }
@@ -511,8 +607,9 @@ __device__ __forceinline__ void cd_phase_quad(const StiffArgs<Real> *s,
 }
 
 // OCC = minimum waves per SIMD the register allocation must allow; picked
-// per workload by measurement.
-template <typename Real, bool ATT, int OCC>
+// per workload by measurement.  MT = material type of the elements of
+// a.list3d (0 isotropic, 1 TI, 2 anisotropic).
+template <typename Real, bool ATT, int OCC, int MT>
 __global__ __launch_bounds__(CD_THREADS, OCC) void k_solid_stiff_cd(
     CardArgs<Real> a) {
     __shared__ CdLds<Real> L;
@@ -539,13 +636,13 @@ __global__ __launch_bounds__(CD_THREADS, OCC) void k_solid_stiff_cd(
             const cpx<Real> *twg = a.twiddle + a.tw_off[nr];
             for (int x = t; x < nr; x += CD_THREADS) L.tw[x] = twg[x];
         }
-        cd_phase_grad<Real>(&a.s, S, nu1max, T, L.G, e, nu1, lane, wv);
+        cd_phase_grad<Real, MT>(&a.s, S, nu1max, T, L.G, e, nu1, lane, wv);
         __syncthreads();
         for (int pp = wv; pp < NPEM; pp += CD_WAVES)
-            cd_phase_point<Real, ATT>(&a, S, b0, b1, L.tw, e, nr, nu1, pp,
-                                      lane);
+            cd_phase_point<Real, ATT, MT>(&a, S, b0, b1, L.tw, e, nr, nu1,
+                                          pp, lane);
         __syncthreads();
-        cd_phase_quad<Real>(&a.s, S, nu1max, T, L.G, e, nu1, lane, wv);
+        cd_phase_quad<Real, MT>(&a.s, S, nu1max, T, L.G, e, nu1, lane, wv);
         // every wave is done with this element's scratch and with the
         // work slot before the next element is fetched
         __syncthreads();
@@ -553,45 +650,74 @@ __global__ __launch_bounds__(CD_THREADS, OCC) void k_solid_stiff_cd(
 }
 
 __global__ void k_reset_counter(int *c) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0;
+    if (threadIdx.x < 3 && blockIdx.x == 0) c[threadIdx.x] = 0;
 }
 
-template <typename Real, bool ATT>
-static void launch_cd_att(hipStream_t st, const CardArgs<Real> &a, int n_slots,
-                          int occ) {
-    switch (occ) {
-        case 4:
-            hipLaunchKernelGGL((k_solid_stiff_cd<Real, ATT, 4>),
-                               dim3(n_slots), dim3(CD_THREADS), 0, st, a);
-            break;
-        case 3:
-            hipLaunchKernelGGL((k_solid_stiff_cd<Real, ATT, 3>),
-                               dim3(n_slots), dim3(CD_THREADS), 0, st, a);
-            break;
-        default:
-            hipLaunchKernelGGL((k_solid_stiff_cd<Real, ATT, 2>),
-                               dim3(n_slots), dim3(CD_THREADS), 0, st, a);
-            break;
+// Occupancy variants for isotropic media (the common case); TI and
+// anisotropic media run the occupancy-2 variant.
+template <typename Real, bool ATT, int MT>
+static void launch_cd_mt(hipStream_t st, const CardArgs<Real> &a,
+                         int n_slots, int occ) {
+    if constexpr (MT != 0) {
+        hipLaunchKernelGGL((k_solid_stiff_cd<Real, ATT, 2, MT>),
+                           dim3(n_slots), dim3(CD_THREADS), 0, st, a);
+    } else {
+        switch (occ) {
+            case 4:
+                hipLaunchKernelGGL((k_solid_stiff_cd<Real, ATT, 4, MT>),
+                                   dim3(n_slots), dim3(CD_THREADS), 0, st, a);
+                break;
+            case 3:
+                hipLaunchKernelGGL((k_solid_stiff_cd<Real, ATT, 3, MT>),
+                                   dim3(n_slots), dim3(CD_THREADS), 0, st, a);
+                break;
+            default:
+                hipLaunchKernelGGL((k_solid_stiff_cd<Real, ATT, 2, MT>),
+                                   dim3(n_slots), dim3(CD_THREADS), 0, st, a);
+                break;
+        }
     }
 }
 
+template <typename Real, bool ATT>
+static void launch_cd_att(hipStream_t st, const CardArgs<Real> &a, int mt,
+                          int n_slots, int occ) {
+    if (mt == 0)
+        launch_cd_mt<Real, ATT, 0>(st, a, n_slots, occ);
+    else if (mt == 1)
+        launch_cd_mt<Real, ATT, 1>(st, a, n_slots, occ);
+    else
+        launch_cd_mt<Real, ATT, 2>(st, a, n_slots, occ);
+}
+
+// One persistent launch per material type present; a.list3d holds the 3D
+// elements grouped by type (each group heaviest first) with the group
+// boundaries in a.type_begin, and a.work_counter one counter per type.
 template <typename Real>
 void launch_solid_stiff_cd(hipStream_t st, const CardArgs<Real> &a,
                            int n_slots, int occ) {
     if (a.n3d <= 0) return;
     hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st,
                        a.work_counter);
-    if (a.nsls > 0)
-        launch_cd_att<Real, true>(st, a, n_slots, occ);
-    else
-        launch_cd_att<Real, false>(st, a, n_slots, occ);
+    for (int mt = 0; mt < 3; mt++) {
+        const int b = a.type_begin[mt], n = a.type_begin[mt + 1] - b;
+        if (n <= 0) continue;
+        CardArgs<Real> c = a;
+        c.list3d = a.list3d + b;
+        c.n3d = n;
+        c.work_counter = a.work_counter + mt;
+        if (a.nsls > 0)
+            launch_cd_att<Real, true>(st, c, mt, n_slots, occ);
+        else
+            launch_cd_att<Real, false>(st, c, mt, n_slots, occ);
+    }
 }
 
 template <typename Real, bool ATT, int OCC>
 static int blocks_per_cu_v() {
     int n = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &n, k_solid_stiff_cd<Real, ATT, OCC>, CD_THREADS, 0) !=
+            &n, k_solid_stiff_cd<Real, ATT, OCC, 0>, CD_THREADS, 0) !=
         hipSuccess)
         return 1;
     return n > 0 ? n : 1;

@@ -26,6 +26,18 @@ template <typename T>
 __device__ __forceinline__ cpx<T> csub(cpx<T> a, cpx<T> b) {
     return {a.re - b.re, a.im - b.im};
 }
+template <typename T>
+__device__ __forceinline__ cpx<T> operator+(cpx<T> a, cpx<T> b) {
+    return {a.re + b.re, a.im + b.im};
+}
+template <typename T>
+__device__ __forceinline__ cpx<T> operator-(cpx<T> a, cpx<T> b) {
+    return {a.re - b.re, a.im - b.im};
+}
+template <typename T>
+__device__ __forceinline__ cpx<T> operator*(cpx<T> a, T s) {
+    return {a.re * s, a.im * s};
+}
 // a + b * s
 template <typename T>
 __device__ __forceinline__ cpx<T> cfma(cpx<T> b, T s, cpx<T> a) {

@@ -100,10 +100,51 @@ void DomainBase::setSolidElements(int n, const int32_t *points,
     mNumElems = n;
 }
 
+static int mat_nparams(int mt) { return mt == 0 ? 2 : (mt == 1 ? 5 : 21); }
+
 void DomainBase::setSolidIso1D(const double *lam_mu) {
     if (mNumElems <= 0)
         throw Error("Domain::setSolidIso1D || Set elements first.");
-    hElemIso.assign(lam_mu, lam_mu + (size_t)mNumElems * 2 * NPEM);
+    for (uint8_t t : hMType)
+        if (t != 0)
+            throw Error("Domain::setSolidIso1D || Not all elements are "
+                        "isotropic; use the general 1D material.");
+    hMat1D.assign(lam_mu, lam_mu + (size_t)mNumElems * 2 * NPEM);
+    hMatOff.resize(mNumElems);
+    for (int e = 0; e < mNumElems; e++) hMatOff[e] = (int64_t)e * 2 * NPEM;
+}
+
+// Material::createElastic picks the medium per element (Material.cpp:
+// 366-379): 0 isotropic, 1 transversely isotropic, 2 anisotropic.
+void DomainBase::setMaterialTypes(const uint8_t *mtype) {
+    if (mNumElems <= 0)
+        throw Error("Domain::setMaterialTypes || Set elements first.");
+    if (!hSliceOff.empty() || !hMat1D.empty())
+        throw Error("Domain::setMaterialTypes || Types must be set before "
+                    "the material.");
+    hMType.assign(mtype, mtype + mNumElems);
+    for (uint8_t t : hMType)
+        if (t > 2)
+            throw Error("Domain::setMaterialTypes || Unknown material type.");
+}
+
+void DomainBase::setSolidMaterial1D(const double *params) {
+    if (mNumElems <= 0)
+        throw Error("Domain::setSolidMaterial1D || Set elements first.");
+    if (hMType.empty()) hMType.assign(mNumElems, 0);
+    hMatOff.resize(mNumElems);
+    int64_t off = 0;
+    for (int e = 0; e < mNumElems; e++) {
+        hMatOff[e] = off;
+        off += (int64_t)mat_nparams(hMType[e]) * NPEM;
+    }
+    hMat1D.assign(params, params + off);
+}
+
+void DomainBase::setRtzTheta(const double *theta) {
+    if (mNumElems <= 0)
+        throw Error("Domain::setRtzTheta || Set elements first.");
+    hTheta.assign(theta, theta + (size_t)mNumElems * NPEM);
 }
 
 void DomainBase::setElementKinds(const uint8_t *kind) {
@@ -145,6 +186,15 @@ void DomainBase::layout3D() {
         }
     }
     mNumSlices = sl;
+    if (hMType.empty()) hMType.assign(mNumElems, 0);
+    hPropOff.assign(mNumElems, -1);
+    int64_t po = 0;
+    for (int e = 0; e < mNumElems; e++)
+        if (hElemKind[e] == 1) {
+            hPropOff[e] = po;
+            po += (int64_t)mat_nparams(hMType[e]) * NPEM * hElemNr[e];
+        }
+    mPropTotal = po;
     hProp3DSet.assign(mNumElems, 0);
     hAtt3DSet.assign(mNumElems, 0);
 }
@@ -164,12 +214,23 @@ static void slice_range(const std::vector<uint8_t> &kind,
 
 void DomainBase::setSolidIso3D(int e_begin, int e_end, const double *lam_mu) {
     layout3D();
+    for (int e = std::max(e_begin, 0); e < std::min(e_end, mNumElems); e++)
+        if (hElemKind[e] == 1 && hMType[e] != 0)
+            throw Error("Domain::setSolidIso3D || Element is not isotropic; "
+                        "use the general 3D material.");
+    setSolidMaterial3D(e_begin, e_end, lam_mu);
+}
+
+void DomainBase::setSolidMaterial3D(int e_begin, int e_end,
+                                    const double *params) {
+    layout3D();
     if (e_begin < 0 || e_end > mNumElems || e_begin > e_end)
-        throw Error("Domain::setSolidIso3D || Element range out of bounds.");
+        throw Error("Domain::setSolidMaterial3D || Element range out of "
+                    "bounds.");
     int64_t s0, ns;
     slice_range(hElemKind, hSliceOff, hElemNr, e_begin, e_end, s0, ns);
     if (ns == 0) return;
-    uploadSlices(dProp3D, 2 * NPEM, e_begin, e_end, lam_mu);
+    uploadProp(e_begin, e_end, params);
     for (int e = e_begin; e < e_end; e++)
         if (hElemKind[e] == 1) hProp3DSet[e] = 1;
 }
@@ -242,8 +303,10 @@ void DomainBase::finalize() {
     if (!(mDt > 0.)) throw Error("Domain::finalize || dt is not set.");
     layout3D();
     const bool has1D = mNum3D < mNumElems;
-    if (has1D && hElemIso.size() != (size_t)mNumElems * 2 * NPEM)
+    if (has1D && hMatOff.size() != (size_t)mNumElems)
         throw Error("Domain::finalize || Material is not set.");
+    if (!hTheta.empty() && hTheta.size() != (size_t)mNumElems * NPEM)
+        throw Error("Domain::finalize || Colatitudes are not set.");
     for (int e = 0; e < mNumElems; e++) {
         if (hElemKind[e] != 1) continue;
         if (!hProp3DSet[e])
@@ -288,9 +351,16 @@ void DomainBase::finalize() {
             nrmax = std::max(nrmax, hPointNr[p]);
         }
         eNu1[e] = nrmax / 2 + 1;
-        if (hElemKind[e] != 1)
-            for (int a = 0; a < eNu1[e]; a++) em.push_back({e, a});
     }
+    // Fourier-path element-modes grouped by material type (one launch per
+    // type), in element order within a group
+    for (int mt = 0; mt < 3; mt++) {
+        mEMTypeBegin[mt] = (int)em.size();
+        for (int e = 0; e < mNumElems; e++)
+            if (hElemKind[e] != 1 && hMType[e] == mt)
+                for (int a = 0; a < eNu1[e]; a++) em.push_back({e, a});
+    }
+    mEMTypeBegin[3] = (int)em.size();
     mNumEM = (int)em.size();
     // 3D elements, heaviest (largest Nr) first for the work queue
     std::vector<int32_t> list3d;
@@ -300,9 +370,17 @@ void DomainBase::finalize() {
             list3d.push_back(e);
             mNu1Max3D = std::max(mNu1Max3D, eNu1[e]);
         }
+    // grouped by material type (one launch per type), heaviest first
     std::stable_sort(list3d.begin(), list3d.end(), [&](int a, int b) {
+        if (hMType[a] != hMType[b]) return hMType[a] < hMType[b];
         return hElemNr[a] > hElemNr[b];
     });
+    for (int mt = 0; mt <= 3; mt++) {
+        int n = 0;
+        for (int32_t e : list3d)
+            if (hMType[e] < mt) n++;
+        mTypeBegin3D[mt] = n;
+    }
     // FFT twiddle tables exp(-2 pi i t / Nr) and stage plans for every 3D
     // Nr; the FFT takes radices 4, 2, 3, 5, 7, 11, 13 in that order of
     // preference
@@ -310,7 +388,8 @@ void DomainBase::finalize() {
     std::vector<double> twTab;
     std::vector<uint32_t> plan;
     if (!list3d.empty()) {
-        int nrmax = hElemNr[list3d.front()];
+        int nrmax = 0;
+        for (int32_t e : list3d) nrmax = std::max(nrmax, hElemNr[e]);
         if (nrmax > CARD_NR_CAP)
             throw Error("Domain::finalize || Nr of a 3D element exceeds the "
                         "cardinal kernel's FFT capacity (832).");
@@ -412,11 +491,12 @@ void DomainBase::finalize() {
     up(dAdj, adj.data(), adj.size() * 4);
     up(dElemNr, hElemNr.data(), hElemNr.size() * 4);
     up(dSliceOff, hSliceOff.data(), hSliceOff.size() * 8);
+    up(dPropOff, hPropOff.data(), hPropOff.size() * 8);
     if (!list3d.empty()) {
         up(dList3D, list3d.data(), list3d.size() * 4);
         up(dTwOff, twOff.data(), twOff.size() * 8);
         up(dPlan, plan.data(), plan.size() * 4);
-        dWork.alloc(sizeof(int));
+        dWork.alloc(3 * sizeof(int));
         dWork.zero(st);
     }
     hTwTab.swap(twTab);
@@ -579,6 +659,9 @@ public:
         a.elem_axial = dElemAxial.as<uint8_t>();
         a.elem_ops = dElemOps.as<Real>();
         a.elem_mat = dElemMat.as<Real>();
+        a.mat_off = dMatOff.as<uint32_t>();
+        a.rot = hTheta.empty() ? nullptr : dRot.as<Real>();
+        for (int k = 0; k < 4; k++) a.em_type_begin[k] = mEMTypeBegin[k];
         a.point_off = dPointOff.as<uint32_t>();
         a.point_nu1 = dPointNu1.as<int32_t>();
         a.displ = dDispl.as<double>();
@@ -603,6 +686,8 @@ public:
         c.elem_nr = dElemNr.as<int32_t>();
         c.slice_off = dSliceOff.as<int64_t>();
         c.prop3d = dProp3D.as<Real>();
+        c.prop_off = dPropOff.as<int64_t>();
+        for (int k = 0; k < 4; k++) c.type_begin[k] = mTypeBegin3D[k];
         c.twiddle = dTwiddle.as<cpx<Real>>();
         c.tw_off = dTwOff.as<int64_t>();
         c.plan = dPlan.as<uint32_t>();
@@ -818,6 +903,40 @@ private:
         flush();
     }
 
+    void uploadProp(int e_begin, int e_end, const double *src) override {
+        if (dProp3D.bytes() == 0)
+            dProp3D.alloc((size_t)std::max<int64_t>(mPropTotal, 1) *
+                          sizeof(Real));
+        const size_t chunk = (size_t)1 << 24;
+        std::vector<Real> tmp;
+        int64_t first = -1;
+        auto flush = [&]() {
+            if (tmp.empty()) return;
+            hip_check(hipMemcpyAsync(dProp3D.as<Real>() + first, tmp.data(),
+                                     tmp.size() * sizeof(Real),
+                                     hipMemcpyHostToDevice, mStream),
+                      "Domain::upload3D");
+            hip_check(hipStreamSynchronize(mStream), "Domain::upload3D");
+            tmp.clear();
+            first = -1;
+        };
+        for (int e = e_begin; e < e_end; e++) {
+            if (hElemKind[e] != 1) continue;
+            const int nr = hElemNr[e];
+            const int width = mat_nparams(hMType[e]) * NPEM;
+            if (first < 0) first = hPropOff[e];
+            const size_t base = tmp.size();
+            tmp.resize(base + (size_t)nr * width);
+            for (int k = 0; k < nr; k++)
+                for (int c = 0; c < width; c++)
+                    tmp[base + (size_t)c * nr + k] =
+                        (Real)src[(size_t)k * width + c];
+            src += (size_t)nr * width;
+            if (tmp.size() >= chunk) flush();
+        }
+        flush();
+    }
+
     // Operands cast to Real after their FP64 products, as the reference's
     // GradientQuadrature constructor does (GradientQuadrature.hpp:127-137).
     void uploadReal() override {
@@ -836,10 +955,37 @@ private:
         }
         dElemOps.alloc(ops.size() * sizeof(Real));
         dElemOps.upload(ops.data(), ops.size() * sizeof(Real), st);
-        std::vector<Real> mat(hElemIso.begin(), hElemIso.end());
-        if (mat.empty()) mat.assign((size_t)mNumElems * 2 * NPEM, (Real)0);
+        std::vector<Real> mat(hMat1D.begin(), hMat1D.end());
+        if (mat.empty()) mat.assign(2 * NPEM, (Real)0);
         dElemMat.alloc(mat.size() * sizeof(Real));
         dElemMat.upload(mat.data(), mat.size() * sizeof(Real), st);
+        {
+            std::vector<uint32_t> mo(mNumElems, 0);
+            for (size_t e = 0; e < hMatOff.size(); e++) {
+                if (hMatOff[e] >= (int64_t)UINT32_MAX)
+                    throw Error("Domain::finalize || 1D material too large.");
+                mo[e] = (uint32_t)hMatOff[e];
+            }
+            dMatOff.alloc(mo.size() * 4);
+            dMatOff.upload(mo.data(), mo.size() * 4, st);
+        }
+        if (!hTheta.empty()) {
+            // sin, cos of theta and 2 theta cast after their FP64
+            // evaluation (CoordTransformSpherical constructor,
+            // CoordTransformSpherical.hpp:36-45)
+            std::vector<Real> rot((size_t)mNumElems * 4 * NPEM);
+            for (int e = 0; e < mNumElems; e++)
+                for (int k = 0; k < NPEM; k++) {
+                    const double t = hTheta[(size_t)e * NPEM + k];
+                    Real *r = rot.data() + (size_t)e * 4 * NPEM + k;
+                    r[0] = (Real)std::sin(t);
+                    r[NPEM] = (Real)std::cos(t);
+                    r[2 * NPEM] = (Real)std::sin(2. * t);
+                    r[3 * NPEM] = (Real)std::cos(2. * t);
+                }
+            dRot.alloc(rot.size() * sizeof(Real));
+            dRot.upload(rot.data(), rot.size() * sizeof(Real), st);
+        }
         std::vector<Real> gt(2 * NPEM);
         for (int k = 0; k < NPEM; k++) {
             gt[k] = (Real)hGll[k];

@@ -60,6 +60,10 @@ public:
     void setSolidIso1D(const double *lam_mu);
     void setElementKinds(const uint8_t *kind);
     void setSolidIso3D(int e_begin, int e_end, const double *lam_mu);
+    void setMaterialTypes(const uint8_t *mtype);
+    void setSolidMaterial1D(const double *params);
+    void setSolidMaterial3D(int e_begin, int e_end, const double *params);
+    void setRtzTheta(const double *theta);
     void setAttenuation(int nsls, int cg4, const double *alpha,
                         const double *beta, const double *gamma);
     void setAttenuation3D(int e_begin, int e_end, const double *dlam_dmu);
@@ -130,7 +134,13 @@ protected:
     std::vector<int32_t> hElemPoints;
     std::vector<uint8_t> hElemAxial;
     std::vector<double> hElemGQ;
-    std::vector<double> hElemIso;
+    // 1D material, packed per element: np(type) x 25 at hMatOff[e]
+    std::vector<double> hMat1D;
+    std::vector<int64_t> hMatOff;
+    std::vector<uint8_t> hMType;        // 0 isotropic, 1 TI, 2 anisotropic
+    std::vector<double> hTheta;         // [e][25] colatitude, or empty
+    int mEMTypeBegin[4] = {0, 0, 0, 0};
+    int mTypeBegin3D[4] = {0, 0, 0, 0};
     std::vector<int32_t> hAxial;
     int hSrcElem = -1, hSrcNu1 = 0;
     std::vector<double> hSrcPattern, hStf;
@@ -147,6 +157,8 @@ protected:
     std::vector<int32_t> hElemNr;       // element Nr (max over its points)
     std::vector<int64_t> hSliceOff;     // first slice of each 3D element
     int64_t mNumSlices = 0;
+    std::vector<int64_t> hPropOff;      // offset of each 3D element's
+    int64_t mPropTotal = 0;             // parameters (np x 25 x Nr)
     int mNum3D = 0, mNu1Max3D = 0, mCardSlots = 0;
     int mCardOcc = 2;                   // chosen occupancy variant
     int mCardSlotsFor[5] = {0, 0, 0, 0, 0};
@@ -168,9 +180,10 @@ protected:
     DeviceBuffer dDispl, dVeloc, dAccel, dStiff;
     // device: elements
     DeviceBuffer dElemPoints, dElemAxial, dElemNu1, dElemOps, dElemMat,
-        dEMList, dGTab, dElemEBOff, dEB, dAdj;
+        dMatOff, dRot, dEMList, dGTab, dElemEBOff, dEB, dAdj;
     // device: cardinal path
-    DeviceBuffer dElemNr, dSliceOff, dList3D, dWork, dProp3D, dTwiddle,
+    DeviceBuffer dElemNr, dSliceOff, dPropOff, dList3D, dWork, dProp3D,
+        dTwiddle,
         dTwOff, dPlan, dCardScratch, dAttABG, dAtt3D, dMemVar, dDStress,
         dAtt1D, dMemVarFR, dDStressFR;
     // 3D operands of elements [e_begin, e_end), host layout [slice][width]
@@ -178,6 +191,9 @@ protected:
     // consecutive slices are consecutive in memory.
     virtual void uploadSlices(DeviceBuffer &b, int width, int e_begin,
                               int e_end, const double *src) = 0;
+    // 3D material of elements [e_begin, e_end): host [Nr][np][25] per
+    // element -> device [np * 25][Nr] at hPropOff[e]
+    virtual void uploadProp(int e_begin, int e_end, const double *src) = 0;
     // device: source and receivers
     DeviceBuffer dSrcPattern, dStf, dRecElem, dRecW, dRecPhi, dSeismo;
     DeviceBuffer dFlag;

@@ -139,6 +139,73 @@ __device__ __forceinline__ void iso_hooke(const cpx<Real> (&eps)[6], Real lam,
     sig[5] = cmul(eps[5], mu);
 }
 
+// Transversely isotropic Hooke's law in RTZ, radial symmetry axis
+// (TransverselyIsotropic::strainToStress, src/core/element/material/
+// elastic/anisotropy/TransverselyIsotropic.hpp:161-180), in the
+// reference's operation order.  T = cpx<Real> or Real.
+template <typename T, typename Real>
+__device__ __forceinline__ void ti_hooke(const T (&e)[6], Real A, Real C,
+                                         Real F, Real L, Real N, T (&s)[6]) {
+    const Real mN2 = N * (Real)-2;
+    const T s3 = e[0] + e[1];
+    const T s4 = s3 * A + e[2] * F;
+    s[0] = s4 + e[1] * mN2;
+    s[1] = s4 + e[0] * mN2;
+    s[2] = s3 * F + e[2] * C;
+    s[3] = e[3] * L;
+    s[4] = e[4] * L;
+    s[5] = e[5] * N;
+}
+
+// Full anisotropy (Anisotropic::strainToStress, Anisotropic.hpp:203-274):
+// s_i = sum_j C_ij e_j, j in order; c = C11 C12 .. C16 C22 .. C66 read
+// through idx(k) (the k-th of the 21).
+template <typename T, typename Real, typename Get>
+__device__ __forceinline__ void aniso_hooke(const T (&e)[6], Get c,
+                                            T (&s)[6]) {
+    // row i, column j -> index in the 21 upper-triangle entries
+    constexpr int ij[6][6] = {{0, 1, 2, 3, 4, 5},      {1, 6, 7, 8, 9, 10},
+                              {2, 7, 11, 12, 13, 14},  {3, 8, 12, 15, 16, 17},
+                              {4, 9, 13, 16, 18, 19},  {5, 10, 14, 17, 19, 20}};
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        T acc = e[0] * (Real)c(ij[i][0]);
+#pragma unroll
+        for (int j = 1; j < 6; j++) acc = acc + e[j] * (Real)c(ij[i][j]);
+        s[i] = acc;
+    }
+}
+
+// (s, phi, z) -> (R, T, Z) for a Voigt strain with engineering shears and
+// back for a stress (CoordTransformSpherical::transformSPZ_RTZ6 /
+// transformRTZ_SPZ6, src/core/element/crd/CoordTransformSpherical.hpp:
+// 186-249); sin, cos of theta and 2 theta of this GLL point.
+template <typename T, typename Real>
+__device__ __forceinline__ void rot_spz_rtz6(T (&e)[6], Real s1, Real c1,
+                                             Real s2, Real c2) {
+    const T s02 = e[0] + e[2];
+    const T d02 = e[0] - e[2];
+    e[0] = (s02 + d02 * c2 - e[4] * s2) * (Real)0.5;
+    e[2] = s02 - e[0];
+    e[4] = e[4] * c2 + d02 * s2;
+    const T e3 = e[3];
+    e[3] = e3 * c1 + e[5] * s1;
+    e[5] = e[5] * c1 - e3 * s1;
+}
+
+template <typename T, typename Real>
+__device__ __forceinline__ void rot_rtz_spz6(T (&s)[6], Real s1, Real c1,
+                                             Real s2, Real c2) {
+    const T s02 = s[0] + s[2];
+    const T d02 = s[0] - s[2];
+    s[0] = (s02 + d02 * c2) * (Real)0.5 + s[4] * s2;
+    s[2] = s02 - s[0];
+    s[4] = s[4] * c2 - d02 * s2 * (Real)0.5;
+    const T s3 = s[3];
+    s[3] = s3 * c1 - s[5] * s1;
+    s[5] = s[5] * c1 + s3 * s1;
+}
+
 // X = dFdS*dzdeta + dFdZ*dsdeta, Y = dFdS*dzdxii + dFdZ*dsdxii
 // (computeQuad_dSdZ, GradientQuadrature.hpp:471-474).
 template <typename Real>

@@ -35,9 +35,11 @@ __device__ __forceinline__ int att_slot_fr(int p, int npa) {
     return p == 6 ? 0 : p == 8 ? 1 : p == 16 ? 2 : p == 18 ? 3 : -1;
 }
 
-template <typename Real, bool ATT>
+// MT: material type of the element-modes [em0, em1) of em_list (0
+// isotropic, 1 TI, 2 anisotropic); em_list is grouped by type.
+template <typename Real, bool ATT, int MT>
 __global__ __launch_bounds__(STIFF_BLOCK) void k_solid_stiff_fr(
-    StiffArgs<Real> a) {
+    StiffArgs<Real> a, int em0, int em1) {
     __shared__ cpx<Real> sU[EMB][3][NPEM];
     __shared__ cpx<Real> sX[EMB][3][NPEM];
     __shared__ cpx<Real> sY[EMB][3][NPEM];
@@ -47,8 +49,8 @@ __global__ __launch_bounds__(STIFF_BLOCK) void k_solid_stiff_fr(
     if (t < 2 * NPEM) sG[t / NPEM][t % NPEM] = a.gtab[t];
     const int eml = t / NPEM;
     const int ipnt = t - eml * NPEM;
-    const int em = blockIdx.x * EMB + eml;
-    const bool active = (eml < EMB) && (em < a.n_em);
+    const int em = em0 + blockIdx.x * EMB + eml;
+    const bool active = (eml < EMB) && (em < em1);
     const int i = ipnt / NPED, j = ipnt - (ipnt / NPED) * NPED;
     const int emx = active ? eml : 0;
 
@@ -85,8 +87,28 @@ __global__ __launch_bounds__(STIFF_BLOCK) void k_solid_stiff_fr(
         load_point_ops(a.elem_ops, e, ipnt, g, q);
         cpx<Real> eps[6];
         grad6_point<Real>(sU[eml], u, ralpha, axial_row0, i, j, G, g, eps);
-        const Real *mat = a.elem_mat + (size_t)e * 2 * NPEM + ipnt;
-        iso_hooke<Real>(eps, mat[0], mat[NPEM], sig);
+        const Real *mat = a.elem_mat + a.mat_off[e] + ipnt;
+        // TI and anisotropic media work in RTZ on spherical meshes
+        // (SolidElement.cpp:177-193)
+        const bool rtz = MT != 0 && a.rot != nullptr;
+        Real rs1 = 0, rc1 = 1, rs2 = 0, rc2 = 1;
+        if (rtz) {
+            const Real *r = a.rot + (size_t)e * 4 * NPEM + ipnt;
+            rs1 = r[0];
+            rc1 = r[NPEM];
+            rs2 = r[2 * NPEM];
+            rc2 = r[3 * NPEM];
+            rot_spz_rtz6<cpx<Real>, Real>(eps, rs1, rc1, rs2, rc2);
+        }
+        if constexpr (MT == 0) {
+            iso_hooke<Real>(eps, mat[0], mat[NPEM], sig);
+        } else if constexpr (MT == 1) {
+            ti_hooke<cpx<Real>, Real>(eps, mat[0], mat[NPEM], mat[2 * NPEM],
+                                      mat[3 * NPEM], mat[4 * NPEM], sig);
+        } else {
+            aniso_hooke<cpx<Real>, Real>(
+                eps, [&](int k) { return mat[k * NPEM]; }, sig);
+        }
         if (ATT) {
             // AttenuationFull/CG4::apply per Fourier order with 1D moduli
             // (AttenuationCG4.hpp:312-330, Attenuation.hpp:92-140):
@@ -125,6 +147,7 @@ __global__ __launch_bounds__(STIFF_BLOCK) void k_solid_stiff_fr(
                 for (int d = 0; d < 6; d++) ds[d * npa] = dsn[d];
             }
         }
+        if (rtz) rot_rtz_spz6<cpx<Real>, Real>(sig, rs1, rc1, rs2, rc2);
     }
     cpx<Real> F[3];
     quad6_point<Real>(sig, ralpha, axial_row0, q, sX[emx], sY[emx], ipnt, i,
@@ -414,13 +437,24 @@ __global__ __launch_bounds__(64) void k_record(
 template <typename Real>
 void launch_solid_stiff_fr(hipStream_t st, const StiffArgs<Real> &a) {
     if (a.n_em <= 0) return;
-    const int grid = (a.n_em + EMB - 1) / EMB;
-    if (a.nsls > 0)
-        hipLaunchKernelGGL((k_solid_stiff_fr<Real, true>), dim3(grid),
-                           dim3(STIFF_BLOCK), 0, st, a);
-    else
-        hipLaunchKernelGGL((k_solid_stiff_fr<Real, false>), dim3(grid),
-                           dim3(STIFF_BLOCK), 0, st, a);
+    // one launch per material type present (em_list grouped by type)
+    for (int mt = 0; mt < 3; mt++) {
+        const int em0 = a.em_type_begin[mt], em1 = a.em_type_begin[mt + 1];
+        if (em1 <= em0) continue;
+        const int grid = (em1 - em0 + EMB - 1) / EMB;
+        const bool att = a.nsls > 0;
+#define AXS3D_FR(ATT_, MT_)                                                   \
+    hipLaunchKernelGGL((k_solid_stiff_fr<Real, ATT_, MT_>), dim3(grid),       \
+                       dim3(STIFF_BLOCK), 0, st, a, em0, em1)
+        if (mt == 0) {
+            if (att) AXS3D_FR(true, 0); else AXS3D_FR(false, 0);
+        } else if (mt == 1) {
+            if (att) AXS3D_FR(true, 1); else AXS3D_FR(false, 1);
+        } else {
+            if (att) AXS3D_FR(true, 2); else AXS3D_FR(false, 2);
+        }
+#undef AXS3D_FR
+    }
 }
 
 template <typename Real>

@@ -24,7 +24,11 @@ struct StiffArgs {
     const int32_t *elem_points;
     const uint8_t *elem_axial;
     const Real *elem_ops;
-    const Real *elem_mat;
+    const Real *elem_mat;          // 1D material, packed per element
+    const uint32_t *mat_off;       // offset of element e in elem_mat
+    const Real *rot;               // [e][4][25] sin, cos of theta, 2 theta
+                                   // (spherical RTZ), or null
+    int em_type_begin[4];          // em_list grouped by material type
     const uint32_t *point_off;
     const int32_t *point_nu1;
     const double *displ;
@@ -55,13 +59,16 @@ template <typename Real>
 struct CardArgs {
     StiffArgs<Real> s;
     int n3d;                       // number of 3D elements
-    const int32_t *list3d;         // 3D elements, heaviest first
-    int *work_counter;
+    const int32_t *list3d;         // 3D elements grouped by material type,
+                                   // each group heaviest first
+    int type_begin[4];             // group boundaries in list3d
+    int *work_counter;             // one per material type
     const int32_t *elem_nr;
     const int64_t *slice_off;      // first slice of each element
     // Per-slice operands are stored per element as [component][Nr], at
     // element offset slice_off * (components per slice).
-    const Real *prop3d;            // [2][25] lambda, mu
+    const Real *prop3d;            // [np][25] parameters of the type
+    const int64_t *prop_off;       // element offsets in prop3d
     const cpx<Real> *twiddle;      // exp(-2 pi i t / Nr), per Nr
     const int64_t *tw_off;         // offset of each Nr's table
     const uint32_t *plan;          // [Nr][CARD_PLAN_W] FFT stages

@@ -768,6 +768,187 @@ def make_3d_isotropic(spec, field, att=None, mask3d=None):
     return spec
 
 
+MAT_NPARAMS = (2, 5, 21)   # isotropic, TI, anisotropic
+
+
+def _iso_to_type(mt, lam, mu, u):
+    """Parameters (np, ...) of material type mt built from an isotropic
+    (lam, mu) and perturbed by the factors (1 + u) (TI, u (5, ...)) or by
+    u (lam + 2 mu) on every Cij (anisotropic, u (21, ...))."""
+    if mt == 0:
+        return np.array([lam, mu])
+    if mt == 1:
+        base = np.array([lam + 2 * mu, lam + 2 * mu, lam, mu, mu])
+        return base * (1.0 + u)
+    c = np.zeros((21,) + np.shape(lam))
+    for k in (0, 6, 11):
+        c[k] = lam + 2 * mu
+    for k in (1, 2, 7):
+        c[k] = lam
+    for k in (15, 18, 20):
+        c[k] = mu
+    return c + u * (lam + 2 * mu)
+
+
+def _voigt_kappa_mu(mt, par):
+    """Voigt averages of Material::createTISO / createAnisotropic
+    (Material.cpp:447-458, 533-536)."""
+    if mt == 0:
+        lam, mu = par
+        return lam + (2.0 / 3.0) * mu, mu
+    if mt == 1:
+        A, C, F, L, N = par
+        return (A * 4. + C + F * 4. - N * 4.) / 9., \
+            (A + C - F * 2. + L * 6. + N * 5.) / 15.
+    c = par
+    kp = (c[0] + c[6] + c[11] + (c[1] + c[7] + c[2]) * 2.) / 9.
+    mu = (c[0] + c[6] + c[11] - (c[1] + c[7] + c[2]) +
+          (c[15] + c[18] + c[20]) * 3.) / 15.
+    return kp, mu
+
+
+def _att_correct(mt, par, qkp, qmu, consts, cg4, wcg4):
+    """Unrelaxed-modulus correction of any medium through its Voigt
+    averages (Material.cpp:438-460 and 525-545): the isotropic part
+    (kappa + 4/3 mu, kappa - 2/3 mu, mu) is taken out, corrected by
+    AttBuilder::createAttenuation and put back.  Returns (par, dlam, dmu)."""
+    par = np.array(par, dtype=np.float64)
+    kp, mu = _voigt_kappa_mu(mt, par)
+    lam_c, mu_c, dl, dm = att_isotropic(kp - (2.0 / 3.0) * mu, mu.copy(),
+                                        qkp, qmu, consts, cg4, wcg4)
+    kp_c = lam_c + (2.0 / 3.0) * mu_c
+    d_n = (kp_c + 4. / 3. * mu_c) - (kp + 4. / 3. * mu)
+    d_o = (kp_c - 2. / 3. * mu_c) - (kp - 2. / 3. * mu)
+    d_s = mu_c - mu
+    if mt == 0:
+        par = np.array([kp_c - (2.0 / 3.0) * mu_c, mu_c])
+    elif mt == 1:
+        par[0] += d_n
+        par[1] += d_n
+        par[2] += d_o
+        par[3] += d_s
+        par[4] += d_s
+    else:
+        for k in (0, 6, 11):
+            par[k] += d_n
+        for k in (1, 7, 2):
+            par[k] += d_o
+        for k in (15, 18, 20):
+            par[k] += d_s
+    return par, dl, dm
+
+
+def gll_colatitude(sz):
+    """Colatitude theta of GLL points (s, z) (geodesy::sz2rtheta, 0 at the
+    centre); sz (E, 2, 25) -> (E, 25)."""
+    s, z = sz[:, 0], sz[:, 1]
+    r = np.hypot(s, z)
+    return np.where(r < np.finfo(float).eps, 0.0,
+                    np.arccos(np.clip(z / np.maximum(r, 1e-300), -1, 1)))
+
+
+def make_media(spec, mtype, field=None, mask3d=None, att=None, seed=0,
+               perturb=0.05):
+    """General solid media: per element material type mtype (0 isotropic, 1
+    TI with vertical (radial) axis, 2 anisotropic), 1D or, where mask3d
+    holds, 3D (the isotropic 1D model scaled per slice by (1 + d)^2 with
+    the random field, as make_3d_isotropic).  TI and anisotropic moduli
+    are the isotropic ones perturbed by +-perturb (seeded, per GLL point
+    and slice).  Attenuation corrects every medium through its Voigt
+    averages.  Spherical meshes get the GLL colatitudes for the RTZ
+    rotation of TI and anisotropic elements.  Fills spec.elem_mtype,
+    mat1d/mat1d_off, prop3d/prop_off, elem_kind/elem_slice_off, att."""
+    E = spec.n_solid_elements
+    mtype = np.asarray(mtype, dtype=np.uint8)
+    if mask3d is None:
+        mask3d = np.zeros(E, dtype=bool)
+    mask3d = np.asarray(mask3d, dtype=bool)
+    rng = np.random.default_rng(seed)
+    npar = np.array(MAT_NPARAMS)[mtype]
+    nr = np.where(mask3d, spec.elem_nr.astype(np.int64), 0)
+    slice_off = np.zeros(E + 1, dtype=np.int64)
+    slice_off[1:] = np.cumsum(nr)
+    mat1d_off = np.zeros(E + 1, dtype=np.int64)
+    mat1d_off[1:] = np.cumsum(npar * NPEM)
+    prop_off = np.zeros(E + 1, dtype=np.int64)
+    prop_off[1:] = np.cumsum(nr * npar * NPEM)
+    mat1d = np.zeros(mat1d_off[-1])
+    prop3d = np.zeros(prop_off[-1])
+    iso = spec.elem_iso.copy()
+    sz = spec._gq["sz"]
+    if att is not None:
+        w, y = att_sls_fit(att["nsls"], att["f_min"], att["f_max"])
+        consts = att_constants(w, y, att["f_min"], att["f_max"],
+                               att["f_ref"])
+        npa = 4 if att["cg4"] else NPEM
+        d3d = np.zeros((slice_off[-1], 2, npa))
+        d1d = np.zeros((E, 2, npa))
+        wcg4 = cg4_weights(spec._gq["ifact"])
+    vmax = np.zeros(E)
+    grow = math.sqrt(1.0 + 21 * perturb)
+    for e in range(E):
+        mt = int(mtype[e])
+        lam, mu = iso[e, 0], iso[e, 1]
+        if not mask3d[e]:
+            u = rng.uniform(-perturb, perturb, (npar[e], NPEM))
+            par = _iso_to_type(mt, lam, mu, u)
+            if att is not None:
+                par, dl, dm = _att_correct(mt, par, att["qkp"][e],
+                                           att["qmu"][e], consts,
+                                           att["cg4"], wcg4[e])
+                d1d[e, 0], d1d[e, 1] = dl, dm
+            mat1d[mat1d_off[e]:mat1d_off[e + 1]] = par.reshape(-1)
+            vmax[e] = spec._vp1d[e].max() * (grow if mt else 1.0)
+            continue
+        k = np.arange(nr[e])
+        phi = 2 * math.pi * k / nr[e]
+        x = sz[e, 0][None, :] * np.cos(phi)[:, None]
+        yy = sz[e, 0][None, :] * np.sin(phi)[:, None]
+        zz = np.broadcast_to(sz[e, 1][None, :], x.shape)
+        sc = (1.0 + field(x, yy, zz)) ** 2 if field is not None else \
+            np.ones_like(x)
+        lam3 = lam[None, :] * sc
+        mu3 = mu[None, :] * sc
+        u = rng.uniform(-perturb, perturb, (npar[e], nr[e], NPEM))
+        par = _iso_to_type(mt, lam3, mu3, u)           # (np, Nr, 25)
+        if att is not None:
+            qk = np.broadcast_to(att["qkp"][e][None, :], lam3.shape)
+            qm = np.broadcast_to(att["qmu"][e][None, :], lam3.shape)
+            par, dl, dm = _att_correct(mt, par, qk, qm, consts, att["cg4"],
+                                       wcg4[e][None, :])
+            d3d[slice_off[e]:slice_off[e + 1], 0] = dl
+            d3d[slice_off[e]:slice_off[e + 1], 1] = dm
+        prop3d[prop_off[e]:prop_off[e + 1]] = \
+            np.transpose(par, (1, 0, 2)).reshape(-1)
+        vmax[e] = (spec._vp1d[e][None, :] * np.sqrt(sc)).max() * \
+            (grow if mt else 1.0)
+    spec.elem_mtype = mtype
+    spec.mat1d = mat1d
+    spec.mat1d_off = mat1d_off[:-1].copy()
+    spec.prop3d = prop3d
+    spec.prop_off = prop_off[:-1].copy()
+    spec.elem_kind = mask3d.astype(np.uint8)
+    spec.elem_slice_off = slice_off[:-1].copy()
+    # isotropic 1D operands for callers that only know them
+    for e in range(E):
+        if mtype[e] == 0 and not mask3d[e]:
+            spec.elem_iso[e] = mat1d[mat1d_off[e]:mat1d_off[e + 1]].reshape(
+                2, NPEM)
+    if getattr(spec._mesh, "spherical", False):
+        spec.elem_theta = gll_colatitude(sz)
+    spec.dt = compute_dt(sz, vmax, spec._courant)
+    if att is not None:
+        spec.att = {"nsls": att["nsls"], "cg4": int(att["cg4"]),
+                    "w": w, "y": y, "d3d": d3d, "d1d": d1d,
+                    "f_min": att["f_min"], "f_max": att["f_max"],
+                    "f_ref": att["f_ref"]}
+        a, b, g = att_alpha_beta_gamma(w, y, spec.dt)
+        spec.att.update(alpha=a, beta=b, gamma=g)
+    else:
+        spec.att = None
+    return spec
+
+
 def layered_profile(seed, r_top, r_bot, nlayer=6, vs=(3200.0, 4800.0),
                     vpvs=(1.73, 1.80), rho=(2600.0, 3600.0)):
     """Seeded 1D model of `nlayer` equal-thickness layers, linear within

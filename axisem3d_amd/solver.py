@@ -60,10 +60,23 @@ class GpuSolver:
                                              ax.ctypes.data, gq.ctypes.data),
                   "set_elements")
         kind = getattr(s, "elem_kind", None)
-        if kind is None or np.any(np.asarray(kind) == 0):
+        mtype = getattr(s, "elem_mtype", None)
+        if mtype is not None:
+            mt = _c(mtype, np.uint8)
+            self._chk(L.axs3d_set_material_types(ctx, mt.ctypes.data),
+                      "set_material_types")
+            m1 = _c(s.mat1d, np.float64)
+            self._chk(L.axs3d_set_solid_material1d(ctx, m1.ctypes.data),
+                      "set_material1d")
+        elif kind is None or np.any(np.asarray(kind) == 0):
             iso = _c(s.elem_iso, np.float64)
             self._chk(L.axs3d_set_solid_iso1d(ctx, iso.ctypes.data),
                       "set_iso1d")
+        theta = getattr(s, "elem_theta", None)
+        if theta is not None:
+            th = _c(theta, np.float64)
+            self._chk(L.axs3d_set_rtz_theta(ctx, th.ctypes.data),
+                      "set_rtz_theta")
         att = getattr(s, "att", None)
         if kind is not None:
             kd = _c(kind, np.uint8)
@@ -82,9 +95,10 @@ class GpuSolver:
         if kind is not None:
             if getattr(s, "prop3d", None) is not None:
                 p3 = _c(s.prop3d, np.float64)
-                self._chk(L.axs3d_set_solid_iso3d(ctx, 0, ep.shape[0],
-                                                  p3.ctypes.data),
-                          "set_iso3d")
+                fn = (L.axs3d_set_solid_material3d if mtype is not None
+                      else L.axs3d_set_solid_iso3d)
+                self._chk(fn(ctx, 0, ep.shape[0], p3.ctypes.data),
+                          "set_material3d")
                 if att is not None and att.get("d3d") is not None:
                     d3 = _c(att["d3d"], np.float64)
                     self._chk(L.axs3d_set_attenuation3d(

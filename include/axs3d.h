@@ -108,6 +108,36 @@ int axs3d_set_element_kinds(axs3d_ctx *ctx, const uint8_t *kind);
 int axs3d_set_solid_iso3d(axs3d_ctx *ctx, int e_begin, int e_end,
                           const double *lam_mu);
 
+/* Material type per element: 0 isotropic, 1 transversely isotropic
+ * (radial axis), 2 fully anisotropic; the choice Material::createElastic
+ * makes (src/preloop/se_model/quad/physical/Material.cpp:366-379).  Set
+ * before any material; default all isotropic. */
+int axs3d_set_material_types(axs3d_ctx *ctx, const uint8_t *mtype);
+
+/* 1D material of every element, packed in element order: per element
+ * np x 25 values, np = 2 (lambda, mu), 5 (A, C, F, L, N) or 21 (C11 C12 ..
+ * C16 C22 .. C26 C33 .. C66) by type (entries of 3D elements ignored).
+ * Replaces the Isotropic / TransverselyIsotropic / Anisotropic 1D
+ * constructors (Isotropic.hpp:26-34, TransverselyIsotropic.hpp:26-40,
+ * Anisotropic.hpp:27-60), after Material::createTISO / createAnisotropic
+ * (Material.cpp:385-560). */
+int axs3d_set_solid_material1d(axs3d_ctx *ctx, const double *params);
+
+/* 3D material of the 3D elements in [e_begin, e_end): per element in
+ * order, Nr slices of [np][25] (np by type as above).  May be called in
+ * chunks.  Replaces the 3D constructors of the same classes (Property<N>
+ * 3D data, src/core/element/common/Property.hpp:39-42). */
+int axs3d_set_solid_material3d(axs3d_ctx *ctx, int e_begin, int e_end,
+                               const double *params);
+
+/* Colatitude theta [n_elem][25] of every GLL point on a spherical mesh
+ * (geodesy::sz2rtheta); TI and anisotropic elements then form stress in
+ * RTZ (Element::createCoordTransform, src/core/element/Element.cpp:
+ * 133-151; CoordTransformSpherical::transformSPZ_RTZ6 / RTZ_SPZ6,
+ * src/core/element/crd/CoordTransformSpherical.hpp:186-249).  Not set:
+ * Cartesian mesh (CoordTransformCartesian, no rotation). */
+int axs3d_set_rtz_theta(axs3d_ctx *ctx, const double *theta);
+
 /* Standard linear solids: alpha, beta, gamma per SLS
  * (AttBuilder::setAlphaBetaGamma, src/preloop/se_model/quad/physical/
  * AttBuilder.cpp:183-198 -> Attenuation::setAlphaBetaGamma,
