@@ -27,7 +27,8 @@ COMMON = [
     "-Wall", "-Wno-unused-result",
 ]
 
-SOURCES = ["axs3d_kernels.hip", "axs3d_cardinal.hip", "axs3d_domain.cpp",
+SOURCES = ["axs3d_kernels.hip", "axs3d_cardinal.hip", "axs3d_fluid.hip",
+           "axs3d_domain.cpp",
            "axs3d_capi.cpp"]
 
 

@@ -15,9 +15,12 @@ AXS3D_FP64 = 0
 AXS3D_FP32_FP64 = 1
 
 FIELD_DISPL, FIELD_VELOC, FIELD_ACCEL, FIELD_STIFF = 0, 1, 2, 3
+FIELD_FLUID_DISPL, FIELD_FLUID_VELOC, FIELD_FLUID_ACCEL = 4, 5, 6
+FIELD_FLUID_STIFF = 7
 KERNEL_STIFFNESS, KERNEL_POINT, KERNEL_SOURCE, KERNEL_RECORD = 0, 1, 2, 3
 KERNEL_STIFFNESS_3D = 4
-NUM_KERNELS = 5
+KERNEL_FLUID_STIFFNESS, KERNEL_FLUID_POINT = 5, 6
+NUM_KERNELS = 7
 
 # every symbol declared in include/axs3d.h, with (restype, argtypes)
 _p = C.c_void_p
@@ -39,6 +42,13 @@ SIGNATURES = {
     "axs3d_set_attenuation3d": (_i, [_p, _i, _i, _p]),
     "axs3d_set_attenuation1d": (_i, [_p, _p]),
     "axs3d_set_material_types": (_i, [_p, _p]),
+    "axs3d_set_fluid_points": (_i, [_p, _i, _p, _p]),
+    "axs3d_set_fluid_elements": (_i, [_p, _i, _p, _p, _p]),
+    "axs3d_set_fluid_k1d": (_i, [_p, _p]),
+    "axs3d_set_fluid_axial_points": (_i, [_p, _i, _p]),
+    "axs3d_set_fluid_surface_points": (_i, [_p, _i, _p]),
+    "axs3d_set_solid_fluid_coupling": (_i, [_p, _i, _p, _p, _p]),
+    "axs3d_num_fluid_modes": (_i, [_p, C.POINTER(C.c_int64)]),
     "axs3d_set_solid_material1d": (_i, [_p, _p]),
     "axs3d_set_solid_material3d": (_i, [_p, _i, _i, _p]),
     "axs3d_set_rtz_theta": (_i, [_p, _p]),

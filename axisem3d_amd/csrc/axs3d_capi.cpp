@@ -111,6 +111,53 @@ int axs3d_set_attenuation3d(axs3d_ctx *ctx, int e_begin, int e_end,
     });
 }
 
+int axs3d_set_fluid_points(axs3d_ctx *ctx, int n, const int32_t *nr,
+                           const double *mass) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) {
+        d.setFluidPoints(n, nr, mass);
+    });
+}
+
+int axs3d_set_fluid_elements(axs3d_ctx *ctx, int n, const int32_t *points,
+                             const uint8_t *axial, const double *gq) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) {
+        d.setFluidElements(n, points, axial, gq);
+    });
+}
+
+int axs3d_set_fluid_k1d(axs3d_ctx *ctx, const double *k) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) { d.setFluidK1D(k); });
+}
+
+int axs3d_set_fluid_axial_points(axs3d_ctx *ctx, int n,
+                                 const int32_t *points) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) {
+        d.setFluidAxialPoints(n, points);
+    });
+}
+
+int axs3d_set_fluid_surface_points(axs3d_ctx *ctx, int n,
+                                   const int32_t *points) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) {
+        d.setFluidSurfacePoints(n, points);
+    });
+}
+
+int axs3d_set_solid_fluid_coupling(axs3d_ctx *ctx, int n,
+                                   const int32_t *solid_points,
+                                   const int32_t *fluid_points,
+                                   const double *normals) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) {
+        d.setSolidFluidCoupling(n, solid_points, fluid_points, normals);
+    });
+}
+
+int axs3d_num_fluid_modes(const axs3d_ctx *ctx, int64_t *nmodes) {
+    if (!ctx || !nmodes) return 2;
+    *nmodes = ctx->dom->numFluidModes();
+    return 0;
+}
+
 int axs3d_set_material_types(axs3d_ctx *ctx, const uint8_t *mtype) {
     return guarded(ctx, [&](axs3d::DomainBase &d) {
         d.setMaterialTypes(mtype);

@@ -141,6 +141,56 @@ void DomainBase::setSolidMaterial1D(const double *params) {
     hMat1D.assign(params, params + off);
 }
 
+void DomainBase::setFluidPoints(int n, const int32_t *nr,
+                                const double *mass) {
+    if (n < 0) throw Error("Domain::setFluidPoints || Negative count.");
+    mNumFPoints = n;
+    hFPointNr.assign(nr, nr + n);
+    hFMass.assign(mass, mass + n);
+    for (int p = 0; p < n; p++) {
+        if (hFPointNr[p] < 1)
+            throw Error("Domain::setFluidPoints || Nr must be positive.");
+        if (!(hFMass[p] > 0.))
+            throw Error("Domain::setFluidPoints || Mass must be positive.");
+    }
+}
+
+void DomainBase::setFluidElements(int n, const int32_t *points,
+                                  const uint8_t *axial, const double *gq) {
+    if (n < 0) throw Error("Domain::setFluidElements || Negative count.");
+    mNumFElems = n;
+    hFElemPoints.assign(points, points + (size_t)n * NPEM);
+    hFElemAxial.assign(axial, axial + n);
+    hFElemGQ.assign(gq, gq + (size_t)n * 6 * NPEM);
+}
+
+// Acoustic 1D: K = 1 / rho per GLL point (Material::createAcoustic,
+// Material.cpp:350-360; Acoustic.hpp:20-25).
+void DomainBase::setFluidK1D(const double *k) {
+    if (mNumFElems <= 0)
+        throw Error("Domain::setFluidK1D || Set fluid elements first.");
+    hFElemK.assign(k, k + (size_t)mNumFElems * NPEM);
+}
+
+void DomainBase::setFluidAxialPoints(int n, const int32_t *points) {
+    hFAxial.assign(points, points + std::max(n, 0));
+}
+
+void DomainBase::setFluidSurfacePoints(int n, const int32_t *points) {
+    hFSurf.assign(points, points + std::max(n, 0));
+}
+
+void DomainBase::setSolidFluidCoupling(int n, const int32_t *solid,
+                                       const int32_t *fluid,
+                                       const double *normals) {
+    if (n < 0)
+        throw Error("Domain::setSolidFluidCoupling || Negative count.");
+    mNumPairs = n;
+    hCplSolid.assign(solid, solid + n);
+    hCplFluid.assign(fluid, fluid + n);
+    hCplNormal.assign(normals, normals + (size_t)n * 4);
+}
+
 void DomainBase::setRtzTheta(const double *theta) {
     if (mNumElems <= 0)
         throw Error("Domain::setRtzTheta || Set elements first.");
@@ -470,6 +520,109 @@ void DomainBase::finalize() {
         if (e < 0 || e >= mNumElems)
             throw Error("Domain::finalize || Receiver element out of range.");
 
+    // fluid points, elements, boundaries and the solid-fluid coupling
+    std::vector<uint32_t> fOff(mNumFPoints + 1, 0), fEbOff, fAdj, cSOff,
+        cFOff;
+    std::vector<int32_t> fNu1(mNumFPoints), fModePoint, fElemNu1,
+        cSolidPair(mNumPoints, -1), cFluidPair(mNumFPoints, -1);
+    std::vector<uint8_t> fBC(mNumFPoints, 0);
+    std::vector<EMItem> fem;
+    std::vector<double> cN, cNM;
+    if (mNumFElems > 0) {
+        if (mNumFPoints <= 0)
+            throw Error("Domain::finalize || Fluid points are not set.");
+        if (hFElemK.size() != (size_t)mNumFElems * NPEM)
+            throw Error("Domain::finalize || Fluid material is not set.");
+        int64_t ft = 0;
+        for (int p = 0; p < mNumFPoints; p++) {
+            fOff[p] = (uint32_t)ft;
+            fNu1[p] = hFPointNr[p] / 2 + 1;
+            ft += fNu1[p];
+        }
+        if (ft >= (int64_t)UINT32_MAX / 4)
+            throw Error("Domain::finalize || Too many fluid modes.");
+        fOff[mNumFPoints] = (uint32_t)ft;
+        mNumFModes = ft;
+        fModePoint.resize(ft);
+        for (int p = 0; p < mNumFPoints; p++)
+            for (int a = 0; a < fNu1[p]; a++) fModePoint[fOff[p] + a] = p;
+        fElemNu1.resize(mNumFElems);
+        fEbOff.resize(mNumFElems);
+        int64_t eb = 0;
+        for (int e = 0; e < mNumFElems; e++) {
+            int nrmax = 0;
+            for (int k = 0; k < NPEM; k++) {
+                const int p = hFElemPoints[(size_t)e * NPEM + k];
+                if (p < 0 || p >= mNumFPoints)
+                    throw Error("Domain::finalize || Fluid element point out "
+                                "of range.");
+                nrmax = std::max(nrmax, hFPointNr[p]);
+            }
+            fElemNu1[e] = nrmax / 2 + 1;
+            for (int a = 0; a < fElemNu1[e]; a++) fem.push_back({e, a});
+            fEbOff[e] = (uint32_t)eb;
+            eb += (int64_t)NPEM * fElemNu1[e];
+        }
+        if (eb >= (int64_t)UINT32_MAX / 4)
+            throw Error("Domain::finalize || Too many fluid element modes.");
+        mNumFEBModes = eb;
+        mNumFEM = (int)fem.size();
+        fAdj.assign((size_t)mNumFPoints * 4, 0xffffffffu);
+        std::vector<uint8_t> fill(mNumFPoints, 0);
+        for (int e = 0; e < mNumFElems; e++)
+            for (int k = 0; k < NPEM; k++) {
+                const int p = hFElemPoints[(size_t)e * NPEM + k];
+                if (fill[p] >= 4)
+                    throw Error("Domain::finalize || A fluid point is shared "
+                                "by more than four elements.");
+                fAdj[(size_t)p * 4 + fill[p]++] =
+                    fEbOff[e] + (uint32_t)k * fElemNu1[e];
+            }
+        for (int32_t p : hFAxial) {
+            if (p < 0 || p >= mNumFPoints)
+                throw Error("Domain::finalize || Fluid axial point out of "
+                            "range.");
+            fBC[p] |= 1;
+        }
+        for (int32_t p : hFSurf) {
+            if (p < 0 || p >= mNumFPoints)
+                throw Error("Domain::finalize || Fluid surface point out of "
+                            "range.");
+            fBC[p] |= 2;
+        }
+        // SolidFluidCoupling constructor and checkCompatibility
+        // (SolidFluidCoupling.cpp:18-44, SolidFluidCoupling1D.cpp): the
+        // fluid-side normal is taken unassembled, the solid side assembled
+        // and divided by the fluid mass
+        cSOff.resize(mNumPairs);
+        cFOff.resize(mNumPairs);
+        cN.resize((size_t)mNumPairs * 2);
+        cNM.resize((size_t)mNumPairs * 2);
+        for (int k = 0; k < mNumPairs; k++) {
+            const int sp = hCplSolid[k], fp = hCplFluid[k];
+            if (sp < 0 || sp >= mNumPoints || fp < 0 || fp >= mNumFPoints)
+                throw Error("SolidFluidCoupling::checkCompatibility || Point "
+                            "out of range.");
+            if (hPointNr[sp] != hFPointNr[fp])
+                throw Error("SolidFluidCoupling::checkCompatibility || "
+                            "Incompatible sizes.");
+            if (cSolidPair[sp] >= 0 || cFluidPair[fp] >= 0)
+                throw Error("SolidFluidCoupling::checkCompatibility || Point "
+                            "coupled twice.");
+            cSolidPair[sp] = k;
+            cFluidPair[fp] = k;
+            cSOff[k] = off[sp];
+            cFOff[k] = fOff[fp];
+            const double *nn = hCplNormal.data() + (size_t)k * 4;
+            cN[2 * k] = nn[0];
+            cN[2 * k + 1] = nn[1];
+            cNM[2 * k] = nn[2] / hFMass[fp];
+            cNM[2 * k + 1] = nn[3] / hFMass[fp];
+        }
+    } else if (mNumPairs > 0) {
+        throw Error("Domain::finalize || Coupling without fluid elements.");
+    }
+
     hipStream_t st = mStream;
     auto up = [&](DeviceBuffer &b, const void *src, size_t bytes) {
         b.alloc(bytes);
@@ -515,6 +668,39 @@ void DomainBase::finalize() {
         up(dRecPhi, hRecPhi.data(), hRecPhi.size() * 8);
         dSeismo.alloc((size_t)std::max(hMaxSamples, 1) * mNumRec * 3 * 8);
         dSeismo.zero(st);
+    }
+    if (mNumFElems > 0) {
+        up(dFPointOff, fOff.data(), fOff.size() * 4);
+        up(dFPointNu1, fNu1.data(), fNu1.size() * 4);
+        up(dFPointNr, hFPointNr.data(), hFPointNr.size() * 4);
+        up(dFPointBC, fBC.data(), fBC.size());
+        std::vector<double> finv(mNumFPoints);
+        for (int p = 0; p < mNumFPoints; p++) finv[p] = 1. / hFMass[p];
+        up(dFInvMass, finv.data(), finv.size() * 8);
+        up(dFModePoint, fModePoint.data(), fModePoint.size() * 4);
+        up(dFAdj, fAdj.data(), fAdj.size() * 4);
+        up(dFElemPoints, hFElemPoints.data(), hFElemPoints.size() * 4);
+        up(dFElemAxial, hFElemAxial.data(), hFElemAxial.size());
+        up(dFElemNu1, fElemNu1.data(), fElemNu1.size() * 4);
+        up(dFEMList, fem.data(), fem.size() * sizeof(EMItem));
+        up(dFEBOff, fEbOff.data(), fEbOff.size() * 4);
+        const size_t fb = (size_t)mNumFModes * 2 * sizeof(double);
+        dChi.alloc(fb);
+        dChiV.alloc(fb);
+        dChiA.alloc(fb);
+        dChi.zero(st);
+        dChiV.zero(st);
+        dChiA.zero(st);
+        dCStiff.alloc(fb);
+        dCStiff.zero(st);
+        if (mNumPairs > 0) {
+            up(dCplSolidPair, cSolidPair.data(), cSolidPair.size() * 4);
+            up(dCplFluidPair, cFluidPair.data(), cFluidPair.size() * 4);
+            up(dCplSOff, cSOff.data(), cSOff.size() * 4);
+            up(dCplFOff, cFOff.data(), cFOff.size() * 4);
+            up(dCplN, cN.data(), cN.size() * 8);
+            up(dCplNM, cNM.data(), cNM.size() * 8);
+        }
     }
     dFlag.alloc(sizeof(int));
     dFlag.zero(st);
@@ -712,6 +898,46 @@ public:
         g.adj = dAdj.as<uint32_t>();
         return g;
     }
+    PointGeom fluidGeom() const {
+        PointGeom g;
+        g.n_modes = mNumFModes;
+        g.mode_point = dFModePoint.as<int32_t>();
+        g.point_off = dFPointOff.as<uint32_t>();
+        g.point_nr = dFPointNr.as<int32_t>();
+        g.point_axial = dFPointBC.as<uint8_t>();
+        g.inv_mass = dFInvMass.as<double>();
+        g.adj = dFAdj.as<uint32_t>();
+        return g;
+    }
+    CouplingGeom couplingGeom() const {
+        CouplingGeom c;
+        c.n_pairs = mNumFElems > 0 ? mNumPairs : 0;
+        c.solid_pair = dCplSolidPair.as<int32_t>();
+        c.fluid_pair = dCplFluidPair.as<int32_t>();
+        c.soff = dCplSOff.as<uint32_t>();
+        c.foff = dCplFOff.as<uint32_t>();
+        c.n = dCplN.as<double>();
+        c.n_m = dCplNM.as<double>();
+        c.cstiff = dCStiff.as<double>();
+        return c;
+    }
+    FluidArgs<Real> fluidArgs() const {
+        FluidArgs<Real> f;
+        f.n_em = mNumFEM;
+        f.em_list = dFEMList.as<EMItem>();
+        f.elem_points = dFElemPoints.as<int32_t>();
+        f.elem_axial = dFElemAxial.as<uint8_t>();
+        f.elem_ops = dFElemOps.as<Real>();
+        f.elem_k = dFElemK.as<Real>();
+        f.point_off = dFPointOff.as<uint32_t>();
+        f.point_nu1 = dFPointNu1.as<int32_t>();
+        f.chi = dChi.as<double>();
+        f.elem_eb_off = dFEBOff.as<uint32_t>();
+        f.elem_nu1 = dFElemNu1.as<int32_t>();
+        f.eb = dFEB.as<Real>();
+        f.gtab = dGTab.as<Real>();
+        return f;
+    }
 
     // Domain::applySources: the source force is subtracted from the point
     // stiffness (stage form).
@@ -744,15 +970,23 @@ public:
                                         mCardOcc);
             tock(AXS3D_KERNEL_STIFFNESS_3D);
         }
+        if (mNumFEM > 0) {
+            tick(AXS3D_KERNEL_FLUID_STIFFNESS);
+            launch_fluid_stiff_fr<Real>(mStream, fluidArgs());
+            tock(AXS3D_KERNEL_FLUID_STIFFNESS);
+        }
     }
 
-    // Domain::computeStiffness (stage form): element kernel, then the
+    // Domain::computeStiffness (stage form): element kernels, then the
     // points subtract the element-local forces.
     void computeStiffness() override {
         requireFinalized("Domain::computeStiffness");
         elementKernel();
         launch_assemble<Real>(mStream, pointGeom(), dEB.as<Real>(),
                               dStiff.as<Real>());
+        if (mNumFElems > 0)
+            launch_fluid_assemble<Real>(mStream, fluidGeom(), dFEB.as<Real>(),
+                                        dFStiff.as<Real>());
     }
 
     void point(int phases) {
@@ -760,39 +994,68 @@ public:
         launch_point<Real>(mStream, phases, pointGeom(), dEB.as<Real>(),
                            dStiff.as<Real>(), dDispl.as<double>(),
                            dVeloc.as<double>(), dAccel.as<double>(), mDt,
-                           dFlag.as<int>());
+                           dFlag.as<int>(), couplingGeom());
         tock(AXS3D_KERNEL_POINT);
+    }
+    // fluid points; runs before the solid points of the same phase (the
+    // coupling reads the solid displacement before it is marched and
+    // leaves the fluid stiffness the solid points subtract)
+    void fluidPoint(int phases) {
+        if (mNumFElems <= 0) return;
+        tick(AXS3D_KERNEL_FLUID_POINT);
+        launch_fluid_point<Real>(mStream, phases, fluidGeom(), dFEB.as<Real>(),
+                                 dFStiff.as<Real>(), dChi.as<double>(),
+                                 dChiV.as<double>(), dChiA.as<double>(), mDt,
+                                 dFlag.as<int>(), couplingGeom(),
+                                 dDispl.as<double>());
+        tock(AXS3D_KERNEL_FLUID_POINT);
+    }
+    // SolidFluidBoundary::apply (Domain.cpp:313-318)
+    void applyBC_BeforeAssemblingStiff() override {
+        requireFinalized("Domain::applyBC_BeforeAssemblingStiff");
+        if (mNumFElems <= 0 || mNumPairs <= 0) return;
+        fluidPoint(PHASE_COUPLE);
+        point(PHASE_COUPLE);
     }
     void applyBC_AfterAssemblingStiff() override {
         requireFinalized("Domain::applyBC_AfterAssemblingStiff");
         point(PHASE_AXIAL);
+        fluidPoint(PHASE_AXIAL);
     }
     void computeStiffToAccel() override {
         requireFinalized("Domain::computeStiffToAccel");
         point(PHASE_MASS);
+        fluidPoint(PHASE_MASS);
     }
     void newmark() override {
         requireFinalized("NewmarkTimeScheme::update");
         point(PHASE_MARCH);
+        fluidPoint(PHASE_MARCH);
     }
 
-    // One Newmark step in its bulk form: the element kernel writes the
+    // One Newmark step in its bulk form: the element kernels write the
     // element-local forces, the source adds into its element's, the
-    // receivers sample u(t), and one point kernel gathers, masks, divides
-    // by the mass and marches.
+    // receivers sample u(t), then one fluid and one solid point kernel
+    // gather, couple, mask, divide by the mass and march.
     void step(int tstep, int isample) override {
         elementKernel();
         sourceKernel(tstep, true);
         if (isample >= 0) recordOutput(isample);
-        point(PHASE_GATHER | PHASE_AXIAL | PHASE_MASS | PHASE_MARCH);
+        const int all = PHASE_GATHER | PHASE_AXIAL | PHASE_MASS | PHASE_MARCH;
+        fluidPoint(all | PHASE_COUPLE);
+        point(all | PHASE_COUPLE);
     }
 
     void getField(int which, double *out) override {
         requireFinalized("Domain::getField");
-        const size_t n = (size_t)mNumModes * 6;
-        if (which == AXS3D_FIELD_STIFF) {
+        const bool fl = which >= AXS3D_FIELD_FLUID_DISPL;
+        const int base = fl ? which - AXS3D_FIELD_FLUID_DISPL : which;
+        const size_t n = fl ? (size_t)mNumFModes * 2 : (size_t)mNumModes * 6;
+        if (n == 0) return;
+        if (base == AXS3D_FIELD_STIFF) {
             std::vector<Real> tmp(n);
-            hip_check(hipMemcpyAsync(tmp.data(), dStiff.as<Real>(),
+            hip_check(hipMemcpyAsync(tmp.data(),
+                                     (fl ? dFStiff : dStiff).template as<Real>(),
                                      n * sizeof(Real), hipMemcpyDeviceToHost,
                                      mStream), "Domain::getField");
             hip_check(hipStreamSynchronize(mStream), "Domain::getField");
@@ -807,12 +1070,16 @@ public:
 
     void setField(int which, const double *in) override {
         requireFinalized("Domain::setField");
-        const size_t n = (size_t)mNumModes * 6;
-        if (which == AXS3D_FIELD_STIFF) {
+        const bool fl = which >= AXS3D_FIELD_FLUID_DISPL;
+        const int base = fl ? which - AXS3D_FIELD_FLUID_DISPL : which;
+        const size_t n = fl ? (size_t)mNumFModes * 2 : (size_t)mNumModes * 6;
+        if (n == 0) return;
+        if (base == AXS3D_FIELD_STIFF) {
             std::vector<Real> tmp(in, in + n);
-            hip_check(hipMemcpyAsync(dStiff.as<Real>(), tmp.data(),
-                                     n * sizeof(Real), hipMemcpyHostToDevice,
-                                     mStream), "Domain::setField");
+            hip_check(hipMemcpyAsync((fl ? dFStiff : dStiff).template as<Real>(),
+                                     tmp.data(), n * sizeof(Real),
+                                     hipMemcpyHostToDevice, mStream),
+                      "Domain::setField");
             hip_check(hipStreamSynchronize(mStream), "Domain::setField");
             return;
         }
@@ -824,12 +1091,25 @@ public:
 
 private:
     double *fieldPtr(int which) {
+        if (which < 0 || which > AXS3D_FIELD_FLUID_STIFF)
+            throw Error("Domain::field || Unknown field id.");
         switch (which) {
             case AXS3D_FIELD_DISPL: return dDispl.as<double>();
             case AXS3D_FIELD_VELOC: return dVeloc.as<double>();
             case AXS3D_FIELD_ACCEL: return dAccel.as<double>();
-            default: throw Error("Domain::field || Unknown field id.");
+            case AXS3D_FIELD_FLUID_DISPL:
+                if (mNumFElems > 0) return dChi.as<double>();
+                break;
+            case AXS3D_FIELD_FLUID_VELOC:
+                if (mNumFElems > 0) return dChiV.as<double>();
+                break;
+            case AXS3D_FIELD_FLUID_ACCEL:
+                if (mNumFElems > 0) return dChiA.as<double>();
+                break;
+            default:
+                break;
         }
+        throw Error("Domain::field || Unknown field id.");
     }
 
     // Times the cardinal kernel's occupancy variants on this workload and
@@ -955,6 +1235,30 @@ private:
         }
         dElemOps.alloc(ops.size() * sizeof(Real));
         dElemOps.upload(ops.data(), ops.size() * sizeof(Real), st);
+        if (mNumFElems > 0) {
+            std::vector<Real> fops((size_t)mNumFElems * ELEM_OPS * NPEM);
+            for (int e = 0; e < mNumFElems; e++) {
+                const double *g = hFElemGQ.data() + (size_t)e * 6 * NPEM;
+                Real *o = fops.data() + (size_t)e * ELEM_OPS * NPEM;
+                for (int k = 0; k < NPEM; k++) {
+                    const double ifk = g[5 * NPEM + k];
+                    for (int r = 0; r < 5; r++) {
+                        o[r * NPEM + k] = (Real)g[r * NPEM + k];
+                        o[(5 + r) * NPEM + k] = (Real)(g[r * NPEM + k] * ifk);
+                    }
+                }
+            }
+            dFElemOps.alloc(fops.size() * sizeof(Real));
+            dFElemOps.upload(fops.data(), fops.size() * sizeof(Real), st);
+            std::vector<Real> fk(hFElemK.begin(), hFElemK.end());
+            dFElemK.alloc(fk.size() * sizeof(Real));
+            dFElemK.upload(fk.data(), fk.size() * sizeof(Real), st);
+            dFEB.alloc((size_t)std::max<int64_t>(mNumFEBModes, 1) * 2 *
+                       sizeof(Real));
+            dFEB.zero(st);
+            dFStiff.alloc((size_t)mNumFModes * 2 * sizeof(Real));
+            dFStiff.zero(st);
+        }
         std::vector<Real> mat(hMat1D.begin(), hMat1D.end());
         if (mat.empty()) mat.assign(2 * NPEM, (Real)0);
         dElemMat.alloc(mat.size() * sizeof(Real));

@@ -64,6 +64,15 @@ public:
     void setSolidMaterial1D(const double *params);
     void setSolidMaterial3D(int e_begin, int e_end, const double *params);
     void setRtzTheta(const double *theta);
+    // fluid (displacement potential) points, elements and boundaries
+    void setFluidPoints(int n, const int32_t *nr, const double *mass);
+    void setFluidElements(int n, const int32_t *points, const uint8_t *axial,
+                          const double *gq);
+    void setFluidK1D(const double *k);
+    void setFluidAxialPoints(int n, const int32_t *points);
+    void setFluidSurfacePoints(int n, const int32_t *points);
+    void setSolidFluidCoupling(int n, const int32_t *solid,
+                               const int32_t *fluid, const double *normals);
     void setAttenuation(int nsls, int cg4, const double *alpha,
                         const double *beta, const double *gamma);
     void setAttenuation3D(int e_begin, int e_end, const double *dlam_dmu);
@@ -80,7 +89,7 @@ public:
     // time-loop phases (Domain.cpp:285-345)
     virtual void applySources(int tstep) = 0;
     virtual void computeStiffness() = 0;
-    void applyBC_BeforeAssemblingStiff() {}
+    virtual void applyBC_BeforeAssemblingStiff() = 0;
     void recordOutput(int isample);
     virtual void applyBC_AfterAssemblingStiff() = 0;
     virtual void computeStiffToAccel() = 0;
@@ -96,6 +105,7 @@ public:
 
     // debug
     int64_t numModes() const { return mNumModes; }
+    int64_t numFluidModes() const { return mNumFModes; }
     virtual void getField(int which, double *out) = 0;
     virtual void setField(int which, const double *in) = 0;
     void synchronize();
@@ -140,6 +150,15 @@ protected:
     std::vector<uint8_t> hMType;        // 0 isotropic, 1 TI, 2 anisotropic
     std::vector<double> hTheta;         // [e][25] colatitude, or empty
     int mEMTypeBegin[4] = {0, 0, 0, 0};
+    // fluid
+    std::vector<int32_t> hFPointNr, hFElemPoints, hFAxial, hFSurf;
+    std::vector<double> hFMass, hFElemGQ, hFElemK;
+    std::vector<uint8_t> hFElemAxial;
+    std::vector<int32_t> hCplSolid, hCplFluid;
+    std::vector<double> hCplNormal;     // [pair][4] ns, nz unassembled,
+                                        // ns, nz assembled
+    int mNumFPoints = 0, mNumFElems = 0, mNumFEM = 0, mNumPairs = 0;
+    int64_t mNumFModes = 0, mNumFEBModes = 0;
     int mTypeBegin3D[4] = {0, 0, 0, 0};
     std::vector<int32_t> hAxial;
     int hSrcElem = -1, hSrcNu1 = 0;
@@ -194,6 +213,12 @@ protected:
     // 3D material of elements [e_begin, e_end): host [Nr][np][25] per
     // element -> device [np * 25][Nr] at hPropOff[e]
     virtual void uploadProp(int e_begin, int e_end, const double *src) = 0;
+    // device: fluid
+    DeviceBuffer dFPointOff, dFPointNu1, dFPointNr, dFPointBC, dFInvMass,
+        dFModePoint, dFAdj, dFElemPoints, dFElemAxial, dFElemNu1, dFElemOps,
+        dFElemK, dFEMList, dFEBOff, dFEB, dChi, dChiV, dChiA, dFStiff;
+    DeviceBuffer dCplSolidPair, dCplFluidPair, dCplSOff, dCplFOff, dCplN,
+        dCplNM, dCStiff;
     // device: source and receivers
     DeviceBuffer dSrcPattern, dStf, dRecElem, dRecW, dRecPhi, dSeismo;
     DeviceBuffer dFlag;

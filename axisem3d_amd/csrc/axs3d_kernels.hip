@@ -304,9 +304,14 @@ struct PointArgs {
     double *displ, *veloc, *accel;
     double dt, half_dt, half_dt_dt;
     int *flag;
+    CouplingGeom cpl;
 };
 
-template <typename Real, bool GATHER, bool AXIAL, bool MASS, bool MARCH>
+//   COUPLE SolidFluidCoupling1D::coupleFluidToSolid (SolidFluidCoupling1D
+//          .hpp:33-37): s -= (ns, nz) / m_fluid * fluid stiffness, after
+//          the fluid points have added the solid displacement
+template <typename Real, bool GATHER, bool AXIAL, bool MASS, bool MARCH,
+          bool COUPLE>
 __global__ __launch_bounds__(256) void k_point(PointArgs a,
                                                const Real *__restrict__ eb,
                                                Real *__restrict__ stiff) {
@@ -320,6 +325,20 @@ __global__ __launch_bounds__(256) void k_point(PointArgs a,
     } else {
 #pragma unroll
         for (int q = 0; q < 6; q++) s[q] = stiff[(size_t)k * 6 + q];
+    }
+    if (COUPLE) {
+        const int pi = a.cpl.solid_pair[p];
+        if (pi >= 0) {
+            const double f_re = a.cpl.cstiff[2 * ((size_t)a.cpl.foff[pi] +
+                                                   alpha)];
+            const double f_im = a.cpl.cstiff[2 * ((size_t)a.cpl.foff[pi] +
+                                                   alpha) + 1];
+            const double ns = a.cpl.n_m[2 * pi], nz = a.cpl.n_m[2 * pi + 1];
+            s[0] = (Real)((double)s[0] - ns * f_re);
+            s[1] = (Real)((double)s[1] - ns * f_im);
+            s[4] = (Real)((double)s[4] - nz * f_re);
+            s[5] = (Real)((double)s[5] - nz * f_im);
+        }
     }
     if (AXIAL && a.point_axial[p]) {
         if (alpha == 0) {
@@ -492,7 +511,8 @@ void launch_assemble(hipStream_t st, const PointGeom &g, const Real *eb,
 template <typename Real>
 void launch_point(hipStream_t st, int phases, const PointGeom &g,
                   const Real *eb, Real *stiff, double *displ, double *veloc,
-                  double *accel, double dt, int *flag) {
+                  double *accel, double dt, int *flag,
+                  const CouplingGeom &cpl) {
     if (g.n_modes <= 0) return;
     const int block = 256;
     const int64_t grid = (g.n_modes + block - 1) / block;
@@ -511,17 +531,21 @@ void launch_point(hipStream_t st, int phases, const PointGeom &g,
     a.half_dt = .5 * dt;
     a.half_dt_dt = .5 * dt * dt;
     a.flag = flag;
-#define AXS3D_POINT_CASE(G, A, M, T)                                          \
-    case (A) | ((M) << 1) | ((T) << 2) | ((G) << 3):                         \
-        hipLaunchKernelGGL((k_point<Real, G, A, M, T>), dim3(grid),            \
+    a.cpl = cpl;
+    if (cpl.n_pairs <= 0) phases &= ~PHASE_COUPLE;
+#define AXS3D_POINT_CASE(G, A, M, T, C)                                       \
+    case (A) | ((M) << 1) | ((T) << 2) | ((G) << 3) | ((C) << 4):            \
+        hipLaunchKernelGGL((k_point<Real, G, A, M, T, C>), dim3(grid),         \
                            dim3(block), 0, st, a, eb, stiff);                 \
         break;
     switch (phases) {
-        AXS3D_POINT_CASE(0, 1, 0, 0)
-        AXS3D_POINT_CASE(0, 0, 1, 0)
-        AXS3D_POINT_CASE(0, 0, 0, 1)
-        AXS3D_POINT_CASE(0, 1, 1, 1)
-        AXS3D_POINT_CASE(1, 1, 1, 1)
+        AXS3D_POINT_CASE(0, 1, 0, 0, 0)
+        AXS3D_POINT_CASE(0, 0, 1, 0, 0)
+        AXS3D_POINT_CASE(0, 0, 0, 1, 0)
+        AXS3D_POINT_CASE(0, 0, 0, 0, 1)
+        AXS3D_POINT_CASE(0, 1, 1, 1, 0)
+        AXS3D_POINT_CASE(1, 1, 1, 1, 0)
+        AXS3D_POINT_CASE(1, 1, 1, 1, 1)
         default:
             break;
     }
@@ -549,7 +573,8 @@ void launch_record(hipStream_t st, int nrec, const int32_t *rec_elem,
                                         const Real *, Real *);                \
     template void launch_point<Real>(hipStream_t, int, const PointGeom &,     \
                                      const Real *, Real *, double *,          \
-                                     double *, double *, double, int *);
+                                     double *, double *, double, int *,       \
+                                     const CouplingGeom &);
 
 AXS3D_INSTANTIATE(float)
 AXS3D_INSTANTIATE(double)

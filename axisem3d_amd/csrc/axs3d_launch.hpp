@@ -10,10 +10,43 @@ namespace axs3d {
 
 // point-update phase bits
 enum PointPhase {
-    PHASE_AXIAL = 1,
+    PHASE_AXIAL = 1,     // axis (and, for fluid points, free surface)
     PHASE_MASS = 2,
     PHASE_MARCH = 4,
-    PHASE_GATHER = 8
+    PHASE_GATHER = 8,
+    PHASE_COUPLE = 16    // solid-fluid coupling
+};
+
+// Solid-fluid coupling pairs (one solid and one fluid point at the same
+// GLL position, same Nr).
+struct CouplingGeom {
+    int n_pairs;
+    const int32_t *solid_pair;     // [solid point] pair or -1
+    const int32_t *fluid_pair;     // [fluid point] pair or -1
+    const uint32_t *soff;          // [pair] solid mode offset
+    const uint32_t *foff;          // [pair] fluid mode offset
+    const double *n;               // [pair][2] ns, nz (fluid -> solid)
+    const double *n_m;             // [pair][2] ns / m_fluid, nz / m_fluid
+    double *cstiff;                // [fluid mode] fluid stiffness after
+                                   // the solid -> fluid term
+};
+
+// Fluid elements (acoustic, 1D), element-modes of the Fourier path.
+template <typename Real>
+struct FluidArgs {
+    int n_em;
+    const EMItem *em_list;
+    const int32_t *elem_points;
+    const uint8_t *elem_axial;
+    const Real *elem_ops;          // [e][10][25] as for solids
+    const Real *elem_k;            // [e][25] K = 1 / rho
+    const uint32_t *point_off;
+    const int32_t *point_nu1;
+    const double *chi;             // fluid displacement (potential)
+    const uint32_t *elem_eb_off;   // element-local force offset, modes
+    const int32_t *elem_nu1;
+    Real *eb;                      // [e][ipnt][alpha] complex
+    const Real *gtab;
 };
 
 // Device operands of the solid element kernels.
@@ -120,7 +153,22 @@ void launch_assemble(hipStream_t st, const PointGeom &g, const Real *eb,
 template <typename Real>
 void launch_point(hipStream_t st, int phases, const PointGeom &g,
                   const Real *eb, Real *stiff, double *displ, double *veloc,
-                  double *accel, double dt, int *flag);
+                  double *accel, double dt, int *flag,
+                  const CouplingGeom &cpl);
+
+template <typename Real>
+void launch_fluid_stiff_fr(hipStream_t st, const FluidArgs<Real> &a);
+
+template <typename Real>
+void launch_fluid_assemble(hipStream_t st, const PointGeom &g,
+                           const Real *eb, Real *stiff);
+
+// point_axial of a fluid PointGeom: bit 0 axis, bit 1 free surface
+template <typename Real>
+void launch_fluid_point(hipStream_t st, int phases, const PointGeom &g,
+                        const Real *eb, Real *stiff, double *chi,
+                        double *chi_v, double *chi_a, double dt, int *flag,
+                        const CouplingGeom &cpl, const double *solid_displ);
 
 void launch_record(hipStream_t st, int nrec, const int32_t *rec_elem,
                    const double *rec_w, const double *rec_phi,

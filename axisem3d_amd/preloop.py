@@ -340,8 +340,13 @@ class DomainSpec:
         return self.elem_points.shape[0]
 
     def n_updates_per_step(self):
-        """N_upd = sum over points of (Nr/2 + 1)."""
-        return int(np.sum(self.solid_nr // 2 + 1))
+        """N_upd = sum over points of (Nr/2 + 1), solid and fluid points
+        counted separately."""
+        n = int(np.sum(self.solid_nr // 2 + 1))
+        fnr = getattr(self, "fluid_nr", None)
+        if fnr is not None:
+            n += int(np.sum(np.asarray(fnr) // 2 + 1))
+        return n
 
 
 def _compact_points(elem_points_grid, npoint_grid):
@@ -458,7 +463,147 @@ def build_domain(mesh: StructuredMesh,
     spec._mesh = mesh
     spec._gq = gq
     spec._vp1d = vp
+    spec._rho1d = rho
     spec._courant = courant
+    spec._elem_of_mesh = np.arange(E)
+    return spec
+
+
+def _edge_normals(J, axial, edge, w_gll, w_glj, sz):
+    """Integrated outward normals of one element edge at its 5 points
+    (Quad::computeNormal, Quad.cpp:438-484): the edge tangent rotated
+    outward (length = edge Jacobian), times the quadrature weight along
+    the edge and the radius s (axial element: GLJ weights and the
+    L'Hospital factor ds/dxi on the axis, s / (1 + xi) off it).  J (25,
+    2, 2) = d(s,z)/d(xi,eta); returns (points (5,), n (5, 2))."""
+    pts = {0: [0, 5, 10, 15, 20], 1: [20, 21, 22, 23, 24],
+           2: [24, 19, 14, 9, 4], 3: [4, 3, 2, 1, 0]}[edge]
+    xi = spectral.xieta_element(axial)[0]
+    out = np.zeros((5, 2))
+    for k, p in enumerate(pts):
+        ipol, jpol = p // NPED, p % NPED
+        if edge in (0, 2):     # along xi
+            ts, tz = J[p, 0, 0], J[p, 1, 0]
+            n = np.array([tz, -ts]) if edge == 0 else np.array([-tz, ts])
+            n = n * (w_glj[ipol] if axial else w_gll[ipol])
+        else:                  # along eta
+            ts, tz = J[p, 0, 1], J[p, 1, 1]
+            n = np.array([tz, -ts]) if edge == 1 else np.array([-tz, ts])
+            n = n * w_gll[jpol]
+        if axial:
+            n = n * (J[p, 0, 0] if ipol == 0 else sz[0, p] / (1.0 + xi[p]))
+        else:
+            n = n * sz[0, p]
+        out[k] = n
+    return pts, out
+
+
+def make_fluid(spec, fluid_elem):
+    """Turn the mesh elements in fluid_elem (bool, per mesh element) into
+    fluid elements (acoustic, displacement potential; Acoustic K = 1/rho,
+    Material.cpp:350-360; fluid mass IF / (rho vp^2), Material.cpp:
+    318-333).  GLL points shared by solid and fluid elements become a
+    solid point and a fluid point joined by a solid-fluid coupling with
+    the assembled normal pointing from fluid to solid (Quad.cpp:123-140).
+    Fluid points on the top of the mesh form the fluid free surface.
+    Call on a fresh build_domain spec, before sources, receivers and 3D
+    models; solid arrays are renumbered."""
+    mesh = spec._mesh
+    gq = spec._gq
+    fl = np.asarray(fluid_elem, dtype=bool)
+    so = ~fl
+    E = mesh.nelem
+    ep_grid = mesh.element_points()
+    remap_all, _ = _compact_points(ep_grid, mesh.npoint_grid)
+    nr_grid = np.zeros(mesh.npoint_grid, dtype=np.int64)
+    nr_grid[ep_grid.reshape(-1)] = spec.solid_nr[
+        remap_all[ep_grid.reshape(-1)]]
+    sol_ids = np.unique(ep_grid[so])
+    flu_ids = np.unique(ep_grid[fl])
+    rs = -np.ones(mesh.npoint_grid, dtype=np.int64)
+    rs[sol_ids] = np.arange(len(sol_ids))
+    rf = -np.ones(mesh.npoint_grid, dtype=np.int64)
+    rf[flu_ids] = np.arange(len(flu_ids))
+    ifact = gq["ifact"]
+    rho, vp = spec._rho1d, spec._vp1d
+    axial = mesh.is_axial()
+    # solid part
+    se = np.nonzero(so)[0]
+    s_points = rs[ep_grid[se]].astype(np.int32)
+    s_mass = np.zeros(len(sol_ids))
+    np.add.at(s_mass, s_points.reshape(-1), (rho[se] * ifact[se]).reshape(-1))
+    sz_grid = np.zeros((mesh.npoint_grid, 2))
+    sz_grid[ep_grid.reshape(-1)] = gq["sz"].transpose(0, 2, 1).reshape(-1, 2)
+    # fluid part
+    fe = np.nonzero(fl)[0]
+    f_points = rf[ep_grid[fe]].astype(np.int32)
+    f_mass = np.zeros(len(flu_ids))
+    np.add.at(f_mass, f_points.reshape(-1),
+              (ifact[fe] / (rho[fe] * vp[fe] ** 2)).reshape(-1))
+    # axis
+    s_axial = np.unique(s_points[axial[se]][:, :NPED].reshape(-1)) \
+        if np.any(axial[se]) else np.zeros(0, np.int64)
+    f_axial = np.unique(f_points[axial[fe]][:, :NPED].reshape(-1)) \
+        if np.any(axial[fe]) else np.zeros(0, np.int64)
+    # interface normals, accumulated per grid point (fluid side + n/2,
+    # solid side - n/2, both pointing from fluid to solid)
+    ix, iz = mesh.element_ix_iz()
+    cell = -np.ones((mesh.nx, mesh.nz), dtype=np.int64)
+    cell[ix, iz] = np.arange(E)
+    w_gll = spectral.weights_1d(False)
+    w_glj = spectral.weights_1d(True)
+    nsum = {}
+    for e in range(E):
+        for edge, (dx, dz) in ((0, (0, -1)), (1, (1, 0)), (2, (0, 1)),
+                               (3, (-1, 0))):
+            jx, jz = ix[e] + dx, iz[e] + dz
+            if not (0 <= jx < mesh.nx and 0 <= jz < mesh.nz):
+                continue
+            if fl[cell[jx, jz]] == fl[e]:
+                continue
+            pts, n = _edge_normals(gq["J"][e], bool(axial[e]), edge, w_gll,
+                                   w_glj, gq["sz"][e])
+            sign = 0.5 if fl[e] else -0.5
+            for k, p in enumerate(pts):
+                g = int(ep_grid[e, p])
+                nsum[g] = nsum.get(g, np.zeros(2)) + sign * n[k]
+    sf_grid = np.array(sorted(nsum.keys()), dtype=np.int64)
+    spec.sf_solid = rs[sf_grid].astype(np.int32)
+    spec.sf_fluid = rf[sf_grid].astype(np.int32)
+    spec.sf_normal = np.array([nsum[g] for g in sf_grid]).reshape(-1, 2)
+    assert (spec.sf_solid >= 0).all() and (spec.sf_fluid >= 0).all()
+    # fluid free surface: fluid points on the top row of the mesh
+    top = ep_grid[fe][:, [4, 9, 14, 19, 24]][iz[fe] == mesh.nz - 1]
+    spec.fluid_surface_points = np.unique(rf[top.reshape(-1)]).astype(
+        np.int32)
+    # fluid arrays
+    spec.fluid_nr = nr_grid[flu_ids].astype(np.int32)
+    spec.fluid_mass = f_mass
+    spec.fluid_elem_points = f_points
+    spec.fluid_elem_axial = axial[fe].astype(np.uint8)
+    spec.fluid_elem_gq = np.ascontiguousarray(spec.elem_gq[
+        spec._elem_of_mesh[fe]])
+    spec.fluid_elem_nr = spec.fluid_nr[f_points].max(axis=1).astype(np.int32)
+    spec.fluid_elem_k = np.ascontiguousarray(1.0 / rho[fe])
+    spec.fluid_axial_points = f_axial.astype(np.int32)
+    # solid arrays renumbered
+    old = spec._elem_of_mesh[se]
+    spec.solid_nr = nr_grid[sol_ids].astype(np.int32)
+    spec.solid_mass = s_mass
+    spec.solid_sz = sz_grid[sol_ids]
+    spec.elem_points = s_points
+    spec.elem_axial = axial[se].astype(np.uint8)
+    spec.elem_gq = np.ascontiguousarray(spec.elem_gq[old])
+    spec.elem_nr = spec.solid_nr[s_points].max(axis=1).astype(np.int32)
+    spec.elem_iso = np.ascontiguousarray(spec.elem_iso[old])
+    spec.axial_points = s_axial.astype(np.int32)
+    spec._gq = {k: (v[se] if isinstance(v, np.ndarray) and v.shape[0] == E
+                    else v) for k, v in gq.items()}
+    spec._vp1d = vp[se]
+    spec._rho1d = rho[se]
+    m = -np.ones(E, dtype=np.int64)
+    m[se] = np.arange(len(se))
+    spec._elem_of_mesh = m
     return spec
 
 
@@ -510,8 +655,11 @@ def add_axial_moment_source(spec: DomainSpec, depth_z: float, mt6,
     t0 = floor(t_start / dt) dt (``main.cpp:574-602``); hdur >= 5 dt
     (``STF.cpp:32``)."""
     mesh = spec._mesh
-    elem, xi, eta = locate(mesh, 0.0, depth_z)
-    w = inplane_factor(mesh, elem, xi, eta)
+    elem_m, xi, eta = locate(mesh, 0.0, depth_z)
+    w = inplane_factor(mesh, elem_m, xi, eta)
+    elem = int(spec._elem_of_mesh[elem_m])
+    if elem < 0:
+        raise ValueError("moment source inside a fluid element")
     nu1 = int(spec.elem_nr[elem]) // 2 + 1
     spec.src_elem = int(elem)
     spec.src_pattern = moment_pattern_axial(mt6, w, nu1)
@@ -531,7 +679,9 @@ def add_receivers(spec: DomainSpec, s_list, z_list, phi_list):
     elems, ws = [], []
     for s, z in zip(s_list, z_list):
         e, xi, eta = locate(mesh, s, z)
-        elems.append(e)
+        if spec._elem_of_mesh[e] < 0:
+            raise ValueError("receiver inside a fluid element")
+        elems.append(int(spec._elem_of_mesh[e]))
         ws.append(inplane_factor(mesh, e, xi, eta))
     spec.rec_elem = np.array(elems, dtype=np.int32)
     spec.rec_weights = np.array(ws, dtype=np.float64)

@@ -37,6 +37,8 @@ class GpuSolver:
         n = C.c_int64()
         self._chk(self.lib.axs3d_num_modes(self.ctx, C.byref(n)))
         self.nmodes = n.value
+        self._chk(self.lib.axs3d_num_fluid_modes(self.ctx, C.byref(n)))
+        self.fnmodes = n.value
         self.step = 0
 
     # ------------------------------------------------------------------
@@ -107,6 +109,36 @@ class GpuSolver:
             elif getattr(s, "_upload_3d", None) is not None:
                 # large domains stream their 3D operands in chunks
                 s._upload_3d(self)
+        if getattr(s, "fluid_nr", None) is not None and len(s.fluid_nr):
+            fnr = _c(s.fluid_nr, np.int32)
+            fm = _c(s.fluid_mass, np.float64)
+            self._chk(L.axs3d_set_fluid_points(ctx, len(fnr), fnr.ctypes.data,
+                                               fm.ctypes.data),
+                      "set_fluid_points")
+            fp = _c(s.fluid_elem_points, np.int32)
+            fa = _c(s.fluid_elem_axial, np.uint8)
+            fg = _c(s.fluid_elem_gq, np.float64)
+            self._chk(L.axs3d_set_fluid_elements(
+                ctx, fp.shape[0], fp.ctypes.data, fa.ctypes.data,
+                fg.ctypes.data), "set_fluid_elements")
+            fk = _c(s.fluid_elem_k, np.float64)
+            self._chk(L.axs3d_set_fluid_k1d(ctx, fk.ctypes.data),
+                      "set_fluid_k1d")
+            ax = _c(s.fluid_axial_points, np.int32)
+            self._chk(L.axs3d_set_fluid_axial_points(ctx, len(ax),
+                                                     ax.ctypes.data),
+                      "set_fluid_axial")
+            su = _c(s.fluid_surface_points, np.int32)
+            self._chk(L.axs3d_set_fluid_surface_points(ctx, len(su),
+                                                       su.ctypes.data),
+                      "set_fluid_surface")
+            cs = _c(s.sf_solid, np.int32)
+            cf = _c(s.sf_fluid, np.int32)
+            nn = _c(np.concatenate([s.sf_normal, s.sf_normal], axis=1),
+                    np.float64)
+            self._chk(L.axs3d_set_solid_fluid_coupling(
+                ctx, len(cs), cs.ctypes.data, cf.ctypes.data, nn.ctypes.data),
+                "set_coupling")
         axp = _c(s.axial_points, np.int32)
         self._chk(L.axs3d_set_axial_points(ctx, len(axp), axp.ctypes.data),
                   "set_axial")
@@ -184,17 +216,24 @@ class GpuSolver:
 
     # ------------------------------------------------------- fields
     _FIELDS = {"displ": capi.FIELD_DISPL, "veloc": capi.FIELD_VELOC,
-               "accel": capi.FIELD_ACCEL, "stiff": capi.FIELD_STIFF}
+               "accel": capi.FIELD_ACCEL, "stiff": capi.FIELD_STIFF,
+               "fdispl": capi.FIELD_FLUID_DISPL,
+               "fveloc": capi.FIELD_FLUID_VELOC,
+               "faccel": capi.FIELD_FLUID_ACCEL,
+               "fstiff": capi.FIELD_FLUID_STIFF}
+
+    def _shape(self, name):
+        return (self.fnmodes,) if name.startswith("f") else (self.nmodes, 3)
 
     def get_field(self, name):
-        out = np.zeros((self.nmodes, 3), dtype=np.complex128)
+        out = np.zeros(self._shape(name), dtype=np.complex128)
         self._chk(self.lib.axs3d_get_field(self.ctx, self._FIELDS[name],
                                            out.ctypes.data))
         return out
 
     def set_field(self, name, values):
         v = _c(values, np.complex128)
-        if v.shape != (self.nmodes, 3):
+        if v.shape != self._shape(name):
             raise ValueError("field shape mismatch")
         self._chk(self.lib.axs3d_set_field(self.ctx, self._FIELDS[name],
                                            v.ctypes.data))

@@ -126,6 +126,12 @@ def xieta_element(axial: bool) -> np.ndarray:
     return np.stack([xi, eta])
 
 
+def weights_1d(axial: bool) -> np.ndarray:
+    """(5,) GLJ (axial) or GLL quadrature weights."""
+    _, _, w_gll, w_glj, _, _ = tables()
+    return (w_glj if axial else w_gll).copy()
+
+
 def weights_element(axial: bool) -> np.ndarray:
     """(25,) tensor-product weights (``spectrals.hpp:319-327``)."""
     _, _, w_gll, w_glj, _, _ = tables()

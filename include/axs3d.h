@@ -48,7 +48,13 @@ enum axs3d_field {
     AXS3D_FIELD_DISPL = 0,
     AXS3D_FIELD_VELOC = 1,
     AXS3D_FIELD_ACCEL = 2,
-    AXS3D_FIELD_STIFF = 3
+    AXS3D_FIELD_STIFF = 3,
+    /* fluid points: displacement potential chi and its derivatives,
+     * complex [fluid point offset + alpha] */
+    AXS3D_FIELD_FLUID_DISPL = 4,
+    AXS3D_FIELD_FLUID_VELOC = 5,
+    AXS3D_FIELD_FLUID_ACCEL = 6,
+    AXS3D_FIELD_FLUID_STIFF = 7
 };
 
 enum axs3d_kernel_id {
@@ -57,7 +63,9 @@ enum axs3d_kernel_id {
     AXS3D_KERNEL_SOURCE = 2,
     AXS3D_KERNEL_RECORD = 3,
     AXS3D_KERNEL_STIFFNESS_3D = 4,/* cardinal-path (3D) element stiffness */
-    AXS3D_NUM_KERNELS = 5
+    AXS3D_KERNEL_FLUID_STIFFNESS = 5, /* fluid element stiffness       */
+    AXS3D_KERNEL_FLUID_POINT = 6, /* fluid coupling/BC/mass/Newmark     */
+    AXS3D_NUM_KERNELS = 7
 };
 
 /* Context.  Replaces the construction of Domain + NewmarkTimeScheme in
@@ -107,6 +115,45 @@ int axs3d_set_element_kinds(axs3d_ctx *ctx, const uint8_t *kind);
  * Property<N> 3D data Nr x 25, src/core/element/common/Property.hpp:39-42). */
 int axs3d_set_solid_iso3d(axs3d_ctx *ctx, int e_begin, int e_end,
                           const double *lam_mu);
+
+/* Fluid points: Nr and assembled mass (integral factor / kappa) per
+ * point, as for solid points (GLLPoint::release / FluidPoint,
+ * src/preloop/se_model/gll/GLLPoint.cpp:78-112; Material::getMass,
+ * Material.cpp:308-346). */
+int axs3d_set_fluid_points(axs3d_ctx *ctx, int n, const int32_t *nr,
+                           const double *mass);
+
+/* Fluid elements (FluidElement, src/core/element/FluidElement.cpp):
+ * points [n][25] -> fluid point, axial flags, GradQuad operands
+ * [n][6][25] as for axs3d_set_solid_elements. */
+int axs3d_set_fluid_elements(axs3d_ctx *ctx, int n, const int32_t *points,
+                             const uint8_t *axial, const double *gq);
+
+/* 1D acoustic medium of the fluid elements: K = 1 / rho [n][25]
+ * (Acoustic 1D constructor, Acoustic.hpp:20-25, from
+ * Material::createAcoustic, Material.cpp:350-360). */
+int axs3d_set_fluid_k1d(axs3d_ctx *ctx, const double *k);
+
+/* Fluid points on the axis (AxialBoundary::addPoint, fluid) and on the
+ * fluid free surface (FluidSurfaceBoundary::addPoint,
+ * src/core/boundary/surface/FluidSurfaceBoundary.hpp). */
+int axs3d_set_fluid_axial_points(axs3d_ctx *ctx, int n,
+                                 const int32_t *points);
+int axs3d_set_fluid_surface_points(axs3d_ctx *ctx, int n,
+                                   const int32_t *points);
+
+/* Solid-fluid coupling pairs: solid point, fluid point (same Nr) and
+ * normals [n][4] = ns, nz unassembled, ns, nz assembled (pointing from
+ * fluid to solid); the fluid mass is the fluid point's.  Replaces the
+ * SolidFluidCoupling1D constructor (src/core/boundary/solid_fluid/
+ * SolidFluidCoupling1D.hpp:18-24, GLLPoint.cpp:114-150). */
+int axs3d_set_solid_fluid_coupling(axs3d_ctx *ctx, int n,
+                                   const int32_t *solid_points,
+                                   const int32_t *fluid_points,
+                                   const double *normals);
+
+/* Number of fluid point-modes (length of a fluid field / 2). */
+int axs3d_num_fluid_modes(const axs3d_ctx *ctx, int64_t *nmodes);
 
 /* Material type per element: 0 isotropic, 1 transversely isotropic
  * (radial axis), 2 fully anisotropic; the choice Material::createElastic

@@ -120,6 +120,18 @@ void oracle_grad6(const double *gq_e, int axial, int alpha, const cplx *u,
     grad_1overs(&g, sA[2], e + 3 * N, 1);
 }
 
+/* computeGrad3, GradientQuadrature.hpp:163-175: p[25] -> p_i[3][25] =
+ * (dp/ds, i alpha p / s, dp/dz). */
+void oracle_grad3(const double *gq_e, int axial, int alpha, const cplx *p,
+                  cplx *pi) {
+    gq_t g = gq_of(gq_e, axial);
+    grad_dsdz(&g, p, pi + 0 * N, pi + 2 * N);
+    cplx sA[N];
+    cplx ai = (double)alpha * I;
+    for (int k = 0; k < N; k++) sA[k] = ai * p[k];
+    grad_1overs(&g, sA, pi + 1 * N, 0);
+}
+
 /* computeQuad_dSdZ, GradientQuadrature.hpp:463-478:
  * X = dFdS.*dzdeta + dFdZ.*dsdeta; Y = dFdS.*dzdxii + dFdZ.*dsdxii;
  * F = G_xi * X + Y * GT_GLL. */
@@ -200,6 +212,25 @@ void oracle_quad6(const double *gq_e, int axial, int alpha, const cplx *sij,
         o[p] = g.inv_s[p] * g.ifact[p];
     }
     quad6_ops(axial, alpha, sij, fi, a, b, c, d, o);
+}
+
+/* computeQuad3, GradientQuadrature.hpp:234-252: q_i[3][25] -> q[25]. */
+void oracle_quad3(const double *gq_e, int axial, int alpha, const cplx *qi,
+                  cplx *q) {
+    gq_t g = gq_of(gq_e, axial);
+    double a[N], b[N], c[N], d[N], o[N];
+    for (int p = 0; p < N; p++) {
+        a[p] = g.dsdxii[p] * g.ifact[p];
+        b[p] = g.dsdeta[p] * g.ifact[p];
+        c[p] = g.dzdxii[p] * g.ifact[p];
+        d[p] = g.dzdeta[p] * g.ifact[p];
+        o[p] = g.inv_s[p] * g.ifact[p];
+    }
+    quad_dsdz(axial, qi + 0 * N, qi + 2 * N, q, a, b, c, d);
+    cplx sA[N];
+    cplx ai = (double)alpha * I;
+    for (int p = 0; p < N; p++) sA[p] = ai * qi[1 * N + p];
+    quad_1overs(axial, sA, q, o, c, d);
 }
 
 /* computeQuad6_NoIntegration, GradientQuadrature.hpp:363-390 */
@@ -795,6 +826,134 @@ void oracle_record(const oracle_domain *d, const cplx *displ, double *out) {
 /* NewmarkTimeScheme::solve (NewmarkTimeScheme.cpp:41-108), phase order:
  * sources -> stiffness -> BC before (none here) -> record -> stability ->
  * BC after (axial) -> stiff to accel -> Newmark. */
+/* FluidElement::computeStiff, 1D acoustic (Fourier path), no PRT
+ * (FluidElement.cpp:91-148): collectDisplFromPoints (zero-padded,
+ * FluidPoint.hpp:82-94) -> computeGrad3 -> Acoustic::strainToStress
+ * (K p_i, Acoustic.hpp:80-92) -> computeQuad3 -> subtracting scatter
+ * (FluidPoint.hpp:97-103). */
+void oracle_fluid_element_stiff(const oracle_domain *d, int e,
+                                const cplx *chi, cplx *fstiff) {
+    const int32_t *pts = d->felem_points + (int64_t)e * N;
+    int nu1 = d->felem_nr[e] / 2 + 1;
+    const double *gq_e = d->felem_gq + (int64_t)e * 6 * N;
+    const double *K = d->felem_k + (int64_t)e * N;
+    int axial = d->felem_axial[e];
+    cplx p[N], pi[3 * N], q[N];
+    for (int alpha = 0; alpha < nu1; alpha++) {
+        for (int k = 0; k < N; k++) {
+            int ip = pts[k];
+            int pnu1 = d->fpoint_nr[ip] / 2 + 1;
+            p[k] = alpha < pnu1 ? chi[d->foffsets[ip] + alpha] : 0.;
+        }
+        oracle_grad3(gq_e, axial, alpha, p, pi);
+        for (int c = 0; c < 3; c++)
+            for (int k = 0; k < N; k++) pi[c * N + k] *= K[k];
+        oracle_quad3(gq_e, axial, alpha, pi, q);
+        for (int k = 0; k < N; k++) {
+            int ip = pts[k];
+            int pnu1 = d->fpoint_nr[ip] / 2 + 1;
+            if (alpha < pnu1) fstiff[d->foffsets[ip] + alpha] -= q[k];
+        }
+    }
+}
+
+/* SolidFluidBoundary::apply -> SolidFluidCoupling::apply, 1D coupling
+ * (SolidFluidCoupling.cpp:30-37, SolidFluidCoupling1D.hpp:26-38), per
+ * pair and in this order: fluid stiff += ns u_s + nz u_z; solid stiff
+ * (s, z) -= (n / m_fluid) fluid stiff.  One rank: the unassembled and
+ * assembled normals coincide. */
+void oracle_coupling(const oracle_domain *d, const cplx *displ, cplx *stiff,
+                     cplx *fstiff) {
+    for (int k = 0; k < d->n_sf; k++) {
+        int sp = d->sf_solid[k], fp = d->sf_fluid[k];
+        int nu1 = d->point_nr[sp] / 2 + 1;
+        double ns = d->sf_normal[2 * k], nz = d->sf_normal[2 * k + 1];
+        double mf = d->fpoint_mass[fp];
+        double ns_m = ns / mf, nz_m = nz / mf;
+        const cplx *u = displ + d->offsets[sp] * 3;
+        cplx *s = stiff + d->offsets[sp] * 3;
+        cplx *f = fstiff + d->foffsets[fp];
+        for (int a = 0; a < nu1; a++) {
+            f[a] += ns * u[a * 3 + 0];
+            f[a] += nz * u[a * 3 + 2];
+        }
+        for (int a = 0; a < nu1; a++) {
+            s[a * 3 + 0] -= ns_m * f[a];
+            s[a * 3 + 2] -= nz_m * f[a];
+        }
+    }
+}
+
+/* AxialBoundary::apply, fluid part (AxialBoundary.cpp:44-47: orders >= 1
+ * zero), then FluidSurfaceBoundary::apply (FluidSurfaceBoundary.cpp:
+ * 20-25: stiff zero). */
+void oracle_fluid_bc(const oracle_domain *d, cplx *fstiff) {
+    for (int k = 0; k < d->n_faxial; k++) {
+        int ip = d->faxial_points[k];
+        int nu1 = d->fpoint_nr[ip] / 2 + 1;
+        for (int a = 1; a < nu1; a++) fstiff[d->foffsets[ip] + a] = 0.;
+    }
+    for (int k = 0; k < d->n_fsurf; k++) {
+        int ip = d->fsurf_points[k];
+        int nu1 = d->fpoint_nr[ip] / 2 + 1;
+        for (int a = 0; a < nu1; a++) fstiff[d->foffsets[ip] + a] = 0.;
+    }
+}
+
+/* FluidPoint::computeStiffToAccel (FluidPoint.cpp:28-53): Nyquist
+ * imaginary part zero (even Nr), divide by the mass (Mass1D). */
+void oracle_fluid_stiff_to_accel(const oracle_domain *d, cplx *fstiff) {
+    for (int ip = 0; ip < d->n_fpoints; ip++) {
+        int nr = d->fpoint_nr[ip];
+        int nu1 = nr / 2 + 1;
+        cplx *s = fstiff + d->foffsets[ip];
+        if (nr % 2 == 0) s[nu1 - 1] = creal(s[nu1 - 1]);
+        double inv = 1. / d->fpoint_mass[ip];
+        for (int k = 0; k < nu1; k++) s[k] *= inv;
+    }
+}
+
+/* NewmarkTimeScheme::solve (NewmarkTimeScheme.cpp:40-104) with solid and
+ * fluid points: source -> stiffness (solid, fluid) -> solid-fluid
+ * coupling -> record -> axial and fluid-surface boundaries -> stiff to
+ * accel -> Newmark on solid and fluid points. */
+int oracle_run_coupled(const oracle_domain *d, int step_begin, int nsteps,
+                       cplx *displ, cplx *veloc, cplx *accel, cplx *stiff,
+                       cplx *fdispl, cplx *fveloc, cplx *faccel,
+                       cplx *fstiff, double *seismo) {
+    int64_t nmf = d->n_fpoints > 0 ? d->foffsets[d->n_fpoints] : 0;
+    for (int ts = step_begin; ts < step_begin + nsteps; ts++) {
+        if (d->stf) oracle_apply_moment_source(d, d->stf[ts], stiff);
+        for (int e = 0; e < d->n_elem; e++)
+            oracle_solid_element_stiff(d, e, displ, stiff);
+        for (int e = 0; e < d->n_felem; e++)
+            oracle_fluid_element_stiff(d, e, fdispl, fstiff);
+        oracle_coupling(d, displ, stiff, fstiff);
+        if (seismo && d->n_rec > 0)
+            oracle_record(d, displ, seismo + (int64_t)(ts - step_begin) *
+                                                 d->n_rec * 3);
+        for (int64_t k = 0; k < d->offsets[d->n_points] * 3; k++)
+            if (!isfinite(creal(displ[k])) || !isfinite(cimag(displ[k])))
+                return 1;
+        oracle_axial_solid(d, stiff);
+        oracle_fluid_bc(d, fstiff);
+        oracle_stiff_to_accel(d, stiff);
+        oracle_fluid_stiff_to_accel(d, fstiff);
+        oracle_newmark(d->offsets[d->n_points], d->dt, stiff, displ, veloc,
+                       accel);
+        /* one component per mode: the 3-component routine on nmf / 3
+         * would not cover a remainder, so march the fluid directly */
+        const double hdt = .5 * d->dt, hdt2 = .5 * d->dt * d->dt;
+        for (int64_t k = 0; k < nmf; k++) {
+            fveloc[k] += hdt * (faccel[k] + fstiff[k]);
+            faccel[k] = fstiff[k];
+            fdispl[k] += d->dt * fveloc[k] + hdt2 * faccel[k];
+            fstiff[k] = 0.;
+        }
+    }
+    return 0;
+}
+
 int oracle_run(const oracle_domain *d, int step_begin, int nsteps,
                cplx *displ, cplx *veloc, cplx *accel, cplx *stiff,
                double *seismo) {

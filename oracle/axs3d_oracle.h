@@ -58,6 +58,26 @@ typedef struct {
     const int64_t *prop_off;
     const double *elem_theta;     /* [e][25] colatitude (spherical), or
                                      NULL (Cartesian)                    */
+    /* fluid (displacement potential, 1 component per mode) */
+    int n_fpoints;
+    const int32_t *fpoint_nr;
+    const double *fpoint_mass;    /* assembled IF / kappa                */
+    const int64_t *foffsets;      /* prefix sum of Nr/2+1                */
+    int n_felem;
+    const int32_t *felem_points;  /* [e][25] -> fluid point              */
+    const int32_t *felem_nr;
+    const uint8_t *felem_axial;
+    const double *felem_gq;       /* [e][6][25]                          */
+    const double *felem_k;        /* [e][25] K = 1 / rho                 */
+    int n_faxial;
+    const int32_t *faxial_points;
+    int n_fsurf;
+    const int32_t *fsurf_points;
+    /* solid-fluid coupling */
+    int n_sf;
+    const int32_t *sf_solid;
+    const int32_t *sf_fluid;
+    const double *sf_normal;      /* [pair][2] (ns, nz), fluid -> solid   */
 } oracle_domain;
 
 void oracle_set_gmat(const double *g_gll, const double *g_glj);
@@ -74,6 +94,24 @@ void oracle_material_stress(int mt, const double *par,
                             const double complex *e, double complex *s);
 void oracle_spz_to_rtz6(const double *theta, double complex *e);
 void oracle_rtz_to_spz6(const double *theta, double complex *s);
+void oracle_grad3(const double *gq_e, int axial, int alpha,
+                  const double complex *p, double complex *pi);
+void oracle_quad3(const double *gq_e, int axial, int alpha,
+                  const double complex *qi, double complex *q);
+void oracle_fluid_element_stiff(const oracle_domain *d, int e,
+                                const double complex *chi,
+                                double complex *fstiff);
+void oracle_coupling(const oracle_domain *d, const double complex *displ,
+                     double complex *stiff, double complex *fstiff);
+void oracle_fluid_bc(const oracle_domain *d, double complex *fstiff);
+void oracle_fluid_stiff_to_accel(const oracle_domain *d,
+                                 double complex *fstiff);
+int oracle_run_coupled(const oracle_domain *d, int step_begin, int nsteps,
+                       double complex *displ, double complex *veloc,
+                       double complex *accel, double complex *stiff,
+                       double complex *fdispl, double complex *fveloc,
+                       double complex *faccel, double complex *fstiff,
+                       double *seismo);
 void oracle_solid_element_stiff(const oracle_domain *d, int e,
                                 const double complex *displ,
                                 double complex *stiff);

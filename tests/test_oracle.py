@@ -477,3 +477,136 @@ def test_anisotropic_is_the_symmetric_voigt_matrix():
         M[iu] = c21[:, p]
         M = M + M.T - np.diag(np.diag(M))
         assert np.allclose(s1[:, p], M @ eps1[:, p], rtol=1e-13, atol=1e-13)
+
+
+# --------------------------------------------------------------- fluid
+@pytest.mark.parametrize("alpha", [0, 1, 3])
+def test_grad3_closed_form_off_axis(alpha):
+    ops, sz = _element(False)
+    s, z = sz
+    f, fs, fz = _poly(np.random.default_rng(20 + alpha), s, z)
+    p = f(s, z)
+    expect = np.array([fs(s, z), 1j * alpha * p / s, fz(s, z)])
+    got = oracle.grad3(ops, 0, alpha, p)
+    assert np.abs(got - expect).max() < 1e-9 * np.abs(expect).max()
+
+
+@pytest.mark.parametrize("alpha", [0, 1, 2])
+def test_grad3_closed_form_on_axis(alpha):
+    """Axis-regular potential: alpha = 0 any field, alpha >= 1 vanishing on
+    the axis (p = s g); the 1/s term takes the L'Hospital limit."""
+    ops, sz = _element(True)
+    s, z = sz
+    g, gs, gz = _poly(np.random.default_rng(30 + alpha), s, z, deg=3)
+    if alpha == 0:
+        p, ps, pz = g(s, z), gs(s, z), gz(s, z)
+    else:
+        p = s * g(s, z)
+        ps = g(s, z) + s * gs(s, z)
+        pz = s * gz(s, z)
+    on = s == 0.0
+    mid = np.where(on, 1j * alpha * ps, 1j * alpha * p / np.where(on, 1, s))
+    expect = np.array([ps, mid, pz])
+    got = oracle.grad3(ops, 1, alpha, p)
+    assert np.abs(got - expect).max() < 1e-9 * np.abs(expect).max()
+
+
+@pytest.mark.parametrize("axial", [False, True])
+@pytest.mark.parametrize("alpha", [0, 1, 3])
+def test_quad3_is_integral_weighted_transpose_of_grad3(axial, alpha):
+    ops, _ = _element(axial)
+    rng = np.random.default_rng(40 + alpha + 7 * axial)
+    q = rng.standard_normal((3, 25)) + 1j * rng.standard_normal((3, 25))
+    v = rng.standard_normal(25) + 1j * rng.standard_normal(25)
+    lhs = np.sum(oracle.quad3(ops, int(axial), alpha, q) * v)
+    rhs = np.sum(ops[5] * q * oracle.grad3(ops, int(axial), -alpha, v))
+    assert abs(lhs - rhs) < 1e-10 * abs(rhs)
+
+
+def _fluid_layer_spec(nsteps=2000, nx=4, nz=6):
+    mesh = preloop.StructuredMesh(np.linspace(0, 4e3, nx + 1),
+                                  np.linspace(0, 6e3, nz + 1))
+
+    def mat(r, rc):
+        inl = (rc > 2e3) & (rc < 4e3)
+        one = np.ones_like(r)
+        return (np.where(inl, 1000.0, 2600.0) * one,
+                np.where(inl, 1500.0, 6000.0) * one,
+                np.where(inl, 0.0, 3464.0) * one)
+    sp = preloop.build_domain(mesh, mat, lambda s, z: np.full(s.shape, 5.0),
+                              courant=0.4)
+    ix, iz = mesh.element_ix_iz()
+    zc = (iz + 0.5) * 1e3
+    preloop.make_fluid(sp, (zc > 2e3) & (zc < 4e3))
+    mt = np.array([1.0, 1.0, 1.0, 0.3, 0.0, 0.0]) * 1e15
+    preloop.add_axial_moment_source(sp, 1e3, mt, 1.628 / math.pi, 1.628,
+                                    nsteps)
+    preloop.add_receivers(sp, np.array([0.5e3]), np.array([5.5e3]),
+                          np.zeros(1))
+    return sp
+
+
+def test_solid_fluid_normals_integrate_the_interface():
+    """The assembled normals of a flat interface z = z0 point from fluid to
+    solid and sum to the integral of s ds over [0, width] (the axisymmetric
+    surface measure without the 2 pi)."""
+    sp = _fluid_layer_spec()
+    z = sp.solid_sz[sp.sf_solid][:, 1]
+    top = z > 3e3
+    assert np.allclose(sp.sf_normal[:, 0], 0.0, atol=1e-6)
+    assert np.isclose(sp.sf_normal[top, 1].sum(), 0.5 * 4e3 ** 2, rtol=1e-12)
+    assert np.isclose(sp.sf_normal[~top, 1].sum(), -0.5 * 4e3 ** 2,
+                      rtol=1e-12)
+
+
+def test_coupling_terms_are_adjoint():
+    """Solid -> fluid adds B^T u to the fluid stiffness, fluid -> solid
+    subtracts B (fluid stiff / m_fluid) from the solid one, with the same
+    B: <c, B^T u> = <B c, u> for any u, c.  A sign flipped on one side, a
+    missing 1/m_fluid or a swapped s/z normal fails."""
+    sp = _fluid_layer_spec(nsteps=1)
+    o = oracle.Oracle(sp)
+    rng = np.random.default_rng(21)
+    u = rng.standard_normal((o.nmodes, 3)) + 1j * rng.standard_normal(
+        (o.nmodes, 3))
+    c = rng.standard_normal(o.fnmodes) + 1j * rng.standard_normal(o.fnmodes)
+    mf = np.repeat(sp.fluid_mass, sp.fluid_nr // 2 + 1)
+    zs = np.zeros_like(u)
+    _, btu = o.coupling(u, zs, np.zeros_like(c))          # B^T u
+    bc, _ = o.coupling(zs, zs, mf * c)                   # -B c
+    lhs = np.vdot(c, btu)
+    rhs = np.vdot(-bc, u)
+    assert abs(lhs - rhs) < 1e-12 * abs(lhs)
+    # the fluid side of coupled points only
+    touched = np.zeros(o.fnmodes, bool)
+    for f in sp.sf_fluid:
+        touched[o.foffsets[f]:o.foffsets[f + 1]] = True
+    assert np.all(btu[~touched] == 0) and np.any(btu[touched] != 0)
+
+
+def test_coupled_energy_is_conserved():
+    """E = 1/2 u'^T M_s u' + 1/2 u^T K_s u + 1/2 chi'^T K_f chi' + 1/2
+    chi''^T M_f chi'' is the energy of the displacement-potential coupling
+    (it follows from the two equations of motion with the adjoint
+    coupling pair); after the source has stopped it stays constant up to
+    the bounded oscillation of the central-difference scheme, and the fluid
+    carries a substantial share of it."""
+    sp = _fluid_layer_spec(nsteps=6500)
+    o = oracle.Oracle(sp)
+    o.run(300, record=False)
+    ms = np.repeat(sp.solid_mass, sp.solid_nr // 2 + 1)[:, None]
+    mf = np.repeat(sp.fluid_mass, sp.fluid_nr // 2 + 1)
+    es, ef = [], []
+    for _ in range(12):
+        e_s = 0.5 * np.sum(ms * np.abs(o.veloc) ** 2) - 0.5 * np.real(
+            np.vdot(o.displ, o.stiffness(o.displ)))
+        e_f = -0.5 * np.real(np.vdot(o.fveloc, o.fluid_stiffness(o.fveloc))) \
+            + 0.5 * np.sum(mf * np.abs(o.faccel) ** 2)
+        es.append(e_s)
+        ef.append(e_f)
+        o.run(500, record=False)
+    tot = np.array(es) + np.array(ef)
+    # the unmodified energy of the central-difference scheme oscillates
+    # by a few per cent here; an inconsistent coupling grows or decays
+    assert (tot.max() - tot.min()) / tot.mean() < 0.1
+    assert np.mean(np.array(ef) / tot) > 0.02
