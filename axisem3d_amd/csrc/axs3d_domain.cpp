@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <sstream>
 
 #include "axs3d_launch.hpp"
@@ -431,11 +432,17 @@ void DomainBase::finalize() {
             if (hMType[e] < mt) n++;
         mTypeBegin3D[mt] = n;
     }
-    // FFT twiddle tables exp(-2 pi i t / Nr) and stage plans for every 3D
-    // Nr; the FFT takes radices 4, 2, 3, 5, 7, 11, 13 in that order of
-    // preference
-    std::vector<int64_t> twOff;
+    // In-place FFT plans, stage twiddle tables and slice positions for
+    // every 3D Nr: powers of 2 as radix 8 (one 4 or 2 for the remainder),
+    // then 3, 5, 7, 11, 13; stages in descending radix (decimation in
+    // frequency order), which keeps the twiddle tables short.  Stage t of
+    // span L_t (L_0 = Nr, L_{t+1} = m_t = L_t / R_t) has the twiddles
+    // exp(-2 pi i j / L_t), j < m_t.  After the inverse (DIF) transform
+    // slice s = s_0 + R_0 s_1 + R_0 R_1 s_2 + ... sits at position
+    // s_0 m_0 + s_1 m_1 + ...
+    std::vector<int64_t> twOff, posOff;
     std::vector<double> twTab;
+    std::vector<uint16_t> posTab;
     std::vector<uint32_t> plan;
     if (!list3d.empty()) {
         int nrmax = 0;
@@ -444,40 +451,63 @@ void DomainBase::finalize() {
             throw Error("Domain::finalize || Nr of a 3D element exceeds the "
                         "cardinal kernel's FFT capacity (832).");
         twOff.assign(nrmax + 1, -1);
+        posOff.assign(nrmax + 1, -1);
         plan.assign((size_t)(nrmax + 1) * CARD_PLAN_W, 0u);
         for (int32_t e : list3d) {
-            int nr = hElemNr[e];
+            const int nr = hElemNr[e];
             if (twOff[nr] >= 0) continue;
-            uint32_t *pl = plan.data() + (size_t)nr * CARD_PLAN_W;
-            int rem = nr, ns = 1, nst = 0;
-            while (rem > 1) {
-                int r = 0;
-                for (int c : {4, 2, 3, 5, 7, 11, 13})
-                    if (rem % c == 0) {
-                        r = c;
-                        break;
-                    }
-                if (r == 0)
-                    throw Error("Domain::finalize || Nr of a 3D element has "
-                                "a prime factor above 13 (use FFT lucky "
-                                "numbers).");
-                if (nst == CARD_MAX_STAGES)
-                    throw Error("Domain::finalize || Too many FFT stages.");
-                pl[1 + 3 * nst] = (uint32_t)r | ((uint32_t)ns << 8);
-                pl[2 + 3 * nst] =
-                    ns == 1 ? 0u
-                            : (uint32_t)((((uint64_t)1 << 32) + ns - 1) / ns);
-                pl[3 + 3 * nst] = (uint32_t)(nr / (ns * r));
-                nst++;
-                ns *= r;
-                rem /= r;
+            std::vector<int> radix;
+            int rem = nr, p2 = 0;
+            while (rem % 2 == 0) {
+                rem /= 2;
+                p2++;
             }
-            pl[0] = (uint32_t)nst;
+            for (int k = 0; k < p2 / 3; k++) radix.push_back(8);
+            if (p2 % 3 == 2) radix.push_back(4);
+            if (p2 % 3 == 1) radix.push_back(2);
+            for (int c : {3, 5, 7, 11, 13})
+                while (rem % c == 0) {
+                    radix.push_back(c);
+                    rem /= c;
+                }
+            if (rem != 1)
+                throw Error("Domain::finalize || Nr of a 3D element has "
+                            "a prime factor above 13 (use FFT lucky "
+                            "numbers).");
+            if ((int)radix.size() > CARD_MAX_STAGES)
+                throw Error("Domain::finalize || Too many FFT stages.");
+            std::sort(radix.begin(), radix.end(), std::greater<int>());
+            uint32_t *pl = plan.data() + (size_t)nr * CARD_PLAN_W;
             twOff[nr] = (int64_t)twTab.size() / 2;
-            for (int t = 0; t < nr; t++) {
-                double ang = -2. * M_PI * (double)t / (double)nr;
-                twTab.push_back(std::cos(ang));
-                twTab.push_back(std::sin(ang));
+            int span = nr, ntw = 0;
+            for (size_t st = 0; st < radix.size(); st++) {
+                const int r = radix[st], m = span / r;
+                pl[1 + 3 * st] = (uint32_t)r | ((uint32_t)m << 8);
+                pl[2 + 3 * st] =
+                    m == 1 ? 0u
+                           : (uint32_t)((((uint64_t)1 << 32) + m - 1) / m);
+                pl[3 + 3 * st] = (uint32_t)ntw;
+                for (int j = 0; j < m; j++) {
+                    const double ang = -2. * M_PI * (double)j / (double)span;
+                    twTab.push_back(std::cos(ang));
+                    twTab.push_back(std::sin(ang));
+                }
+                ntw += m;
+                span = m;
+            }
+            if (ntw > CARD_TW_CAP)
+                throw Error("Domain::finalize || FFT twiddle table exceeds "
+                            "the cardinal kernel's capacity.");
+            pl[0] = (uint32_t)radix.size() | ((uint32_t)ntw << 8);
+            posOff[nr] = (int64_t)posTab.size();
+            for (int sl = 0; sl < nr; sl++) {
+                int rest = sl, pos = 0, sp = nr;
+                for (int r : radix) {
+                    sp /= r;
+                    pos += (rest % r) * sp;
+                    rest /= r;
+                }
+                posTab.push_back((uint16_t)pos);
             }
         }
     }
@@ -649,9 +679,10 @@ void DomainBase::finalize() {
         up(dList3D, list3d.data(), list3d.size() * 4);
         up(dTwOff, twOff.data(), twOff.size() * 8);
         up(dPlan, plan.data(), plan.size() * 4);
-        dWork.alloc(3 * sizeof(int));
-        dWork.zero(st);
+        up(dPosOff, posOff.data(), posOff.size() * 8);
+        up(dPosTab, posTab.data(), posTab.size() * 2);
     }
+    hList3D.swap(list3d);
     hTwTab.swap(twTab);
     if (mNsls > 0) up(dAttABG, hAttABG.data(), hAttABG.size() * 8);
     const size_t fbytes = (size_t)mNumModes * 6 * sizeof(double);
@@ -866,9 +897,9 @@ public:
     CardArgs<Real> cardArgs() const {
         CardArgs<Real> c;
         c.s = stiffArgs();
+        c.s.em_list = dEM3D.as<EMItem>();
         c.n3d = mNum3D;
         c.list3d = dList3D.as<int32_t>();
-        c.work_counter = dWork.as<int>();
         c.elem_nr = dElemNr.as<int32_t>();
         c.slice_off = dSliceOff.as<int64_t>();
         c.prop3d = dProp3D.as<Real>();
@@ -877,8 +908,10 @@ public:
         c.twiddle = dTwiddle.as<cpx<Real>>();
         c.tw_off = dTwOff.as<int64_t>();
         c.plan = dPlan.as<uint32_t>();
+        c.slice_pos = dPosTab.as<uint16_t>();
+        c.pos_off = dPosOff.as<int64_t>();
         c.scratch = dCardScratch.as<Real>();
-        c.nu1max = mNu1Max3D;
+        c.scr_off = dScrOff.as<uint32_t>();
         c.nsls = mNsls;
         c.npa = mAttCG4 ? 4 : NPEM;
         c.abg = dAttABG.as<double>();
@@ -966,8 +999,9 @@ public:
         }
         if (mNum3D > 0) {
             tick(AXS3D_KERNEL_STIFFNESS_3D);
-            launch_solid_stiff_cd<Real>(mStream, cardArgs(), mCardSlots,
-                                        mCardOcc);
+            launch_solid_stiff_cd<Real>(mStream, cardArgs(),
+                                        mCardBatches.data(),
+                                        (int)mCardBatches.size(), mCardOcc);
             tock(AXS3D_KERNEL_STIFFNESS_3D);
         }
         if (mNumFEM > 0) {
@@ -1115,7 +1149,8 @@ private:
     // Times the cardinal kernel's occupancy variants on this workload and
     // keeps the fastest.  Runs right after finalize, on zero fields: the
     // element forces come out zero and the attenuation state stays zero,
-    // so the trials leave no trace.  AXS3D_AUTOTUNE=0 keeps variant 2.
+    // so the trials leave no trace.  AXS3D_AUTOTUNE=0 keeps the lowest
+    // variant.
     void autotune() override {
         if (mNum3D == 0) return;
         const char *env = std::getenv("AXS3D_AUTOTUNE");
@@ -1124,13 +1159,15 @@ private:
         hip_check(hipEventCreate(&a), "Domain::autotune");
         hip_check(hipEventCreate(&b), "Domain::autotune");
         double best = 1e300;
-        int bestOcc = 2;
-        for (int occ = 2; occ <= 4; occ++) {
+        int bestOcc = CARD_OCC_MIN;
+        for (int occ = CARD_OCC_MIN; occ <= CARD_OCC_MAX; occ++) {
             mCardOcc = occ;
             mCardSlots = mCardSlotsFor[occ];
-            launch_solid_stiff_cd<Real>(mStream, cardArgs(), mCardSlots, occ);
+            const CardBatch *bt = mCardBatches.data();
+            const int nb = (int)mCardBatches.size();
+            launch_solid_stiff_cd<Real>(mStream, cardArgs(), bt, nb, occ);
             hip_check(hipEventRecord(a, mStream), "Domain::autotune");
-            launch_solid_stiff_cd<Real>(mStream, cardArgs(), mCardSlots, occ);
+            launch_solid_stiff_cd<Real>(mStream, cardArgs(), bt, nb, occ);
             hip_check(hipEventRecord(b, mStream), "Domain::autotune");
             hip_check(hipEventSynchronize(b), "Domain::autotune");
             float ms = 0.f;
@@ -1320,17 +1357,53 @@ private:
             hip_check(hipDeviceGetAttribute(
                           &ncu, hipDeviceAttributeMultiprocessorCount,
                           mDevice), "Domain::finalize");
-            int maxSlots = 1;
-            for (int occ = 2; occ <= 4; occ++) {
+            for (int occ = CARD_OCC_MIN; occ <= CARD_OCC_MAX; occ++) {
                 const int per_cu = card_blocks_per_cu<Real>(mNsls > 0, occ);
                 mCardSlotsFor[occ] = std::max(1, std::min(mNum3D,
                                                           per_cu * ncu));
-                maxSlots = std::max(maxSlots, mCardSlotsFor[occ]);
             }
-            mCardOcc = 2;
-            mCardSlots = mCardSlotsFor[2];
-            dCardScratch.alloc((size_t)maxSlots * 6 * NPEM * mNu1Max3D *
-                               2 * sizeof(Real));
+            mCardOcc = CARD_OCC_MIN;
+            mCardSlots = mCardSlotsFor[CARD_OCC_MIN];
+            // batches: runs of one material type whose strain and stress
+            // ([150][nu1] complex per element) fit the scratch together;
+            // the 3D element-modes in list order for the gradient and
+            // quadrature kernels
+            size_t budget = CARD_SCRATCH_BYTES;
+            if (const char *env = std::getenv("AXS3D_CARD_SCRATCH_MB"))
+                budget = (size_t)std::max(1L, std::atol(env)) << 20;
+            budget = std::min(budget / sizeof(cpx<Real>),
+                              (size_t)UINT32_MAX);
+            std::vector<uint32_t> scrOff(mNumElems, 0u);
+            std::vector<EMItem> em3d;
+            mCardBatches.clear();
+            size_t scrMax = 0;
+            for (int mt = 0; mt < 3; mt++) {
+                size_t used = 0;
+                for (int w = mTypeBegin3D[mt]; w < mTypeBegin3D[mt + 1];
+                     w++) {
+                    const int e = hList3D[w];
+                    const int nu1 = hElemNr[e] / 2 + 1;
+                    const size_t need = (size_t)6 * NPEM * nu1;
+                    if (mCardBatches.empty() ||
+                        mCardBatches.back().mtype != mt ||
+                        used + need > budget) {
+                        mCardBatches.push_back(
+                            {mt, w, w, (int)em3d.size(), (int)em3d.size()});
+                        used = 0;
+                    }
+                    scrOff[e] = (uint32_t)used;
+                    used += need;
+                    for (int al = 0; al < nu1; al++) em3d.push_back({e, al});
+                    mCardBatches.back().end = w + 1;
+                    mCardBatches.back().em_end = (int)em3d.size();
+                    scrMax = std::max(scrMax, used);
+                }
+            }
+            dScrOff.alloc(scrOff.size() * 4);
+            dScrOff.upload(scrOff.data(), scrOff.size() * 4, st);
+            dEM3D.alloc(em3d.size() * sizeof(EMItem));
+            dEM3D.upload(em3d.data(), em3d.size() * sizeof(EMItem), st);
+            dCardScratch.alloc(scrMax * sizeof(cpx<Real>));
             if (mNsls > 0) {
                 const int npa = mAttCG4 ? 4 : NPEM;
                 dMemVar.alloc((size_t)mNumSlices * mNsls * 6 * npa *

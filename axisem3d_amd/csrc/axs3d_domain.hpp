@@ -19,6 +19,7 @@
 
 #include "axs3d.h"
 #include "axs3d_common.hpp"
+#include "axs3d_launch.hpp"
 
 namespace axs3d {
 
@@ -119,7 +120,7 @@ public:
     void autotuneInfo(int *occ, int *slots, double *ms) const {
         *occ = mCardOcc;
         *slots = mCardSlots;
-        for (int k = 0; k < 3; k++) ms[k] = mCardTuneMs[k + 2];
+        for (int k = 0; k < 3; k++) ms[k] = mCardTuneMs[k + CARD_OCC_MIN];
     }
     int precision() const { return mPrecision; }
 
@@ -179,9 +180,11 @@ protected:
     std::vector<int64_t> hPropOff;      // offset of each 3D element's
     int64_t mPropTotal = 0;             // parameters (np x 25 x Nr)
     int mNum3D = 0, mNu1Max3D = 0, mCardSlots = 0;
-    int mCardOcc = 2;                   // chosen occupancy variant
-    int mCardSlotsFor[5] = {0, 0, 0, 0, 0};
-    double mCardTuneMs[5] = {0, 0, 0, 0, 0};
+    std::vector<int32_t> hList3D;       // 3D elements, launch order
+    std::vector<CardBatch> mCardBatches;
+    int mCardOcc = CARD_OCC_MIN;        // chosen occupancy variant
+    int mCardSlotsFor[CARD_OCC_MAX + 1] = {};
+    double mCardTuneMs[CARD_OCC_MAX + 1] = {};
     std::vector<uint8_t> hProp3DSet, hAtt3DSet;
     // attenuation
     int mNsls = 0, mAttCG4 = 0;
@@ -201,7 +204,8 @@ protected:
     DeviceBuffer dElemPoints, dElemAxial, dElemNu1, dElemOps, dElemMat,
         dMatOff, dRot, dEMList, dGTab, dElemEBOff, dEB, dAdj;
     // device: cardinal path
-    DeviceBuffer dElemNr, dSliceOff, dPropOff, dList3D, dWork, dProp3D,
+    DeviceBuffer dElemNr, dSliceOff, dPropOff, dList3D, dScrOff, dEM3D,
+        dPosOff, dPosTab, dProp3D,
         dTwiddle,
         dTwOff, dPlan, dCardScratch, dAttABG, dAtt3D, dMemVar, dDStress,
         dAtt1D, dMemVarFR, dDStressFR;

@@ -80,33 +80,46 @@ struct StiffArgs {
 };
 
 // Largest element Nr of the cardinal kernel (one complex series of Nr per
-// wave in LDS, two more in registers).
+// wave in LDS).
 constexpr int CARD_NR_CAP = 832;
-// FFT plan per Nr: word 0 = number of stages; per stage s, words 1+3s..:
-// radix | (Ns << 8), ceil(2^32 / Ns) (0 for Ns = 1), Nr / (Ns * radix).
+// Occupancy variants of the cardinal kernel (minimum waves per SIMD).
+constexpr int CARD_OCC_MIN = 3;
+constexpr int CARD_OCC_MAX = 5;
+// FFT plan per Nr (in-place, decimation-in-frequency stage order): word 0
+// = number of stages | (length of the stage twiddle table << 8); per
+// stage s, words 1+3s..: radix | (m << 8) with m = span / radix,
+// ceil(2^32 / m) (0 for m = 1), offset of the stage's twiddles
+// exp(-2 pi i j / span), j < m, in the Nr's table.
 constexpr int CARD_MAX_STAGES = 12;
+// Largest stage twiddle table (sum of m over the stages, radices in
+// descending order; 364 for the 13-smooth Nr up to 832).
+constexpr int CARD_TW_CAP = 384;
 constexpr int CARD_PLAN_W = 1 + 3 * CARD_MAX_STAGES;
 
 // Device operands of the cardinal-path (3D material) element kernel.
 template <typename Real>
 struct CardArgs {
     StiffArgs<Real> s;
-    int n3d;                       // number of 3D elements
+    int n3d;                       // number of 3D elements; s.em_list is
+                                   // their element-modes in list3d order
     const int32_t *list3d;         // 3D elements grouped by material type,
                                    // each group heaviest first
     int type_begin[4];             // group boundaries in list3d
-    int *work_counter;             // one per material type
     const int32_t *elem_nr;
     const int64_t *slice_off;      // first slice of each element
     // Per-slice operands are stored per element as [component][Nr], at
     // element offset slice_off * (components per slice).
     const Real *prop3d;            // [np][25] parameters of the type
     const int64_t *prop_off;       // element offsets in prop3d
-    const cpx<Real> *twiddle;      // exp(-2 pi i t / Nr), per Nr
+    const cpx<Real> *twiddle;      // stage twiddle tables, per Nr
     const int64_t *tw_off;         // offset of each Nr's table
     const uint32_t *plan;          // [Nr][CARD_PLAN_W] FFT stages
-    Real *scratch;                 // [slot][150][nu1max] complex
-    int nu1max;
+    const uint16_t *slice_pos;     // buffer position of each slice after
+    const int64_t *pos_off;        // the inverse transform, per Nr
+    Real *scratch;                 // batch scratch, complex
+    const uint32_t *scr_off;       // [element] its part of its batch's
+                                   // scratch ([150][nu1] complex), in
+                                   // complex values
     // attenuation (nsls = 0: none)
     int nsls;
     int npa;                       // 4 (CG4) or 25 (FULL)
@@ -131,11 +144,20 @@ struct PointGeom {
 template <typename Real>
 void launch_solid_stiff_fr(hipStream_t st, const StiffArgs<Real> &a);
 
-// occ: 2, 3 or 4 = minimum waves per SIMD of the register allocation;
-// n_slots persistent blocks, each with its scratch slot
+// A batch of the cardinal path: list3d positions [begin, end), all of
+// material type mtype, whose strain and stress fit the scratch together.
+struct CardBatch {
+    int mtype, begin, end;
+    int em_begin, em_end;          // its element-modes in the 3D list
+};
+// Default scratch of one batch (bytes).
+constexpr size_t CARD_SCRATCH_BYTES = (size_t)4 << 30;
+
+// occ: CARD_OCC_MIN..CARD_OCC_MAX = minimum waves per SIMD of the register
+// allocation; one block per element, one launch per batch
 template <typename Real>
 void launch_solid_stiff_cd(hipStream_t st, const CardArgs<Real> &a,
-                           int n_slots, int occ);
+                           const CardBatch *batches, int n_batches, int occ);
 
 template <typename Real>
 int card_blocks_per_cu(bool att, int occ);

@@ -259,4 +259,4 @@ class GpuSolver:
                                                C.byref(blocks), ms))
         return {"cardinal_occupancy": occ.value,
                 "cardinal_blocks": blocks.value,
-                "trial_ms": {"2": ms[0], "3": ms[1], "4": ms[2]}}
+                "trial_ms": {"3": ms[0], "4": ms[1], "5": ms[2]}}

@@ -288,8 +288,8 @@ int axs3d_profile_reset(axs3d_ctx *ctx);
 
 /* Kernel variants chosen at axs3d_finalize by timing them on the
  * workload (no reference counterpart; the reference has no GPU): the
- * cardinal kernel's occupancy (minimum waves per SIMD, 2-4), its number
- * of persistent blocks, and the trial times (ms) of variants 2, 3, 4. */
+ * cardinal kernel's occupancy (minimum waves per SIMD, 3-5), its number
+ * of persistent blocks, and the trial times (ms) of variants 3, 4, 5. */
 int axs3d_autotune_info(const axs3d_ctx *ctx, int *cardinal_occupancy,
                         int *cardinal_blocks, double *trial_ms3);
 
