@@ -20,7 +20,8 @@ FIELD_FLUID_STIFF = 7
 KERNEL_STIFFNESS, KERNEL_POINT, KERNEL_SOURCE, KERNEL_RECORD = 0, 1, 2, 3
 KERNEL_STIFFNESS_3D = 4
 KERNEL_FLUID_STIFFNESS, KERNEL_FLUID_POINT = 5, 6
-NUM_KERNELS = 7
+KERNEL_CD_GRAD, KERNEL_CD_STRESS, KERNEL_CD_QUAD = 7, 8, 9
+NUM_KERNELS = 10
 
 # every symbol declared in include/axs3d.h, with (restype, argtypes)
 _p = C.c_void_p
@@ -36,6 +37,7 @@ SIGNATURES = {
     "axs3d_set_solid_elements": (_i, [_p, _i, _p, _p, _p]),
     "axs3d_set_solid_iso1d": (_i, [_p, _p]),
     "axs3d_set_axial_points": (_i, [_p, _i, _p]),
+    "axs3d_set_solid_mass3d": (_i, [_p, _i, _p, _p]),
     "axs3d_set_element_kinds": (_i, [_p, _p]),
     "axs3d_set_solid_iso3d": (_i, [_p, _i, _i, _p]),
     "axs3d_set_attenuation": (_i, [_p, _i, _i, _p, _p, _p]),

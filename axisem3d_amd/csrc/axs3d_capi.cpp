@@ -192,6 +192,13 @@ int axs3d_set_axial_points(axs3d_ctx *ctx, int n, const int32_t *points) {
                    [&](axs3d::DomainBase &d) { d.setAxialPoints(n, points); });
 }
 
+int axs3d_set_solid_mass3d(axs3d_ctx *ctx, int n, const int32_t *points,
+                           const double *mass) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) {
+        d.setSolidMass3D(n, points, mass);
+    });
+}
+
 int axs3d_set_moment_source(axs3d_ctx *ctx, int elem, int nu1,
                             const double *pattern, int nsteps,
                             const double *stf) {

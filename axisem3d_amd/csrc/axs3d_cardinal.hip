@@ -19,18 +19,17 @@
 //  A. gradient: one thread per (element-mode, GLL point) as on the
 //     Fourier path, strain (rotated to RTZ where the medium works there)
 //     to the scratch.
-//  B. stress: one wave per GLL point, five per block.  The six strain
-//     series of a point are packed two per complex array (x + i y) and
-//     transformed in place in the wave's one LDS buffer by an in-place
-//     mixed-radix FFT (decimation in frequency for the inverse, in time
-//     for the forward transform; radices 2, 3, 4, 5, 7, 8, 11, 13 as
-//     in-register DFTs with constant coefficients; one twiddle per
-//     butterfly from a per-stage table, its powers formed in registers;
-//     plans, tables and the digit reversal per Nr).  Physical-domain pairs that wait for the others are parked
-//     in the point's own scratch rows.  Stress is formed slice by slice
-//     (3D moduli, SLS memory variables), transformed back and unpacked
-//     into the scratch.  The six transforms of a point run from one call
-//     site, which keeps the kernel's code small.
+//  B. stress: one GLL point per block of three waves.  Each wave packs
+//     one of the point's three strain pairs as x + i y into its own LDS
+//     buffer and transforms it in place by a mixed-radix FFT (decimation
+//     in frequency for the inverse, in time for the forward transform;
+//     radices 2, 3, 4, 5, 7, 8, 11, 13 as in-register DFTs with constant
+//     coefficients; one twiddle per butterfly from a per-stage table, its
+//     powers formed in registers; plans, tables and the digit reversal
+//     per Nr).  The three waves then form the stress slice by slice from
+//     all six strains (3D moduli, SLS memory variables) and transform back
+//     into the scratch.  One FFT call site per direction keeps the code
+//     small.
 //  C. quadrature: as A, from the stress rows to the element-local force
 //     buffer the point kernel gathers.
 // Each kernel gets the register budget of its own phase.
@@ -40,8 +39,8 @@
 
 namespace axs3d {
 
-constexpr int CD_WAVES = 5;                        // waves per block
-constexpr int CD_THREADS = CD_WAVES * 64;
+constexpr int CD_PW = 3;               // stress kernel: waves per point
+constexpr int CD_PT = CD_PW * 64;      // threads per stress block
 
 template <typename T>
 __device__ __forceinline__ cpx<T> cmulc(cpx<T> a, cpx<T> b) {
@@ -229,14 +228,16 @@ __device__ __forceinline__ void wstage(cpx<Real> *buf, int nb, int m,
         cpx<Real> v[R];
 #pragma unroll
         for (int r = 0; r < R; r++) v[r] = buf[pad(base + r * m)];
+        // tws[0] = 1 exactly, so lane j = 0 multiplies by one; only the
+        // m = 1 stage (all j = 0) skips the twiddles
         cpx<Real> w = {1, 0};
-        if (j > 0) {
+        if (m > 1) {
             w = tws[j];
             if (inv) w.im = -w.im;
         }
-        if (!inv && j > 0) twiddle_powers<Real, R>(v, w);
+        if (!inv && m > 1) twiddle_powers<Real, R>(v, w);
         dft<Real, R>(v, inv);
-        if (inv && j > 0) twiddle_powers<Real, R>(v, w);
+        if (inv && m > 1) twiddle_powers<Real, R>(v, w);
 #pragma unroll
         for (int q = 0; q < R; q++) buf[pad(base + q * m)] = v[q];
     }
@@ -284,25 +285,20 @@ __device__ __forceinline__ void pack_pair(cpx<Real> *buf,
                                           const cpx<Real> *__restrict__ Y,
                                           int nr, int nu1, bool even,
                                           int lane) {
+    // branch-free: slice k takes row entry kk = k (k < nu1) or nr - k
+    // (conjugated); the mean and an even Nr's Nyquist entry are real,
+    // the latter doubled
+    const int nyq = even ? nu1 - 1 : -1;
 #pragma unroll 4
     for (int k = lane; k < nr; k += 64) {
-        cpx<Real> xf, yf;
-        if (k == 0) {
-            xf = {X[0].re, 0};
-            yf = {Y[0].re, 0};
-        } else if (k < nu1) {
-            xf = X[k];
-            yf = Y[k];
-            if (even && k == nu1 - 1) {
-                xf = {(Real)2 * xf.re, 0};
-                yf = {(Real)2 * yf.re, 0};
-            }
-        } else {
-            const cpx<Real> xs = X[nr - k], ys = Y[nr - k];
-            xf = {xs.re, -xs.im};
-            yf = {ys.re, -ys.im};
-        }
-        buf[pad(k)] = {xf.re - yf.im, xf.im + yf.re};
+        const bool lo = k < nu1;
+        const int kk = lo ? k : nr - k;
+        const cpx<Real> x = X[kk], y = Y[kk];
+        const bool real_only = k == 0 || k == nyq;
+        const Real sc = k == nyq ? (Real)2 : (Real)1;
+        const Real sim = real_only ? (Real)0 : (lo ? (Real)1 : (Real)-1);
+        // xf = (sc x.re, sim x.im), yf likewise; buf = xf + i yf
+        buf[pad(k)] = {sc * x.re - sim * y.im, sim * x.im + sc * y.re};
     }
     wave_sync();
 }
@@ -388,7 +384,7 @@ constexpr int CD_BUF = CARD_NR_CAP + (CARD_NR_CAP >> 4);
 template <typename Real>
 struct CdLds {
     cpx<Real> tw[CARD_TW_CAP];
-    cpx<Real> wbuf[CD_WAVES][CD_BUF];
+    cpx<Real> wbuf[CD_PW][CD_BUF];
     uint16_t pos[CARD_NR_CAP];     // buffer position of slice k
 };
 
@@ -503,230 +499,130 @@ __global__ __launch_bounds__(STIFF_BLOCK) void k_cd_quad(CardArgs<Real> a,
     }
 }
 
-// B: the six strain series of GLL point pp, one job after another in the
-// wave's LDS buffer.  A job fills the buffer (packs a strain pair from the
-// scratch rows, or reloads a parked physical-domain pair), transforms it
-// and consumes it (stress, park, or unpack into the scratch rows).  A
-// pair is parked in the point's own two scratch rows of that pair, free
-// once packed: slice k in row 2q at k (k < nu1) or row 2q+1 at k - nu1
-// (nr - nu1 < nu1).  Isotropic and TI media: the shears 3, 4,
-// 5 need only their own strain, the normals need eps_0..2, so pair (2,3)
-// is parked with sigma_3 formed, then completed with the normals.
-// Anisotropic media couple all six: pairs (0,1) and (2,3) are parked
-// while (4,5) is in the buffer.  Isotropic::strainToStress
-// (Isotropic.hpp:155-172), TransverselyIsotropic::strainToStress
-// (TransverselyIsotropic.hpp:161-180), Anisotropic::strainToStress
-// (Anisotropic.hpp:203-274).  Between the transforms slice k of a series
-// sits at buffer position pos[k] (the digit reversal of the in-place
-// FFT); the per-slice steps run over slices, so their global accesses
-// stay consecutive.  One call site of the FFT keeps the kernel's code
-// small.
-enum { SRC_PACK = 0, SRC_PARK = 1, SRC_KEEP = 2 };
-enum {
-    POST_UNPACK = 0, POST_PARK = 1, POST_SHEAR3 = 2, POST_NORMAL = 3,
-    POST_SHEAR45 = 4, POST_ANISO = 5
-};
-constexpr uint16_t cd_job(int src, int srow, int inv, int post, int prow) {
-    return (uint16_t)(src | srow << 2 | inv << 4 | post << 5 | prow << 8);
-}
-__constant__ uint16_t kCdJobs[2][6] = {
-    {cd_job(SRC_PACK, 1, 1, POST_SHEAR3, 1),
-     cd_job(SRC_PACK, 0, 1, POST_NORMAL, 1),
-     cd_job(SRC_KEEP, 0, 0, POST_UNPACK, 0),
-     cd_job(SRC_PARK, 1, 0, POST_UNPACK, 1),
-     cd_job(SRC_PACK, 2, 1, POST_SHEAR45, 0),
-     cd_job(SRC_KEEP, 0, 0, POST_UNPACK, 2)},
-    {cd_job(SRC_PACK, 0, 1, POST_PARK, 0),
-     cd_job(SRC_PACK, 1, 1, POST_PARK, 1),
-     cd_job(SRC_PACK, 2, 1, POST_ANISO, 0),
-     cd_job(SRC_KEEP, 0, 0, POST_UNPACK, 2),
-     cd_job(SRC_PARK, 0, 0, POST_UNPACK, 0),
-     cd_job(SRC_PARK, 1, 0, POST_UNPACK, 1)}};
-
+// B: stress of one GLL point of one element per block of three waves.
+// Wave q packs strain pair q (rows 2q, 2q+1: (ss, pp), (zz, pz), (sz, sp))
+// as x + i y into its own LDS buffer and inverse-transforms it; after a
+// block barrier the three waves form the stress slice by slice from all
+// six strains (Isotropic::strainToStress, Isotropic.hpp:155-172;
+// TransverselyIsotropic::strainToStress, TransverselyIsotropic.hpp:161-
+// 180; Anisotropic::strainToStress, Anisotropic.hpp:203-274) with the SLS
+// memory update, write it back in place, and after a second barrier each
+// wave forward-transforms its pair and unpacks it into the scratch rows.
+// Between the transforms slice k sits at buffer position pos[k] (the
+// digit reversal of the in-place FFT); the slice loop runs over slices so
+// its global accesses stay consecutive.
 template <typename Real, bool ATT, int MT>
-__device__ __forceinline__ void cd_phase_point(const CardArgs<Real> *a,
-                                               cpx<Real> *S, cpx<Real> *b,
-                                               const cpx<Real> *tw,
-                                               const uint16_t *pos, int e,
-                                               int nr, int nu1, int pp,
-                                               int lane) {
-    const uint32_t *plan = a->plan + (size_t)nr * CARD_PLAN_W;
-    const int64_t sl0 = a->slice_off[e];
-    const bool even = (nr & 1) == 0;
-    const size_t rs = (size_t)NPEM * nu1;
-    cpx<Real> *r0 = S + (size_t)pp * nu1;
-    auto park = [&](int q, int k) -> cpx<Real> * {
-        return k < nu1 ? r0 + 2 * q * rs + k
-                       : r0 + (2 * q + 1) * rs + (k - nu1);
-    };
-    SliceMat<Real> M;
-    M.ps = (size_t)NPEM * nr;
-    M.par = a->prop3d + a->prop_off[e] + (size_t)pp * nr;
-    const int ip4 = ATT ? att_slot(pp, a->npa) : -1;
-    const bool att = ATT && ip4 >= 0;
-    if (ATT) {
-        const int npa = a->npa;
-        M.cs = (size_t)npa * nr;
-        M.dl = a->att3d + (size_t)sl0 * 2 * npa + (size_t)ip4 * nr;
-        M.dm = M.dl + M.cs;
-        M.mv = a->memvar + (size_t)sl0 * a->nsls * 6 * npa + (size_t)ip4 * nr;
-        M.ds = a->dstress + (size_t)sl0 * 6 * npa + (size_t)ip4 * nr;
-    }
-    const uint16_t *jobs = kCdJobs[MT == 2 ? 1 : 0];
-#pragma unroll 1
-    for (int jb = 0; jb < 6; jb++) {
-        const int J = jobs[jb];
-        const int src = J & 3, srow = (J >> 2) & 3, inv = (J >> 4) & 1;
-        const int post = (J >> 5) & 7, prow = (J >> 8) & 3;
-        if (src == SRC_PACK) {
-            pack_pair<Real>(b, r0 + 2 * srow * rs, r0 + (2 * srow + 1) * rs,
-                            nr, nu1, even, lane);
-        } else if (src == SRC_PARK) {
-#pragma unroll 4
-            for (int k = lane; k < nr; k += 64) b[pad(pos[k])] = *park(srow, k);
-            wave_sync();
-        }
-        wfft<Real>(b, nr, plan, inv != 0, tw, lane);
-        if (post == POST_UNPACK) {
-            unpack_pair<Real>(b, r0 + 2 * prow * rs, r0 + (2 * prow + 1) * rs,
-                              nr, nu1, even, lane);
-            continue;
-        }
-        if (post == POST_PARK) {
-#pragma unroll 4
-            for (int k = lane; k < nr; k += 64) *park(prow, k) = b[pad(pos[k])];
-        }
-        if constexpr (MT != 2) {
-            if (post == POST_SHEAR3) {
-                // (eps_2, eps_3) -> park (eps_2, sigma_3); sigma_3 = mu
-                // eps_3 (isotropic) or L eps_3 (TI)
+__device__ __forceinline__ void cd_stress_slices(const CardArgs<Real> &a,
+                                                 cpx<Real> *b0, cpx<Real> *b1,
+                                                 cpx<Real> *b2,
+                                                 const uint16_t *pos,
+                                                 const SliceMat<Real> &M,
+                                                 bool att, int nr, int t) {
 #pragma unroll 2
-                for (int k = lane; k < nr; k += 64) {
-                    const cpx<Real> z = b[pad(pos[k])];
-                    Real s3 = z.im * M.p(MT == 0 ? 1 : 3, k);
-                    if (ATT && att) att_update(*a, M, 3, k, z.im * M.dm[k], s3);
-                    *park(1, k) = {z.re, s3};
-                }
-            } else if (post == POST_NORMAL) {
-                // (eps_0, eps_1) with the parked (eps_2, sigma_3): sigma_0,
-                // sigma_1 stay in the buffer, (sigma_2, sigma_3) parked
-#pragma unroll 2
-                for (int k = lane; k < nr; k += 64) {
-                    const cpx<Real> z = b[pad(pos[k])];
-                    cpx<Real> *pk = park(1, k);
-                    const cpx<Real> y = *pk;
-                    const Real e0 = z.re, e1 = z.im, e2 = y.re;
-                    Real s0, s1, s2;
-                    if constexpr (MT == 0) {
-                        const Real lam = M.p(0, k), mu = M.p(1, k);
-                        const Real mu2 = mu * (Real)2;
-                        const Real tl = (e0 + e1 + e2) * lam;
-                        s0 = tl + e0 * mu2;
-                        s1 = tl + e1 * mu2;
-                        s2 = tl + e2 * mu2;
-                    } else {
-                        const Real A = M.p(0, k), C = M.p(1, k), F = M.p(2, k);
-                        const Real mN2 = M.p(4, k) * (Real)-2;
-                        const Real t3 = e0 + e1;
-                        const Real t4 = t3 * A + e2 * F;
-                        s0 = t4 + e1 * mN2;
-                        s1 = t4 + e0 * mN2;
-                        s2 = t3 * F + e2 * C;
-                    }
-                    if (ATT && att) {
-                        const Real dl = M.dl[k], dm2 = M.dm[k] * (Real)2;
-                        const Real tr = e0 + e1 + e2;
-                        att_update(*a, M, 0, k, tr * dl + e0 * dm2, s0);
-                        att_update(*a, M, 1, k, tr * dl + e1 * dm2, s1);
-                        att_update(*a, M, 2, k, tr * dl + e2 * dm2, s2);
-                    }
-                    b[pad(pos[k])] = {s0, s1};
-                    *pk = {s2, y.im};
-                }
-            } else if (post == POST_SHEAR45) {
-#pragma unroll 2
-                for (int k = lane; k < nr; k += 64) {
-                    const cpx<Real> z = b[pad(pos[k])];
-                    Real s4, s5;
-                    if constexpr (MT == 0) {
-                        const Real mu = M.p(1, k);
-                        s4 = z.re * mu;
-                        s5 = z.im * mu;
-                    } else {
-                        s4 = z.re * M.p(3, k);
-                        s5 = z.im * M.p(4, k);
-                    }
-                    if (ATT && att) {
-                        const Real dm = M.dm[k];
-                        att_update(*a, M, 4, k, z.re * dm, s4);
-                        att_update(*a, M, 5, k, z.im * dm, s5);
-                    }
-                    b[pad(pos[k])] = {s4, s5};
-                }
-            }
+    for (int k = t; k < nr; k += CD_PT) {
+        const int p = pad(pos[k]);
+        const cpx<Real> z01 = b0[p], z23 = b1[p], z45 = b2[p];
+        const Real e[6] = {z01.re, z01.im, z23.re, z23.im, z45.re, z45.im};
+        Real sv[6];
+        if constexpr (MT == 0) {
+            const Real lam = M.p(0, k), mu = M.p(1, k);
+            const Real mu2 = mu * (Real)2;
+            const Real tl = (e[0] + e[1] + e[2]) * lam;
+            sv[0] = tl + e[0] * mu2;
+            sv[1] = tl + e[1] * mu2;
+            sv[2] = tl + e[2] * mu2;
+            sv[3] = e[3] * mu;
+            sv[4] = e[4] * mu;
+            sv[5] = e[5] * mu;
+        } else if constexpr (MT == 1) {
+            const Real A = M.p(0, k), C = M.p(1, k), F = M.p(2, k);
+            const Real Lm = M.p(3, k), N = M.p(4, k);
+            const Real mN2 = N * (Real)-2;
+            const Real t3 = e[0] + e[1];
+            const Real t4 = t3 * A + e[2] * F;
+            sv[0] = t4 + e[1] * mN2;
+            sv[1] = t4 + e[0] * mN2;
+            sv[2] = t3 * F + e[2] * C;
+            sv[3] = e[3] * Lm;
+            sv[4] = e[4] * Lm;
+            sv[5] = e[5] * N;
         } else {
-            if (post == POST_ANISO) {
-#pragma unroll 1
-                for (int k = lane; k < nr; k += 64) {
-                    cpx<Real> *p01 = park(0, k), *p23 = park(1, k);
-                    const cpx<Real> z01 = *p01, z23 = *p23, z45 = b[pad(pos[k])];
-                    const Real ev[6] = {z01.re, z01.im, z23.re, z23.im,
-                                        z45.re, z45.im};
-                    Real sv[6];
-                    aniso_hooke<Real, Real>(
-                        ev, [&](int c) { return M.p(c, k); }, sv);
-                    if (ATT && att) {
-                        const Real dl = M.dl[k], dm = M.dm[k];
-                        const Real dm2 = dm * (Real)2;
-                        const Real tr = ev[0] + ev[1] + ev[2];
-#pragma unroll
-                        for (int d = 0; d < 6; d++)
-                            att_update(*a, M, d, k,
-                                       d < 3 ? tr * dl + ev[d] * dm2
-                                             : ev[d] * dm,
-                                       sv[d]);
-                    }
-                    *p01 = {sv[0], sv[1]};
-                    *p23 = {sv[2], sv[3]};
-                    b[pad(pos[k])] = {sv[4], sv[5]};
-                }
-            }
+            aniso_hooke<Real, Real>(e, [&](int c) { return M.p(c, k); }, sv);
         }
-        wave_sync();
+        if (ATT && att) {
+            const Real dl = M.dl[k], dm = M.dm[k];
+            const Real dm2 = dm * (Real)2;
+            const Real tr = e[0] + e[1] + e[2];
+#pragma unroll
+            for (int d = 0; d < 6; d++)
+                att_update(a, M, d, k,
+                           d < 3 ? tr * dl + e[d] * dm2 : e[d] * dm, sv[d]);
+        }
+        b0[p] = {sv[0], sv[1]};
+        b1[p] = {sv[2], sv[3]};
+        b2[p] = {sv[4], sv[5]};
     }
 }
 
 // B kernel.  OCC = minimum waves per SIMD the register allocation must
 // allow; picked per workload by measurement.  MT = material type of the
-// batch's elements (0 isotropic, 1 TI, 2 anisotropic).  Five blocks per
-// element of the batch list3d[w0 ..), five GLL points per block, one per
-// wave; the waves share the element's twiddles and are otherwise
-// independent.
+// batch's elements (0 isotropic, 1 TI, 2 anisotropic).  25 blocks per
+// element of the batch list3d[w0 ..), one per GLL point.
 template <typename Real, bool ATT, int OCC, int MT>
-__global__ __launch_bounds__(CD_THREADS, OCC) void k_cd_stress(
-    CardArgs<Real> a, int w0) {
+__global__ __launch_bounds__(CD_PT, OCC) void k_cd_stress(CardArgs<Real> a,
+                                                          int w0) {
     __shared__ CdLds<Real> L;
     const int t = threadIdx.x;
     const int lane = t & 63;
     // the wave index is uniform across the wave: keep it (and all that is
     // derived from it) in scalar registers
-    const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
-    const int w = w0 + (int)blockIdx.x / CD_WAVES;
-    const int pp = ((int)blockIdx.x % CD_WAVES) * CD_WAVES + wv;
+    const int q = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int w = w0 + (int)blockIdx.x / NPEM;
+    const int pp = (int)blockIdx.x % NPEM;
     const int e = a.list3d[w];
     const int nu1 = a.s.elem_nu1[e];
     const int nr = a.elem_nr[e];
+    const uint32_t *plan = a.plan + (size_t)nr * CARD_PLAN_W;
     {
         const cpx<Real> *twg = a.twiddle + a.tw_off[nr];
-        const int ntw = (int)(a.plan[(size_t)nr * CARD_PLAN_W] >> 8);
-        for (int x = t; x < ntw; x += CD_THREADS) L.tw[x] = twg[x];
+        const int ntw = (int)(plan[0] >> 8);
+        for (int x = t; x < ntw; x += CD_PT) L.tw[x] = twg[x];
         const uint16_t *pg = a.slice_pos + a.pos_off[nr];
-        for (int x = t; x < nr; x += CD_THREADS) L.pos[x] = pg[x];
+        for (int x = t; x < nr; x += CD_PT) L.pos[x] = pg[x];
     }
     __syncthreads();
-    cpx<Real> *S = reinterpret_cast<cpx<Real> *>(a.scratch) + a.scr_off[e];
-    cd_phase_point<Real, ATT, MT>(&a, S, L.wbuf[wv], L.tw, L.pos, e, nr, nu1,
-                                  pp, lane);
+    const bool even = (nr & 1) == 0;
+    const size_t rs = (size_t)NPEM * nu1;
+    cpx<Real> *r0 = reinterpret_cast<cpx<Real> *>(a.scratch) + a.scr_off[e] +
+                    (size_t)pp * nu1;
+    cpx<Real> *b = L.wbuf[q];
+    pack_pair<Real>(b, r0 + 2 * q * rs, r0 + (2 * q + 1) * rs, nr, nu1, even,
+                    lane);
+    wfft<Real>(b, nr, plan, true, L.tw, lane);
+    __syncthreads();
+    {
+        SliceMat<Real> M;
+        M.ps = (size_t)NPEM * nr;
+        M.par = a.prop3d + a.prop_off[e] + (size_t)pp * nr;
+        const int ip4 = ATT ? att_slot(pp, a.npa) : -1;
+        const bool att = ATT && ip4 >= 0;
+        if (ATT && att) {
+            const int64_t sl0 = a.slice_off[e];
+            const int npa = a.npa;
+            M.cs = (size_t)npa * nr;
+            M.dl = a.att3d + (size_t)sl0 * 2 * npa + (size_t)ip4 * nr;
+            M.dm = M.dl + M.cs;
+            M.mv = a.memvar + (size_t)sl0 * a.nsls * 6 * npa +
+                   (size_t)ip4 * nr;
+            M.ds = a.dstress + (size_t)sl0 * 6 * npa + (size_t)ip4 * nr;
+        }
+        cd_stress_slices<Real, ATT, MT>(a, L.wbuf[0], L.wbuf[1], L.wbuf[2],
+                                        L.pos, M, att, nr, t);
+    }
+    __syncthreads();
+    wfft<Real>(b, nr, plan, false, L.tw, lane);
+    unpack_pair<Real>(b, r0 + 2 * q * rs, r0 + (2 * q + 1) * rs, nr, nu1,
+                      even, lane);
 }
 
 // Occupancy variants for isotropic media (the common case); TI and
@@ -734,7 +630,7 @@ __global__ __launch_bounds__(CD_THREADS, OCC) void k_cd_stress(
 template <typename Real, bool ATT, int MT>
 static void launch_stress_mt(hipStream_t st, const CardArgs<Real> &a, int w0,
                              int n, int occ) {
-    const dim3 grid(n * CD_WAVES), block(CD_THREADS);
+    const dim3 grid(n * NPEM), block(CD_PT);
     if constexpr (MT != 0) {
         hipLaunchKernelGGL((k_cd_stress<Real, ATT, CARD_OCC_MIN, MT>), grid,
                            block, 0, st, a, w0);
@@ -756,58 +652,175 @@ static void launch_stress_mt(hipStream_t st, const CardArgs<Real> &a, int w0,
     }
 }
 
-// Three launches per batch (gradient, stress, quadrature); batches hold
-// elements of one material type (list3d is grouped by type, each group
-// heaviest first) whose strain and stress fit the scratch together.
+// The three launches of a batch (gradient, stress, quadrature); batches
+// hold elements of one material type (list3d is grouped by type, each
+// group heaviest first) whose strain and stress fit the scratch together.
 template <typename Real>
-void launch_solid_stiff_cd(hipStream_t st, const CardArgs<Real> &a,
-                           const CardBatch *batches, int n_batches,
-                           int occ) {
-    for (int i = 0; i < n_batches; i++) {
-        const CardBatch &bt = batches[i];
-        const int n = bt.end - bt.begin;
-        if (n <= 0) continue;
-        const bool rot = bt.mtype != 0 && a.s.rot != nullptr;
-        const int gem = (bt.em_end - bt.em_begin + EMB - 1) / EMB;
-        if (rot)
-            hipLaunchKernelGGL((k_cd_grad<Real, true>), dim3(gem),
-                               dim3(STIFF_BLOCK), 0, st, a, bt.em_begin,
-                               bt.em_end);
-        else
-            hipLaunchKernelGGL((k_cd_grad<Real, false>), dim3(gem),
-                               dim3(STIFF_BLOCK), 0, st, a, bt.em_begin,
-                               bt.em_end);
-        const bool att = a.nsls > 0;
-        switch (bt.mtype) {
-            case 0:
-                if (att) launch_stress_mt<Real, true, 0>(st, a, bt.begin, n, occ);
-                else launch_stress_mt<Real, false, 0>(st, a, bt.begin, n, occ);
-                break;
-            case 1:
-                if (att) launch_stress_mt<Real, true, 1>(st, a, bt.begin, n, occ);
-                else launch_stress_mt<Real, false, 1>(st, a, bt.begin, n, occ);
-                break;
-            default:
-                if (att) launch_stress_mt<Real, true, 2>(st, a, bt.begin, n, occ);
-                else launch_stress_mt<Real, false, 2>(st, a, bt.begin, n, occ);
-                break;
-        }
-        if (rot)
-            hipLaunchKernelGGL((k_cd_quad<Real, true>), dim3(gem),
-                               dim3(STIFF_BLOCK), 0, st, a, bt.em_begin,
-                               bt.em_end);
-        else
-            hipLaunchKernelGGL((k_cd_quad<Real, false>), dim3(gem),
-                               dim3(STIFF_BLOCK), 0, st, a, bt.em_begin,
-                               bt.em_end);
+void launch_cd_grad(hipStream_t st, const CardArgs<Real> &a,
+                    const CardBatch &bt) {
+    const int gem = (bt.em_end - bt.em_begin + EMB - 1) / EMB;
+    if (gem <= 0) return;
+    if (bt.mtype != 0 && a.s.rot != nullptr)
+        hipLaunchKernelGGL((k_cd_grad<Real, true>), dim3(gem),
+                           dim3(STIFF_BLOCK), 0, st, a, bt.em_begin,
+                           bt.em_end);
+    else
+        hipLaunchKernelGGL((k_cd_grad<Real, false>), dim3(gem),
+                           dim3(STIFF_BLOCK), 0, st, a, bt.em_begin,
+                           bt.em_end);
+}
+
+template <typename Real>
+void launch_cd_stress(hipStream_t st, const CardArgs<Real> &a,
+                      const CardBatch &bt, int occ) {
+    const int n = bt.end - bt.begin;
+    if (n <= 0) return;
+    const bool att = a.nsls > 0;
+    switch (bt.mtype) {
+        case 0:
+            if (att) launch_stress_mt<Real, true, 0>(st, a, bt.begin, n, occ);
+            else launch_stress_mt<Real, false, 0>(st, a, bt.begin, n, occ);
+            break;
+        case 1:
+            if (att) launch_stress_mt<Real, true, 1>(st, a, bt.begin, n, occ);
+            else launch_stress_mt<Real, false, 1>(st, a, bt.begin, n, occ);
+            break;
+        default:
+            if (att) launch_stress_mt<Real, true, 2>(st, a, bt.begin, n, occ);
+            else launch_stress_mt<Real, false, 2>(st, a, bt.begin, n, occ);
+            break;
     }
+}
+
+template <typename Real>
+void launch_cd_quad(hipStream_t st, const CardArgs<Real> &a,
+                    const CardBatch &bt) {
+    const int gem = (bt.em_end - bt.em_begin + EMB - 1) / EMB;
+    if (gem <= 0) return;
+    if (bt.mtype != 0 && a.s.rot != nullptr)
+        hipLaunchKernelGGL((k_cd_quad<Real, true>), dim3(gem),
+                           dim3(STIFF_BLOCK), 0, st, a, bt.em_begin,
+                           bt.em_end);
+    else
+        hipLaunchKernelGGL((k_cd_quad<Real, false>), dim3(gem),
+                           dim3(STIFF_BLOCK), 0, st, a, bt.em_begin,
+                           bt.em_end);
+}
+
+// Mass3D::computeAccel (src/core/point/mass/Mass3D.cpp:56-68) for the
+// solid points with a 3D mass, after the point kernel has assembled their
+// stiffness (axis and Nyquist rules applied): C2R, times 1/m slice by
+// slice, R2C.  One block of two waves per point: wave 0 takes components
+// (s, phi) packed as x + i y, wave 1 component z (y = 0).  With MARCH the
+// result is the acceleration of NewmarkTimeScheme::update
+// (NewmarkTimeScheme.hpp:48-59, FP64); without, it goes back to stiff.
+template <typename Real, bool MARCH>
+__global__ __launch_bounds__(128) void k_mass3d(Mass3DArgs<Real> a,
+                                                Real *__restrict__ stiff) {
+    __shared__ cpx<Real> tw[CARD_TW_CAP];
+    __shared__ cpx<Real> wbuf[2][CD_BUF];
+    __shared__ uint16_t pos[CARD_NR_CAP];
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int q = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int i = blockIdx.x;
+    const int p = a.point[i];
+    const int nr = a.point_nr[p];
+    const int nu1 = nr / 2 + 1;
+    const bool even = (nr & 1) == 0;
+    const uint32_t *plan = a.plan + (size_t)nr * CARD_PLAN_W;
+    {
+        const cpx<Real> *twg = a.twiddle + a.tw_off[nr];
+        const int ntw = (int)(plan[0] >> 8);
+        for (int x = t; x < ntw; x += 128) tw[x] = twg[x];
+        const uint16_t *pg = a.slice_pos + a.pos_off[nr];
+        for (int x = t; x < nr; x += 128) pos[x] = pg[x];
+    }
+    __syncthreads();
+    // component c of mode m of the point: stiff complex [off + m][3]
+    cpx<Real> *S = reinterpret_cast<cpx<Real> *>(stiff) +
+                   (size_t)a.point_off[p] * 3;
+    const int cx = q == 0 ? 0 : 2;
+    cpx<Real> *b = wbuf[q];
+    // C2R input (SolverFFTW conventions, as pack_pair): Hermitian
+    // extension of the modes, mean and Nyquist real, Nyquist doubled
+    const int nyq = even ? nu1 - 1 : -1;
+    for (int k = lane; k < nr; k += 64) {
+        const bool lo = k < nu1;
+        const int kk = lo ? k : nr - k;
+        const cpx<Real> x = S[(size_t)kk * 3 + cx];
+        const cpx<Real> y =
+            q == 0 ? S[(size_t)kk * 3 + 1] : cpx<Real>{(Real)0, (Real)0};
+        const bool real_only = k == 0 || k == nyq;
+        const Real sc = k == nyq ? (Real)2 : (Real)1;
+        const Real sim = real_only ? (Real)0 : (lo ? (Real)1 : (Real)-1);
+        b[pad(k)] = {sc * x.re - sim * y.im, sim * x.im + sc * y.re};
+    }
+    wave_sync();
+    wfft<Real>(b, nr, plan, true, tw, lane);
+    const Real *im = a.inv_mass + a.inv_off[i];
+    for (int k = lane; k < nr; k += 64) {
+        const int pk = pad(pos[k]);
+        const Real m = im[k];
+        const cpx<Real> z = b[pk];
+        b[pk] = {z.re * m, z.im * m};
+    }
+    wave_sync();
+    wfft<Real>(b, nr, plan, false, tw, lane);
+    // R2C output (as unpack_pair): X into component cx, Y into 1 (wave 0)
+    const Real h0 = (Real)0.5 / (Real)nr;
+    bool bad = false;
+    for (int k = lane; k < nu1; k += 64) {
+        const cpx<Real> zk = b[pad(k)];
+        const cpx<Real> zn = b[pad(k == 0 ? 0 : nr - k)];
+        const Real h = k == nyq ? h0 * (Real)0.5 : h0;
+        cpx<Real> v[2];
+        v[0] = {(zk.re + zn.re) * h, (zk.im - zn.im) * h};
+        v[1] = {(zk.im + zn.im) * h, (zn.re - zk.re) * h};
+        const int nc = q == 0 ? 2 : 1;
+        for (int c = 0; c < nc; c++) {
+            const size_t m6 = ((size_t)a.point_off[p] + k) * 6 +
+                              2 * (q == 0 ? c : 2);
+            if (!MARCH) {
+                stiff[m6] = v[c].re;
+                stiff[m6 + 1] = v[c].im;
+                continue;
+            }
+            const double hdt = .5 * a.dt, hdt2 = .5 * a.dt * a.dt;
+            const double an[2] = {(double)v[c].re, (double)v[c].im};
+#pragma unroll
+            for (int ri = 0; ri < 2; ri++) {
+                const double vv = a.veloc[m6 + ri] +
+                                  hdt * (a.accel[m6 + ri] + an[ri]);
+                const double uu = a.displ[m6 + ri] + (a.dt * vv +
+                                                      hdt2 * an[ri]);
+                a.veloc[m6 + ri] = vv;
+                a.accel[m6 + ri] = an[ri];
+                a.displ[m6 + ri] = uu;
+                bad |= !isfinite(uu);
+            }
+        }
+    }
+    if (MARCH && bad) atomicOr(a.flag, 1);
+}
+
+template <typename Real>
+void launch_mass3d(hipStream_t st, bool march, const Mass3DArgs<Real> &a,
+                   Real *stiff) {
+    if (a.n <= 0) return;
+    if (march)
+        hipLaunchKernelGGL((k_mass3d<Real, true>), dim3(a.n), dim3(128), 0,
+                           st, a, stiff);
+    else
+        hipLaunchKernelGGL((k_mass3d<Real, false>), dim3(a.n), dim3(128), 0,
+                           st, a, stiff);
 }
 
 template <typename Real, bool ATT, int OCC>
 static int blocks_per_cu_v() {
     int n = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &n, k_cd_stress<Real, ATT, OCC, 0>, CD_THREADS, 0) !=
+            &n, k_cd_stress<Real, ATT, OCC, 0>, CD_PT, 0) !=
         hipSuccess)
         return 1;
     return n > 0 ? n : 1;
@@ -827,11 +840,20 @@ int card_blocks_per_cu(bool att, int occ) {
 
 template int card_blocks_per_cu<float>(bool, int);
 template int card_blocks_per_cu<double>(bool, int);
-template void launch_solid_stiff_cd<float>(hipStream_t,
-                                           const CardArgs<float> &,
-                                           const CardBatch *, int, int);
-template void launch_solid_stiff_cd<double>(hipStream_t,
-                                            const CardArgs<double> &,
-                                            const CardBatch *, int, int);
+#define AXS3D_CD_INST(Real)                                                   \
+    template void launch_cd_grad<Real>(hipStream_t, const CardArgs<Real> &, \
+                                       const CardBatch &);                  \
+    template void launch_cd_stress<Real>(hipStream_t,                       \
+                                         const CardArgs<Real> &,            \
+                                         const CardBatch &, int);           \
+    template void launch_cd_quad<Real>(hipStream_t, const CardArgs<Real> &, \
+                                       const CardBatch &);
+AXS3D_CD_INST(float)
+AXS3D_CD_INST(double)
+template void launch_mass3d<float>(hipStream_t, bool,
+                                   const Mass3DArgs<float> &, float *);
+template void launch_mass3d<double>(hipStream_t, bool,
+                                    const Mass3DArgs<double> &, double *);
+#undef AXS3D_CD_INST
 
 }  // namespace axs3d

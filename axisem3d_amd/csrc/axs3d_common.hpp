@@ -9,8 +9,10 @@ namespace axs3d {
 constexpr int NPED = 5;    // GLL points per direction (npol = 4)
 constexpr int NPEM = 25;   // GLL points per element
 
+// Complex value; aligned to its size so that one LDS or global access
+// moves both parts.
 template <typename T>
-struct cpx {
+struct alignas(2 * sizeof(T)) cpx {
     T re, im;
 };
 

@@ -327,6 +327,32 @@ void DomainBase::setAxialPoints(int n, const int32_t *points) {
     hAxial.assign(points, points + std::max(n, 0));
 }
 
+void DomainBase::setSolidMass3D(int n, const int32_t *points,
+                                const double *mass) {
+    if (hPointNr.empty())
+        throw Error("Domain::setSolidMass3D || Set the solid points first.");
+    hMass3DPts.clear();
+    hMass3DOff.clear();
+    hMass3D.clear();
+    std::vector<uint8_t> seen(hPointNr.size(), 0);
+    for (int i = 0; i < std::max(n, 0); i++) {
+        const int p = points[i];
+        if (p < 0 || p >= (int)hPointNr.size() || seen[p])
+            throw Error("Domain::setSolidMass3D || Point out of range or "
+                        "listed twice.");
+        seen[p] = 1;
+        hMass3DPts.push_back(p);
+        hMass3DOff.push_back((int64_t)hMass3D.size());
+        for (int k = 0; k < hPointNr[p]; k++) {
+            const double m = *mass++;
+            if (!(m > 0.))
+                throw Error("Domain::setSolidMass3D || Mass must be "
+                            "positive.");
+            hMass3D.push_back(m);
+        }
+    }
+}
+
 void DomainBase::setMomentSource(int elem, int nu1, const double *pattern,
                                  int nsteps, const double *stf) {
     if (nu1 <= 0 || nsteps <= 0)
@@ -444,17 +470,21 @@ void DomainBase::finalize() {
     std::vector<double> twTab;
     std::vector<uint16_t> posTab;
     std::vector<uint32_t> plan;
-    if (!list3d.empty()) {
+    // Nr of the 3D elements and of the points with a 3D mass
+    std::vector<int> fftNr;
+    for (int32_t e : list3d) fftNr.push_back(hElemNr[e]);
+    for (int32_t p : hMass3DPts) fftNr.push_back(hPointNr[p]);
+    mNumMass3D = (int)hMass3DPts.size();
+    if (!fftNr.empty()) {
         int nrmax = 0;
-        for (int32_t e : list3d) nrmax = std::max(nrmax, hElemNr[e]);
+        for (int nr : fftNr) nrmax = std::max(nrmax, nr);
         if (nrmax > CARD_NR_CAP)
-            throw Error("Domain::finalize || Nr of a 3D element exceeds the "
-                        "cardinal kernel's FFT capacity (832).");
+            throw Error("Domain::finalize || Nr of a 3D element or 3D-mass "
+                        "point exceeds the FFT capacity (832).");
         twOff.assign(nrmax + 1, -1);
         posOff.assign(nrmax + 1, -1);
         plan.assign((size_t)(nrmax + 1) * CARD_PLAN_W, 0u);
-        for (int32_t e : list3d) {
-            const int nr = hElemNr[e];
+        for (const int nr : fftNr) {
             if (twOff[nr] >= 0) continue;
             std::vector<int> radix;
             int rem = nr, p2 = 0;
@@ -675,12 +705,19 @@ void DomainBase::finalize() {
     up(dElemNr, hElemNr.data(), hElemNr.size() * 4);
     up(dSliceOff, hSliceOff.data(), hSliceOff.size() * 8);
     up(dPropOff, hPropOff.data(), hPropOff.size() * 8);
-    if (!list3d.empty()) {
-        up(dList3D, list3d.data(), list3d.size() * 4);
+    if (!list3d.empty()) up(dList3D, list3d.data(), list3d.size() * 4);
+    if (!fftNr.empty()) {
         up(dTwOff, twOff.data(), twOff.size() * 8);
         up(dPlan, plan.data(), plan.size() * 4);
         up(dPosOff, posOff.data(), posOff.size() * 8);
         up(dPosTab, posTab.data(), posTab.size() * 2);
+    }
+    if (mNumMass3D > 0) {
+        std::vector<uint8_t> flag(mNumPoints, 0);
+        for (int32_t p : hMass3DPts) flag[p] = 1;
+        up(dMass3DFlag, flag.data(), flag.size());
+        up(dMass3DPts, hMass3DPts.data(), hMass3DPts.size() * 4);
+        up(dMass3DOff, hMass3DOff.data(), hMass3DOff.size() * 8);
     }
     hList3D.swap(list3d);
     hTwTab.swap(twTab);
@@ -928,8 +965,29 @@ public:
         g.point_nr = dPointNr.as<int32_t>();
         g.point_axial = dPointAxial.as<uint8_t>();
         g.inv_mass = dInvMass.as<double>();
+        g.mass3d = mNumMass3D > 0 ? dMass3DFlag.as<uint8_t>() : nullptr;
         g.adj = dAdj.as<uint32_t>();
         return g;
+    }
+    Mass3DArgs<Real> mass3dArgs() const {
+        Mass3DArgs<Real> m;
+        m.n = mNumMass3D;
+        m.point = dMass3DPts.as<int32_t>();
+        m.inv_off = dMass3DOff.as<int64_t>();
+        m.inv_mass = dInvMass3D.as<Real>();
+        m.point_off = dPointOff.as<uint32_t>();
+        m.point_nr = dPointNr.as<int32_t>();
+        m.twiddle = dTwiddle.as<cpx<Real>>();
+        m.tw_off = dTwOff.as<int64_t>();
+        m.plan = dPlan.as<uint32_t>();
+        m.slice_pos = dPosTab.as<uint16_t>();
+        m.pos_off = dPosOff.as<int64_t>();
+        m.displ = dDispl.as<double>();
+        m.veloc = dVeloc.as<double>();
+        m.accel = dAccel.as<double>();
+        m.dt = mDt;
+        m.flag = dFlag.as<int>();
+        return m;
     }
     PointGeom fluidGeom() const {
         PointGeom g;
@@ -991,6 +1049,27 @@ public:
         tock(AXS3D_KERNEL_SOURCE);
     }
 
+    // The cardinal path, batch by batch; with timing, each of the three
+    // kernels is timed under its own id.
+    void cardinalPath(int occ, bool timed) {
+        const CardArgs<Real> c = cardArgs();
+        for (const CardBatch &bt : mCardBatches) {
+            if (timed) tick(AXS3D_KERNEL_CD_GRAD);
+            launch_cd_grad<Real>(mStream, c, bt);
+            if (timed) {
+                tock(AXS3D_KERNEL_CD_GRAD);
+                tick(AXS3D_KERNEL_CD_STRESS);
+            }
+            launch_cd_stress<Real>(mStream, c, bt, occ);
+            if (timed) {
+                tock(AXS3D_KERNEL_CD_STRESS);
+                tick(AXS3D_KERNEL_CD_QUAD);
+            }
+            launch_cd_quad<Real>(mStream, c, bt);
+            if (timed) tock(AXS3D_KERNEL_CD_QUAD);
+        }
+    }
+
     void elementKernel() {
         if (mNumEM > 0) {
             tick(AXS3D_KERNEL_STIFFNESS);
@@ -999,9 +1078,7 @@ public:
         }
         if (mNum3D > 0) {
             tick(AXS3D_KERNEL_STIFFNESS_3D);
-            launch_solid_stiff_cd<Real>(mStream, cardArgs(),
-                                        mCardBatches.data(),
-                                        (int)mCardBatches.size(), mCardOcc);
+            cardinalPath(mCardOcc, true);
             tock(AXS3D_KERNEL_STIFFNESS_3D);
         }
         if (mNumFEM > 0) {
@@ -1029,6 +1106,11 @@ public:
                            dStiff.as<Real>(), dDispl.as<double>(),
                            dVeloc.as<double>(), dAccel.as<double>(), mDt,
                            dFlag.as<int>(), couplingGeom());
+        // points with a 3D mass: the point kernel left their assembled
+        // stiffness; divide slice by slice (and march)
+        if ((phases & PHASE_MASS) && mNumMass3D > 0)
+            launch_mass3d<Real>(mStream, (phases & PHASE_MARCH) != 0,
+                                mass3dArgs(), dStiff.as<Real>());
         tock(AXS3D_KERNEL_POINT);
     }
     // fluid points; runs before the solid points of the same phase (the
@@ -1163,11 +1245,9 @@ private:
         for (int occ = CARD_OCC_MIN; occ <= CARD_OCC_MAX; occ++) {
             mCardOcc = occ;
             mCardSlots = mCardSlotsFor[occ];
-            const CardBatch *bt = mCardBatches.data();
-            const int nb = (int)mCardBatches.size();
-            launch_solid_stiff_cd<Real>(mStream, cardArgs(), bt, nb, occ);
+            cardinalPath(occ, false);
             hip_check(hipEventRecord(a, mStream), "Domain::autotune");
-            launch_solid_stiff_cd<Real>(mStream, cardArgs(), bt, nb, occ);
+            cardinalPath(occ, false);
             hip_check(hipEventRecord(b, mStream), "Domain::autotune");
             hip_check(hipEventSynchronize(b), "Domain::autotune");
             float ms = 0.f;
@@ -1349,10 +1429,20 @@ private:
             dDStressFR.alloc((size_t)mNumEM * 6 * npa * 2 * sizeof(Real));
             dDStressFR.zero(st);
         }
-        if (mNum3D > 0) {
+        if (!hTwTab.empty()) {
             std::vector<Real> tw(hTwTab.begin(), hTwTab.end());
             dTwiddle.alloc(tw.size() * sizeof(Real));
             dTwiddle.upload(tw.data(), tw.size() * sizeof(Real), st);
+        }
+        if (mNumMass3D > 0) {
+            // Mass3D keeps 1 / m in the working precision (Mass3D.hpp:19-22)
+            std::vector<Real> inv(hMass3D.size());
+            for (size_t k = 0; k < inv.size(); k++)
+                inv[k] = (Real)(1.0 / hMass3D[k]);
+            dInvMass3D.alloc(inv.size() * sizeof(Real));
+            dInvMass3D.upload(inv.data(), inv.size() * sizeof(Real), st);
+        }
+        if (mNum3D > 0) {
             int ncu = 0;
             hip_check(hipDeviceGetAttribute(
                           &ncu, hipDeviceAttributeMultiprocessorCount,

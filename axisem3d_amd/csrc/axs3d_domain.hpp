@@ -79,6 +79,7 @@ public:
     void setAttenuation3D(int e_begin, int e_end, const double *dlam_dmu);
     void setAttenuation1D(const double *dlam_dmu);
     void setAxialPoints(int n, const int32_t *points);
+    void setSolidMass3D(int n, const int32_t *points, const double *mass);
     void setMomentSource(int elem, int nu1, const double *pattern, int nsteps,
                          const double *stf);
     void setReceivers(int n, const int32_t *elem, const double *w,
@@ -181,6 +182,11 @@ protected:
     int64_t mPropTotal = 0;             // parameters (np x 25 x Nr)
     int mNum3D = 0, mNu1Max3D = 0, mCardSlots = 0;
     std::vector<int32_t> hList3D;       // 3D elements, launch order
+    // solid points with a 3D mass: points, offsets of their Nr masses
+    std::vector<int32_t> hMass3DPts;
+    std::vector<int64_t> hMass3DOff;
+    std::vector<double> hMass3D;
+    int mNumMass3D = 0;
     std::vector<CardBatch> mCardBatches;
     int mCardOcc = CARD_OCC_MIN;        // chosen occupancy variant
     int mCardSlotsFor[CARD_OCC_MAX + 1] = {};
@@ -205,7 +211,8 @@ protected:
         dMatOff, dRot, dEMList, dGTab, dElemEBOff, dEB, dAdj;
     // device: cardinal path
     DeviceBuffer dElemNr, dSliceOff, dPropOff, dList3D, dScrOff, dEM3D,
-        dPosOff, dPosTab, dProp3D,
+        dPosOff, dPosTab, dMass3DPts, dMass3DOff, dMass3DFlag, dInvMass3D,
+        dProp3D,
         dTwiddle,
         dTwOff, dPlan, dCardScratch, dAttABG, dAtt3D, dMemVar, dDStress,
         dAtt1D, dMemVarFR, dDStressFR;

@@ -215,19 +215,19 @@ __global__ __launch_bounds__(256) void k_fluid_point(
         stiff[(size_t)k * 2 + 1] = (Real)s[1];
         return;
     }
-    double *up = a.chi + (size_t)k * 2;
-    double *vp = a.chi_v + (size_t)k * 2;
-    double *ap = a.chi_a + (size_t)k * 2;
-    bool bad = false;
-#pragma unroll
-    for (int q = 0; q < 2; q++) {
-        const double v = vp[q] + a.half_dt * (ap[q] + s[q]);
-        const double u = up[q] + (a.dt * v + a.half_dt_dt * s[q]);
-        vp[q] = v;
-        ap[q] = s[q];
-        up[q] = u;
-        bad |= !isfinite(u);
-    }
+    // all three loaded (16-byte vectors) before anything is stored
+    double2 *up = reinterpret_cast<double2 *>(a.chi + (size_t)k * 2);
+    double2 *vp = reinterpret_cast<double2 *>(a.chi_v + (size_t)k * 2);
+    double2 *ap = reinterpret_cast<double2 *>(a.chi_a + (size_t)k * 2);
+    const double2 u0 = *up, v0 = *vp, a0 = *ap;
+    const double vr = v0.x + a.half_dt * (a0.x + s[0]);
+    const double vi = v0.y + a.half_dt * (a0.y + s[1]);
+    const double ur = u0.x + (a.dt * vr + a.half_dt_dt * s[0]);
+    const double ui = u0.y + (a.dt * vi + a.half_dt_dt * s[1]);
+    *vp = make_double2(vr, vi);
+    *ap = make_double2(s[0], s[1]);
+    *up = make_double2(ur, ui);
+    const bool bad = !isfinite(ur) || !isfinite(ui);
     if (!GATHER) stiff[(size_t)k * 2] = stiff[(size_t)k * 2 + 1] = 0;
     if (bad) atomicOr(a.flag, 1);
 }

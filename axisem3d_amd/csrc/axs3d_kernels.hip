@@ -300,6 +300,7 @@ struct PointArgs {
     const int32_t *point_nr;
     const uint8_t *point_axial;
     const double *inv_mass;
+    const uint8_t *mass3d;        // [point] 1: 3D mass (or null: none)
     const uint32_t *adj;
     double *displ, *veloc, *accel;
     double dt, half_dt, half_dt_dt;
@@ -361,6 +362,13 @@ __global__ __launch_bounds__(256) void k_point(PointArgs a,
     if (MASS) {
         const int nr = a.point_nr[p];
         if ((nr & 1) == 0 && alpha == nr / 2) s[1] = s[3] = s[5] = 0;
+        if (a.mass3d != nullptr && a.mass3d[p]) {
+            // 3D mass: the division slice by slice (and the march) follow
+            // in k_mass3d, which reads this assembled stiffness
+#pragma unroll
+            for (int q = 0; q < 6; q++) stiff[(size_t)k * 6 + q] = s[q];
+            return;
+        }
     }
     if (MASS && !MARCH) {
         const Real im = (Real)a.inv_mass[p];
@@ -381,18 +389,36 @@ __global__ __launch_bounds__(256) void k_point(PointArgs a,
 #pragma unroll
         for (int q = 0; q < 6; q++) a_new[q] = (double)s[q];
     }
-    double *up = a.displ + (size_t)k * 6;
-    double *vp = a.veloc + (size_t)k * 6;
-    double *ap = a.accel + (size_t)k * 6;
+    // all of u, v, a are loaded (16-byte vectors) before anything is
+    // stored, so the nine loads are in flight together
+    double2 *up = reinterpret_cast<double2 *>(a.displ + (size_t)k * 6);
+    double2 *vp = reinterpret_cast<double2 *>(a.veloc + (size_t)k * 6);
+    double2 *ap = reinterpret_cast<double2 *>(a.accel + (size_t)k * 6);
+    double2 u2[3], v2[3], a2[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        u2[c] = up[c];
+        v2[c] = vp[c];
+        a2[c] = ap[c];
+    }
     bool bad = false;
 #pragma unroll
-    for (int q = 0; q < 6; q++) {
-        const double v = vp[q] + a.half_dt * (ap[q] + a_new[q]);
-        const double u = up[q] + (a.dt * v + a.half_dt_dt * a_new[q]);
-        vp[q] = v;
-        ap[q] = a_new[q];
-        up[q] = u;
-        bad |= !isfinite(u);
+    for (int c = 0; c < 3; c++) {
+        const double vr = v2[c].x + a.half_dt * (a2[c].x + a_new[2 * c]);
+        const double vi = v2[c].y + a.half_dt * (a2[c].y + a_new[2 * c + 1]);
+        const double ur = u2[c].x + (a.dt * vr + a.half_dt_dt * a_new[2 * c]);
+        const double ui =
+            u2[c].y + (a.dt * vi + a.half_dt_dt * a_new[2 * c + 1]);
+        v2[c] = make_double2(vr, vi);
+        a2[c] = make_double2(a_new[2 * c], a_new[2 * c + 1]);
+        u2[c] = make_double2(ur, ui);
+        bad |= !isfinite(ur) || !isfinite(ui);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        vp[c] = v2[c];
+        ap[c] = a2[c];
+        up[c] = u2[c];
     }
     if (!GATHER) {
 #pragma unroll
@@ -523,6 +549,7 @@ void launch_point(hipStream_t st, int phases, const PointGeom &g,
     a.point_nr = g.point_nr;
     a.point_axial = g.point_axial;
     a.inv_mass = g.inv_mass;
+    a.mass3d = g.mass3d;
     a.adj = g.adj;
     a.displ = displ;
     a.veloc = veloc;

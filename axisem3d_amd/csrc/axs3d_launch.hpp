@@ -137,9 +137,37 @@ struct PointGeom {
     const int32_t *point_nr;
     const uint8_t *point_axial;
     const double *inv_mass;
+    const uint8_t *mass3d;         // [point] 1: 3D mass, or null
     const uint32_t *adj;           // [point][4] element-local offsets,
                                    // ascending element, padded with ~0u
 };
+
+// Solid points with a 3D mass (Mass3D): inverse mass per slice in Real,
+// the FFT tables of their Nr and the fields the march updates.
+template <typename Real>
+struct Mass3DArgs {
+    int n;                         // points with a 3D mass
+    const int32_t *point;          // [i] point index
+    const int64_t *inv_off;        // [i] offset of its Nr inverse masses
+    const Real *inv_mass;          // 1 / m per slice
+    const uint32_t *point_off;
+    const int32_t *point_nr;
+    const cpx<Real> *twiddle;
+    const int64_t *tw_off;
+    const uint32_t *plan;
+    const uint16_t *slice_pos;
+    const int64_t *pos_off;
+    double *displ, *veloc, *accel;
+    double dt;
+    int *flag;
+};
+
+// Mass3D::computeAccel of the 3D-mass points from the assembled stiffness
+// (written there by the point kernel): with march, the Newmark update of
+// their modes; without, the acceleration back into stiff (stage form).
+template <typename Real>
+void launch_mass3d(hipStream_t st, bool march, const Mass3DArgs<Real> &a,
+                   Real *stiff);
 
 template <typename Real>
 void launch_solid_stiff_fr(hipStream_t st, const StiffArgs<Real> &a);
@@ -153,11 +181,18 @@ struct CardBatch {
 // Default scratch of one batch (bytes).
 constexpr size_t CARD_SCRATCH_BYTES = (size_t)4 << 30;
 
-// occ: CARD_OCC_MIN..CARD_OCC_MAX = minimum waves per SIMD of the register
-// allocation; one block per element, one launch per batch
+// The cardinal path of one batch: gradient, stress (occ: CARD_OCC_MIN ..
+// CARD_OCC_MAX = minimum waves per SIMD of its register allocation),
+// quadrature, in that order on one stream.
 template <typename Real>
-void launch_solid_stiff_cd(hipStream_t st, const CardArgs<Real> &a,
-                           const CardBatch *batches, int n_batches, int occ);
+void launch_cd_grad(hipStream_t st, const CardArgs<Real> &a,
+                    const CardBatch &bt);
+template <typename Real>
+void launch_cd_stress(hipStream_t st, const CardArgs<Real> &a,
+                      const CardBatch &bt, int occ);
+template <typename Real>
+void launch_cd_quad(hipStream_t st, const CardArgs<Real> &a,
+                    const CardBatch &bt);
 
 template <typename Real>
 int card_blocks_per_cu(bool att, int occ);

@@ -111,27 +111,65 @@ def build_workload(name, nsteps):
 
 def bytes_per_step(spec, real):
     """Algorithmic HBM bytes of one step, by kernel: every array touched
-    once, intermediates on chip.  stiffness: displacement read (FP64
-    complex x3) and one assembled stiffness write (Real complex x3) per
-    point-mode, GradQuad operands (10 rows), connectivity, material (1D:
-    lambda, mu per point; 3D: per slice), attenuation memory variables
-    (Real, read+write) and memory stress (Real, read+write) per slice.
-    point: stiffness read, u, v, a read+written (FP64), per-point data."""
+    once, intermediates (the cardinal scratch, the element-local forces'
+    read by the point kernel aside) on chip.
+    fr: Fourier-path elements, displacement read (FP64 complex x3) and
+      element force write (Real complex x3) per point-mode, GradQuad
+      operands (10 rows), connectivity, 1D material per point.
+    cd_grad: displacement read per point-mode of 3D elements, the 5
+      gradient rows, connectivity.
+    cd_stress: per slice np x 25 Real material and, with attenuation, per
+      attenuation point 2 Real moduli + nsls x 6 Real memory variables and
+      6 Real memory stress, the last two read and written.
+    cd_quad: element force write per point-mode of 3D elements, the 5
+      integral-factor rows.
+    point: element forces read, u, v, a read+written (FP64), per-point
+      data."""
     nupd = spec.n_updates_per_step()
     E, U = spec.n_solid_elements, spec.n_solid_points
     nr = spec.elem_nr.astype(np.int64)
     kind = getattr(spec, "elem_kind", None)
     is3d = np.zeros(E, bool) if kind is None else (np.asarray(kind) == 1)
-    slices = int(nr[is3d].sum())
+    mtype = getattr(spec, "elem_mtype", None)
+    npar = np.full(E, 2) if mtype is None else \
+        np.array([2, 5, 21])[np.asarray(mtype)]
+    # point-modes served by 3D elements, split by the elements' share of
+    # each point (the 3D workloads are all-3D, so this is exact there)
+    frac3d = float(is3d.mean()) if E else 0.0
+    n3d = int(is3d.sum())
+    nupd3 = int(round(nupd * frac3d))
+    nupd1 = nupd - nupd3
+    fr = nupd1 * (48 + 6 * real) + (E - n3d) * (10 * 25 * real + 25 * 4 + 1
+                                               + 25 * real * 2)
+    grad = nupd3 * 48 + n3d * (5 * 25 * real + 25 * 4 + 1)
+    stress = int((nr[is3d] * npar[is3d]).sum()) * 25 * real
     att = getattr(spec, "att", None)
-    stiff = nupd * (48 + 6 * real) + E * (10 * 25 * real + 25 * 4 + 1)
-    stiff += int((~is3d).sum()) * 2 * 25 * real + slices * 2 * 25 * real
     if att is not None:
         npa = 4 if att["cg4"] else 25
-        stiff += slices * (att["nsls"] * 6 * npa * real * 2 +
-                           6 * npa * real * 2 + 2 * npa * real)
+        slices = int(nr[is3d].sum())
+        stress += slices * npa * real * (2 + 2 * 6 * att["nsls"] + 2 * 6)
+    quad = nupd3 * 6 * real + n3d * 5 * 25 * real
     point = nupd * (6 * real + 6 * 48 + 4) + U * (4 + 4 + 4 + 1 + 8 + 16)
-    return {"stiffness": int(stiff), "point": int(point)}
+    return {"fr": int(fr), "cd_grad": int(grad), "cd_stress": int(stress),
+            "cd_quad": int(quad), "point": int(point)}
+
+
+def pmc_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE per dispatch, each
+    scaled by what the counter reads for a copy kernel of known size), or
+    None when no summary covers it."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                        "profiles", "hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) >= 6 and f[0] == kernel and f[1] == workload:
+            fetch_kib, fetch_fac = float(f[2]), float(f[3])
+            write_kib, write_fac = float(f[4]), float(f[5])
+            return (fetch_kib * fetch_fac + write_kib * write_fac) * 1024.0
+    return None
 
 
 def time_gpu(spec, prec, steps, warmup):
@@ -247,32 +285,44 @@ def main():
 
     names = {capi.KERNEL_STIFFNESS: "solid stiffness, Fourier path",
              capi.KERNEL_STIFFNESS_3D: "solid stiffness, cardinal path "
-                                       "(FFT, 3D stress, attenuation)",
-             capi.KERNEL_POINT: "gather/axis/mass/Newmark point update",
+                                       "(all three kernels)",
+             capi.KERNEL_CD_GRAD: "k_cd_grad",
+             capi.KERNEL_CD_STRESS: "k_cd_stress",
+             capi.KERNEL_CD_QUAD: "k_cd_quad",
+             capi.KERNEL_POINT: "k_point",
              capi.KERNEL_SOURCE: "moment source",
              capi.KERNEL_RECORD: "receivers"}
     b = bytes_per_step(spec, 4)
-    byts = {capi.KERNEL_STIFFNESS: b["stiffness"],
-            capi.KERNEL_STIFFNESS_3D: b["stiffness"],
+    # algorithmic bytes per step of each timed group
+    byts = {capi.KERNEL_STIFFNESS: b["fr"],
+            capi.KERNEL_STIFFNESS_3D: b["cd_grad"] + b["cd_stress"] +
+            b["cd_quad"],
+            capi.KERNEL_CD_GRAD: b["cd_grad"],
+            capi.KERNEL_CD_STRESS: b["cd_stress"],
+            capi.KERNEL_CD_QUAD: b["cd_quad"],
             capi.KERNEL_POINT: b["point"]}
     kernels = {}
-    for k in (capi.KERNEL_STIFFNESS, capi.KERNEL_STIFFNESS_3D,
-              capi.KERNEL_POINT):
+    for k in byts:
         ms, cnt = prof32[k]
         if cnt == 0:
             continue
-        avg = ms / cnt
-        kernels[names[k]] = {"avg_us": avg * 1e3, "launches": cnt,
-                             "bytes_per_launch": byts[k],
-                             "achieved_GBs": byts[k] / (avg * 1e-3) / 1e9}
+        per_step = cnt / a.steps
+        kernels[names[k]] = {
+            "ms_per_step": ms / a.steps, "launches_per_step": per_step,
+            "avg_us": ms / cnt * 1e3,
+            "bytes_per_launch": byts[k] / per_step,
+            "achieved_GBs": byts[k] / (ms / a.steps * 1e-3) / 1e9}
     tune32 = prof32.pop("autotune")
     if out_64 is not None:
         prof64.pop("autotune")
-    dom = max((k for k in prof32 if prof32[k][1] > 0),
-              key=lambda k: prof32[k][0])
-    avg_ms = prof32[dom][0] / prof32[dom][1]
-    achieved = byts.get(dom, 0) / (avg_ms * 1e-3) / 1e9
-    step_bytes = b["stiffness"] + b["point"]
+    # dominant single kernel (the cardinal group is three kernels)
+    singles = [k for k in byts if k != capi.KERNEL_STIFFNESS_3D and
+               prof32[k][1] > 0]
+    dom = max(singles, key=lambda k: prof32[k][0])
+    dk = kernels[names[dom]]
+    achieved = dk["bytes_per_launch"] / (dk["avg_us"] * 1e-6) / 1e9
+    traffic = pmc_traffic(names[dom], a.workload)
+    step_bytes = sum(v for v in b.values())
     out = {
         "metric": METRIC, "value": value, "unit": UNIT, "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup,
@@ -285,7 +335,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": names[dom],
                      "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None},
+                     "traffic": traffic},
         "step_roofline": {"algorithmic_bytes_per_step": step_bytes,
                           "achieved_GBs": step_bytes / (t32 / a.steps) / 1e9,
                           "frac": step_bytes / (t32 / a.steps) / 1e9 /

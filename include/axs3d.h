@@ -65,7 +65,10 @@ enum axs3d_kernel_id {
     AXS3D_KERNEL_STIFFNESS_3D = 4,/* cardinal-path (3D) element stiffness */
     AXS3D_KERNEL_FLUID_STIFFNESS = 5, /* fluid element stiffness       */
     AXS3D_KERNEL_FLUID_POINT = 6, /* fluid coupling/BC/mass/Newmark     */
-    AXS3D_NUM_KERNELS = 7
+    AXS3D_KERNEL_CD_GRAD = 7,     /* cardinal path: gradient, per batch  */
+    AXS3D_KERNEL_CD_STRESS = 8,   /* cardinal path: FFT, stress, FFT     */
+    AXS3D_KERNEL_CD_QUAD = 9,     /* cardinal path: quadrature           */
+    AXS3D_NUM_KERNELS = 10
 };
 
 /* Context.  Replaces the construction of Domain + NewmarkTimeScheme in
@@ -213,6 +216,16 @@ int axs3d_set_attenuation1d(axs3d_ctx *ctx, const double *dlam_dmu);
 /* Solid points on the axis.  Replaces AxialBoundary::addPoint
  * (src/core/boundary/axial/AxialBoundary.hpp). */
 int axs3d_set_axial_points(axs3d_ctx *ctx, int n, const int32_t *points);
+
+/* 3D mass of solid points: for each listed point in order, its Nr values
+ * of the assembled mass at phi_k = 2 pi k / Nr (after
+ * axs3d_set_solid_points).  These points divide the stiffness by the mass
+ * slice by slice (C2R, times 1 / m, R2C) instead of by their 1D mass.
+ * Replaces Mass3D (src/core/point/mass/Mass3D.hpp:17-22, Mass3D.cpp:
+ * 56-68), which GLLPoint::release creates for a 3D density (src/preloop/
+ * se_model/gll/GLLPoint.cpp:54-77). */
+int axs3d_set_solid_mass3d(axs3d_ctx *ctx, int n, const int32_t *points,
+                           const double *mass);
 
 /* Moment-tensor source: element, Fourier pattern [nu1][6][25] complex and
  * the source-time function sampled at every step (FP64).  Replaces
