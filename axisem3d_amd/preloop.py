@@ -689,6 +689,30 @@ def add_receivers(spec: DomainSpec, s_list, z_list, phi_list):
     return spec
 
 
+def make_mass3d(spec: DomainSpec, points, amp=0.2, orders=3, seed=0):
+    """3D mass of the given solid points for a density rho_1d (1 + d(phi))
+    with d a seeded low-order Fourier series in phi (|d| <= amp): the
+    assembled mass of slice k is m_1d (1 + d(2 pi k / Nr)), since the
+    integral factors are the same for every slice (GLLPoint::release with
+    a 3D density, src/preloop/se_model/gll/GLLPoint.cpp:54-77).  Sets
+    spec.mass3d_points and spec.mass3d (the points' Nr values, in order)."""
+    rng = np.random.default_rng(seed)
+    points = np.asarray(points, dtype=np.int32)
+    out = []
+    for p in points:
+        nr = int(spec.solid_nr[p])
+        phi = 2.0 * np.pi * np.arange(nr) / nr
+        d = np.zeros(nr)
+        for n in range(1, orders + 1):
+            a, b = rng.uniform(-1.0, 1.0, 2)
+            d += a * np.cos(n * phi) + b * np.sin(n * phi)
+        d *= amp / max(np.abs(d).max(), 1e-300)
+        out.append(spec.solid_mass[p] * (1.0 + d))
+    spec.mass3d_points = points
+    spec.mass3d = np.concatenate(out) if out else np.zeros(0)
+    return spec
+
+
 # ----------------------------------------------------------- workloads
 def cartesian_homogeneous(nx=80, nz=100, width=80e3, depth=80e3, nu=2,
                           vp=6000.0, vs=3464.0, rho=2600.0, nsteps=3000,

@@ -142,6 +142,12 @@ class GpuSolver:
         axp = _c(s.axial_points, np.int32)
         self._chk(L.axs3d_set_axial_points(ctx, len(axp), axp.ctypes.data),
                   "set_axial")
+        m3p = getattr(s, "mass3d_points", None)
+        if m3p is not None and len(m3p) > 0:
+            m3p = _c(m3p, np.int32)
+            m3 = _c(s.mass3d, np.float64)
+            self._chk(L.axs3d_set_solid_mass3d(ctx, len(m3p), m3p.ctypes.data,
+                                               m3.ctypes.data), "set_mass3d")
         if s.src_elem >= 0 and s.src_pattern is not None:
             pat = _c(s.src_pattern, np.complex128)
             stf = _c(s.stf, np.float64)

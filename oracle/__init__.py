@@ -62,6 +62,7 @@ class OracleDomain(C.Structure):
         ("n_fsurf", C.c_int), ("fsurf_points", _p),
         ("n_sf", C.c_int), ("sf_solid", _p), ("sf_fluid", _p),
         ("sf_normal", _p),
+        ("mass3d_off", _p), ("mass3d", _p),
     ]
 
 
@@ -327,6 +328,17 @@ class Oracle:
                                                       dtype=np.int32))
             d.sf_normal = self._k(np.ascontiguousarray(spec.sf_normal,
                                                        dtype=np.float64))
+        m3p = getattr(spec, "mass3d_points", None)
+        if m3p is not None and len(m3p) > 0:
+            off = np.full(len(spec.solid_nr), -1, dtype=np.int64)
+            nr = np.asarray(spec.solid_nr, dtype=np.int64)
+            cum = 0
+            for p in np.asarray(m3p, dtype=np.int64):
+                off[p] = cum
+                cum += nr[p]
+            d.mass3d_off = self._k(off)
+            d.mass3d = self._k(np.ascontiguousarray(spec.mass3d,
+                                                    dtype=np.float64))
         d.dt = float(spec.dt)
         self.d = d
         self._att_state = None

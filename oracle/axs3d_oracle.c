@@ -774,6 +774,23 @@ void oracle_stiff_to_accel(const oracle_domain *d, cplx *stiff) {
         if (nr % 2 == 0)
             for (int c = 0; c < 3; c++)
                 s[(nu1 - 1) * 3 + c] = creal(s[(nu1 - 1) * 3 + c]);
+        if (d->mass3d_off && d->mass3d_off[ip] >= 0) {
+            /* Mass3D::computeAccel (src/core/point/mass/Mass3D.cpp:56-68):
+             * per component C2R, times 1 / m slice by slice, R2C */
+            const double *m = d->mass3d + d->mass3d_off[ip];
+            cplx *col = malloc(sizeof(cplx) * nu1);
+            double *r = malloc(sizeof(double) * nr);
+            for (int c = 0; c < 3; c++) {
+                for (int a = 0; a < nu1; a++) col[a] = s[a * 3 + c];
+                oracle_c2r(col, nr, r);
+                for (int k = 0; k < nr; k++) r[k] *= 1. / m[k];
+                oracle_r2c(r, nr, col);
+                for (int a = 0; a < nu1; a++) s[a * 3 + c] = col[a];
+            }
+            free(col);
+            free(r);
+            continue;
+        }
         double inv = 1. / d->point_mass[ip];
         for (int k = 0; k < nu1 * 3; k++) s[k] *= inv;
     }

@@ -78,6 +78,10 @@ typedef struct {
     const int32_t *sf_solid;
     const int32_t *sf_fluid;
     const double *sf_normal;      /* [pair][2] (ns, nz), fluid -> solid   */
+    /* 3D mass (NULL: all 1D): per solid point the offset of its Nr masses
+     * in mass3d, or -1 for a 1D mass */
+    const int64_t *mass3d_off;
+    const double *mass3d;
 } oracle_domain;
 
 void oracle_set_gmat(const double *g_gll, const double *g_glj);

@@ -610,3 +610,76 @@ def test_coupled_energy_is_conserved():
     # by a few per cent here; an inconsistent coupling grows or decays
     assert (tot.max() - tot.min()) / tot.mean() < 0.1
     assert np.mean(np.array(ef) / tot) > 0.02
+
+
+def _mass3d_spec(seed=5):
+    def nodal_nr(s, z):
+        return 3.0 + 9.0 * (s / 6e3) + 0.0 * z
+    spec = preloop.cartesian_homogeneous(
+        nx=4, nz=4, width=6e3, depth=4e3, nodal_nr=nodal_nr, nsteps=10,
+        src_depth=2.0e3, n_receivers=2, seed=seed)
+    pts = np.arange(0, len(spec.solid_nr), 3)
+    return preloop.make_mass3d(spec, pts, amp=0.25, seed=seed), pts
+
+
+def test_mass3d_constant_equals_mass1d():
+    # a 3D mass equal to the 1D mass in every slice divides like Mass1D,
+    # except that the C2R / R2C round trip drops the imaginary part of the
+    # mean (SolverFFTW ignores it), so that part is zero on 3D points
+    spec, pts = _mass3d_spec()
+    rng = np.random.default_rng(1)
+    o1 = oracle.Oracle(spec)
+    n = o1.nmodes
+    s = rng.standard_normal((n, 3)) + 1j * rng.standard_normal((n, 3))
+    ref = oracle.Oracle(_drop_mass3d(spec)).stiff_to_accel(s)
+    off = np.concatenate([[0], np.cumsum(spec.solid_nr // 2 + 1)])
+    ref[off[pts]] = ref[off[pts]].real
+    spec.mass3d = np.concatenate([np.full(int(spec.solid_nr[p]),
+                                          spec.solid_mass[p]) for p in pts])
+    got = oracle.Oracle(spec).stiff_to_accel(s)
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def _drop_mass3d(spec):
+    import copy
+    s2 = copy.copy(spec)
+    s2.mass3d_points = None
+    s2.mass3d = None
+    return s2
+
+
+def test_mass3d_matches_numpy_fft():
+    # per component: synthesis r_k = Re c_0 + 2 sum_{a>=1} Re(c_a e^{i a
+    # phi_k}) (SolverFFTW C2R, Nyquist doubled), r / m, then numpy's rfft /
+    # Nr with the Nyquist row halved (R2C); the Nyquist imaginary part is
+    # zeroed first (SolidPoint::computeStiffToAccel)
+    spec, pts = _mass3d_spec(seed=7)
+    o = oracle.Oracle(spec)
+    rng = np.random.default_rng(2)
+    s = rng.standard_normal((o.nmodes, 3)) + 1j * rng.standard_normal(
+        (o.nmodes, 3))
+    got = o.stiff_to_accel(s)
+    off = np.concatenate([[0], np.cumsum(spec.solid_nr // 2 + 1)])
+    moff = 0
+    checked = 0
+    for p in pts:
+        nr = int(spec.solid_nr[p])
+        nu1 = nr // 2 + 1
+        m = spec.mass3d[moff:moff + nr]
+        moff += nr
+        c = s[off[p]:off[p + 1]].copy()
+        if nr % 2 == 0:
+            c[-1] = c[-1].real
+        phi = 2 * np.pi * np.arange(nr) / nr
+        for comp in range(3):
+            r = np.full(nr, c[0, comp].real)
+            for a in range(1, nu1):
+                r += 2 * np.real(c[a, comp] * np.exp(1j * a * phi))
+            r /= m
+            ref = np.fft.rfft(r)[:nu1] / nr
+            if nr % 2 == 0:
+                ref[-1] *= 0.5
+            np.testing.assert_allclose(got[off[p]:off[p + 1], comp], ref,
+                                       rtol=0, atol=1e-12 * np.abs(ref).max())
+            checked += 1
+    assert checked == 3 * len(pts)
