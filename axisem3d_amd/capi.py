@@ -38,6 +38,8 @@ SIGNATURES = {
     "axs3d_set_solid_iso1d": (_i, [_p, _p]),
     "axs3d_set_axial_points": (_i, [_p, _i, _p]),
     "axs3d_set_solid_mass3d": (_i, [_p, _i, _p, _p]),
+    "axs3d_set_fluid_kinds": (_i, [_p, _p]),
+    "axs3d_set_fluid_k3d": (_i, [_p, _i, _i, _p]),
     "axs3d_set_element_kinds": (_i, [_p, _p]),
     "axs3d_set_solid_iso3d": (_i, [_p, _i, _i, _p]),
     "axs3d_set_attenuation": (_i, [_p, _i, _i, _p, _p, _p]),

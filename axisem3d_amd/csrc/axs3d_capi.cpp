@@ -192,6 +192,18 @@ int axs3d_set_axial_points(axs3d_ctx *ctx, int n, const int32_t *points) {
                    [&](axs3d::DomainBase &d) { d.setAxialPoints(n, points); });
 }
 
+int axs3d_set_fluid_kinds(axs3d_ctx *ctx, const uint8_t *kind) {
+    return guarded(ctx,
+                   [&](axs3d::DomainBase &d) { d.setFluidKinds(kind); });
+}
+
+int axs3d_set_fluid_k3d(axs3d_ctx *ctx, int e_begin, int e_end,
+                        const double *k) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) {
+        d.setFluidK3D(e_begin, e_end, k);
+    });
+}
+
 int axs3d_set_solid_mass3d(axs3d_ctx *ctx, int n, const int32_t *points,
                            const double *mass) {
     return guarded(ctx, [&](axs3d::DomainBase &d) {

@@ -39,6 +39,8 @@
 
 namespace axs3d {
 
+constexpr int CD_EMB = 16;             // gradient/quadrature: element-modes
+constexpr int CD_EMB_THREADS = CD_EMB * NPEM;  // per block
 constexpr int CD_PW = 3;               // stress kernel: waves per point
 constexpr int CD_PT = CD_PW * 64;      // threads per stress block
 
@@ -222,9 +224,10 @@ __device__ __forceinline__ void wstage(cpx<Real> *buf, int nb, int m,
     const int L = m * R;
 #pragma unroll 1
     for (int b = lane; b < nb; b += 64) {
+        // index products stay below 2^24: full-rate 24-bit multiplies
         const int blk = m == 1 ? b : (int)__umulhi((uint32_t)b, mg);
-        const int j = b - blk * m;
-        const int base = blk * L + j;
+        const int j = b - __mul24(blk, m);
+        const int base = __mul24(blk, L) + j;
         cpx<Real> v[R];
 #pragma unroll
         for (int r = 0; r < R; r++) v[r] = buf[pad(base + r * m)];
@@ -346,36 +349,48 @@ struct SliceMat {
 // AttenuationFull/CG4::apply): sigma -= sum_l m_l; m_l = alpha_l m_l +
 // beta_l ds_old + gamma_l ds_new; ds_old = ds_new, where dsn = ds_new is
 // the (dlambda, dmu) isotropic stress of the strain in the frame the
-// stress is formed in.  The solids go four at a time so their loads are
-// in flight together.
-template <typename Real>
-__device__ __forceinline__ void att_update(const CardArgs<Real> &a,
-                                           const SliceMat<Real> &M, int d,
-                                           int k, Real dsn, Real &sig) {
-    Real *ds = M.ds + (size_t)d * M.cs + k;
-    const Real dso = *ds;
-    Real *mv = M.mv + (size_t)d * M.cs + k;
-    const size_t ls = 6 * M.cs;
-    const int nsls = a.nsls;
-    for (int l0 = 0; l0 < nsls; l0 += 4) {
-        Real m[4];
+// stress is formed in.  mvk, dsk point at slice k of the slot's memory
+// variables and memory stress; abg = alpha, beta, gamma in the working
+// precision (LDS, [3][CD_MAX_SLS]).  The solids go four at a time so
+// their loads are in flight together.
+constexpr int CD_MAX_SLS = 16;
+template <typename Real, int NS>
+__device__ __forceinline__ void att_update(Real *mvk, Real *dsk, int cs,
+                                           int nsls, const Real *abg, int d,
+                                           Real dsn, Real &sig) {
+    const int od = d * cs;
+    const int ls = 6 * cs;
+    const Real dso = dsk[od];
+    if constexpr (NS > 0) {
+        // the common five-solid case (or any fixed count): unrolled
+        Real m[NS];
 #pragma unroll
-        for (int q = 0; q < 4; q++)
-            m[q] = l0 + q < nsls ? mv[(l0 + q) * ls] : (Real)0;
+        for (int l = 0; l < NS; l++) m[l] = mvk[od + l * ls];
 #pragma unroll
-        for (int q = 0; q < 4; q++) sig -= m[q];
+        for (int l = 0; l < NS; l++) sig -= m[l];
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int l = l0 + q;
-            if (l < nsls) {
-                const Real al = (Real)a.abg[l];
-                const Real be = (Real)a.abg[nsls + l];
-                const Real ga = (Real)a.abg[2 * nsls + l];
-                mv[l * ls] = al * m[q] + be * dso + ga * dsn;
+        for (int l = 0; l < NS; l++)
+            mvk[od + l * ls] = abg[l] * m[l] + abg[CD_MAX_SLS + l] * dso +
+                               abg[2 * CD_MAX_SLS + l] * dsn;
+    } else {
+        for (int l0 = 0; l0 < nsls; l0 += 4) {
+            Real m[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                m[q] = l0 + q < nsls ? mvk[od + (l0 + q) * ls] : (Real)0;
+#pragma unroll
+            for (int q = 0; q < 4; q++) sig -= m[q];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int l = l0 + q;
+                if (l < nsls)
+                    mvk[od + l * ls] = abg[l] * m[q] +
+                                       abg[CD_MAX_SLS + l] * dso +
+                                       abg[2 * CD_MAX_SLS + l] * dsn;
             }
         }
     }
-    *ds = dsn;
+    dsk[od] = dsn;
 }
 
 // Block LDS of the stress kernel: the stage twiddles of the element's Nr
@@ -385,6 +400,7 @@ template <typename Real>
 struct CdLds {
     cpx<Real> tw[CARD_TW_CAP];
     cpx<Real> wbuf[CD_PW][CD_BUF];
+    Real abg[3 * CD_MAX_SLS];      // SLS alpha, beta, gamma
     uint16_t pos[CARD_NR_CAP];     // buffer position of slice k
 };
 
@@ -393,19 +409,20 @@ struct CdLds {
 // RTZ for TI and anisotropic media on spherical meshes (SolidElement.cpp:
 // 177-193).  One thread per (element-mode, GLL point), ten element-modes
 // per block, the Fourier order fastest so that a row's consecutive orders
-// are written by consecutive lanes.
+// are written by consecutive lanes (CD_EMB = 16 orders: 128-byte runs in
+// FP32).
 template <typename Real, bool ROT>
-__global__ __launch_bounds__(STIFF_BLOCK) void k_cd_grad(CardArgs<Real> a,
+__global__ __launch_bounds__(CD_EMB_THREADS) void k_cd_grad(CardArgs<Real> a,
                                                          int em0, int em1) {
-    __shared__ cpx<Real> sU[EMB][3][NPEM];
+    __shared__ cpx<Real> sU[CD_EMB][3][NPEM];
     __shared__ Real sG[2][NPEM];
     const StiffArgs<Real> &s = a.s;
     const int t = threadIdx.x;
     if (t < 2 * NPEM) sG[t / NPEM][t % NPEM] = s.gtab[t];
-    const int eml = t % EMB;
-    const int ipnt = t < EMB * NPEM ? t / EMB : 0;
-    const int em = em0 + blockIdx.x * EMB + eml;
-    const bool active = t < EMB * NPEM && em < em1;
+    const int eml = t % CD_EMB;
+    const int ipnt = t < CD_EMB * NPEM ? t / CD_EMB : 0;
+    const int em = em0 + blockIdx.x * CD_EMB + eml;
+    const bool active = t < CD_EMB * NPEM && em < em1;
     const int i = ipnt / NPED, j = ipnt - (ipnt / NPED) * NPED;
     int e = 0, alpha = 0;
     cpx<Real> u[3] = {{0, 0}, {0, 0}, {0, 0}};
@@ -448,19 +465,19 @@ __global__ __launch_bounds__(STIFF_BLOCK) void k_cd_grad(CardArgs<Real> a,
 // C: quadrature of the element-modes [em0, em1) from the scratch stress
 // rows (rotated back from RTZ) into the element-local force buffer.
 template <typename Real, bool ROT>
-__global__ __launch_bounds__(STIFF_BLOCK) void k_cd_quad(CardArgs<Real> a,
+__global__ __launch_bounds__(CD_EMB_THREADS) void k_cd_quad(CardArgs<Real> a,
                                                          int em0, int em1) {
-    __shared__ cpx<Real> sX[EMB][3][NPEM];
-    __shared__ cpx<Real> sY[EMB][3][NPEM];
+    __shared__ cpx<Real> sX[CD_EMB][3][NPEM];
+    __shared__ cpx<Real> sY[CD_EMB][3][NPEM];
     __shared__ Real sG[2][NPEM];
     const StiffArgs<Real> &s = a.s;
     const int t = threadIdx.x;
     if (t < 2 * NPEM) sG[t / NPEM][t % NPEM] = s.gtab[t];
     __syncthreads();
-    const int eml = t % EMB;
-    const int ipnt = t < EMB * NPEM ? t / EMB : 0;
-    const int em = em0 + blockIdx.x * EMB + eml;
-    const bool active = t < EMB * NPEM && em < em1;
+    const int eml = t % CD_EMB;
+    const int ipnt = t < CD_EMB * NPEM ? t / CD_EMB : 0;
+    const int em = em0 + blockIdx.x * CD_EMB + eml;
+    const bool active = t < CD_EMB * NPEM && em < em1;
     const int i = ipnt / NPED, j = ipnt - (ipnt / NPED) * NPED;
     int e = 0, alpha = 0, nu1 = 0;
     bool axial = false;
@@ -511,14 +528,18 @@ __global__ __launch_bounds__(STIFF_BLOCK) void k_cd_quad(CardArgs<Real> a,
 // Between the transforms slice k sits at buffer position pos[k] (the
 // digit reversal of the in-place FFT); the slice loop runs over slices so
 // its global accesses stay consecutive.
-template <typename Real, bool ATT, int MT>
+template <typename Real, int NS, int MT>
 __device__ __forceinline__ void cd_stress_slices(const CardArgs<Real> &a,
                                                  cpx<Real> *b0, cpx<Real> *b1,
                                                  cpx<Real> *b2,
                                                  const uint16_t *pos,
                                                  const SliceMat<Real> &M,
-                                                 bool att, int nr, int t) {
-#pragma unroll 2
+                                                 const Real *abg, bool att,
+                                                 int nr, int t) {
+    // two slices in flight without attenuation; with it the memory
+    // variables of one slice already fill the registers
+    constexpr int UNR = NS != 0 ? 1 : 2;
+#pragma unroll UNR
     for (int k = t; k < nr; k += CD_PT) {
         const int p = pad(pos[k]);
         const cpx<Real> z01 = b0[p], z23 = b1[p], z45 = b2[p];
@@ -549,14 +570,17 @@ __device__ __forceinline__ void cd_stress_slices(const CardArgs<Real> &a,
         } else {
             aniso_hooke<Real, Real>(e, [&](int c) { return M.p(c, k); }, sv);
         }
-        if (ATT && att) {
+        if (NS != 0 && att) {
             const Real dl = M.dl[k], dm = M.dm[k];
             const Real dm2 = dm * (Real)2;
             const Real tr = e[0] + e[1] + e[2];
+            Real *mvk = M.mv + k, *dsk = M.ds + k;
+            const int cs = (int)M.cs;
 #pragma unroll
             for (int d = 0; d < 6; d++)
-                att_update(a, M, d, k,
-                           d < 3 ? tr * dl + e[d] * dm2 : e[d] * dm, sv[d]);
+                att_update<Real, NS>(
+                    mvk, dsk, cs, a.nsls, abg, d,
+                    d < 3 ? tr * dl + e[d] * dm2 : e[d] * dm, sv[d]);
         }
         b0[p] = {sv[0], sv[1]};
         b1[p] = {sv[2], sv[3]};
@@ -564,13 +588,13 @@ __device__ __forceinline__ void cd_stress_slices(const CardArgs<Real> &a,
     }
 }
 
-// B kernel.  OCC = minimum waves per SIMD the register allocation must
-// allow; picked per workload by measurement.  MT = material type of the
-// batch's elements (0 isotropic, 1 TI, 2 anisotropic).  25 blocks per
-// element of the batch list3d[w0 ..), one per GLL point.
-template <typename Real, bool ATT, int OCC, int MT>
-__global__ __launch_bounds__(CD_PT, OCC) void k_cd_stress(CardArgs<Real> a,
-                                                          int w0) {
+// B kernel.  MT = material type of the batch's elements (0 isotropic, 1
+// TI, 2 anisotropic); NS = number of SLS (0: no attenuation, -1: taken
+// from the arguments).  25 blocks per element of the batch list3d[w0 ..),
+// one per GLL point.
+template <typename Real, int NS, int MT>
+__global__ __launch_bounds__(CD_PT, CARD_OCC) void k_cd_stress(
+    CardArgs<Real> a, int w0) {
     __shared__ CdLds<Real> L;
     const int t = threadIdx.x;
     const int lane = t & 63;
@@ -589,6 +613,10 @@ __global__ __launch_bounds__(CD_PT, OCC) void k_cd_stress(CardArgs<Real> a,
         for (int x = t; x < ntw; x += CD_PT) L.tw[x] = twg[x];
         const uint16_t *pg = a.slice_pos + a.pos_off[nr];
         for (int x = t; x < nr; x += CD_PT) L.pos[x] = pg[x];
+        if (NS != 0 && t < 3 * a.nsls) {
+            const int c = t / a.nsls, l = t - c * a.nsls;
+            L.abg[c * CD_MAX_SLS + l] = (Real)a.abg[t];
+        }
     }
     __syncthreads();
     const bool even = (nr & 1) == 0;
@@ -598,15 +626,20 @@ __global__ __launch_bounds__(CD_PT, OCC) void k_cd_stress(CardArgs<Real> a,
     cpx<Real> *b = L.wbuf[q];
     pack_pair<Real>(b, r0 + 2 * q * rs, r0 + (2 * q + 1) * rs, nr, nu1, even,
                     lane);
-    wfft<Real>(b, nr, plan, true, L.tw, lane);
-    __syncthreads();
+    // both transforms from one call site (the inverse, the stress, then
+    // the forward transform), which halves the FFT's code
+#pragma unroll 1
+    for (int pass = 0; pass < 2; pass++) {
+        wfft<Real>(b, nr, plan, pass == 0, L.tw, lane);
+        if (pass == 1) break;
+        __syncthreads();
     {
         SliceMat<Real> M;
         M.ps = (size_t)NPEM * nr;
         M.par = a.prop3d + a.prop_off[e] + (size_t)pp * nr;
-        const int ip4 = ATT ? att_slot(pp, a.npa) : -1;
-        const bool att = ATT && ip4 >= 0;
-        if (ATT && att) {
+        const int ip4 = NS != 0 ? att_slot(pp, a.npa) : -1;
+        const bool att = NS != 0 && ip4 >= 0;
+        if (NS != 0 && att) {
             const int64_t sl0 = a.slice_off[e];
             const int npa = a.npa;
             M.cs = (size_t)npa * nr;
@@ -616,40 +649,28 @@ __global__ __launch_bounds__(CD_PT, OCC) void k_cd_stress(CardArgs<Real> a,
                    (size_t)ip4 * nr;
             M.ds = a.dstress + (size_t)sl0 * 6 * npa + (size_t)ip4 * nr;
         }
-        cd_stress_slices<Real, ATT, MT>(a, L.wbuf[0], L.wbuf[1], L.wbuf[2],
-                                        L.pos, M, att, nr, t);
+        cd_stress_slices<Real, NS, MT>(a, L.wbuf[0], L.wbuf[1], L.wbuf[2],
+                                        L.pos, M, L.abg, att, nr, t);
     }
-    __syncthreads();
-    wfft<Real>(b, nr, plan, false, L.tw, lane);
+        __syncthreads();
+    }
     unpack_pair<Real>(b, r0 + 2 * q * rs, r0 + (2 * q + 1) * rs, nr, nu1,
                       even, lane);
 }
 
-// Occupancy variants for isotropic media (the common case); TI and
-// anisotropic media run the lowest-occupancy variant.
-template <typename Real, bool ATT, int MT>
+template <typename Real, int MT>
 static void launch_stress_mt(hipStream_t st, const CardArgs<Real> &a, int w0,
-                             int n, int occ) {
+                             int n) {
     const dim3 grid(n * NPEM), block(CD_PT);
-    if constexpr (MT != 0) {
-        hipLaunchKernelGGL((k_cd_stress<Real, ATT, CARD_OCC_MIN, MT>), grid,
-                           block, 0, st, a, w0);
-    } else {
-        switch (occ) {
-            case 5:
-                hipLaunchKernelGGL((k_cd_stress<Real, ATT, 5, MT>), grid,
-                                   block, 0, st, a, w0);
-                break;
-            case 4:
-                hipLaunchKernelGGL((k_cd_stress<Real, ATT, 4, MT>), grid,
-                                   block, 0, st, a, w0);
-                break;
-            default:
-                hipLaunchKernelGGL((k_cd_stress<Real, ATT, 3, MT>), grid,
-                                   block, 0, st, a, w0);
-                break;
-        }
-    }
+    if (a.nsls == 0)
+        hipLaunchKernelGGL((k_cd_stress<Real, 0, MT>), grid, block, 0, st, a,
+                           w0);
+    else if (a.nsls == 5)
+        hipLaunchKernelGGL((k_cd_stress<Real, 5, MT>), grid, block, 0, st, a,
+                           w0);
+    else
+        hipLaunchKernelGGL((k_cd_stress<Real, -1, MT>), grid, block, 0, st,
+                           a, w0);
 }
 
 // The three launches of a batch (gradient, stress, quadrature); batches
@@ -658,53 +679,248 @@ static void launch_stress_mt(hipStream_t st, const CardArgs<Real> &a, int w0,
 template <typename Real>
 void launch_cd_grad(hipStream_t st, const CardArgs<Real> &a,
                     const CardBatch &bt) {
-    const int gem = (bt.em_end - bt.em_begin + EMB - 1) / EMB;
+    const int gem = (bt.em_end - bt.em_begin + CD_EMB - 1) / CD_EMB;
     if (gem <= 0) return;
     if (bt.mtype != 0 && a.s.rot != nullptr)
         hipLaunchKernelGGL((k_cd_grad<Real, true>), dim3(gem),
-                           dim3(STIFF_BLOCK), 0, st, a, bt.em_begin,
+                           dim3(CD_EMB_THREADS), 0, st, a, bt.em_begin,
                            bt.em_end);
     else
         hipLaunchKernelGGL((k_cd_grad<Real, false>), dim3(gem),
-                           dim3(STIFF_BLOCK), 0, st, a, bt.em_begin,
+                           dim3(CD_EMB_THREADS), 0, st, a, bt.em_begin,
                            bt.em_end);
 }
 
 template <typename Real>
 void launch_cd_stress(hipStream_t st, const CardArgs<Real> &a,
-                      const CardBatch &bt, int occ) {
+                      const CardBatch &bt) {
     const int n = bt.end - bt.begin;
     if (n <= 0) return;
-    const bool att = a.nsls > 0;
     switch (bt.mtype) {
-        case 0:
-            if (att) launch_stress_mt<Real, true, 0>(st, a, bt.begin, n, occ);
-            else launch_stress_mt<Real, false, 0>(st, a, bt.begin, n, occ);
-            break;
-        case 1:
-            if (att) launch_stress_mt<Real, true, 1>(st, a, bt.begin, n, occ);
-            else launch_stress_mt<Real, false, 1>(st, a, bt.begin, n, occ);
-            break;
-        default:
-            if (att) launch_stress_mt<Real, true, 2>(st, a, bt.begin, n, occ);
-            else launch_stress_mt<Real, false, 2>(st, a, bt.begin, n, occ);
-            break;
+        case 0: launch_stress_mt<Real, 0>(st, a, bt.begin, n); break;
+        case 1: launch_stress_mt<Real, 1>(st, a, bt.begin, n); break;
+        default: launch_stress_mt<Real, 2>(st, a, bt.begin, n); break;
     }
 }
 
 template <typename Real>
 void launch_cd_quad(hipStream_t st, const CardArgs<Real> &a,
                     const CardBatch &bt) {
-    const int gem = (bt.em_end - bt.em_begin + EMB - 1) / EMB;
+    const int gem = (bt.em_end - bt.em_begin + CD_EMB - 1) / CD_EMB;
     if (gem <= 0) return;
     if (bt.mtype != 0 && a.s.rot != nullptr)
         hipLaunchKernelGGL((k_cd_quad<Real, true>), dim3(gem),
-                           dim3(STIFF_BLOCK), 0, st, a, bt.em_begin,
+                           dim3(CD_EMB_THREADS), 0, st, a, bt.em_begin,
                            bt.em_end);
     else
         hipLaunchKernelGGL((k_cd_quad<Real, false>), dim3(gem),
-                           dim3(STIFF_BLOCK), 0, st, a, bt.em_begin,
+                           dim3(CD_EMB_THREADS), 0, st, a, bt.em_begin,
                            bt.em_end);
+}
+
+// ------------------------------------------------- fluid, 3D medium
+// FluidElement::displToStiff, cardinal branch (src/core/element/
+// FluidElement.cpp:101-148): computeGrad3 (GradientQuadrature.hpp:163-
+// 175) of the displacement potential into the scratch rows [c][25][nu1]
+// (c = dp/ds, i alpha p / s, dp/dz), C2R of the three series, K = 1 / rho
+// per slice (Acoustic::strainToStress _3D_CD, Acoustic.hpp:80-92), R2C,
+// computeQuad3 (GradientQuadrature.hpp:234-252) into the fluid
+// element-local forces.  The same three-kernel split as the solids.
+template <typename Real>
+__global__ __launch_bounds__(CD_EMB_THREADS) void k_cdf_grad(
+    FluidCardArgs<Real> a, int em0, int em1) {
+    __shared__ cpx<Real> sP[CD_EMB][NPEM];
+    __shared__ Real sG[2][NPEM];
+    const FluidArgs<Real> &f = a.f;
+    const int t = threadIdx.x;
+    if (t < 2 * NPEM) sG[t / NPEM][t % NPEM] = f.gtab[t];
+    const int eml = t % CD_EMB;
+    const int ipnt = t < CD_EMB * NPEM ? t / CD_EMB : 0;
+    const int em = em0 + blockIdx.x * CD_EMB + eml;
+    const bool active = t < CD_EMB * NPEM && em < em1;
+    const int i = ipnt / NPED, j = ipnt - (ipnt / NPED) * NPED;
+    int e = 0, alpha = 0;
+    cpx<Real> p = {0, 0};
+    if (active) {
+        const EMItem it = f.em_list[em];
+        e = it.elem;
+        alpha = it.alpha;
+        const int ip = f.elem_points[e * NPEM + ipnt];
+        if (alpha < f.point_nu1[ip]) {
+            const double *src = f.chi + (size_t)(f.point_off[ip] + alpha) * 2;
+            p = {(Real)src[0], (Real)src[1]};
+        }
+        sP[eml][ipnt] = p;
+    }
+    __syncthreads();
+    if (!active) return;
+    const bool axial = f.elem_axial[e] != 0;
+    PointG<Real> G;
+    load_point_g<Real>(sG, axial, i, j, G);
+    PointOps<Real> g{}, q{};
+    load_point_ops(f.elem_ops, e, ipnt, g, q);
+    cpx<Real> x = {0, 0}, y = {0, 0};
+#pragma unroll
+    for (int k = 0; k < NPED; k++) {
+        x = cfma(sP[eml][k * NPED + j], G.gxi_col[k], x);
+        y = cfma(sP[eml][i * NPED + k], G.g_col[k], y);
+    }
+    const Real ralpha = (Real)alpha;
+    const cpx<Real> dpds = cadd(cmul(x, g.dzdeta), cmul(y, g.dzdxii));
+    const cpx<Real> dpdz = cadd(cmul(x, g.dsdeta), cmul(y, g.dsdxii));
+    const cpx<Real> ps = (axial && i == 0) ? ialpha(ralpha, dpds)
+                                           : cmul(ialpha(ralpha, p), g.invs);
+    const int nu1 = f.elem_nu1[e];
+    cpx<Real> *dst = reinterpret_cast<cpx<Real> *>(a.scratch) + a.scr_off[e] +
+                     (size_t)ipnt * nu1 + alpha;
+    const size_t rs = (size_t)NPEM * nu1;
+    dst[0] = dpds;
+    dst[rs] = ps;
+    dst[2 * rs] = dpdz;
+}
+
+// One fluid GLL point per block of two waves: wave 0 transforms the pair
+// (dp/ds, p_phi), wave 1 dp/dz (y = 0); each scales its own buffer by K
+// slice by slice and transforms it back.
+template <typename Real>
+__global__ __launch_bounds__(128) void k_cdf_stress(FluidCardArgs<Real> a,
+                                                    int w0) {
+    __shared__ cpx<Real> tw[CARD_TW_CAP];
+    __shared__ cpx<Real> wbuf[2][CD_BUF];
+    __shared__ uint16_t pos[CARD_NR_CAP];
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int q = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int w = w0 + (int)blockIdx.x / NPEM;
+    const int pp = (int)blockIdx.x % NPEM;
+    const int e = a.list3d[w];
+    const int nr = a.elem_nr[e];
+    const int nu1 = nr / 2 + 1;
+    const uint32_t *plan = a.plan + (size_t)nr * CARD_PLAN_W;
+    {
+        const cpx<Real> *twg = a.twiddle + a.tw_off[nr];
+        const int ntw = (int)(plan[0] >> 8);
+        for (int x = t; x < ntw; x += 128) tw[x] = twg[x];
+        const uint16_t *pg = a.slice_pos + a.pos_off[nr];
+        for (int x = t; x < nr; x += 128) pos[x] = pg[x];
+    }
+    __syncthreads();
+    const bool even = (nr & 1) == 0;
+    const size_t rs = (size_t)NPEM * nu1;
+    cpx<Real> *r0 = reinterpret_cast<cpx<Real> *>(a.scratch) + a.scr_off[e] +
+                    (size_t)pp * nu1;
+    cpx<Real> *b = wbuf[q];
+    // wave 1: rows (2, zero); the zero row is read as row 2's imaginary
+    // partner, so pack it from row 2 and clear the y part
+    if (q == 0) {
+        pack_pair<Real>(b, r0, r0 + rs, nr, nu1, even, lane);
+    } else {
+        const cpx<Real> *X = r0 + 2 * rs;
+        const int nyq = even ? nu1 - 1 : -1;
+        for (int k = lane; k < nr; k += 64) {
+            const bool lo = k < nu1;
+            const cpx<Real> x = X[lo ? k : nr - k];
+            const bool real_only = k == 0 || k == nyq;
+            const Real sc = k == nyq ? (Real)2 : (Real)1;
+            const Real sim = real_only ? (Real)0 : (lo ? (Real)1 : (Real)-1);
+            b[pad(k)] = {sc * x.re, sim * x.im};
+        }
+        wave_sync();
+    }
+    wfft<Real>(b, nr, plan, true, tw, lane);
+    const Real *K = a.k3d + a.k3d_off[e] + (size_t)pp * nr;
+    for (int k = lane; k < nr; k += 64) {
+        const int pk = pad(pos[k]);
+        const Real kk = K[k];
+        const cpx<Real> z = b[pk];
+        b[pk] = {z.re * kk, z.im * kk};
+    }
+    wave_sync();
+    wfft<Real>(b, nr, plan, false, tw, lane);
+    if (q == 0) {
+        unpack_pair<Real>(b, r0, r0 + rs, nr, nu1, even, lane);
+    } else {
+        cpx<Real> *X = r0 + 2 * rs;
+        const Real h0 = (Real)0.5 / (Real)nr;
+        for (int k = lane; k < nu1; k += 64) {
+            const cpx<Real> zk = b[pad(k)];
+            const cpx<Real> zn = b[pad(k == 0 ? 0 : nr - k)];
+            const Real h = (even && k == nu1 - 1) ? h0 * (Real)0.5 : h0;
+            X[k] = {(zk.re + zn.re) * h, (zk.im - zn.im) * h};
+        }
+    }
+}
+
+template <typename Real>
+__global__ __launch_bounds__(CD_EMB_THREADS) void k_cdf_quad(
+    FluidCardArgs<Real> a, int em0, int em1) {
+    __shared__ cpx<Real> sX[CD_EMB][NPEM];
+    __shared__ cpx<Real> sY[CD_EMB][NPEM];
+    __shared__ Real sG[2][NPEM];
+    const FluidArgs<Real> &f = a.f;
+    const int t = threadIdx.x;
+    if (t < 2 * NPEM) sG[t / NPEM][t % NPEM] = f.gtab[t];
+    const int eml = t % CD_EMB;
+    const int ipnt = t < CD_EMB * NPEM ? t / CD_EMB : 0;
+    const int em = em0 + blockIdx.x * CD_EMB + eml;
+    const bool active = t < CD_EMB * NPEM && em < em1;
+    const int i = ipnt / NPED, j = ipnt - (ipnt / NPED) * NPED;
+    int e = 0, alpha = 0, nu1 = 0;
+    bool axial = false;
+    cpx<Real> Lt = {0, 0};
+    if (active) {
+        const EMItem it = f.em_list[em];
+        e = it.elem;
+        alpha = it.alpha;
+        nu1 = f.elem_nu1[e];
+        axial = f.elem_axial[e] != 0;
+        const cpx<Real> *src = reinterpret_cast<const cpx<Real> *>(a.scratch) +
+                               a.scr_off[e] + (size_t)ipnt * nu1 + alpha;
+        const size_t rs = (size_t)NPEM * nu1;
+        const cpx<Real> q0 = src[0], q1 = src[rs], q2 = src[2 * rs];
+        PointOps<Real> g{}, q{};
+        load_point_ops(f.elem_ops, e, ipnt, g, q);
+        cpx<Real> X, Y;
+        quad_xy(q0, q2, q, X, Y);
+        const cpx<Real> bq = ialpha((Real)alpha, q1);
+        Lt = cmul(bq, -q.invs);
+        if (axial && i == 0) {
+            X = csub(X, cmul(bq, q.dzdeta));
+            Y = csub(Y, cmul(bq, q.dzdxii));
+        }
+        sX[eml][ipnt] = X;
+        sY[eml][ipnt] = Y;
+    }
+    __syncthreads();
+    if (!active) return;
+    if (alpha >= f.point_nu1[f.elem_points[e * NPEM + ipnt]]) return;
+    PointG<Real> G;
+    load_point_g<Real>(sG, axial, i, j, G);
+    cpx<Real> acc = {0, 0}, bsum = {0, 0};
+#pragma unroll
+    for (int k = 0; k < NPED; k++) {
+        acc = cfma(sX[eml][k * NPED + j], G.gxi_row[k], acc);
+        bsum = cfma(sY[eml][i * NPED + k], G.g_row[k], bsum);
+    }
+    const cpx<Real> F = cadd(cadd(acc, bsum), Lt);
+    Real *dst = f.eb + ((size_t)f.elem_eb_off[e] + (size_t)ipnt * nu1 +
+                        alpha) * 2;
+    dst[0] = F.re;
+    dst[1] = F.im;
+}
+
+template <typename Real>
+void launch_cdf_batch(hipStream_t st, const FluidCardArgs<Real> &a,
+                      const CardBatch &bt) {
+    const int n = bt.end - bt.begin;
+    const int gem = (bt.em_end - bt.em_begin + CD_EMB - 1) / CD_EMB;
+    if (n <= 0 || gem <= 0) return;
+    hipLaunchKernelGGL(k_cdf_grad<Real>, dim3(gem), dim3(CD_EMB_THREADS), 0,
+                       st, a, bt.em_begin, bt.em_end);
+    hipLaunchKernelGGL(k_cdf_stress<Real>, dim3(n * NPEM), dim3(128), 0, st,
+                       a, bt.begin);
+    hipLaunchKernelGGL(k_cdf_quad<Real>, dim3(gem), dim3(CD_EMB_THREADS), 0,
+                       st, a, bt.em_begin, bt.em_end);
 }
 
 // Mass3D::computeAccel (src/core/point/mass/Mass3D.cpp:56-68) for the
@@ -816,40 +1032,36 @@ void launch_mass3d(hipStream_t st, bool march, const Mass3DArgs<Real> &a,
                            st, a, stiff);
 }
 
-template <typename Real, bool ATT, int OCC>
-static int blocks_per_cu_v() {
+template <typename Real>
+int card_blocks_per_cu(bool att) {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &n, k_cd_stress<Real, ATT, OCC, 0>, CD_PT, 0) !=
-        hipSuccess)
-        return 1;
+    const hipError_t r =
+        att ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                  &n, k_cd_stress<Real, 5, 0>, CD_PT, 0)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                  &n, k_cd_stress<Real, 0, 0>, CD_PT, 0);
+    if (r != hipSuccess) return 1;
     return n > 0 ? n : 1;
 }
 
-template <typename Real>
-int card_blocks_per_cu(bool att, int occ) {
-    if (att) {
-        if (occ == 5) return blocks_per_cu_v<Real, true, 5>();
-        if (occ == 4) return blocks_per_cu_v<Real, true, 4>();
-        return blocks_per_cu_v<Real, true, 3>();
-    }
-    if (occ == 5) return blocks_per_cu_v<Real, false, 5>();
-    if (occ == 4) return blocks_per_cu_v<Real, false, 4>();
-    return blocks_per_cu_v<Real, false, 3>();
-}
-
-template int card_blocks_per_cu<float>(bool, int);
-template int card_blocks_per_cu<double>(bool, int);
+template int card_blocks_per_cu<float>(bool);
+template int card_blocks_per_cu<double>(bool);
 #define AXS3D_CD_INST(Real)                                                   \
     template void launch_cd_grad<Real>(hipStream_t, const CardArgs<Real> &, \
                                        const CardBatch &);                  \
     template void launch_cd_stress<Real>(hipStream_t,                       \
                                          const CardArgs<Real> &,            \
-                                         const CardBatch &, int);           \
+                                         const CardBatch &);                \
     template void launch_cd_quad<Real>(hipStream_t, const CardArgs<Real> &, \
                                        const CardBatch &);
 AXS3D_CD_INST(float)
 AXS3D_CD_INST(double)
+template void launch_cdf_batch<float>(hipStream_t,
+                                      const FluidCardArgs<float> &,
+                                      const CardBatch &);
+template void launch_cdf_batch<double>(hipStream_t,
+                                       const FluidCardArgs<double> &,
+                                       const CardBatch &);
 template void launch_mass3d<float>(hipStream_t, bool,
                                    const Mass3DArgs<float> &, float *);
 template void launch_mass3d<double>(hipStream_t, bool,

@@ -173,6 +173,49 @@ void DomainBase::setFluidK1D(const double *k) {
     hFElemK.assign(k, k + (size_t)mNumFElems * NPEM);
 }
 
+void DomainBase::setFluidKinds(const uint8_t *kind) {
+    if (mNumFElems <= 0 || hFPointNr.empty())
+        throw Error("Domain::setFluidKinds || Set fluid points and elements "
+                    "first.");
+    hFElemKind.assign(kind, kind + mNumFElems);
+    hFElemNr.assign(mNumFElems, 0);
+    hFK3DOff.assign(mNumFElems, -1);
+    int64_t total = 0;
+    for (int e = 0; e < mNumFElems; e++) {
+        if (hFElemKind[e] > 1)
+            throw Error("Domain::setFluidKinds || Unknown fluid element "
+                        "kind.");
+        int nrmax = 0;
+        for (int k = 0; k < NPEM; k++) {
+            const int p = hFElemPoints[(size_t)e * NPEM + k];
+            if (p < 0 || p >= (int)hFPointNr.size())
+                throw Error("Domain::setFluidKinds || Fluid element point "
+                            "out of range.");
+            nrmax = std::max(nrmax, hFPointNr[p]);
+        }
+        hFElemNr[e] = nrmax;
+        if (hFElemKind[e] == 1) {
+            hFK3DOff[e] = total;
+            total += (int64_t)nrmax * NPEM;
+        }
+    }
+    hFK3D.assign((size_t)total, -1.0);
+}
+
+void DomainBase::setFluidK3D(int e_begin, int e_end, const double *k) {
+    if (hFElemKind.empty())
+        throw Error("Domain::setFluidK3D || Set the fluid element kinds "
+                    "first.");
+    if (e_begin < 0 || e_end > mNumFElems || e_begin > e_end)
+        throw Error("Domain::setFluidK3D || Element range out of bounds.");
+    for (int e = e_begin; e < e_end; e++) {
+        if (hFElemKind[e] != 1) continue;
+        const size_t n = (size_t)hFElemNr[e] * NPEM;
+        std::copy(k, k + n, hFK3D.begin() + hFK3DOff[e]);
+        k += n;
+    }
+}
+
 void DomainBase::setFluidAxialPoints(int n, const int32_t *points) {
     hFAxial.assign(points, points + std::max(n, 0));
 }
@@ -474,6 +517,23 @@ void DomainBase::finalize() {
     std::vector<int> fftNr;
     for (int32_t e : list3d) fftNr.push_back(hElemNr[e]);
     for (int32_t p : hMass3DPts) fftNr.push_back(hPointNr[p]);
+    // fluid elements with a 3D medium, heaviest first
+    hFList3D.clear();
+    if (mNumFElems > 0 && !hFElemKind.empty()) {
+        for (int e = 0; e < mNumFElems; e++)
+            if (hFElemKind[e] == 1) {
+                for (size_t k = 0; k < (size_t)hFElemNr[e] * NPEM; k++)
+                    if (!(hFK3D[hFK3DOff[e] + k] > 0.))
+                        throw Error("Domain::finalize || 3D fluid medium "
+                                    "not set (or not positive).");
+                hFList3D.push_back(e);
+                fftNr.push_back(hFElemNr[e]);
+            }
+        std::stable_sort(hFList3D.begin(), hFList3D.end(), [&](int a, int b) {
+            return hFElemNr[a] > hFElemNr[b];
+        });
+    }
+    mNumF3D = (int)hFList3D.size();
     mNumMass3D = (int)hMass3DPts.size();
     if (!fftNr.empty()) {
         int nrmax = 0;
@@ -619,7 +679,8 @@ void DomainBase::finalize() {
                 nrmax = std::max(nrmax, hFPointNr[p]);
             }
             fElemNu1[e] = nrmax / 2 + 1;
-            for (int a = 0; a < fElemNu1[e]; a++) fem.push_back({e, a});
+            if (hFElemKind.empty() || hFElemKind[e] == 0)
+                for (int a = 0; a < fElemNu1[e]; a++) fem.push_back({e, a});
             fEbOff[e] = (uint32_t)eb;
             eb += (int64_t)NPEM * fElemNu1[e];
         }
@@ -750,7 +811,12 @@ void DomainBase::finalize() {
         up(dFElemPoints, hFElemPoints.data(), hFElemPoints.size() * 4);
         up(dFElemAxial, hFElemAxial.data(), hFElemAxial.size());
         up(dFElemNu1, fElemNu1.data(), fElemNu1.size() * 4);
-        up(dFEMList, fem.data(), fem.size() * sizeof(EMItem));
+        if (!fem.empty())
+            up(dFEMList, fem.data(), fem.size() * sizeof(EMItem));
+        if (mNumF3D > 0) {
+            up(dFList3D, hFList3D.data(), hFList3D.size() * 4);
+            up(dFElemNr, hFElemNr.data(), hFElemNr.size() * 4);
+        }
         up(dFEBOff, fEbOff.data(), fEbOff.size() * 4);
         const size_t fb = (size_t)mNumFModes * 2 * sizeof(double);
         dChi.alloc(fb);
@@ -1030,6 +1096,24 @@ public:
         return f;
     }
 
+    FluidCardArgs<Real> fluidCardArgs() const {
+        FluidCardArgs<Real> c;
+        c.f = fluidArgs();
+        c.f.em_list = dFEM3D.as<EMItem>();
+        c.list3d = dFList3D.as<int32_t>();
+        c.elem_nr = dFElemNr.as<int32_t>();
+        c.k3d = dFK3D.as<Real>();
+        c.k3d_off = dFK3DOff.as<int64_t>();
+        c.twiddle = dTwiddle.as<cpx<Real>>();
+        c.tw_off = dTwOff.as<int64_t>();
+        c.plan = dPlan.as<uint32_t>();
+        c.slice_pos = dPosTab.as<uint16_t>();
+        c.pos_off = dPosOff.as<int64_t>();
+        c.scratch = dCardScratch.as<Real>();
+        c.scr_off = dFScrOff.as<uint32_t>();
+        return c;
+    }
+
     // Domain::applySources: the source force is subtracted from the point
     // stiffness (stage form).
     void applySources(int tstep) override {
@@ -1051,7 +1135,7 @@ public:
 
     // The cardinal path, batch by batch; with timing, each of the three
     // kernels is timed under its own id.
-    void cardinalPath(int occ, bool timed) {
+    void cardinalPath(bool timed) {
         const CardArgs<Real> c = cardArgs();
         for (const CardBatch &bt : mCardBatches) {
             if (timed) tick(AXS3D_KERNEL_CD_GRAD);
@@ -1060,7 +1144,7 @@ public:
                 tock(AXS3D_KERNEL_CD_GRAD);
                 tick(AXS3D_KERNEL_CD_STRESS);
             }
-            launch_cd_stress<Real>(mStream, c, bt, occ);
+            launch_cd_stress<Real>(mStream, c, bt);
             if (timed) {
                 tock(AXS3D_KERNEL_CD_STRESS);
                 tick(AXS3D_KERNEL_CD_QUAD);
@@ -1078,12 +1162,18 @@ public:
         }
         if (mNum3D > 0) {
             tick(AXS3D_KERNEL_STIFFNESS_3D);
-            cardinalPath(mCardOcc, true);
+            cardinalPath(true);
             tock(AXS3D_KERNEL_STIFFNESS_3D);
         }
-        if (mNumFEM > 0) {
+        if (mNumFEM > 0 || mNumF3D > 0) {
             tick(AXS3D_KERNEL_FLUID_STIFFNESS);
-            launch_fluid_stiff_fr<Real>(mStream, fluidArgs());
+            if (mNumFEM > 0)
+                launch_fluid_stiff_fr<Real>(mStream, fluidArgs());
+            if (mNumF3D > 0) {
+                const FluidCardArgs<Real> fc = fluidCardArgs();
+                for (const CardBatch &bt : mFCardBatches)
+                    launch_cdf_batch<Real>(mStream, fc, bt);
+            }
             tock(AXS3D_KERNEL_FLUID_STIFFNESS);
         }
     }
@@ -1228,11 +1318,10 @@ private:
         throw Error("Domain::field || Unknown field id.");
     }
 
-    // Times the cardinal kernel's occupancy variants on this workload and
-    // keeps the fastest.  Runs right after finalize, on zero fields: the
-    // element forces come out zero and the attenuation state stays zero,
-    // so the trials leave no trace.  AXS3D_AUTOTUNE=0 keeps the lowest
-    // variant.
+    // Times one pass of the cardinal path right after finalize, on zero
+    // fields (the element forces come out zero and the attenuation state
+    // stays zero, so the trial leaves no trace), after one untimed pass
+    // that loads the code.  AXS3D_AUTOTUNE=0 skips it.
     void autotune() override {
         if (mNum3D == 0) return;
         const char *env = std::getenv("AXS3D_AUTOTUNE");
@@ -1240,26 +1329,14 @@ private:
         hipEvent_t a, b;
         hip_check(hipEventCreate(&a), "Domain::autotune");
         hip_check(hipEventCreate(&b), "Domain::autotune");
-        double best = 1e300;
-        int bestOcc = CARD_OCC_MIN;
-        for (int occ = CARD_OCC_MIN; occ <= CARD_OCC_MAX; occ++) {
-            mCardOcc = occ;
-            mCardSlots = mCardSlotsFor[occ];
-            cardinalPath(occ, false);
-            hip_check(hipEventRecord(a, mStream), "Domain::autotune");
-            cardinalPath(occ, false);
-            hip_check(hipEventRecord(b, mStream), "Domain::autotune");
-            hip_check(hipEventSynchronize(b), "Domain::autotune");
-            float ms = 0.f;
-            hip_check(hipEventElapsedTime(&ms, a, b), "Domain::autotune");
-            mCardTuneMs[occ] = ms;
-            if (ms < best) {
-                best = ms;
-                bestOcc = occ;
-            }
-        }
-        mCardOcc = bestOcc;
-        mCardSlots = mCardSlotsFor[bestOcc];
+        cardinalPath(false);
+        hip_check(hipEventRecord(a, mStream), "Domain::autotune");
+        cardinalPath(false);
+        hip_check(hipEventRecord(b, mStream), "Domain::autotune");
+        hip_check(hipEventSynchronize(b), "Domain::autotune");
+        float ms = 0.f;
+        hip_check(hipEventElapsedTime(&ms, a, b), "Domain::autotune");
+        mCardTuneMs = ms;
         (void)hipEventDestroy(a);
         (void)hipEventDestroy(b);
     }
@@ -1447,13 +1524,8 @@ private:
             hip_check(hipDeviceGetAttribute(
                           &ncu, hipDeviceAttributeMultiprocessorCount,
                           mDevice), "Domain::finalize");
-            for (int occ = CARD_OCC_MIN; occ <= CARD_OCC_MAX; occ++) {
-                const int per_cu = card_blocks_per_cu<Real>(mNsls > 0, occ);
-                mCardSlotsFor[occ] = std::max(1, std::min(mNum3D,
-                                                          per_cu * ncu));
-            }
-            mCardOcc = CARD_OCC_MIN;
-            mCardSlots = mCardSlotsFor[CARD_OCC_MIN];
+            mCardSlots = std::max(1, card_blocks_per_cu<Real>(mNsls > 0) *
+                                         ncu);
             // batches: runs of one material type whose strain and stress
             // ([150][nu1] complex per element) fit the scratch together;
             // the 3D element-modes in list order for the gradient and
@@ -1493,6 +1565,7 @@ private:
             dScrOff.upload(scrOff.data(), scrOff.size() * 4, st);
             dEM3D.alloc(em3d.size() * sizeof(EMItem));
             dEM3D.upload(em3d.data(), em3d.size() * sizeof(EMItem), st);
+            mCardScrMax = scrMax;
             dCardScratch.alloc(scrMax * sizeof(cpx<Real>));
             if (mNsls > 0) {
                 const int npa = mAttCG4 ? 4 : NPEM;
@@ -1501,6 +1574,60 @@ private:
                 dMemVar.zero(st);
                 dDStress.alloc((size_t)mNumSlices * 6 * npa * sizeof(Real));
                 dDStress.zero(st);
+            }
+        }
+        if (mNumF3D > 0) {
+            // K of the 3D fluid elements, host [Nr][25] -> device [25][Nr]
+            // per element, cast to Real
+            std::vector<Real> k3;
+            std::vector<int64_t> k3off(mNumFElems, 0);
+            for (int e = 0; e < mNumFElems; e++) {
+                if (hFElemKind[e] != 1) continue;
+                const int nr = hFElemNr[e];
+                k3off[e] = (int64_t)k3.size();
+                const double *src = hFK3D.data() + hFK3DOff[e];
+                for (int k = 0; k < NPEM; k++)
+                    for (int sl = 0; sl < nr; sl++)
+                        k3.push_back((Real)src[(size_t)sl * NPEM + k]);
+            }
+            dFK3D.alloc(k3.size() * sizeof(Real));
+            dFK3D.upload(k3.data(), k3.size() * sizeof(Real), st);
+            dFK3DOff.alloc(k3off.size() * 8);
+            dFK3DOff.upload(k3off.data(), k3off.size() * 8, st);
+            // batches whose [3][25][nu1] rows fit the scratch together
+            size_t budget = CARD_SCRATCH_BYTES;
+            if (const char *env = std::getenv("AXS3D_CARD_SCRATCH_MB"))
+                budget = (size_t)std::max(1L, std::atol(env)) << 20;
+            budget = std::min(budget / sizeof(cpx<Real>),
+                              (size_t)UINT32_MAX);
+            std::vector<uint32_t> fscr(mNumFElems, 0u);
+            std::vector<EMItem> fem3d;
+            mFCardBatches.clear();
+            size_t used = 0, scrMax = 0;
+            for (int w = 0; w < mNumF3D; w++) {
+                const int e = hFList3D[w];
+                const int nu1 = hFElemNr[e] / 2 + 1;
+                const size_t need = (size_t)3 * NPEM * nu1;
+                if (mFCardBatches.empty() || used + need > budget) {
+                    mFCardBatches.push_back(
+                        {0, w, w, (int)fem3d.size(), (int)fem3d.size()});
+                    used = 0;
+                }
+                fscr[e] = (uint32_t)used;
+                used += need;
+                for (int al = 0; al < nu1; al++) fem3d.push_back({e, al});
+                mFCardBatches.back().end = w + 1;
+                mFCardBatches.back().em_end = (int)fem3d.size();
+                scrMax = std::max(scrMax, used);
+            }
+            dFScrOff.alloc(fscr.size() * 4);
+            dFScrOff.upload(fscr.data(), fscr.size() * 4, st);
+            dFEM3D.alloc(fem3d.size() * sizeof(EMItem));
+            dFEM3D.upload(fem3d.data(), fem3d.size() * sizeof(EMItem), st);
+            // one scratch for both paths (they run one after the other)
+            if (scrMax > mCardScrMax) {
+                mCardScrMax = scrMax;
+                dCardScratch.alloc(scrMax * sizeof(cpx<Real>));
             }
         }
         if (hSrcElem >= 0) {

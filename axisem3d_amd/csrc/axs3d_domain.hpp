@@ -70,6 +70,8 @@ public:
     void setFluidElements(int n, const int32_t *points, const uint8_t *axial,
                           const double *gq);
     void setFluidK1D(const double *k);
+    void setFluidKinds(const uint8_t *kind);
+    void setFluidK3D(int e_begin, int e_end, const double *k);
     void setFluidAxialPoints(int n, const int32_t *points);
     void setFluidSurfacePoints(int n, const int32_t *points);
     void setSolidFluidCoupling(int n, const int32_t *solid,
@@ -121,7 +123,8 @@ public:
     void autotuneInfo(int *occ, int *slots, double *ms) const {
         *occ = mCardOcc;
         *slots = mCardSlots;
-        for (int k = 0; k < 3; k++) ms[k] = mCardTuneMs[k + CARD_OCC_MIN];
+        ms[0] = mCardTuneMs;
+        ms[1] = ms[2] = 0.;
     }
     int precision() const { return mPrecision; }
 
@@ -182,15 +185,23 @@ protected:
     int64_t mPropTotal = 0;             // parameters (np x 25 x Nr)
     int mNum3D = 0, mNu1Max3D = 0, mCardSlots = 0;
     std::vector<int32_t> hList3D;       // 3D elements, launch order
+    // fluid elements with a 3D medium
+    std::vector<uint8_t> hFElemKind;    // 0 1D, 1 3D
+    std::vector<int32_t> hFElemNr;      // fluid element Nr
+    std::vector<int64_t> hFK3DOff;      // [e] offset of [Nr][25] K, or -1
+    std::vector<double> hFK3D;
+    std::vector<int32_t> hFList3D;      // 3D fluid elements, launch order
+    std::vector<CardBatch> mFCardBatches;
+    int mNumF3D = 0;
+    size_t mCardScrMax = 0;             // scratch size, complex values
     // solid points with a 3D mass: points, offsets of their Nr masses
     std::vector<int32_t> hMass3DPts;
     std::vector<int64_t> hMass3DOff;
     std::vector<double> hMass3D;
     int mNumMass3D = 0;
     std::vector<CardBatch> mCardBatches;
-    int mCardOcc = CARD_OCC_MIN;        // chosen occupancy variant
-    int mCardSlotsFor[CARD_OCC_MAX + 1] = {};
-    double mCardTuneMs[CARD_OCC_MAX + 1] = {};
+    int mCardOcc = CARD_OCC;            // stress kernel waves per SIMD
+    double mCardTuneMs = 0.;            // one cardinal pass at finalize
     std::vector<uint8_t> hProp3DSet, hAtt3DSet;
     // attenuation
     int mNsls = 0, mAttCG4 = 0;
@@ -212,6 +223,7 @@ protected:
     // device: cardinal path
     DeviceBuffer dElemNr, dSliceOff, dPropOff, dList3D, dScrOff, dEM3D,
         dPosOff, dPosTab, dMass3DPts, dMass3DOff, dMass3DFlag, dInvMass3D,
+        dFList3D, dFElemNr, dFK3D, dFK3DOff, dFScrOff, dFEM3D,
         dProp3D,
         dTwiddle,
         dTwOff, dPlan, dCardScratch, dAttABG, dAtt3D, dMemVar, dDStress,

@@ -49,6 +49,24 @@ struct FluidArgs {
     const Real *gtab;
 };
 
+// Fluid elements with a 3D medium (cardinal path): their element-modes in
+// f.em_list, K per slice and the FFT tables.
+template <typename Real>
+struct FluidCardArgs {
+    FluidArgs<Real> f;             // f.em_list: 3D fluid element-modes
+    const int32_t *list3d;         // 3D fluid elements, launch order
+    const int32_t *elem_nr;        // fluid element Nr
+    const Real *k3d;               // per element [25][Nr] at k3d_off[e]
+    const int64_t *k3d_off;
+    const cpx<Real> *twiddle;
+    const int64_t *tw_off;
+    const uint32_t *plan;
+    const uint16_t *slice_pos;
+    const int64_t *pos_off;
+    Real *scratch;                 // batch scratch, complex
+    const uint32_t *scr_off;       // [fluid element] [3][25][nu1] part
+};
+
 // Device operands of the solid element kernels.
 template <typename Real>
 struct StiffArgs {
@@ -82,9 +100,8 @@ struct StiffArgs {
 // Largest element Nr of the cardinal kernel (one complex series of Nr per
 // wave in LDS).
 constexpr int CARD_NR_CAP = 832;
-// Occupancy variants of the cardinal kernel (minimum waves per SIMD).
-constexpr int CARD_OCC_MIN = 3;
-constexpr int CARD_OCC_MAX = 5;
+// Minimum waves per SIMD the stress kernel's register allocation allows.
+constexpr int CARD_OCC = 5;
 // FFT plan per Nr (in-place, decimation-in-frequency stage order): word 0
 // = number of stages | (length of the stage twiddle table << 8); per
 // stage s, words 1+3s..: radix | (m << 8) with m = span / radix,
@@ -181,21 +198,27 @@ struct CardBatch {
 // Default scratch of one batch (bytes).
 constexpr size_t CARD_SCRATCH_BYTES = (size_t)4 << 30;
 
-// The cardinal path of one batch: gradient, stress (occ: CARD_OCC_MIN ..
-// CARD_OCC_MAX = minimum waves per SIMD of its register allocation),
-// quadrature, in that order on one stream.
+// The cardinal path of one batch: gradient, stress, quadrature, in that
+// order on one stream.
 template <typename Real>
 void launch_cd_grad(hipStream_t st, const CardArgs<Real> &a,
                     const CardBatch &bt);
 template <typename Real>
 void launch_cd_stress(hipStream_t st, const CardArgs<Real> &a,
-                      const CardBatch &bt, int occ);
+                      const CardBatch &bt);
 template <typename Real>
 void launch_cd_quad(hipStream_t st, const CardArgs<Real> &a,
                     const CardBatch &bt);
 
+// The cardinal path of a batch of 3D fluid elements: gradient, K per
+// slice through the FFT, quadrature into the fluid element-local forces.
 template <typename Real>
-int card_blocks_per_cu(bool att, int occ);
+void launch_cdf_batch(hipStream_t st, const FluidCardArgs<Real> &a,
+                      const CardBatch &bt);
+
+// resident stress blocks per CU (with or without attenuation)
+template <typename Real>
+int card_blocks_per_cu(bool att);
 
 template <typename Real>
 void launch_moment_source(hipStream_t st, bool to_elem,

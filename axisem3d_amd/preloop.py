@@ -689,6 +689,36 @@ def add_receivers(spec: DomainSpec, s_list, z_list, phi_list):
     return spec
 
 
+def make_fluid3d(spec: DomainSpec, elems, amp=0.1, orders=3, seed=0):
+    """3D acoustic medium for the given fluid elements: K(phi) = K_1d (1 +
+    d(phi)) per GLL point with d a seeded low-order Fourier series in phi
+    (|d| <= amp), evaluated at the element's Nr slices (Material::
+    createAcoustic with a 3D density, Material.cpp:350-360).  Sets
+    spec.fluid_elem_kind (1 for these elements) and spec.fluid_k3d
+    ([Nr][25] per 3D element, in element order)."""
+    rng = np.random.default_rng(seed)
+    nfe = len(spec.fluid_elem_nr)
+    kind = np.zeros(nfe, dtype=np.uint8)
+    kind[np.asarray(elems, dtype=np.int64)] = 1
+    out = []
+    for e in range(nfe):
+        if not kind[e]:
+            continue
+        nr = int(spec.fluid_elem_nr[e])
+        phi = 2.0 * np.pi * np.arange(nr) / nr
+        d = np.zeros((nr, 25))
+        for n in range(1, orders + 1):
+            a, b = rng.uniform(-1.0, 1.0, (2, 25))
+            d += a[None, :] * np.cos(n * phi)[:, None] + \
+                b[None, :] * np.sin(n * phi)[:, None]
+        d *= amp / max(np.abs(d).max(), 1e-300)
+        out.append(spec.fluid_elem_k[e][None, :] * (1.0 + d))
+    spec.fluid_elem_kind = kind
+    spec.fluid_k3d = (np.concatenate([o.reshape(-1) for o in out])
+                      if out else np.zeros(0))
+    return spec
+
+
 def make_mass3d(spec: DomainSpec, points, amp=0.2, orders=3, seed=0):
     """3D mass of the given solid points for a density rho_1d (1 + d(phi))
     with d a seeded low-order Fourier series in phi (|d| <= amp): the

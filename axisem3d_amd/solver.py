@@ -124,6 +124,15 @@ class GpuSolver:
             fk = _c(s.fluid_elem_k, np.float64)
             self._chk(L.axs3d_set_fluid_k1d(ctx, fk.ctypes.data),
                       "set_fluid_k1d")
+            fkind = getattr(s, "fluid_elem_kind", None)
+            if fkind is not None:
+                fkind = _c(fkind, np.uint8)
+                self._chk(L.axs3d_set_fluid_kinds(ctx, fkind.ctypes.data),
+                          "set_fluid_kinds")
+                k3 = _c(s.fluid_k3d, np.float64)
+                self._chk(L.axs3d_set_fluid_k3d(ctx, 0, len(fkind),
+                                                k3.ctypes.data),
+                          "set_fluid_k3d")
             ax = _c(s.fluid_axial_points, np.int32)
             self._chk(L.axs3d_set_fluid_axial_points(ctx, len(ax),
                                                      ax.ctypes.data),
@@ -265,4 +274,4 @@ class GpuSolver:
                                                C.byref(blocks), ms))
         return {"cardinal_occupancy": occ.value,
                 "cardinal_blocks": blocks.value,
-                "trial_ms": {"3": ms[0], "4": ms[1], "5": ms[2]}}
+                "cardinal_pass_ms": ms[0]}

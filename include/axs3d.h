@@ -137,6 +137,23 @@ int axs3d_set_fluid_elements(axs3d_ctx *ctx, int n, const int32_t *points,
  * Material::createAcoustic, Material.cpp:350-360). */
 int axs3d_set_fluid_k1d(axs3d_ctx *ctx, const double *k);
 
+/* Fluid element kinds, one per fluid element: 0 = 1D medium (Fourier
+ * path), 1 = 3D medium (cardinal path: C2R of the three gradient
+ * components, K per slice, R2C).  Replaces the branch of
+ * FluidElement::displToStiff (src/core/element/FluidElement.cpp:101-148)
+ * that Acoustic's 1D/3D property selects.  After the fluid points and
+ * elements, before the 3D medium. */
+int axs3d_set_fluid_kinds(axs3d_ctx *ctx, const uint8_t *kind);
+
+/* 3D acoustic medium of the 3D fluid elements in [e_begin, e_end): per
+ * such element in order, Nr slices (element Nr, max over its points) of
+ * [25] K = 1 / rho at phi_k = 2 pi k / Nr.  May be called in chunks.
+ * Replaces the Acoustic 3D constructor (src/core/element/material/
+ * acoustic/Acoustic.hpp:27-35) built by Material::createAcoustic
+ * (src/preloop/se_model/quad/physical/Material.cpp:350-360). */
+int axs3d_set_fluid_k3d(axs3d_ctx *ctx, int e_begin, int e_end,
+                        const double *k);
+
 /* Fluid points on the axis (AxialBoundary::addPoint, fluid) and on the
  * fluid free surface (FluidSurfaceBoundary::addPoint,
  * src/core/boundary/surface/FluidSurfaceBoundary.hpp). */
@@ -299,10 +316,11 @@ int axs3d_profile_read(axs3d_ctx *ctx, int kernel, double *total_ms,
                        int64_t *count);
 int axs3d_profile_reset(axs3d_ctx *ctx);
 
-/* Kernel variants chosen at axs3d_finalize by timing them on the
- * workload (no reference counterpart; the reference has no GPU): the
- * cardinal kernel's occupancy (minimum waves per SIMD, 3-5), its number
- * of persistent blocks, and the trial times (ms) of variants 3, 4, 5. */
+/* The cardinal path as set up at axs3d_finalize (no reference
+ * counterpart; the reference has no GPU): the stress kernel's minimum
+ * waves per SIMD, its resident blocks on the device, and in trial_ms3[0]
+ * the time (ms) of one cardinal pass measured at finalize (entries 1, 2
+ * are 0). */
 int axs3d_autotune_info(const axs3d_ctx *ctx, int *cardinal_occupancy,
                         int *cardinal_blocks, double *trial_ms3);
 

@@ -63,6 +63,7 @@ class OracleDomain(C.Structure):
         ("n_sf", C.c_int), ("sf_solid", _p), ("sf_fluid", _p),
         ("sf_normal", _p),
         ("mass3d_off", _p), ("mass3d", _p),
+        ("felem_kind", _p), ("fk3d", _p), ("fk3d_off", _p),
     ]
 
 
@@ -328,6 +329,20 @@ class Oracle:
                                                       dtype=np.int32))
             d.sf_normal = self._k(np.ascontiguousarray(spec.sf_normal,
                                                        dtype=np.float64))
+        fkind = getattr(spec, "fluid_elem_kind", None)
+        if fkind is not None and getattr(spec, "fluid_nr", None) is not None:
+            fkind = np.ascontiguousarray(fkind, dtype=np.uint8)
+            fnr = np.asarray(spec.fluid_elem_nr, dtype=np.int64)
+            off = np.full(len(fkind), -1, dtype=np.int64)
+            cum = 0
+            for e in range(len(fkind)):
+                if fkind[e] == 1:
+                    off[e] = cum
+                    cum += fnr[e] * 25
+            d.felem_kind = self._k(fkind)
+            d.fk3d = self._k(np.ascontiguousarray(spec.fluid_k3d,
+                                                  dtype=np.float64))
+            d.fk3d_off = self._k(off)
         m3p = getattr(spec, "mass3d_points", None)
         if m3p is not None and len(m3p) > 0:
             off = np.full(len(spec.solid_nr), -1, dtype=np.int64)

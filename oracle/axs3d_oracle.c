@@ -848,8 +848,57 @@ void oracle_record(const oracle_domain *d, const cplx *displ, double *out) {
  * FluidPoint.hpp:82-94) -> computeGrad3 -> Acoustic::strainToStress
  * (K p_i, Acoustic.hpp:80-92) -> computeQuad3 -> subtracting scatter
  * (FluidPoint.hpp:97-103). */
+/* FluidElement::displToStiff with a 3D medium (FluidElement.cpp:101-148,
+ * the cardinal branch): computeGrad3 for every order, gFFT_N3.computeC2R
+ * of the 75 series, K per slice (Acoustic::strainToStress _3D_CD,
+ * Acoustic.hpp:80-92), computeR2C, computeQuad3, subtracting scatter. */
+static void fluid_element_stiff3d(const oracle_domain *d, int e,
+                                  const cplx *chi, cplx *fstiff) {
+    const int32_t *pts = d->felem_points + (int64_t)e * N;
+    const int nr = d->felem_nr[e];
+    const int nu1 = nr / 2 + 1;
+    const double *gq_e = d->felem_gq + (int64_t)e * 6 * N;
+    const double *K3 = d->fk3d + d->fk3d_off[e];
+    const int axial = d->felem_axial[e];
+    cplx *PI = malloc(sizeof(cplx) * nu1 * 3 * N);
+    cplx *col = malloc(sizeof(cplx) * nu1);
+    double *r = malloc(sizeof(double) * nr);
+    cplx p[N], q[N];
+    for (int alpha = 0; alpha < nu1; alpha++) {
+        for (int k = 0; k < N; k++) {
+            int ip = pts[k];
+            int pnu1 = d->fpoint_nr[ip] / 2 + 1;
+            p[k] = alpha < pnu1 ? chi[d->foffsets[ip] + alpha] : 0.;
+        }
+        oracle_grad3(gq_e, axial, alpha, p, PI + (int64_t)alpha * 3 * N);
+    }
+    for (int c = 0; c < 3; c++)
+        for (int k = 0; k < N; k++) {
+            for (int a = 0; a < nu1; a++) col[a] = PI[(a * 3 + c) * N + k];
+            oracle_c2r(col, nr, r);
+            for (int sl = 0; sl < nr; sl++) r[sl] *= K3[sl * N + k];
+            oracle_r2c(r, nr, col);
+            for (int a = 0; a < nu1; a++) PI[(a * 3 + c) * N + k] = col[a];
+        }
+    for (int alpha = 0; alpha < nu1; alpha++) {
+        oracle_quad3(gq_e, axial, alpha, PI + (int64_t)alpha * 3 * N, q);
+        for (int k = 0; k < N; k++) {
+            int ip = pts[k];
+            int pnu1 = d->fpoint_nr[ip] / 2 + 1;
+            if (alpha < pnu1) fstiff[d->foffsets[ip] + alpha] -= q[k];
+        }
+    }
+    free(PI);
+    free(col);
+    free(r);
+}
+
 void oracle_fluid_element_stiff(const oracle_domain *d, int e,
                                 const cplx *chi, cplx *fstiff) {
+    if (d->felem_kind && d->felem_kind[e] == 1) {
+        fluid_element_stiff3d(d, e, chi, fstiff);
+        return;
+    }
     const int32_t *pts = d->felem_points + (int64_t)e * N;
     int nu1 = d->felem_nr[e] / 2 + 1;
     const double *gq_e = d->felem_gq + (int64_t)e * 6 * N;

@@ -82,6 +82,12 @@ typedef struct {
      * in mass3d, or -1 for a 1D mass */
     const int64_t *mass3d_off;
     const double *mass3d;
+    /* 3D acoustic media (NULL: all fluid elements 1D): per fluid element
+     * 0 = 1D (felem_k), 1 = 3D with K per slice at fk3d + fk3d_off[e],
+     * [Nr][25] */
+    const uint8_t *felem_kind;
+    const double *fk3d;
+    const int64_t *fk3d_off;
 } oracle_domain;
 
 void oracle_set_gmat(const double *g_gll, const double *g_glj);
