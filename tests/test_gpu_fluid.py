@@ -102,3 +102,48 @@ def test_fluid_seismograms_match_oracle(prec, tol, layer):
     print(f"fluid {layer} {prec}: relative seismogram L2 = {err:.3e}")
     assert err < tol
     assert _rel(chi, o.fdispl) < 10 * tol
+
+
+def _spec3d(layer, nsteps=300):
+    # element Nr 5 to 11 (point Nr 3 to 11), half the fluid elements 3D,
+    # two of them on the axis
+    sp = _spec(layer, nsteps)
+    nfe = len(sp.fluid_elem_nr)
+    return preloop.make_fluid3d(sp, np.arange(0, nfe, 2), amp=0.3, seed=5)
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-11), ("fp32", 1e-5)])
+@pytest.mark.parametrize("layer", ["middle", "ocean"])
+def test_fluid3d_stiffness_matches_oracle(prec, tol, layer):
+    spec = _spec3d(layer)
+    assert spec.fluid_elem_kind.any() and not spec.fluid_elem_kind.all()
+    o = oracle.Oracle(spec)
+    g = _solver(spec, prec)
+    rng = np.random.default_rng(41)
+    chi = rng.standard_normal(o.fnmodes) + 1j * rng.standard_normal(
+        o.fnmodes)
+    g.set_field("displ", np.zeros((o.nmodes, 3), complex))
+    g.set_field("fdispl", chi)
+    g.set_field("stiff", np.zeros((o.nmodes, 3), complex))
+    g.set_field("fstiff", np.zeros_like(chi))
+    g.computeStiffness()
+    kf = o.fluid_stiffness(chi)
+    assert _rel(g.get_field("fstiff"), kf) < tol
+    g.close()
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("fp32", 1e-4)])
+def test_fluid3d_seismograms_match_oracle(prec, tol):
+    nsteps = 300
+    spec = _spec3d("middle", nsteps)
+    o = oracle.Oracle(spec)
+    ref = o.run(nsteps)
+    g = _solver(spec, prec)
+    g.run(nsteps)
+    got = g.seismograms()
+    chi = g.get_field("fdispl")
+    g.close()
+    err = _rel(got, ref)
+    print(f"fluid 3D {prec}: relative seismogram L2 = {err:.3e}")
+    assert err < tol
+    assert _rel(chi, o.fdispl) < 10 * tol

@@ -683,3 +683,87 @@ def test_mass3d_matches_numpy_fft():
                                        rtol=0, atol=1e-12 * np.abs(ref).max())
             checked += 1
     assert checked == 3 * len(pts)
+
+
+def _fluid3d_spec(amp=0.3, seed=3):
+    sp = _fluid_layer_spec(nsteps=1)
+    nfe = len(sp.fluid_elem_nr)
+    return preloop.make_fluid3d(sp, np.arange(0, nfe, 2), amp=amp,
+                                seed=seed)
+
+
+def _drop_fluid3d(spec):
+    import copy
+    s2 = copy.copy(spec)
+    s2.fluid_elem_kind = None
+    s2.fluid_k3d = None
+    return s2
+
+
+def _fluid_field(o, rng, real_mean=True):
+    chi = rng.standard_normal(o.fnmodes) + 1j * rng.standard_normal(
+        o.fnmodes)
+    if real_mean:
+        chi[o.foffsets[:-1]] = chi[o.foffsets[:-1]].real
+    return chi
+
+
+def test_fluid3d_constant_k_equals_fluid1d():
+    """A 3D bulk modulus equal to the 1D one in every slice (but varying
+    over the 25 GLL points) gives the 1D fluid stiffness on a field whose
+    mean is real (Nr = 5 everywhere: no Nyquist mode).  A K table read
+    with GLL point and slice swapped fails."""
+    sp = _fluid3d_spec()
+    assert np.all(sp.fluid_elem_nr % 2 == 1)
+    chi = _fluid_field(oracle.Oracle(sp), np.random.default_rng(4))
+    ref = oracle.Oracle(_drop_fluid3d(sp)).fluid_stiffness(chi)
+    sp.fluid_k3d = np.concatenate([
+        np.tile(sp.fluid_elem_k[e], int(sp.fluid_elem_nr[e]))
+        for e in range(len(sp.fluid_elem_nr)) if sp.fluid_elem_kind[e]])
+    got = oracle.Oracle(sp).fluid_stiffness(chi)
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_fluid3d_stiffness_is_symmetric():
+    """-int K grad chi . grad psi dV is symmetric in (chi, psi) for any
+    K(phi) > 0: <psi, A chi> = <chi, A psi> in the real-field inner
+    product sum_p (Re c_0 d_0 + 2 Re sum_{a>0} conj(c_a) d_a), and
+    -<chi, A chi> > 0.  A C2R / R2C pair with mismatched conventions
+    around the K product breaks the symmetry."""
+    sp = _fluid3d_spec()
+    o = oracle.Oracle(sp)
+    rng = np.random.default_rng(8)
+    chi, psi = _fluid_field(o, rng), _fluid_field(o, rng)
+    w = np.full(o.fnmodes, 2.0)
+    w[o.foffsets[:-1]] = 1.0
+
+    def dot(a, b):
+        return np.real(np.sum(w * np.conj(a) * b))
+    a_chi, a_psi = o.fluid_stiffness(chi), o.fluid_stiffness(psi)
+    lhs, rhs = dot(psi, a_chi), dot(chi, a_psi)
+    assert abs(lhs - rhs) < 1e-11 * abs(lhs)
+    assert -dot(chi, a_chi) > 0
+
+
+def test_fluid3d_rotation_equivariance():
+    """Rotating K by one slice (K'(phi) = K(phi - 2 pi / Nr)) and the field
+    by the same angle (c_a -> c_a exp(-i a 2 pi / Nr)) rotates the
+    stiffness: the slice k of the K table sits at phi = 2 pi k / Nr, the
+    synthesis convention of the C2R.  A K table read in reversed azimuth
+    fails."""
+    sp = _fluid3d_spec()
+    nr = 5
+    assert np.all(sp.fluid_elem_nr == nr) and np.all(sp.fluid_nr == nr)
+    o = oracle.Oracle(sp)
+    chi = _fluid_field(o, np.random.default_rng(6))
+    a = np.concatenate([np.arange(n // 2 + 1) for n in sp.fluid_nr])
+    rot = np.exp(-1j * a * 2 * np.pi / nr)
+    base = o.fluid_stiffness(chi)
+    k3 = sp.fluid_k3d.reshape(-1, nr, 25)
+    sp.fluid_k3d = np.roll(k3, 1, axis=1).reshape(-1)
+    got = oracle.Oracle(sp).fluid_stiffness(rot * chi)
+    assert np.abs(got - rot * base).max() <= 1e-11 * np.abs(base).max()
+    # the reversed azimuth does not satisfy it
+    sp.fluid_k3d = np.roll(k3, -1, axis=1).reshape(-1)
+    bad = oracle.Oracle(sp).fluid_stiffness(rot * chi)
+    assert np.abs(bad - rot * base).max() > 1e-3 * np.abs(base).max()
