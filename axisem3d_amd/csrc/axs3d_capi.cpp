@@ -276,6 +276,11 @@ int axs3d_run(axs3d_ctx *ctx, int step_begin, int nsteps) {
                    [&](axs3d::DomainBase &d) { d.solve(step_begin, nsteps); });
 }
 
+int axs3d_set_graph(axs3d_ctx *ctx, int enable) {
+    return guarded(ctx,
+                   [&](axs3d::DomainBase &d) { d.useGraph(enable != 0); });
+}
+
 int axs3d_set_record_origin(axs3d_ctx *ctx, int record_step0) {
     return guarded(ctx, [&](axs3d::DomainBase &d) {
         d.setRecordOrigin(record_step0);

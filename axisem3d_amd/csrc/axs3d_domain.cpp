@@ -59,6 +59,7 @@ DomainBase::DomainBase(int device, int precision)
 DomainBase::~DomainBase() {
     (void)hipSetDevice(mDevice);
     if (mStream) (void)hipStreamSynchronize(mStream);
+    dropGraph();
     for (auto &p : mPending) {
         (void)hipEventDestroy(p.a);
         (void)hipEventDestroy(p.b);
@@ -838,16 +839,18 @@ void DomainBase::finalize() {
     }
     dFlag.alloc(sizeof(int));
     dFlag.zero(st);
+    dStepCtr.alloc(2 * sizeof(int));
+    dStepCtr.zero(st);
     uploadReal();
     hip_check(hipStreamSynchronize(st), "Domain::finalize");
     mFinalized = true;
     autotune();
 }
 
-void DomainBase::recordOutput(int isample) {
+void DomainBase::recordOutput(int isample, bool dev) {
     requireFinalized("Domain::recordOutput");
     if (mNumRec == 0) return;
-    if (isample < 0 || isample >= hMaxSamples)
+    if (!dev && (isample < 0 || isample >= hMaxSamples))
         throw Error("Domain::recordOutput || Sample index out of range.");
     tick(AXS3D_KERNEL_RECORD);
     launch_record(mStream, mNumRec, dRecElem.as<int32_t>(),
@@ -855,7 +858,9 @@ void DomainBase::recordOutput(int isample) {
                   dElemPoints.as<int32_t>(), dElemNu1.as<int32_t>(),
                   dPointOff.as<uint32_t>(), dPointNu1.as<int32_t>(),
                   dDispl.as<double>(),
-                  dSeismo.as<double>() + (size_t)isample * mNumRec * 3);
+                  dSeismo.as<double>() +
+                      (dev ? 0 : (size_t)isample * mNumRec * 3),
+                  dev ? dStepCtr.as<int>() : nullptr, hMaxSamples);
     tock(AXS3D_KERNEL_RECORD);
 }
 
@@ -877,18 +882,65 @@ void DomainBase::checkStability(int tstep) {
 
 // NewmarkTimeScheme::solve (NewmarkTimeScheme.cpp:41-108).  Stability is
 // checked from the device flag at the end of the batch (the point kernel
-// raises it on any non-finite displacement).
+// raises it on any non-finite displacement).  By default the steps are
+// replays of one captured step graph, whose source and receiver kernels
+// read the step from the device counter set here; with profiling on, or
+// useGraph(false), every step is launched kernel by kernel.
 void DomainBase::solve(int step_begin, int nsteps) {
     requireFinalized("NewmarkTimeScheme::solve");
-    for (int tstep = step_begin; tstep < step_begin + nsteps; tstep++) {
-        int isample = -1;
-        if (mNumRec > 0) {
-            isample = tstep - mRecordStep0;
-            if (isample < 0 || isample >= hMaxSamples) isample = -1;
+    if (nsteps > 0 && hSrcElem >= 0 &&
+        (step_begin < 0 || step_begin + nsteps > mStfSteps))
+        throw Error("Domain::applySources || Step outside the STF.");
+    if (nsteps > 0 && mUseGraph && !mProfile) {
+        if (!mGraphExec) captureStep();
+        launch_step_set(mStream, dStepCtr.as<int>(), step_begin,
+                        mRecordStep0);
+        for (int i = 0; i < nsteps; i++)
+            hip_check(hipGraphLaunch(mGraphExec, mStream),
+                      "NewmarkTimeScheme::solve");
+    } else {
+        for (int tstep = step_begin; tstep < step_begin + nsteps; tstep++) {
+            int isample = -1;
+            if (mNumRec > 0) {
+                isample = tstep - mRecordStep0;
+                if (isample < 0 || isample >= hMaxSamples) isample = -1;
+            }
+            step(tstep, isample, false);
         }
-        step(tstep, isample);
     }
     checkStability(step_begin + nsteps);
+}
+
+// Captures one step in its device-counter form.  The captured kernels take
+// only device pointers fixed at finalize, the time step and the launch
+// shapes, so the graph stays valid until dt changes (setDt drops it).
+void DomainBase::captureStep() {
+    hip_check(hipStreamBeginCapture(mStream, hipStreamCaptureModeThreadLocal),
+              "NewmarkTimeScheme::solve");
+    try {
+        step(0, -1, true);
+    } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(mStream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    hip_check(hipStreamEndCapture(mStream, &mGraph),
+              "NewmarkTimeScheme::solve");
+    const hipError_t rc =
+        hipGraphInstantiate(&mGraphExec, mGraph, nullptr, nullptr, 0);
+    if (rc != hipSuccess) {
+        mGraphExec = nullptr;
+        dropGraph();
+        hip_check(rc, "NewmarkTimeScheme::solve");
+    }
+}
+
+void DomainBase::dropGraph() {
+    if (mGraphExec) (void)hipGraphExecDestroy(mGraphExec);
+    if (mGraph) (void)hipGraphDestroy(mGraph);
+    mGraphExec = nullptr;
+    mGraph = nullptr;
 }
 
 void DomainBase::readSeismograms(double *out, int nsamples) {
@@ -1121,14 +1173,15 @@ public:
         sourceKernel(tstep, false);
     }
 
-    void sourceKernel(int tstep, bool toElem) {
+    void sourceKernel(int tstep, bool toElem, bool dev = false) {
         if (hSrcElem < 0) return;
-        if (tstep < 0 || tstep >= mStfSteps)
+        if (!dev && (tstep < 0 || tstep >= mStfSteps))
             throw Error("Domain::applySources || Step outside the STF.");
         tick(AXS3D_KERNEL_SOURCE);
         launch_moment_source<Real>(mStream, toElem, stiffArgs(), hSrcElem,
                                    mSrcNu1c, dSrcPattern.as<Real>(),
-                                   dStf.as<double>(), tstep,
+                                   dStf.as<double>(), tstep, mStfSteps,
+                                   dev ? dStepCtr.as<int>() : nullptr,
                                    dStiff.as<Real>());
         tock(AXS3D_KERNEL_SOURCE);
     }
@@ -1243,13 +1296,14 @@ public:
     // element-local forces, the source adds into its element's, the
     // receivers sample u(t), then one fluid and one solid point kernel
     // gather, couple, mask, divide by the mass and march.
-    void step(int tstep, int isample) override {
+    void step(int tstep, int isample, bool dev) override {
         elementKernel();
-        sourceKernel(tstep, true);
-        if (isample >= 0) recordOutput(isample);
+        sourceKernel(tstep, true, dev);
+        if (dev || isample >= 0) recordOutput(isample, dev);
         const int all = PHASE_GATHER | PHASE_AXIAL | PHASE_MASS | PHASE_MARCH;
         fluidPoint(all | PHASE_COUPLE);
         point(all | PHASE_COUPLE);
+        if (dev) launch_step_advance(mStream, dStepCtr.as<int>());
     }
 
     void getField(int which, double *out) override {

@@ -86,7 +86,10 @@ public:
                          const double *stf);
     void setReceivers(int n, const int32_t *elem, const double *w,
                       const double *phi, int max_samples);
-    void setDt(double dt) { mDt = dt; }
+    void setDt(double dt) {
+        mDt = dt;
+        dropGraph();  // dt is a kernel argument of the captured step
+    }
     void finalize();
     bool finalized() const { return mFinalized; }
 
@@ -94,17 +97,22 @@ public:
     virtual void applySources(int tstep) = 0;
     virtual void computeStiffness() = 0;
     virtual void applyBC_BeforeAssemblingStiff() = 0;
-    void recordOutput(int isample);
+    // isample < 0 with dev: the sample follows the device step counter
+    void recordOutput(int isample, bool dev = false);
     virtual void applyBC_AfterAssemblingStiff() = 0;
     virtual void computeStiffToAccel() = 0;
     virtual void newmark() = 0;
-    // one bulk Newmark step (sources, stiffness, receivers, points)
-    virtual void step(int tstep, int isample) = 0;
+    // one bulk Newmark step (sources, stiffness, receivers, points); with
+    // dev, the step and sample come from the device step counter, which
+    // the step advances (the form captured into a HIP graph)
+    virtual void step(int tstep, int isample, bool dev) = 0;
     void checkStability(int tstep);
 
     // bulk
     void solve(int step_begin, int nsteps);
     void setRecordOrigin(int step0) { mRecordStep0 = step0; }
+    // replay a captured step graph in solve (default) or launch eagerly
+    void useGraph(bool on) { mUseGraph = on; }
     void readSeismograms(double *out, int nsamples);
 
     // debug
@@ -138,6 +146,12 @@ protected:
     int mDevice = 0;
     int mPrecision = 0;
     hipStream_t mStream = nullptr;
+    // one step captured as a HIP graph (solve replays it)
+    bool mUseGraph = true;
+    hipGraph_t mGraph = nullptr;
+    hipGraphExec_t mGraphExec = nullptr;
+    void captureStep();
+    void dropGraph();
     bool mFinalized = false;
     double mDt = 0.;
     int mRecordStep0 = 0;
@@ -245,6 +259,7 @@ protected:
     // device: source and receivers
     DeviceBuffer dSrcPattern, dStf, dRecElem, dRecW, dRecPhi, dSeismo;
     DeviceBuffer dFlag;
+    DeviceBuffer dStepCtr;              // [0] step, [1] first recorded step
 
     virtual void uploadReal() = 0;   // precision-specific operand upload
     virtual void autotune() = 0;     // pick kernel variants by timing

@@ -173,7 +173,8 @@ __global__ __launch_bounds__(STIFF_BLOCK) void k_solid_stiff_fr(
 template <typename Real, bool TO_ELEM>
 __global__ __launch_bounds__(STIFF_BLOCK) void k_moment_source(
     int elem, int nu1c, const Real *__restrict__ pattern, const double *stf,
-    int tstep, const int32_t *__restrict__ elem_points,
+    int tstep, int nstf, const int *__restrict__ step_ctr,
+    const int32_t *__restrict__ elem_points,
     const uint8_t *__restrict__ elem_axial, const Real *__restrict__ elem_ops,
     const uint32_t *__restrict__ point_off,
     const int32_t *__restrict__ point_nu1, Real *__restrict__ stiff,
@@ -183,6 +184,10 @@ __global__ __launch_bounds__(STIFF_BLOCK) void k_moment_source(
     __shared__ cpx<Real> sX[EMB][3][NPEM];
     __shared__ cpx<Real> sY[EMB][3][NPEM];
     __shared__ Real sG[2][NPEM];
+    // a replayed step takes its index from the device counter; outside
+    // the STF the whole block leaves before the first barrier
+    if (step_ctr) tstep = step_ctr[0];
+    if (tstep < 0 || tstep >= nstf) return;
     const int t = threadIdx.x;
     if (t < 2 * NPEM) sG[t / NPEM][t % NPEM] = gtab[t];
     __syncthreads();
@@ -438,9 +443,17 @@ __global__ __launch_bounds__(64) void k_record(
     const int32_t *__restrict__ elem_nu1,
     const uint32_t *__restrict__ point_off,
     const int32_t *__restrict__ point_nu1, const double *__restrict__ displ,
-    double *__restrict__ out) {
+    double *__restrict__ out, const int *__restrict__ step_ctr,
+    int max_samples) {
     const int r = blockIdx.x;
     if (r >= nrec) return;
+    // a replayed step: sample = step - first recorded step (counter words
+    // 0 and 1), nothing outside the seismogram buffer
+    if (step_ctr) {
+        const int isample = step_ctr[0] - step_ctr[1];
+        if (isample < 0 || isample >= max_samples) return;
+        out += (size_t)isample * nrec * 3;
+    }
     const int lane = threadIdx.x;
     const int e = rec_elem[r];
     const int nu1 = elem_nu1[e];
@@ -506,19 +519,19 @@ template <typename Real>
 void launch_moment_source(hipStream_t st, bool to_elem,
                           const StiffArgs<Real> &a, int elem, int nu1c,
                           const Real *pattern, const double *stf, int tstep,
-                          Real *stiff) {
+                          int nstf, const int *step_ctr, Real *stiff) {
     if (nu1c <= 0) return;
     const int grid = (nu1c + EMB - 1) / EMB;
     if (to_elem)
         hipLaunchKernelGGL((k_moment_source<Real, true>), dim3(grid),
                            dim3(STIFF_BLOCK), 0, st, elem, nu1c, pattern, stf,
-                           tstep, a.elem_points, a.elem_axial, a.elem_ops,
+                           tstep, nstf, step_ctr, a.elem_points, a.elem_axial, a.elem_ops,
                            a.point_off, a.point_nu1, stiff, a.elem_eb_off,
                            a.elem_nu1, a.eb, a.gtab);
     else
         hipLaunchKernelGGL((k_moment_source<Real, false>), dim3(grid),
                            dim3(STIFF_BLOCK), 0, st, elem, nu1c, pattern, stf,
-                           tstep, a.elem_points, a.elem_axial, a.elem_ops,
+                           tstep, nstf, step_ctr, a.elem_points, a.elem_axial, a.elem_ops,
                            a.point_off, a.point_nu1, stiff, a.elem_eb_off,
                            a.elem_nu1, a.eb, a.gtab);
 }
@@ -583,11 +596,34 @@ void launch_record(hipStream_t st, int nrec, const int32_t *rec_elem,
                    const double *rec_w, const double *rec_phi,
                    const int32_t *elem_points, const int32_t *elem_nu1,
                    const uint32_t *point_off, const int32_t *point_nu1,
-                   const double *displ, double *out) {
+                   const double *displ, double *out, const int *step_ctr,
+                   int max_samples) {
     if (nrec <= 0) return;
     hipLaunchKernelGGL(k_record, dim3(nrec), dim3(64), 0, st, nrec, rec_elem,
                        rec_w, rec_phi, elem_points, elem_nu1, point_off,
-                       point_nu1, displ, out);
+                       point_nu1, displ, out, step_ctr, max_samples);
+}
+
+// The step counter of replayed steps: [0] the step, [1] the first
+// recorded step.
+__global__ void k_step_set(int *ctr, int tstep, int record_step0) {
+    if (threadIdx.x == 0) {
+        ctr[0] = tstep;
+        ctr[1] = record_step0;
+    }
+}
+
+__global__ void k_step_advance(int *ctr) {
+    if (threadIdx.x == 0) ctr[0] += 1;
+}
+
+void launch_step_set(hipStream_t st, int *ctr, int tstep, int record_step0) {
+    hipLaunchKernelGGL(k_step_set, dim3(1), dim3(64), 0, st, ctr, tstep,
+                       record_step0);
+}
+
+void launch_step_advance(hipStream_t st, int *ctr) {
+    hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, st, ctr);
 }
 
 #define AXS3D_INSTANTIATE(Real)                                               \
@@ -595,7 +631,7 @@ void launch_record(hipStream_t st, int nrec, const int32_t *rec_elem,
                                               const StiffArgs<Real> &);      \
     template void launch_moment_source<Real>(                                 \
         hipStream_t, bool, const StiffArgs<Real> &, int, int, const Real *,   \
-        const double *, int, Real *);                                         \
+        const double *, int, int, const int *, Real *);                       \
     template void launch_assemble<Real>(hipStream_t, const PointGeom &,       \
                                         const Real *, Real *);                \
     template void launch_point<Real>(hipStream_t, int, const PointGeom &,     \

@@ -224,7 +224,7 @@ template <typename Real>
 void launch_moment_source(hipStream_t st, bool to_elem,
                           const StiffArgs<Real> &a, int elem, int nu1c,
                           const Real *pattern, const double *stf, int tstep,
-                          Real *stiff);
+                          int nstf, const int *step_ctr, Real *stiff);
 
 template <typename Real>
 void launch_assemble(hipStream_t st, const PointGeom &g, const Real *eb,
@@ -254,6 +254,11 @@ void launch_record(hipStream_t st, int nrec, const int32_t *rec_elem,
                    const double *rec_w, const double *rec_phi,
                    const int32_t *elem_points, const int32_t *elem_nu1,
                    const uint32_t *point_off, const int32_t *point_nu1,
-                   const double *displ, double *out);
+                   const double *displ, double *out, const int *step_ctr,
+                   int max_samples);
+
+// step counter of a replayed (graph) step; see k_step_set
+void launch_step_set(hipStream_t st, int *ctr, int tstep, int record_step0);
+void launch_step_advance(hipStream_t st, int *ctr);
 
 }  // namespace axs3d

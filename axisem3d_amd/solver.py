@@ -193,6 +193,12 @@ class GpuSolver:
         self._chk(self.lib.axs3d_run(self.ctx, self.step, int(nsteps)), "run")
         self.step += nsteps
 
+    def use_graph(self, enable=True):
+        """Replay one captured step graph per step in run() (the default),
+        or launch every kernel from the host."""
+        self._chk(self.lib.axs3d_set_graph(self.ctx, int(bool(enable))),
+                  "set_graph")
+
     def seismograms(self, nsamples=None):
         n = self.step if nsamples is None else nsamples
         out = np.zeros((n, max(self.nrec, 1), 3))

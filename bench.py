@@ -172,7 +172,11 @@ def pmc_traffic(kernel, workload):
     return None
 
 
-def time_gpu(spec, prec, steps, warmup):
+def time_gpu(spec, prec, steps, warmup, graph=False):
+    """Times `steps` steps launched kernel by kernel with HIP events around
+    every kernel (the per-kernel averages come from these same steps);
+    with graph, then the same number of steps as replays of the captured
+    step graph, without events."""
     from axisem3d_amd import capi
     from axisem3d_amd.solver import GpuSolver
     t0 = time.perf_counter()
@@ -191,9 +195,17 @@ def time_gpu(spec, prec, steps, warmup):
     prof = {k: g.profile_read(k) for k in range(capi.NUM_KERNELS)}
     g.profile(False)
     seis = g.seismograms(steps + warmup)
-    g.close()
     log(f"{prec}: {steps} steps in {t1 - t0:.3f} s")
     prof["autotune"] = tune
+    if graph:
+        g.run(1)    # captures the step graph outside the timed region
+        g.synchronize()
+        t2 = time.perf_counter()
+        g.run(steps)
+        g.synchronize()
+        prof["graph_s"] = time.perf_counter() - t2
+        log(f"{prec}: {steps} graph replays in {prof['graph_s']:.3f} s")
+    g.close()
     return (t1 - t0), prof, seis
 
 
@@ -264,7 +276,8 @@ def main():
         copy_calibration(a.copy_calibration)
         return
     from axisem3d_amd import capi
-    nsteps = a.steps + a.warmup
+    # warm-up, timed steps, one capture step and the graph replays
+    nsteps = 2 * a.steps + a.warmup + 1
     t0 = time.perf_counter()
     spec, cfg = build_workload(a.workload, nsteps)
     log(f"workload built in {time.perf_counter() - t0:.1f} s: "
@@ -272,7 +285,8 @@ def main():
         "point-modes per step")
     nupd = spec.n_updates_per_step()
 
-    t32, prof32, seis32 = time_gpu(spec, "fp32", a.steps, a.warmup)
+    t32, prof32, seis32 = time_gpu(spec, "fp32", a.steps, a.warmup,
+                                   graph=True)
     value = nupd * a.steps / t32
     out_64 = None
     l2 = None
@@ -313,6 +327,7 @@ def main():
             "bytes_per_launch": byts[k] / per_step,
             "achieved_GBs": byts[k] / (ms / a.steps * 1e-3) / 1e9}
     tune32 = prof32.pop("autotune")
+    tg = prof32.pop("graph_s")
     if out_64 is not None:
         prof64.pop("autotune")
     # dominant single kernel (the cardinal group is three kernels)
@@ -341,6 +356,8 @@ def main():
                           "frac": step_bytes / (t32 / a.steps) / 1e9 /
                           HBM_PEAK_GBS},
         "kernels": kernels,
+        "graph_replay": {"ms_per_step": tg / a.steps * 1e3,
+                         "value": nupd * a.steps / tg},
         "autotune": tune32,
     }
     if not a.no_cpu:

@@ -298,6 +298,13 @@ int axs3d_check_stability(axs3d_ctx *ctx, int tstep);
  * BC-after/mass/Newmark phases run as one fused kernel. */
 int axs3d_run(axs3d_ctx *ctx, int step_begin, int nsteps);
 int axs3d_set_record_origin(axs3d_ctx *ctx, int record_step0);
+/* Launch form of axs3d_run, no counterpart in the reference (its steps are
+ * host loops): enable != 0 (the default) captures one step as a HIP graph
+ * on the first axs3d_run and replays it once per step, with the step index
+ * in device memory; 0 launches every kernel of every step from the host.
+ * Both forms give bitwise the same fields.  Profiling (axs3d_profile_enable)
+ * uses the launch-by-launch form. */
+int axs3d_set_graph(axs3d_ctx *ctx, int enable);
 /* Copies samples [0, nsamples) of the seismograms, [sample][rec][3]. */
 int axs3d_read_seismograms(axs3d_ctx *ctx, double *out, int nsamples);
 
