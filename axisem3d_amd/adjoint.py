@@ -1,0 +1,112 @@
+"""Gradient harness over the adjoint run.
+
+The reference has no adjoint solver.  Here the sensitivity of any
+seismogram misfit to the elastic model comes from the discrete adjoint of
+the GPU time loop (``axs3d_adjoint_run``): the forward run from rest,
+checkpoints on the device, the adjoint field backwards in time, and the
+sensitivity densities accumulated on the fly.  This module wraps it as a
+``torch.autograd.Function`` so that a misfit written in torch can be
+differentiated end to end; torch is the harness only, every number comes
+from the HIP kernels.
+
+Model parameterisation: rho, vp, vs per GLL point of every element
+(shape (n_elements, 25)), giving lambda = rho (vp^2 - 2 vs^2), mu = rho
+vs^2 (Material::createIsotropic, Material.cpp:566-598) and the assembled
+mass sum rho IF (GLLPoint.cpp:54-77).  The time step stays that of the
+spec, so a perturbed model runs the same steps.
+"""
+import copy
+
+import numpy as np
+
+from .solver import GpuSolver
+
+
+def with_model(spec, rho, vp, vs):
+    """A copy of spec (1D isotropic solid) with lambda, mu and the point
+    masses of the model (rho, vp, vs) at every element GLL point."""
+    rho, vp, vs = (np.asarray(x, dtype=np.float64) for x in (rho, vp, vs))
+    ne = len(spec.elem_points)
+    for x in (rho, vp, vs):
+        if x.shape != (ne, 25):
+            raise ValueError("model arrays must be (n_elements, 25)")
+    sp = copy.copy(spec)
+    lam = rho * (vp ** 2 - 2.0 * vs ** 2)
+    mu = rho * vs ** 2
+    sp.elem_iso = np.ascontiguousarray(np.stack([lam, mu], axis=1))
+    ifac = np.asarray(spec.elem_gq)[:, 5, :]
+    mass = np.zeros(len(spec.solid_nr))
+    np.add.at(mass, np.asarray(spec.elem_points).reshape(-1),
+              (rho * ifac).reshape(-1))
+    sp.solid_mass = mass
+    sp._rho1d = rho
+    return sp
+
+
+def model_of(spec):
+    """(rho, vp, vs) per element GLL point of a 1D isotropic spec, from its
+    lambda, mu and the density the spec was built with."""
+    rho = np.asarray(spec._rho1d, dtype=np.float64)
+    lam, mu = spec.elem_iso[:, 0, :], spec.elem_iso[:, 1, :]
+    vs = np.sqrt(mu / rho)
+    vp = np.sqrt((lam + 2.0 * mu) / rho)
+    return rho, vp, vs
+
+
+def chain_rule(kern, rho, vp, vs):
+    """dJ/d(rho, vp, vs) from dJ/d(lambda, mu, rho at fixed lambda, mu),
+    kern of shape (n_elements, 25, 3)."""
+    kl, km, kr = kern[..., 0], kern[..., 1], kern[..., 2]
+    d_rho = kr + (vp ** 2 - 2.0 * vs ** 2) * kl + vs ** 2 * km
+    d_vp = 2.0 * rho * vp * kl
+    d_vs = -4.0 * rho * vs * kl + 2.0 * rho * vs * km
+    return d_rho, d_vp, d_vs
+
+
+def forward_seismograms(spec, rho, vp, vs, nsteps, precision="fp64"):
+    """Seismograms (nsteps, n_receivers, 3) of the model from rest."""
+    g = GpuSolver(with_model(spec, rho, vp, vs), precision,
+                  max_samples=nsteps)
+    try:
+        g.run(nsteps)
+        return g.seismograms(nsteps)
+    finally:
+        g.close()
+
+
+def gradient(spec, rho, vp, vs, djdr, precision="fp64",
+             checkpoint_interval=None):
+    """dJ/d(rho, vp, vs) of a misfit J whose derivative with respect to
+    the seismograms of the model is djdr (nsteps, n_receivers, 3)."""
+    g = GpuSolver(with_model(spec, rho, vp, vs), precision,
+                  max_samples=len(djdr))
+    try:
+        kern = g.adjoint_run(djdr, checkpoint_interval)
+    finally:
+        g.close()
+    return chain_rule(kern, np.asarray(rho), np.asarray(vp), np.asarray(vs))
+
+
+def seismogram_function():
+    """torch.autograd.Function: (rho, vp, vs) float64 CPU tensors ->
+    seismograms, backward through the adjoint run.  Built on first use so
+    that importing this module does not import torch."""
+    import torch
+
+    class Seismograms(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, rho, vp, vs, spec, nsteps, precision, interval):
+            m = [x.detach().cpu().numpy() for x in (rho, vp, vs)]
+            ctx.model = m
+            ctx.spec, ctx.precision, ctx.interval = spec, precision, interval
+            seis = forward_seismograms(spec, *m, nsteps, precision)
+            return torch.from_numpy(seis)
+
+        @staticmethod
+        def backward(ctx, grad):
+            d = gradient(ctx.spec, *ctx.model, grad.detach().cpu().numpy(),
+                         ctx.precision, ctx.interval)
+            return (torch.from_numpy(d[0]), torch.from_numpy(d[1]),
+                    torch.from_numpy(d[2]), None, None, None, None)
+
+    return Seismograms

@@ -281,6 +281,23 @@ int axs3d_set_graph(axs3d_ctx *ctx, int enable) {
                    [&](axs3d::DomainBase &d) { d.useGraph(enable != 0); });
 }
 
+int axs3d_adjoint_run(axs3d_ctx *ctx, int nsteps, int checkpoint_interval,
+                      const double *djdr, double *kernels) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) {
+        if (!kernels) throw axs3d::Error("Domain::adjoint || Null output.");
+        d.adjointRun(nsteps, checkpoint_interval, djdr, kernels);
+    });
+}
+
+int axs3d_adjoint_density(axs3d_ctx *ctx, const double *psi, const double *u,
+                          const double *a, double *kernels) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) {
+        if (!psi || !u || !a || !kernels)
+            throw axs3d::Error("Domain::adjointDensity || Null pointer.");
+        d.adjointDensity(psi, u, a, kernels);
+    });
+}
+
 int axs3d_set_record_origin(axs3d_ctx *ctx, int record_step0) {
     return guarded(ctx, [&](axs3d::DomainBase &d) {
         d.setRecordOrigin(record_step0);

@@ -637,9 +637,12 @@ void DomainBase::finalize() {
     }
     // receivers
     mNumRec = (int)hRecElem.size();
-    for (int32_t e : hRecElem)
+    mRecNu1Max = 0;
+    for (int32_t e : hRecElem) {
         if (e < 0 || e >= mNumElems)
             throw Error("Domain::finalize || Receiver element out of range.");
+        mRecNu1Max = std::max(mRecNu1Max, eNu1[e]);
+    }
 
     // fluid points, elements, boundaries and the solid-fluid coupling
     std::vector<uint32_t> fOff(mNumFPoints + 1, 0), fEbOff, fAdj, cSOff,
@@ -934,6 +937,80 @@ void DomainBase::captureStep() {
         dropGraph();
         hip_check(rc, "NewmarkTimeScheme::solve");
     }
+}
+
+// ------------------------------------------------------------ adjoint
+void DomainBase::adjointCheck() const {
+    requireFinalized("Domain::adjoint");
+    bool iso = true;
+    for (uint8_t t : hMType) iso = iso && t == 0;
+    if (mNum3D > 0 || mNsls > 0 || mNumFElems > 0 || mNumMass3D > 0 || !iso)
+        throw Error("Domain::adjoint || The adjoint run supports solid "
+                    "elements with a 1D isotropic elastic medium and no "
+                    "attenuation, fluid or 3D mass.");
+}
+
+void DomainBase::adjointOperands() {
+    if (dAdjGQ.bytes() > 0) return;
+    dAdjGQ.alloc(hElemGQ.size() * sizeof(double));
+    dAdjGQ.upload(hElemGQ.data(), hElemGQ.size() * sizeof(double), mStream);
+    std::vector<double> gt(2 * NPEM);
+    for (int k = 0; k < NPEM; k++) {
+        gt[k] = hGll[k];
+        gt[NPEM + k] = hGlj[k];
+    }
+    dAdjGTab.alloc(gt.size() * sizeof(double));
+    dAdjGTab.upload(gt.data(), gt.size() * sizeof(double), mStream);
+}
+
+SensArgs DomainBase::sensArgs(const double *psi, const double *u,
+                              const double *a, double *kern,
+                              double scale) const {
+    SensArgs s;
+    s.n_elem = mNumElems;
+    s.elem_points = dElemPoints.as<int32_t>();
+    s.elem_axial = dElemAxial.as<uint8_t>();
+    s.gq = dAdjGQ.as<double>();
+    s.elem_nu1 = dElemNu1.as<int32_t>();
+    s.point_off = dPointOff.as<uint32_t>();
+    s.point_nu1 = dPointNu1.as<int32_t>();
+    s.gtab = dAdjGTab.as<double>();
+    s.psi = psi;
+    s.u = u;
+    s.a = a;
+    s.kern = kern;
+    s.scale = scale;
+    return s;
+}
+
+void DomainBase::adjointDensity(const double *psi, const double *u,
+                                const double *a, double *kern) {
+    adjointCheck();
+    adjointOperands();
+    const size_t nf = (size_t)mNumModes * 6 * sizeof(double);
+    const size_t nk = (size_t)mNumElems * NPEM * 3 * sizeof(double);
+    DeviceBuffer bp, bu, ba, bk;
+    bp.alloc(nf);
+    bu.alloc(nf);
+    ba.alloc(nf);
+    bk.alloc(nk);
+    bp.upload(psi, nf, mStream);
+    bu.upload(u, nf, mStream);
+    ba.upload(a, nf, mStream);
+    bk.zero(mStream);
+    launch_sensitivity(mStream, sensArgs(bp.as<double>(), bu.as<double>(),
+                                         ba.as<double>(), bk.as<double>(),
+                                         1.0));
+    hip_check(hipMemcpyAsync(kern, bk.as<double>(), nk,
+                             hipMemcpyDeviceToHost, mStream),
+              "Domain::adjointDensity");
+    hip_check(hipStreamSynchronize(mStream), "Domain::adjointDensity");
+}
+
+void DomainBase::swapAdjointFields() {
+    dDispl.swap(dAdjU);
+    dVeloc.swap(dAdjV);
+    dAccel.swap(dAdjA);
 }
 
 void DomainBase::dropGraph() {
@@ -1304,6 +1381,118 @@ public:
         fluidPoint(all | PHASE_COUPLE);
         point(all | PHASE_COUPLE);
         if (dev) launch_step_advance(mStream, dStepCtr.as<int>());
+    }
+
+    // Discrete adjoint of the bulk step (see axs3d_adjoint.hip for the
+    // equations).  Forward from rest with checkpoints of (u, v, a) every
+    // seg steps; then, segment by segment from the last, the forward
+    // fields of the segment are recomputed from its checkpoint (u_n before
+    // and a_n after each step kept on the device), and the adjoint field
+    // (the same fields, swapped in) steps backwards through the segment:
+    // densities of (psi_n, u_n, a_n) scaled by -dt^2 into the kernels,
+    // then one step with the adjoint force instead of the source.
+    void adjointRun(int nsteps, int seg, const double *djdr,
+                    double *kern) override {
+        adjointCheck();
+        if (nsteps <= 0 || seg <= 0)
+            throw Error("Domain::adjoint || The number of steps and the "
+                        "checkpoint interval must be positive.");
+        if (hSrcElem >= 0 && nsteps > mStfSteps)
+            throw Error("Domain::adjoint || Step outside the STF.");
+        adjointOperands();
+        const size_t nd = (size_t)mNumModes * 6;
+        const size_t nf = nd * sizeof(double);
+        const int nck = (nsteps + seg - 1) / seg;
+        const int seglen = std::min(seg, nsteps);
+        size_t freeb = 0, totalb = 0;
+        hip_check(hipMemGetInfo(&freeb, &totalb), "Domain::adjoint");
+        const double need = (double)nf * (3.0 * nck + 2.0 * seglen + 3.0);
+        if (need > 0.9 * (double)freeb) {
+            std::ostringstream ss;
+            ss << "Domain::adjoint || Checkpoints and segment fields need "
+               << need / 1e9 << " GB, " << freeb / 1e9
+               << " GB are free; a checkpoint interval near sqrt(nsteps) "
+                  "needs the least.";
+            throw Error(ss.str());
+        }
+        dAdjU.alloc(nf);
+        dAdjV.alloc(nf);
+        dAdjA.alloc(nf);
+        dAdjCkpt.alloc(nf * 3 * nck);
+        dAdjSegU.alloc(nf * seglen);
+        dAdjSegA.alloc(nf * seglen);
+        const size_t ng = (size_t)nsteps * mNumRec * 3;
+        dAdjG.alloc(std::max<size_t>(ng, 1) * sizeof(double));
+        if (ng > 0) dAdjG.upload(djdr, ng * sizeof(double), mStream);
+        const size_t nk = (size_t)mNumElems * NPEM * 3;
+        dAdjKern.alloc(nk * sizeof(double));
+        dAdjKern.zero(mStream);
+        const int all = PHASE_GATHER | PHASE_AXIAL | PHASE_MASS | PHASE_MARCH;
+        double *ck = dAdjCkpt.as<double>();
+        auto d2d = [&](double *dst, const double *src) {
+            hip_check(hipMemcpyAsync(dst, src, nf, hipMemcpyDeviceToDevice,
+                                     mStream),
+                      "Domain::adjoint");
+        };
+        // forward from rest up to the last checkpoint
+        dDispl.zero(mStream);
+        dVeloc.zero(mStream);
+        dAccel.zero(mStream);
+        for (int n = 0; n <= (nck - 1) * seg; n++) {
+            if (n % seg == 0) {
+                double *c = ck + (size_t)(n / seg) * 3 * nd;
+                d2d(c, dDispl.as<double>());
+                d2d(c + nd, dVeloc.as<double>());
+                d2d(c + 2 * nd, dAccel.as<double>());
+            }
+            if (n < (nck - 1) * seg) step(n, -1, false);
+        }
+        // backwards, segment by segment
+        dAdjU.zero(mStream);
+        dAdjV.zero(mStream);
+        dAdjA.zero(mStream);
+        const double dt2 = mDt * mDt;
+        double *su = dAdjSegU.as<double>(), *sa = dAdjSegA.as<double>();
+        for (int s = nck - 1; s >= 0; s--) {
+            const int n0 = s * seg, n1 = std::min(n0 + seg, nsteps);
+            const double *c = ck + (size_t)s * 3 * nd;
+            d2d(dDispl.as<double>(), c);
+            d2d(dVeloc.as<double>(), c + nd);
+            d2d(dAccel.as<double>(), c + 2 * nd);
+            for (int n = n0; n < n1; n++) {
+                d2d(su + (size_t)(n - n0) * nd, dDispl.as<double>());
+                step(n, -1, false);
+                d2d(sa + (size_t)(n - n0) * nd, dAccel.as<double>());
+            }
+            swapAdjointFields();
+            for (int n = n1 - 1; n >= n0; n--) {
+                launch_sensitivity(
+                    mStream, sensArgs(dDispl.as<double>(),
+                                      su + (size_t)(n - n0) * nd,
+                                      sa + (size_t)(n - n0) * nd,
+                                      dAdjKern.as<double>(), -dt2));
+                elementKernel();
+                launch_adjoint_source<Real>(
+                    mStream, mNumRec, dRecElem.as<int32_t>(),
+                    dRecW.as<double>(), dRecPhi.as<double>(),
+                    dAdjG.as<double>() + (size_t)n * mNumRec * 3, 1.0 / dt2,
+                    mRecNu1Max, dElemPoints.as<int32_t>(),
+                    dElemNu1.as<int32_t>(), dPointNu1.as<int32_t>(),
+                    dElemEBOff.as<uint32_t>(), dEB.as<Real>());
+                point(all);
+            }
+            swapAdjointFields();
+        }
+        hip_check(hipMemcpyAsync(kern, dAdjKern.as<double>(),
+                                 nk * sizeof(double), hipMemcpyDeviceToHost,
+                                 mStream),
+                  "Domain::adjoint");
+        hip_check(hipStreamSynchronize(mStream), "Domain::adjoint");
+        checkStability(nsteps);
+        // the segment buffers are the largest; give them back
+        dAdjCkpt.reset();
+        dAdjSegU.reset();
+        dAdjSegA.reset();
     }
 
     void getField(int which, double *out) override {

@@ -44,6 +44,10 @@ public:
     template <typename T>
     T *as() const { return static_cast<T *>(ptr_); }
     size_t bytes() const { return bytes_; }
+    void swap(DeviceBuffer &o) {
+        std::swap(ptr_, o.ptr_);
+        std::swap(bytes_, o.bytes_);
+    }
 
 private:
     void *ptr_ = nullptr;
@@ -114,6 +118,17 @@ public:
     // replay a captured step graph in solve (default) or launch eagerly
     void useGraph(bool on) { mUseGraph = on; }
     void readSeismograms(double *out, int nsamples);
+
+    // adjoint (sensitivity) run of 1D isotropic elastic domains: a forward
+    // run from rest over nsteps with checkpoints every seg steps, then the
+    // adjoint run backwards with the force R^* djdr[n] / dt^2, recomputing
+    // each segment's forward fields from its checkpoint; kern[e][k][3] =
+    // dJ/d(lambda, mu, rho) at GLL point k of element e
+    virtual void adjointRun(int nsteps, int seg, const double *djdr,
+                            double *kern) = 0;
+    // sensitivity densities of one (psi, u, a) triple, FP64, kern[e][k][3]
+    void adjointDensity(const double *psi, const double *u, const double *a,
+                        double *kern);
 
     // debug
     int64_t numModes() const { return mNumModes; }
@@ -260,6 +275,16 @@ protected:
     DeviceBuffer dSrcPattern, dStf, dRecElem, dRecW, dRecPhi, dSeismo;
     DeviceBuffer dFlag;
     DeviceBuffer dStepCtr;              // [0] step, [1] first recorded step
+    // adjoint: FP64 operators, densities, adjoint fields, checkpoints and
+    // the forward fields of one segment
+    DeviceBuffer dAdjGQ, dAdjGTab, dAdjKern, dAdjU, dAdjV, dAdjA, dAdjCkpt,
+        dAdjSegU, dAdjSegA, dAdjG;
+    int mRecNu1Max = 0;
+    void adjointCheck() const;
+    void adjointOperands();
+    SensArgs sensArgs(const double *psi, const double *u, const double *a,
+                      double *kern, double scale) const;
+    void swapAdjointFields();
 
     virtual void uploadReal() = 0;   // precision-specific operand upload
     virtual void autotune() = 0;     // pick kernel variants by timing

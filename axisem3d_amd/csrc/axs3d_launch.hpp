@@ -257,6 +257,33 @@ void launch_record(hipStream_t st, int nrec, const int32_t *rec_elem,
                    const double *displ, double *out, const int *step_ctr,
                    int max_samples);
 
+// adjoint: sensitivity densities of one step, kern[e][k][3] += scale *
+// (lambda, mu, rho) densities of (psi, u, a); FP64 throughout
+struct SensArgs {
+    int n_elem;
+    const int32_t *elem_points;
+    const uint8_t *elem_axial;
+    const double *gq;              // [e][6][25], row 5 the integral factor
+    const int32_t *elem_nu1;
+    const uint32_t *point_off;
+    const int32_t *point_nu1;
+    const double *gtab;            // [2][25] G_GLL, G_GLJ
+    const double *psi, *u, *a;     // [mode][3] complex
+    double *kern;
+    double scale;
+};
+void launch_sensitivity(hipStream_t st, const SensArgs &a);
+
+// adjoint force R^* g * scale into the receivers' element-local forces
+template <typename Real>
+void launch_adjoint_source(hipStream_t st, int nrec, const int32_t *rec_elem,
+                           const double *rec_w, const double *rec_phi,
+                           const double *g, double scale, int nu1max,
+                           const int32_t *elem_points,
+                           const int32_t *elem_nu1,
+                           const int32_t *point_nu1,
+                           const uint32_t *elem_eb_off, Real *eb);
+
 // step counter of a replayed (graph) step; see k_step_set
 void launch_step_set(hipStream_t st, int *ctr, int tstep, int record_step0);
 void launch_step_advance(hipStream_t st, int *ctr);

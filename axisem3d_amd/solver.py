@@ -259,6 +259,33 @@ class GpuSolver:
         self._chk(self.lib.axs3d_set_field(self.ctx, self._FIELDS[name],
                                            v.ctypes.data))
 
+    # --------------------------------------------------------- adjoint
+    def adjoint_run(self, djdr, checkpoint_interval=None):
+        """Discrete adjoint of run() from rest over len(djdr) steps
+        (axs3d_adjoint_run): djdr[n, r, c] = dJ / d seismogram[n, r, c].
+        Returns dJ/d(lambda, mu, rho) per element GLL point, shape
+        (n_elements, 25, 3).  The wavefields are left undefined."""
+        g = _c(djdr, np.float64)
+        nsteps = g.shape[0]
+        if checkpoint_interval is None:
+            checkpoint_interval = max(1, int(round(np.sqrt(nsteps))))
+        kern = np.zeros((len(self.spec.elem_points), 25, 3))
+        self._chk(self.lib.axs3d_adjoint_run(self.ctx, int(nsteps),
+                                             int(checkpoint_interval),
+                                             g.ctypes.data, kern.ctypes.data),
+                  "adjoint_run")
+        return kern
+
+    def adjoint_density(self, psi, u, a):
+        """Unscaled per-step sensitivity densities (axs3d_adjoint_density)
+        of complex (n_modes, 3) fields; shape (n_elements, 25, 3)."""
+        psi, u, a = (_c(x, np.complex128) for x in (psi, u, a))
+        kern = np.zeros((len(self.spec.elem_points), 25, 3))
+        self._chk(self.lib.axs3d_adjoint_density(
+            self.ctx, psi.ctypes.data, u.ctypes.data, a.ctypes.data,
+            kern.ctypes.data), "adjoint_density")
+        return kern
+
     # ------------------------------------------------------- profiling
     def profile(self, enable=True):
         self._chk(self.lib.axs3d_profile_enable(self.ctx, int(enable)))

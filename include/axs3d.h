@@ -305,6 +305,29 @@ int axs3d_set_record_origin(axs3d_ctx *ctx, int record_step0);
  * Both forms give bitwise the same fields.  Profiling (axs3d_profile_enable)
  * uses the launch-by-launch form. */
 int axs3d_set_graph(axs3d_ctx *ctx, int enable);
+
+/* ---------------- adjoint --------------------------------------------- */
+/* No counterpart in the reference (it has no adjoint solver): the
+ * discrete adjoint of axs3d_run for solid domains whose elements are 1D
+ * isotropic elastic without attenuation (other domains: error).  Runs the
+ * forward problem from rest over steps [0, nsteps), keeping (u, v, a)
+ * every checkpoint_interval steps on the device, then the adjoint field
+ * backwards in time with the force R^* djdr[n] / dt^2 (djdr:
+ * [nsteps][n_receivers][3] FP64, the derivative of the misfit with respect
+ * to the seismogram sample of step n, as axs3d_read_seismograms lays it
+ * out), recomputing each interval's forward fields from its checkpoint.
+ * kernels ([n_elements][25][3] FP64) receives dJ/dlambda, dJ/dmu, dJ/drho
+ * at every GLL point of every element (rho through the assembled mass
+ * sum rho IF).  Leaves the wavefields undefined (a later axs3d_run must
+ * start from fields set to rest) and the seismograms untouched. */
+int axs3d_adjoint_run(axs3d_ctx *ctx, int nsteps, int checkpoint_interval,
+                      const double *djdr, double *kernels);
+/* The per-step densities of axs3d_adjoint_run, unscaled, for one adjoint
+ * field psi, forward displacement u and forward acceleration a (each laid
+ * out as axs3d_get_field): kernels[e][k] = (<psi, dK/dlambda u>,
+ * <psi, dK/dmu u>, <psi, dM/drho a>) at GLL point k of element e. */
+int axs3d_adjoint_density(axs3d_ctx *ctx, const double *psi, const double *u,
+                          const double *a, double *kernels);
 /* Copies samples [0, nsamples) of the seismograms, [sample][rec][3]. */
 int axs3d_read_seismograms(axs3d_ctx *ctx, double *out, int nsamples);
 
