@@ -219,6 +219,21 @@ int axs3d_set_moment_source(axs3d_ctx *ctx, int elem, int nu1,
     });
 }
 
+int axs3d_add_moment_source(axs3d_ctx *ctx, int elem, int nu1,
+                            const double *pattern, int nsteps,
+                            const double *stf) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) {
+        d.addMomentSource(elem, nu1, pattern, nsteps, stf);
+    });
+}
+
+int axs3d_set_instances(axs3d_ctx *ctx, int n, const int32_t *point_inst,
+                        const double *dt) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) {
+        d.setInstances(n, point_inst, dt);
+    });
+}
+
 int axs3d_set_receivers(axs3d_ctx *ctx, int n, const int32_t *elem,
                         const double *weights, const double *phi,
                         int max_samples) {

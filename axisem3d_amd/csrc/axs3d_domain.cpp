@@ -399,12 +399,50 @@ void DomainBase::setSolidMass3D(int n, const int32_t *points,
 
 void DomainBase::setMomentSource(int elem, int nu1, const double *pattern,
                                  int nsteps, const double *stf) {
+    hSrcElem = -1;
+    hSrcElems.clear();
+    hSrcNu1s.clear();
+    hSrcPatOff.clear();
+    hSrcPattern.clear();
+    hStf.clear();
+    addMomentSource(elem, nu1, pattern, nsteps, stf);
+}
+
+// Sources of batched instances (one per instance, each in its own
+// element); all STFs have the same number of steps.
+void DomainBase::addMomentSource(int elem, int nu1, const double *pattern,
+                                 int nsteps, const double *stf) {
     if (nu1 <= 0 || nsteps <= 0)
         throw Error("Domain::setMomentSource || Empty source.");
-    hSrcElem = elem;
-    hSrcNu1 = nu1;
-    hSrcPattern.assign(pattern, pattern + (size_t)nu1 * 6 * NPEM * 2);
-    hStf.assign(stf, stf + nsteps);
+    if (!hSrcElems.empty() && (size_t)nsteps * hSrcElems.size() != hStf.size())
+        throw Error("Domain::setMomentSource || Every source needs the same "
+                    "number of STF steps.");
+    for (int32_t e : hSrcElems)
+        if (e == elem)
+            throw Error("Domain::setMomentSource || Two sources in one "
+                        "element.");
+    if (hSrcElems.empty()) hSrcElem = elem;
+    hSrcElems.push_back(elem);
+    hSrcNu1s.push_back(nu1);
+    hSrcPatOff.push_back(hSrcPattern.size() / 2);
+    hSrcPattern.insert(hSrcPattern.end(), pattern,
+                       pattern + (size_t)nu1 * 6 * NPEM * 2);
+    hStf.insert(hStf.end(), stf, stf + nsteps);
+}
+
+void DomainBase::setInstances(int n, const int32_t *point_inst,
+                              const double *dt) {
+    if (n <= 0 || !point_inst || !dt)
+        throw Error("Domain::setInstances || No instances.");
+    hInstDt.assign(dt, dt + n);
+    for (double v : hInstDt)
+        if (!(v > 0.))
+            throw Error("Domain::setInstances || Time steps must be "
+                        "positive.");
+    hPointInst.assign(point_inst, point_inst + mNumPoints);
+    for (int32_t i : hPointInst)
+        if (i < 0 || i >= n)
+            throw Error("Domain::setInstances || Instance out of range.");
 }
 
 void DomainBase::setReceivers(int n, const int32_t *elem, const double *w,
@@ -628,12 +666,24 @@ void DomainBase::finalize() {
                     ebOff[e] + (uint32_t)k * eNu1[e];
             }
     }
-    // source
-    if (hSrcElem >= 0) {
-        if (hSrcElem >= mNumElems)
+    // sources
+    mSrcNu1c.clear();
+    for (size_t i = 0; i < hSrcElems.size(); i++) {
+        const int e = hSrcElems[i];
+        if (e < 0 || e >= mNumElems)
             throw Error("Domain::finalize || Source element out of range.");
-        mSrcNu1c = std::min(eNu1[hSrcElem], hSrcNu1);
-        mStfSteps = (int)hStf.size();
+        mSrcNu1c.push_back(std::min(eNu1[e], hSrcNu1s[i]));
+    }
+    if (!hSrcElems.empty())
+        mStfSteps = (int)(hStf.size() / hSrcElems.size());
+    // batched instances
+    if (!hInstDt.empty()) {
+        if ((int)hPointInst.size() != mNumPoints)
+            throw Error("Domain::finalize || Instances set before the "
+                        "points.");
+        if (mNumFElems > 0 || mNumMass3D > 0)
+            throw Error("Domain::finalize || Batched instances with fluid "
+                        "or 3D-mass points are not supported.");
     }
     // receivers
     mNumRec = (int)hRecElem.size();
@@ -795,6 +845,11 @@ void DomainBase::finalize() {
     dVeloc.zero(st);
     dAccel.zero(st);
     if (hSrcElem >= 0) up(dStf, hStf.data(), hStf.size() * 8);
+    if (!hInstDt.empty()) {
+        std::vector<double> pdt(mNumPoints);
+        for (int p = 0; p < mNumPoints; p++) pdt[p] = hInstDt[hPointInst[p]];
+        up(dPointDt, pdt.data(), pdt.size() * 8);
+    }
     if (mNumRec > 0) {
         up(dRecElem, hRecElem.data(), hRecElem.size() * 4);
         up(dRecW, hRecW.data(), hRecW.size() * 8);
@@ -944,10 +999,11 @@ void DomainBase::adjointCheck() const {
     requireFinalized("Domain::adjoint");
     bool iso = true;
     for (uint8_t t : hMType) iso = iso && t == 0;
-    if (mNum3D > 0 || mNsls > 0 || mNumFElems > 0 || mNumMass3D > 0 || !iso)
+    if (mNum3D > 0 || mNsls > 0 || mNumFElems > 0 || mNumMass3D > 0 ||
+        !iso || !hInstDt.empty())
         throw Error("Domain::adjoint || The adjoint run supports solid "
                     "elements with a 1D isotropic elastic medium and no "
-                    "attenuation, fluid or 3D mass.");
+                    "attenuation, fluid, 3D mass or batched instances.");
 }
 
 void DomainBase::adjointOperands() {
@@ -1162,6 +1218,7 @@ public:
         g.inv_mass = dInvMass.as<double>();
         g.mass3d = mNumMass3D > 0 ? dMass3DFlag.as<uint8_t>() : nullptr;
         g.adj = dAdj.as<uint32_t>();
+        if (!hInstDt.empty()) g.point_dt = dPointDt.as<double>();
         return g;
     }
     Mass3DArgs<Real> mass3dArgs() const {
@@ -1255,11 +1312,13 @@ public:
         if (!dev && (tstep < 0 || tstep >= mStfSteps))
             throw Error("Domain::applySources || Step outside the STF.");
         tick(AXS3D_KERNEL_SOURCE);
-        launch_moment_source<Real>(mStream, toElem, stiffArgs(), hSrcElem,
-                                   mSrcNu1c, dSrcPattern.as<Real>(),
-                                   dStf.as<double>(), tstep, mStfSteps,
-                                   dev ? dStepCtr.as<int>() : nullptr,
-                                   dStiff.as<Real>());
+        const StiffArgs<Real> sa = stiffArgs();
+        for (size_t i = 0; i < hSrcElems.size(); i++)
+            launch_moment_source<Real>(
+                mStream, toElem, sa, hSrcElems[i], mSrcNu1c[i],
+                dSrcPattern.as<Real>() + 2 * hSrcPatOff[i],
+                dStf.as<double>() + i * (size_t)mStfSteps, tstep, mStfSteps,
+                dev ? dStepCtr.as<int>() : nullptr, dStiff.as<Real>());
         tock(AXS3D_KERNEL_SOURCE);
     }
 

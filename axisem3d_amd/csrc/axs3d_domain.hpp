@@ -88,6 +88,9 @@ public:
     void setSolidMass3D(int n, const int32_t *points, const double *mass);
     void setMomentSource(int elem, int nu1, const double *pattern, int nsteps,
                          const double *stf);
+    void addMomentSource(int elem, int nu1, const double *pattern, int nsteps,
+                         const double *stf);
+    void setInstances(int n, const int32_t *point_inst, const double *dt);
     void setReceivers(int n, const int32_t *elem, const double *w,
                       const double *phi, int max_samples);
     void setDt(double dt) {
@@ -195,8 +198,15 @@ protected:
     int64_t mNumFModes = 0, mNumFEBModes = 0;
     int mTypeBegin3D[4] = {0, 0, 0, 0};
     std::vector<int32_t> hAxial;
-    int hSrcElem = -1, hSrcNu1 = 0;
+    // moment sources: hSrcElem is the first source's element (-1: none);
+    // patterns and STFs of all sources back to back
+    int hSrcElem = -1;
+    std::vector<int32_t> hSrcElems, hSrcNu1s;
+    std::vector<size_t> hSrcPatOff;     // complex values before source i
     std::vector<double> hSrcPattern, hStf;
+    // batched instances: instance of each solid point, dt of each instance
+    std::vector<int32_t> hPointInst;
+    std::vector<double> hInstDt;
     std::vector<int32_t> hRecElem;
     std::vector<double> hRecW, hRecPhi;
     int hMaxSamples = 0;
@@ -238,7 +248,7 @@ protected:
     std::vector<double> hAtt1D;         // [e][2][npa] dlambda, dmu
     std::vector<double> hTwTab;         // interleaved twiddles, FP64
     void layout3D();                    // element Nr and slice offsets
-    int mSrcNu1c = 0;
+    std::vector<int> mSrcNu1c;          // orders each source applies
     int mStfSteps = 0;
 
     // device: points
@@ -273,6 +283,7 @@ protected:
         dCplNM, dCStiff;
     // device: source and receivers
     DeviceBuffer dSrcPattern, dStf, dRecElem, dRecW, dRecPhi, dSeismo;
+    DeviceBuffer dPointDt;
     DeviceBuffer dFlag;
     DeviceBuffer dStepCtr;              // [0] step, [1] first recorded step
     // adjoint: FP64 operators, densities, adjoint fields, checkpoints and

@@ -309,6 +309,7 @@ struct PointArgs {
     const uint32_t *adj;
     double *displ, *veloc, *accel;
     double dt, half_dt, half_dt_dt;
+    const double *point_dt;       // per-point time steps, or null
     int *flag;
     CouplingGeom cpl;
 };
@@ -407,13 +408,19 @@ __global__ __launch_bounds__(256) void k_point(PointArgs a,
         a2[c] = ap[c];
     }
     bool bad = false;
+    double dt = a.dt, half_dt = a.half_dt, half_dt_dt = a.half_dt_dt;
+    if (a.point_dt != nullptr) {
+        dt = a.point_dt[p];
+        half_dt = .5 * dt;
+        half_dt_dt = .5 * dt * dt;
+    }
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-        const double vr = v2[c].x + a.half_dt * (a2[c].x + a_new[2 * c]);
-        const double vi = v2[c].y + a.half_dt * (a2[c].y + a_new[2 * c + 1]);
-        const double ur = u2[c].x + (a.dt * vr + a.half_dt_dt * a_new[2 * c]);
+        const double vr = v2[c].x + half_dt * (a2[c].x + a_new[2 * c]);
+        const double vi = v2[c].y + half_dt * (a2[c].y + a_new[2 * c + 1]);
+        const double ur = u2[c].x + (dt * vr + half_dt_dt * a_new[2 * c]);
         const double ui =
-            u2[c].y + (a.dt * vi + a.half_dt_dt * a_new[2 * c + 1]);
+            u2[c].y + (dt * vi + half_dt_dt * a_new[2 * c + 1]);
         v2[c] = make_double2(vr, vi);
         a2[c] = make_double2(a_new[2 * c], a_new[2 * c + 1]);
         u2[c] = make_double2(ur, ui);
@@ -570,6 +577,7 @@ void launch_point(hipStream_t st, int phases, const PointGeom &g,
     a.dt = dt;
     a.half_dt = .5 * dt;
     a.half_dt_dt = .5 * dt * dt;
+    a.point_dt = g.point_dt;
     a.flag = flag;
     a.cpl = cpl;
     if (cpl.n_pairs <= 0) phases &= ~PHASE_COUPLE;

@@ -157,6 +157,9 @@ struct PointGeom {
     const uint8_t *mass3d;         // [point] 1: 3D mass, or null
     const uint32_t *adj;           // [point][4] element-local offsets,
                                    // ascending element, padded with ~0u
+    // independent instances batched in one domain: the time step of each
+    // point's instance (null: one time step for all)
+    const double *point_dt = nullptr;
 };
 
 // Solid points with a 3D mass (Mass3D): inverse mass per slice in Real,

@@ -1278,3 +1278,103 @@ def subdomain(spec, elems, prop3d=None, d3d=None):
             sub.att = dict(att)
             sub.att["d3d"] = d3d
     return sub
+
+
+# ------------------------------------------------------ batched instances
+def local_instance(seed, n_elem, nu, nsteps, n_receivers=4, h=1.0e3,
+                   courant=0.5, m0=1e16, monitors=0):
+    """One independent local simulation: a Cartesian box of about n_elem
+    elements of size h, Fourier order nu (bounded by the in-plane
+    resolution as the reference bounds it), a seeded isotropic two-layer
+    model, a seeded axial moment tensor with a Ricker STF, surface
+    receivers and `monitors` receivers 1.5 km apart at the source depth
+    (signal within a few hundred steps on large boxes)."""
+    rng = np.random.default_rng(seed)
+    nx = max(2, int(round(math.sqrt(n_elem))))
+    nz = max(2, int(round(n_elem / nx)))
+    width, depth = nx * h, nz * h
+    mesh = StructuredMesh(np.linspace(0.0, width, nx + 1),
+                          np.linspace(0.0, depth, nz + 1))
+    vs = rng.uniform(2800.0, 3600.0, 2)
+    vp = vs * rng.uniform(1.70, 1.80, 2)
+    rho = rng.uniform(2400.0, 3000.0, 2)
+    z_if = depth * rng.uniform(0.4, 0.7)
+
+    def material(r, rc):
+        top = rc > z_if
+        one = np.ones_like(r)
+        return (np.where(top, rho[0], rho[1]) * one,
+                np.where(top, vp[0], vp[1]) * one,
+                np.where(top, vs[0], vs[1]) * one)
+    nr = 2 * int(nu) + 1
+    spec = build_domain(mesh, material, lambda s, z: np.full(s.shape, nr),
+                        courant=courant)
+    mt = rng.standard_normal(6)
+    mt *= m0 / np.linalg.norm(mt)
+    f = 2.0 * math.pi * 1.0
+    z_src = depth - 0.3 * depth
+    add_axial_moment_source(spec, z_src, mt, 1.628 / f, 1.628, nsteps)
+    s_rec = np.linspace(width / n_receivers, width, n_receivers)
+    z_rec = np.full(n_receivers, depth)
+    if monitors:
+        s_rec = np.concatenate([s_rec, 1.5e3 * np.arange(1, monitors + 1)])
+        z_rec = np.concatenate([z_rec, np.full(monitors, z_src)])
+    phi = rng.uniform(0.0, 2.0 * math.pi, len(s_rec))
+    add_receivers(spec, s_rec, z_rec, phi)
+    spec.name = f"local_E{mesh.nelem}_nu{nu}_seed{seed}"
+    return spec
+
+
+def batch_instances(specs):
+    """One domain holding independent solid instances (1D isotropic, no
+    fluid): points, elements, sources and receivers concatenated, each
+    instance keeping its own time step (spec.point_instance,
+    spec.instance_dt).  spec._instances lists, per instance, its point,
+    element and receiver ranges."""
+    if not specs:
+        raise ValueError("no instances")
+    nsteps = specs[0].nsteps
+    for s in specs:
+        if getattr(s, "fluid_nr", None) is not None or \
+                getattr(s, "elem_kind", None) is not None or \
+                getattr(s, "elem_mtype", None) is not None:
+            raise ValueError("batched instances must be 1D isotropic solids")
+        if s.nsteps != nsteps or len(s.stf) != len(specs[0].stf):
+            raise ValueError("instances need the same number of steps")
+    p0 = np.cumsum([0] + [len(s.solid_nr) for s in specs])
+    e0 = np.cumsum([0] + [s.n_solid_elements for s in specs])
+    r0 = np.cumsum([0] + [0 if s.rec_elem is None else len(s.rec_elem)
+                          for s in specs])
+    cat = np.concatenate
+    out = DomainSpec(
+        solid_nr=cat([s.solid_nr for s in specs]).astype(np.int32),
+        solid_mass=cat([s.solid_mass for s in specs]),
+        solid_sz=cat([s.solid_sz for s in specs]),
+        elem_points=cat([s.elem_points + p for s, p in
+                         zip(specs, p0)]).astype(np.int32),
+        elem_axial=cat([s.elem_axial for s in specs]),
+        elem_gq=np.ascontiguousarray(cat([s.elem_gq for s in specs])),
+        elem_nr=cat([s.elem_nr for s in specs]).astype(np.int32),
+        elem_iso=np.ascontiguousarray(cat([s.elem_iso for s in specs])),
+        axial_points=cat([s.axial_points + p for s, p in
+                          zip(specs, p0)]).astype(np.int32),
+        dt=float(specs[0].dt))
+    out.src_elem = int(specs[0].src_elem)
+    out.src_pattern = specs[0].src_pattern
+    out.stf = specs[0].stf
+    out.extra_sources = [(int(s.src_elem + e), s.src_pattern, s.stf)
+                         for s, e in zip(specs[1:], e0[1:])]
+    out.rec_elem = cat([s.rec_elem + e for s, e in
+                        zip(specs, e0)]).astype(np.int32)
+    out.rec_weights = cat([s.rec_weights for s in specs])
+    out.rec_phi = cat([s.rec_phi for s in specs])
+    out.nsteps = nsteps
+    out.point_instance = cat([np.full(len(s.solid_nr), i, np.int32)
+                              for i, s in enumerate(specs)])
+    out.instance_dt = np.array([s.dt for s in specs])
+    out._instances = [((int(p0[i]), int(p0[i + 1])),
+                       (int(e0[i]), int(e0[i + 1])),
+                       (int(r0[i]), int(r0[i + 1])))
+                      for i in range(len(specs))]
+    out.name = f"batch_{len(specs)}"
+    return out

@@ -163,6 +163,19 @@ class GpuSolver:
             self._chk(L.axs3d_set_moment_source(
                 ctx, int(s.src_elem), pat.shape[0], pat.ctypes.data,
                 len(stf), stf.ctypes.data), "set_source")
+        # further sources of batched instances
+        for (e, pat, stf) in getattr(s, "extra_sources", None) or []:
+            pat = _c(pat, np.complex128)
+            stf = _c(stf, np.float64)
+            self._chk(L.axs3d_add_moment_source(
+                ctx, int(e), pat.shape[0], pat.ctypes.data, len(stf),
+                stf.ctypes.data), "add_source")
+        pinst = getattr(s, "point_instance", None)
+        if pinst is not None:
+            pinst = _c(pinst, np.int32)
+            idt = _c(s.instance_dt, np.float64)
+            self._chk(L.axs3d_set_instances(ctx, len(idt), pinst.ctypes.data,
+                                            idt.ctypes.data), "set_instances")
         self.nrec = 0 if s.rec_elem is None else len(s.rec_elem)
         if self.nrec:
             ms = int(max_samples if max_samples is not None else

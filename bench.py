@@ -45,7 +45,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="regional",
-                    choices=["regional", "cartesian8000"])
+                    choices=["regional", "cartesian8000", "batched"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
@@ -74,7 +74,27 @@ def copy_calibration(mib):
 
 def build_workload(name, nsteps):
     from axisem3d_amd import preloop, synth
-    if name == "cartesian8000":
+    if name == "batched":
+        # one GPU's share (6 of 48) of the batched uneven instances:
+        # elements ~ U[4k, 40k], nu ~ U[2, 256], seeded two-layer models
+        rng = np.random.default_rng(2024)
+        specs = [preloop.local_instance(1000 + i,
+                                        int(rng.integers(4000, 40001)),
+                                        int(rng.integers(2, 257)), nsteps,
+                                        monitors=2)
+                 for i in range(6)]
+        spec = preloop.batch_instances(specs)
+        spec._specs = specs
+        cfg = {"workload": "batched uneven local instances, one GPU's share "
+               "(6 of 48): elements ~ U[4k, 40k], nu ~ U[2, 256] (Nr bounded "
+               "by the in-plane resolution), seeded isotropic two-layer "
+               "models and moment tensors, per-instance dt, 4 surface "
+               "receivers and 2 near-source monitors each",
+               "instances": [{"elements": s.n_solid_elements,
+                              "nr_max": int(s.elem_nr.max()),
+                              "updates_per_step": s.n_updates_per_step(),
+                              "dt": s.dt} for s in specs]}
+    elif name == "cartesian8000":
         spec = preloop.cartesian_homogeneous(nsteps=nsteps)
         cfg = {"workload": "cartesian box 80x80 km, 8000 elements, npol 4, "
                "nu=2 (Nr=5), homogeneous isotropic solid, axial moment "
@@ -219,6 +239,11 @@ def cpu_baseline(spec, seconds, workload):
     if workload == "cartesian8000":
         sub, scale = spec, 1.0
         what = "the whole workload"
+    elif workload == "batched":
+        sub = min(spec._specs, key=lambda s: s.n_updates_per_step())
+        scale = spec.n_updates_per_step() / sub.n_updates_per_step()
+        what = (f"the smallest instance ({sub.n_solid_elements} elements), "
+                "scaled to the batch by point-modes")
     else:
         nx = spec._mesh.nx
         nz = spec._mesh.nz
@@ -263,6 +288,35 @@ def cpu_baseline(spec, seconds, workload):
             "kind": "port",
             "sample": f"{n} Newmark steps of {what}; FP64 oracle, single "
                       f"thread, {t:.1f} s"}
+
+
+def batched_extras(spec, seis32, seis64, steps, warmup, t_batch):
+    """Per-instance seismogram L2 of FP32+FP64 against FP64, and the time
+    of the same instances run one after the other, each alone (graph
+    replays, against the batch's graph replays t_batch)."""
+    from axisem3d_amd.solver import GpuSolver
+    out = {"seismogram_rel_l2_vs_fp64": [], "alone_ms_per_step": []}
+    for i, (_, _, (r0, r1)) in enumerate(spec._instances):
+        if seis64 is not None:
+            a, b = seis32[:, r0:r1], seis64[:, r0:r1]
+            out["seismogram_rel_l2_vs_fp64"].append(
+                float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)))
+    total = 0.0
+    for s in spec._specs:
+        g = GpuSolver(s, "fp32", max_samples=steps + warmup)
+        g.run(warmup)
+        g.synchronize()
+        t0 = time.perf_counter()
+        g.run(steps)
+        g.synchronize()
+        dt = time.perf_counter() - t0
+        g.close()
+        out["alone_ms_per_step"].append(dt / steps * 1e3)
+        total += dt
+    out["alone_total_ms_per_step"] = total / steps * 1e3
+    out["batched_speedup_vs_alone"] = total / t_batch
+    log(f"instances alone: {total / steps * 1e3:.3f} ms per step")
+    return out
 
 
 def main():
@@ -360,6 +414,10 @@ def main():
                          "value": nupd * a.steps / tg},
         "autotune": tune32,
     }
+    if a.workload == "batched":
+        out["per_instance"] = batched_extras(spec, seis32,
+                                             seis64 if out_64 else None,
+                                             a.steps, a.warmup, tg)
     if not a.no_cpu:
         t0 = time.perf_counter()
         out["cpu_baseline"] = cpu_baseline(spec, a.cpu_seconds, a.workload)

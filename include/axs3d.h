@@ -251,6 +251,20 @@ int axs3d_set_solid_mass3d(axs3d_ctx *ctx, int n, const int32_t *points,
 int axs3d_set_moment_source(axs3d_ctx *ctx, int elem, int nu1,
                             const double *pattern, int nsteps,
                             const double *stf);
+/* One more moment source, for batched instances (no reference
+ * counterpart: the reference runs one simulation per process).  Each
+ * source lies in its own element; every STF has the same nsteps. */
+int axs3d_add_moment_source(axs3d_ctx *ctx, int elem, int nu1,
+                            const double *pattern, int nsteps,
+                            const double *stf);
+/* Batched independent instances in one domain (no reference counterpart):
+ * point_inst[p] is the instance of solid point p (after
+ * axs3d_set_solid_points) and dt[i] the time step of instance i, used by
+ * the Newmark update of its points.  The instances' meshes must not share
+ * points; their elements, sources and receivers are simply concatenated.
+ * Not with fluid or 3D-mass points. */
+int axs3d_set_instances(axs3d_ctx *ctx, int n, const int32_t *point_inst,
+                        const double *dt);
 
 /* Receivers: element, in-plane weights [n][25], azimuth phi.  Replaces
  * StationSolid::setElement (src/core/output/station-wise/station/
