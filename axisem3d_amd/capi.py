@@ -71,6 +71,8 @@ SIGNATURES = {
     "axs3d_run": (_i, [_p, _i, _i]),
     "axs3d_set_record_origin": (_i, [_p, _i]),
     "axs3d_set_graph": (_i, [_p, _i]),
+    "axs3d_set_fluid_mass3d": (_i, [_p, _i, _p, _p]),
+    "axs3d_set_coupling3d": (_i, [_p, _i, _p, _p, _p]),
     "axs3d_add_moment_source": (_i, [_p, _i, _i, _p, _i, _p]),
     "axs3d_set_instances": (_i, [_p, _i, _p, _p]),
     "axs3d_adjoint_run": (_i, [_p, _i, _i, _p, _p]),

@@ -219,6 +219,21 @@ int axs3d_set_moment_source(axs3d_ctx *ctx, int elem, int nu1,
     });
 }
 
+int axs3d_set_fluid_mass3d(axs3d_ctx *ctx, int n, const int32_t *points,
+                           const double *mass) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) {
+        d.setFluidMass3D(n, points, mass);
+    });
+}
+
+int axs3d_set_coupling3d(axs3d_ctx *ctx, int n, const int32_t *pairs,
+                         const double *n_unassembled,
+                         const double *n_assembled) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) {
+        d.setCoupling3D(n, pairs, n_unassembled, n_assembled);
+    });
+}
+
 int axs3d_add_moment_source(axs3d_ctx *ctx, int elem, int nu1,
                             const double *pattern, int nsteps,
                             const double *stf) {

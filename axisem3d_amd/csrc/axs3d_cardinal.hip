@@ -1032,6 +1032,233 @@ void launch_mass3d(hipStream_t st, bool march, const Mass3DArgs<Real> &a,
                            st, a, stiff);
 }
 
+// C2R input of one wave's buffer (SolverFFTW conventions, as pack_pair):
+// the Hermitian extension of the modes X(k) + i Y(k), mean and Nyquist
+// real, Nyquist doubled.
+template <typename Real, typename GX, typename GY>
+__device__ __forceinline__ void c2r_pack(cpx<Real> *b, int nr, int nu1,
+                                         bool even, int lane, GX gx, GY gy) {
+    const int nyq = even ? nu1 - 1 : -1;
+    for (int k = lane; k < nr; k += 64) {
+        const bool lo = k < nu1;
+        const int kk = lo ? k : nr - k;
+        const cpx<Real> x = gx(kk);
+        const cpx<Real> y = gy(kk);
+        const bool real_only = k == 0 || k == nyq;
+        const Real sc = k == nyq ? (Real)2 : (Real)1;
+        const Real sim = real_only ? (Real)0 : (lo ? (Real)1 : (Real)-1);
+        b[pad(k)] = {sc * x.re - sim * y.im, sim * x.im + sc * y.re};
+    }
+}
+
+// R2C output of one wave's buffer (as unpack_pair): the modes of the real
+// series x (v0) and y (v1) packed as x + i y, 1 / Nr, Nyquist halved.
+template <typename Real, typename F>
+__device__ __forceinline__ void r2c_unpack(const cpx<Real> *b, int nr,
+                                           int nu1, bool even, int lane,
+                                           F put) {
+    const int nyq = even ? nu1 - 1 : -1;
+    const Real h0 = (Real)0.5 / (Real)nr;
+    for (int k = lane; k < nu1; k += 64) {
+        const cpx<Real> zk = b[pad(k)];
+        const cpx<Real> zn = b[pad(k == 0 ? 0 : nr - k)];
+        const Real h = k == nyq ? h0 * (Real)0.5 : h0;
+        put(k, cpx<Real>{(zk.re + zn.re) * h, (zk.im - zn.im) * h},
+            cpx<Real>{(zk.im + zn.im) * h, (zn.re - zk.re) * h});
+    }
+}
+
+// Deferred fluid points, one block of two waves each (see FluidDeferArgs):
+//   COUPLE  for a 3D pair (SolidFluidCoupling3D::coupleSolidToFluid,
+//           SolidFluidCoupling3D.cpp:55-73): C2R of the solid displacement
+//           (wave 0: s + i phi, wave 1: z), n . u per slice, R2C (wave 1)
+//           added to the fluid stiffness; then coupleFluidToSolid (:76-94):
+//           C2R of the fluid stiffness, times n / m_fluid per slice, R2C
+//           (wave 0: s + i phi, wave 1: z) into csolid3d
+//   AXIAL   fluid axis (orders >= 1 zero) and free surface (all zero)
+//   MASS    Nyquist imaginary part zero, then Mass3D (C2R, times 1 / m per
+//           slice, R2C) or Mass1D
+//   MARCH   Newmark (FP64), fluid stiffness zeroed
+template <typename Real, bool COUPLE, bool AXIAL, bool MASS, bool MARCH>
+__global__ __launch_bounds__(128) void k_fluid_defer(FluidDeferArgs<Real> a) {
+    __shared__ cpx<Real> tw[CARD_TW_CAP];
+    __shared__ cpx<Real> wbuf[2][CD_BUF];
+    __shared__ uint16_t pos[CARD_NR_CAP];
+    __shared__ cpx<Real> sf[CARD_NR_CAP / 2 + 1];
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int q = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int i = blockIdx.x;
+    const int p = a.point[i];
+    const int nr = a.point_nr[p];
+    const int nu1 = nr / 2 + 1;
+    const bool even = (nr & 1) == 0;
+    const uint32_t *plan = a.plan + (size_t)nr * CARD_PLAN_W;
+    const size_t foff = a.point_off[p];
+    {
+        const cpx<Real> *twg = a.twiddle + a.tw_off[nr];
+        const int ntw = (int)(plan[0] >> 8);
+        for (int x = t; x < ntw; x += 128) tw[x] = twg[x];
+        const uint16_t *pg = a.slice_pos + a.pos_off[nr];
+        for (int x = t; x < nr; x += 128) pos[x] = pg[x];
+        const cpx<Real> *fs = reinterpret_cast<const cpx<Real> *>(a.fstiff);
+        for (int k = t; k < nu1; k += 128) sf[k] = fs[foff + k];
+    }
+    __syncthreads();
+    const cpx<Real> zero = {(Real)0, (Real)0};
+    cpx<Real> *b = wbuf[q];
+    const int j = COUPLE ? a.pair3d[i] : -1;
+    if (COUPLE && j >= 0) {
+        const int sp = a.pair_solid[j];
+        const double *u = a.solid_displ + (size_t)a.solid_off[sp] * 6;
+        const int cx = q == 0 ? 0 : 2;
+        c2r_pack<Real>(
+            b, nr, nu1, even, lane,
+            [&](int k) {
+                return cpx<Real>{(Real)u[k * 6 + 2 * cx],
+                                 (Real)u[k * 6 + 2 * cx + 1]};
+            },
+            [&](int k) {
+                return q == 0 ? cpx<Real>{(Real)u[k * 6 + 2],
+                                          (Real)u[k * 6 + 3]}
+                              : zero;
+            });
+        wave_sync();
+        wfft<Real>(b, nr, plan, true, tw, lane);
+        __syncthreads();
+        const Real *nn = a.nu + a.n_off[j] * 3;
+        for (int k = t; k < nr; k += 128) {
+            const int pk = pad(pos[k]);
+            const cpx<Real> sphi = wbuf[0][pk];
+            const Real f = nn[3 * k] * sphi.re + nn[3 * k + 1] * sphi.im +
+                           nn[3 * k + 2] * wbuf[1][pk].re;
+            wbuf[1][pk] = {f, (Real)0};
+        }
+        __syncthreads();
+        if (q == 1) {
+            wfft<Real>(b, nr, plan, false, tw, lane);
+            r2c_unpack<Real>(b, nr, nu1, even, lane,
+                             [&](int k, cpx<Real> v0, cpx<Real>) {
+                                 sf[k] = cadd(sf[k], v0);
+                             });
+        }
+        __syncthreads();
+        if (q == 0) {
+            c2r_pack<Real>(b, nr, nu1, even, lane,
+                           [&](int k) { return sf[k]; },
+                           [&](int) { return zero; });
+            wave_sync();
+            wfft<Real>(b, nr, plan, true, tw, lane);
+        }
+        __syncthreads();
+        const Real *na = a.nam + a.n_off[j] * 3;
+        for (int k = t; k < nr; k += 128) {
+            const int pk = pad(pos[k]);
+            const Real g = wbuf[0][pk].re;
+            wbuf[0][pk] = {g * na[3 * k], g * na[3 * k + 1]};
+            wbuf[1][pk] = {g * na[3 * k + 2], (Real)0};
+        }
+        __syncthreads();
+        wfft<Real>(b, nr, plan, false, tw, lane);
+        double *cs = a.csolid3d + (size_t)a.c3off[j] * 6;
+        r2c_unpack<Real>(b, nr, nu1, even, lane,
+                         [&](int k, cpx<Real> v0, cpx<Real> v1) {
+                             if (q == 0) {
+                                 cs[k * 6 + 0] = (double)v0.re;
+                                 cs[k * 6 + 1] = (double)v0.im;
+                                 cs[k * 6 + 2] = (double)v1.re;
+                                 cs[k * 6 + 3] = (double)v1.im;
+                             } else {
+                                 cs[k * 6 + 4] = (double)v0.re;
+                                 cs[k * 6 + 5] = (double)v0.im;
+                             }
+                         });
+        __syncthreads();
+    }
+    if (AXIAL) {
+        const uint8_t bc = a.point_bc[p];
+        for (int k = t; k < nu1; k += 128)
+            if (((bc & 1) && k >= 1) || (bc & 2)) sf[k] = zero;
+        __syncthreads();
+    }
+    if (MASS) {
+        if (even && t == 0) sf[nu1 - 1].im = (Real)0;
+        __syncthreads();
+        const int64_t mo = a.minv_off[i];
+        if (mo >= 0) {
+            if (q == 0) {
+                c2r_pack<Real>(b, nr, nu1, even, lane,
+                               [&](int k) { return sf[k]; },
+                               [&](int) { return zero; });
+                wave_sync();
+                wfft<Real>(b, nr, plan, true, tw, lane);
+                const Real *m = a.minv + mo;
+                for (int k = lane; k < nr; k += 64) {
+                    const int pk = pad(pos[k]);
+                    b[pk] = {b[pk].re * m[k], (Real)0};
+                }
+                wave_sync();
+                wfft<Real>(b, nr, plan, false, tw, lane);
+                r2c_unpack<Real>(b, nr, nu1, even, lane,
+                                 [&](int k, cpx<Real> v0, cpx<Real>) {
+                                     sf[k] = v0;
+                                 });
+            }
+        } else {
+            const Real im = (Real)a.inv_mass[p];
+            for (int k = t; k < nu1; k += 128)
+                sf[k] = {sf[k].re * im, sf[k].im * im};
+        }
+        __syncthreads();
+    }
+    cpx<Real> *fs = reinterpret_cast<cpx<Real> *>(a.fstiff) + foff;
+    if (!MARCH) {
+        for (int k = t; k < nu1; k += 128) fs[k] = sf[k];
+        return;
+    }
+    const double hdt = .5 * a.dt, hdt2 = .5 * a.dt * a.dt;
+    bool bad = false;
+    for (int k = t; k < nu1; k += 128) {
+        const size_t m2 = (foff + k) * 2;
+        const double an[2] = {(double)sf[k].re, (double)sf[k].im};
+#pragma unroll
+        for (int ri = 0; ri < 2; ri++) {
+            const double vv = a.chi_v[m2 + ri] + hdt * (a.chi_a[m2 + ri] +
+                                                         an[ri]);
+            const double uu = a.chi[m2 + ri] + (a.dt * vv + hdt2 * an[ri]);
+            a.chi_v[m2 + ri] = vv;
+            a.chi_a[m2 + ri] = an[ri];
+            a.chi[m2 + ri] = uu;
+            bad |= !isfinite(uu);
+        }
+        fs[k] = zero;
+    }
+    if (bad) atomicOr(a.flag, 1);
+}
+
+template <typename Real>
+void launch_fluid_defer(hipStream_t st, int phases,
+                        const FluidDeferArgs<Real> &a) {
+    if (a.n <= 0) return;
+    const bool c = (phases & PHASE_COUPLE) != 0;
+    const bool x = (phases & PHASE_AXIAL) != 0;
+    const bool m = (phases & PHASE_MASS) != 0;
+    const bool t = (phases & PHASE_MARCH) != 0;
+#define AXS3D_DEFER(C_, X_, M_, T_)                                           \
+    if (c == C_ && x == X_ && m == M_ && t == T_) {                           \
+        hipLaunchKernelGGL((k_fluid_defer<Real, C_, X_, M_, T_>), dim3(a.n),  \
+                           dim3(128), 0, st, a);                              \
+        return;                                                               \
+    }
+    AXS3D_DEFER(true, false, false, false)
+    AXS3D_DEFER(false, true, false, false)
+    AXS3D_DEFER(false, false, true, false)
+    AXS3D_DEFER(false, false, false, true)
+    AXS3D_DEFER(false, true, true, true)
+    AXS3D_DEFER(true, true, true, true)
+#undef AXS3D_DEFER
+}
+
 template <typename Real>
 int card_blocks_per_cu(bool att) {
     int n = 0;
@@ -1062,6 +1289,10 @@ template void launch_cdf_batch<float>(hipStream_t,
 template void launch_cdf_batch<double>(hipStream_t,
                                        const FluidCardArgs<double> &,
                                        const CardBatch &);
+template void launch_fluid_defer<float>(hipStream_t, int,
+                                        const FluidDeferArgs<float> &);
+template void launch_fluid_defer<double>(hipStream_t, int,
+                                         const FluidDeferArgs<double> &);
 template void launch_mass3d<float>(hipStream_t, bool,
                                    const Mass3DArgs<float> &, float *);
 template void launch_mass3d<double>(hipStream_t, bool,

@@ -236,6 +236,49 @@ void DomainBase::setSolidFluidCoupling(int n, const int32_t *solid,
     hCplNormal.assign(normals, normals + (size_t)n * 4);
 }
 
+void DomainBase::setFluidMass3D(int n, const int32_t *points,
+                                const double *mass) {
+    if (hFPointNr.empty())
+        throw Error("Domain::setFluidMass3D || Set the fluid points first.");
+    hFMass3DPts.clear();
+    hFMass3DOff.clear();
+    hFMass3D.clear();
+    std::vector<uint8_t> seen(hFPointNr.size(), 0);
+    for (int i = 0; i < std::max(n, 0); i++) {
+        const int p = points[i];
+        if (p < 0 || p >= (int)hFPointNr.size() || seen[p])
+            throw Error("Domain::setFluidMass3D || Point out of range or "
+                        "listed twice.");
+        seen[p] = 1;
+        hFMass3DPts.push_back(p);
+        hFMass3DOff.push_back((int64_t)hFMass3D.size());
+        for (int k = 0; k < hFPointNr[p]; k++) {
+            const double m = *mass++;
+            if (!(m > 0.))
+                throw Error("Domain::setFluidMass3D || Mass must be "
+                            "positive.");
+            hFMass3D.push_back(m);
+        }
+    }
+}
+
+void DomainBase::setCoupling3D(int n, const int32_t *pairs, const double *n_u,
+                               const double *n_a) {
+    if (n < 0 || (n > 0 && (!pairs || !n_u || !n_a)))
+        throw Error("Domain::setCoupling3D || Bad arguments.");
+    hCpl3DPairs.assign(pairs, pairs + n);
+    size_t tot = 0;
+    for (int k : hCpl3DPairs) {
+        if (k < 0 || k >= mNumPairs || hCplSolid[k] < 0 ||
+            hCplSolid[k] >= (int)hPointNr.size())
+            throw Error("Domain::setCoupling3D || Pair out of range (set "
+                        "the coupling pairs and solid points first).");
+        tot += (size_t)hPointNr[hCplSolid[k]] * 3;
+    }
+    hCpl3DNu.assign(n_u, n_u + tot);
+    hCpl3DNa.assign(n_a, n_a + tot);
+}
+
 void DomainBase::setRtzTheta(const double *theta) {
     if (mNumElems <= 0)
         throw Error("Domain::setRtzTheta || Set elements first.");
@@ -556,6 +599,8 @@ void DomainBase::finalize() {
     std::vector<int> fftNr;
     for (int32_t e : list3d) fftNr.push_back(hElemNr[e]);
     for (int32_t p : hMass3DPts) fftNr.push_back(hPointNr[p]);
+    for (int32_t p : hFMass3DPts) fftNr.push_back(hFPointNr[p]);
+    for (int32_t k : hCpl3DPairs) fftNr.push_back(hPointNr[hCplSolid[k]]);
     // fluid elements with a 3D medium, heaviest first
     hFList3D.clear();
     if (mNumFElems > 0 && !hFElemKind.empty()) {
@@ -702,6 +747,27 @@ void DomainBase::finalize() {
     std::vector<uint8_t> fBC(mNumFPoints, 0);
     std::vector<EMItem> fem;
     std::vector<double> cN, cNM;
+    // 3D pairs (index j among them) and the deferred fluid points
+    std::vector<int32_t> pair3d(std::max(mNumPairs, 0), -1),
+        cSolidPair3D(mNumPoints, -1), fPair3D(mNumFPoints, -1), dfPts,
+        dfPair, cpl3Solid;
+    std::vector<int64_t> fMass3DOff(mNumFPoints, -1), dfMOff, cpl3NOff;
+    std::vector<uint32_t> cpl3COff;
+    std::vector<uint8_t> dfFlag;
+    uint32_t cpl3Modes = 0;
+    if (!hFMass3DPts.empty() && mNumFElems <= 0)
+        throw Error("Domain::finalize || Fluid 3D mass without fluid "
+                    "elements.");
+    for (size_t i = 0; i < hFMass3DPts.size(); i++)
+        fMass3DOff[hFMass3DPts[i]] = hFMass3DOff[i];
+    mNumCpl3D = (int)hCpl3DPairs.size();
+    for (int j = 0; j < mNumCpl3D; j++) {
+        const int k = hCpl3DPairs[j];
+        if (k < 0 || k >= mNumPairs || pair3d[k] >= 0)
+            throw Error("Domain::finalize || 3D coupling pair out of range "
+                        "or listed twice.");
+        pair3d[k] = j;
+    }
     if (mNumFElems > 0) {
         if (mNumFPoints <= 0)
             throw Error("Domain::finalize || Fluid points are not set.");
@@ -773,6 +839,7 @@ void DomainBase::finalize() {
         cFOff.resize(mNumPairs);
         cN.resize((size_t)mNumPairs * 2);
         cNM.resize((size_t)mNumPairs * 2);
+        std::vector<uint8_t> sSeen(mNumPoints, 0), fSeen(mNumFPoints, 0);
         for (int k = 0; k < mNumPairs; k++) {
             const int sp = hCplSolid[k], fp = hCplFluid[k];
             if (sp < 0 || sp >= mNumPoints || fp < 0 || fp >= mNumFPoints)
@@ -781,9 +848,19 @@ void DomainBase::finalize() {
             if (hPointNr[sp] != hFPointNr[fp])
                 throw Error("SolidFluidCoupling::checkCompatibility || "
                             "Incompatible sizes.");
-            if (cSolidPair[sp] >= 0 || cFluidPair[fp] >= 0)
+            if (sSeen[sp] || fSeen[fp])
                 throw Error("SolidFluidCoupling::checkCompatibility || Point "
                             "coupled twice.");
+            sSeen[sp] = fSeen[fp] = 1;
+            if (pair3d[k] >= 0) {
+                cSolidPair3D[sp] = pair3d[k];
+                fPair3D[fp] = pair3d[k];
+                continue;
+            }
+            if (fMass3DOff[fp] >= 0)
+                throw Error("SolidFluidCoupling::checkCompatibility || A "
+                            "fluid point with a 3D mass needs a 3D "
+                            "coupling.");
             cSolidPair[sp] = k;
             cFluidPair[fp] = k;
             cSOff[k] = off[sp];
@@ -793,6 +870,39 @@ void DomainBase::finalize() {
             cN[2 * k + 1] = nn[1];
             cNM[2 * k] = nn[2] / hFMass[fp];
             cNM[2 * k + 1] = nn[3] / hFMass[fp];
+        }
+        // the deferred fluid points, ascending
+        dfFlag.assign(mNumFPoints, 0);
+        for (int p = 0; p < mNumFPoints; p++)
+            if (fMass3DOff[p] >= 0 || fPair3D[p] >= 0) {
+                dfFlag[p] = 1;
+                dfPts.push_back(p);
+                dfPair.push_back(fPair3D[p]);
+                dfMOff.push_back(fMass3DOff[p]);
+            }
+        mNumFDefer = (int)dfPts.size();
+        // 3D pairs: slice and mode offsets, assembled normals over the
+        // fluid mass per slice (SolidFluidCoupling3D constructor,
+        // SolidFluidCoupling3D.cpp:15-24)
+        hCpl3DNam.assign(hCpl3DNa.size(), 0.);
+        int64_t sl = 0;
+        for (int j = 0; j < mNumCpl3D; j++) {
+            const int k = hCpl3DPairs[j];
+            const int sp = hCplSolid[k], fp = hCplFluid[k];
+            const int nr = hPointNr[sp];
+            cpl3Solid.push_back(sp);
+            cpl3NOff.push_back(sl);
+            cpl3COff.push_back(cpl3Modes);
+            for (int s = 0; s < nr; s++) {
+                const double mf = fMass3DOff[fp] >= 0
+                                      ? hFMass3D[fMass3DOff[fp] + s]
+                                      : hFMass[fp];
+                for (int c = 0; c < 3; c++)
+                    hCpl3DNam[(sl + s) * 3 + c] =
+                        (1. / mf) * hCpl3DNa[(sl + s) * 3 + c];
+            }
+            sl += nr;
+            cpl3Modes += (uint32_t)(nr / 2 + 1);
         }
     } else if (mNumPairs > 0) {
         throw Error("Domain::finalize || Coupling without fluid elements.");
@@ -886,6 +996,20 @@ void DomainBase::finalize() {
         dChiA.zero(st);
         dCStiff.alloc(fb);
         dCStiff.zero(st);
+        if (mNumFDefer > 0) {
+            up(dFDeferPts, dfPts.data(), dfPts.size() * 4);
+            up(dFDeferPair, dfPair.data(), dfPair.size() * 4);
+            up(dFDeferMOff, dfMOff.data(), dfMOff.size() * 8);
+            up(dFDeferFlag, dfFlag.data(), dfFlag.size());
+        }
+        if (mNumCpl3D > 0) {
+            up(dCpl3DSolid, cpl3Solid.data(), cpl3Solid.size() * 4);
+            up(dCpl3DNOff, cpl3NOff.data(), cpl3NOff.size() * 8);
+            up(dCpl3DCOff, cpl3COff.data(), cpl3COff.size() * 4);
+            up(dCplSolidPair3D, cSolidPair3D.data(), cSolidPair3D.size() * 4);
+            dCSolid3D.alloc((size_t)cpl3Modes * 6 * sizeof(double));
+            dCSolid3D.zero(st);
+        }
         if (mNumPairs > 0) {
             up(dCplSolidPair, cSolidPair.data(), cSolidPair.size() * 4);
             up(dCplFluidPair, cFluidPair.data(), cFluidPair.size() * 4);
@@ -1249,8 +1373,41 @@ public:
         g.point_nr = dFPointNr.as<int32_t>();
         g.point_axial = dFPointBC.as<uint8_t>();
         g.inv_mass = dFInvMass.as<double>();
+        g.mass3d = mNumFDefer > 0 ? dFDeferFlag.as<uint8_t>() : nullptr;
         g.adj = dFAdj.as<uint32_t>();
         return g;
+    }
+    FluidDeferArgs<Real> fluidDeferArgs() const {
+        FluidDeferArgs<Real> f;
+        f.n = mNumFDefer;
+        f.point = dFDeferPts.as<int32_t>();
+        f.pair3d = dFDeferPair.as<int32_t>();
+        f.minv_off = dFDeferMOff.as<int64_t>();
+        f.minv = dFInvMass3D.as<Real>();
+        f.inv_mass = dFInvMass.as<double>();
+        f.point_off = dFPointOff.as<uint32_t>();
+        f.point_nr = dFPointNr.as<int32_t>();
+        f.point_bc = dFPointBC.as<uint8_t>();
+        f.fstiff = dFStiff.as<Real>();
+        f.chi = dChi.as<double>();
+        f.chi_v = dChiV.as<double>();
+        f.chi_a = dChiA.as<double>();
+        f.dt = mDt;
+        f.flag = dFlag.as<int>();
+        f.pair_solid = dCpl3DSolid.as<int32_t>();
+        f.n_off = dCpl3DNOff.as<int64_t>();
+        f.nu = dCpl3DNu.as<Real>();
+        f.nam = dCpl3DNam.as<Real>();
+        f.solid_off = dPointOff.as<uint32_t>();
+        f.solid_displ = dDispl.as<double>();
+        f.csolid3d = dCSolid3D.as<double>();
+        f.c3off = dCpl3DCOff.as<uint32_t>();
+        f.twiddle = dTwiddle.as<cpx<Real>>();
+        f.tw_off = dTwOff.as<int64_t>();
+        f.plan = dPlan.as<uint32_t>();
+        f.slice_pos = dPosTab.as<uint16_t>();
+        f.pos_off = dPosOff.as<int64_t>();
+        return f;
     }
     CouplingGeom couplingGeom() const {
         CouplingGeom c;
@@ -1262,6 +1419,11 @@ public:
         c.n = dCplN.as<double>();
         c.n_m = dCplNM.as<double>();
         c.cstiff = dCStiff.as<double>();
+        if (mNumCpl3D > 0) {
+            c.solid_pair3d = dCplSolidPair3D.as<int32_t>();
+            c.c3off = dCpl3DCOff.as<uint32_t>();
+            c.csolid3d = dCSolid3D.as<double>();
+        }
         return c;
     }
     FluidArgs<Real> fluidArgs() const {
@@ -1403,6 +1565,11 @@ public:
                                  dChiV.as<double>(), dChiA.as<double>(), mDt,
                                  dFlag.as<int>(), couplingGeom(),
                                  dDispl.as<double>());
+        if (mNumFDefer > 0) {
+            int ph = phases & ~PHASE_GATHER;
+            if (mNumPairs <= 0) ph &= ~PHASE_COUPLE;
+            launch_fluid_defer<Real>(mStream, ph, fluidDeferArgs());
+        }
         tock(AXS3D_KERNEL_FLUID_POINT);
     }
     // SolidFluidBoundary::apply (Domain.cpp:313-318)
@@ -1820,6 +1987,22 @@ private:
                 inv[k] = (Real)(1.0 / hMass3D[k]);
             dInvMass3D.alloc(inv.size() * sizeof(Real));
             dInvMass3D.upload(inv.data(), inv.size() * sizeof(Real), st);
+        }
+        if (!hFMass3D.empty()) {
+            std::vector<Real> inv(hFMass3D.size());
+            for (size_t k = 0; k < inv.size(); k++)
+                inv[k] = (Real)(1.0 / hFMass3D[k]);
+            dFInvMass3D.alloc(inv.size() * sizeof(Real));
+            dFInvMass3D.upload(inv.data(), inv.size() * sizeof(Real), st);
+        }
+        if (mNumCpl3D > 0) {
+            // SolidFluidCoupling3D keeps both normal sets in Real
+            std::vector<Real> nu(hCpl3DNu.begin(), hCpl3DNu.end());
+            std::vector<Real> nam(hCpl3DNam.begin(), hCpl3DNam.end());
+            dCpl3DNu.alloc(nu.size() * sizeof(Real));
+            dCpl3DNu.upload(nu.data(), nu.size() * sizeof(Real), st);
+            dCpl3DNam.alloc(nam.size() * sizeof(Real));
+            dCpl3DNam.upload(nam.data(), nam.size() * sizeof(Real), st);
         }
         if (mNum3D > 0) {
             int ncu = 0;

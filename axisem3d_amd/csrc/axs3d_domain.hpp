@@ -80,6 +80,9 @@ public:
     void setFluidSurfacePoints(int n, const int32_t *points);
     void setSolidFluidCoupling(int n, const int32_t *solid,
                                const int32_t *fluid, const double *normals);
+    void setFluidMass3D(int n, const int32_t *points, const double *mass);
+    void setCoupling3D(int n, const int32_t *pairs, const double *n_u,
+                       const double *n_a);
     void setAttenuation(int nsls, int cg4, const double *alpha,
                         const double *beta, const double *gamma);
     void setAttenuation3D(int e_begin, int e_end, const double *dlam_dmu);
@@ -238,6 +241,14 @@ protected:
     std::vector<int64_t> hMass3DOff;
     std::vector<double> hMass3D;
     int mNumMass3D = 0;
+    // fluid points with a 3D mass; 3D solid-fluid pairs (normals [Nr][3]
+    // per pair, unassembled and assembled); the deferred fluid points
+    std::vector<int32_t> hFMass3DPts;
+    std::vector<int64_t> hFMass3DOff;
+    std::vector<double> hFMass3D;
+    std::vector<int32_t> hCpl3DPairs;
+    std::vector<double> hCpl3DNu, hCpl3DNa, hCpl3DNam;
+    int mNumFDefer = 0, mNumCpl3D = 0;
     std::vector<CardBatch> mCardBatches;
     int mCardOcc = CARD_OCC;            // stress kernel waves per SIMD
     double mCardTuneMs = 0.;            // one cardinal pass at finalize
@@ -281,6 +292,9 @@ protected:
         dFElemK, dFEMList, dFEBOff, dFEB, dChi, dChiV, dChiA, dFStiff;
     DeviceBuffer dCplSolidPair, dCplFluidPair, dCplSOff, dCplFOff, dCplN,
         dCplNM, dCStiff;
+    DeviceBuffer dFDeferPts, dFDeferPair, dFDeferMOff, dFDeferFlag,
+        dFInvMass3D, dCpl3DSolid, dCpl3DNOff, dCpl3DNu, dCpl3DNam,
+        dCpl3DCOff, dCplSolidPair3D, dCSolid3D;
     // device: source and receivers
     DeviceBuffer dSrcPattern, dStf, dRecElem, dRecW, dRecPhi, dSeismo;
     DeviceBuffer dPointDt;

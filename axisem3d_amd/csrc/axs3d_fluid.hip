@@ -160,6 +160,7 @@ struct FluidPointArgs {
     int *flag;
     CouplingGeom cpl;
     const double *solid_displ;
+    const uint8_t *defer;          // [point] 1: finished by k_fluid_defer
 };
 
 // Fluid point update, one thread per (point, order), phases as for the
@@ -198,6 +199,13 @@ __global__ __launch_bounds__(256) void k_fluid_point(
             a.cpl.cstiff[2 * (size_t)k] = s[0];
             a.cpl.cstiff[2 * (size_t)k + 1] = s[1];
         }
+    }
+    if (a.defer != nullptr && a.defer[p]) {
+        // a 3D mass or a 3D coupling: the remaining phases follow in
+        // k_fluid_defer, from the stiffness gathered so far
+        stiff[(size_t)k * 2] = (Real)s[0];
+        stiff[(size_t)k * 2 + 1] = (Real)s[1];
+        return;
     }
     if (AXIAL) {
         const uint8_t bc = a.point_bc[p];
@@ -274,6 +282,7 @@ void launch_fluid_point(hipStream_t st, int phases, const PointGeom &g,
     a.flag = flag;
     a.cpl = cpl;
     a.solid_displ = solid_displ;
+    a.defer = g.mass3d;
     if (cpl.n_pairs <= 0) phases &= ~PHASE_COUPLE;
 #define AXS3D_FPOINT_CASE(G, A, M, T, C)                                      \
     case (A) | ((M) << 1) | ((T) << 2) | ((G) << 3) | ((C) << 4):            \

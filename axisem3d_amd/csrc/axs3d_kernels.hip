@@ -345,6 +345,17 @@ __global__ __launch_bounds__(256) void k_point(PointArgs a,
             s[1] = (Real)((double)s[1] - ns * f_im);
             s[4] = (Real)((double)s[4] - nz * f_re);
             s[5] = (Real)((double)s[5] - nz * f_im);
+        } else if (a.cpl.solid_pair3d != nullptr) {
+            // SolidFluidCoupling3D::coupleFluidToSolid, computed with the
+            // deferred fluid points
+            const int j = a.cpl.solid_pair3d[p];
+            if (j >= 0) {
+                const double *c = a.cpl.csolid3d +
+                                  ((size_t)a.cpl.c3off[j] + alpha) * 6;
+#pragma unroll
+                for (int q = 0; q < 6; q++)
+                    s[q] = (Real)((double)s[q] - c[q]);
+            }
         }
     }
     if (AXIAL && a.point_axial[p]) {

@@ -29,6 +29,11 @@ struct CouplingGeom {
     const double *n_m;             // [pair][2] ns / m_fluid, nz / m_fluid
     double *cstiff;                // [fluid mode] fluid stiffness after
                                    // the solid -> fluid term
+    // 3D pairs (SolidFluidCoupling3D): the fluid -> solid term of each,
+    // computed with the fluid points that wait for it (k_fluid_defer)
+    const int32_t *solid_pair3d = nullptr;  // [solid point] 3D pair or -1
+    const uint32_t *c3off = nullptr;        // [3D pair] mode offset
+    const double *csolid3d = nullptr;       // [mode][3] complex
 };
 
 // Fluid elements (acoustic, 1D), element-modes of the Fourier path.
@@ -181,6 +186,47 @@ struct Mass3DArgs {
     double dt;
     int *flag;
 };
+
+// Fluid points whose update needs transforms over the azimuth: a 3D mass
+// (Mass3D::computeAccel for one component, Mass3D.cpp:41-53) and/or a 3D
+// coupling (SolidFluidCoupling3D.cpp:55-94).  The fluid point kernel
+// leaves their gathered stiffness in fstiff; k_fluid_defer does their
+// phases (coupling, axis/surface, mass, march) and the fluid -> solid term
+// of their 3D pairs, which the solid point kernel subtracts.
+template <typename Real>
+struct FluidDeferArgs {
+    int n;                         // deferred fluid points
+    const int32_t *point;          // [i] fluid point
+    const int32_t *pair3d;         // [i] 3D pair or -1
+    const int64_t *minv_off;       // [i] offset of Nr inverse masses or -1
+    const Real *minv;              // 1 / m per slice (3D mass)
+    const double *inv_mass;        // [fluid point] 1 / m (1D mass)
+    const uint32_t *point_off;     // fluid
+    const int32_t *point_nr;
+    const uint8_t *point_bc;       // bit 0 axis, bit 1 free surface
+    Real *fstiff;
+    double *chi, *chi_v, *chi_a;
+    double dt;
+    int *flag;
+    // 3D pairs
+    const int32_t *pair_solid;     // [j] solid point
+    const int64_t *n_off;          // [j] offset of its [Nr][3] normals
+    const Real *nu;                // unassembled normals (solid -> fluid)
+    const Real *nam;               // assembled normals / m_fluid per slice
+    const uint32_t *solid_off;     // solid point_off
+    const double *solid_displ;
+    double *csolid3d;              // [mode][3] complex, at c3off[j]
+    const uint32_t *c3off;
+    // FFT tables
+    const cpx<Real> *twiddle;
+    const int64_t *tw_off;
+    const uint32_t *plan;
+    const uint16_t *slice_pos;
+    const int64_t *pos_off;
+};
+template <typename Real>
+void launch_fluid_defer(hipStream_t st, int phases,
+                        const FluidDeferArgs<Real> &a);
 
 // Mass3D::computeAccel of the 3D-mass points from the assembled stiffness
 // (written there by the point kernel): with march, the Newmark update of

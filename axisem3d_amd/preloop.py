@@ -1378,3 +1378,66 @@ def batch_instances(specs):
                       for i in range(len(specs))]
     out.name = f"batch_{len(specs)}"
     return out
+
+
+# ------------------------------------------- 3D fluid mass and coupling
+def _phi_series(rng, nr, orders, width):
+    """Seeded low-order Fourier series in phi at the nr slices, (nr, width),
+    scaled to max |d| = 1."""
+    phi = 2.0 * np.pi * np.arange(nr) / nr
+    d = np.zeros((nr, width))
+    for n in range(1, orders + 1):
+        a, b = rng.uniform(-1.0, 1.0, (2, width))
+        d += a[None, :] * np.cos(n * phi)[:, None] + \
+            b[None, :] * np.sin(n * phi)[:, None]
+    return d / max(np.abs(d).max(), 1e-300)
+
+
+def make_fluid_mass3d(spec, points, amp=0.2, orders=3, seed=0):
+    """3D mass of the given fluid points, m_1d (1 + d(phi)) per slice with
+    d a seeded low-order Fourier series (|d| <= amp): the fluid mass IF /
+    kappa of a 3D bulk modulus (GLLPoint.cpp:78-112 -> Mass3D).  Sets
+    spec.fluid_mass3d_points and spec.fluid_mass3d."""
+    rng = np.random.default_rng(seed)
+    points = np.asarray(points, dtype=np.int32)
+    out = []
+    for p in points:
+        nr = int(spec.fluid_nr[p])
+        d = _phi_series(rng, nr, orders, 1)[:, 0]
+        out.append(spec.fluid_mass[p] * (1.0 + amp * d))
+    spec.fluid_mass3d_points = points
+    spec.fluid_mass3d = np.concatenate(out) if out else np.zeros(0)
+    return spec
+
+
+def make_coupling3d(spec, pairs=None, amp=0.1, tilt=0.1, orders=2,
+                    seed=0):
+    """3D solid-fluid coupling: per slice normals (s, phi, z) of an
+    undulated interface, the 1D normal scaled by (1 + d(phi)) with a phi
+    component tilt |n| d'(phi) (seeded series, |d| <= amp).  Every pair
+    whose fluid point has a 3D mass becomes 3D as well (GLLPoint.cpp:
+    129-143).  Sets spec.sf3d_pairs (ascending) and spec.sf3d_normal
+    ([Nr][3] per 3D pair, in pair order)."""
+    rng = np.random.default_rng(seed)
+    sel = set(range(len(spec.sf_solid)) if pairs is None else
+              (int(k) for k in pairs))
+    fm3 = getattr(spec, "fluid_mass3d_points", None)
+    if fm3 is not None:
+        f3 = set(int(p) for p in fm3)
+        sel |= {k for k in range(len(spec.sf_fluid))
+                if int(spec.sf_fluid[k]) in f3}
+    sel = sorted(sel)
+    out = []
+    for k in sel:
+        nr = int(spec.solid_nr[spec.sf_solid[k]])
+        ns, nz = spec.sf_normal[k]
+        mag = math.hypot(ns, nz)
+        d = _phi_series(rng, nr, orders, 3)
+        n3 = np.stack([ns * (1.0 + amp * d[:, 0]),
+                       tilt * mag * d[:, 1],
+                       nz * (1.0 + amp * d[:, 2])], axis=1)
+        out.append(n3)
+    spec.sf3d_pairs = np.asarray(sel, dtype=np.int32)
+    spec.sf3d_normal = (np.concatenate(out).reshape(-1) if out
+                        else np.zeros(0))
+    return spec

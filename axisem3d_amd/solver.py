@@ -148,6 +148,21 @@ class GpuSolver:
             self._chk(L.axs3d_set_solid_fluid_coupling(
                 ctx, len(cs), cs.ctypes.data, cf.ctypes.data, nn.ctypes.data),
                 "set_coupling")
+            fm3 = getattr(s, "fluid_mass3d_points", None)
+            if fm3 is not None and len(fm3) > 0:
+                fm3 = _c(fm3, np.int32)
+                fmv = _c(s.fluid_mass3d, np.float64)
+                self._chk(L.axs3d_set_fluid_mass3d(
+                    ctx, len(fm3), fm3.ctypes.data, fmv.ctypes.data),
+                    "set_fluid_mass3d")
+            c3 = getattr(s, "sf3d_pairs", None)
+            if c3 is not None and len(c3) > 0:
+                # one rank: the unassembled and assembled normals coincide
+                c3 = _c(c3, np.int32)
+                n3 = _c(s.sf3d_normal, np.float64)
+                self._chk(L.axs3d_set_coupling3d(
+                    ctx, len(c3), c3.ctypes.data, n3.ctypes.data,
+                    n3.ctypes.data), "set_coupling3d")
         axp = _c(s.axial_points, np.int32)
         self._chk(L.axs3d_set_axial_points(ctx, len(axp), axp.ctypes.data),
                   "set_axial")

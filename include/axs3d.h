@@ -171,6 +171,22 @@ int axs3d_set_solid_fluid_coupling(axs3d_ctx *ctx, int n,
                                    const int32_t *solid_points,
                                    const int32_t *fluid_points,
                                    const double *normals);
+/* Fluid points with a 3D mass: points and their Nr masses per slice, in
+ * order.  Replaces Mass3D for fluid points (src/core/point/mass/
+ * Mass3D.cpp:41-53, built by GLLPoint::release, GLLPoint.cpp:78-112).  A
+ * coupled fluid point with a 3D mass needs its pair in
+ * axs3d_set_coupling3d. */
+int axs3d_set_fluid_mass3d(axs3d_ctx *ctx, int n, const int32_t *points,
+                           const double *mass);
+/* 3D solid-fluid coupling of the listed pairs (indices into the pairs of
+ * axs3d_set_solid_fluid_coupling): normals (s, phi, z) per slice of the
+ * solid point's Nr, unassembled and assembled, [Nr][3] per pair in order;
+ * the fluid mass per slice is the fluid point's (1D or 3D).  Replaces the
+ * SolidFluidCoupling3D constructor (src/core/boundary/solid_fluid/
+ * SolidFluidCoupling3D.cpp:15-24, GLLPoint.cpp:129-143). */
+int axs3d_set_coupling3d(axs3d_ctx *ctx, int n, const int32_t *pairs,
+                         const double *n_unassembled,
+                         const double *n_assembled);
 
 /* Number of fluid point-modes (length of a fluid field / 2). */
 int axs3d_num_fluid_modes(const axs3d_ctx *ctx, int64_t *nmodes);

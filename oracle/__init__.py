@@ -64,6 +64,8 @@ class OracleDomain(C.Structure):
         ("sf_normal", _p),
         ("mass3d_off", _p), ("mass3d", _p),
         ("felem_kind", _p), ("fk3d", _p), ("fk3d_off", _p),
+        ("fmass3d_off", _p), ("fmass3d", _p),
+        ("sf3d_off", _p), ("sf3d_normal", _p),
     ]
 
 
@@ -354,6 +356,28 @@ class Oracle:
             d.mass3d_off = self._k(off)
             d.mass3d = self._k(np.ascontiguousarray(spec.mass3d,
                                                     dtype=np.float64))
+        fm3 = getattr(spec, "fluid_mass3d_points", None)
+        if fm3 is not None and len(fm3) > 0:
+            off = np.full(len(spec.fluid_nr), -1, dtype=np.int64)
+            nr = np.asarray(spec.fluid_nr, dtype=np.int64)
+            cum = 0
+            for p in np.asarray(fm3, dtype=np.int64):
+                off[p] = cum
+                cum += nr[p]
+            d.fmass3d_off = self._k(off)
+            d.fmass3d = self._k(np.ascontiguousarray(spec.fluid_mass3d,
+                                                     dtype=np.float64))
+        c3 = getattr(spec, "sf3d_pairs", None)
+        if c3 is not None and len(c3) > 0:
+            off = np.full(len(spec.sf_solid), -1, dtype=np.int64)
+            nr = np.asarray(spec.solid_nr, dtype=np.int64)
+            cum = 0
+            for k in np.asarray(c3, dtype=np.int64):
+                off[k] = cum
+                cum += nr[spec.sf_solid[k]]
+            d.sf3d_off = self._k(off)
+            d.sf3d_normal = self._k(np.ascontiguousarray(
+                spec.sf3d_normal, dtype=np.float64))
         d.dt = float(spec.dt)
         self.d = d
         self._att_state = None
@@ -454,6 +478,11 @@ class Oracle:
         stiff = np.ascontiguousarray(stiff, dtype=np.complex128).copy()
         lib().oracle_stiff_to_accel(C.byref(self.d), stiff.ctypes.data)
         return stiff
+
+    def fluid_stiff_to_accel(self, fstiff):
+        fstiff = np.ascontiguousarray(fstiff, dtype=np.complex128).copy()
+        lib().oracle_fluid_stiff_to_accel(C.byref(self.d), fstiff.ctypes.data)
+        return fstiff
 
     def newmark(self, stiff, displ, veloc, accel):
         """Returns updated (stiff, displ, veloc, accel) copies."""

@@ -928,9 +928,53 @@ void oracle_fluid_element_stiff(const oracle_domain *d, int e,
  * pair and in this order: fluid stiff += ns u_s + nz u_z; solid stiff
  * (s, z) -= (n / m_fluid) fluid stiff.  One rank: the unassembled and
  * assembled normals coincide. */
+/* SolidFluidCoupling3D (SolidFluidCoupling3D.cpp:55-94), per slice:
+ * fluid stiff += R2C(n . C2R(u_s)); solid stiff -= R2C(C2R(fluid stiff)
+ * n / m_fluid), with m_fluid the fluid point's mass per slice (1D or 3D,
+ * GLLPoint.cpp:129-143). */
+static void coupling3d(const oracle_domain *d, int k, const cplx *displ,
+                       cplx *stiff, cplx *fstiff) {
+    int sp = d->sf_solid[k], fp = d->sf_fluid[k];
+    int nr = d->point_nr[sp];
+    int nu1 = nr / 2 + 1;
+    const double *nn = d->sf3d_normal + d->sf3d_off[k] * 3;
+    const double *m3 = (d->fmass3d_off && d->fmass3d_off[fp] >= 0)
+                           ? d->fmass3d + d->fmass3d_off[fp] : NULL;
+    const cplx *u = displ + d->offsets[sp] * 3;
+    cplx *s = stiff + d->offsets[sp] * 3;
+    cplx *f = fstiff + d->foffsets[fp];
+    double *r = malloc(sizeof(double) * nr * 4);
+    double *fr = r + 3 * nr;
+    cplx *col = malloc(sizeof(cplx) * nu1);
+    for (int c = 0; c < 3; c++) {
+        for (int a = 0; a < nu1; a++) col[a] = u[a * 3 + c];
+        oracle_c2r(col, nr, r + c * nr);
+    }
+    for (int sl = 0; sl < nr; sl++)
+        fr[sl] = nn[sl * 3] * r[sl] + nn[sl * 3 + 1] * r[nr + sl] +
+                 nn[sl * 3 + 2] * r[2 * nr + sl];
+    oracle_r2c(fr, nr, col);
+    for (int a = 0; a < nu1; a++) f[a] += col[a];
+    oracle_c2r(f, nr, fr);
+    for (int c = 0; c < 3; c++) {
+        for (int sl = 0; sl < nr; sl++) {
+            double mf = m3 ? m3[sl] : d->fpoint_mass[fp];
+            r[sl] = fr[sl] * ((1. / mf) * nn[sl * 3 + c]);
+        }
+        oracle_r2c(r, nr, col);
+        for (int a = 0; a < nu1; a++) s[a * 3 + c] -= col[a];
+    }
+    free(r);
+    free(col);
+}
+
 void oracle_coupling(const oracle_domain *d, const cplx *displ, cplx *stiff,
                      cplx *fstiff) {
     for (int k = 0; k < d->n_sf; k++) {
+        if (d->sf3d_off && d->sf3d_off[k] >= 0) {
+            coupling3d(d, k, displ, stiff, fstiff);
+            continue;
+        }
         int sp = d->sf_solid[k], fp = d->sf_fluid[k];
         int nu1 = d->point_nr[sp] / 2 + 1;
         double ns = d->sf_normal[2 * k], nz = d->sf_normal[2 * k + 1];
@@ -967,13 +1011,23 @@ void oracle_fluid_bc(const oracle_domain *d, cplx *fstiff) {
 }
 
 /* FluidPoint::computeStiffToAccel (FluidPoint.cpp:28-53): Nyquist
- * imaginary part zero (even Nr), divide by the mass (Mass1D). */
+ * imaginary part zero (even Nr), divide by the mass: Mass1D, or Mass3D
+ * for one component (Mass3D.cpp:41-53: C2R, times 1/m per slice, R2C). */
 void oracle_fluid_stiff_to_accel(const oracle_domain *d, cplx *fstiff) {
     for (int ip = 0; ip < d->n_fpoints; ip++) {
         int nr = d->fpoint_nr[ip];
         int nu1 = nr / 2 + 1;
         cplx *s = fstiff + d->foffsets[ip];
         if (nr % 2 == 0) s[nu1 - 1] = creal(s[nu1 - 1]);
+        if (d->fmass3d_off && d->fmass3d_off[ip] >= 0) {
+            const double *m = d->fmass3d + d->fmass3d_off[ip];
+            double *r = malloc(sizeof(double) * nr);
+            oracle_c2r(s, nr, r);
+            for (int sl = 0; sl < nr; sl++) r[sl] *= 1. / m[sl];
+            oracle_r2c(r, nr, s);
+            free(r);
+            continue;
+        }
         double inv = 1. / d->fpoint_mass[ip];
         for (int k = 0; k < nu1; k++) s[k] *= inv;
     }

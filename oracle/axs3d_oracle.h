@@ -88,6 +88,14 @@ typedef struct {
     const uint8_t *felem_kind;
     const double *fk3d;
     const int64_t *fk3d_off;
+    /* 3D fluid mass (NULL: all 1D): per fluid point the offset of its Nr
+     * masses in fmass3d, or -1 */
+    const int64_t *fmass3d_off;
+    const double *fmass3d;
+    /* 3D solid-fluid coupling (NULL: all 1D): per pair the offset of its
+     * Nr normals (s, phi, z) in sf3d_normal, or -1 for the 1D normal */
+    const int64_t *sf3d_off;
+    const double *sf3d_normal;
 } oracle_domain;
 
 void oracle_set_gmat(const double *g_gll, const double *g_glj);

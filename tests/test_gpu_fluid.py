@@ -147,3 +147,61 @@ def test_fluid3d_seismograms_match_oracle(prec, tol):
     print(f"fluid 3D {prec}: relative seismogram L2 = {err:.3e}")
     assert err < tol
     assert _rel(chi, o.fdispl) < 10 * tol
+
+
+def _spec_c3d(layer, nsteps=300):
+    # fluid 3D mass on every third fluid point (coupled ones among them),
+    # all pairs 3D (undulated interface normals with a phi component)
+    sp = _spec(layer, nsteps)
+    pts = np.unique(np.concatenate([np.arange(0, len(sp.fluid_nr), 3),
+                                    sp.sf_fluid[::2]]))
+    preloop.make_fluid_mass3d(sp, pts, amp=0.2, seed=7)
+    return preloop.make_coupling3d(sp, amp=0.15, tilt=0.15, seed=8)
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-11), ("fp32", 1e-5)])
+@pytest.mark.parametrize("layer", ["middle", "ocean"])
+def test_coupling3d_and_fluid_mass3d_stages_match_oracle(prec, tol, layer):
+    spec = _spec_c3d(layer)
+    assert len(spec.sf3d_pairs) == len(spec.sf_solid)
+    o = oracle.Oracle(spec)
+    g = _solver(spec, prec)
+    rng = np.random.default_rng(51)
+    u = rng.standard_normal((o.nmodes, 3)) + 1j * rng.standard_normal(
+        (o.nmodes, 3))
+    ks = rng.standard_normal((o.nmodes, 3)) + 1j * rng.standard_normal(
+        (o.nmodes, 3))
+    kf = rng.standard_normal(o.fnmodes) + 1j * rng.standard_normal(o.fnmodes)
+    # coupling (applyBC_BeforeAssemblingStiff)
+    g.set_field("displ", u)
+    g.set_field("stiff", ks)
+    g.set_field("fstiff", kf)
+    g.applyBC_BeforeAssemblingStiff()
+    ws, wf = o.coupling(u, ks, kf)
+    assert _rel(g.get_field("fstiff"), wf) < tol
+    assert _rel(g.get_field("stiff"), ws) < tol
+    # stiffness to acceleration (fluid Mass3D)
+    g.set_field("stiff", ks)
+    g.set_field("fstiff", kf)
+    g.computeStiffToAccel()
+    assert _rel(g.get_field("fstiff"), o.fluid_stiff_to_accel(kf)) < tol
+    assert _rel(g.get_field("stiff"), o.stiff_to_accel(ks)) < tol
+    g.close()
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("fp32", 1e-4)])
+@pytest.mark.parametrize("layer", ["middle", "ocean"])
+def test_coupling3d_seismograms_match_oracle(prec, tol, layer):
+    nsteps = 300
+    spec = _spec_c3d(layer, nsteps)
+    o = oracle.Oracle(spec)
+    ref = o.run(nsteps)
+    g = _solver(spec, prec)
+    g.run(nsteps)
+    got = g.seismograms()
+    chi = g.get_field("fdispl")
+    g.close()
+    err = _rel(got, ref)
+    print(f"3D coupling {layer} {prec}: relative seismogram L2 = {err:.3e}")
+    assert err < tol
+    assert _rel(chi, o.fdispl) < 10 * tol

@@ -767,3 +767,139 @@ def test_fluid3d_rotation_equivariance():
     sp.fluid_k3d = np.roll(k3, -1, axis=1).reshape(-1)
     bad = oracle.Oracle(sp).fluid_stiffness(rot * chi)
     assert np.abs(bad - rot * base).max() > 1e-3 * np.abs(base).max()
+
+
+def _wdot_modes(a, b, nr):
+    w = np.concatenate([np.r_[1.0, np.full(n // 2, 2.0)] for n in nr])
+    return float(np.sum(w * np.real(np.conj(a) * b)))
+
+
+def _real_mean(x, offsets):
+    x = x.copy()
+    x[offsets[:-1]] = x[offsets[:-1]].real
+    return x
+
+
+def test_fluid_mass3d_constant_equals_mass1d_and_matches_numpy():
+    sp = _fluid_layer_spec(nsteps=1)
+    pts = np.arange(0, len(sp.fluid_nr), 3)
+    rng = np.random.default_rng(12)
+    o1 = oracle.Oracle(sp)
+    s = rng.standard_normal(o1.fnmodes) + 1j * rng.standard_normal(
+        o1.fnmodes)
+    ref1 = o1.fluid_stiff_to_accel(s)
+    # constant 3D mass: Mass1D except the imaginary part of the mean
+    sp.fluid_mass3d_points = pts
+    sp.fluid_mass3d = np.concatenate([np.full(int(sp.fluid_nr[p]),
+                                              sp.fluid_mass[p]) for p in pts])
+    got = oracle.Oracle(sp).fluid_stiff_to_accel(s)
+    ref1[o1.foffsets[pts]] = ref1[o1.foffsets[pts]].real
+    assert np.abs(got - ref1).max() <= 1e-12 * np.abs(ref1).max()
+    # varying 3D mass against numpy: synthesis, division, rfft / Nr
+    preloop.make_fluid_mass3d(sp, pts, amp=0.3, seed=4)
+    got = oracle.Oracle(sp).fluid_stiff_to_accel(s)
+    moff = 0
+    for p in pts:
+        nr = int(sp.fluid_nr[p])
+        nu1 = nr // 2 + 1
+        m = sp.fluid_mass3d[moff:moff + nr]
+        moff += nr
+        c = s[o1.foffsets[p]:o1.foffsets[p + 1]].copy()
+        phi = 2 * np.pi * np.arange(nr) / nr
+        r = np.full(nr, c[0].real)
+        for a in range(1, nu1):
+            r += 2 * np.real(c[a] * np.exp(1j * a * phi))
+        ref = np.fft.rfft(r / m)[:nu1] / nr
+        np.testing.assert_allclose(got[o1.foffsets[p]:o1.foffsets[p + 1]],
+                                   ref, rtol=0,
+                                   atol=1e-12 * np.abs(ref).max())
+
+
+def _coupling3d_spec(amp=0.2, tilt=0.2):
+    sp = _fluid_layer_spec(nsteps=1)
+    return preloop.make_coupling3d(sp, amp=amp, tilt=tilt, seed=6)
+
+
+def test_coupling3d_constant_normal_equals_coupling1d():
+    """3D normals equal to the 1D ones in every slice (no phi component)
+    couple like the 1D pairs on fields with a real mean (Nr = 5: no
+    Nyquist).  A normal table read with slice and component swapped
+    fails."""
+    sp = _coupling3d_spec()
+    o1 = oracle.Oracle(sp)
+    rng = np.random.default_rng(13)
+    u = rng.standard_normal((o1.nmodes, 3)) + 1j * rng.standard_normal(
+        (o1.nmodes, 3))
+    u = _real_mean(u, o1.offsets)
+    c = _real_mean(rng.standard_normal(o1.fnmodes) +
+                   1j * rng.standard_normal(o1.fnmodes), o1.foffsets)
+    s0 = rng.standard_normal((o1.nmodes, 3)) + 0j
+    s0 = _real_mean(s0, o1.offsets)
+    sp1 = _drop_coupling3d(sp)
+    ws, wf = oracle.Oracle(sp1).coupling(u, s0, c)
+    nr = sp.solid_nr[sp.sf_solid]
+    sp.sf3d_normal = np.concatenate([
+        np.tile(np.r_[sp.sf_normal[k][0], 0.0, sp.sf_normal[k][1]],
+                (int(nr[k]), 1)) for k in sp.sf3d_pairs]).reshape(-1)
+    gs, gf = oracle.Oracle(sp).coupling(u, s0, c)
+    assert np.abs(gf - wf).max() <= 1e-12 * np.abs(wf).max()
+    assert np.abs(gs - ws).max() <= 1e-12 * np.abs(ws).max()
+
+
+def _drop_coupling3d(spec):
+    import copy
+    s2 = copy.copy(spec)
+    s2.sf3d_pairs = None
+    s2.sf3d_normal = None
+    return s2
+
+
+def test_coupling3d_terms_are_adjoint():
+    """With per-slice normals n(phi): <c, T u> = <S(m c), u> in the
+    real-field inner product, T u = R2C(n . C2R u) the solid -> fluid
+    term and S f = R2C(C2R(f) n / m) the fluid -> solid one (1D fluid
+    mass).  A sign, a missing 1 / m or a phi component applied on one side
+    only fails."""
+    sp = _coupling3d_spec()
+    o = oracle.Oracle(sp)
+    rng = np.random.default_rng(21)
+    u = _real_mean(rng.standard_normal((o.nmodes, 3)) +
+                   1j * rng.standard_normal((o.nmodes, 3)), o.offsets)
+    c = _real_mean(rng.standard_normal(o.fnmodes) +
+                   1j * rng.standard_normal(o.fnmodes), o.foffsets)
+    mf = np.repeat(sp.fluid_mass, sp.fluid_nr // 2 + 1)
+    zs = np.zeros_like(u)
+    _, tu = o.coupling(u, zs, np.zeros_like(c))
+    smc, _ = o.coupling(zs, zs, mf * c)
+    lhs = _wdot_modes(c, tu, sp.fluid_nr)
+    rhs = -sum(_wdot_modes(smc[:, k], u[:, k], sp.solid_nr)
+               for k in range(3))
+    assert abs(lhs - rhs) < 1e-12 * abs(lhs)
+
+
+def test_coupling3d_rotation_equivariance():
+    """Normals rotated by one slice and fields rotated by 2 pi / Nr rotate
+    both coupling terms (slice k at phi = 2 pi k / Nr); the reversed
+    azimuth does not."""
+    sp = _coupling3d_spec()
+    nr = 5
+    assert np.all(sp.solid_nr[sp.sf_solid] == nr)
+    o = oracle.Oracle(sp)
+    rng = np.random.default_rng(23)
+    u = _real_mean(rng.standard_normal((o.nmodes, 3)) +
+                   1j * rng.standard_normal((o.nmodes, 3)), o.offsets)
+    c = _real_mean(rng.standard_normal(o.fnmodes) +
+                   1j * rng.standard_normal(o.fnmodes), o.foffsets)
+    a_s = np.concatenate([np.arange(n // 2 + 1) for n in sp.solid_nr])
+    a_f = np.concatenate([np.arange(n // 2 + 1) for n in sp.fluid_nr])
+    rs = np.exp(-1j * a_s * 2 * np.pi / nr)[:, None]
+    rf = np.exp(-1j * a_f * 2 * np.pi / nr)
+    zs = np.zeros_like(u)
+    bs, bf = o.coupling(u, zs, c)
+    n3 = sp.sf3d_normal.reshape(-1, nr, 3)
+    for shift, ok in ((1, True), (-1, False)):
+        sp.sf3d_normal = np.roll(n3, shift, axis=1).reshape(-1)
+        gs, gf = oracle.Oracle(sp).coupling(rs * u, zs, rf * c)
+        close = (np.abs(gf - rf * bf).max() <= 1e-11 * np.abs(bf).max() and
+                 np.abs(gs - rs * bs).max() <= 1e-11 * np.abs(bs).max())
+        assert close == ok
