@@ -421,7 +421,7 @@ __global__ __launch_bounds__(CD_EMB_THREADS) void k_cd_grad(CardArgs<Real> a,
     if (t < 2 * NPEM) sG[t / NPEM][t % NPEM] = s.gtab[t];
     const int eml = t % CD_EMB;
     const int ipnt = t < CD_EMB * NPEM ? t / CD_EMB : 0;
-    const int em = em0 + blockIdx.x * CD_EMB + eml;
+    const int em = em0 + xcd_block() * CD_EMB + eml;
     const bool active = t < CD_EMB * NPEM && em < em1;
     const int i = ipnt / NPED, j = ipnt - (ipnt / NPED) * NPED;
     int e = 0, alpha = 0;

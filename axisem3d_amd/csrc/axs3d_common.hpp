@@ -61,6 +61,20 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Workgroups go to the 8 XCDs of the chip in turn (workgroup b to XCD
+// b mod 8), and each XCD has its own L2.  This maps the workgroup index to
+// a work index such that every XCD takes one contiguous range of the work
+// (a bijection of [0, gridDim.x)): consecutive element-modes of one
+// element and their neighbours, which share GLL points, meet in one L2.
+__device__ __forceinline__ int xcd_block() {
+    constexpr int NXCD = 8;
+    const int b = (int)blockIdx.x;
+    const int n = (int)gridDim.x;
+    const int q = n / NXCD, rem = n - q * NXCD;
+    const int x = b % NXCD, r = b / NXCD;
+    return x * q + (x < rem ? x : rem) + r;
+}
+
 // Element-mode work item: element index and Fourier order alpha.
 struct EMItem {
     int32_t elem;

@@ -34,7 +34,7 @@ __global__ __launch_bounds__(STIFF_BLOCK) void k_fluid_stiff_fr(
     if (t < 2 * NPEM) sG[t / NPEM][t % NPEM] = a.gtab[t];
     const int eml = t / NPEM;
     const int ipnt = t - eml * NPEM;
-    const int em = blockIdx.x * EMB + eml;
+    const int em = xcd_block() * EMB + eml;
     const bool active = (eml < EMB) && (em < a.n_em);
     const int i = ipnt / NPED, j = ipnt - (ipnt / NPED) * NPED;
     const int emx = active ? eml : 0;

@@ -49,7 +49,7 @@ __global__ __launch_bounds__(STIFF_BLOCK) void k_solid_stiff_fr(
     if (t < 2 * NPEM) sG[t / NPEM][t % NPEM] = a.gtab[t];
     const int eml = t / NPEM;
     const int ipnt = t - eml * NPEM;
-    const int em = em0 + blockIdx.x * EMB + eml;
+    const int em = em0 + xcd_block() * EMB + eml;
     const bool active = (eml < EMB) && (em < em1);
     const int i = ipnt / NPED, j = ipnt - (ipnt / NPED) * NPED;
     const int emx = active ? eml : 0;
