@@ -22,9 +22,12 @@ import numpy as np
 from .solver import GpuSolver
 
 
-def with_model(spec, rho, vp, vs):
-    """A copy of spec (1D isotropic solid) with lambda, mu and the point
-    masses of the model (rho, vp, vs) at every element GLL point."""
+def with_model(spec, rho, vp, vs, vp3d=None, vs3d=None):
+    """A copy of spec (isotropic solid) with lambda, mu and the point
+    masses of the model (rho, vp, vs) at every element GLL point; 3D
+    elements take their velocities per slice from vp3d, vs3d (the layout
+    of spec.prop3d without the parameter axis, [slice][25]) with the
+    density of their GLL point."""
     rho, vp, vs = (np.asarray(x, dtype=np.float64) for x in (rho, vp, vs))
     ne = len(spec.elem_points)
     for x in (rho, vp, vs):
@@ -40,7 +43,27 @@ def with_model(spec, rho, vp, vs):
               (rho * ifac).reshape(-1))
     sp.solid_mass = mass
     sp._rho1d = rho
+    if vp3d is not None:
+        r3 = _slice_rho(spec, rho)
+        vp3d, vs3d = (np.asarray(x, dtype=np.float64) for x in (vp3d, vs3d))
+        sp.prop3d = np.ascontiguousarray(np.stack(
+            [r3 * (vp3d ** 2 - 2.0 * vs3d ** 2), r3 * vs3d ** 2], axis=1))
     return sp
+
+
+def _slice_rho(spec, rho):
+    """rho of each slice's GLL points, [slice][25] (3D elements)."""
+    kind = np.asarray(spec.elem_kind)
+    nr = np.asarray(spec.elem_nr)
+    return np.concatenate([np.repeat(rho[e][None, :], int(nr[e]), axis=0)
+                           for e in range(len(kind)) if kind[e] == 1])
+
+
+def model3d_of(spec):
+    """(vp3d, vs3d) per slice of the 3D elements, [slice][25]."""
+    r3 = _slice_rho(spec, np.asarray(spec._rho1d, dtype=np.float64))
+    lam, mu = spec.prop3d[:, 0, :], spec.prop3d[:, 1, :]
+    return np.sqrt((lam + 2.0 * mu) / r3), np.sqrt(mu / r3)
 
 
 def model_of(spec):
@@ -81,10 +104,52 @@ def gradient(spec, rho, vp, vs, djdr, precision="fp64",
     g = GpuSolver(with_model(spec, rho, vp, vs), precision,
                   max_samples=len(djdr))
     try:
-        kern = g.adjoint_run(djdr, checkpoint_interval)
+        kern, _ = g.adjoint_run(djdr, checkpoint_interval)
     finally:
         g.close()
     return chain_rule(kern, np.asarray(rho), np.asarray(vp), np.asarray(vs))
+
+
+def chain_rule3d(spec, kern, kern3d, rho, vp, vs, vp3d, vs3d):
+    """dJ/d(rho, vp, vs) per element GLL point and dJ/d(vp3d, vs3d) per
+    slice, from the adjoint's dJ/d(lambda, mu, rho) (1D elements and the
+    mass) and dJ/d(lambda, mu) per slice (3D elements), with lambda = rho
+    (vp^2 - 2 vs^2), mu = rho vs^2 slice by slice."""
+    d_rho, d_vp, d_vs = chain_rule(kern, rho, vp, vs)
+    r3 = _slice_rho(spec, rho)
+    kl, km = kern3d[:, 0, :], kern3d[:, 1, :]
+    d_vp3 = 2.0 * r3 * vp3d * kl
+    d_vs3 = 2.0 * r3 * vs3d * (km - 2.0 * kl)
+    per_slice = (vp3d ** 2 - 2.0 * vs3d ** 2) * kl + vs3d ** 2 * km
+    kind = np.asarray(spec.elem_kind)
+    nr = np.asarray(spec.elem_nr)
+    off = 0
+    for e in range(len(kind)):
+        if kind[e] == 1:
+            d_rho[e] += per_slice[off:off + int(nr[e])].sum(axis=0)
+            off += int(nr[e])
+    return d_rho, d_vp, d_vs, d_vp3, d_vs3
+
+
+def forward_seismograms3d(spec, model, nsteps, precision="fp64"):
+    """Seismograms of the model (rho, vp, vs, vp3d, vs3d) from rest."""
+    g = GpuSolver(with_model(spec, *model), precision, max_samples=nsteps)
+    try:
+        g.run(nsteps)
+        return g.seismograms(nsteps)
+    finally:
+        g.close()
+
+
+def gradient3d(spec, model, djdr, precision="fp64", checkpoint_interval=None):
+    """dJ/d of the model (rho, vp, vs, vp3d, vs3d) of a domain with 3D
+    isotropic elements, from the adjoint run."""
+    g = GpuSolver(with_model(spec, *model), precision, max_samples=len(djdr))
+    try:
+        kern, kern3d = g.adjoint_run(djdr, checkpoint_interval)
+    finally:
+        g.close()
+    return chain_rule3d(spec, kern, kern3d, *model)
 
 
 def seismogram_function():

@@ -51,6 +51,8 @@ __global__ __launch_bounds__(STIFF_BLOCK) void k_sensitivity(SensArgs a) {
     const bool axial = a.elem_axial[e] != 0;
     const bool axial_row0 = axial && (i == 0);
     const int nu1e = a.elem_nu1[e];
+    // 3D elements: only the mass density here (k_cd_sens does lambda, mu)
+    const bool kind3d = a.elem_kind != nullptr && a.elem_kind[e] == 1;
     const int p = lane_ok ? a.elem_points[e * NPEM + ipnt] : 0;
     const int nu1p = lane_ok ? a.point_nu1[p] : 0;
     const size_t poff = lane_ok ? a.point_off[p] : 0;
@@ -101,11 +103,13 @@ __global__ __launch_bounds__(STIFF_BLOCK) void k_sensitivity(SensArgs a) {
                                 j, G, g, ep);
             const cpx<double> tu = cadd(cadd(eu[0], eu[1]), eu[2]);
             const cpx<double> tp = cadd(cadd(ep[0], ep[1]), ep[2]);
-            kl += w * rdot(tp, tu);
-            km += w * (2. * (rdot(ep[0], eu[0]) + rdot(ep[1], eu[1]) +
-                             rdot(ep[2], eu[2])) +
-                       rdot(ep[3], eu[3]) + rdot(ep[4], eu[4]) +
-                       rdot(ep[5], eu[5]));
+            if (!kind3d) {
+                kl += w * rdot(tp, tu);
+                km += w * (2. * (rdot(ep[0], eu[0]) + rdot(ep[1], eu[1]) +
+                                 rdot(ep[2], eu[2])) +
+                           rdot(ep[3], eu[3]) + rdot(ep[4], eu[4]) +
+                           rdot(ep[5], eu[5]));
+            }
             kr += w * (rdot(ps[0], ac[0]) + rdot(ps[1], ac[1]) +
                        rdot(ps[2], ac[2]));
         }

@@ -312,10 +312,11 @@ int axs3d_set_graph(axs3d_ctx *ctx, int enable) {
 }
 
 int axs3d_adjoint_run(axs3d_ctx *ctx, int nsteps, int checkpoint_interval,
-                      const double *djdr, double *kernels) {
+                      const double *djdr, double *kernels,
+                      double *kernels3d) {
     return guarded(ctx, [&](axs3d::DomainBase &d) {
         if (!kernels) throw axs3d::Error("Domain::adjoint || Null output.");
-        d.adjointRun(nsteps, checkpoint_interval, djdr, kernels);
+        d.adjointRun(nsteps, checkpoint_interval, djdr, kernels, kernels3d);
     });
 }
 

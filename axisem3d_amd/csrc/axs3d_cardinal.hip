@@ -1259,6 +1259,89 @@ void launch_fluid_defer(hipStream_t st, int phases,
 #undef AXS3D_DEFER
 }
 
+// Adjoint sensitivity of 3D isotropic elements per slice (one block of
+// three waves per GLL point of each element of the batch): the strain
+// rows of psi (scratch, from k_cd_grad on psi) and of u (scrb, from
+// k_cd_grad on u) to the slices by C2R, then with IF the integral factor
+// and Nr slices
+//   kern3d[slice][0][k] += scale IF / Nr tr e_psi tr e_u
+//   kern3d[slice][1][k] += scale IF / Nr (2 sum_ii e_psi,ii e_u,ii +
+//                                          sum_shear g_psi g_u)
+// (Parseval: the Fourier-space pairing of strain and stress equals the
+// mean over the slices of their product), in the layout of the host's
+// per-slice lambda, mu ([slice][2][25], slices at slice_off[e]).
+template <typename Real>
+__global__ __launch_bounds__(CD_PT) void k_cd_sens(CardArgs<Real> a,
+                                                   const Real *scrb, int w0,
+                                                   const double *ifac,
+                                                   double *kern3d,
+                                                   double scale) {
+    __shared__ cpx<Real> tw[CARD_TW_CAP];
+    __shared__ cpx<Real> bpsi[CD_PW][CD_BUF];
+    __shared__ cpx<Real> bu[CD_PW][CD_BUF];
+    __shared__ uint16_t pos[CARD_NR_CAP];
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int q = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int w = w0 + (int)blockIdx.x / NPEM;
+    const int pp = (int)blockIdx.x % NPEM;
+    const int e = a.list3d[w];
+    const int nu1 = a.s.elem_nu1[e];
+    const int nr = a.elem_nr[e];
+    const uint32_t *plan = a.plan + (size_t)nr * CARD_PLAN_W;
+    {
+        const cpx<Real> *twg = a.twiddle + a.tw_off[nr];
+        const int ntw = (int)(plan[0] >> 8);
+        for (int x = t; x < ntw; x += CD_PT) tw[x] = twg[x];
+        const uint16_t *pg = a.slice_pos + a.pos_off[nr];
+        for (int x = t; x < nr; x += CD_PT) pos[x] = pg[x];
+    }
+    __syncthreads();
+    const bool even = (nr & 1) == 0;
+    const size_t rs = (size_t)NPEM * nu1;
+    const cpx<Real> *ra = reinterpret_cast<const cpx<Real> *>(a.scratch) +
+                          a.scr_off[e] + (size_t)pp * nu1;
+    const cpx<Real> *rb = reinterpret_cast<const cpx<Real> *>(scrb) +
+                          a.scr_off[e] + (size_t)pp * nu1;
+    pack_pair<Real>(bpsi[q], ra + 2 * q * rs, ra + (2 * q + 1) * rs, nr, nu1,
+                    even, lane);
+    wfft<Real>(bpsi[q], nr, plan, true, tw, lane);
+    pack_pair<Real>(bu[q], rb + 2 * q * rs, rb + (2 * q + 1) * rs, nr, nu1,
+                    even, lane);
+    wfft<Real>(bu[q], nr, plan, true, tw, lane);
+    __syncthreads();
+    const double f = scale * ifac[(size_t)e * 6 * NPEM + 5 * NPEM + pp] /
+                     (double)nr;
+    double *K = kern3d + (size_t)a.slice_off[e] * 2 * NPEM;
+    for (int k = t; k < nr; k += CD_PT) {
+        const int pk = pad(pos[k]);
+        double ep[6], eu[6];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            ep[2 * c] = (double)bpsi[c][pk].re;
+            ep[2 * c + 1] = (double)bpsi[c][pk].im;
+            eu[2 * c] = (double)bu[c][pk].re;
+            eu[2 * c + 1] = (double)bu[c][pk].im;
+        }
+        const double tp = ep[0] + ep[1] + ep[2], tu = eu[0] + eu[1] + eu[2];
+        const double dm = 2. * (ep[0] * eu[0] + ep[1] * eu[1] +
+                                ep[2] * eu[2]) +
+                          ep[3] * eu[3] + ep[4] * eu[4] + ep[5] * eu[5];
+        K[((size_t)k * 2) * NPEM + pp] += f * tp * tu;
+        K[((size_t)k * 2 + 1) * NPEM + pp] += f * dm;
+    }
+}
+
+template <typename Real>
+void launch_cd_sens(hipStream_t st, const CardArgs<Real> &a, const Real *scrb,
+                    const CardBatch &bt, const double *ifac, double *kern3d,
+                    double scale) {
+    const int n = bt.end - bt.begin;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_cd_sens<Real>, dim3(n * NPEM), dim3(CD_PT), 0, st,
+                       a, scrb, bt.begin, ifac, kern3d, scale);
+}
+
 template <typename Real>
 int card_blocks_per_cu(bool att) {
     int n = 0;
@@ -1289,6 +1372,12 @@ template void launch_cdf_batch<float>(hipStream_t,
 template void launch_cdf_batch<double>(hipStream_t,
                                        const FluidCardArgs<double> &,
                                        const CardBatch &);
+template void launch_cd_sens<float>(hipStream_t, const CardArgs<float> &,
+                                    const float *, const CardBatch &,
+                                    const double *, double *, double);
+template void launch_cd_sens<double>(hipStream_t, const CardArgs<double> &,
+                                     const double *, const CardBatch &,
+                                     const double *, double *, double);
 template void launch_fluid_defer<float>(hipStream_t, int,
                                         const FluidDeferArgs<float> &);
 template void launch_fluid_defer<double>(hipStream_t, int,

@@ -1123,11 +1123,12 @@ void DomainBase::adjointCheck() const {
     requireFinalized("Domain::adjoint");
     bool iso = true;
     for (uint8_t t : hMType) iso = iso && t == 0;
-    if (mNum3D > 0 || mNsls > 0 || mNumFElems > 0 || mNumMass3D > 0 ||
-        !iso || !hInstDt.empty())
+    if (mNsls > 0 || mNumFElems > 0 || mNumMass3D > 0 || !iso ||
+        !hInstDt.empty())
         throw Error("Domain::adjoint || The adjoint run supports solid "
-                    "elements with a 1D isotropic elastic medium and no "
-                    "attenuation, fluid, 3D mass or batched instances.");
+                    "elements with an isotropic elastic medium (1D or 3D) "
+                    "and no attenuation, fluid, 3D mass or batched "
+                    "instances.");
 }
 
 void DomainBase::adjointOperands() {
@@ -1141,6 +1142,10 @@ void DomainBase::adjointOperands() {
     }
     dAdjGTab.alloc(gt.size() * sizeof(double));
     dAdjGTab.upload(gt.data(), gt.size() * sizeof(double), mStream);
+    if (mNum3D > 0) {
+        dAdjKind.alloc(hElemKind.size());
+        dAdjKind.upload(hElemKind.data(), hElemKind.size(), mStream);
+    }
 }
 
 SensArgs DomainBase::sensArgs(const double *psi, const double *u,
@@ -1160,6 +1165,7 @@ SensArgs DomainBase::sensArgs(const double *psi, const double *u,
     s.a = a;
     s.kern = kern;
     s.scale = scale;
+    s.elem_kind = mNum3D > 0 ? dAdjKind.as<uint8_t>() : nullptr;
     return s;
 }
 
@@ -1617,9 +1623,12 @@ public:
     // (the same fields, swapped in) steps backwards through the segment:
     // densities of (psi_n, u_n, a_n) scaled by -dt^2 into the kernels,
     // then one step with the adjoint force instead of the source.
-    void adjointRun(int nsteps, int seg, const double *djdr,
-                    double *kern) override {
+    void adjointRun(int nsteps, int seg, const double *djdr, double *kern,
+                    double *kern3d) override {
         adjointCheck();
+        if (mNum3D > 0 && !kern3d)
+            throw Error("Domain::adjoint || A domain with 3D elements needs "
+                        "the per-slice output.");
         if (nsteps <= 0 || seg <= 0)
             throw Error("Domain::adjoint || The number of steps and the "
                         "checkpoint interval must be positive.");
@@ -1632,7 +1641,10 @@ public:
         const int seglen = std::min(seg, nsteps);
         size_t freeb = 0, totalb = 0;
         hip_check(hipMemGetInfo(&freeb, &totalb), "Domain::adjoint");
-        const double need = (double)nf * (3.0 * nck + 2.0 * seglen + 3.0);
+        const double need =
+            (double)nf * (3.0 * nck + 2.0 * seglen + 3.0) +
+            (double)mCardScrMax * sizeof(cpx<Real>) +
+            (double)mNumSlices * 2 * NPEM * sizeof(double);
         if (need > 0.9 * (double)freeb) {
             std::ostringstream ss;
             ss << "Domain::adjoint || Checkpoints and segment fields need "
@@ -1653,6 +1665,12 @@ public:
         const size_t nk = (size_t)mNumElems * NPEM * 3;
         dAdjKern.alloc(nk * sizeof(double));
         dAdjKern.zero(mStream);
+        const size_t nk3 = (size_t)mNumSlices * 2 * NPEM;
+        if (mNum3D > 0) {
+            dAdjKern3D.alloc(nk3 * sizeof(double));
+            dAdjKern3D.zero(mStream);
+            dAdjScrB.alloc(mCardScrMax * sizeof(cpx<Real>));
+        }
         const int all = PHASE_GATHER | PHASE_AXIAL | PHASE_MASS | PHASE_MARCH;
         double *ck = dAdjCkpt.as<double>();
         auto d2d = [&](double *dst, const double *src) {
@@ -1697,6 +1715,21 @@ public:
                                       su + (size_t)(n - n0) * nd,
                                       sa + (size_t)(n - n0) * nd,
                                       dAdjKern.as<double>(), -dt2));
+                if (mNum3D > 0) {
+                    // per-slice densities: strain rows of psi and u_n,
+                    // batch by batch
+                    const CardArgs<Real> ca = cardArgs();
+                    CardArgs<Real> cb = ca;
+                    cb.s.displ = su + (size_t)(n - n0) * nd;
+                    cb.scratch = dAdjScrB.as<Real>();
+                    for (const CardBatch &bt : mCardBatches) {
+                        launch_cd_grad<Real>(mStream, ca, bt);
+                        launch_cd_grad<Real>(mStream, cb, bt);
+                        launch_cd_sens<Real>(mStream, ca, cb.scratch, bt,
+                                             dAdjGQ.as<double>(),
+                                             dAdjKern3D.as<double>(), -dt2);
+                    }
+                }
                 elementKernel();
                 launch_adjoint_source<Real>(
                     mStream, mNumRec, dRecElem.as<int32_t>(),
@@ -1713,12 +1746,18 @@ public:
                                  nk * sizeof(double), hipMemcpyDeviceToHost,
                                  mStream),
                   "Domain::adjoint");
+        if (mNum3D > 0)
+            hip_check(hipMemcpyAsync(kern3d, dAdjKern3D.as<double>(),
+                                     nk3 * sizeof(double),
+                                     hipMemcpyDeviceToHost, mStream),
+                      "Domain::adjoint");
         hip_check(hipStreamSynchronize(mStream), "Domain::adjoint");
         checkStability(nsteps);
         // the segment buffers are the largest; give them back
         dAdjCkpt.reset();
         dAdjSegU.reset();
         dAdjSegA.reset();
+        dAdjScrB.reset();
     }
 
     void getField(int which, double *out) override {

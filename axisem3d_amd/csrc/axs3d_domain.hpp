@@ -131,7 +131,7 @@ public:
     // each segment's forward fields from its checkpoint; kern[e][k][3] =
     // dJ/d(lambda, mu, rho) at GLL point k of element e
     virtual void adjointRun(int nsteps, int seg, const double *djdr,
-                            double *kern) = 0;
+                            double *kern, double *kern3d) = 0;
     // sensitivity densities of one (psi, u, a) triple, FP64, kern[e][k][3]
     void adjointDensity(const double *psi, const double *u, const double *a,
                         double *kern);
@@ -303,7 +303,7 @@ protected:
     // adjoint: FP64 operators, densities, adjoint fields, checkpoints and
     // the forward fields of one segment
     DeviceBuffer dAdjGQ, dAdjGTab, dAdjKern, dAdjU, dAdjV, dAdjA, dAdjCkpt,
-        dAdjSegU, dAdjSegA, dAdjG;
+        dAdjSegU, dAdjSegA, dAdjG, dAdjKind, dAdjKern3D, dAdjScrB;
     int mRecNu1Max = 0;
     void adjointCheck() const;
     void adjointOperands();

@@ -320,8 +320,16 @@ struct SensArgs {
     const double *psi, *u, *a;     // [mode][3] complex
     double *kern;
     double scale;
+    const uint8_t *elem_kind;      // 1: 3D element (lambda, mu per slice
+                                   // elsewhere), or null
 };
 void launch_sensitivity(hipStream_t st, const SensArgs &a);
+// per-slice lambda, mu densities of the 3D elements of one batch, from the
+// strain rows of psi (a.scratch) and of u (scrb); see k_cd_sens
+template <typename Real>
+void launch_cd_sens(hipStream_t st, const CardArgs<Real> &a, const Real *scrb,
+                    const CardBatch &bt, const double *ifac, double *kern3d,
+                    double scale);
 
 // adjoint force R^* g * scale into the receivers' element-local forces
 template <typename Real>

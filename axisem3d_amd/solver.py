@@ -292,17 +292,23 @@ class GpuSolver:
         """Discrete adjoint of run() from rest over len(djdr) steps
         (axs3d_adjoint_run): djdr[n, r, c] = dJ / d seismogram[n, r, c].
         Returns dJ/d(lambda, mu, rho) per element GLL point, shape
-        (n_elements, 25, 3).  The wavefields are left undefined."""
+        (n_elements, 25, 3), and, with 3D elements, dJ/d(lambda, mu) per
+        slice in the layout of spec.prop3d (else None).  The wavefields
+        are left undefined."""
         g = _c(djdr, np.float64)
         nsteps = g.shape[0]
         if checkpoint_interval is None:
             checkpoint_interval = max(1, int(round(np.sqrt(nsteps))))
         kern = np.zeros((len(self.spec.elem_points), 25, 3))
-        self._chk(self.lib.axs3d_adjoint_run(self.ctx, int(nsteps),
-                                             int(checkpoint_interval),
-                                             g.ctypes.data, kern.ctypes.data),
-                  "adjoint_run")
-        return kern
+        kind = getattr(self.spec, "elem_kind", None)
+        k3 = None
+        if kind is not None and np.any(np.asarray(kind) == 1):
+            k3 = np.zeros(np.asarray(self.spec.prop3d).shape)
+        self._chk(self.lib.axs3d_adjoint_run(
+            self.ctx, int(nsteps), int(checkpoint_interval), g.ctypes.data,
+            kern.ctypes.data, None if k3 is None else k3.ctypes.data),
+            "adjoint_run")
+        return kern, k3
 
     def adjoint_density(self, psi, u, a):
         """Unscaled per-step sensitivity densities (axs3d_adjoint_density)

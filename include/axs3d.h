@@ -338,9 +338,9 @@ int axs3d_set_graph(axs3d_ctx *ctx, int enable);
 
 /* ---------------- adjoint --------------------------------------------- */
 /* No counterpart in the reference (it has no adjoint solver): the
- * discrete adjoint of axs3d_run for solid domains whose elements are 1D
- * isotropic elastic without attenuation (other domains: error).  Runs the
- * forward problem from rest over steps [0, nsteps), keeping (u, v, a)
+ * discrete adjoint of axs3d_run for solid domains of isotropic elastic
+ * elements, 1D or 3D, without attenuation (other domains: error).  Runs
+ * the forward problem from rest over steps [0, nsteps), keeping (u, v, a)
  * every checkpoint_interval steps on the device, then the adjoint field
  * backwards in time with the force R^* djdr[n] / dt^2 (djdr:
  * [nsteps][n_receivers][3] FP64, the derivative of the misfit with respect
@@ -348,10 +348,16 @@ int axs3d_set_graph(axs3d_ctx *ctx, int enable);
  * out), recomputing each interval's forward fields from its checkpoint.
  * kernels ([n_elements][25][3] FP64) receives dJ/dlambda, dJ/dmu, dJ/drho
  * at every GLL point of every element (rho through the assembled mass
- * sum rho IF).  Leaves the wavefields undefined (a later axs3d_run must
- * start from fields set to rest) and the seismograms untouched. */
+ * sum rho IF; lambda and mu zero for 3D elements); kernels3d ([slice][2]
+ * [25] FP64, the layout of axs3d_set_solid_iso3d; NULL without 3D
+ * elements) dJ/dlambda and dJ/dmu per slice of the 3D elements.  Exact for
+ * odd element Nr; an even Nr's Nyquist row enters the transforms with
+ * weights that make the stiffness only nearly self-adjoint.  Leaves the
+ * wavefields undefined (a later axs3d_run must start from fields set to
+ * rest) and the seismograms untouched. */
 int axs3d_adjoint_run(axs3d_ctx *ctx, int nsteps, int checkpoint_interval,
-                      const double *djdr, double *kernels);
+                      const double *djdr, double *kernels,
+                      double *kernels3d);
 /* The per-step densities of axs3d_adjoint_run, unscaled, for one adjoint
  * field psi, forward displacement u and forward acceleration a (each laid
  * out as axs3d_get_field): kernels[e][k] = (<psi, dK/dlambda u>,

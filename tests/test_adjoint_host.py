@@ -50,3 +50,35 @@ def test_chain_rule_matches_finite_differences():
         fd = (J(*mp) - J(*mm)) / (2 * h)
         ad = np.sum(d[which] * dirn)
         assert abs(fd - ad) < 1e-6 * abs(ad), which
+
+
+def test_chain_rule3d_matches_finite_differences():
+    """Per-slice lambda, mu of 3D elements from (rho per GLL point, vp3d,
+    vs3d per slice): chain_rule3d against central differences of a linear
+    functional of (lambda, mu per element GLL point, lambda, mu per slice,
+    rho)."""
+    sp = _spec()
+    E = sp.n_solid_elements
+    preloop.make_3d_isotropic(sp, lambda x, y, z: 0.0 * x,
+                              mask3d=(np.arange(E) % 2) == 0)
+    model = list(adjoint.model_of(sp)) + list(adjoint.model3d_of(sp))
+    rng = np.random.default_rng(5)
+    kern = rng.standard_normal((E, 25, 3))
+    kern3d = rng.standard_normal(sp.prop3d.shape)
+
+    def J(m):
+        s = adjoint.with_model(sp, *m)
+        return (np.sum(kern[..., 0] * s.elem_iso[:, 0, :]) +
+                np.sum(kern[..., 1] * s.elem_iso[:, 1, :]) +
+                np.sum(kern[..., 2] * m[0]) + np.sum(kern3d * s.prop3d))
+    d = adjoint.chain_rule3d(sp, kern, kern3d, *model)
+    for which in range(5):
+        dirn = rng.standard_normal(model[which].shape)
+        h = 1e-6 * np.abs(model[which]).mean()
+        mp = [x.copy() for x in model]
+        mm = [x.copy() for x in model]
+        mp[which] += h * dirn
+        mm[which] -= h * dirn
+        fd = (J(mp) - J(mm)) / (2 * h)
+        ad = np.sum(d[which] * dirn)
+        assert abs(fd - ad) < 1e-6 * abs(ad), which

@@ -144,3 +144,70 @@ def test_adjoint_rejects_unsupported_domains():
     with pytest.raises(capi.Axs3dError, match="adjoint"):
         g.adjoint_run(np.zeros((10, 2, 3)))
     g.close()
+
+
+def _spec3d(nsteps=NSTEPS):
+    # odd element Nr (5) everywhere, every other element 3D isotropic
+    sp = preloop.cartesian_homogeneous(
+        nx=6, nz=7, width=6e3, depth=5.6e3, nsteps=nsteps, src_depth=2e3,
+        n_receivers=4, seed=2, nodal_nr=lambda s, z: np.full(s.shape, 5.0))
+    E = sp.n_solid_elements
+    preloop.make_3d_isotropic(sp, lambda x, y, z: 0.0 * x,
+                              mask3d=(np.arange(E) % 2) == 0)
+    return sp
+
+
+def _smooth3d(spec, rng, amp):
+    """A smooth seeded relative perturbation per slice of the 3D
+    elements, [slice][25], from the slices' Cartesian coordinates."""
+    sz = spec._gq["sz"]
+    out = []
+    for e in np.nonzero(np.asarray(spec.elem_kind) == 1)[0]:
+        nr = int(spec.elem_nr[e])
+        phi = 2 * np.pi * np.arange(nr) / nr
+        s, z = sz[e, 0], sz[e, 1]
+        out.append((s[None, :] * np.cos(phi)[:, None],
+                    s[None, :] * np.sin(phi)[:, None],
+                    np.broadcast_to(z[None, :], (nr, 25))))
+    x = np.concatenate([o[0] for o in out])
+    y = np.concatenate([o[1] for o in out])
+    z = np.concatenate([o[2] for o in out])
+    d = np.zeros_like(x)
+    for _ in range(4):
+        k = rng.uniform(-1.0, 1.0, 3) * 2 * np.pi / 3e3
+        d += np.cos(k[0] * x + k[1] * y + k[2] * z + rng.uniform(0, 6.3))
+    return amp * d / np.abs(d).max()
+
+
+def test_adjoint3d_gradient_matches_finite_differences():
+    """Mixed 1D/3D isotropic domain: the gradient with respect to rho, vp,
+    vs per GLL point and vp, vs per slice of the 3D elements against
+    central differences of forward runs along 8 seeded directions
+    (relative 1e-3)."""
+    sp = _spec3d()
+    model = list(adjoint.model_of(sp)) + list(adjoint.model3d_of(sp))
+    rng = np.random.default_rng(19)
+    true = list(model)
+    true[4] = model[4] * (1 + _smooth3d(sp, rng, 0.02))
+    true[2] = model[2] * (1 + _smooth(sp, rng, 0.02))
+    d_obs = adjoint.forward_seismograms3d(sp, true, NSTEPS)
+    r = adjoint.forward_seismograms3d(sp, model, NSTEPS)
+    grads = adjoint.gradient3d(sp, model, r - d_obs)
+    assert np.abs(grads[3]).max() > 0 and np.abs(grads[4]).max() > 0
+
+    def misfit(m):
+        return 0.5 * np.sum((adjoint.forward_seismograms3d(sp, m, NSTEPS) -
+                             d_obs) ** 2)
+    worst = 0.0
+    for trial in range(8):
+        dirs = [model[i] * (_smooth(sp, rng, 1.0) if i < 3 else
+                            _smooth3d(sp, rng, 1.0)) for i in range(5)]
+        h = 1e-4
+        fd = (misfit([m + h * d for m, d in zip(model, dirs)]) -
+              misfit([m - h * d for m, d in zip(model, dirs)])) / (2 * h)
+        ad = sum(float(np.sum(g * d)) for g, d in zip(grads, dirs))
+        err = abs(fd - ad) / abs(fd)
+        print(f"3D direction {trial}: FD {fd:.9e} adjoint {ad:.9e} "
+              f"relative difference {err:.2e}")
+        worst = max(worst, err)
+    assert worst < 1e-3
