@@ -45,7 +45,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="regional",
-                    choices=["regional", "cartesian8000", "batched"])
+                    choices=["regional", "cartesian8000", "batched",
+                             "adjoint"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
@@ -319,6 +320,59 @@ def batched_extras(spec, seis32, seis64, steps, warmup, t_batch):
     return out
 
 
+def adjoint_bench(a):
+    """The adjoint run on the 8000-element Cartesian workload (1D
+    isotropic, Nr 5) with half the elements 3D isotropic: a forward run of
+    `steps` steps from rest against axs3d_adjoint_run over the same steps
+    (forward with checkpoints, recomputation, adjoint, sensitivities), FP32
+    stiffness with FP64 accumulation."""
+    from axisem3d_amd import adjoint, preloop
+    from axisem3d_amd.solver import GpuSolver
+    nsteps = a.steps
+    spec = preloop.cartesian_homogeneous(nsteps=nsteps)
+    E = spec.n_solid_elements
+    preloop.make_3d_isotropic(spec, lambda x, y, z: 0.0 * x,
+                              mask3d=(np.arange(E) % 2) == 0)
+    nupd = spec.n_updates_per_step()
+    model = list(adjoint.model_of(spec)) + list(adjoint.model3d_of(spec))
+    sp = adjoint.with_model(spec, *model)
+    g = GpuSolver(sp, "fp32", max_samples=nsteps)
+    g.run(nsteps)          # warm-up of the forward (graph capture)
+    g.synchronize()
+    g.close()
+    g = GpuSolver(sp, "fp32", max_samples=nsteps)
+    t0 = time.perf_counter()
+    g.run(nsteps)
+    g.synchronize()
+    t_fwd = time.perf_counter() - t0
+    seis = g.seismograms(nsteps)
+    interval = max(1, int(round(np.sqrt(nsteps))))
+    t0 = time.perf_counter()
+    kern, k3 = g.adjoint_run(seis, interval)
+    t_adj = time.perf_counter() - t0
+    g.close()
+    log(f"adjoint: forward {t_fwd:.3f} s, adjoint run {t_adj:.3f} s")
+    out = {
+        "metric": "GLL-point*mode updates/s of the adjoint run (forward "
+                  "with checkpoints, recomputation, adjoint field, "
+                  "sensitivities), per forward step",
+        "value": nupd * nsteps / t_adj, "unit": UNIT, "n_gpus": 1,
+        "steps": nsteps, "warmup": 0, "ms_per_step": t_adj / nsteps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": "cartesian box 80x80 km, 8000 elements, nu=2 "
+                   "(Nr 5), isotropic, every other element 3D, axial "
+                   "moment tensor, 10 receivers; misfit J = sum(s^2)/2",
+                   "checkpoint_interval": interval,
+                   "updates_per_step": nupd},
+        "forward_ms_per_step": t_fwd / nsteps * 1e3,
+        "adjoint_over_forward": t_adj / t_fwd,
+        "kernel_norms": {"lambda_mu_rho": float(np.linalg.norm(kern)),
+                         "lambda_mu_per_slice": float(np.linalg.norm(k3))},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -328,6 +382,9 @@ def main():
         sys.exit(2)
     if a.copy_calibration > 0:
         copy_calibration(a.copy_calibration)
+        return
+    if a.workload == "adjoint":
+        adjoint_bench(a)
         return
     from axisem3d_amd import capi
     # warm-up, timed steps, one capture step and the graph replays
