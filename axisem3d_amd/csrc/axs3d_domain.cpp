@@ -1123,12 +1123,11 @@ void DomainBase::adjointCheck() const {
     requireFinalized("Domain::adjoint");
     bool iso = true;
     for (uint8_t t : hMType) iso = iso && t == 0;
-    if (mNsls > 0 || mNumFElems > 0 || mNumMass3D > 0 || !iso ||
-        !hInstDt.empty())
+    if (mNumFElems > 0 || mNumMass3D > 0 || !iso || !hInstDt.empty())
         throw Error("Domain::adjoint || The adjoint run supports solid "
-                    "elements with an isotropic elastic medium (1D or 3D) "
-                    "and no attenuation, fluid, 3D mass or batched "
-                    "instances.");
+                    "elements with an isotropic elastic medium (1D or 3D, "
+                    "with or without attenuation) and no fluid, 3D mass or "
+                    "batched instances.");
 }
 
 void DomainBase::adjointOperands() {
@@ -1197,6 +1196,7 @@ void DomainBase::swapAdjointFields() {
     dDispl.swap(dAdjU);
     dVeloc.swap(dAdjV);
     dAccel.swap(dAdjA);
+    for (int i = 0; i < 4; i++) attState(i)->swap(dAdjAtt[i]);
 }
 
 void DomainBase::dropGraph() {
@@ -1641,9 +1641,14 @@ public:
         const int seglen = std::min(seg, nsteps);
         size_t freeb = 0, totalb = 0;
         hip_check(hipMemGetInfo(&freeb, &totalb), "Domain::adjoint");
+        // the forward state of a checkpoint: u, v, a and the SLS memory
+        // variables and stress increments
+        size_t attBytes = 0;
+        for (int i = 0; i < 4; i++) attBytes += attState(i)->bytes();
+        const size_t ckBytes = 3 * nf + attBytes;
         const double need =
-            (double)nf * (3.0 * nck + 2.0 * seglen + 3.0) +
-            (double)mCardScrMax * sizeof(cpx<Real>) +
+            (double)ckBytes * nck + (double)nf * (2.0 * seglen + 3.0) +
+            (double)attBytes + (double)mCardScrMax * sizeof(cpx<Real>) +
             (double)mNumSlices * 2 * NPEM * sizeof(double);
         if (need > 0.9 * (double)freeb) {
             std::ostringstream ss;
@@ -1656,7 +1661,8 @@ public:
         dAdjU.alloc(nf);
         dAdjV.alloc(nf);
         dAdjA.alloc(nf);
-        dAdjCkpt.alloc(nf * 3 * nck);
+        for (int i = 0; i < 4; i++) dAdjAtt[i].alloc(attState(i)->bytes());
+        dAdjCkpt.alloc(ckBytes * nck);
         dAdjSegU.alloc(nf * seglen);
         dAdjSegA.alloc(nf * seglen);
         const size_t ng = (size_t)nsteps * mNumRec * 3;
@@ -1672,37 +1678,67 @@ public:
             dAdjScrB.alloc(mCardScrMax * sizeof(cpx<Real>));
         }
         const int all = PHASE_GATHER | PHASE_AXIAL | PHASE_MASS | PHASE_MARCH;
-        double *ck = dAdjCkpt.as<double>();
+        char *ck = dAdjCkpt.as<char>();
         auto d2d = [&](double *dst, const double *src) {
             hip_check(hipMemcpyAsync(dst, src, nf, hipMemcpyDeviceToDevice,
                                      mStream),
                       "Domain::adjoint");
         };
+        // forward state <-> checkpoint s
+        auto save = [&](int s) {
+            char *c = ck + (size_t)s * ckBytes;
+            d2d(reinterpret_cast<double *>(c), dDispl.as<double>());
+            d2d(reinterpret_cast<double *>(c + nf), dVeloc.as<double>());
+            d2d(reinterpret_cast<double *>(c + 2 * nf), dAccel.as<double>());
+            size_t o = 3 * nf;
+            for (int i = 0; i < 4; i++) {
+                DeviceBuffer *b = attState(i);
+                if (b->bytes() > 0)
+                    hip_check(hipMemcpyAsync(c + o, b->as<char>(), b->bytes(),
+                                             hipMemcpyDeviceToDevice,
+                                             mStream),
+                              "Domain::adjoint");
+                o += b->bytes();
+            }
+        };
+        auto load = [&](int s) {
+            const char *c = ck + (size_t)s * ckBytes;
+            d2d(dDispl.as<double>(), reinterpret_cast<const double *>(c));
+            d2d(dVeloc.as<double>(),
+                reinterpret_cast<const double *>(c + nf));
+            d2d(dAccel.as<double>(),
+                reinterpret_cast<const double *>(c + 2 * nf));
+            size_t o = 3 * nf;
+            for (int i = 0; i < 4; i++) {
+                DeviceBuffer *b = attState(i);
+                if (b->bytes() > 0)
+                    hip_check(hipMemcpyAsync(b->as<char>(), c + o, b->bytes(),
+                                             hipMemcpyDeviceToDevice,
+                                             mStream),
+                              "Domain::adjoint");
+                o += b->bytes();
+            }
+        };
         // forward from rest up to the last checkpoint
         dDispl.zero(mStream);
         dVeloc.zero(mStream);
         dAccel.zero(mStream);
+        for (int i = 0; i < 4; i++) attState(i)->zero(mStream);
         for (int n = 0; n <= (nck - 1) * seg; n++) {
-            if (n % seg == 0) {
-                double *c = ck + (size_t)(n / seg) * 3 * nd;
-                d2d(c, dDispl.as<double>());
-                d2d(c + nd, dVeloc.as<double>());
-                d2d(c + 2 * nd, dAccel.as<double>());
-            }
+            if (n % seg == 0) save(n / seg);
             if (n < (nck - 1) * seg) step(n, -1, false);
         }
-        // backwards, segment by segment
+        // backwards, segment by segment; the adjoint field starts at rest
+        // with its own (zero) SLS state
         dAdjU.zero(mStream);
         dAdjV.zero(mStream);
         dAdjA.zero(mStream);
+        for (int i = 0; i < 4; i++) dAdjAtt[i].zero(mStream);
         const double dt2 = mDt * mDt;
         double *su = dAdjSegU.as<double>(), *sa = dAdjSegA.as<double>();
         for (int s = nck - 1; s >= 0; s--) {
             const int n0 = s * seg, n1 = std::min(n0 + seg, nsteps);
-            const double *c = ck + (size_t)s * 3 * nd;
-            d2d(dDispl.as<double>(), c);
-            d2d(dVeloc.as<double>(), c + nd);
-            d2d(dAccel.as<double>(), c + 2 * nd);
+            load(s);
             for (int n = n0; n < n1; n++) {
                 d2d(su + (size_t)(n - n0) * nd, dDispl.as<double>());
                 step(n, -1, false);
@@ -1758,6 +1794,7 @@ public:
         dAdjSegU.reset();
         dAdjSegA.reset();
         dAdjScrB.reset();
+        for (int i = 0; i < 4; i++) dAdjAtt[i].reset();
     }
 
     void getField(int which, double *out) override {

@@ -304,6 +304,13 @@ protected:
     // the forward fields of one segment
     DeviceBuffer dAdjGQ, dAdjGTab, dAdjKern, dAdjU, dAdjV, dAdjA, dAdjCkpt,
         dAdjSegU, dAdjSegA, dAdjG, dAdjKind, dAdjKern3D, dAdjScrB;
+    // the adjoint field's own SLS memory variables and stress increments
+    // (cardinal and Fourier path), swapped in with its wavefields
+    DeviceBuffer dAdjAtt[4];
+    DeviceBuffer *attState(int i) {
+        DeviceBuffer *b[4] = {&dMemVar, &dDStress, &dMemVarFR, &dDStressFR};
+        return b[i];
+    }
     int mRecNu1Max = 0;
     void adjointCheck() const;
     void adjointOperands();

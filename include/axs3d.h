@@ -339,9 +339,12 @@ int axs3d_set_graph(axs3d_ctx *ctx, int enable);
 /* ---------------- adjoint --------------------------------------------- */
 /* No counterpart in the reference (it has no adjoint solver): the
  * discrete adjoint of axs3d_run for solid domains of isotropic elastic
- * elements, 1D or 3D, without attenuation (other domains: error).  Runs
+ * elements, 1D or 3D, with or without SLS attenuation (other domains:
+ * error; with attenuation the derivatives are with respect to the
+ * unrelaxed moduli at fixed anelastic coefficients).  Runs
  * the forward problem from rest over steps [0, nsteps), keeping (u, v, a)
- * every checkpoint_interval steps on the device, then the adjoint field
+ * (and the SLS memory variables) every checkpoint_interval steps on the
+ * device, then the adjoint field (with memory variables of its own)
  * backwards in time with the force R^* djdr[n] / dt^2 (djdr:
  * [nsteps][n_receivers][3] FP64, the derivative of the misfit with respect
  * to the seismogram sample of step n, as axs3d_read_seismograms lays it
