@@ -2,8 +2,10 @@
 //
 // The reference has no adjoint solver; this is the discrete adjoint of its
 // time loop (NewmarkTimeScheme.cpp:41-108, the central-difference form of
-// NewmarkTimeScheme.hpp:48-71) for 1D isotropic elastic elements on the
-// Fourier path (Isotropic.hpp:91-124).  With u_{n+1} = 2 u_n - u_{n-1} +
+// NewmarkTimeScheme.hpp:48-71) for isotropic elastic elements
+// (Isotropic.hpp:91-124); this file holds the densities of the Fourier-path
+// elements and of the mass, and the adjoint force (the per-slice densities
+// of 3D elements are k_cd_sens).  With u_{n+1} = 2 u_n - u_{n-1} +
 // dt^2 M^-1 P (f_n - K u_n), receivers r_n = R u_n and dJ/dr_n = G_n, the
 // adjoint field psi obeys the same recursion backwards in time with the
 // force R^* G_n / dt^2, and
@@ -18,7 +20,11 @@
 //   <psi, dM/drho a>    = IF w Re conj(psi_p) . a_p
 // (the element stiffness is Quad(sigma(Grad u)) with Quad the integral-
 // factor-weighted transpose of Grad, GradientQuadrature.hpp:207-324; the
-// mass is sum rho IF, GLLPoint.cpp:54-77).
+// mass is sum rho IF, GLLPoint.cpp:54-77).  With SLS attenuation the
+// stress is C eps_n - sum_{j>=1} h_j D eps_{n-j} with h_j fixed by (alpha,
+// beta, gamma) and D symmetric (Attenuation.hpp:92-155): the adjoint field
+// runs the same recursion with memory variables of its own, and the
+// densities above are those of the unrelaxed moduli in C.
 #include <hip/hip_runtime.h>
 
 #include "axs3d_common.hpp"
