@@ -175,6 +175,21 @@ def bytes_per_step(spec, real):
             "cd_quad": int(quad), "point": int(point)}
 
 
+def pmc_valu_busy(kernel, workload):
+    """Fraction of the VALU issue slots `kernel` keeps busy on `workload`
+    (SQ_INSTS_VALU over the CU cycles, from the committed rocprofv3
+    counter summary), or None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                        "profiles", "valu_busy.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) >= 3 and f[0] == kernel and f[1] == workload:
+            return float(f[2])
+    return None
+
+
 def pmc_traffic(kernel, workload):
     """HBM bytes per launch of `kernel` on `workload` from the committed
     rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE per dispatch, each
@@ -461,7 +476,8 @@ def main():
         "roofline": {"bound": "hbm", "kernel": names[dom],
                      "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic},
+                     "traffic": traffic,
+                     "valu_busy": pmc_valu_busy(names[dom], a.workload)},
         "step_roofline": {"algorithmic_bytes_per_step": step_bytes,
                           "achieved_GBs": step_bytes / (t32 / a.steps) / 1e9,
                           "frac": step_bytes / (t32 / a.steps) / 1e9 /
