@@ -1863,14 +1863,54 @@ private:
         throw Error("Domain::field || Unknown field id.");
     }
 
-    // Times one pass of the cardinal path right after finalize, on zero
-    // fields (the element forces come out zero and the attenuation state
-    // stays zero, so the trial leaves no trace), after one untimed pass
-    // that loads the code.  AXS3D_AUTOTUNE=0 skips it.
-    void autotune() override {
-        if (mNum3D == 0) return;
-        const char *env = std::getenv("AXS3D_AUTOTUNE");
-        if (env && env[0] == '0') return;
+    // Cardinal batches for a scratch of `bytes`: runs of one material
+    // type whose strain and stress ([150][nu1] complex per element) fit
+    // the scratch together, and the 3D element-modes in list order for the
+    // gradient and quadrature kernels.  The scratch also holds the 3D
+    // fluid batches' rows.
+    void buildCardBatches(size_t bytes) {
+        const size_t budget =
+            std::min(bytes / sizeof(cpx<Real>), (size_t)UINT32_MAX);
+        std::vector<uint32_t> scrOff(mNumElems, 0u);
+        std::vector<EMItem> em3d;
+        mCardBatches.clear();
+        size_t scrMax = 0;
+        for (int mt = 0; mt < 3; mt++) {
+            size_t used = 0;
+            for (int w = mTypeBegin3D[mt]; w < mTypeBegin3D[mt + 1]; w++) {
+                const int e = hList3D[w];
+                const int nu1 = hElemNr[e] / 2 + 1;
+                const size_t need = (size_t)6 * NPEM * nu1;
+                if (mCardBatches.empty() || mCardBatches.back().mtype != mt ||
+                    used + need > budget) {
+                    mCardBatches.push_back(
+                        {mt, w, w, (int)em3d.size(), (int)em3d.size()});
+                    used = 0;
+                }
+                scrOff[e] = (uint32_t)used;
+                used += need;
+                for (int al = 0; al < nu1; al++) em3d.push_back({e, al});
+                mCardBatches.back().end = w + 1;
+                mCardBatches.back().em_end = (int)em3d.size();
+                scrMax = std::max(scrMax, used);
+            }
+        }
+        dScrOff.alloc(scrOff.size() * 4);
+        dScrOff.upload(scrOff.data(), scrOff.size() * 4, mStream);
+        dEM3D.alloc(em3d.size() * sizeof(EMItem));
+        dEM3D.upload(em3d.data(), em3d.size() * sizeof(EMItem), mStream);
+        scrMax = std::max(scrMax, mFCardScrMax);
+        if (scrMax != mCardScrMax || dCardScratch.bytes() == 0) {
+            mCardScrMax = scrMax;
+            dCardScratch.alloc(scrMax * sizeof(cpx<Real>));
+        }
+        mScratchMiB = (double)bytes / (1 << 20);
+    }
+
+    // One untimed and one timed pass of the cardinal path, on zero fields
+    // (the element forces come out zero and the attenuation state stays
+    // zero, so a trial leaves no trace).
+    float timeCardinalPass() {
         hipEvent_t a, b;
         hip_check(hipEventCreate(&a), "Domain::autotune");
         hip_check(hipEventCreate(&b), "Domain::autotune");
@@ -1881,9 +1921,36 @@ private:
         hip_check(hipEventSynchronize(b), "Domain::autotune");
         float ms = 0.f;
         hip_check(hipEventElapsedTime(&ms, a, b), "Domain::autotune");
-        mCardTuneMs = ms;
         (void)hipEventDestroy(a);
         (void)hipEventDestroy(b);
+        return ms;
+    }
+
+    // Picks the cardinal scratch size (batch size) by timing one pass for
+    // each of 4096, 1024 and 256 MiB (fewer, larger batches against
+    // batches whose scratch stays in the 256 MB Infinity Cache), unless
+    // AXS3D_CARD_SCRATCH_MB fixes it; AXS3D_AUTOTUNE=0 skips the timing.
+    void autotune() override {
+        if (mNum3D == 0) return;
+        const char *env = std::getenv("AXS3D_AUTOTUNE");
+        if (env && env[0] == '0') return;
+        mCardTuneMs = timeCardinalPass();
+        mTuneTrials = 1;
+        if (mScratchFixed) return;
+        double best_mib = mScratchMiB;
+        float best = (float)mCardTuneMs;
+        for (const size_t mib : {(size_t)1024, (size_t)256}) {
+            buildCardBatches(mib << 20);
+            const float ms = timeCardinalPass();
+            mTuneTrials++;
+            if (ms < best) {
+                best = ms;
+                best_mib = (double)mib;
+            }
+        }
+        if (best_mib != mScratchMiB)
+            buildCardBatches((size_t)best_mib * ((size_t)1 << 20));
+        mCardTuneMs = best;
     }
 
     // 3D operands, host [slice][width] per element -> device [width][Nr]
@@ -2087,47 +2154,14 @@ private:
                           mDevice), "Domain::finalize");
             mCardSlots = std::max(1, card_blocks_per_cu<Real>(mNsls > 0) *
                                          ncu);
-            // batches: runs of one material type whose strain and stress
-            // ([150][nu1] complex per element) fit the scratch together;
-            // the 3D element-modes in list order for the gradient and
-            // quadrature kernels
             size_t budget = CARD_SCRATCH_BYTES;
-            if (const char *env = std::getenv("AXS3D_CARD_SCRATCH_MB"))
-                budget = (size_t)std::max(1L, std::atol(env)) << 20;
-            budget = std::min(budget / sizeof(cpx<Real>),
-                              (size_t)UINT32_MAX);
-            std::vector<uint32_t> scrOff(mNumElems, 0u);
-            std::vector<EMItem> em3d;
-            mCardBatches.clear();
-            size_t scrMax = 0;
-            for (int mt = 0; mt < 3; mt++) {
-                size_t used = 0;
-                for (int w = mTypeBegin3D[mt]; w < mTypeBegin3D[mt + 1];
-                     w++) {
-                    const int e = hList3D[w];
-                    const int nu1 = hElemNr[e] / 2 + 1;
-                    const size_t need = (size_t)6 * NPEM * nu1;
-                    if (mCardBatches.empty() ||
-                        mCardBatches.back().mtype != mt ||
-                        used + need > budget) {
-                        mCardBatches.push_back(
-                            {mt, w, w, (int)em3d.size(), (int)em3d.size()});
-                        used = 0;
-                    }
-                    scrOff[e] = (uint32_t)used;
-                    used += need;
-                    for (int al = 0; al < nu1; al++) em3d.push_back({e, al});
-                    mCardBatches.back().end = w + 1;
-                    mCardBatches.back().em_end = (int)em3d.size();
-                    scrMax = std::max(scrMax, used);
-                }
+            if (const char *env = std::getenv("AXS3D_CARD_SCRATCH_MB")) {
+                // MiB, fractions allowed (at least one element per batch)
+                budget = (size_t)std::max(
+                    1.0, std::atof(env) * (double)((size_t)1 << 20));
+                mScratchFixed = true;
             }
-            dScrOff.alloc(scrOff.size() * 4);
-            dScrOff.upload(scrOff.data(), scrOff.size() * 4, st);
-            dEM3D.alloc(em3d.size() * sizeof(EMItem));
-            dEM3D.upload(em3d.data(), em3d.size() * sizeof(EMItem), st);
-            mCardScrMax = scrMax;
-            dCardScratch.alloc(scrMax * sizeof(cpx<Real>));
+            buildCardBatches(budget);
             if (mNsls > 0) {
                 const int npa = mAttCG4 ? 4 : NPEM;
                 dMemVar.alloc((size_t)mNumSlices * mNsls * 6 * npa *
@@ -2158,7 +2192,8 @@ private:
             // batches whose [3][25][nu1] rows fit the scratch together
             size_t budget = CARD_SCRATCH_BYTES;
             if (const char *env = std::getenv("AXS3D_CARD_SCRATCH_MB"))
-                budget = (size_t)std::max(1L, std::atol(env)) << 20;
+                budget = (size_t)std::max(
+                    1.0, std::atof(env) * (double)((size_t)1 << 20));
             budget = std::min(budget / sizeof(cpx<Real>),
                               (size_t)UINT32_MAX);
             std::vector<uint32_t> fscr(mNumFElems, 0u);
@@ -2186,6 +2221,7 @@ private:
             dFEM3D.alloc(fem3d.size() * sizeof(EMItem));
             dFEM3D.upload(fem3d.data(), fem3d.size() * sizeof(EMItem), st);
             // one scratch for both paths (they run one after the other)
+            mFCardScrMax = scrMax;
             if (scrMax > mCardScrMax) {
                 mCardScrMax = scrMax;
                 dCardScratch.alloc(scrMax * sizeof(cpx<Real>));

@@ -153,7 +153,8 @@ public:
         *occ = mCardOcc;
         *slots = mCardSlots;
         ms[0] = mCardTuneMs;
-        ms[1] = ms[2] = 0.;
+        ms[1] = mScratchMiB;
+        ms[2] = mTuneTrials;
     }
     int precision() const { return mPrecision; }
 
@@ -252,6 +253,10 @@ protected:
     std::vector<CardBatch> mCardBatches;
     int mCardOcc = CARD_OCC;            // stress kernel waves per SIMD
     double mCardTuneMs = 0.;            // one cardinal pass at finalize
+    double mScratchMiB = 0.;            // cardinal scratch budget in use
+    bool mScratchFixed = false;         // set by AXS3D_CARD_SCRATCH_MB
+    int mTuneTrials = 0;                // scratch sizes timed
+    size_t mFCardScrMax = 0;            // 3D fluid rows in the scratch
     std::vector<uint8_t> hProp3DSet, hAtt3DSet;
     // attenuation
     int mNsls = 0, mAttCG4 = 0;

@@ -341,4 +341,6 @@ class GpuSolver:
                                                C.byref(blocks), ms))
         return {"cardinal_occupancy": occ.value,
                 "cardinal_blocks": blocks.value,
-                "cardinal_pass_ms": ms[0]}
+                "cardinal_pass_ms": ms[0],
+                "cardinal_scratch_mib": ms[1],
+                "scratch_sizes_timed": int(ms[2])}

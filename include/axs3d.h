@@ -387,9 +387,10 @@ int axs3d_profile_reset(axs3d_ctx *ctx);
 
 /* The cardinal path as set up at axs3d_finalize (no reference
  * counterpart; the reference has no GPU): the stress kernel's minimum
- * waves per SIMD, its resident blocks on the device, and in trial_ms3[0]
- * the time (ms) of one cardinal pass measured at finalize (entries 1, 2
- * are 0). */
+ * waves per SIMD, its resident blocks on the device, and in trial_ms3 the
+ * time (ms) of one cardinal pass with the chosen batches, the cardinal
+ * scratch size chosen by the autotuner (MiB; it times one pass per
+ * candidate size) and the number of sizes timed. */
 int axs3d_autotune_info(const axs3d_ctx *ctx, int *cardinal_occupancy,
                         int *cardinal_blocks, double *trial_ms3);
 

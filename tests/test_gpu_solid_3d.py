@@ -120,3 +120,27 @@ def _run(spec, prec, nsteps):
     out = g.seismograms()
     g.close()
     return out
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_batching_does_not_change_results(prec, monkeypatch):
+    """A 50 KiB scratch splits the 3D elements into many batches (a few
+    elements each); the kernels see the same operands, so the fields are
+    bitwise those of the default (autotuned) batches."""
+    nsteps = 60
+    spec = _spec(nsteps)
+    out = []
+    for mib in (None, "0.05"):
+        if mib is None:
+            monkeypatch.delenv("AXS3D_CARD_SCRATCH_MB", raising=False)
+        else:
+            monkeypatch.setenv("AXS3D_CARD_SCRATCH_MB", mib)
+        g = _solver(spec, prec)
+        info = g.autotune_info()
+        g.run(nsteps)
+        out.append((g.get_field("displ"), g.seismograms(), info))
+        g.close()
+    assert abs(out[1][2]["cardinal_scratch_mib"] - 0.05) < 1e-6
+    assert np.abs(out[0][0]).max() > 0
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
