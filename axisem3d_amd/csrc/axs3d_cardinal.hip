@@ -339,7 +339,7 @@ struct SliceMat {
     const Real *par;             // this point's parameter 0
     size_t ps;                   // parameter step, 25 Nr
     const Real *dl, *dm;         // moduli of its attenuation slot
-    Real *mv;                    // memory variables [nsls][6] of the slot
+    double *mv;                  // memory variables [nsls][6] of the slot
     Real *ds;                    // memory stress [6] of the slot
     size_t cs;                   // component step of mv, ds: npa Nr
     __device__ Real p(int c, int k) const { return par[c * ps + k]; }
@@ -354,42 +354,47 @@ struct SliceMat {
 // precision (LDS, [3][CD_MAX_SLS]).  The solids go four at a time so
 // their loads are in flight together.
 constexpr int CD_MAX_SLS = 16;
+// The memory variables and the recursion are FP64 in both precision
+// modes (the stress increments stay in the working precision).
 template <typename Real, int NS>
-__device__ __forceinline__ void att_update(Real *mvk, Real *dsk, int cs,
-                                           int nsls, const Real *abg, int d,
-                                           Real dsn, Real &sig) {
+__device__ __forceinline__ void att_update(double *mvk, Real *dsk, int cs,
+                                           int nsls, const double *abg,
+                                           int d, Real dsn, Real &sig) {
     const int od = d * cs;
     const int ls = 6 * cs;
-    const Real dso = dsk[od];
+    const double dso = (double)dsk[od];
+    const double dn = (double)dsn;
+    double sd = (double)sig;
     if constexpr (NS > 0) {
         // the common five-solid case (or any fixed count): unrolled
-        Real m[NS];
+        double m[NS];
 #pragma unroll
         for (int l = 0; l < NS; l++) m[l] = mvk[od + l * ls];
 #pragma unroll
-        for (int l = 0; l < NS; l++) sig -= m[l];
+        for (int l = 0; l < NS; l++) sd -= m[l];
 #pragma unroll
         for (int l = 0; l < NS; l++)
             mvk[od + l * ls] = abg[l] * m[l] + abg[CD_MAX_SLS + l] * dso +
-                               abg[2 * CD_MAX_SLS + l] * dsn;
+                               abg[2 * CD_MAX_SLS + l] * dn;
     } else {
         for (int l0 = 0; l0 < nsls; l0 += 4) {
-            Real m[4];
+            double m[4];
 #pragma unroll
             for (int q = 0; q < 4; q++)
-                m[q] = l0 + q < nsls ? mvk[od + (l0 + q) * ls] : (Real)0;
+                m[q] = l0 + q < nsls ? mvk[od + (l0 + q) * ls] : 0.;
 #pragma unroll
-            for (int q = 0; q < 4; q++) sig -= m[q];
+            for (int q = 0; q < 4; q++) sd -= m[q];
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const int l = l0 + q;
                 if (l < nsls)
                     mvk[od + l * ls] = abg[l] * m[q] +
                                        abg[CD_MAX_SLS + l] * dso +
-                                       abg[2 * CD_MAX_SLS + l] * dsn;
+                                       abg[2 * CD_MAX_SLS + l] * dn;
             }
         }
     }
+    sig = (Real)sd;
     dsk[od] = dsn;
 }
 
@@ -400,7 +405,7 @@ template <typename Real>
 struct CdLds {
     cpx<Real> tw[CARD_TW_CAP];
     cpx<Real> wbuf[CD_PW][CD_BUF];
-    Real abg[3 * CD_MAX_SLS];      // SLS alpha, beta, gamma
+    double abg[3 * CD_MAX_SLS];    // SLS alpha, beta, gamma
     uint16_t pos[CARD_NR_CAP];     // buffer position of slice k
 };
 
@@ -534,7 +539,7 @@ __device__ __forceinline__ void cd_stress_slices(const CardArgs<Real> &a,
                                                  cpx<Real> *b2,
                                                  const uint16_t *pos,
                                                  const SliceMat<Real> &M,
-                                                 const Real *abg, bool att,
+                                                 const double *abg, bool att,
                                                  int nr, int t) {
     // two slices in flight without attenuation; with it the memory
     // variables of one slice already fill the registers
@@ -574,7 +579,8 @@ __device__ __forceinline__ void cd_stress_slices(const CardArgs<Real> &a,
             const Real dl = M.dl[k], dm = M.dm[k];
             const Real dm2 = dm * (Real)2;
             const Real tr = e[0] + e[1] + e[2];
-            Real *mvk = M.mv + k, *dsk = M.ds + k;
+            double *mvk = M.mv + k;
+            Real *dsk = M.ds + k;
             const int cs = (int)M.cs;
 #pragma unroll
             for (int d = 0; d < 6; d++)
@@ -615,7 +621,7 @@ __global__ __launch_bounds__(CD_PT, CARD_OCC) void k_cd_stress(
         for (int x = t; x < nr; x += CD_PT) L.pos[x] = pg[x];
         if (NS != 0 && t < 3 * a.nsls) {
             const int c = t / a.nsls, l = t - c * a.nsls;
-            L.abg[c * CD_MAX_SLS + l] = (Real)a.abg[t];
+            L.abg[c * CD_MAX_SLS + l] = a.abg[t];
         }
     }
     __syncthreads();

@@ -1308,7 +1308,7 @@ public:
         a.npa = mAttCG4 ? 4 : NPEM;
         a.abg = dAttABG.as<double>();
         a.att1d = dAtt1D.as<Real>();
-        a.mv_fr = dMemVarFR.as<Real>();
+        a.mv_fr = dMemVarFR.as<double>();
         a.ds_fr = dDStressFR.as<Real>();
         return a;
     }
@@ -1334,7 +1334,7 @@ public:
         c.npa = mAttCG4 ? 4 : NPEM;
         c.abg = dAttABG.as<double>();
         c.att3d = dAtt3D.as<Real>();
-        c.memvar = dMemVar.as<Real>();
+        c.memvar = dMemVar.as<double>();
         c.dstress = dDStress.as<Real>();
         return c;
     }
@@ -2113,7 +2113,7 @@ private:
             dAtt1D.alloc(a1.size() * sizeof(Real));
             dAtt1D.upload(a1.data(), a1.size() * sizeof(Real), st);
             dMemVarFR.alloc((size_t)mNumEM * mNsls * 6 * npa * 2 *
-                            sizeof(Real));
+                            sizeof(double));
             dMemVarFR.zero(st);
             dDStressFR.alloc((size_t)mNumEM * 6 * npa * 2 * sizeof(Real));
             dDStressFR.zero(st);
@@ -2165,7 +2165,7 @@ private:
             if (mNsls > 0) {
                 const int npa = mAttCG4 ? 4 : NPEM;
                 dMemVar.alloc((size_t)mNumSlices * mNsls * 6 * npa *
-                              sizeof(Real));
+                              sizeof(double));
                 dMemVar.zero(st);
                 dDStress.alloc((size_t)mNumSlices * 6 * npa * sizeof(Real));
                 dDStress.zero(st);

@@ -120,27 +120,39 @@ __global__ __launch_bounds__(STIFF_BLOCK) void k_solid_stiff_fr(
                 const Real *dlm = a.att1d + (size_t)e * 2 * npa + qa;
                 cpx<Real> dsn[6];
                 iso_hooke<Real>(eps, dlm[0], dlm[npa], dsn);
-                cpx<Real> *mv = reinterpret_cast<cpx<Real> *>(a.mv_fr) +
-                                (size_t)em * nsls * 6 * npa + qa;
+                // the memory variables and their recursion in FP64 (the
+                // stress increments in the working precision)
+                cpx<double> *mv = reinterpret_cast<cpx<double> *>(a.mv_fr) +
+                                  (size_t)em * nsls * 6 * npa + qa;
                 cpx<Real> *ds = reinterpret_cast<cpx<Real> *>(a.ds_fr) +
                                 (size_t)em * 6 * npa + qa;
+                cpx<double> sd[6];
+#pragma unroll
+                for (int d = 0; d < 6; d++)
+                    sd[d] = {(double)sig[d].re, (double)sig[d].im};
                 for (int l = 0; l < nsls; l++)
 #pragma unroll
                     for (int d = 0; d < 6; d++)
-                        sig[d] = csub(sig[d], mv[(l * 6 + d) * npa]);
-                cpx<Real> dso[6];
+                        sd[d] = csub(sd[d], mv[(l * 6 + d) * npa]);
 #pragma unroll
-                for (int d = 0; d < 6; d++) dso[d] = ds[d * npa];
+                for (int d = 0; d < 6; d++)
+                    sig[d] = {(Real)sd[d].re, (Real)sd[d].im};
+                cpx<double> dso[6], dn[6];
+#pragma unroll
+                for (int d = 0; d < 6; d++) {
+                    dso[d] = {(double)ds[d * npa].re, (double)ds[d * npa].im};
+                    dn[d] = {(double)dsn[d].re, (double)dsn[d].im};
+                }
                 for (int l = 0; l < nsls; l++) {
-                    const Real al = (Real)a.abg[l];
-                    const Real be = (Real)a.abg[nsls + l];
-                    const Real ga = (Real)a.abg[2 * nsls + l];
+                    const double al = a.abg[l];
+                    const double be = a.abg[nsls + l];
+                    const double ga = a.abg[2 * nsls + l];
 #pragma unroll
                     for (int d = 0; d < 6; d++) {
-                        cpx<Real> &m = mv[(l * 6 + d) * npa];
-                        cpx<Real> x = cmul(m, al);
+                        cpx<double> &m = mv[(l * 6 + d) * npa];
+                        cpx<double> x = cmul(m, al);
                         x = cfma(dso[d], be, x);
-                        m = cfma(dsn[d], ga, x);
+                        m = cfma(dn[d], ga, x);
                     }
                 }
 #pragma unroll

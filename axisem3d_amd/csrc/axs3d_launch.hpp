@@ -98,7 +98,7 @@ struct StiffArgs {
     int npa;                       // 4 (CG4) or 25 (FULL)
     const double *abg;             // [3][nsls] alpha, beta, gamma
     const Real *att1d;             // [e][2][npa] dlambda, dmu
-    Real *mv_fr;                   // [em][nsls][6][npa] complex
+    double *mv_fr;                 // [em][nsls][6][npa] complex, FP64
     Real *ds_fr;                   // [em][6][npa] complex
 };
 
@@ -147,7 +147,7 @@ struct CardArgs {
     int npa;                       // 4 (CG4) or 25 (FULL)
     const double *abg;             // [3][nsls] alpha, beta, gamma
     const Real *att3d;             // [2][npa] dlambda, dmu
-    Real *memvar;                  // [nsls][6][npa]
+    double *memvar;                // [nsls][6][npa], FP64
     Real *dstress;                 // [6][npa]
 };
 

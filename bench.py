@@ -140,8 +140,8 @@ def bytes_per_step(spec, real):
     cd_grad: displacement read per point-mode of 3D elements, the 5
       gradient rows, connectivity.
     cd_stress: per slice np x 25 Real material and, with attenuation, per
-      attenuation point 2 Real moduli + nsls x 6 Real memory variables and
-      6 Real memory stress, the last two read and written.
+      attenuation point 2 Real moduli + nsls x 6 FP64 memory variables
+      and 6 Real memory stress, the last two read and written.
     cd_quad: element force write per point-mode of 3D elements, the 5
       integral-factor rows.
     point: element forces read, u, v, a read+written (FP64), per-point
@@ -168,7 +168,10 @@ def bytes_per_step(spec, real):
     if att is not None:
         npa = 4 if att["cg4"] else 25
         slices = int(nr[is3d].sum())
-        stress += slices * npa * real * (2 + 2 * 6 * att["nsls"] + 2 * 6)
+        # moduli and memory stress in the working precision, memory
+        # variables FP64 (read and written)
+        stress += slices * npa * (real * (2 + 2 * 6) +
+                                  8 * 2 * 6 * att["nsls"])
     quad = nupd3 * 6 * real + n3d * 5 * 25 * real
     point = nupd * (6 * real + 6 * 48 + 4) + U * (4 + 4 + 4 + 1 + 8 + 16)
     return {"fr": int(fr), "cd_grad": int(grad), "cd_stress": int(stress),
