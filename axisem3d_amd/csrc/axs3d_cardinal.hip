@@ -416,7 +416,12 @@ struct CdLds {
 // per block, the Fourier order fastest so that a row's consecutive orders
 // are written by consecutive lanes (CD_EMB = 16 orders: 128-byte runs in
 // FP32).
-template <typename Real, bool ROT>
+// AX: the launch over the axial elements' element-modes (a.em_ax) that
+// forms the strain of their axis row (GLJ, L'Hospital terms) only; the
+// launch over all element-modes leaves those points out, which keeps the
+// axis code and its registers out of the kernel that does nearly all the
+// work.
+template <typename Real, bool ROT, bool AX>
 __global__ __launch_bounds__(CD_EMB_THREADS) void k_cd_grad(CardArgs<Real> a,
                                                          int em0, int em1) {
     __shared__ cpx<Real> sU[CD_EMB][3][NPEM];
@@ -432,7 +437,7 @@ __global__ __launch_bounds__(CD_EMB_THREADS) void k_cd_grad(CardArgs<Real> a,
     int e = 0, alpha = 0;
     cpx<Real> u[3] = {{0, 0}, {0, 0}, {0, 0}};
     if (active) {
-        const EMItem it = s.em_list[em];
+        const EMItem it = (AX ? a.em_ax : s.em_list)[em];
         e = it.elem;
         alpha = it.alpha;
         const int p = s.elem_points[e * NPEM + ipnt];
@@ -448,13 +453,13 @@ __global__ __launch_bounds__(CD_EMB_THREADS) void k_cd_grad(CardArgs<Real> a,
     __syncthreads();
     if (!active) return;
     const bool axial = s.elem_axial[e] != 0;
+    if ((axial && i == 0) != AX) return;
     PointG<Real> G;
     load_point_g<Real>(sG, axial, i, j, G);
     PointOps<Real> g{}, q{};
     load_point_ops(s.elem_ops, e, ipnt, g, q);
     cpx<Real> eps[6];
-    grad6_point<Real>(sU[eml], u, (Real)alpha, axial && i == 0, i, j, G, g,
-                      eps);
+    grad6_point<Real>(sU[eml], u, (Real)alpha, AX, i, j, G, g, eps);
     if (ROT) {
         const Real *r = s.rot + (size_t)e * 4 * NPEM + ipnt;
         rot_spz_rtz6<cpx<Real>, Real>(eps, r[0], r[NPEM], r[2 * NPEM],
@@ -687,14 +692,25 @@ void launch_cd_grad(hipStream_t st, const CardArgs<Real> &a,
                     const CardBatch &bt) {
     const int gem = (bt.em_end - bt.em_begin + CD_EMB - 1) / CD_EMB;
     if (gem <= 0) return;
-    if (bt.mtype != 0 && a.s.rot != nullptr)
-        hipLaunchKernelGGL((k_cd_grad<Real, true>), dim3(gem),
+    const bool rot = bt.mtype != 0 && a.s.rot != nullptr;
+    if (rot)
+        hipLaunchKernelGGL((k_cd_grad<Real, true, false>), dim3(gem),
                            dim3(CD_EMB_THREADS), 0, st, a, bt.em_begin,
                            bt.em_end);
     else
-        hipLaunchKernelGGL((k_cd_grad<Real, false>), dim3(gem),
+        hipLaunchKernelGGL((k_cd_grad<Real, false, false>), dim3(gem),
                            dim3(CD_EMB_THREADS), 0, st, a, bt.em_begin,
                            bt.em_end);
+    const int gax = (bt.ax_end - bt.ax_begin + CD_EMB - 1) / CD_EMB;
+    if (gax <= 0) return;
+    if (rot)
+        hipLaunchKernelGGL((k_cd_grad<Real, true, true>), dim3(gax),
+                           dim3(CD_EMB_THREADS), 0, st, a, bt.ax_begin,
+                           bt.ax_end);
+    else
+        hipLaunchKernelGGL((k_cd_grad<Real, false, true>), dim3(gax),
+                           dim3(CD_EMB_THREADS), 0, st, a, bt.ax_begin,
+                           bt.ax_end);
 }
 
 template <typename Real>

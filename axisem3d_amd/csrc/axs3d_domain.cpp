@@ -1316,6 +1316,7 @@ public:
         CardArgs<Real> c;
         c.s = stiffArgs();
         c.s.em_list = dEM3D.as<EMItem>();
+        c.em_ax = dEM3DAx.as<EMItem>();
         c.n3d = mNum3D;
         c.list3d = dList3D.as<int32_t>();
         c.elem_nr = dElemNr.as<int32_t>();
@@ -1872,7 +1873,7 @@ private:
         const size_t budget =
             std::min(bytes / sizeof(cpx<Real>), (size_t)UINT32_MAX);
         std::vector<uint32_t> scrOff(mNumElems, 0u);
-        std::vector<EMItem> em3d;
+        std::vector<EMItem> em3d, emAx;
         mCardBatches.clear();
         size_t scrMax = 0;
         for (int mt = 0; mt < 3; mt++) {
@@ -1884,14 +1885,18 @@ private:
                 if (mCardBatches.empty() || mCardBatches.back().mtype != mt ||
                     used + need > budget) {
                     mCardBatches.push_back(
-                        {mt, w, w, (int)em3d.size(), (int)em3d.size()});
+                        {mt, w, w, (int)em3d.size(), (int)em3d.size(),
+                         (int)emAx.size(), (int)emAx.size()});
                     used = 0;
                 }
                 scrOff[e] = (uint32_t)used;
                 used += need;
                 for (int al = 0; al < nu1; al++) em3d.push_back({e, al});
+                if (hElemAxial[e])
+                    for (int al = 0; al < nu1; al++) emAx.push_back({e, al});
                 mCardBatches.back().end = w + 1;
                 mCardBatches.back().em_end = (int)em3d.size();
+                mCardBatches.back().ax_end = (int)emAx.size();
                 scrMax = std::max(scrMax, used);
             }
         }
@@ -1899,6 +1904,10 @@ private:
         dScrOff.upload(scrOff.data(), scrOff.size() * 4, mStream);
         dEM3D.alloc(em3d.size() * sizeof(EMItem));
         dEM3D.upload(em3d.data(), em3d.size() * sizeof(EMItem), mStream);
+        dEM3DAx.alloc(std::max<size_t>(emAx.size(), 1) * sizeof(EMItem));
+        if (!emAx.empty())
+            dEM3DAx.upload(emAx.data(), emAx.size() * sizeof(EMItem),
+                           mStream);
         scrMax = std::max(scrMax, mFCardScrMax);
         if (scrMax != mCardScrMax || dCardScratch.bytes() == 0) {
             mCardScrMax = scrMax;

@@ -278,6 +278,7 @@ protected:
     // device: cardinal path
     DeviceBuffer dElemNr, dSliceOff, dPropOff, dList3D, dScrOff, dEM3D,
         dPosOff, dPosTab, dMass3DPts, dMass3DOff, dMass3DFlag, dInvMass3D,
+        dEM3DAx,
         dFList3D, dFElemNr, dFK3D, dFK3DOff, dFScrOff, dFEM3D,
         dProp3D,
         dTwiddle,

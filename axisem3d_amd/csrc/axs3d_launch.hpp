@@ -124,6 +124,7 @@ struct CardArgs {
     StiffArgs<Real> s;
     int n3d;                       // number of 3D elements; s.em_list is
                                    // their element-modes in list3d order
+    const EMItem *em_ax;           // the element-modes of the axial ones
     const int32_t *list3d;         // 3D elements grouped by material type,
                                    // each group heaviest first
     int type_begin[4];             // group boundaries in list3d
@@ -243,6 +244,8 @@ void launch_solid_stiff_fr(hipStream_t st, const StiffArgs<Real> &a);
 struct CardBatch {
     int mtype, begin, end;
     int em_begin, em_end;          // its element-modes in the 3D list
+    int ax_begin, ax_end;          // its axial elements' element-modes in
+                                   // the axial list
 };
 // Default scratch of one batch (bytes).
 constexpr size_t CARD_SCRATCH_BYTES = (size_t)4 << 30;
