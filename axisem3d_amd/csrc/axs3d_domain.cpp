@@ -555,12 +555,16 @@ void DomainBase::finalize() {
         eNu1[e] = nrmax / 2 + 1;
     }
     // Fourier-path element-modes grouped by material type (one launch per
-    // type), in element order within a group
+    // type), in element order within a group, the axial elements last
     for (int mt = 0; mt < 3; mt++) {
         mEMTypeBegin[mt] = (int)em.size();
-        for (int e = 0; e < mNumElems; e++)
-            if (hElemKind[e] != 1 && hMType[e] == mt)
-                for (int a = 0; a < eNu1[e]; a++) em.push_back({e, a});
+        for (int ax = 0; ax < 2; ax++) {
+            if (ax) mEMAxBegin[mt] = (int)em.size();
+            for (int e = 0; e < mNumElems; e++)
+                if (hElemKind[e] != 1 && hMType[e] == mt &&
+                    (hElemAxial[e] != 0) == (ax != 0))
+                    for (int a = 0; a < eNu1[e]; a++) em.push_back({e, a});
+        }
     }
     mEMTypeBegin[3] = (int)em.size();
     mNumEM = (int)em.size();
@@ -1297,6 +1301,7 @@ public:
         a.mat_off = dMatOff.as<uint32_t>();
         a.rot = hTheta.empty() ? nullptr : dRot.as<Real>();
         for (int k = 0; k < 4; k++) a.em_type_begin[k] = mEMTypeBegin[k];
+        for (int k = 0; k < 3; k++) a.em_ax_begin[k] = mEMAxBegin[k];
         a.point_off = dPointOff.as<uint32_t>();
         a.point_nu1 = dPointNu1.as<int32_t>();
         a.displ = dDispl.as<double>();

@@ -191,6 +191,7 @@ protected:
     std::vector<uint8_t> hMType;        // 0 isotropic, 1 TI, 2 anisotropic
     std::vector<double> hTheta;         // [e][25] colatitude, or empty
     int mEMTypeBegin[4] = {0, 0, 0, 0};
+    int mEMAxBegin[3] = {0, 0, 0};
     // fluid
     std::vector<int32_t> hFPointNr, hFElemPoints, hFAxial, hFSurf;
     std::vector<double> hFMass, hFElemGQ, hFElemK;

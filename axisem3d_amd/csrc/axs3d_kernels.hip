@@ -37,7 +37,12 @@ __device__ __forceinline__ int att_slot_fr(int p, int npa) {
 
 // MT: material type of the element-modes [em0, em1) of em_list (0
 // isotropic, 1 TI, 2 anisotropic); em_list is grouped by type.
-template <typename Real, bool ATT, int MT>
+// AX: the element-modes [em0, em1) all belong to axial elements (GLJ in
+// xi, L'Hospital terms on the axis row) or all to non-axial ones; the
+// element-mode list keeps the axial elements of each type at its end, so
+// that the axis code and its registers stay out of the launch that does
+// nearly all the work.
+template <typename Real, bool ATT, int MT, bool AX>
 __global__ __launch_bounds__(STIFF_BLOCK) void k_solid_stiff_fr(
     StiffArgs<Real> a, int em0, int em1) {
     __shared__ cpx<Real> sU[EMB][3][NPEM];
@@ -65,7 +70,7 @@ __global__ __launch_bounds__(STIFF_BLOCK) void k_solid_stiff_fr(
         const int p = a.elem_points[e * NPEM + ipnt];
         nu1p = a.point_nu1[p];
         off = a.point_off[p];
-        axial = a.elem_axial[e] != 0;
+        axial = AX;
         if (alpha < nu1p) {
             const double *src = a.displ + (size_t)(off + alpha) * 6;
 #pragma unroll
@@ -527,21 +532,33 @@ void launch_solid_stiff_fr(hipStream_t st, const StiffArgs<Real> &a) {
     if (a.n_em <= 0) return;
     // one launch per material type present (em_list grouped by type)
     for (int mt = 0; mt < 3; mt++) {
-        const int em0 = a.em_type_begin[mt], em1 = a.em_type_begin[mt + 1];
-        if (em1 <= em0) continue;
-        const int grid = (em1 - em0 + EMB - 1) / EMB;
+        const int eb = a.em_type_begin[mt], ee = a.em_type_begin[mt + 1];
+        const int ea = a.em_ax_begin[mt];
         const bool att = a.nsls > 0;
+        for (int ax = 0; ax < 2; ax++) {
+            const int em0 = ax ? ea : eb, em1 = ax ? ee : ea;
+            if (em1 <= em0) continue;
+            const int grid = (em1 - em0 + EMB - 1) / EMB;
 #define AXS3D_FR(ATT_, MT_)                                                   \
-    hipLaunchKernelGGL((k_solid_stiff_fr<Real, ATT_, MT_>), dim3(grid),       \
-                       dim3(STIFF_BLOCK), 0, st, a, em0, em1)
-        if (mt == 0) {
-            if (att) AXS3D_FR(true, 0); else AXS3D_FR(false, 0);
-        } else if (mt == 1) {
-            if (att) AXS3D_FR(true, 1); else AXS3D_FR(false, 1);
-        } else {
-            if (att) AXS3D_FR(true, 2); else AXS3D_FR(false, 2);
-        }
+    do {                                                                      \
+        if (ax)                                                               \
+            hipLaunchKernelGGL((k_solid_stiff_fr<Real, ATT_, MT_, true>),     \
+                               dim3(grid), dim3(STIFF_BLOCK), 0, st, a, em0,  \
+                               em1);                                          \
+        else                                                                  \
+            hipLaunchKernelGGL((k_solid_stiff_fr<Real, ATT_, MT_, false>),    \
+                               dim3(grid), dim3(STIFF_BLOCK), 0, st, a, em0,  \
+                               em1);                                          \
+    } while (0)
+            if (mt == 0) {
+                if (att) AXS3D_FR(true, 0); else AXS3D_FR(false, 0);
+            } else if (mt == 1) {
+                if (att) AXS3D_FR(true, 1); else AXS3D_FR(false, 1);
+            } else {
+                if (att) AXS3D_FR(true, 2); else AXS3D_FR(false, 2);
+            }
 #undef AXS3D_FR
+        }
     }
 }
 

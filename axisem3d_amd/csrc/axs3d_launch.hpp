@@ -85,6 +85,8 @@ struct StiffArgs {
     const Real *rot;               // [e][4][25] sin, cos of theta, 2 theta
                                    // (spherical RTZ), or null
     int em_type_begin[4];          // em_list grouped by material type
+    int em_ax_begin[3];            // within type t, the axial elements'
+                                   // element-modes from em_ax_begin[t]
     const uint32_t *point_off;
     const int32_t *point_nu1;
     const double *displ;
