@@ -28,8 +28,8 @@
  *  - Precision: AXS3D_FP64 computes everything in FP64 (the reference
  *    built with -Ddouble=true); AXS3D_FP32_FP64 computes gradient,
  *    stress and quadrature in FP32 and keeps displacement, velocity,
- *    acceleration, mass division, source-time function and receivers in
- *    FP64.
+ *    acceleration, mass division, source-time function, receivers and
+ *    the SLS memory variables in FP64.
  */
 #ifndef AXS3D_H
 #define AXS3D_H
