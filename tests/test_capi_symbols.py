@@ -29,3 +29,29 @@ def test_errors_without_context():
     lib = capi.load()
     assert lib.axs3d_last_error(None) == b"axs3d || Null context."
     assert lib.axs3d_finalize(None) != 0
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    # no fallback: without the HIP library the solver refuses to start
+    import pytest
+    from axisem3d_amd import preloop
+    from axisem3d_amd.solver import GpuSolver
+    monkeypatch.setattr(capi, "LIB_PATH", str(tmp_path / "libaxs3d.so"))
+    monkeypatch.setattr(capi, "_lib", None)
+    spec = preloop.cartesian_homogeneous(nx=2, nz=2, width=2e3, depth=2e3,
+                                         nsteps=4, n_receivers=1)
+    with pytest.raises(capi.Axs3dError, match="not built"):
+        GpuSolver(spec, "fp64")
+
+
+def test_product_package_does_not_use_the_oracle():
+    pkg = os.path.join(ROOT, "axisem3d_amd")
+    pat = re.compile(r"^\s*(import\s+oracle|from\s+oracle\b)|oracle/", re.M)
+    offenders = []
+    for dirpath, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith((".py", ".cpp", ".hpp", ".hip", ".h")):
+                path = os.path.join(dirpath, f)
+                if pat.search(open(path, errors="replace").read()):
+                    offenders.append(os.path.relpath(path, ROOT))
+    assert not offenders, offenders
