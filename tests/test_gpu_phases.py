@@ -1,6 +1,8 @@
 """The per-phase entry points, called in the order of
 NewmarkTimeScheme::solve (NewmarkTimeScheme.cpp:41-108), against the bulk
-step (axs3d_run) on the same domain: seismograms and final displacement.
+step (axs3d_run) on the same domain: seismograms and final displacement,
+for a solid Fourier-path domain, a fluid layer with 3D coupling and 3D
+fluid mass, and a domain mixing 1D and 3D elements with attenuation.
 The two forms sum the source and the element forces in a different order,
 so they agree to rounding, not bitwise: relative 1e-12 in FP64, 1e-4 in
 FP32+FP64 (the FP32 stiffness rounding of 60 steps)."""
@@ -54,15 +56,33 @@ def _fluid_spec():
     return preloop.make_coupling3d(sp, amp=0.15, tilt=0.15, seed=8)
 
 
+def _mixed_att_spec():
+    # spherical regional domain, every third element 3D (cardinal path),
+    # CG4 attenuation on all of them
+    spec = preloop.spherical_regional(
+        ntheta=6, nrad=6, theta_max_deg=2.0, depth=60e3, nu_min=3,
+        nu_max=12, nsteps=300, n_receivers=3, attenuation=True, cg4=True,
+        corr_len=10e3)
+    E = spec.n_solid_elements
+    spec._build_3d((np.arange(E) % 3) == 1)
+    preloop.finish_regional(spec, 300, src_depth=20e3, f_peak=0.5,
+                            n_receivers=3, theta_max_deg=2.0)
+    return spec
+
+
+SPECS = {"solid": _solid_spec, "fluid": _fluid_spec,
+         "mixed3d_att": _mixed_att_spec}
+
+
 def _rel(a, b):
     return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-@pytest.mark.parametrize("case", ["solid", "fluid"])
+@pytest.mark.parametrize("case", sorted(SPECS))
 def test_phase_calls_match_bulk_step(case, prec):
     from axisem3d_amd.solver import GpuSolver
-    spec = _solid_spec() if case == "solid" else _fluid_spec()
+    spec = SPECS[case]()
     bulk = GpuSolver(spec, prec)
     phases = GpuSolver(spec, prec)
     try:
