@@ -33,10 +33,24 @@ def with_model(spec, rho, vp, vs, vp3d=None, vs3d=None):
     for x in (rho, vp, vs):
         if x.shape != (ne, 25):
             raise ValueError("model arrays must be (n_elements, 25)")
+    mtype = getattr(spec, "elem_mtype", None)
+    if mtype is not None and np.any(np.asarray(mtype) != 0):
+        raise ValueError("with_model: the adjoint covers isotropic media "
+                         "only")
     sp = copy.copy(spec)
     lam = rho * (vp ** 2 - 2.0 * vs ** 2)
     mu = rho * vs ** 2
     sp.elem_iso = np.ascontiguousarray(np.stack([lam, mu], axis=1))
+    if mtype is not None:
+        # a spec with the general material arrays is uploaded from mat1d
+        # (one (lambda, mu) block of 2 x 25 per element), not elem_iso
+        off = np.asarray(spec.mat1d_off, dtype=np.int64)
+        mat1d = np.array(spec.mat1d, dtype=np.float64, copy=True)
+        kind = getattr(spec, "elem_kind", None)
+        for e in range(ne):
+            if kind is None or int(kind[e]) == 0:
+                mat1d[off[e]:off[e] + 50] = sp.elem_iso[e].reshape(-1)
+        sp.mat1d = mat1d
     ifac = np.asarray(spec.elem_gq)[:, 5, :]
     mass = np.zeros(len(spec.solid_nr))
     np.add.at(mass, np.asarray(spec.elem_points).reshape(-1),
@@ -46,8 +60,11 @@ def with_model(spec, rho, vp, vs, vp3d=None, vs3d=None):
     if vp3d is not None:
         r3 = _slice_rho(spec, rho)
         vp3d, vs3d = (np.asarray(x, dtype=np.float64) for x in (vp3d, vs3d))
-        sp.prop3d = np.ascontiguousarray(np.stack(
+        p3 = np.ascontiguousarray(np.stack(
             [r3 * (vp3d ** 2 - 2.0 * vs3d ** 2), r3 * vs3d ** 2], axis=1))
+        # general-material specs keep prop3d flat, [slice][2][25] in
+        # element order: the same values in the same order
+        sp.prop3d = p3.reshape(-1) if mtype is not None else p3
     return sp
 
 
@@ -62,7 +79,8 @@ def _slice_rho(spec, rho):
 def model3d_of(spec):
     """(vp3d, vs3d) per slice of the 3D elements, [slice][25]."""
     r3 = _slice_rho(spec, np.asarray(spec._rho1d, dtype=np.float64))
-    lam, mu = spec.prop3d[:, 0, :], spec.prop3d[:, 1, :]
+    p3 = np.asarray(spec.prop3d).reshape(-1, 2, 25)
+    lam, mu = p3[:, 0, :], p3[:, 1, :]
     return np.sqrt((lam + 2.0 * mu) / r3), np.sqrt(mu / r3)
 
 
@@ -117,7 +135,8 @@ def chain_rule3d(spec, kern, kern3d, rho, vp, vs, vp3d, vs3d):
     (vp^2 - 2 vs^2), mu = rho vs^2 slice by slice."""
     d_rho, d_vp, d_vs = chain_rule(kern, rho, vp, vs)
     r3 = _slice_rho(spec, rho)
-    kl, km = kern3d[:, 0, :], kern3d[:, 1, :]
+    k3 = np.asarray(kern3d).reshape(-1, 2, 25)
+    kl, km = k3[:, 0, :], k3[:, 1, :]
     d_vp3 = 2.0 * r3 * vp3d * kl
     d_vs3 = 2.0 * r3 * vs3d * (km - 2.0 * kl)
     per_slice = (vp3d ** 2 - 2.0 * vs3d ** 2) * kl + vs3d ** 2 * km

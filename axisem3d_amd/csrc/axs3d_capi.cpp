@@ -19,7 +19,14 @@ int guarded(axs3d_ctx *ctx, F &&f) {
     if (!ctx) return 2;
     try {
         (void)hipSetDevice(ctx->device);
+        // a kernel launch that the runtime rejects (grid, LDS) leaves its
+        // error here only: clear it first, check it after the call
+        (void)hipGetLastError();
         f(*ctx->dom);
+        const hipError_t le = hipGetLastError();
+        if (le != hipSuccess)
+            throw axs3d::Error(std::string("axs3d || HIP launch error: ") +
+                               hipGetErrorString(le));
         ctx->err.clear();
         return 0;
     } catch (const std::exception &e) {
@@ -29,6 +36,20 @@ int guarded(axs3d_ctx *ctx, F &&f) {
         ctx->err = "axs3d || Unknown exception.";
         return 1;
     }
+}
+
+
+// Set-up calls: the device buffers, the captured step graph and the adjoint
+// caches are built once by finalize, so every set-up call after it (and a
+// second finalize) is refused instead of leaving them stale.
+template <typename F>
+int setup(axs3d_ctx *ctx, const char *where, F &&f) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) {
+        if (d.finalized())
+            throw axs3d::Error(std::string(where) +
+                               " || Domain is already finalized.");
+        f(d);
+    });
 }
 
 }  // namespace
@@ -65,33 +86,33 @@ int axs3d_precision_of(const axs3d_ctx *ctx) {
 }
 
 int axs3d_set_gmat(axs3d_ctx *ctx, const double *g_gll, const double *g_glj) {
-    return guarded(ctx, [&](axs3d::DomainBase &d) { d.setGMat(g_gll, g_glj); });
+    return setup(ctx, "Domain::set_gmat", [&](axs3d::DomainBase &d) { d.setGMat(g_gll, g_glj); });
 }
 
 int axs3d_set_solid_points(axs3d_ctx *ctx, int n, const int32_t *nr,
                            const double *mass) {
-    return guarded(ctx,
+    return setup(ctx, "Domain::set_solid_points",
                    [&](axs3d::DomainBase &d) { d.setSolidPoints(n, nr, mass); });
 }
 
 int axs3d_set_solid_elements(axs3d_ctx *ctx, int n, const int32_t *points,
                              const uint8_t *axial, const double *gq) {
-    return guarded(ctx, [&](axs3d::DomainBase &d) {
+    return setup(ctx, "Domain::set_solid_elements", [&](axs3d::DomainBase &d) {
         d.setSolidElements(n, points, axial, gq);
     });
 }
 
 int axs3d_set_solid_iso1d(axs3d_ctx *ctx, const double *lam_mu) {
-    return guarded(ctx, [&](axs3d::DomainBase &d) { d.setSolidIso1D(lam_mu); });
+    return setup(ctx, "Domain::set_solid_iso1d", [&](axs3d::DomainBase &d) { d.setSolidIso1D(lam_mu); });
 }
 
 int axs3d_set_element_kinds(axs3d_ctx *ctx, const uint8_t *kind) {
-    return guarded(ctx, [&](axs3d::DomainBase &d) { d.setElementKinds(kind); });
+    return setup(ctx, "Domain::set_element_kinds", [&](axs3d::DomainBase &d) { d.setElementKinds(kind); });
 }
 
 int axs3d_set_solid_iso3d(axs3d_ctx *ctx, int e_begin, int e_end,
                           const double *lam_mu) {
-    return guarded(ctx, [&](axs3d::DomainBase &d) {
+    return setup(ctx, "Domain::set_solid_iso3d", [&](axs3d::DomainBase &d) {
         d.setSolidIso3D(e_begin, e_end, lam_mu);
     });
 }
@@ -99,46 +120,46 @@ int axs3d_set_solid_iso3d(axs3d_ctx *ctx, int e_begin, int e_end,
 int axs3d_set_attenuation(axs3d_ctx *ctx, int nsls, int cg4,
                           const double *alpha, const double *beta,
                           const double *gamma) {
-    return guarded(ctx, [&](axs3d::DomainBase &d) {
+    return setup(ctx, "Domain::set_attenuation", [&](axs3d::DomainBase &d) {
         d.setAttenuation(nsls, cg4, alpha, beta, gamma);
     });
 }
 
 int axs3d_set_attenuation3d(axs3d_ctx *ctx, int e_begin, int e_end,
                             const double *dlam_dmu) {
-    return guarded(ctx, [&](axs3d::DomainBase &d) {
+    return setup(ctx, "Domain::set_attenuation3d", [&](axs3d::DomainBase &d) {
         d.setAttenuation3D(e_begin, e_end, dlam_dmu);
     });
 }
 
 int axs3d_set_fluid_points(axs3d_ctx *ctx, int n, const int32_t *nr,
                            const double *mass) {
-    return guarded(ctx, [&](axs3d::DomainBase &d) {
+    return setup(ctx, "Domain::set_fluid_points", [&](axs3d::DomainBase &d) {
         d.setFluidPoints(n, nr, mass);
     });
 }
 
 int axs3d_set_fluid_elements(axs3d_ctx *ctx, int n, const int32_t *points,
                              const uint8_t *axial, const double *gq) {
-    return guarded(ctx, [&](axs3d::DomainBase &d) {
+    return setup(ctx, "Domain::set_fluid_elements", [&](axs3d::DomainBase &d) {
         d.setFluidElements(n, points, axial, gq);
     });
 }
 
 int axs3d_set_fluid_k1d(axs3d_ctx *ctx, const double *k) {
-    return guarded(ctx, [&](axs3d::DomainBase &d) { d.setFluidK1D(k); });
+    return setup(ctx, "Domain::set_fluid_k1d", [&](axs3d::DomainBase &d) { d.setFluidK1D(k); });
 }
 
 int axs3d_set_fluid_axial_points(axs3d_ctx *ctx, int n,
                                  const int32_t *points) {
-    return guarded(ctx, [&](axs3d::DomainBase &d) {
+    return setup(ctx, "Domain::set_fluid_axial_points", [&](axs3d::DomainBase &d) {
         d.setFluidAxialPoints(n, points);
     });
 }
 
 int axs3d_set_fluid_surface_points(axs3d_ctx *ctx, int n,
                                    const int32_t *points) {
-    return guarded(ctx, [&](axs3d::DomainBase &d) {
+    return setup(ctx, "Domain::set_fluid_surface_points", [&](axs3d::DomainBase &d) {
         d.setFluidSurfacePoints(n, points);
     });
 }
@@ -147,7 +168,7 @@ int axs3d_set_solid_fluid_coupling(axs3d_ctx *ctx, int n,
                                    const int32_t *solid_points,
                                    const int32_t *fluid_points,
                                    const double *normals) {
-    return guarded(ctx, [&](axs3d::DomainBase &d) {
+    return setup(ctx, "Domain::set_solid_fluid_coupling", [&](axs3d::DomainBase &d) {
         d.setSolidFluidCoupling(n, solid_points, fluid_points, normals);
     });
 }
@@ -159,54 +180,54 @@ int axs3d_num_fluid_modes(const axs3d_ctx *ctx, int64_t *nmodes) {
 }
 
 int axs3d_set_material_types(axs3d_ctx *ctx, const uint8_t *mtype) {
-    return guarded(ctx, [&](axs3d::DomainBase &d) {
+    return setup(ctx, "Domain::set_material_types", [&](axs3d::DomainBase &d) {
         d.setMaterialTypes(mtype);
     });
 }
 
 int axs3d_set_solid_material1d(axs3d_ctx *ctx, const double *params) {
-    return guarded(ctx, [&](axs3d::DomainBase &d) {
+    return setup(ctx, "Domain::set_solid_material1d", [&](axs3d::DomainBase &d) {
         d.setSolidMaterial1D(params);
     });
 }
 
 int axs3d_set_solid_material3d(axs3d_ctx *ctx, int e_begin, int e_end,
                                const double *params) {
-    return guarded(ctx, [&](axs3d::DomainBase &d) {
+    return setup(ctx, "Domain::set_solid_material3d", [&](axs3d::DomainBase &d) {
         d.setSolidMaterial3D(e_begin, e_end, params);
     });
 }
 
 int axs3d_set_rtz_theta(axs3d_ctx *ctx, const double *theta) {
-    return guarded(ctx, [&](axs3d::DomainBase &d) { d.setRtzTheta(theta); });
+    return setup(ctx, "Domain::set_rtz_theta", [&](axs3d::DomainBase &d) { d.setRtzTheta(theta); });
 }
 
 int axs3d_set_attenuation1d(axs3d_ctx *ctx, const double *dlam_dmu) {
-    return guarded(ctx, [&](axs3d::DomainBase &d) {
+    return setup(ctx, "Domain::set_attenuation1d", [&](axs3d::DomainBase &d) {
         d.setAttenuation1D(dlam_dmu);
     });
 }
 
 int axs3d_set_axial_points(axs3d_ctx *ctx, int n, const int32_t *points) {
-    return guarded(ctx,
+    return setup(ctx, "Domain::set_axial_points",
                    [&](axs3d::DomainBase &d) { d.setAxialPoints(n, points); });
 }
 
 int axs3d_set_fluid_kinds(axs3d_ctx *ctx, const uint8_t *kind) {
-    return guarded(ctx,
+    return setup(ctx, "Domain::set_fluid_kinds",
                    [&](axs3d::DomainBase &d) { d.setFluidKinds(kind); });
 }
 
 int axs3d_set_fluid_k3d(axs3d_ctx *ctx, int e_begin, int e_end,
                         const double *k) {
-    return guarded(ctx, [&](axs3d::DomainBase &d) {
+    return setup(ctx, "Domain::set_fluid_k3d", [&](axs3d::DomainBase &d) {
         d.setFluidK3D(e_begin, e_end, k);
     });
 }
 
 int axs3d_set_solid_mass3d(axs3d_ctx *ctx, int n, const int32_t *points,
                            const double *mass) {
-    return guarded(ctx, [&](axs3d::DomainBase &d) {
+    return setup(ctx, "Domain::set_solid_mass3d", [&](axs3d::DomainBase &d) {
         d.setSolidMass3D(n, points, mass);
     });
 }
@@ -214,14 +235,14 @@ int axs3d_set_solid_mass3d(axs3d_ctx *ctx, int n, const int32_t *points,
 int axs3d_set_moment_source(axs3d_ctx *ctx, int elem, int nu1,
                             const double *pattern, int nsteps,
                             const double *stf) {
-    return guarded(ctx, [&](axs3d::DomainBase &d) {
+    return setup(ctx, "Domain::set_moment_source", [&](axs3d::DomainBase &d) {
         d.setMomentSource(elem, nu1, pattern, nsteps, stf);
     });
 }
 
 int axs3d_set_fluid_mass3d(axs3d_ctx *ctx, int n, const int32_t *points,
                            const double *mass) {
-    return guarded(ctx, [&](axs3d::DomainBase &d) {
+    return setup(ctx, "Domain::set_fluid_mass3d", [&](axs3d::DomainBase &d) {
         d.setFluidMass3D(n, points, mass);
     });
 }
@@ -229,7 +250,7 @@ int axs3d_set_fluid_mass3d(axs3d_ctx *ctx, int n, const int32_t *points,
 int axs3d_set_coupling3d(axs3d_ctx *ctx, int n, const int32_t *pairs,
                          const double *n_unassembled,
                          const double *n_assembled) {
-    return guarded(ctx, [&](axs3d::DomainBase &d) {
+    return setup(ctx, "Domain::set_coupling3d", [&](axs3d::DomainBase &d) {
         d.setCoupling3D(n, pairs, n_unassembled, n_assembled);
     });
 }
@@ -237,14 +258,14 @@ int axs3d_set_coupling3d(axs3d_ctx *ctx, int n, const int32_t *pairs,
 int axs3d_add_moment_source(axs3d_ctx *ctx, int elem, int nu1,
                             const double *pattern, int nsteps,
                             const double *stf) {
-    return guarded(ctx, [&](axs3d::DomainBase &d) {
+    return setup(ctx, "Domain::add_moment_source", [&](axs3d::DomainBase &d) {
         d.addMomentSource(elem, nu1, pattern, nsteps, stf);
     });
 }
 
 int axs3d_set_instances(axs3d_ctx *ctx, int n, const int32_t *point_inst,
                         const double *dt) {
-    return guarded(ctx, [&](axs3d::DomainBase &d) {
+    return setup(ctx, "Domain::set_instances", [&](axs3d::DomainBase &d) {
         d.setInstances(n, point_inst, dt);
     });
 }
@@ -252,17 +273,17 @@ int axs3d_set_instances(axs3d_ctx *ctx, int n, const int32_t *point_inst,
 int axs3d_set_receivers(axs3d_ctx *ctx, int n, const int32_t *elem,
                         const double *weights, const double *phi,
                         int max_samples) {
-    return guarded(ctx, [&](axs3d::DomainBase &d) {
+    return setup(ctx, "Domain::set_receivers", [&](axs3d::DomainBase &d) {
         d.setReceivers(n, elem, weights, phi, max_samples);
     });
 }
 
 int axs3d_set_dt(axs3d_ctx *ctx, double dt) {
-    return guarded(ctx, [&](axs3d::DomainBase &d) { d.setDt(dt); });
+    return setup(ctx, "Domain::set_dt", [&](axs3d::DomainBase &d) { d.setDt(dt); });
 }
 
 int axs3d_finalize(axs3d_ctx *ctx) {
-    return guarded(ctx, [&](axs3d::DomainBase &d) { d.finalize(); });
+    return setup(ctx, "Domain::finalize", [&](axs3d::DomainBase &d) { d.finalize(); });
 }
 
 int axs3d_apply_sources(axs3d_ctx *ctx, int tstep) {

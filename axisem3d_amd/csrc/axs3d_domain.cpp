@@ -1023,8 +1023,13 @@ void DomainBase::finalize() {
             up(dCplNM, cNM.data(), cNM.size() * 8);
         }
     }
-    dFlag.alloc(sizeof(int));
-    dFlag.zero(st);
+    {
+        // [0] raised on a non-finite displacement, [1] the first step
+        // after which it was up
+        const int flag0[2] = {0, -1};
+        dFlag.alloc(sizeof(flag0));
+        dFlag.upload(flag0, sizeof(flag0), st);
+    }
     dStepCtr.alloc(2 * sizeof(int));
     dStepCtr.zero(st);
     uploadReal();
@@ -1052,16 +1057,18 @@ void DomainBase::recordOutput(int isample, bool dev) {
 
 void DomainBase::checkStability(int tstep) {
     requireFinalized("Domain::checkStability");
-    int flag = 0;
-    hip_check(hipMemcpyAsync(&flag, dFlag.as<int>(), sizeof(int),
+    int flag[2] = {0, -1};
+    hip_check(hipMemcpyAsync(flag, dFlag.as<int>(), sizeof(flag),
                              hipMemcpyDeviceToHost, mStream),
               "Domain::checkStability");
     hip_check(hipStreamSynchronize(mStream), "Domain::checkStability");
-    if (flag) {
+    if (flag[0]) {
+        // the step whose update first left a non-finite displacement
+        // (the steps of a batch after it ran on and are void)
         std::ostringstream ss;
         ss << "Domain::checkStability || Simulation has blown up, with dt = "
            << mDt << " s. || When the instability occurred: || "
-           << "* current step  =  " << tstep;
+           << "* current step  =  " << (flag[1] >= 0 ? flag[1] : tstep);
         throw Error(ss.str());
     }
 }
@@ -1618,7 +1625,10 @@ public:
         const int all = PHASE_GATHER | PHASE_AXIAL | PHASE_MASS | PHASE_MARCH;
         fluidPoint(all | PHASE_COUPLE);
         point(all | PHASE_COUPLE);
-        if (dev) launch_step_advance(mStream, dStepCtr.as<int>());
+        if (dev)
+            launch_step_advance(mStream, dStepCtr.as<int>(), dFlag.as<int>());
+        else
+            launch_flag_note(mStream, dFlag.as<int>(), tstep);
     }
 
     // Discrete adjoint of the bulk step (see axs3d_adjoint.hip for the

@@ -661,8 +661,17 @@ __global__ void k_step_set(int *ctr, int tstep, int record_step0) {
     }
 }
 
-__global__ void k_step_advance(int *ctr) {
-    if (threadIdx.x == 0) ctr[0] += 1;
+// flag[0]: raised by the point kernels on a non-finite displacement;
+// flag[1]: the first step after which it was up (-1: none yet).
+__global__ void k_step_advance(int *ctr, int *flag) {
+    if (threadIdx.x == 0) {
+        if (flag[0] != 0 && flag[1] < 0) flag[1] = ctr[0];
+        ctr[0] += 1;
+    }
+}
+
+__global__ void k_flag_note(int *flag, int tstep) {
+    if (threadIdx.x == 0 && flag[0] != 0 && flag[1] < 0) flag[1] = tstep;
 }
 
 void launch_step_set(hipStream_t st, int *ctr, int tstep, int record_step0) {
@@ -670,8 +679,12 @@ void launch_step_set(hipStream_t st, int *ctr, int tstep, int record_step0) {
                        record_step0);
 }
 
-void launch_step_advance(hipStream_t st, int *ctr) {
-    hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, st, ctr);
+void launch_step_advance(hipStream_t st, int *ctr, int *flag) {
+    hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, st, ctr, flag);
+}
+
+void launch_flag_note(hipStream_t st, int *flag, int tstep) {
+    hipLaunchKernelGGL(k_flag_note, dim3(1), dim3(64), 0, st, flag, tstep);
 }
 
 #define AXS3D_INSTANTIATE(Real)                                               \

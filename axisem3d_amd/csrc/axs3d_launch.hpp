@@ -348,6 +348,8 @@ void launch_adjoint_source(hipStream_t st, int nrec, const int32_t *rec_elem,
 
 // step counter of a replayed (graph) step; see k_step_set
 void launch_step_set(hipStream_t st, int *ctr, int tstep, int record_step0);
-void launch_step_advance(hipStream_t st, int *ctr);
+void launch_step_advance(hipStream_t st, int *ctr, int *flag);
+// eager steps: note tstep as the first blown-up step once flag[0] is up
+void launch_flag_note(hipStream_t st, int *flag, int tstep);
 
 }  // namespace axs3d

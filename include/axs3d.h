@@ -20,6 +20,13 @@
  *    axs3d_last_error().
  *  - One host thread per context.  A context owns its HIP stream and all
  *    its device memory; host arrays may be freed after each call returns.
+ *  - The axs3d_set_* and axs3d_add_* calls describe the domain before
+ *    axs3d_finalize (except axs3d_set_graph, axs3d_set_record_origin and
+ *    axs3d_set_field, which act on a finalized domain).  Once finalized,
+ *    the domain is immutable in shape, as the reference's is after its
+ *    preloop: a second axs3d_finalize or a set-up call returns 1.
+ *  - A kernel launch the runtime rejects makes the call return 1 with
+ *    the HIP error in the message.
  *  - Complex arrays are interleaved (re, im) doubles.
  *  - Point fields are [mode][3] complex per point, points concatenated in
  *    index order, mode count Nr/2+1 per point (SolidPoint::Fields,

@@ -82,3 +82,45 @@ def test_chain_rule3d_matches_finite_differences():
         fd = (J(mp) - J(mm)) / (2 * h)
         ad = np.sum(d[which] * dirn)
         assert abs(fd - ad) < 1e-6 * abs(ad), which
+
+
+def test_with_model_rewrites_general_material_arrays():
+    """A spec built by make_media (all isotropic) is uploaded from mat1d
+    and a flat prop3d; with_model must write the new (lambda, mu) there
+    too, else the stiffness keeps the old model while the mass takes the
+    new density.  Non-isotropic media are refused."""
+    sp = _spec()
+    E = sp.n_solid_elements
+    preloop.make_media(sp, np.zeros(E, dtype=np.uint8),
+                       field=lambda x, y, z: 0.01 * np.cos(x / 700.0),
+                       mask3d=(np.arange(E) % 2) == 1)
+    rho, vp, vs = adjoint.model_of(sp)
+    vp3, vs3 = adjoint.model3d_of(sp)
+    rng = np.random.default_rng(2)
+    r2 = rho * (1 + 0.01 * rng.standard_normal(rho.shape))
+    p2 = vp * (1 + 0.01 * rng.standard_normal(rho.shape))
+    s2 = vs * (1 + 0.01 * rng.standard_normal(rho.shape))
+    v3 = vs3 * 1.02
+    sp2 = adjoint.with_model(sp, r2, p2, s2, vp3, v3)
+    off = np.asarray(sp2.mat1d_off)
+    for e in range(E):
+        if sp.elem_kind[e] == 0:
+            got = sp2.mat1d[off[e]:off[e] + 50].reshape(2, 25)
+            lam = r2[e] * (p2[e] ** 2 - 2 * s2[e] ** 2)
+            assert np.allclose(got[0], lam, rtol=1e-14, atol=0)
+            assert np.allclose(got[1], r2[e] * s2[e] ** 2, rtol=1e-14, atol=0)
+    assert sp2.prop3d.shape == np.asarray(sp.prop3d).shape
+    p3 = sp2.prop3d.reshape(-1, 2, 25)
+    r3 = adjoint._slice_rho(sp, r2)
+    assert np.allclose(p3[:, 1], r3 * v3 ** 2, rtol=1e-14, atol=0)
+    # the original spec is left unchanged
+    assert np.array_equal(np.asarray(sp.mat1d)[off[0]:off[0] + 50],
+                          sp.elem_iso[0].reshape(-1))
+    ti = _spec()
+    preloop.make_media(ti, np.ones(ti.n_solid_elements, dtype=np.uint8))
+    try:
+        adjoint.with_model(ti, *adjoint.model_of(ti))
+    except ValueError:
+        pass
+    else:
+        raise AssertionError("TI media must be refused")

@@ -80,3 +80,38 @@ def test_free_oscillation_without_source_or_receivers(prec, tol):
     err = _rel(got, o.displ)
     print(f"free oscillation {prec}: displacement L2 {err:.3e}")
     assert err < tol
+
+
+def test_setup_calls_after_finalize_are_refused():
+    """The device buffers, the captured step graph and the adjoint caches
+    are built once by finalize: a second finalize, or any set-up call
+    after it, returns an error instead of leaving them stale, and the
+    context keeps running."""
+    import ctypes as C
+    from axisem3d_amd import capi
+    from axisem3d_amd.solver import GpuSolver
+    spec = _small_nr_spec(20, False)
+    g = GpuSolver(spec, "fp64")
+    L = capi.load()
+    g.run(5)
+    assert L.axs3d_finalize(g.ctx) != 0
+    assert b"already finalized" in L.axs3d_last_error(g.ctx)
+    pat = np.ascontiguousarray(spec.src_pattern, dtype=np.complex128)
+    stf = np.ascontiguousarray(spec.stf, dtype=np.float64)
+    assert L.axs3d_add_moment_source(
+        g.ctx, int(spec.src_elem), pat.shape[0], pat.ctypes.data, len(stf),
+        stf.ctypes.data) != 0
+    assert L.axs3d_set_dt(g.ctx, C.c_double(1e-3)) != 0
+    mass = np.ones(len(spec.solid_nr))
+    nr = np.ascontiguousarray(spec.solid_nr, dtype=np.int32)
+    assert L.axs3d_set_solid_points(g.ctx, len(nr), nr.ctypes.data,
+                                    mass.ctypes.data) != 0
+    # the domain is unchanged: it runs on and matches a fresh run
+    g.run(5)
+    got = g.seismograms()
+    g.close()
+    h = GpuSolver(spec, "fp64")
+    h.run(10)
+    ref = h.seismograms()
+    h.close()
+    assert np.array_equal(got, ref)

@@ -148,3 +148,16 @@ def test_stability_flag_raises():
     from axisem3d_amd.capi import Axs3dError
     with pytest.raises(Axs3dError, match="checkStability"):
         g.run(1)
+    g.close()
+    # the message names the step whose update first went non-finite, not
+    # the end of the batch (NewmarkTimeScheme.cpp:72-74 checks per step)
+    for graph in (True, False):
+        g = _solver(spec, "fp64")
+        g.use_graph(graph)
+        g.run(3)
+        u = g.get_field("displ")
+        u[5, 1] = np.nan
+        g.set_field("displ", u)
+        with pytest.raises(Axs3dError, match=r"current step  =  3\b"):
+            g.run(10)
+        g.close()
