@@ -288,18 +288,6 @@ def moment_pattern_axial(mt6, inplane, nu1_elem):
     return pat
 
 
-def moment_pattern_offaxis(mt6_zsp, inplane, nu1_elem, phi):
-    """Pattern of a non-axial source (``Mechanism.cpp:146-160``); mt6_zsp is
-    already rotated into (z, s, phi)."""
-    mzz, mss, mpp, mzs, mzp, msp = [float(v) for v in mt6_zsp]
-    pat = np.zeros((nu1_elem, 6, NPEM), dtype=np.complex128)
-    for a in range(nu1_elem):
-        e = np.exp(-1j * a * phi)
-        for k, m in enumerate([mss, mpp, mzz, mzp, mzs, msp]):
-            pat[a, k] = e * m * inplane
-    return pat
-
-
 # --------------------------------------------------------------- the spec
 @dataclasses.dataclass
 class DomainSpec:
@@ -579,6 +567,7 @@ def make_fluid(spec, fluid_elem):
     # fluid arrays
     spec.fluid_nr = nr_grid[flu_ids].astype(np.int32)
     spec.fluid_mass = f_mass
+    spec.fluid_sz = sz_grid[flu_ids]
     spec.fluid_elem_points = f_points
     spec.fluid_elem_axial = axial[fe].astype(np.uint8)
     spec.fluid_elem_gq = np.ascontiguousarray(spec.elem_gq[
