@@ -86,6 +86,7 @@ SIGNATURES = {
     "axs3d_profile_read": (_i, [_p, _i, C.POINTER(C.c_double),
                                 C.POINTER(C.c_int64)]),
     "axs3d_profile_reset": (_i, [_p]),
+    "axs3d_fft_check": (_i, [_i, _i, _i, _p, C.POINTER(_i), C.POINTER(_i)]),
     "axs3d_autotune_info": (_i, [_p, C.POINTER(_i), C.POINTER(_i),
                                  C.POINTER(C.c_double)]),
 }

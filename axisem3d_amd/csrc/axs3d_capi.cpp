@@ -5,6 +5,7 @@
 
 #include "axs3d.h"
 #include "axs3d_domain.hpp"
+#include "axs3d_fft.hpp"
 
 struct axs3d_ctx {
     std::unique_ptr<axs3d::DomainBase> dom;
@@ -397,6 +398,37 @@ int axs3d_autotune_info(const axs3d_ctx *ctx, int *cardinal_occupancy,
     if (!ctx || !cardinal_occupancy || !cardinal_blocks || !trial_ms3)
         return 2;
     ctx->dom->autotuneInfo(cardinal_occupancy, cardinal_blocks, trial_ms3);
+    return 0;
+}
+
+int axs3d_fft_check(int nr, int max_radix, int inverse, double *data,
+                    int *radices, int *npasses) {
+    if (!data || !radices || !npasses || nr < 1 || nr > axs3d::FFT_NR_CAP ||
+        max_radix < 2 || max_radix > 16)
+        return 2;
+    using axs3d::cpx;
+    const std::vector<int> rad = axs3d::fft_choose_radices(nr, max_radix);
+    if (rad.empty() && nr > 1) return 1;
+    std::vector<uint32_t> plan(axs3d::FFT_PLAN_W);
+    std::vector<double> tw;
+    std::vector<uint16_t> pos;
+    axs3d::fft_build_plan(nr, rad, plan.data(), tw, pos);
+    std::vector<cpx<double>> twc(tw.size() / 2);
+    for (size_t i = 0; i < twc.size(); i++) twc[i] = {tw[2 * i], tw[2 * i + 1]};
+    std::vector<cpx<double>> buf(axs3d::padded(nr));
+    cpx<double> *x = reinterpret_cast<cpx<double> *>(data);
+    if (inverse) {
+        for (int k = 0; k < nr; k++) buf[axs3d::pad(k)] = x[k];
+        axs3d::fft_host_run<double>(buf.data(), plan.data(), true, twc.data());
+        for (int k = 0; k < nr; k++) x[k] = buf[axs3d::pad(pos[k])];
+    } else {
+        for (int k = 0; k < nr; k++) buf[axs3d::pad(pos[k])] = x[k];
+        axs3d::fft_host_run<double>(buf.data(), plan.data(), false,
+                                    twc.data());
+        for (int k = 0; k < nr; k++) x[k] = buf[axs3d::pad(k)];
+    }
+    *npasses = (int)rad.size();
+    for (size_t i = 0; i < rad.size(); i++) radices[i] = rad[i];
     return 0;
 }
 

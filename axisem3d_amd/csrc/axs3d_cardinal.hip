@@ -35,248 +35,33 @@
 // Each kernel gets the register budget of its own phase.
 #include "axs3d_common.hpp"
 #include "axs3d_element.hpp"
+#include "axs3d_fft.hpp"
 #include "axs3d_launch.hpp"
 
 namespace axs3d {
+
+// dynamic LDS of the FFT kernels (layout of FftLds, sized per launch)
+extern __shared__ __align__(16) unsigned char fft_lds[];
+
+// The block's FFT tables of its Nr (twiddle powers and slice positions)
+// into its LDS.
+template <typename Real>
+__device__ __forceinline__ void fft_load_tables(
+    const FftLds<Real> &L, const cpx<Real> *__restrict__ twiddle,
+    const int64_t *__restrict__ tw_off, const uint16_t *__restrict__ spos,
+    const int64_t *__restrict__ pos_off, const uint32_t *plan, int nr,
+    int t, int nthreads) {
+    const cpx<Real> *twg = twiddle + tw_off[nr];
+    const int ntw = (int)(plan[0] >> 8);
+    for (int x = t; x < ntw; x += nthreads) L.tw[x] = twg[x];
+    const uint16_t *pg = spos + pos_off[nr];
+    for (int x = t; x < nr; x += nthreads) L.pos[x] = pg[x];
+}
 
 constexpr int CD_EMB = 16;             // gradient/quadrature: element-modes
 constexpr int CD_EMB_THREADS = CD_EMB * NPEM;  // per block
 constexpr int CD_PW = 3;               // stress kernel: waves per point
 constexpr int CD_PT = CD_PW * 64;      // threads per stress block
-
-template <typename T>
-__device__ __forceinline__ cpx<T> cmulc(cpx<T> a, cpx<T> b) {
-    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
-}
-
-// LDS index of element i of a wave's series: one complex of padding every
-// 16, so that the strided accesses of the early Stockham stages (stride R
-// or runs of Ns < 16) spread over all banks.
-__device__ __forceinline__ int pad(int i) { return i + (i >> 4); }
-
-// In-register R-point DFT of x (natural order in and out): forward W =
-// exp(-2 pi i / R), inverse (inv) its conjugate, unscaled.
-template <typename Real>
-__device__ __forceinline__ void dft4(cpx<Real> &x0, cpx<Real> &x1,
-                                     cpx<Real> &x2, cpx<Real> &x3,
-                                     Real sg) {
-    const cpx<Real> s02 = cadd(x0, x2), d02 = csub(x0, x2);
-    const cpx<Real> s13 = cadd(x1, x3), d13 = csub(x1, x3);
-    // (-i) d13 forward (sg = -1), (+i) d13 inverse (sg = +1)
-    const cpx<Real> rd13 = {-sg * d13.im, sg * d13.re};
-    x0 = cadd(s02, s13);
-    x2 = csub(s02, s13);
-    x1 = cadd(d02, rd13);
-    x3 = csub(d02, rd13);
-}
-
-// cos and sin of 2 pi j / R, j = 1 .. (R-1)/2, for the odd prime radices.
-template <int R>
-struct PrimeTab;
-template <>
-struct PrimeTab<7> {
-    static constexpr double c[3] = {0.6234898018587336, -0.22252093395631434, -0.900968867902419};
-    static constexpr double s[3] = {0.7818314824680298, 0.9749279121818236, 0.43388373911755823};
-};
-
-template <>
-struct PrimeTab<11> {
-    static constexpr double c[5] = {0.8412535328311812, 0.41541501300188644, -0.142314838273285, -0.654860733945285, -0.9594929736144974};
-    static constexpr double s[5] = {0.5406408174555976, 0.9096319953545183, 0.9898214418809328, 0.7557495743542583, 0.28173255684142967};
-};
-
-template <>
-struct PrimeTab<13> {
-    static constexpr double c[6] = {0.8854560256532099, 0.5680647467311559, 0.120536680255323, -0.35460488704253545, -0.7485107481711012, -0.970941817426052};
-    static constexpr double s[6] = {0.4647231720437685, 0.8229838658936564, 0.992708874098054, 0.9350162426854148, 0.6631226582407952, 0.23931566428755768};
-};
-
-template <typename Real, int R>
-__device__ __forceinline__ void dft(cpx<Real> (&x)[R], bool inv) {
-    const Real sg = inv ? (Real)1 : (Real)-1;
-    if constexpr (R == 2) {
-        const cpx<Real> a = x[0];
-        x[0] = cadd(a, x[1]);
-        x[1] = csub(a, x[1]);
-    } else if constexpr (R == 4) {
-        dft4<Real>(x[0], x[1], x[2], x[3], sg);
-    } else if constexpr (R == 8) {
-        // radix-2 split x_r +- x_{r+4}, W_8^r on the differences, then two
-        // 4-point DFTs: even outputs from the sums, odd from the
-        // differences
-        constexpr Real h = (Real)0.70710678118654752440;
-        cpx<Real> a[4], b[4];
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            a[r] = cadd(x[r], x[r + 4]);
-            b[r] = csub(x[r], x[r + 4]);
-        }
-        // W_8 = (1 + sg i) / sqrt 2, W_8^2 = sg i, W_8^3 = (-1 + sg i) /
-        // sqrt 2 (forward sg = -1)
-        b[1] = {(b[1].re - sg * b[1].im) * h, (b[1].im + sg * b[1].re) * h};
-        b[2] = {-sg * b[2].im, sg * b[2].re};
-        b[3] = {(-b[3].re - sg * b[3].im) * h, (-b[3].im + sg * b[3].re) * h};
-        dft4<Real>(a[0], a[1], a[2], a[3], sg);
-        dft4<Real>(b[0], b[1], b[2], b[3], sg);
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            x[2 * q] = a[q];
-            x[2 * q + 1] = b[q];
-        }
-    } else if constexpr (R == 3) {
-        const Real c1 = (Real)-0.5;
-        const Real s1 = sg * (Real)0.86602540378443864676;
-        const cpx<Real> tt = cadd(x[1], x[2]);
-        const cpx<Real> d = csub(x[1], x[2]);
-        const cpx<Real> mm = cfma(tt, c1, x[0]);
-        const cpx<Real> id = {-s1 * d.im, s1 * d.re};
-        x[0] = cadd(x[0], tt);
-        x[1] = cadd(mm, id);
-        x[2] = csub(mm, id);
-    } else if constexpr (R == 5) {
-        const Real c1 = (Real)0.30901699437494742410;   // cos 2pi/5
-        const Real c2 = (Real)-0.80901699437494742410;  // cos 4pi/5
-        const Real s1 = sg * (Real)0.95105651629515357212;
-        const Real s2 = sg * (Real)0.58778525229247312917;
-        const cpx<Real> t1 = cadd(x[1], x[4]), d1 = csub(x[1], x[4]);
-        const cpx<Real> t2 = cadd(x[2], x[3]), d2 = csub(x[2], x[3]);
-        const cpx<Real> m1 = cadd(x[0], cadd(cmul(t1, c1), cmul(t2, c2)));
-        const cpx<Real> m2 = cadd(x[0], cadd(cmul(t1, c2), cmul(t2, c1)));
-        // i (s1 d1 + s2 d2) and i (s2 d1 - s1 d2)
-        const cpx<Real> e1 = cadd(cmul(d1, s1), cmul(d2, s2));
-        const cpx<Real> e2 = csub(cmul(d1, s2), cmul(d2, s1));
-        const cpx<Real> ie1 = {-e1.im, e1.re}, ie2 = {-e2.im, e2.re};
-        x[0] = cadd(x[0], cadd(t1, t2));
-        x[1] = cadd(m1, ie1);
-        x[4] = csub(m1, ie1);
-        x[2] = cadd(m2, ie2);
-        x[3] = csub(m2, ie2);
-    } else {
-        // odd prime R: with t_r = x_r + x_{R-r}, d_r = x_r - x_{R-r},
-        // X_q = x_0 + sum_r cos(2 pi r q / R) t_r + sg i sum_r
-        // sin(2 pi r q / R) d_r and X_{R-q} the same with -sg
-        constexpr int H = (R - 1) / 2;
-        cpx<Real> t[H], d[H];
-#pragma unroll
-        for (int r = 1; r <= H; r++) {
-            t[r - 1] = cadd(x[r], x[R - r]);
-            d[r - 1] = csub(x[r], x[R - r]);
-        }
-        const cpx<Real> x0 = x[0];
-        cpx<Real> sum = x0;
-#pragma unroll
-        for (int r = 0; r < H; r++) sum = cadd(sum, t[r]);
-        x[0] = sum;
-#pragma unroll
-        for (int q = 1; q <= H; q++) {
-            cpx<Real> A = x0, B = {0, 0};
-#pragma unroll
-            for (int r = 1; r <= H; r++) {
-                const int j = (r * q) % R;
-                // cos is even in j, sin odd: fold j > H onto R - j
-                const Real c = (Real)PrimeTab<R>::c[(j <= H ? j : R - j) - 1];
-                const Real s = j <= H ? (Real)PrimeTab<R>::s[j - 1]
-                                      : -(Real)PrimeTab<R>::s[R - j - 1];
-                A = cfma(t[r - 1], c, A);
-                B = cfma(d[r - 1], s, B);
-            }
-            // sg i B
-            const cpx<Real> iB = {-sg * B.im, sg * B.re};
-            x[q] = cadd(A, iB);
-            x[R - q] = csub(A, iB);
-        }
-    }
-}
-
-// x_r *= w^r, r = 1 .. R-1, each power the product of two lower ones
-// (w^r = w^{2^m} w^{r - 2^m}, or the square of w^{r/2}), so that a power
-// is at most 2 log2 r products away from w.
-template <typename Real, int R>
-__device__ __forceinline__ void twiddle_powers(cpx<Real> (&x)[R],
-                                               cpx<Real> w) {
-    cpx<Real> p[R];
-    p[1] = w;
-#pragma unroll
-    for (int r = 2; r < R; r++) {
-        int hi = 1;
-        while (hi * 2 <= r) hi *= 2;
-        p[r] = hi == r ? cmulc(p[r / 2], p[r / 2]) : cmulc(p[hi], p[r - hi]);
-    }
-#pragma unroll
-    for (int r = 1; r < R; r++) x[r] = cmulc(x[r], p[r]);
-}
-
-// One stage of radix R of an in-place mixed-radix FFT of the series
-// buf[0..n) held by one wave.  Stage of span L = R m: butterfly b < n/R
-// takes the elements j0 + j + r m, r < R, with j0 = (b / m) L and j = b %
-// m, and writes its outputs back to the same elements, so the lanes never
-// touch each other's data within a stage.  Decimation in frequency
-// (Sande-Tukey; the inverse transform, stages in plan order): R-point
-// DFT, then output q times w_j^q; decimation in time (Cooley-Tukey; the
-// forward transform, stages in reverse order): input r times w_j^r, then
-// the DFT.  w_j = exp(-/+ 2 pi i j / L) from the stage's table tws[j], j
-// < m.  b / m comes from the magic multiplier mg = ceil(2^32 / m), exact
-// for b m < 2^32 (m = 1: mg unused).
-template <typename Real, int R>
-__device__ __forceinline__ void wstage(cpx<Real> *buf, int nb, int m,
-                                       uint32_t mg, bool inv,
-                                       const cpx<Real> *tws, int lane) {
-    const int L = m * R;
-#pragma unroll 1
-    for (int b = lane; b < nb; b += 64) {
-        // index products stay below 2^24: full-rate 24-bit multiplies
-        const int blk = m == 1 ? b : (int)__umulhi((uint32_t)b, mg);
-        const int j = b - __mul24(blk, m);
-        const int base = __mul24(blk, L) + j;
-        cpx<Real> v[R];
-#pragma unroll
-        for (int r = 0; r < R; r++) v[r] = buf[pad(base + r * m)];
-        // tws[0] = 1 exactly, so lane j = 0 multiplies by one; only the
-        // m = 1 stage (all j = 0) skips the twiddles
-        cpx<Real> w = {1, 0};
-        if (m > 1) {
-            w = tws[j];
-            if (inv) w.im = -w.im;
-        }
-        if (!inv && m > 1) twiddle_powers<Real, R>(v, w);
-        dft<Real, R>(v, inv);
-        if (inv && m > 1) twiddle_powers<Real, R>(v, w);
-#pragma unroll
-        for (int q = 0; q < R; q++) buf[pad(base + q * m)] = v[q];
-    }
-}
-
-// Whole FFT of buf[0..n) by one wave.  Plan word 0: number of stages |
-// twiddle table length << 8; per stage s, in decimation-in-frequency
-// order: radix | m << 8, the magic multiplier of m, the offset of the
-// stage's twiddles in tw.  The inverse transform (DIF) takes the natural
-// order and leaves slice s at position pos[s] (digit reversal); the
-// forward transform (DIT) takes that order and leaves the natural one.
-template <typename Real>
-__device__ __forceinline__ void wfft(cpx<Real> *buf, int n,
-                                     const uint32_t *__restrict__ plan,
-                                     bool inv, const cpx<Real> *tw, int lane) {
-    const int nst = (int)(plan[0] & 255u);
-    for (int i = 0; i < nst; i++) {
-        const int st = inv ? i : nst - 1 - i;
-        const uint32_t w0 = plan[1 + 3 * st];
-        const uint32_t mg = plan[2 + 3 * st];
-        const cpx<Real> *tws = tw + plan[3 + 3 * st];
-        const int R = (int)(w0 & 255u), m = (int)(w0 >> 8);
-        const int nb = n / R;
-        switch (R) {
-            case 8: wstage<Real, 8>(buf, nb, m, mg, inv, tws, lane); break;
-            case 4: wstage<Real, 4>(buf, nb, m, mg, inv, tws, lane); break;
-            case 2: wstage<Real, 2>(buf, nb, m, mg, inv, tws, lane); break;
-            case 3: wstage<Real, 3>(buf, nb, m, mg, inv, tws, lane); break;
-            case 5: wstage<Real, 5>(buf, nb, m, mg, inv, tws, lane); break;
-            case 7: wstage<Real, 7>(buf, nb, m, mg, inv, tws, lane); break;
-            case 11: wstage<Real, 11>(buf, nb, m, mg, inv, tws, lane); break;
-            default: wstage<Real, 13>(buf, nb, m, mg, inv, tws, lane); break;
-        }
-        wave_sync();
-    }
-}
 
 // C2R input of the strain pair (X, Y) = rows (2q, 2q+1) of a point:
 // buf[k] = Xfull[k] + i Yfull[k], with Xfull the Hermitian extension under
@@ -397,17 +182,6 @@ __device__ __forceinline__ void att_update(double *mvk, Real *dsk, int cs,
     sig = (Real)sd;
     dsk[od] = dsn;
 }
-
-// Block LDS of the stress kernel: the stage twiddles of the element's Nr
-// and one padded complex series of Nr per wave (FFT buffer).
-constexpr int CD_BUF = CARD_NR_CAP + (CARD_NR_CAP >> 4);
-template <typename Real>
-struct CdLds {
-    cpx<Real> tw[CARD_TW_CAP];
-    cpx<Real> wbuf[CD_PW][CD_BUF];
-    double abg[3 * CD_MAX_SLS];    // SLS alpha, beta, gamma
-    uint16_t pos[CARD_NR_CAP];     // buffer position of slice k
-};
 
 // A: gradient of the element-modes [em0, em1) of the 3D element-mode list
 // into the scratch rows [d*25 + ipnt][alpha] of each element, rotated to
@@ -604,9 +378,9 @@ __device__ __forceinline__ void cd_stress_slices(const CardArgs<Real> &a,
 // from the arguments).  25 blocks per element of the batch list3d[w0 ..),
 // one per GLL point.
 template <typename Real, int NS, int MT>
-__global__ __launch_bounds__(CD_PT, CARD_OCC) void k_cd_stress(
+__global__ __launch_bounds__(CD_PT, CardOcc<Real>::value) void k_cd_stress(
     CardArgs<Real> a, int w0) {
-    __shared__ CdLds<Real> L;
+    __shared__ double abg[3 * CD_MAX_SLS];    // SLS alpha, beta, gamma
     const int t = threadIdx.x;
     const int lane = t & 63;
     // the wave index is uniform across the wave: keep it (and all that is
@@ -618,30 +392,27 @@ __global__ __launch_bounds__(CD_PT, CARD_OCC) void k_cd_stress(
     const int nu1 = a.s.elem_nu1[e];
     const int nr = a.elem_nr[e];
     const uint32_t *plan = a.plan + (size_t)nr * CARD_PLAN_W;
-    {
-        const cpx<Real> *twg = a.twiddle + a.tw_off[nr];
-        const int ntw = (int)(plan[0] >> 8);
-        for (int x = t; x < ntw; x += CD_PT) L.tw[x] = twg[x];
-        const uint16_t *pg = a.slice_pos + a.pos_off[nr];
-        for (int x = t; x < nr; x += CD_PT) L.pos[x] = pg[x];
-        if (NS != 0 && t < 3 * a.nsls) {
-            const int c = t / a.nsls, l = t - c * a.nsls;
-            L.abg[c * CD_MAX_SLS + l] = a.abg[t];
-        }
+    FftLds<Real> L;
+    L.carve(fft_lds, nr, CD_PW);
+    fft_load_tables<Real>(L, a.twiddle, a.tw_off, a.slice_pos, a.pos_off,
+                          plan, nr, t, CD_PT);
+    if (NS != 0 && t < 3 * a.nsls) {
+        const int c = t / a.nsls, l = t - c * a.nsls;
+        abg[c * CD_MAX_SLS + l] = a.abg[t];
     }
     __syncthreads();
     const bool even = (nr & 1) == 0;
     const size_t rs = (size_t)NPEM * nu1;
     cpx<Real> *r0 = reinterpret_cast<cpx<Real> *>(a.scratch) + a.scr_off[e] +
                     (size_t)pp * nu1;
-    cpx<Real> *b = L.wbuf[q];
+    cpx<Real> *b = L.buf(q);
     pack_pair<Real>(b, r0 + 2 * q * rs, r0 + (2 * q + 1) * rs, nr, nu1, even,
                     lane);
     // both transforms from one call site (the inverse, the stress, then
     // the forward transform), which halves the FFT's code
 #pragma unroll 1
     for (int pass = 0; pass < 2; pass++) {
-        wfft<Real>(b, nr, plan, pass == 0, L.tw, lane);
+        wave_fft<Real>(b, plan, pass == 0, L.tw, lane);
         if (pass == 1) break;
         __syncthreads();
     {
@@ -660,8 +431,8 @@ __global__ __launch_bounds__(CD_PT, CARD_OCC) void k_cd_stress(
                    (size_t)ip4 * nr;
             M.ds = a.dstress + (size_t)sl0 * 6 * npa + (size_t)ip4 * nr;
         }
-        cd_stress_slices<Real, NS, MT>(a, L.wbuf[0], L.wbuf[1], L.wbuf[2],
-                                        L.pos, M, L.abg, att, nr, t);
+        cd_stress_slices<Real, NS, MT>(a, L.buf(0), L.buf(1), L.buf(2),
+                                        L.pos, M, abg, att, nr, t);
     }
         __syncthreads();
     }
@@ -671,16 +442,17 @@ __global__ __launch_bounds__(CD_PT, CARD_OCC) void k_cd_stress(
 
 template <typename Real, int MT>
 static void launch_stress_mt(hipStream_t st, const CardArgs<Real> &a, int w0,
-                             int n) {
+                             int n, int nr_max) {
     const dim3 grid(n * NPEM), block(CD_PT);
+    const size_t lds = fft_lds_bytes<Real>(nr_max, CD_PW);
     if (a.nsls == 0)
-        hipLaunchKernelGGL((k_cd_stress<Real, 0, MT>), grid, block, 0, st, a,
-                           w0);
+        hipLaunchKernelGGL((k_cd_stress<Real, 0, MT>), grid, block, lds, st,
+                           a, w0);
     else if (a.nsls == 5)
-        hipLaunchKernelGGL((k_cd_stress<Real, 5, MT>), grid, block, 0, st, a,
-                           w0);
+        hipLaunchKernelGGL((k_cd_stress<Real, 5, MT>), grid, block, lds, st,
+                           a, w0);
     else
-        hipLaunchKernelGGL((k_cd_stress<Real, -1, MT>), grid, block, 0, st,
+        hipLaunchKernelGGL((k_cd_stress<Real, -1, MT>), grid, block, lds, st,
                            a, w0);
 }
 
@@ -719,9 +491,11 @@ void launch_cd_stress(hipStream_t st, const CardArgs<Real> &a,
     const int n = bt.end - bt.begin;
     if (n <= 0) return;
     switch (bt.mtype) {
-        case 0: launch_stress_mt<Real, 0>(st, a, bt.begin, n); break;
-        case 1: launch_stress_mt<Real, 1>(st, a, bt.begin, n); break;
-        default: launch_stress_mt<Real, 2>(st, a, bt.begin, n); break;
+        case 0: launch_stress_mt<Real, 0>(st, a, bt.begin, n, bt.nr_max); break;
+        case 1: launch_stress_mt<Real, 1>(st, a, bt.begin, n, bt.nr_max); break;
+        default:
+            launch_stress_mt<Real, 2>(st, a, bt.begin, n, bt.nr_max);
+            break;
     }
 }
 
@@ -807,9 +581,6 @@ __global__ __launch_bounds__(CD_EMB_THREADS) void k_cdf_grad(
 template <typename Real>
 __global__ __launch_bounds__(128) void k_cdf_stress(FluidCardArgs<Real> a,
                                                     int w0) {
-    __shared__ cpx<Real> tw[CARD_TW_CAP];
-    __shared__ cpx<Real> wbuf[2][CD_BUF];
-    __shared__ uint16_t pos[CARD_NR_CAP];
     const int t = threadIdx.x;
     const int lane = t & 63;
     const int q = __builtin_amdgcn_readfirstlane(t >> 6);
@@ -819,19 +590,18 @@ __global__ __launch_bounds__(128) void k_cdf_stress(FluidCardArgs<Real> a,
     const int nr = a.elem_nr[e];
     const int nu1 = nr / 2 + 1;
     const uint32_t *plan = a.plan + (size_t)nr * CARD_PLAN_W;
-    {
-        const cpx<Real> *twg = a.twiddle + a.tw_off[nr];
-        const int ntw = (int)(plan[0] >> 8);
-        for (int x = t; x < ntw; x += 128) tw[x] = twg[x];
-        const uint16_t *pg = a.slice_pos + a.pos_off[nr];
-        for (int x = t; x < nr; x += 128) pos[x] = pg[x];
-    }
+    FftLds<Real> L;
+    L.carve(fft_lds, nr, 2);
+    fft_load_tables<Real>(L, a.twiddle, a.tw_off, a.slice_pos, a.pos_off,
+                          plan, nr, t, 128);
+    const cpx<Real> *tw = L.tw;
+    const uint16_t *pos = L.pos;
     __syncthreads();
     const bool even = (nr & 1) == 0;
     const size_t rs = (size_t)NPEM * nu1;
     cpx<Real> *r0 = reinterpret_cast<cpx<Real> *>(a.scratch) + a.scr_off[e] +
                     (size_t)pp * nu1;
-    cpx<Real> *b = wbuf[q];
+    cpx<Real> *b = L.buf(q);
     // wave 1: rows (2, zero); the zero row is read as row 2's imaginary
     // partner, so pack it from row 2 and clear the y part
     if (q == 0) {
@@ -849,7 +619,7 @@ __global__ __launch_bounds__(128) void k_cdf_stress(FluidCardArgs<Real> a,
         }
         wave_sync();
     }
-    wfft<Real>(b, nr, plan, true, tw, lane);
+    wave_fft<Real>(b, plan, true, tw, lane);
     const Real *K = a.k3d + a.k3d_off[e] + (size_t)pp * nr;
     for (int k = lane; k < nr; k += 64) {
         const int pk = pad(pos[k]);
@@ -858,7 +628,7 @@ __global__ __launch_bounds__(128) void k_cdf_stress(FluidCardArgs<Real> a,
         b[pk] = {z.re * kk, z.im * kk};
     }
     wave_sync();
-    wfft<Real>(b, nr, plan, false, tw, lane);
+    wave_fft<Real>(b, plan, false, tw, lane);
     if (q == 0) {
         unpack_pair<Real>(b, r0, r0 + rs, nr, nu1, even, lane);
     } else {
@@ -939,8 +709,8 @@ void launch_cdf_batch(hipStream_t st, const FluidCardArgs<Real> &a,
     if (n <= 0 || gem <= 0) return;
     hipLaunchKernelGGL(k_cdf_grad<Real>, dim3(gem), dim3(CD_EMB_THREADS), 0,
                        st, a, bt.em_begin, bt.em_end);
-    hipLaunchKernelGGL(k_cdf_stress<Real>, dim3(n * NPEM), dim3(128), 0, st,
-                       a, bt.begin);
+    hipLaunchKernelGGL(k_cdf_stress<Real>, dim3(n * NPEM), dim3(128),
+                       fft_lds_bytes<Real>(bt.nr_max, 2), st, a, bt.begin);
     hipLaunchKernelGGL(k_cdf_quad<Real>, dim3(gem), dim3(CD_EMB_THREADS), 0,
                        st, a, bt.em_begin, bt.em_end);
 }
@@ -955,9 +725,6 @@ void launch_cdf_batch(hipStream_t st, const FluidCardArgs<Real> &a,
 template <typename Real, bool MARCH>
 __global__ __launch_bounds__(128) void k_mass3d(Mass3DArgs<Real> a,
                                                 Real *__restrict__ stiff) {
-    __shared__ cpx<Real> tw[CARD_TW_CAP];
-    __shared__ cpx<Real> wbuf[2][CD_BUF];
-    __shared__ uint16_t pos[CARD_NR_CAP];
     const int t = threadIdx.x;
     const int lane = t & 63;
     const int q = __builtin_amdgcn_readfirstlane(t >> 6);
@@ -967,19 +734,18 @@ __global__ __launch_bounds__(128) void k_mass3d(Mass3DArgs<Real> a,
     const int nu1 = nr / 2 + 1;
     const bool even = (nr & 1) == 0;
     const uint32_t *plan = a.plan + (size_t)nr * CARD_PLAN_W;
-    {
-        const cpx<Real> *twg = a.twiddle + a.tw_off[nr];
-        const int ntw = (int)(plan[0] >> 8);
-        for (int x = t; x < ntw; x += 128) tw[x] = twg[x];
-        const uint16_t *pg = a.slice_pos + a.pos_off[nr];
-        for (int x = t; x < nr; x += 128) pos[x] = pg[x];
-    }
+    FftLds<Real> L;
+    L.carve(fft_lds, nr, 2);
+    fft_load_tables<Real>(L, a.twiddle, a.tw_off, a.slice_pos, a.pos_off,
+                          plan, nr, t, 128);
+    const cpx<Real> *tw = L.tw;
+    const uint16_t *pos = L.pos;
     __syncthreads();
     // component c of mode m of the point: stiff complex [off + m][3]
     cpx<Real> *S = reinterpret_cast<cpx<Real> *>(stiff) +
                    (size_t)a.point_off[p] * 3;
     const int cx = q == 0 ? 0 : 2;
-    cpx<Real> *b = wbuf[q];
+    cpx<Real> *b = L.buf(q);
     // C2R input (SolverFFTW conventions, as pack_pair): Hermitian
     // extension of the modes, mean and Nyquist real, Nyquist doubled
     const int nyq = even ? nu1 - 1 : -1;
@@ -995,7 +761,7 @@ __global__ __launch_bounds__(128) void k_mass3d(Mass3DArgs<Real> a,
         b[pad(k)] = {sc * x.re - sim * y.im, sim * x.im + sc * y.re};
     }
     wave_sync();
-    wfft<Real>(b, nr, plan, true, tw, lane);
+    wave_fft<Real>(b, plan, true, tw, lane);
     const Real *im = a.inv_mass + a.inv_off[i];
     for (int k = lane; k < nr; k += 64) {
         const int pk = pad(pos[k]);
@@ -1004,7 +770,7 @@ __global__ __launch_bounds__(128) void k_mass3d(Mass3DArgs<Real> a,
         b[pk] = {z.re * m, z.im * m};
     }
     wave_sync();
-    wfft<Real>(b, nr, plan, false, tw, lane);
+    wave_fft<Real>(b, plan, false, tw, lane);
     // R2C output (as unpack_pair): X into component cx, Y into 1 (wave 0)
     const Real h0 = (Real)0.5 / (Real)nr;
     bool bad = false;
@@ -1046,12 +812,13 @@ template <typename Real>
 void launch_mass3d(hipStream_t st, bool march, const Mass3DArgs<Real> &a,
                    Real *stiff) {
     if (a.n <= 0) return;
+    const size_t lds = fft_lds_bytes<Real>(a.nr_max, 2);
     if (march)
-        hipLaunchKernelGGL((k_mass3d<Real, true>), dim3(a.n), dim3(128), 0,
+        hipLaunchKernelGGL((k_mass3d<Real, true>), dim3(a.n), dim3(128), lds,
                            st, a, stiff);
     else
-        hipLaunchKernelGGL((k_mass3d<Real, false>), dim3(a.n), dim3(128), 0,
-                           st, a, stiff);
+        hipLaunchKernelGGL((k_mass3d<Real, false>), dim3(a.n), dim3(128),
+                           lds, st, a, stiff);
 }
 
 // C2R input of one wave's buffer (SolverFFTW conventions, as pack_pair):
@@ -1103,10 +870,6 @@ __device__ __forceinline__ void r2c_unpack(const cpx<Real> *b, int nr,
 //   MARCH   Newmark (FP64), fluid stiffness zeroed
 template <typename Real, bool COUPLE, bool AXIAL, bool MASS, bool MARCH>
 __global__ __launch_bounds__(128) void k_fluid_defer(FluidDeferArgs<Real> a) {
-    __shared__ cpx<Real> tw[CARD_TW_CAP];
-    __shared__ cpx<Real> wbuf[2][CD_BUF];
-    __shared__ uint16_t pos[CARD_NR_CAP];
-    __shared__ cpx<Real> sf[CARD_NR_CAP / 2 + 1];
     const int t = threadIdx.x;
     const int lane = t & 63;
     const int q = __builtin_amdgcn_readfirstlane(t >> 6);
@@ -1117,18 +880,21 @@ __global__ __launch_bounds__(128) void k_fluid_defer(FluidDeferArgs<Real> a) {
     const bool even = (nr & 1) == 0;
     const uint32_t *plan = a.plan + (size_t)nr * CARD_PLAN_W;
     const size_t foff = a.point_off[p];
+    FftLds<Real> L;
+    L.carve(fft_lds, nr, 2);
+    fft_load_tables<Real>(L, a.twiddle, a.tw_off, a.slice_pos, a.pos_off,
+                          plan, nr, t, 128);
+    const cpx<Real> *tw = L.tw;
+    const uint16_t *pos = L.pos;
+    // the point's fluid stiffness (nu1 complex) after the buffers
+    cpx<Real> *sf = reinterpret_cast<cpx<Real> *>(L.extra);
     {
-        const cpx<Real> *twg = a.twiddle + a.tw_off[nr];
-        const int ntw = (int)(plan[0] >> 8);
-        for (int x = t; x < ntw; x += 128) tw[x] = twg[x];
-        const uint16_t *pg = a.slice_pos + a.pos_off[nr];
-        for (int x = t; x < nr; x += 128) pos[x] = pg[x];
         const cpx<Real> *fs = reinterpret_cast<const cpx<Real> *>(a.fstiff);
         for (int k = t; k < nu1; k += 128) sf[k] = fs[foff + k];
     }
     __syncthreads();
     const cpx<Real> zero = {(Real)0, (Real)0};
-    cpx<Real> *b = wbuf[q];
+    cpx<Real> *b = L.buf(q);
     const int j = COUPLE ? a.pair3d[i] : -1;
     if (COUPLE && j >= 0) {
         const int sp = a.pair_solid[j];
@@ -1146,19 +912,19 @@ __global__ __launch_bounds__(128) void k_fluid_defer(FluidDeferArgs<Real> a) {
                               : zero;
             });
         wave_sync();
-        wfft<Real>(b, nr, plan, true, tw, lane);
+        wave_fft<Real>(b, plan, true, tw, lane);
         __syncthreads();
         const Real *nn = a.nu + a.n_off[j] * 3;
         for (int k = t; k < nr; k += 128) {
             const int pk = pad(pos[k]);
-            const cpx<Real> sphi = wbuf[0][pk];
+            const cpx<Real> sphi = L.buf(0)[pk];
             const Real f = nn[3 * k] * sphi.re + nn[3 * k + 1] * sphi.im +
-                           nn[3 * k + 2] * wbuf[1][pk].re;
-            wbuf[1][pk] = {f, (Real)0};
+                           nn[3 * k + 2] * L.buf(1)[pk].re;
+            L.buf(1)[pk] = {f, (Real)0};
         }
         __syncthreads();
         if (q == 1) {
-            wfft<Real>(b, nr, plan, false, tw, lane);
+            wave_fft<Real>(b, plan, false, tw, lane);
             r2c_unpack<Real>(b, nr, nu1, even, lane,
                              [&](int k, cpx<Real> v0, cpx<Real>) {
                                  sf[k] = cadd(sf[k], v0);
@@ -1170,18 +936,18 @@ __global__ __launch_bounds__(128) void k_fluid_defer(FluidDeferArgs<Real> a) {
                            [&](int k) { return sf[k]; },
                            [&](int) { return zero; });
             wave_sync();
-            wfft<Real>(b, nr, plan, true, tw, lane);
+            wave_fft<Real>(b, plan, true, tw, lane);
         }
         __syncthreads();
         const Real *na = a.nam + a.n_off[j] * 3;
         for (int k = t; k < nr; k += 128) {
             const int pk = pad(pos[k]);
-            const Real g = wbuf[0][pk].re;
-            wbuf[0][pk] = {g * na[3 * k], g * na[3 * k + 1]};
-            wbuf[1][pk] = {g * na[3 * k + 2], (Real)0};
+            const Real g = L.buf(0)[pk].re;
+            L.buf(0)[pk] = {g * na[3 * k], g * na[3 * k + 1]};
+            L.buf(1)[pk] = {g * na[3 * k + 2], (Real)0};
         }
         __syncthreads();
-        wfft<Real>(b, nr, plan, false, tw, lane);
+        wave_fft<Real>(b, plan, false, tw, lane);
         double *cs = a.csolid3d + (size_t)a.c3off[j] * 6;
         r2c_unpack<Real>(b, nr, nu1, even, lane,
                          [&](int k, cpx<Real> v0, cpx<Real> v1) {
@@ -1213,14 +979,14 @@ __global__ __launch_bounds__(128) void k_fluid_defer(FluidDeferArgs<Real> a) {
                                [&](int k) { return sf[k]; },
                                [&](int) { return zero; });
                 wave_sync();
-                wfft<Real>(b, nr, plan, true, tw, lane);
+                wave_fft<Real>(b, plan, true, tw, lane);
                 const Real *m = a.minv + mo;
                 for (int k = lane; k < nr; k += 64) {
                     const int pk = pad(pos[k]);
                     b[pk] = {b[pk].re * m[k], (Real)0};
                 }
                 wave_sync();
-                wfft<Real>(b, nr, plan, false, tw, lane);
+                wave_fft<Real>(b, plan, false, tw, lane);
                 r2c_unpack<Real>(b, nr, nu1, even, lane,
                                  [&](int k, cpx<Real> v0, cpx<Real>) {
                                      sf[k] = v0;
@@ -1262,6 +1028,9 @@ template <typename Real>
 void launch_fluid_defer(hipStream_t st, int phases,
                         const FluidDeferArgs<Real> &a) {
     if (a.n <= 0) return;
+    // two series and the point's nu1 fluid stiffness values
+    const size_t lds = fft_lds_bytes<Real>(
+        a.nr_max, 2, (a.nr_max / 2 + 1) * (int)sizeof(cpx<Real>));
     const bool c = (phases & PHASE_COUPLE) != 0;
     const bool x = (phases & PHASE_AXIAL) != 0;
     const bool m = (phases & PHASE_MASS) != 0;
@@ -1269,7 +1038,7 @@ void launch_fluid_defer(hipStream_t st, int phases,
 #define AXS3D_DEFER(C_, X_, M_, T_)                                           \
     if (c == C_ && x == X_ && m == M_ && t == T_) {                           \
         hipLaunchKernelGGL((k_fluid_defer<Real, C_, X_, M_, T_>), dim3(a.n),  \
-                           dim3(128), 0, st, a);                              \
+                           dim3(128), lds, st, a);                            \
         return;                                                               \
     }
     AXS3D_DEFER(true, false, false, false)
@@ -1298,10 +1067,6 @@ __global__ __launch_bounds__(CD_PT) void k_cd_sens(CardArgs<Real> a,
                                                    const double *ifac,
                                                    double *kern3d,
                                                    double scale) {
-    __shared__ cpx<Real> tw[CARD_TW_CAP];
-    __shared__ cpx<Real> bpsi[CD_PW][CD_BUF];
-    __shared__ cpx<Real> bu[CD_PW][CD_BUF];
-    __shared__ uint16_t pos[CARD_NR_CAP];
     const int t = threadIdx.x;
     const int lane = t & 63;
     const int q = __builtin_amdgcn_readfirstlane(t >> 6);
@@ -1311,13 +1076,12 @@ __global__ __launch_bounds__(CD_PT) void k_cd_sens(CardArgs<Real> a,
     const int nu1 = a.s.elem_nu1[e];
     const int nr = a.elem_nr[e];
     const uint32_t *plan = a.plan + (size_t)nr * CARD_PLAN_W;
-    {
-        const cpx<Real> *twg = a.twiddle + a.tw_off[nr];
-        const int ntw = (int)(plan[0] >> 8);
-        for (int x = t; x < ntw; x += CD_PT) tw[x] = twg[x];
-        const uint16_t *pg = a.slice_pos + a.pos_off[nr];
-        for (int x = t; x < nr; x += CD_PT) pos[x] = pg[x];
-    }
+    FftLds<Real> L;
+    L.carve(fft_lds, nr, 2 * CD_PW);     // psi's three series, then u's
+    fft_load_tables<Real>(L, a.twiddle, a.tw_off, a.slice_pos, a.pos_off,
+                          plan, nr, t, CD_PT);
+    const cpx<Real> *tw = L.tw;
+    const uint16_t *pos = L.pos;
     __syncthreads();
     const bool even = (nr & 1) == 0;
     const size_t rs = (size_t)NPEM * nu1;
@@ -1325,12 +1089,12 @@ __global__ __launch_bounds__(CD_PT) void k_cd_sens(CardArgs<Real> a,
                           a.scr_off[e] + (size_t)pp * nu1;
     const cpx<Real> *rb = reinterpret_cast<const cpx<Real> *>(scrb) +
                           a.scr_off[e] + (size_t)pp * nu1;
-    pack_pair<Real>(bpsi[q], ra + 2 * q * rs, ra + (2 * q + 1) * rs, nr, nu1,
-                    even, lane);
-    wfft<Real>(bpsi[q], nr, plan, true, tw, lane);
-    pack_pair<Real>(bu[q], rb + 2 * q * rs, rb + (2 * q + 1) * rs, nr, nu1,
-                    even, lane);
-    wfft<Real>(bu[q], nr, plan, true, tw, lane);
+    pack_pair<Real>(L.buf(q), ra + 2 * q * rs, ra + (2 * q + 1) * rs, nr,
+                    nu1, even, lane);
+    wave_fft<Real>(L.buf(q), plan, true, tw, lane);
+    pack_pair<Real>(L.buf(CD_PW + q), rb + 2 * q * rs,
+                    rb + (2 * q + 1) * rs, nr, nu1, even, lane);
+    wave_fft<Real>(L.buf(CD_PW + q), plan, true, tw, lane);
     __syncthreads();
     const double f = scale * ifac[(size_t)e * 6 * NPEM + 5 * NPEM + pp] /
                      (double)nr;
@@ -1340,10 +1104,11 @@ __global__ __launch_bounds__(CD_PT) void k_cd_sens(CardArgs<Real> a,
         double ep[6], eu[6];
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            ep[2 * c] = (double)bpsi[c][pk].re;
-            ep[2 * c + 1] = (double)bpsi[c][pk].im;
-            eu[2 * c] = (double)bu[c][pk].re;
-            eu[2 * c + 1] = (double)bu[c][pk].im;
+            const cpx<Real> zp = L.buf(c)[pk], zu = L.buf(CD_PW + c)[pk];
+            ep[2 * c] = (double)zp.re;
+            ep[2 * c + 1] = (double)zp.im;
+            eu[2 * c] = (double)zu.re;
+            eu[2 * c + 1] = (double)zu.im;
         }
         const double tp = ep[0] + ep[1] + ep[2], tu = eu[0] + eu[1] + eu[2];
         const double dm = 2. * (ep[0] * eu[0] + ep[1] * eu[1] +
@@ -1360,24 +1125,26 @@ void launch_cd_sens(hipStream_t st, const CardArgs<Real> &a, const Real *scrb,
                     double scale) {
     const int n = bt.end - bt.begin;
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_cd_sens<Real>, dim3(n * NPEM), dim3(CD_PT), 0, st,
-                       a, scrb, bt.begin, ifac, kern3d, scale);
+    hipLaunchKernelGGL(k_cd_sens<Real>, dim3(n * NPEM), dim3(CD_PT),
+                       fft_lds_bytes<Real>(bt.nr_max, 2 * CD_PW), st, a, scrb,
+                       bt.begin, ifac, kern3d, scale);
 }
 
 template <typename Real>
-int card_blocks_per_cu(bool att) {
+int card_blocks_per_cu(bool att, int nr_max) {
     int n = 0;
+    const size_t lds = fft_lds_bytes<Real>(nr_max, CD_PW);
     const hipError_t r =
         att ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                  &n, k_cd_stress<Real, 5, 0>, CD_PT, 0)
+                  &n, k_cd_stress<Real, 5, 0>, CD_PT, lds)
             : hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                  &n, k_cd_stress<Real, 0, 0>, CD_PT, 0);
+                  &n, k_cd_stress<Real, 0, 0>, CD_PT, lds);
     if (r != hipSuccess) return 1;
     return n > 0 ? n : 1;
 }
 
-template int card_blocks_per_cu<float>(bool);
-template int card_blocks_per_cu<double>(bool);
+template int card_blocks_per_cu<float>(bool, int);
+template int card_blocks_per_cu<double>(bool, int);
 #define AXS3D_CD_INST(Real)                                                   \
     template void launch_cd_grad<Real>(hipStream_t, const CardArgs<Real> &, \
                                        const CardBatch &);                  \

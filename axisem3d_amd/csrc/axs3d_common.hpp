@@ -17,37 +17,37 @@ struct alignas(2 * sizeof(T)) cpx {
 };
 
 template <typename T>
-__device__ __forceinline__ cpx<T> cmul(cpx<T> a, T s) {
+__host__ __device__ __forceinline__ cpx<T> cmul(cpx<T> a, T s) {
     return {a.re * s, a.im * s};
 }
 template <typename T>
-__device__ __forceinline__ cpx<T> cadd(cpx<T> a, cpx<T> b) {
+__host__ __device__ __forceinline__ cpx<T> cadd(cpx<T> a, cpx<T> b) {
     return {a.re + b.re, a.im + b.im};
 }
 template <typename T>
-__device__ __forceinline__ cpx<T> csub(cpx<T> a, cpx<T> b) {
+__host__ __device__ __forceinline__ cpx<T> csub(cpx<T> a, cpx<T> b) {
     return {a.re - b.re, a.im - b.im};
 }
 template <typename T>
-__device__ __forceinline__ cpx<T> operator+(cpx<T> a, cpx<T> b) {
+__host__ __device__ __forceinline__ cpx<T> operator+(cpx<T> a, cpx<T> b) {
     return {a.re + b.re, a.im + b.im};
 }
 template <typename T>
-__device__ __forceinline__ cpx<T> operator-(cpx<T> a, cpx<T> b) {
+__host__ __device__ __forceinline__ cpx<T> operator-(cpx<T> a, cpx<T> b) {
     return {a.re - b.re, a.im - b.im};
 }
 template <typename T>
-__device__ __forceinline__ cpx<T> operator*(cpx<T> a, T s) {
+__host__ __device__ __forceinline__ cpx<T> operator*(cpx<T> a, T s) {
     return {a.re * s, a.im * s};
 }
 // a + b * s
 template <typename T>
-__device__ __forceinline__ cpx<T> cfma(cpx<T> b, T s, cpx<T> a) {
+__host__ __device__ __forceinline__ cpx<T> cfma(cpx<T> b, T s, cpx<T> a) {
     return {a.re + b.re * s, a.im + b.im * s};
 }
 // i * alpha * a
 template <typename T>
-__device__ __forceinline__ cpx<T> ialpha(T alpha, cpx<T> a) {
+__host__ __device__ __forceinline__ cpx<T> ialpha(T alpha, cpx<T> a) {
     return {-alpha * a.im, alpha * a.re};
 }
 

@@ -498,6 +498,13 @@ void DomainBase::setReceivers(int n, const int32_t *elem, const double *w,
 
 void DomainBase::finalize() {
     hip_check(hipSetDevice(mDevice), "Domain::finalize");
+    if (const char *env = std::getenv("AXS3D_FFT_MAX_RADIX")) {
+        const int r = std::atoi(env);
+        if (r < 2 || r > 16)
+            throw Error("Domain::finalize || AXS3D_FFT_MAX_RADIX must be in "
+                        "2..16.");
+        mFftMaxRadix = r;
+    }
     if (hGll.size() != NPEM || hGlj.size() != NPEM)
         throw Error("Domain::finalize || G matrices are not set.");
     if (mNumPoints <= 0 || mNumElems <= 0)
@@ -628,67 +635,32 @@ void DomainBase::finalize() {
         for (int nr : fftNr) nrmax = std::max(nrmax, nr);
         if (nrmax > CARD_NR_CAP)
             throw Error("Domain::finalize || Nr of a 3D element or 3D-mass "
-                        "point exceeds the FFT capacity (832).");
+                        "point exceeds the FFT capacity (2048).");
         twOff.assign(nrmax + 1, -1);
         posOff.assign(nrmax + 1, -1);
         plan.assign((size_t)(nrmax + 1) * CARD_PLAN_W, 0u);
+        std::vector<double> tw1;
+        std::vector<uint16_t> pos1;
         for (const int nr : fftNr) {
             if (twOff[nr] >= 0) continue;
-            std::vector<int> radix;
-            int rem = nr, p2 = 0;
-            while (rem % 2 == 0) {
-                rem /= 2;
-                p2++;
-            }
-            for (int k = 0; k < p2 / 3; k++) radix.push_back(8);
-            if (p2 % 3 == 2) radix.push_back(4);
-            if (p2 % 3 == 1) radix.push_back(2);
-            for (int c : {3, 5, 7, 11, 13})
-                while (rem % c == 0) {
-                    radix.push_back(c);
-                    rem /= c;
-                }
-            if (rem != 1)
+            const std::vector<int> rad = fft_choose_radices(nr, mFftMaxRadix);
+            if (rad.empty() && nr > 1)
                 throw Error("Domain::finalize || Nr of a 3D element has "
                             "a prime factor above 13 (use FFT lucky "
                             "numbers).");
-            if ((int)radix.size() > CARD_MAX_STAGES)
-                throw Error("Domain::finalize || Too many FFT stages.");
-            std::sort(radix.begin(), radix.end(), std::greater<int>());
+            if ((int)rad.size() > FFT_MAX_PASSES)
+                throw Error("Domain::finalize || Too many FFT passes.");
             uint32_t *pl = plan.data() + (size_t)nr * CARD_PLAN_W;
+            fft_build_plan(nr, rad, pl, tw1, pos1);
             twOff[nr] = (int64_t)twTab.size() / 2;
-            int span = nr, ntw = 0;
-            for (size_t st = 0; st < radix.size(); st++) {
-                const int r = radix[st], m = span / r;
-                pl[1 + 3 * st] = (uint32_t)r | ((uint32_t)m << 8);
-                pl[2 + 3 * st] =
-                    m == 1 ? 0u
-                           : (uint32_t)((((uint64_t)1 << 32) + m - 1) / m);
-                pl[3 + 3 * st] = (uint32_t)ntw;
-                for (int j = 0; j < m; j++) {
-                    const double ang = -2. * M_PI * (double)j / (double)span;
-                    twTab.push_back(std::cos(ang));
-                    twTab.push_back(std::sin(ang));
-                }
-                ntw += m;
-                span = m;
-            }
-            if (ntw > CARD_TW_CAP)
-                throw Error("Domain::finalize || FFT twiddle table exceeds "
-                            "the cardinal kernel's capacity.");
-            pl[0] = (uint32_t)radix.size() | ((uint32_t)ntw << 8);
+            twTab.insert(twTab.end(), tw1.begin(), tw1.end());
             posOff[nr] = (int64_t)posTab.size();
-            for (int sl = 0; sl < nr; sl++) {
-                int rest = sl, pos = 0, sp = nr;
-                for (int r : radix) {
-                    sp /= r;
-                    pos += (rest % r) * sp;
-                    rest /= r;
-                }
-                posTab.push_back((uint16_t)pos);
-            }
+            posTab.insert(posTab.end(), pos1.begin(), pos1.end());
         }
     }
+    mMass3DNrMax = 0;
+    for (int32_t p : hMass3DPts)
+        mMass3DNrMax = std::max(mMass3DNrMax, (int)hPointNr[p]);
     // element-local force buffer [e][ipnt][alpha]: offsets in triples
     std::vector<uint32_t> ebOff(mNumElems);
     int64_t ebTotal = 0;
@@ -885,6 +857,9 @@ void DomainBase::finalize() {
                 dfMOff.push_back(fMass3DOff[p]);
             }
         mNumFDefer = (int)dfPts.size();
+        mFDeferNrMax = 0;
+        for (int32_t p : dfPts)
+            mFDeferNrMax = std::max(mFDeferNrMax, (int)hFPointNr[p]);
         // 3D pairs: slice and mode offsets, assembled normals over the
         // fluid mass per slice (SolidFluidCoupling3D constructor,
         // SolidFluidCoupling3D.cpp:15-24)
@@ -1367,6 +1342,7 @@ public:
     Mass3DArgs<Real> mass3dArgs() const {
         Mass3DArgs<Real> m;
         m.n = mNumMass3D;
+        m.nr_max = mMass3DNrMax;
         m.point = dMass3DPts.as<int32_t>();
         m.inv_off = dMass3DOff.as<int64_t>();
         m.inv_mass = dInvMass3D.as<Real>();
@@ -1399,6 +1375,7 @@ public:
     FluidDeferArgs<Real> fluidDeferArgs() const {
         FluidDeferArgs<Real> f;
         f.n = mNumFDefer;
+        f.nr_max = mFDeferNrMax;
         f.point = dFDeferPts.as<int32_t>();
         f.pair3d = dFDeferPair.as<int32_t>();
         f.minv_off = dFDeferMOff.as<int64_t>();
@@ -1906,6 +1883,8 @@ private:
                 }
                 scrOff[e] = (uint32_t)used;
                 used += need;
+                mCardBatches.back().nr_max =
+                    std::max(mCardBatches.back().nr_max, (int)hElemNr[e]);
                 for (int al = 0; al < nu1; al++) em3d.push_back({e, al});
                 if (hElemAxial[e])
                     for (int al = 0; al < nu1; al++) emAx.push_back({e, al});
@@ -2176,8 +2155,11 @@ private:
             hip_check(hipDeviceGetAttribute(
                           &ncu, hipDeviceAttributeMultiprocessorCount,
                           mDevice), "Domain::finalize");
-            mCardSlots = std::max(1, card_blocks_per_cu<Real>(mNsls > 0) *
-                                         ncu);
+            int nr3d = 0;
+            for (int32_t e : hList3D)
+                nr3d = std::max(nr3d, (int)hElemNr[e]);
+            mCardSlots = std::max(
+                1, card_blocks_per_cu<Real>(mNsls > 0, nr3d) * ncu);
             size_t budget = CARD_SCRATCH_BYTES;
             if (const char *env = std::getenv("AXS3D_CARD_SCRATCH_MB")) {
                 // MiB, fractions allowed (at least one element per batch)
@@ -2235,6 +2217,8 @@ private:
                 }
                 fscr[e] = (uint32_t)used;
                 used += need;
+                mFCardBatches.back().nr_max =
+                    std::max(mFCardBatches.back().nr_max, (int)hFElemNr[e]);
                 for (int al = 0; al < nu1; al++) fem3d.push_back({e, al});
                 mFCardBatches.back().end = w + 1;
                 mFCardBatches.back().em_end = (int)fem3d.size();

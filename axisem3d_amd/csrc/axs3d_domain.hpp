@@ -240,6 +240,10 @@ protected:
     size_t mCardScrMax = 0;             // scratch size, complex values
     // solid points with a 3D mass: points, offsets of their Nr masses
     std::vector<int32_t> hMass3DPts;
+    int mMass3DNrMax = 0;               // largest Nr of the 3D-mass points
+    int mFDeferNrMax = 0;               // and of the deferred fluid points
+    // largest FFT pass radix (2..16); AXS3D_FFT_MAX_RADIX to change it
+    int mFftMaxRadix = 16;
     std::vector<int64_t> hMass3DOff;
     std::vector<double> hMass3D;
     int mNumMass3D = 0;

@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include "axs3d_common.hpp"
+#include "axs3d_fft.hpp"
 
 namespace axs3d {
 
@@ -104,21 +105,20 @@ struct StiffArgs {
     Real *ds_fr;                   // [em][6][npa] complex
 };
 
-// Largest element Nr of the cardinal kernel (one complex series of Nr per
-// wave in LDS).
-constexpr int CARD_NR_CAP = 832;
-// Minimum waves per SIMD the stress kernel's register allocation allows.
+// Largest element Nr of the cardinal kernels (one padded complex series
+// of Nr per wave in LDS, sized per launch).
+constexpr int CARD_NR_CAP = FFT_NR_CAP;
+// Minimum waves per SIMD the stress kernel's register allocation allows:
+// five in FP32; in FP64 the LDS of an Nr of several hundred holds at most
+// two or three blocks per CU anyway, and three waves leave the radix-16
+// passes their registers.
 constexpr int CARD_OCC = 5;
-// FFT plan per Nr (in-place, decimation-in-frequency stage order): word 0
-// = number of stages | (length of the stage twiddle table << 8); per
-// stage s, words 1+3s..: radix | (m << 8) with m = span / radix,
-// ceil(2^32 / m) (0 for m = 1), offset of the stage's twiddles
-// exp(-2 pi i j / span), j < m, in the Nr's table.
-constexpr int CARD_MAX_STAGES = 12;
-// Largest stage twiddle table (sum of m over the stages, radices in
-// descending order; 364 for the 13-smooth Nr up to 832).
-constexpr int CARD_TW_CAP = 384;
-constexpr int CARD_PLAN_W = 1 + 3 * CARD_MAX_STAGES;
+template <typename Real>
+struct CardOcc {
+    static constexpr int value = sizeof(Real) == 4 ? CARD_OCC : 3;
+};
+// FFT plan words per Nr (see axs3d_fft.hpp).
+constexpr int CARD_PLAN_W = FFT_PLAN_W;
 
 // Device operands of the cardinal-path (3D material) element kernel.
 template <typename Real>
@@ -136,9 +136,9 @@ struct CardArgs {
     // element offset slice_off * (components per slice).
     const Real *prop3d;            // [np][25] parameters of the type
     const int64_t *prop_off;       // element offsets in prop3d
-    const cpx<Real> *twiddle;      // stage twiddle tables, per Nr
+    const cpx<Real> *twiddle;      // pass twiddle tables, per Nr
     const int64_t *tw_off;         // offset of each Nr's table
-    const uint32_t *plan;          // [Nr][CARD_PLAN_W] FFT stages
+    const uint32_t *plan;          // [Nr][CARD_PLAN_W] FFT passes
     const uint16_t *slice_pos;     // buffer position of each slice after
     const int64_t *pos_off;        // the inverse transform, per Nr
     Real *scratch;                 // batch scratch, complex
@@ -175,6 +175,7 @@ struct PointGeom {
 template <typename Real>
 struct Mass3DArgs {
     int n;                         // points with a 3D mass
+    int nr_max;                    // their largest Nr (LDS of a launch)
     const int32_t *point;          // [i] point index
     const int64_t *inv_off;        // [i] offset of its Nr inverse masses
     const Real *inv_mass;          // 1 / m per slice
@@ -199,6 +200,7 @@ struct Mass3DArgs {
 template <typename Real>
 struct FluidDeferArgs {
     int n;                         // deferred fluid points
+    int nr_max;                    // their largest Nr (LDS of a launch)
     const int32_t *point;          // [i] fluid point
     const int32_t *pair3d;         // [i] 3D pair or -1
     const int64_t *minv_off;       // [i] offset of Nr inverse masses or -1
@@ -248,6 +250,7 @@ struct CardBatch {
     int em_begin, em_end;          // its element-modes in the 3D list
     int ax_begin, ax_end;          // its axial elements' element-modes in
                                    // the axial list
+    int nr_max = 0;                // largest Nr of its elements
 };
 // Default scratch of one batch (bytes).
 constexpr size_t CARD_SCRATCH_BYTES = (size_t)4 << 30;
@@ -270,9 +273,10 @@ template <typename Real>
 void launch_cdf_batch(hipStream_t st, const FluidCardArgs<Real> &a,
                       const CardBatch &bt);
 
-// resident stress blocks per CU (with or without attenuation)
+// resident stress blocks per CU (with or without attenuation) for the
+// LDS of an Nr of nr_max
 template <typename Real>
-int card_blocks_per_cu(bool att);
+int card_blocks_per_cu(bool att, int nr_max);
 
 template <typename Real>
 void launch_moment_source(hipStream_t st, bool to_elem,

@@ -390,6 +390,20 @@ int axs3d_synchronize(axs3d_ctx *ctx);
 int axs3d_profile_enable(axs3d_ctx *ctx, int enable);
 int axs3d_profile_read(axs3d_ctx *ctx, int kernel, double *total_ms,
                        int64_t *count);
+/* Diagnostic, no reference counterpart: runs the library's azimuthal
+ * FFT for Nr slices on the host, pass by pass and lane by lane exactly as
+ * the device kernels do (same plan, radices, twiddle table, digit
+ * reversal), in FP64, so that the plans can be checked against a DFT
+ * without a GPU.  The reference's transform is FFTW's (SolverFFTW.hpp:
+ * 101-168); this one follows its sign conventions, unscaled: inverse != 0
+ * computes X_k = sum_j x_j exp(+2 pi i jk / nr), else exp(-2 pi i jk /
+ * nr); data holds nr interleaved complex values, in and out in natural
+ * order.  radices (at least 12 ints) and npasses receive the pass radices
+ * chosen with the largest radix max_radix (2..16).  Returns 0, 1 for an
+ * Nr with a prime factor above 13, 2 for bad arguments. */
+int axs3d_fft_check(int nr, int max_radix, int inverse, double *data,
+                    int *radices, int *npasses);
+
 int axs3d_profile_reset(axs3d_ctx *ctx);
 
 /* The cardinal path as set up at axs3d_finalize (no reference
