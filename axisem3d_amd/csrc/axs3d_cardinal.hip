@@ -14,22 +14,20 @@
 //
 // Design: the 3D elements are processed in batches (elements of one
 // material type sorted by Nr, heaviest first, whose strain and stress fit
-// the batch scratch, series-major [6*25][nu1] complex per element), three
-// launches per batch:
+// the batch scratch, series-major [6*25][nu1] complex per element, each
+// row padded to 16 orders), three launches per batch:
 //  A. gradient: one thread per (element-mode, GLL point) as on the
 //     Fourier path, strain (rotated to RTZ where the medium works there)
 //     to the scratch.
 //  B. stress: one GLL point per block of three waves.  Each wave packs
 //     one of the point's three strain pairs as x + i y into its own LDS
-//     buffer and transforms it in place by a mixed-radix FFT (decimation
-//     in frequency for the inverse, in time for the forward transform;
-//     radices 2, 3, 4, 5, 7, 8, 11, 13 as in-register DFTs with constant
-//     coefficients; one twiddle per butterfly from a per-stage table, its
-//     powers formed in registers; plans, tables and the digit reversal
-//     per Nr).  The three waves then form the stress slice by slice from
+//     buffer and transforms it in place (axs3d_fft.hpp: passes of radix 2
+//     to 16 with compile-time DFTs and a table of every twiddle power;
+//     plans, tables and the digit reversal per Nr, in LDS sized per
+//     launch).  The three waves then form the stress slice by slice from
 //     all six strains (3D moduli, SLS memory variables) and transform back
-//     into the scratch.  One FFT call site per direction keeps the code
-//     small.
+//     into the scratch.  One FFT call site for both directions keeps the
+//     code small.
 //  C. quadrature: as A, from the stress rows to the element-local force
 //     buffer the point kernel gathers.
 // Each kernel gets the register budget of its own phase.
@@ -241,9 +239,10 @@ __global__ __launch_bounds__(CD_EMB_THREADS) void k_cd_grad(CardArgs<Real> a,
     }
     const int nu1 = s.elem_nu1[e];
     cpx<Real> *dst = reinterpret_cast<cpx<Real> *>(a.scratch) +
-                     a.scr_off[e] + (size_t)ipnt * nu1 + alpha;
+                     a.scr_off[e] + (size_t)ipnt * scr_row(nu1) + alpha;
 #pragma unroll
-    for (int d = 0; d < 6; d++) dst[(size_t)d * NPEM * nu1] = eps[d];
+    for (int d = 0; d < 6; d++)
+        dst[(size_t)d * NPEM * scr_row(nu1)] = eps[d];
 }
 
 // C: quadrature of the element-modes [em0, em1) from the scratch stress
@@ -260,7 +259,7 @@ __global__ __launch_bounds__(CD_EMB_THREADS) void k_cd_quad(CardArgs<Real> a,
     __syncthreads();
     const int eml = t % CD_EMB;
     const int ipnt = t < CD_EMB * NPEM ? t / CD_EMB : 0;
-    const int em = em0 + blockIdx.x * CD_EMB + eml;
+    const int em = em0 + xcd_block() * CD_EMB + eml;
     const bool active = t < CD_EMB * NPEM && em < em1;
     const int i = ipnt / NPED, j = ipnt - (ipnt / NPED) * NPED;
     int e = 0, alpha = 0, nu1 = 0;
@@ -274,9 +273,11 @@ __global__ __launch_bounds__(CD_EMB_THREADS) void k_cd_quad(CardArgs<Real> a,
         nu1 = s.elem_nu1[e];
         axial = s.elem_axial[e] != 0;
         const cpx<Real> *src = reinterpret_cast<const cpx<Real> *>(a.scratch) +
-                               a.scr_off[e] + (size_t)ipnt * nu1 + alpha;
+                               a.scr_off[e] + (size_t)ipnt * scr_row(nu1) +
+                               alpha;
 #pragma unroll
-        for (int d = 0; d < 6; d++) sig[d] = src[(size_t)d * NPEM * nu1];
+        for (int d = 0; d < 6; d++)
+            sig[d] = src[(size_t)d * NPEM * scr_row(nu1)];
         if (ROT) {
             const Real *r = s.rot + (size_t)e * 4 * NPEM + ipnt;
             rot_rtz_spz6<cpx<Real>, Real>(sig, r[0], r[NPEM], r[2 * NPEM],
@@ -402,9 +403,9 @@ __global__ __launch_bounds__(CD_PT, CardOcc<Real>::value) void k_cd_stress(
     }
     __syncthreads();
     const bool even = (nr & 1) == 0;
-    const size_t rs = (size_t)NPEM * nu1;
+    const size_t rs = (size_t)NPEM * scr_row(nu1);
     cpx<Real> *r0 = reinterpret_cast<cpx<Real> *>(a.scratch) + a.scr_off[e] +
-                    (size_t)pp * nu1;
+                    (size_t)pp * scr_row(nu1);
     cpx<Real> *b = L.buf(q);
     pack_pair<Real>(b, r0 + 2 * q * rs, r0 + (2 * q + 1) * rs, nr, nu1, even,
                     lane);
@@ -532,7 +533,7 @@ __global__ __launch_bounds__(CD_EMB_THREADS) void k_cdf_grad(
     if (t < 2 * NPEM) sG[t / NPEM][t % NPEM] = f.gtab[t];
     const int eml = t % CD_EMB;
     const int ipnt = t < CD_EMB * NPEM ? t / CD_EMB : 0;
-    const int em = em0 + blockIdx.x * CD_EMB + eml;
+    const int em = em0 + xcd_block() * CD_EMB + eml;
     const bool active = t < CD_EMB * NPEM && em < em1;
     const int i = ipnt / NPED, j = ipnt - (ipnt / NPED) * NPED;
     int e = 0, alpha = 0;
@@ -568,8 +569,8 @@ __global__ __launch_bounds__(CD_EMB_THREADS) void k_cdf_grad(
                                            : cmul(ialpha(ralpha, p), g.invs);
     const int nu1 = f.elem_nu1[e];
     cpx<Real> *dst = reinterpret_cast<cpx<Real> *>(a.scratch) + a.scr_off[e] +
-                     (size_t)ipnt * nu1 + alpha;
-    const size_t rs = (size_t)NPEM * nu1;
+                     (size_t)ipnt * scr_row(nu1) + alpha;
+    const size_t rs = (size_t)NPEM * scr_row(nu1);
     dst[0] = dpds;
     dst[rs] = ps;
     dst[2 * rs] = dpdz;
@@ -598,9 +599,9 @@ __global__ __launch_bounds__(128) void k_cdf_stress(FluidCardArgs<Real> a,
     const uint16_t *pos = L.pos;
     __syncthreads();
     const bool even = (nr & 1) == 0;
-    const size_t rs = (size_t)NPEM * nu1;
+    const size_t rs = (size_t)NPEM * scr_row(nu1);
     cpx<Real> *r0 = reinterpret_cast<cpx<Real> *>(a.scratch) + a.scr_off[e] +
-                    (size_t)pp * nu1;
+                    (size_t)pp * scr_row(nu1);
     cpx<Real> *b = L.buf(q);
     // wave 1: rows (2, zero); the zero row is read as row 2's imaginary
     // partner, so pack it from row 2 and clear the y part
@@ -654,7 +655,7 @@ __global__ __launch_bounds__(CD_EMB_THREADS) void k_cdf_quad(
     if (t < 2 * NPEM) sG[t / NPEM][t % NPEM] = f.gtab[t];
     const int eml = t % CD_EMB;
     const int ipnt = t < CD_EMB * NPEM ? t / CD_EMB : 0;
-    const int em = em0 + blockIdx.x * CD_EMB + eml;
+    const int em = em0 + xcd_block() * CD_EMB + eml;
     const bool active = t < CD_EMB * NPEM && em < em1;
     const int i = ipnt / NPED, j = ipnt - (ipnt / NPED) * NPED;
     int e = 0, alpha = 0, nu1 = 0;
@@ -667,8 +668,9 @@ __global__ __launch_bounds__(CD_EMB_THREADS) void k_cdf_quad(
         nu1 = f.elem_nu1[e];
         axial = f.elem_axial[e] != 0;
         const cpx<Real> *src = reinterpret_cast<const cpx<Real> *>(a.scratch) +
-                               a.scr_off[e] + (size_t)ipnt * nu1 + alpha;
-        const size_t rs = (size_t)NPEM * nu1;
+                               a.scr_off[e] + (size_t)ipnt * scr_row(nu1) +
+                               alpha;
+        const size_t rs = (size_t)NPEM * scr_row(nu1);
         const cpx<Real> q0 = src[0], q1 = src[rs], q2 = src[2 * rs];
         PointOps<Real> g{}, q{};
         load_point_ops(f.elem_ops, e, ipnt, g, q);
@@ -773,7 +775,7 @@ __global__ __launch_bounds__(128) void k_mass3d(Mass3DArgs<Real> a,
     wave_fft<Real>(b, plan, false, tw, lane);
     // R2C output (as unpack_pair): X into component cx, Y into 1 (wave 0)
     const Real h0 = (Real)0.5 / (Real)nr;
-    bool bad = false;
+    bool bad = false, bad0 = false;
     for (int k = lane; k < nu1; k += 64) {
         const cpx<Real> zk = b[pad(k)];
         const cpx<Real> zn = b[pad(k == 0 ? 0 : nr - k)];
@@ -794,18 +796,21 @@ __global__ __launch_bounds__(128) void k_mass3d(Mass3DArgs<Real> a,
             const double an[2] = {(double)v[c].re, (double)v[c].im};
 #pragma unroll
             for (int ri = 0; ri < 2; ri++) {
+                const double u0 = a.displ[m6 + ri];
                 const double vv = a.veloc[m6 + ri] +
                                   hdt * (a.accel[m6 + ri] + an[ri]);
-                const double uu = a.displ[m6 + ri] + (a.dt * vv +
-                                                      hdt2 * an[ri]);
+                const double uu = u0 + (a.dt * vv + hdt2 * an[ri]);
                 a.veloc[m6 + ri] = vv;
                 a.accel[m6 + ri] = an[ri];
                 a.displ[m6 + ri] = uu;
                 bad |= !isfinite(uu);
+                bad0 |= !isfinite(u0);
             }
         }
     }
-    if (MARCH && bad) atomicOr(a.flag, 1);
+    // bit 0: started from a non-finite displacement, bit 1: left one
+    if (MARCH && (bad0 || bad))
+        atomicOr(a.flag, (bad0 ? 1 : 0) | (bad ? 2 : 0));
 }
 
 template <typename Real>
@@ -1005,23 +1010,26 @@ __global__ __launch_bounds__(128) void k_fluid_defer(FluidDeferArgs<Real> a) {
         return;
     }
     const double hdt = .5 * a.dt, hdt2 = .5 * a.dt * a.dt;
-    bool bad = false;
+    bool bad = false, bad0 = false;
     for (int k = t; k < nu1; k += 128) {
         const size_t m2 = (foff + k) * 2;
         const double an[2] = {(double)sf[k].re, (double)sf[k].im};
 #pragma unroll
         for (int ri = 0; ri < 2; ri++) {
+            const double u0 = a.chi[m2 + ri];
             const double vv = a.chi_v[m2 + ri] + hdt * (a.chi_a[m2 + ri] +
                                                          an[ri]);
-            const double uu = a.chi[m2 + ri] + (a.dt * vv + hdt2 * an[ri]);
+            const double uu = u0 + (a.dt * vv + hdt2 * an[ri]);
             a.chi_v[m2 + ri] = vv;
             a.chi_a[m2 + ri] = an[ri];
             a.chi[m2 + ri] = uu;
             bad |= !isfinite(uu);
+            bad0 |= !isfinite(u0);
         }
         fs[k] = zero;
     }
-    if (bad) atomicOr(a.flag, 1);
+    // bit 0: started from a non-finite potential, bit 1: left one
+    if (bad0 || bad) atomicOr(a.flag, (bad0 ? 1 : 0) | (bad ? 2 : 0));
 }
 
 template <typename Real>
@@ -1084,11 +1092,11 @@ __global__ __launch_bounds__(CD_PT) void k_cd_sens(CardArgs<Real> a,
     const uint16_t *pos = L.pos;
     __syncthreads();
     const bool even = (nr & 1) == 0;
-    const size_t rs = (size_t)NPEM * nu1;
+    const size_t rs = (size_t)NPEM * scr_row(nu1);
     const cpx<Real> *ra = reinterpret_cast<const cpx<Real> *>(a.scratch) +
-                          a.scr_off[e] + (size_t)pp * nu1;
+                          a.scr_off[e] + (size_t)pp * scr_row(nu1);
     const cpx<Real> *rb = reinterpret_cast<const cpx<Real> *>(scrb) +
-                          a.scr_off[e] + (size_t)pp * nu1;
+                          a.scr_off[e] + (size_t)pp * scr_row(nu1);
     pack_pair<Real>(L.buf(q), ra + 2 * q * rs, ra + (2 * q + 1) * rs, nr,
                     nu1, even, lane);
     wave_fft<Real>(L.buf(q), plan, true, tw, lane);

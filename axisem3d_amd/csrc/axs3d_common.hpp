@@ -75,6 +75,14 @@ __device__ __forceinline__ int xcd_block() {
     return x * q + (x < rem ? x : rem) + r;
 }
 
+// Row stride (complex values) of an element's rows in the cardinal
+// scratch: the nu1 orders of a row padded to a multiple of 16, so that
+// every row starts on a 128-byte line and a run of 16 orders read or
+// written by 16 lanes never straddles two lines.
+__host__ __device__ __forceinline__ int scr_row(int nu1) {
+    return (nu1 + 15) & ~15;
+}
+
 // Element-mode work item: element index and Fourier order alpha.
 struct EMItem {
     int32_t elem;

@@ -1038,12 +1038,15 @@ void DomainBase::checkStability(int tstep) {
               "Domain::checkStability");
     hip_check(hipStreamSynchronize(mStream), "Domain::checkStability");
     if (flag[0]) {
-        // the step whose update first left a non-finite displacement
-        // (the steps of a batch after it ran on and are void)
+        // the reference checks at the start of each step and names it
+        // from 1 (NewmarkTimeScheme.cpp:72-74 passes tstep + 1): the first
+        // step that started from a non-finite displacement, or, when only
+        // the batch's last update left one, the caller's step (the steps
+        // of a batch after the blow-up ran on and are void)
         std::ostringstream ss;
         ss << "Domain::checkStability || Simulation has blown up, with dt = "
            << mDt << " s. || When the instability occurred: || "
-           << "* current step  =  " << (flag[1] >= 0 ? flag[1] : tstep);
+           << "* current step  =  " << (flag[1] >= 0 ? flag[1] + 1 : tstep);
         throw Error(ss.str());
     }
 }
@@ -1076,7 +1079,7 @@ void DomainBase::solve(int step_begin, int nsteps) {
             step(tstep, isample, false);
         }
     }
-    checkStability(step_begin + nsteps);
+    checkStability(step_begin + nsteps + 1);
 }
 
 // Captures one step in its device-counter form.  The captured kernels take
@@ -1781,7 +1784,7 @@ public:
                                      hipMemcpyDeviceToHost, mStream),
                       "Domain::adjoint");
         hip_check(hipStreamSynchronize(mStream), "Domain::adjoint");
-        checkStability(nsteps);
+        checkStability(nsteps + 1);
         // the segment buffers are the largest; give them back
         dAdjCkpt.reset();
         dAdjSegU.reset();
@@ -1873,7 +1876,7 @@ private:
             for (int w = mTypeBegin3D[mt]; w < mTypeBegin3D[mt + 1]; w++) {
                 const int e = hList3D[w];
                 const int nu1 = hElemNr[e] / 2 + 1;
-                const size_t need = (size_t)6 * NPEM * nu1;
+                const size_t need = (size_t)6 * NPEM * scr_row(nu1);
                 if (mCardBatches.empty() || mCardBatches.back().mtype != mt ||
                     used + need > budget) {
                     mCardBatches.push_back(
@@ -2209,7 +2212,7 @@ private:
             for (int w = 0; w < mNumF3D; w++) {
                 const int e = hFList3D[w];
                 const int nu1 = hFElemNr[e] / 2 + 1;
-                const size_t need = (size_t)3 * NPEM * nu1;
+                const size_t need = (size_t)3 * NPEM * scr_row(nu1);
                 if (mFCardBatches.empty() || used + need > budget) {
                     mFCardBatches.push_back(
                         {0, w, w, (int)fem3d.size(), (int)fem3d.size()});

@@ -236,8 +236,10 @@ __global__ __launch_bounds__(256) void k_fluid_point(
     *ap = make_double2(s[0], s[1]);
     *up = make_double2(ur, ui);
     const bool bad = !isfinite(ur) || !isfinite(ui);
+    const bool bad0 = !isfinite(u0.x) || !isfinite(u0.y);
     if (!GATHER) stiff[(size_t)k * 2] = stiff[(size_t)k * 2 + 1] = 0;
-    if (bad) atomicOr(a.flag, 1);
+    // bit 0: started from a non-finite potential, bit 1: left one
+    if (bad0 || bad) atomicOr(a.flag, (bad0 ? 1 : 0) | (bad ? 2 : 0));
 }
 
 template <typename Real>

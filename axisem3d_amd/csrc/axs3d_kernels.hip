@@ -435,7 +435,7 @@ __global__ __launch_bounds__(256) void k_point(PointArgs a,
         v2[c] = vp[c];
         a2[c] = ap[c];
     }
-    bool bad = false;
+    bool bad = false, bad0 = false;
     double dt = a.dt, half_dt = a.half_dt, half_dt_dt = a.half_dt_dt;
     if (a.point_dt != nullptr) {
         dt = a.point_dt[p];
@@ -444,6 +444,7 @@ __global__ __launch_bounds__(256) void k_point(PointArgs a,
     }
 #pragma unroll
     for (int c = 0; c < 3; c++) {
+        bad0 |= !isfinite(u2[c].x) || !isfinite(u2[c].y);
         const double vr = v2[c].x + half_dt * (a2[c].x + a_new[2 * c]);
         const double vi = v2[c].y + half_dt * (a2[c].y + a_new[2 * c + 1]);
         const double ur = u2[c].x + (dt * vr + half_dt_dt * a_new[2 * c]);
@@ -464,7 +465,10 @@ __global__ __launch_bounds__(256) void k_point(PointArgs a,
 #pragma unroll
         for (int q = 0; q < 6; q++) stiff[(size_t)k * 6 + q] = 0;
     }
-    if (bad) atomicOr(a.flag, 1);
+    // bit 0: the displacement this step started from was non-finite (what
+    // Domain::checkStability sees at the start of a step); bit 1: the one
+    // it leaves is
+    if (bad0 || bad) atomicOr(a.flag, (bad0 ? 1 : 0) | (bad ? 2 : 0));
 }
 
 
@@ -661,17 +665,20 @@ __global__ void k_step_set(int *ctr, int tstep, int record_step0) {
     }
 }
 
-// flag[0]: raised by the point kernels on a non-finite displacement;
-// flag[1]: the first step after which it was up (-1: none yet).
+// flag[0]: raised by the point kernels, bit 0 when a step started from a
+// non-finite displacement (what Domain::checkStability tests at the start
+// of a step), bit 1 when a step left one; flag[1]: the first step (from
+// 0) that started from one (-1: none yet).
 __global__ void k_step_advance(int *ctr, int *flag) {
     if (threadIdx.x == 0) {
-        if (flag[0] != 0 && flag[1] < 0) flag[1] = ctr[0];
+        if ((flag[0] & 1) != 0 && flag[1] < 0) flag[1] = ctr[0];
         ctr[0] += 1;
     }
 }
 
 __global__ void k_flag_note(int *flag, int tstep) {
-    if (threadIdx.x == 0 && flag[0] != 0 && flag[1] < 0) flag[1] = tstep;
+    if (threadIdx.x == 0 && (flag[0] & 1) != 0 && flag[1] < 0)
+        flag[1] = tstep;
 }
 
 void launch_step_set(hipStream_t st, int *ctr, int tstep, int record_step0) {

@@ -50,6 +50,10 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--accuracy-steps", type=int, default=1000,
+                    help="both precisions run on, untimed, to this many "
+                         "steps from rest; their seismograms give "
+                         "seismogram_rel_l2_vs_fp64")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     metavar="MIB",
                     help="only run a plain device copy of MIB MiB three "
@@ -211,15 +215,18 @@ def pmc_traffic(kernel, workload):
     return None
 
 
-def time_gpu(spec, prec, steps, warmup, graph=False):
+def time_gpu(spec, prec, steps, warmup, graph=False, total=0):
     """Times `steps` steps launched kernel by kernel with HIP events around
     every kernel (the per-kernel averages come from these same steps);
     with graph, then the same number of steps as replays of the captured
-    step graph, without events."""
+    step graph, without events.  With total, the run then goes on
+    (untimed) to `total` steps from rest, and the seismograms of all of
+    them are returned (the accuracy leg)."""
     from axisem3d_amd import capi
     from axisem3d_amd.solver import GpuSolver
     t0 = time.perf_counter()
-    g = GpuSolver(spec, prec, max_samples=steps + warmup)
+    g = GpuSolver(spec, prec,
+                  max_samples=max(total, 2 * steps + warmup + 1))
     tune = g.autotune_info()
     log(f"{prec}: set-up {time.perf_counter() - t0:.1f} s, "
         f"{g.nmodes} point-modes, autotune {tune}")
@@ -233,7 +240,6 @@ def time_gpu(spec, prec, steps, warmup, graph=False):
     t1 = time.perf_counter()
     prof = {k: g.profile_read(k) for k in range(capi.NUM_KERNELS)}
     g.profile(False)
-    seis = g.seismograms(steps + warmup)
     log(f"{prec}: {steps} steps in {t1 - t0:.3f} s")
     prof["autotune"] = tune
     if graph:
@@ -244,15 +250,42 @@ def time_gpu(spec, prec, steps, warmup, graph=False):
         g.synchronize()
         prof["graph_s"] = time.perf_counter() - t2
         log(f"{prec}: {steps} graph replays in {prof['graph_s']:.3f} s")
+    if total > g.step:
+        t3 = time.perf_counter()
+        g.run(total - g.step)
+        g.synchronize()
+        log(f"{prec}: accuracy leg to step {total} in "
+            f"{time.perf_counter() - t3:.1f} s")
+    seis = g.seismograms(g.step)
     g.close()
     return (t1 - t0), prof, seis
 
 
+def _fft_weight(nr):
+    """Cost weight of an element of Nr slices on the CPU: its transforms
+    grow as Nr log2 Nr."""
+    nr = np.asarray(nr, dtype=np.float64)
+    return nr * np.log2(np.maximum(nr, 2.0))
+
+
+def host_threads():
+    """Threads the host offers this process (OMP_NUM_THREADS when set)."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
 def cpu_baseline(spec, seconds, workload):
-    """The FP64 oracle (a restatement of the reference loop, one core)
-    timed on the host.  Cartesian: the whole workload.  Regional: a
-    stratified sample of 24 blocks of 8 elements across the Nr range with
-    their 3D model, scaled to the whole domain by sum(Nr)."""
+    """The FP64 oracle (a restatement of the reference loop) timed on the
+    host with every host thread (OpenMP over the elements, the forces
+    scattered in element order) and with one.  Cartesian: the whole
+    workload.  Regional: a stratified sample of 24 blocks of 8 elements
+    across the Nr range with their 3D model, scaled to the whole domain by
+    sum(Nr log2 Nr), the growth of its transforms."""
     import oracle
     from axisem3d_amd import preloop, synth
     if workload == "cartesian8000":
@@ -285,28 +318,62 @@ def cpu_baseline(spec, seconds, workload):
             atts.append(a)
         sub = preloop.subdomain(spec, elems, np.concatenate(props),
                                 np.concatenate(atts))
-        nr = spec.elem_nr.astype(np.int64)
-        scale = float(nr.sum()) / float(nr[elems].sum())
+        w = _fft_weight(spec.elem_nr)
+        scale = float(w.sum()) / float(w[elems].sum())
         what = (f"{len(elems)} of {spec.n_solid_elements} elements (24 "
                 "blocks of 8 across the Nr range) with their 3D model and "
-                "attenuation, scaled to the whole domain by sum(Nr)")
-    o = oracle.Oracle(sub)
-    rng = np.random.default_rng(0)
-    o.displ[:] = rng.standard_normal(o.displ.shape) * 1e-9
-    o.run(1, record=False)
-    n, t = 0, 0.0
-    while t < seconds:
-        k = max(1, n)
-        t0 = time.perf_counter()
-        o.run(k, record=False)
-        t += time.perf_counter() - t0
-        n += k
-    step_full = t / n * scale
+                "attenuation, scaled to the whole domain by sum(Nr log2 Nr)")
     nupd = spec.n_updates_per_step()
-    return {"value": nupd / step_full, "unit": UNIT, "cores": 1,
-            "kind": "port",
-            "sample": f"{n} Newmark steps of {what}; FP64 oracle, single "
-                      f"thread, {t:.1f} s"}
+
+    def timed(threads):
+        oracle.set_threads(threads)
+        o = oracle.Oracle(sub)
+        rng = np.random.default_rng(0)
+        o.displ[:] = rng.standard_normal(o.displ.shape) * 1e-9
+        o.run(1, record=False)
+        n, t = 0, 0.0
+        while t < seconds:
+            k = max(1, n)
+            t0 = time.perf_counter()
+            o.run(k, record=False)
+            t += time.perf_counter() - t0
+            n += k
+        return nupd / (t / n * scale), n, t
+
+    nth = host_threads()
+    v_all, n_all, t_all = timed(nth)
+    v_one, n_one, t_one = timed(1)
+    oracle.set_threads(1)
+    return {"value": v_all, "unit": UNIT, "cores": nth, "kind": "port",
+            "sample": f"{n_all} Newmark steps of {what}; FP64 oracle, "
+                      f"{nth} OpenMP threads, {t_all:.1f} s",
+            "cores_1": {"value": v_one, "cores": 1,
+                        "sample": f"{n_one} steps, one thread, "
+                                  f"{t_one:.1f} s"}}
+
+
+def accuracy_leg(spec, seis32, seis64):
+    """The FP32+FP64 seismograms against the FP64 ones over the whole run
+    from rest: relative L2 over all receivers, components and samples,
+    and per receiver, with the step of the source-time function's peak
+    (the wavelet must have passed the near receivers for the figure to
+    mean anything)."""
+    n = seis64.shape[0]
+    per = np.linalg.norm(seis32 - seis64, axis=(0, 2)) / np.maximum(
+        np.linalg.norm(seis64, axis=(0, 2)), 1e-300)
+    amp = np.abs(seis64).max(axis=(0, 2))
+    live = amp > 1e-6 * amp.max() if amp.max() > 0 else amp > 0
+    stf = getattr(spec, "stf", None)
+    out = {"steps": int(n),
+           "stf_peak_step": (int(np.argmax(np.abs(stf[:n])))
+                             if stf is not None else None),
+           "rel_l2": float(np.linalg.norm(seis32 - seis64) /
+                           max(np.linalg.norm(seis64), 1e-300)),
+           "receivers_with_signal": int(live.sum()),
+           "receivers": int(len(per)),
+           "rel_l2_worst_receiver": float(per[live].max()) if live.any()
+           else None}
+    return out
 
 
 def batched_extras(spec, seis32, seis64, steps, warmup, t_batch):
@@ -405,8 +472,11 @@ def main():
         adjoint_bench(a)
         return
     from axisem3d_amd import capi
-    # warm-up, timed steps, one capture step and the graph replays
-    nsteps = 2 * a.steps + a.warmup + 1
+    # warm-up, timed steps, one capture step and the graph replays; both
+    # precisions then run on to the accuracy leg's step count
+    acc = 0 if a.no_fp64 else max(a.accuracy_steps, 2 * a.steps + a.warmup
+                                  + 1)
+    nsteps = max(2 * a.steps + a.warmup + 1, acc)
     t0 = time.perf_counter()
     spec, cfg = build_workload(a.workload, nsteps)
     log(f"workload built in {time.perf_counter() - t0:.1f} s: "
@@ -415,16 +485,19 @@ def main():
     nupd = spec.n_updates_per_step()
 
     t32, prof32, seis32 = time_gpu(spec, "fp32", a.steps, a.warmup,
-                                   graph=True)
+                                   graph=True, total=acc)
     value = nupd * a.steps / t32
     out_64 = None
     l2 = None
+    accuracy = None
     if not a.no_fp64:
-        t64, prof64, seis64 = time_gpu(spec, "fp64", a.steps, a.warmup)
+        t64, prof64, seis64 = time_gpu(spec, "fp64", a.steps, a.warmup,
+                                       total=acc)
         out_64 = {"value": nupd * a.steps / t64,
                   "ms_per_step": t64 / a.steps * 1e3}
         l2 = float(np.linalg.norm(seis32 - seis64) /
                    max(np.linalg.norm(seis64), 1e-300))
+        accuracy = accuracy_leg(spec, seis32, seis64)
 
     names = {capi.KERNEL_STIFFNESS: "solid stiffness, Fourier path",
              capi.KERNEL_STIFFNESS_3D: "solid stiffness, cardinal path "
@@ -476,6 +549,7 @@ def main():
         "fp64_plain": out_64,
         "speedup_vs_fp64": (value / out_64["value"]) if out_64 else None,
         "seismogram_rel_l2_vs_fp64": l2,
+        "accuracy_leg": accuracy,
         "roofline": {"bound": "hbm", "kernel": names[dom],
                      "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,

@@ -325,7 +325,11 @@ int axs3d_stiff_to_accel(axs3d_ctx *ctx);
 /* NewmarkTimeScheme::update on all points (NewmarkTimeScheme.hpp:48-71);
  * expects the stiffness field to hold the acceleration. */
 int axs3d_newmark(axs3d_ctx *ctx);
-/* Domain::checkStability (Domain.cpp:574-612). */
+/* Domain::checkStability (Domain.cpp:574-612): raises when a displacement
+ * is non-finite.  tstep is the step number the message names, counted
+ * from 1 as NewmarkTimeScheme::solve passes it (tstep + 1 of its loop);
+ * runs of axs3d_run instead name the first step that started from a
+ * non-finite displacement. */
 int axs3d_check_stability(axs3d_ctx *ctx, int tstep);
 
 /* ---------------- bulk ---------------------------------------------- */

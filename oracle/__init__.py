@@ -107,11 +107,20 @@ def _setup(L):
     L.oracle_run_coupled.restype = C.c_int
     L.oracle_material_stress.argtypes = [C.c_int, _p, _p, _p]
     L.oracle_rtz_to_spz6.argtypes = [_p, _p]
+    L.oracle_set_threads.argtypes = [C.c_int]
+    # serial unless a caller asks for threads (set_threads)
+    L.oracle_set_threads(1)
     from axisem3d_amd import spectral
     t = spectral.tables()
     g_gll = np.ascontiguousarray(t[4])
     g_glj = np.ascontiguousarray(t[5])
     L.oracle_set_gmat(g_gll.ctypes.data, g_glj.ctypes.data)
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's element loop (results are the same
+    for any count)."""
+    lib().oracle_set_threads(int(n))
 
 
 def _ptr(a):

@@ -25,6 +25,9 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #include "axs3d_oracle.h"
 
@@ -636,8 +639,23 @@ static int64_t *make_offsets(int n, const int32_t *nr) {
  * gFFT_N6.computeR2C -> computeQuad6.  TI and anisotropic media work in
  * RTZ on spherical meshes: transformSPZ_RTZ6 after the gradient and
  * transformRTZ_SPZ6 before the quadrature (SolidElement.cpp:160-196). */
+static void element_forces(const oracle_domain *d, int e, const cplx *displ,
+                           cplx *fout);
+static void element_scatter(const oracle_domain *d, int e, const cplx *fout,
+                            cplx *stiff);
+
 void oracle_solid_element_stiff(const oracle_domain *d, int e,
                                 const cplx *displ, cplx *stiff) {
+    int nu1 = d->elem_nr[e] / 2 + 1;
+    cplx *f = (cplx *)malloc(sizeof(cplx) * nu1 * 3 * N);
+    element_forces(d, e, displ, f);
+    element_scatter(d, e, f, stiff);
+    free(f);
+}
+
+/* The element's forces, fout[alpha][c][p] (before the subtraction). */
+static void element_forces(const oracle_domain *d, int e, const cplx *displ,
+                           cplx *fout) {
     const int32_t *pts = d->elem_points + (int64_t)e * N;
     int nr = d->elem_nr[e];
     int nu1 = nr / 2 + 1;
@@ -647,7 +665,7 @@ void oracle_solid_element_stiff(const oracle_domain *d, int e,
     int mt = elem_mt(d, e), rtz = elem_rtz(d, e);
     cplx *eps = (cplx *)malloc(sizeof(cplx) * nu1 * 6 * N);
     cplx *sig = (cplx *)malloc(sizeof(cplx) * nu1 * 6 * N);
-    cplx u[3 * N], f[3 * N];
+    cplx u[3 * N];
     for (int alpha = 0; alpha < nu1; alpha++) {
         for (int p = 0; p < N; p++) {
             int ip = pts[p];
@@ -700,7 +718,21 @@ void oracle_solid_element_stiff(const oracle_domain *d, int e,
         if (rtz)
             oracle_rtz_to_spz6(d->elem_theta + (int64_t)e * N,
                                sig + (int64_t)alpha * 6 * N);
-        oracle_quad6(gq_e, axial, alpha, sig + (int64_t)alpha * 6 * N, f);
+        oracle_quad6(gq_e, axial, alpha, sig + (int64_t)alpha * 6 * N,
+                     fout + (int64_t)alpha * 3 * N);
+    }
+    free(eps);
+    free(sig);
+}
+
+/* addStiffToPoints: point stiff -= element force, over the point's own
+ * orders (SolidPoint.hpp:101-109). */
+static void element_scatter(const oracle_domain *d, int e, const cplx *fout,
+                            cplx *stiff) {
+    const int32_t *pts = d->elem_points + (int64_t)e * N;
+    int nu1 = d->elem_nr[e] / 2 + 1;
+    for (int alpha = 0; alpha < nu1; alpha++) {
+        const cplx *f = fout + (int64_t)alpha * 3 * N;
         for (int p = 0; p < N; p++) {
             int ip = pts[p];
             int pnu1 = d->point_nr[ip] / 2 + 1;
@@ -709,8 +741,41 @@ void oracle_solid_element_stiff(const oracle_domain *d, int e,
                     stiff[(d->offsets[ip] + alpha) * 3 + c] -= f[c * N + p];
         }
     }
-    free(eps);
-    free(sig);
+}
+
+/* Stiffness of all solid elements, subtracted into stiff in ascending
+ * element order (Domain::computeStiffness).  The element forces are
+ * computed by the OpenMP threads in any order, then scattered by one
+ * thread in element order, so the result does not depend on the thread
+ * count. */
+static void solid_stiffness(const oracle_domain *d, const cplx *displ,
+                            cplx *stiff) {
+    int ne = d->n_elem;
+    int64_t *foff = (int64_t *)malloc(sizeof(int64_t) * (ne + 1));
+    foff[0] = 0;
+    for (int e = 0; e < ne; e++)
+        foff[e + 1] = foff[e] + (int64_t)(d->elem_nr[e] / 2 + 1) * 3 * N;
+    cplx *fbuf = (cplx *)malloc(sizeof(cplx) * (size_t)foff[ne]);
+    /* the transform tables of every Nr before the threads share them */
+    for (int e = 0; e < ne; e++)
+        if (elem_is3d(d, e)) {
+            twiddles(d->elem_nr[e], -1);
+            twiddles(d->elem_nr[e], +1);
+        }
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int e = 0; e < ne; e++) element_forces(d, e, displ, fbuf + foff[e]);
+    for (int e = 0; e < ne; e++) element_scatter(d, e, fbuf + foff[e], stiff);
+    free(fbuf);
+    free(foff);
+}
+
+/* Threads of the element loop (OpenMP; 1 = serial). */
+void oracle_set_threads(int n) {
+#ifdef _OPENMP
+    omp_set_num_threads(n > 0 ? n : 1);
+#else
+    (void)n;
+#endif
 }
 
 /* SolidMoment::apply + SolidElement::addMomentSource (SolidMoment.cpp:29-33,
@@ -1044,8 +1109,7 @@ int oracle_run_coupled(const oracle_domain *d, int step_begin, int nsteps,
     int64_t nmf = d->n_fpoints > 0 ? d->foffsets[d->n_fpoints] : 0;
     for (int ts = step_begin; ts < step_begin + nsteps; ts++) {
         if (d->stf) oracle_apply_moment_source(d, d->stf[ts], stiff);
-        for (int e = 0; e < d->n_elem; e++)
-            oracle_solid_element_stiff(d, e, displ, stiff);
+        solid_stiffness(d, displ, stiff);
         for (int e = 0; e < d->n_felem; e++)
             oracle_fluid_element_stiff(d, e, fdispl, fstiff);
         oracle_coupling(d, displ, stiff, fstiff);
@@ -1079,8 +1143,7 @@ int oracle_run(const oracle_domain *d, int step_begin, int nsteps,
                double *seismo) {
     for (int ts = step_begin; ts < step_begin + nsteps; ts++) {
         if (d->stf) oracle_apply_moment_source(d, d->stf[ts], stiff);
-        for (int e = 0; e < d->n_elem; e++)
-            oracle_solid_element_stiff(d, e, displ, stiff);
+        solid_stiffness(d, displ, stiff);
         if (seismo && d->n_rec > 0)
             oracle_record(d, displ, seismo + (int64_t)(ts - step_begin) *
                                                  d->n_rec * 3);

@@ -157,4 +157,8 @@ void oracle_att_apply_cd(const oracle_domain *d, int e, int slice,
                          const double *strain, double *stress);
 void oracle_free(void *p);
 
+/* Threads of the element loop of oracle_run / oracle_run_coupled
+ * (OpenMP; the result does not depend on the count). */
+void oracle_set_threads(int n);
+
 #endif

@@ -149,8 +149,10 @@ def test_stability_flag_raises():
     with pytest.raises(Axs3dError, match="checkStability"):
         g.run(1)
     g.close()
-    # the message names the step whose update first went non-finite, not
-    # the end of the batch (NewmarkTimeScheme.cpp:72-74 checks per step)
+    # the message names the step at which the reference's per-step check
+    # (NewmarkTimeScheme.cpp:72-74, at the start of step tstep, reported
+    # as tstep + 1) sees the non-finite displacement, not the end of the
+    # batch: started from a NaN at step 3, it reports 4
     for graph in (True, False):
         g = _solver(spec, "fp64")
         g.use_graph(graph)
@@ -158,6 +160,6 @@ def test_stability_flag_raises():
         u = g.get_field("displ")
         u[5, 1] = np.nan
         g.set_field("displ", u)
-        with pytest.raises(Axs3dError, match=r"current step  =  3\b"):
+        with pytest.raises(Axs3dError, match=r"current step  =  4\b"):
             g.run(10)
         g.close()
