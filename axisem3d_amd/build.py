@@ -28,7 +28,7 @@ COMMON = [
 ]
 
 SOURCES = ["axs3d_kernels.hip", "axs3d_cardinal.hip", "axs3d_fluid.hip",
-           "axs3d_adjoint.hip",
+           "axs3d_adjoint.hip", "axs3d_scratch.hip",
            "axs3d_domain.cpp",
            "axs3d_capi.cpp"]
 

@@ -62,6 +62,7 @@ SIGNATURES = {
     "axs3d_finalize": (_i, [_p]),
     "axs3d_apply_sources": (_i, [_p, _i]),
     "axs3d_compute_stiffness": (_i, [_p]),
+    "axs3d_compute_stiffness_static": (_i, [_p]),
     "axs3d_apply_bc_before": (_i, [_p]),
     "axs3d_record": (_i, [_p, _i]),
     "axs3d_apply_bc_after": (_i, [_p]),

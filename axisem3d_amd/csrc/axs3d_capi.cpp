@@ -295,6 +295,11 @@ int axs3d_compute_stiffness(axs3d_ctx *ctx) {
     return guarded(ctx, [&](axs3d::DomainBase &d) { d.computeStiffness(); });
 }
 
+int axs3d_compute_stiffness_static(axs3d_ctx *ctx) {
+    return guarded(ctx,
+                   [&](axs3d::DomainBase &d) { d.computeStiffnessStatic(); });
+}
+
 int axs3d_apply_bc_before(axs3d_ctx *ctx) {
     return guarded(ctx, [&](axs3d::DomainBase &d) {
         d.applyBC_BeforeAssemblingStiff();

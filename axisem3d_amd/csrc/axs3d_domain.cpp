@@ -54,6 +54,9 @@ DomainBase::DomainBase(int device, int precision)
     hip_check(hipSetDevice(device), "Domain::Domain");
     hip_check(hipStreamCreateWithFlags(&mStream, hipStreamNonBlocking),
               "Domain::Domain");
+    // this device's address of the static scratch, looked up here at set-up
+    // (with the device current) so that the static entry point never does
+    mStaticScratch = static_scratch(device);
 }
 
 DomainBase::~DomainBase() {
@@ -1486,8 +1489,11 @@ public:
     // The cardinal path, batch by batch; with timing, each of the three
     // kernels is timed under its own id.
     void cardinalPath(bool timed) {
-        const CardArgs<Real> c = cardArgs();
-        for (const CardBatch &bt : mCardBatches) {
+        cardinalPathOn(cardArgs(), mCardBatches, timed);
+    }
+    void cardinalPathOn(const CardArgs<Real> &c,
+                        const std::vector<CardBatch> &batches, bool timed) {
+        for (const CardBatch &bt : batches) {
             if (timed) tick(AXS3D_KERNEL_CD_GRAD);
             launch_cd_grad<Real>(mStream, c, bt);
             if (timed) {
@@ -1504,7 +1510,9 @@ public:
         }
     }
 
-    void elementKernel() {
+    // The element kernels; with stat, the cardinal paths run through the
+    // static scratch in the batches planned for it.
+    void elementKernel(bool stat = false) {
         if (mNumEM > 0) {
             tick(AXS3D_KERNEL_STIFFNESS);
             launch_solid_stiff_fr<Real>(mStream, stiffArgs());
@@ -1512,7 +1520,14 @@ public:
         }
         if (mNum3D > 0) {
             tick(AXS3D_KERNEL_STIFFNESS_3D);
-            cardinalPath(true);
+            if (stat) {
+                CardArgs<Real> c = cardArgs();
+                c.scratch = static_cast<Real *>(mStaticScratch);
+                c.scr_off = dScrOffS.as<uint32_t>();
+                cardinalPathOn(c, mCardBatchesS, true);
+            } else {
+                cardinalPath(true);
+            }
             tock(AXS3D_KERNEL_STIFFNESS_3D);
         }
         if (mNumFEM > 0 || mNumF3D > 0) {
@@ -1520,12 +1535,33 @@ public:
             if (mNumFEM > 0)
                 launch_fluid_stiff_fr<Real>(mStream, fluidArgs());
             if (mNumF3D > 0) {
-                const FluidCardArgs<Real> fc = fluidCardArgs();
-                for (const CardBatch &bt : mFCardBatches)
+                FluidCardArgs<Real> fc = fluidCardArgs();
+                if (stat) {
+                    fc.scratch = static_cast<Real *>(mStaticScratch);
+                    fc.scr_off = dFScrOffS.as<uint32_t>();
+                }
+                for (const CardBatch &bt : stat ? mFCardBatchesS
+                                                : mFCardBatches)
                     launch_cdf_batch<Real>(mStream, fc, bt);
             }
             tock(AXS3D_KERNEL_FLUID_STIFFNESS);
         }
+    }
+
+    void computeStiffnessStatic() override {
+        requireFinalized("Domain::computeStiffnessStatic");
+        if (mStaticNeedMax > STATIC_SCRATCH_BYTES)
+            throw Error("Domain::computeStiffnessStatic || An element needs " +
+                        std::to_string(mStaticNeedMax) +
+                        " bytes of cardinal scratch; the static scratch "
+                        "holds " +
+                        std::to_string(STATIC_SCRATCH_BYTES) + ".");
+        elementKernel(true);
+        launch_assemble<Real>(mStream, pointGeom(), dEB.as<Real>(),
+                              dStiff.as<Real>());
+        if (mNumFElems > 0)
+            launch_fluid_assemble<Real>(mStream, fluidGeom(), dFEB.as<Real>(),
+                                        dFStiff.as<Real>());
     }
 
     // Domain::computeStiffness (stage form): element kernels, then the
@@ -1864,39 +1900,118 @@ private:
     // the scratch together, and the 3D element-modes in list order for the
     // gradient and quadrature kernels.  The scratch also holds the 3D
     // fluid batches' rows.
-    void buildCardBatches(size_t bytes) {
+    // Batches of the 3D solid elements for a scratch of `bytes`, their
+    // elements' scratch offsets (complex values) and the largest batch
+    // (complex values); the element-mode lists do not depend on the
+    // budget.
+    void planCardBatches(size_t bytes, std::vector<CardBatch> &batches,
+                         std::vector<uint32_t> &scrOff,
+                         std::vector<EMItem> &em3d,
+                         std::vector<EMItem> &emAx, size_t &scrMax) const {
         const size_t budget =
             std::min(bytes / sizeof(cpx<Real>), (size_t)UINT32_MAX);
-        std::vector<uint32_t> scrOff(mNumElems, 0u);
-        std::vector<EMItem> em3d, emAx;
-        mCardBatches.clear();
-        size_t scrMax = 0;
+        scrOff.assign(mNumElems, 0u);
+        em3d.clear();
+        emAx.clear();
+        batches.clear();
+        scrMax = 0;
         for (int mt = 0; mt < 3; mt++) {
             size_t used = 0;
             for (int w = mTypeBegin3D[mt]; w < mTypeBegin3D[mt + 1]; w++) {
                 const int e = hList3D[w];
                 const int nu1 = hElemNr[e] / 2 + 1;
                 const size_t need = (size_t)6 * NPEM * scr_row(nu1);
-                if (mCardBatches.empty() || mCardBatches.back().mtype != mt ||
+                if (batches.empty() || batches.back().mtype != mt ||
                     used + need > budget) {
-                    mCardBatches.push_back(
+                    batches.push_back(
                         {mt, w, w, (int)em3d.size(), (int)em3d.size(),
                          (int)emAx.size(), (int)emAx.size()});
                     used = 0;
                 }
                 scrOff[e] = (uint32_t)used;
                 used += need;
-                mCardBatches.back().nr_max =
-                    std::max(mCardBatches.back().nr_max, (int)hElemNr[e]);
+                batches.back().nr_max =
+                    std::max(batches.back().nr_max, (int)hElemNr[e]);
                 for (int al = 0; al < nu1; al++) em3d.push_back({e, al});
                 if (hElemAxial[e])
                     for (int al = 0; al < nu1; al++) emAx.push_back({e, al});
-                mCardBatches.back().end = w + 1;
-                mCardBatches.back().em_end = (int)em3d.size();
-                mCardBatches.back().ax_end = (int)emAx.size();
+                batches.back().end = w + 1;
+                batches.back().em_end = (int)em3d.size();
+                batches.back().ax_end = (int)emAx.size();
                 scrMax = std::max(scrMax, used);
             }
         }
+    }
+
+    // Batches of the 3D fluid elements ([3][25][nu1] rows each) for a
+    // scratch of `bytes`, as planCardBatches.
+    void planFluidCardBatches(size_t bytes, std::vector<CardBatch> &batches,
+                              std::vector<uint32_t> &fscr,
+                              std::vector<EMItem> &fem3d,
+                              size_t &scrMax) const {
+        const size_t budget =
+            std::min(bytes / sizeof(cpx<Real>), (size_t)UINT32_MAX);
+        fscr.assign(mNumFElems, 0u);
+        fem3d.clear();
+        batches.clear();
+        size_t used = 0;
+        scrMax = 0;
+        for (int w = 0; w < mNumF3D; w++) {
+            const int e = hFList3D[w];
+            const int nu1 = hFElemNr[e] / 2 + 1;
+            const size_t need = (size_t)3 * NPEM * scr_row(nu1);
+            if (batches.empty() || used + need > budget) {
+                batches.push_back(
+                    {0, w, w, (int)fem3d.size(), (int)fem3d.size()});
+                used = 0;
+            }
+            fscr[e] = (uint32_t)used;
+            used += need;
+            batches.back().nr_max =
+                std::max(batches.back().nr_max, (int)hFElemNr[e]);
+            for (int al = 0; al < nu1; al++) fem3d.push_back({e, al});
+            batches.back().end = w + 1;
+            batches.back().em_end = (int)fem3d.size();
+            scrMax = std::max(scrMax, used);
+        }
+    }
+
+    // The batches of the static scratch (solid and fluid 3D elements), and
+    // the largest scratch a single element needs; set up at finalize.
+    void planStaticBatches() {
+        std::vector<uint32_t> off;
+        std::vector<EMItem> em, ax;
+        size_t mx = 0;
+        mStaticNeedMax = 0;
+        for (int32_t e : hList3D)
+            mStaticNeedMax = std::max(
+                mStaticNeedMax, (size_t)6 * NPEM *
+                                    scr_row(hElemNr[e] / 2 + 1) *
+                                    sizeof(cpx<Real>));
+        for (int32_t e : hFList3D)
+            mStaticNeedMax = std::max(
+                mStaticNeedMax, (size_t)3 * NPEM *
+                                    scr_row(hFElemNr[e] / 2 + 1) *
+                                    sizeof(cpx<Real>));
+        if (mNum3D > 0) {
+            planCardBatches(STATIC_SCRATCH_BYTES, mCardBatchesS, off, em, ax,
+                            mx);
+            dScrOffS.alloc(off.size() * 4);
+            dScrOffS.upload(off.data(), off.size() * 4, mStream);
+        }
+        if (mNumF3D > 0) {
+            planFluidCardBatches(STATIC_SCRATCH_BYTES, mFCardBatchesS, off,
+                                 em, mx);
+            dFScrOffS.alloc(off.size() * 4);
+            dFScrOffS.upload(off.data(), off.size() * 4, mStream);
+        }
+    }
+
+    void buildCardBatches(size_t bytes) {
+        std::vector<uint32_t> scrOff;
+        std::vector<EMItem> em3d, emAx;
+        size_t scrMax = 0;
+        planCardBatches(bytes, mCardBatches, scrOff, em3d, emAx, scrMax);
         dScrOff.alloc(scrOff.size() * 4);
         dScrOff.upload(scrOff.data(), scrOff.size() * 4, mStream);
         dEM3D.alloc(em3d.size() * sizeof(EMItem));
@@ -2203,30 +2318,10 @@ private:
             if (const char *env = std::getenv("AXS3D_CARD_SCRATCH_MB"))
                 budget = (size_t)std::max(
                     1.0, std::atof(env) * (double)((size_t)1 << 20));
-            budget = std::min(budget / sizeof(cpx<Real>),
-                              (size_t)UINT32_MAX);
-            std::vector<uint32_t> fscr(mNumFElems, 0u);
+            std::vector<uint32_t> fscr;
             std::vector<EMItem> fem3d;
-            mFCardBatches.clear();
-            size_t used = 0, scrMax = 0;
-            for (int w = 0; w < mNumF3D; w++) {
-                const int e = hFList3D[w];
-                const int nu1 = hFElemNr[e] / 2 + 1;
-                const size_t need = (size_t)3 * NPEM * scr_row(nu1);
-                if (mFCardBatches.empty() || used + need > budget) {
-                    mFCardBatches.push_back(
-                        {0, w, w, (int)fem3d.size(), (int)fem3d.size()});
-                    used = 0;
-                }
-                fscr[e] = (uint32_t)used;
-                used += need;
-                mFCardBatches.back().nr_max =
-                    std::max(mFCardBatches.back().nr_max, (int)hFElemNr[e]);
-                for (int al = 0; al < nu1; al++) fem3d.push_back({e, al});
-                mFCardBatches.back().end = w + 1;
-                mFCardBatches.back().em_end = (int)fem3d.size();
-                scrMax = std::max(scrMax, used);
-            }
+            size_t scrMax = 0;
+            planFluidCardBatches(budget, mFCardBatches, fscr, fem3d, scrMax);
             dFScrOff.alloc(fscr.size() * 4);
             dFScrOff.upload(fscr.data(), fscr.size() * 4, st);
             dFEM3D.alloc(fem3d.size() * sizeof(EMItem));
@@ -2238,6 +2333,7 @@ private:
                 dCardScratch.alloc(scrMax * sizeof(cpx<Real>));
             }
         }
+        planStaticBatches();
         if (hSrcElem >= 0) {
             std::vector<Real> pat(hSrcPattern.begin(), hSrcPattern.end());
             dSrcPattern.alloc(pat.size() * sizeof(Real));

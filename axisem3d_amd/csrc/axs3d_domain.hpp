@@ -30,6 +30,13 @@ struct Error : public std::runtime_error {
 
 void hip_check(hipError_t e, const char *where);
 
+// Size of the file-scope device scratch of the allocation-free stiffness
+// entry point (axs3d_compute_stiffness_static), per device.
+constexpr size_t STATIC_SCRATCH_BYTES = (size_t)64 << 20;
+// Device address of the current device's copy of that scratch (looked up
+// once per device with hipGetSymbolAddress, axs3d_scratch.hip).
+void *static_scratch(int device);
+
 // Owning device allocation.
 class DeviceBuffer {
 public:
@@ -106,6 +113,11 @@ public:
     // time-loop phases (Domain.cpp:285-345)
     virtual void applySources(int tstep) = 0;
     virtual void computeStiffness() = 0;
+    // computeStiffness with the cardinal path through the static device
+    // scratch (STATIC_SCRATCH_BYTES) in batches planned for it at
+    // finalize; allocates nothing.  Calls that overlap on one device race
+    // on that scratch.
+    virtual void computeStiffnessStatic() = 0;
     virtual void applyBC_BeforeAssemblingStiff() = 0;
     // isample < 0 with dev: the sample follows the device step counter
     void recordOutput(int isample, bool dev = false);
@@ -262,6 +274,13 @@ protected:
     bool mScratchFixed = false;         // set by AXS3D_CARD_SCRATCH_MB
     int mTuneTrials = 0;                // scratch sizes timed
     size_t mFCardScrMax = 0;            // 3D fluid rows in the scratch
+    // the same batches planned for the static scratch (this device's
+    // address of it in mStaticScratch), and the largest scratch one
+    // element needs (bytes; above STATIC_SCRATCH_BYTES the static entry
+    // point refuses to run)
+    void *mStaticScratch = nullptr;
+    std::vector<CardBatch> mCardBatchesS, mFCardBatchesS;
+    size_t mStaticNeedMax = 0;
     std::vector<uint8_t> hProp3DSet, hAtt3DSet;
     // attenuation
     int mNsls = 0, mAttCG4 = 0;
@@ -288,7 +307,7 @@ protected:
         dProp3D,
         dTwiddle,
         dTwOff, dPlan, dCardScratch, dAttABG, dAtt3D, dMemVar, dDStress,
-        dAtt1D, dMemVarFR, dDStressFR;
+        dAtt1D, dMemVarFR, dDStressFR, dScrOffS, dFScrOffS;
     // 3D operands of elements [e_begin, e_end), host layout [slice][width]
     // in element order; stored per element as [width][Nr] so that
     // consecutive slices are consecutive in memory.

@@ -312,6 +312,18 @@ int axs3d_finalize(axs3d_ctx *ctx);
 int axs3d_apply_sources(axs3d_ctx *ctx, int tstep);
 /* Domain::computeStiffness (Domain.cpp:298-308). */
 int axs3d_compute_stiffness(axs3d_ctx *ctx);
+/* No counterpart in the reference.  For callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops):
+ * axs3d_compute_stiffness with the cardinal path's strain and stress rows
+ * in a file-scope device array of 64 MiB per device (held until the
+ * process ends) instead of the context's scratch, in batches planned for
+ * it at axs3d_finalize; bitwise the same stiffness.  Calls no hipMalloc,
+ * hipFree or other allocating routine.  Returns 1 (with the message in
+ * axs3d_last_error) when one 3D element's rows exceed 64 MiB.  Every
+ * context on a device shares that device's array: calls that overlap on
+ * one device (contexts on different streams) race.  Nothing in the array
+ * needs to start at zero: each batch writes every row it reads. */
+int axs3d_compute_stiffness_static(axs3d_ctx *ctx);
 /* Domain::applyBC_BeforeAssemblingStiff (Domain.cpp:313-317). */
 int axs3d_apply_bc_before(axs3d_ctx *ctx);
 /* Domain::recordOutput for station displacement (Domain.cpp:385-409),
