@@ -42,20 +42,29 @@ def _run(cmd):
     return r.stdout
 
 
-def build(verbose=False):
+def build(verbose=False, jobs=None):
+    """Compiles every source (in parallel, `jobs` hipcc processes at a
+    time, default min(8, CPUs)) and links the library."""
+    from concurrent.futures import ThreadPoolExecutor
     os.makedirs(BUILD, exist_ok=True)
-    objs = []
     t0 = time.time()
+    cmds, objs = [], []
     for src in SOURCES:
         path = os.path.join(CSRC, src)
         obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
         cmd = [HIPCC] + COMMON + ["-c", path, "-o", obj]
         if src.endswith(".cpp"):
             cmd = [HIPCC] + COMMON + ["-x", "hip", "-c", path, "-o", obj]
-        out = _run(cmd)
-        if verbose and out.strip():
-            print(out)
+        cmds.append(cmd)
         objs.append(obj)
+    if jobs is None:
+        jobs = max(1, min(8, os.cpu_count() or 1))
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        outs = list(ex.map(_run, cmds))
+    if verbose:
+        for out in outs:
+            if out.strip():
+                print(out)
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp]
          + objs)
