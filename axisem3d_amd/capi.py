@@ -38,6 +38,7 @@ SIGNATURES = {
     "axs3d_set_solid_iso1d": (_i, [_p, _p]),
     "axs3d_set_axial_points": (_i, [_p, _i, _p]),
     "axs3d_set_solid_mass3d": (_i, [_p, _i, _p, _p]),
+    "axs3d_set_ocean_load": (_i, [_p, _i, _p, _p, _p]),
     "axs3d_set_fluid_kinds": (_i, [_p, _p]),
     "axs3d_set_fluid_k3d": (_i, [_p, _i, _i, _p]),
     "axs3d_set_element_kinds": (_i, [_p, _p]),
@@ -58,6 +59,11 @@ SIGNATURES = {
     "axs3d_set_rtz_theta": (_i, [_p, _p]),
     "axs3d_set_moment_source": (_i, [_p, _i, _i, _p, _i, _p]),
     "axs3d_set_receivers": (_i, [_p, _i, _p, _p, _p, _i]),
+    "axs3d_set_receiver_frames": (_i, [_p, _i, _p, _p, _p, _i]),
+    "axs3d_add_clayton_solid": (_i, [_p, _i, _p, _p, _p, _p, _p]),
+    "axs3d_add_clayton_solid3d": (_i, [_p, _i, _p, _p, _p, _p, _p]),
+    "axs3d_add_clayton_fluid": (_i, [_p, _i, _p, _i, _p, _p]),
+    "axs3d_add_sponge": (_i, [_p, _i, _i, _p, _i, _p]),
     "axs3d_set_dt": (_i, [_p, _d]),
     "axs3d_finalize": (_i, [_p]),
     "axs3d_apply_sources": (_i, [_p, _i]),
@@ -66,6 +72,10 @@ SIGNATURES = {
     "axs3d_apply_bc_before": (_i, [_p]),
     "axs3d_record": (_i, [_p, _i]),
     "axs3d_apply_bc_after": (_i, [_p]),
+    "axs3d_apply_bc_after_accel": (_i, [_p]),
+    "axs3d_set_time_scheme": (_i, [_p, _i]),
+    "axs3d_symplectic_launch": (_i, [_p]),
+    "axs3d_symplectic_update": (_i, [_p, _i]),
     "axs3d_stiff_to_accel": (_i, [_p]),
     "axs3d_newmark": (_i, [_p]),
     "axs3d_check_stability": (_i, [_p, _i]),
@@ -90,6 +100,8 @@ SIGNATURES = {
     "axs3d_fft_check": (_i, [_i, _i, _i, _p, C.POINTER(_i), C.POINTER(_i)]),
     "axs3d_autotune_info": (_i, [_p, C.POINTER(_i), C.POINTER(_i),
                                  C.POINTER(C.c_double)]),
+    "axs3d_autotune_report": (_i, [_p, C.POINTER(C.c_double), _i,
+                                   C.POINTER(_i)]),
 }
 
 _lib = None

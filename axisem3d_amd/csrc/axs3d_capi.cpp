@@ -233,6 +233,13 @@ int axs3d_set_solid_mass3d(axs3d_ctx *ctx, int n, const int32_t *points,
     });
 }
 
+int axs3d_set_ocean_load(axs3d_ctx *ctx, int n, const int32_t *points,
+                         const int32_t *nval, const double *values) {
+    return setup(ctx, "Domain::set_ocean_load", [&](axs3d::DomainBase &d) {
+        d.setOceanLoad(n, points, nval, values);
+    });
+}
+
 int axs3d_set_moment_source(axs3d_ctx *ctx, int elem, int nu1,
                             const double *pattern, int nsteps,
                             const double *stf) {
@@ -279,6 +286,59 @@ int axs3d_set_receivers(axs3d_ctx *ctx, int n, const int32_t *elem,
     });
 }
 
+int axs3d_set_receiver_frames(axs3d_ctx *ctx, int n, const int32_t *wcs,
+                              const double *theta,
+                              const double *back_azimuth, int cartesian) {
+    if (n > 0 && (!wcs || !theta || !back_azimuth)) return 2;
+    return setup(ctx, "Domain::set_receiver_frames",
+                 [&](axs3d::DomainBase &d) {
+                     d.setReceiverFrames(n, wcs, theta, back_azimuth,
+                                         cartesian);
+                 });
+}
+
+int axs3d_add_clayton_solid(axs3d_ctx *ctx, int n, const int32_t *points,
+                            const double *rho_vp, const double *rho_vs,
+                            const double *area, const double *theta) {
+    if (n < 0 || (n > 0 && (!points || !rho_vp || !rho_vs || !area ||
+                            !theta)))
+        return 2;
+    return setup(ctx, "Domain::add_clayton_solid", [&](axs3d::DomainBase &d) {
+        d.addClaytonSolid(n, points, rho_vp, rho_vs, area, theta);
+    });
+}
+
+int axs3d_add_clayton_solid3d(axs3d_ctx *ctx, int n, const int32_t *points,
+                              const double *rho_vp, const double *rho_vs,
+                              const double *area,
+                              const double *unit_normal) {
+    if (n < 0 || (n > 0 && (!points || !rho_vp || !rho_vs || !area ||
+                            !unit_normal)))
+        return 2;
+    return setup(ctx, "Domain::add_clayton_solid3d",
+                 [&](axs3d::DomainBase &d) {
+                     d.addClaytonSolid3D(n, points, rho_vp, rho_vs, area,
+                                         unit_normal);
+                 });
+}
+
+int axs3d_add_clayton_fluid(axs3d_ctx *ctx, int n, const int32_t *points,
+                            int per_slice, const double *rho_vp,
+                            const double *area) {
+    if (n < 0 || (n > 0 && (!points || !rho_vp || !area))) return 2;
+    return setup(ctx, "Domain::add_clayton_fluid", [&](axs3d::DomainBase &d) {
+        d.addClaytonFluid(n, points, per_slice, rho_vp, area);
+    });
+}
+
+int axs3d_add_sponge(axs3d_ctx *ctx, int fluid, int n, const int32_t *points,
+                     int per_slice, const double *gamma) {
+    if (n < 0 || (n > 0 && (!points || !gamma))) return 2;
+    return setup(ctx, "Domain::add_sponge", [&](axs3d::DomainBase &d) {
+        d.addSponge(fluid, n, points, per_slice, gamma);
+    });
+}
+
 int axs3d_set_dt(axs3d_ctx *ctx, double dt) {
     return setup(ctx, "Domain::set_dt", [&](axs3d::DomainBase &d) { d.setDt(dt); });
 }
@@ -313,6 +373,28 @@ int axs3d_record(axs3d_ctx *ctx, int isample) {
 int axs3d_apply_bc_after(axs3d_ctx *ctx) {
     return guarded(ctx, [&](axs3d::DomainBase &d) {
         d.applyBC_AfterAssemblingStiff();
+    });
+}
+
+int axs3d_set_time_scheme(axs3d_ctx *ctx, int scheme) {
+    return setup(ctx, "Domain::set_time_scheme", [&](axs3d::DomainBase &d) {
+        d.setTimeScheme(scheme);
+    });
+}
+
+int axs3d_symplectic_launch(axs3d_ctx *ctx) {
+    return guarded(ctx,
+                   [&](axs3d::DomainBase &d) { d.symplecticLaunch(); });
+}
+
+int axs3d_symplectic_update(axs3d_ctx *ctx, int isub) {
+    return guarded(ctx,
+                   [&](axs3d::DomainBase &d) { d.symplecticUpdate(isub); });
+}
+
+int axs3d_apply_bc_after_accel(axs3d_ctx *ctx) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) {
+        d.applyBC_AfterComputingAccel();
     });
 }
 
@@ -434,6 +516,13 @@ int axs3d_fft_check(int nr, int max_radix, int inverse, double *data,
     }
     *npasses = (int)rad.size();
     for (size_t i = 0; i < rad.size(); i++) radices[i] = rad[i];
+    return 0;
+}
+
+int axs3d_autotune_report(const axs3d_ctx *ctx, double *out, int n,
+                          int *written) {
+    if (!ctx || !out || !written || n < 0) return 2;
+    *written = ctx->dom->autotuneReport(out, n);
     return 0;
 }
 

@@ -765,17 +765,39 @@ __global__ __launch_bounds__(128) void k_mass3d(Mass3DArgs<Real> a,
     wave_sync();
     wave_fft<Real>(b, plan, true, tw, lane);
     const Real *im = a.inv_mass + a.inv_off[i];
-    for (int k = lane; k < nr; k += 64) {
-        const int pk = pad(pos[k]);
-        const Real m = im[k];
-        const cpx<Real> z = b[pk];
-        b[pk] = {z.re * m, z.im * m};
+    const int64_t ko = a.k_off != nullptr ? a.k_off[i] : -1;
+    if (ko >= 0) {
+        // MassOceanLoad3D::computeAccel (MassOceanLoad3D.cpp:41-56): per
+        // slice a = F / m - (F . k) k, with F = (s, phi) from wave 0's
+        // buffer and z from wave 1's (block-uniform branch)
+        __syncthreads();
+        cpx<Real> *b0 = L.buf(0), *b1 = L.buf(1);
+        const Real *kk = a.ocean_k + ko;
+        for (int k = t; k < nr; k += 128) {
+            const int pk = pad(pos[k]);
+            const Real m = im[k];
+            const Real ks = kk[3 * k], kp = kk[3 * k + 1], kz = kk[3 * k + 2];
+            const cpx<Real> sp = b0[pk];
+            const Real fz = b1[pk].re;
+            const Real dot = sp.re * ks + sp.im * kp + fz * kz;
+            b0[pk] = {sp.re * m - dot * ks, sp.im * m - dot * kp};
+            b1[pk] = {fz * m - dot * kz, (Real)0};
+        }
+        __syncthreads();
+    } else {
+        for (int k = lane; k < nr; k += 64) {
+            const int pk = pad(pos[k]);
+            const Real m = im[k];
+            const cpx<Real> z = b[pk];
+            b[pk] = {z.re * m, z.im * m};
+        }
+        wave_sync();
     }
-    wave_sync();
     wave_fft<Real>(b, plan, false, tw, lane);
     // R2C output (as unpack_pair): X into component cx, Y into 1 (wave 0)
     const Real h0 = (Real)0.5 / (Real)nr;
     bool bad = false, bad0 = false;
+    const int ai = a.abc.idx != nullptr ? a.abc.idx[p] : -1;
     for (int k = lane; k < nu1; k += 64) {
         const cpx<Real> zk = b[pad(k)];
         const cpx<Real> zn = b[pad(k == 0 ? 0 : nr - k)];
@@ -793,15 +815,30 @@ __global__ __launch_bounds__(128) void k_mass3d(Mass3DArgs<Real> a,
                 continue;
             }
             const double hdt = .5 * a.dt, hdt2 = .5 * a.dt * a.dt;
-            const double an[2] = {(double)v[c].re, (double)v[c].im};
+            double an[2] = {(double)v[c].re, (double)v[c].im};
+            if (ai >= 0) {
+                // sponge (Sponge.hpp:74-96), after the mass
+                const double *tt =
+                    a.abc.sterm +
+                    ((size_t)a.abc.term_off[ai] + k) * 6 + 2 * (q == 0 ? c : 2);
+                an[0] -= tt[0];
+                an[1] -= tt[1];
+            }
 #pragma unroll
             for (int ri = 0; ri < 2; ri++) {
                 const double u0 = a.displ[m6 + ri];
-                const double vv = a.veloc[m6 + ri] +
-                                  hdt * (a.accel[m6 + ri] + an[ri]);
-                const double uu = u0 + (a.dt * vv + hdt2 * an[ri]);
+                double vv, uu;
+                if (a.scheme == 1) {
+                    // SymplecticTimeScheme::update
+                    vv = a.veloc[m6 + ri] + a.sym_pi * a.dt * an[ri];
+                    uu = u0 + a.sym_kappa * a.dt * vv;
+                } else {
+                    vv = a.veloc[m6 + ri] +
+                         hdt * (a.accel[m6 + ri] + an[ri]);
+                    uu = u0 + (a.dt * vv + hdt2 * an[ri]);
+                    a.accel[m6 + ri] = an[ri];
+                }
                 a.veloc[m6 + ri] = vv;
-                a.accel[m6 + ri] = an[ri];
                 a.displ[m6 + ri] = uu;
                 bad |= !isfinite(uu);
                 bad0 |= !isfinite(u0);
@@ -824,6 +861,180 @@ void launch_mass3d(hipStream_t st, bool march, const Mass3DArgs<Real> &a,
     else
         hipLaunchKernelGGL((k_mass3d<Real, false>), dim3(a.n), dim3(128),
                            lds, st, a, stiff);
+}
+
+// C2R input of a pair of real series from FP64 Fourier columns (st
+// doubles between modes, re/im interleaved): buf[k] = Xfull[k] + i
+// Yfull[k] under SolverFFTW's C2R rules, as pack_pair; Y null: zero.
+template <typename Real>
+__device__ __forceinline__ void pack_fields(cpx<Real> *buf,
+                                            const double *__restrict__ X,
+                                            const double *__restrict__ Y,
+                                            int st, int nr, int nu1,
+                                            bool even, int lane) {
+    const int nyq = even ? nu1 - 1 : -1;
+    for (int k = lane; k < nr; k += 64) {
+        const bool lo = k < nu1;
+        const size_t kk = (size_t)(lo ? k : nr - k) * st;
+        const cpx<Real> x = {(Real)X[kk], (Real)X[kk + 1]};
+        const cpx<Real> y = Y != nullptr
+                                ? cpx<Real>{(Real)Y[kk], (Real)Y[kk + 1]}
+                                : cpx<Real>{(Real)0, (Real)0};
+        const bool real_only = k == 0 || k == nyq;
+        const Real sc = k == nyq ? (Real)2 : (Real)1;
+        const Real sim = real_only ? (Real)0 : (lo ? (Real)1 : (Real)-1);
+        buf[pad(k)] = {sc * x.re - sim * y.im, sim * x.im + sc * y.re};
+    }
+    wave_sync();
+}
+
+// R2C output of a pair (as unpack_pair), added to FP64 Fourier columns.
+template <typename Real>
+__device__ __forceinline__ void unpack_add(const cpx<Real> *buf,
+                                           double *__restrict__ X,
+                                           double *__restrict__ Y, int st,
+                                           int nr, int nu1, bool even,
+                                           int lane) {
+    const Real h0 = (Real)0.5 / (Real)nr;
+    for (int k = lane; k < nu1; k += 64) {
+        const cpx<Real> zk = buf[pad(k)];
+        const cpx<Real> zn = buf[pad(k == 0 ? 0 : nr - k)];
+        const Real h = (even && k == nu1 - 1) ? h0 * (Real)0.5 : h0;
+        const size_t m = (size_t)k * st;
+        X[m] += (double)((zk.re + zn.re) * h);
+        X[m + 1] += (double)((zk.im - zn.im) * h);
+        if (Y != nullptr) {
+            Y[m] += (double)((zk.im + zn.im) * h);
+            Y[m + 1] += (double)((zn.re - zk.re) * h);
+        }
+    }
+}
+
+// Terms of the absorbing boundaries of one step (AbcArgs), one block of
+// three waves per boundary point; NC = 3 (solid) or 1 (fluid).
+//  1D parts, order by order: the ClaytonSolid1D force (ClaytonSolid1D.cpp:
+//    16-29) or ClaytonFluid1D's v area / (rho vp) (ClaytonFluid1D.cpp:
+//    15-22), and the sponge's 2 gamma v + gamma^2 u (Sponge.hpp:80-83).
+//  3D Clayton (ClaytonSolid3D.cpp:43-67, ClaytonFluid3D.cpp:33-51): the
+//    velocity to the slices (wave 0: s + i phi, wave 1: z; fluid: wave 0),
+//    the operator slice by slice, back to orders.
+//  3D sponge (Sponge.hpp:84-92): wave c transforms u_c + i v_c, slice by
+//    slice w_c = 2 gamma v_c + gamma^2 u_c, back to orders (the point
+//    kernel subtracts the term from the acceleration and drops the
+//    imaginary part of its order 0, as the reference's C2R/R2C round trip
+//    of the acceleration does).
+template <typename Real, int NC>
+__global__ __launch_bounds__(192) void k_abc(AbcArgs<Real> a) {
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int q = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int i = blockIdx.x;
+    const int p = a.point[i];
+    const int nr = a.point_nr[p];
+    const int nu1 = nr / 2 + 1;
+    const bool even = (nr & 1) == 0;
+    const int st = 2 * NC;
+    const double *U = a.displ + (size_t)a.point_off[p] * st;
+    const double *V = a.veloc + (size_t)a.point_off[p] * st;
+    double *CT = a.cterm + (size_t)a.term_off[i] * st;
+    double *ST = a.sterm + (size_t)a.term_off[i] * st;
+    const double *c1 = a.cl1 + (size_t)i * (NC == 3 ? 4 : 1);
+    const double g1 = a.sp1[2 * i], g2 = a.sp1[2 * i + 1];
+    for (int al = t; al < nu1; al += 192) {
+        const size_t m = (size_t)al * st;
+#pragma unroll
+        for (int ri = 0; ri < 2; ri++) {
+            if (NC == 3) {
+                const double vs = V[m + ri], vp = V[m + 2 + ri],
+                             vz = V[m + 4 + ri];
+                CT[m + ri] = c1[0] * vs + c1[2] * vz;
+                CT[m + 2 + ri] = c1[3] * vp;
+                CT[m + 4 + ri] = c1[2] * vs + c1[1] * vz;
+            } else {
+                CT[m + ri] = c1[0] * V[m + ri];
+            }
+#pragma unroll
+            for (int c = 0; c < NC; c++)
+                ST[m + 2 * c + ri] =
+                    g1 * V[m + 2 * c + ri] + g2 * U[m + 2 * c + ri];
+        }
+    }
+    const int64_t c3 = a.cl3_off[i], s3 = a.sp3_off[i];
+    if (c3 < 0 && s3 < 0) return;
+    const uint32_t *plan = a.plan + (size_t)nr * CARD_PLAN_W;
+    FftLds<Real> L;
+    L.carve(fft_lds, nr, 3);
+    fft_load_tables<Real>(L, a.twiddle, a.tw_off, a.slice_pos, a.pos_off,
+                          plan, nr, t, 192);
+    __syncthreads();
+    cpx<Real> *b = L.buf(q);
+    constexpr int NW = NC == 3 ? 2 : 1;   // waves of the real outputs
+    if (c3 >= 0) {
+        if (q < NW) {
+            pack_fields<Real>(b, V + (q == 0 ? 0 : 4),
+                              (NC == 3 && q == 0) ? V + 2 : nullptr, st, nr,
+                              nu1, even, lane);
+            wave_fft<Real>(b, plan, true, L.tw, lane);
+        }
+        __syncthreads();
+        const Real *M = a.cl3 + (size_t)c3 * (NC == 3 ? 6 : 1);
+        for (int k = t; k < nr; k += 192) {
+            const int pk = pad(L.pos[k]);
+            if (NC == 3) {
+                const cpx<Real> z0 = L.buf(0)[pk], z1 = L.buf(1)[pk];
+                const Real vs = z0.re, vp = z0.im, vz = z1.re;
+                const Real *m = M + (size_t)k * 6;
+                L.buf(0)[pk] = {m[0] * vs + m[3] * vp + m[4] * vz,
+                                m[3] * vs + m[1] * vp + m[5] * vz};
+                L.buf(1)[pk] = {m[4] * vs + m[5] * vp + m[2] * vz, (Real)0};
+            } else {
+                L.buf(0)[pk] = {M[k] * L.buf(0)[pk].re, (Real)0};
+            }
+        }
+        __syncthreads();
+        if (q < NW) {
+            wave_fft<Real>(b, plan, false, L.tw, lane);
+            unpack_add<Real>(b, CT + (q == 0 ? 0 : 4),
+                             (NC == 3 && q == 0) ? CT + 2 : nullptr, st, nr,
+                             nu1, even, lane);
+        }
+        __syncthreads();
+    }
+    if (s3 >= 0) {
+        if (q < NC) {
+            pack_fields<Real>(b, U + 2 * q, V + 2 * q, st, nr, nu1, even,
+                              lane);
+            wave_fft<Real>(b, plan, true, L.tw, lane);
+        }
+        __syncthreads();
+        const Real *G = a.sp3 + (size_t)s3 * 2;
+        for (int k = t; k < nr; k += 192) {
+            const int pk = pad(L.pos[k]);
+            const Real ga = G[2 * k], gb = G[2 * k + 1];
+            Real w[3] = {(Real)0, (Real)0, (Real)0};
+#pragma unroll
+            for (int c = 0; c < NC; c++) {
+                const cpx<Real> z = L.buf(c)[pk];
+                w[c] = ga * z.im + gb * z.re;
+            }
+            L.buf(0)[pk] = {w[0], NC == 3 ? w[1] : (Real)0};
+            if (NC == 3) L.buf(1)[pk] = {w[2], (Real)0};
+        }
+        __syncthreads();
+        if (q < NW) {
+            wave_fft<Real>(b, plan, false, L.tw, lane);
+            unpack_add<Real>(b, ST + (q == 0 ? 0 : 4),
+                             (NC == 3 && q == 0) ? ST + 2 : nullptr, st, nr,
+                             nu1, even, lane);
+        }
+    }
+}
+
+template <typename Real, int NC>
+void launch_abc(hipStream_t st, const AbcArgs<Real> &a) {
+    if (a.n <= 0) return;
+    const size_t lds = fft_lds_bytes<Real>(std::max(a.nr_max, 1), 3);
+    hipLaunchKernelGGL((k_abc<Real, NC>), dim3(a.n), dim3(192), lds, st, a);
 }
 
 // C2R input of one wave's buffer (SolverFFTW conventions, as pack_pair):
@@ -1011,17 +1222,29 @@ __global__ __launch_bounds__(128) void k_fluid_defer(FluidDeferArgs<Real> a) {
     }
     const double hdt = .5 * a.dt, hdt2 = .5 * a.dt * a.dt;
     bool bad = false, bad0 = false;
+    const int ai = a.abc.idx != nullptr ? a.abc.idx[p] : -1;
     for (int k = t; k < nu1; k += 128) {
         const size_t m2 = (foff + k) * 2;
-        const double an[2] = {(double)sf[k].re, (double)sf[k].im};
+        double an[2] = {(double)sf[k].re, (double)sf[k].im};
+        // sponge (Sponge.hpp:74-96), after the mass in the fused form;
+        // the per-phase form has applied it in its own phase (the fluid
+        // point kernel's SPONGE pass covers the deferred points too)
+        if (MASS && ai >= 0)
+            sponge_sub<1>(a.abc, ai, k, even ? nu1 - 1 : -1, an);
 #pragma unroll
         for (int ri = 0; ri < 2; ri++) {
             const double u0 = a.chi[m2 + ri];
-            const double vv = a.chi_v[m2 + ri] + hdt * (a.chi_a[m2 + ri] +
-                                                         an[ri]);
-            const double uu = u0 + (a.dt * vv + hdt2 * an[ri]);
+            double vv, uu;
+            if (a.scheme == 1) {
+                // SymplecticTimeScheme::update
+                vv = a.chi_v[m2 + ri] + a.sym_pi * a.dt * an[ri];
+                uu = u0 + a.sym_kappa * a.dt * vv;
+            } else {
+                vv = a.chi_v[m2 + ri] + hdt * (a.chi_a[m2 + ri] + an[ri]);
+                uu = u0 + (a.dt * vv + hdt2 * an[ri]);
+                a.chi_a[m2 + ri] = an[ri];
+            }
             a.chi_v[m2 + ri] = vv;
-            a.chi_a[m2 + ri] = an[ri];
             a.chi[m2 + ri] = uu;
             bad |= !isfinite(uu);
             bad0 |= !isfinite(u0);
@@ -1183,6 +1406,10 @@ template void launch_mass3d<float>(hipStream_t, bool,
                                    const Mass3DArgs<float> &, float *);
 template void launch_mass3d<double>(hipStream_t, bool,
                                     const Mass3DArgs<double> &, double *);
+template void launch_abc<float, 3>(hipStream_t, const AbcArgs<float> &);
+template void launch_abc<float, 1>(hipStream_t, const AbcArgs<float> &);
+template void launch_abc<double, 3>(hipStream_t, const AbcArgs<double> &);
+template void launch_abc<double, 1>(hipStream_t, const AbcArgs<double> &);
 #undef AXS3D_CD_INST
 
 }  // namespace axs3d

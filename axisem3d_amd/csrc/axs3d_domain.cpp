@@ -438,7 +438,69 @@ void DomainBase::setSolidMass3D(int n, const int32_t *points,
             if (!(m > 0.))
                 throw Error("Domain::setSolidMass3D || Mass must be "
                             "positive.");
-            hMass3D.push_back(m);
+            hMass3D.push_back(1. / m);
+        }
+    }
+    mMass3DUserPts = hMass3DPts.size();
+    mMass3DUserVals = hMass3D.size();
+}
+
+void DomainBase::setOceanLoad(int n, const int32_t *points,
+                              const int32_t *nval, const double *values) {
+    if (hPointNr.empty())
+        throw Error("Domain::setOceanLoad || Set the solid points first.");
+    hOcean1DPts.clear();
+    hOcean3DPts.clear();
+    hOcean1D.clear();
+    hOcean3D.clear();
+    std::vector<uint8_t> seen(hPointNr.size(), 0);
+    for (int i = 0; i < std::max(n, 0); i++) {
+        const int p = points[i];
+        if (p < 0 || p >= (int)hPointNr.size() || seen[p])
+            throw Error("Domain::setOceanLoad || Point out of range or "
+                        "listed twice.");
+        seen[p] = 1;
+        const int nv = nval[i];
+        if (nv != 1 && nv != hPointNr[p])
+            throw Error("Domain::setOceanLoad || An entry has 1 value (1D) "
+                        "or one per slice of its point (3D).");
+        std::vector<double> &dst = nv == 1 ? hOcean1D : hOcean3D;
+        (nv == 1 ? hOcean1DPts : hOcean3DPts).push_back(p);
+        for (int k = 0; k < 4 * nv; k++) dst.push_back(*values++);
+    }
+}
+
+// The 3D ocean-load points join the 3D-mass list (after the points with a
+// plain 3D mass): their 1 / m per slice with the masses, their k per slice
+// in hMass3DK.  Idempotent: a second finalize rebuilds the same list.
+void DomainBase::mergeOceanLoad() {
+    hMass3DPts.resize(mMass3DUserPts);
+    hMass3DOff.resize(mMass3DUserPts);
+    hMass3D.resize(mMass3DUserVals);
+    hMass3DKOff.assign(mMass3DUserPts, -1);
+    hMass3DK.clear();
+    std::vector<uint8_t> m3(hPointNr.size(), 0);
+    for (int32_t p : hMass3DPts) m3[p] = 1;
+    for (int32_t p : hOcean1DPts)
+        if (m3[p])
+            throw Error("Domain::finalize || A point with an ocean load takes "
+                        "its mass from the ocean-load entry, not from "
+                        "set_solid_mass3d.");
+    const double *o = hOcean3D.data();
+    for (int32_t p : hOcean3DPts) {
+        if (m3[p])
+            throw Error("Domain::finalize || A point with an ocean load takes "
+                        "its mass from the ocean-load entry, not from "
+                        "set_solid_mass3d.");
+        hMass3DPts.push_back(p);
+        hMass3DOff.push_back((int64_t)hMass3D.size());
+        hMass3DKOff.push_back((int64_t)hMass3DK.size());
+        for (int k = 0; k < hPointNr[p]; k++, o += 4) {
+            if (!(o[0] > 0.))
+                throw Error("Domain::finalize || Ocean load: 1 / m must be "
+                            "positive.");
+            hMass3D.push_back(o[0]);
+            for (int c = 1; c < 4; c++) hMass3DK.push_back(o[c]);
         }
     }
 }
@@ -497,6 +559,194 @@ void DomainBase::setReceivers(int n, const int32_t *elem, const double *w,
     hRecW.assign(w, w + (size_t)std::max(n, 0) * NPEM);
     hRecPhi.assign(phi, phi + std::max(n, 0));
     hMaxSamples = std::max(max_samples, 0);
+    hRecRot.clear();
+}
+
+// Station::rotateField for the displacement (Station.hpp:79-118), which
+// the element hands over in (s, phi, z) (SolidElement::displInRTZ is
+// false): up to two rotations about axis k by an angle, as
+// Station::rotate(V, nrow, k, angle) (Station.cpp:41-55).
+void DomainBase::setReceiverFrames(int n, const int32_t *wcs,
+                                   const double *theta, const double *baz,
+                                   int cartesian) {
+    if (n != (int)hRecElem.size())
+        throw Error("Domain::setReceiverFrames || One frame per receiver "
+                    "(set the receivers first).");
+    constexpr double kHalfPi = 1.57079632679489661923;
+    hRecRot.assign((size_t)n * 6, 0.);
+    for (int i = 0; i < n; i++) {
+        double *r = hRecRot.data() + (size_t)i * 6;
+        r[0] = r[3] = -1.;
+        auto put = [&](int slot, int k, double ang) {
+            r[3 * slot] = k;
+            r[3 * slot + 1] = std::cos(ang);
+            r[3 * slot + 2] = std::sin(ang);
+        };
+        const int w = wcs[i];
+        if (w < 0 || w > 3)
+            throw Error("Domain::setReceiverFrames || Unknown wavefield "
+                        "coordinate system.");
+        const double bazp90 = baz[i] + kHalfPi;
+        if (cartesian) {
+            // RTZ = SPZ
+            if (w == 2) put(0, 2, bazp90);
+            if (w == 3) put(0, 2, -hRecPhi[i]);
+        } else {
+            if (w == 1) put(0, 1, theta[i]);
+            if (w == 2) {
+                put(0, 1, theta[i]);
+                put(1, 2, bazp90);
+            }
+            if (w == 3) put(0, 2, -hRecPhi[i]);
+        }
+    }
+}
+
+void DomainBase::setTimeScheme(int scheme) {
+    if (scheme != 0 && scheme != 1)
+        throw Error("Domain::setTimeScheme || Unknown time scheme.");
+    mScheme = scheme;
+}
+
+// ------------------------------------------------- absorbing boundaries
+int DomainBase::abcRecord(AbcHost &h, int npoints, int p, int w1) {
+    if (p < 0 || p >= npoints)
+        throw Error("Domain::addAbsorbingBoundary || Point out of range.");
+    if (h.rec.size() != (size_t)npoints) h.rec.assign(npoints, -1);
+    if (h.rec[p] < 0) {
+        h.rec[p] = (int32_t)h.point.size();
+        h.point.push_back(p);
+        h.cl1.insert(h.cl1.end(), (size_t)w1, 0.);
+        h.cl3.emplace_back();
+        h.sp1.insert(h.sp1.end(), 2, 0.);
+        h.sp3.emplace_back();
+    }
+    return h.rec[p];
+}
+
+// ClaytonSolid1D (ClaytonSolid1D.hpp:19-33): the four coefficients of its
+// constructor, summed over the point's entries.
+void DomainBase::addClaytonSolid(int n, const int32_t *points,
+                                 const double *rho_vp, const double *rho_vs,
+                                 const double *area, const double *theta) {
+    if (mNumPoints <= 0)
+        throw Error("Domain::addClaytonSolid || Set the points first.");
+    for (int i = 0; i < n; i++) {
+        const int r = abcRecord(mAbcS, mNumPoints, points[i], 4);
+        const double rsa = rho_vs[i] * area[i], rpa = rho_vp[i] * area[i];
+        const double c = std::cos(theta[i]), s = std::sin(theta[i]);
+        double *k = mAbcS.cl1.data() + (size_t)r * 4;
+        k[0] += rsa * c * c + rpa * s * s;
+        k[1] += rsa * s * s + rpa * c * c;
+        k[2] += (rpa - rsa) * c * s;
+        k[3] += rsa;
+    }
+}
+
+// ClaytonSolid3D (ClaytonSolid3D.hpp:19-27): per slice rsa = rho vs area
+// and k = sqrt((rho vp - rho vs) area) n; its force rsa V + (V.k) k is the
+// symmetric operator rsa I + k k^T, summed over the point's entries.
+void DomainBase::addClaytonSolid3D(int n, const int32_t *points,
+                                   const double *rho_vp,
+                                   const double *rho_vs, const double *area,
+                                   const double *unit_normal) {
+    if (mNumPoints <= 0)
+        throw Error("Domain::addClaytonSolid3D || Set the points first.");
+    size_t v = 0;
+    for (int i = 0; i < n; i++) {
+        const int r = abcRecord(mAbcS, mNumPoints, points[i], 4);
+        const int nr = hPointNr[points[i]];
+        std::vector<double> &m = mAbcS.cl3[r];
+        if (m.empty()) m.assign((size_t)nr * 6, 0.);
+        for (int k = 0; k < nr; k++, v++) {
+            const double rsa = rho_vs[v] * area[v];
+            const double a = std::sqrt((rho_vp[v] - rho_vs[v]) * area[v]);
+            const double ks = a * unit_normal[3 * v],
+                         kp = a * unit_normal[3 * v + 1],
+                         kz = a * unit_normal[3 * v + 2];
+            double *o = m.data() + (size_t)k * 6;
+            o[0] += rsa + ks * ks;
+            o[1] += rsa + kp * kp;
+            o[2] += rsa + kz * kz;
+            o[3] += ks * kp;
+            o[4] += ks * kz;
+            o[5] += kp * kz;
+        }
+    }
+}
+
+// ClaytonFluid1D/3D (ClaytonFluid1D.hpp:19-24, ClaytonFluid3D.hpp:19-25):
+// area / (rho vp), one value or one per slice of the point.
+void DomainBase::addClaytonFluid(int n, const int32_t *points, int per_slice,
+                                 const double *rho_vp, const double *area) {
+    if (mNumFPoints <= 0)
+        throw Error("Domain::addClaytonFluid || Set the fluid points first.");
+    size_t v = 0;
+    for (int i = 0; i < n; i++) {
+        const int r = abcRecord(mAbcF, mNumFPoints, points[i], 1);
+        if (!per_slice) {
+            mAbcF.cl1[r] += area[v] / rho_vp[v];
+            v++;
+            continue;
+        }
+        const int nr = hFPointNr[points[i]];
+        std::vector<double> &m = mAbcF.cl3[r];
+        if (m.empty()) m.assign(nr, 0.);
+        for (int k = 0; k < nr; k++, v++) m[k] += area[v] / rho_vp[v];
+    }
+}
+
+// Sponge<SolidPoint/FluidPoint> (Sponge.hpp:30-45): gamma, one value or
+// one per slice of the point; the force 2 gamma v + gamma^2 u kept as
+// (2 gamma, gamma^2), summed over the point's entries.
+void DomainBase::addSponge(int fluid, int n, const int32_t *points,
+                           int per_slice, const double *gamma) {
+    AbcHost &h = fluid ? mAbcF : mAbcS;
+    const int np = fluid ? mNumFPoints : mNumPoints;
+    if (np <= 0) throw Error("Domain::addSponge || Set the points first.");
+    size_t v = 0;
+    for (int i = 0; i < n; i++) {
+        const int r = abcRecord(h, np, points[i], fluid ? 1 : 4);
+        if (!per_slice) {
+            h.sp1[2 * r] += 2. * gamma[v];
+            h.sp1[2 * r + 1] += gamma[v] * gamma[v];
+            v++;
+            continue;
+        }
+        const int nr = fluid ? hFPointNr[points[i]] : hPointNr[points[i]];
+        std::vector<double> &m = h.sp3[r];
+        if (m.empty()) m.assign((size_t)nr * 2, 0.);
+        for (int k = 0; k < nr; k++, v++) {
+            m[2 * k] += 2. * gamma[v];
+            m[2 * k + 1] += gamma[v] * gamma[v];
+        }
+    }
+}
+
+// Plan words, twiddles (FP64, interleaved) and slice positions of every
+// Nr of mFftNrs with pass radices of at most maxRadix (axs3d_fft.hpp), in
+// the layout finalize's offsets describe.
+void DomainBase::buildFftTables(int maxRadix, std::vector<uint32_t> &plan,
+                                std::vector<double> &tw,
+                                std::vector<uint16_t> &pos) const {
+    plan.assign((size_t)(mFftNrMax + 1) * CARD_PLAN_W, 0u);
+    tw.clear();
+    pos.clear();
+    std::vector<double> tw1;
+    std::vector<uint16_t> pos1;
+    for (const int nr : mFftNrs) {
+        const std::vector<int> rad = fft_choose_radices(nr, maxRadix);
+        if (rad.empty() && nr > 1)
+            throw Error("Domain::finalize || Nr of a 3D element has "
+                        "a prime factor above 13 (use FFT lucky "
+                        "numbers).");
+        if ((int)rad.size() > FFT_MAX_PASSES)
+            throw Error("Domain::finalize || Too many FFT passes.");
+        uint32_t *pl = plan.data() + (size_t)nr * CARD_PLAN_W;
+        fft_build_plan(nr, rad, pl, tw1, pos1);
+        tw.insert(tw.end(), tw1.begin(), tw1.end());
+        pos.insert(pos.end(), pos1.begin(), pos1.end());
+    }
 }
 
 void DomainBase::finalize() {
@@ -507,12 +757,17 @@ void DomainBase::finalize() {
             throw Error("Domain::finalize || AXS3D_FFT_MAX_RADIX must be in "
                         "2..16.");
         mFftMaxRadix = r;
+        mFftRadixFixed = true;
     }
+    mergeOceanLoad();
     if (hGll.size() != NPEM || hGlj.size() != NPEM)
         throw Error("Domain::finalize || G matrices are not set.");
     if (mNumPoints <= 0 || mNumElems <= 0)
         throw Error("Domain::finalize || Points or elements are not set.");
     if (!(mDt > 0.)) throw Error("Domain::finalize || dt is not set.");
+    if (mScheme == 1 && !hInstDt.empty())
+        throw Error("Domain::finalize || The symplectic scheme takes one "
+                    "time step for the whole domain.");
     layout3D();
     const bool has1D = mNum3D < mNumElems;
     if (has1D && hMatOff.size() != (size_t)mNumElems)
@@ -615,6 +870,13 @@ void DomainBase::finalize() {
     for (int32_t p : hMass3DPts) fftNr.push_back(hPointNr[p]);
     for (int32_t p : hFMass3DPts) fftNr.push_back(hFPointNr[p]);
     for (int32_t k : hCpl3DPairs) fftNr.push_back(hPointNr[hCplSolid[k]]);
+    // absorbing-boundary points with a per-slice part
+    for (size_t r = 0; r < mAbcS.point.size(); r++)
+        if (!mAbcS.cl3[r].empty() || !mAbcS.sp3[r].empty())
+            fftNr.push_back(hPointNr[mAbcS.point[r]]);
+    for (size_t r = 0; r < mAbcF.point.size(); r++)
+        if (!mAbcF.cl3[r].empty() || !mAbcF.sp3[r].empty())
+            fftNr.push_back(hFPointNr[mAbcF.point[r]]);
     // fluid elements with a 3D medium, heaviest first
     hFList3D.clear();
     if (mNumFElems > 0 && !hFElemKind.empty()) {
@@ -641,25 +903,21 @@ void DomainBase::finalize() {
                         "point exceeds the FFT capacity (2048).");
         twOff.assign(nrmax + 1, -1);
         posOff.assign(nrmax + 1, -1);
-        plan.assign((size_t)(nrmax + 1) * CARD_PLAN_W, 0u);
-        std::vector<double> tw1;
-        std::vector<uint16_t> pos1;
+        mFftNrs.clear();
+        mFftNrMax = nrmax;
+        // the distinct Nr in order of first use; the tables of each take
+        // Nr - 1 twiddles and Nr positions whatever the pass radices, so
+        // these offsets hold for every plan buildFftTables makes
+        int64_t tcum = 0, pcum = 0;
         for (const int nr : fftNr) {
             if (twOff[nr] >= 0) continue;
-            const std::vector<int> rad = fft_choose_radices(nr, mFftMaxRadix);
-            if (rad.empty() && nr > 1)
-                throw Error("Domain::finalize || Nr of a 3D element has "
-                            "a prime factor above 13 (use FFT lucky "
-                            "numbers).");
-            if ((int)rad.size() > FFT_MAX_PASSES)
-                throw Error("Domain::finalize || Too many FFT passes.");
-            uint32_t *pl = plan.data() + (size_t)nr * CARD_PLAN_W;
-            fft_build_plan(nr, rad, pl, tw1, pos1);
-            twOff[nr] = (int64_t)twTab.size() / 2;
-            twTab.insert(twTab.end(), tw1.begin(), tw1.end());
-            posOff[nr] = (int64_t)posTab.size();
-            posTab.insert(posTab.end(), pos1.begin(), pos1.end());
+            mFftNrs.push_back(nr);
+            twOff[nr] = tcum;
+            posOff[nr] = pcum;
+            tcum += std::max(nr - 1, 0);
+            pcum += nr;
         }
+        buildFftTables(mFftMaxRadix, plan, twTab, posTab);
     }
     mMass3DNrMax = 0;
     for (int32_t p : hMass3DPts)
@@ -925,6 +1183,15 @@ void DomainBase::finalize() {
         up(dMass3DFlag, flag.data(), flag.size());
         up(dMass3DPts, hMass3DPts.data(), hMass3DPts.size() * 4);
         up(dMass3DOff, hMass3DOff.data(), hMass3DOff.size() * 8);
+        if (!hMass3DK.empty())
+            up(dMass3DKOff, hMass3DKOff.data(), hMass3DKOff.size() * 8);
+    }
+    if (!hOcean1DPts.empty()) {
+        std::vector<int32_t> idx(mNumPoints, -1);
+        for (size_t i = 0; i < hOcean1DPts.size(); i++)
+            idx[hOcean1DPts[i]] = (int32_t)i;
+        up(dOceanIdx, idx.data(), idx.size() * 4);
+        up(dOcean1D, hOcean1D.data(), hOcean1D.size() * 8);
     }
     hList3D.swap(list3d);
     hTwTab.swap(twTab);
@@ -942,10 +1209,51 @@ void DomainBase::finalize() {
         for (int p = 0; p < mNumPoints; p++) pdt[p] = hInstDt[hPointInst[p]];
         up(dPointDt, pdt.data(), pdt.size() * 8);
     }
+    // absorbing boundaries: records, their FP64 parts and term buffers (the
+    // per-slice parts follow in the working precision in uploadReal)
+    auto abcUp = [&](AbcHost &h, AbcDev &dv, int npts,
+                     const std::vector<int32_t> &nrOf) {
+        dv.n = (int)h.point.size();
+        if (dv.n == 0) return;
+        const int nc = &h == &mAbcS ? 3 : 1;
+        std::vector<int64_t> toff(dv.n), c3(dv.n, -1), s3(dv.n, -1);
+        int64_t t = 0, c = 0, s = 0;
+        dv.nr_max = 0;
+        for (int r = 0; r < dv.n; r++) {
+            const int nr = nrOf[h.point[r]];
+            toff[r] = t;
+            t += nr / 2 + 1;
+            if (!h.cl3[r].empty()) {
+                c3[r] = c;
+                c += nr;
+            }
+            if (!h.sp3[r].empty()) {
+                s3[r] = s;
+                s += nr;
+            }
+            if (c3[r] >= 0 || s3[r] >= 0) dv.nr_max = std::max(dv.nr_max, nr);
+        }
+        std::vector<int32_t> idx(npts, -1);
+        for (int r = 0; r < dv.n; r++) idx[h.point[r]] = r;
+        up(dv.idx, idx.data(), idx.size() * 4);
+        up(dv.point, h.point.data(), h.point.size() * 4);
+        up(dv.cl1, h.cl1.data(), h.cl1.size() * 8);
+        up(dv.cl3off, c3.data(), c3.size() * 8);
+        up(dv.sp1, h.sp1.data(), h.sp1.size() * 8);
+        up(dv.sp3off, s3.data(), s3.size() * 8);
+        up(dv.termoff, toff.data(), toff.size() * 8);
+        dv.cterm.alloc((size_t)std::max<int64_t>(t, 1) * 2 * nc * 8);
+        dv.sterm.alloc((size_t)std::max<int64_t>(t, 1) * 2 * nc * 8);
+        dv.cterm.zero(st);
+        dv.sterm.zero(st);
+    };
+    abcUp(mAbcS, mAbcSD, mNumPoints, hPointNr);
+    if (mNumFElems > 0) abcUp(mAbcF, mAbcFD, mNumFPoints, hFPointNr);
     if (mNumRec > 0) {
         up(dRecElem, hRecElem.data(), hRecElem.size() * 4);
         up(dRecW, hRecW.data(), hRecW.size() * 8);
         up(dRecPhi, hRecPhi.data(), hRecPhi.size() * 8);
+        if (!hRecRot.empty()) up(dRecRot, hRecRot.data(), hRecRot.size() * 8);
         dSeismo.alloc((size_t)std::max(hMaxSamples, 1) * mNumRec * 3 * 8);
         dSeismo.zero(st);
     }
@@ -1029,7 +1337,8 @@ void DomainBase::recordOutput(int isample, bool dev) {
                   dDispl.as<double>(),
                   dSeismo.as<double>() +
                       (dev ? 0 : (size_t)isample * mNumRec * 3),
-                  dev ? dStepCtr.as<int>() : nullptr, hMaxSamples);
+                  dev ? dStepCtr.as<int>() : nullptr, hMaxSamples,
+                  hRecRot.empty() ? nullptr : dRecRot.as<double>());
     tock(AXS3D_KERNEL_RECORD);
 }
 
@@ -1115,11 +1424,17 @@ void DomainBase::adjointCheck() const {
     requireFinalized("Domain::adjoint");
     bool iso = true;
     for (uint8_t t : hMType) iso = iso && t == 0;
-    if (mNumFElems > 0 || mNumMass3D > 0 || !iso || !hInstDt.empty())
+    bool rotated = false;
+    for (size_t i = 0; i < hRecRot.size(); i += 3)
+        rotated = rotated || hRecRot[i] >= 0.;
+    if (mNumFElems > 0 || mNumMass3D > 0 || !iso || !hInstDt.empty() ||
+        !mAbcS.point.empty() || rotated || mScheme != 0 ||
+        !hOcean1DPts.empty())
         throw Error("Domain::adjoint || The adjoint run supports solid "
                     "elements with an isotropic elastic medium (1D or 3D, "
-                    "with or without attenuation) and no fluid, 3D mass or "
-                    "batched instances.");
+                    "with or without attenuation) and no fluid, 3D mass, "
+                    "ocean load, batched instances, absorbing boundaries, "
+                    "rotated station frames or the symplectic scheme.");
 }
 
 void DomainBase::adjointOperands() {
@@ -1341,9 +1656,76 @@ public:
         g.point_axial = dPointAxial.as<uint8_t>();
         g.inv_mass = dInvMass.as<double>();
         g.mass3d = mNumMass3D > 0 ? dMass3DFlag.as<uint8_t>() : nullptr;
+        if (!hOcean1DPts.empty()) {
+            g.ocean_idx = dOceanIdx.as<int32_t>();
+            g.ocean1d = dOcean1D.as<double>();
+        }
         g.adj = dAdj.as<uint32_t>();
         if (!hInstDt.empty()) g.point_dt = dPointDt.as<double>();
+        g.abc = abcGeom(mAbcSD);
+        g.scheme = mScheme;
+        g.sym_pi = mSymPi;
+        g.sym_kappa = mSymKappa;
         return g;
+    }
+    // absorbing boundaries: what the point kernels read, the terms' kernel
+    AbcGeom abcGeom(const AbcDev &dv) const {
+        AbcGeom g;
+        if (dv.n == 0) return g;
+        g.idx = dv.idx.as<int32_t>();
+        g.term_off = dv.termoff.as<int64_t>();
+        g.sp3_off = dv.sp3off.as<int64_t>();
+        g.cterm = dv.cterm.as<double>();
+        g.sterm = dv.sterm.as<double>();
+        return g;
+    }
+    AbcArgs<Real> abcArgs(const AbcDev &dv) const {
+        const bool fl = &dv == &mAbcFD;
+        AbcArgs<Real> a;
+        a.n = dv.n;
+        a.nr_max = dv.nr_max;
+        a.point = dv.point.as<int32_t>();
+        a.point_off = fl ? dFPointOff.as<uint32_t>() : dPointOff.as<uint32_t>();
+        a.point_nr = fl ? dFPointNr.as<int32_t>() : dPointNr.as<int32_t>();
+        a.displ = fl ? dChi.as<double>() : dDispl.as<double>();
+        a.veloc = fl ? dChiV.as<double>() : dVeloc.as<double>();
+        a.cl1 = dv.cl1.as<double>();
+        a.cl3_off = dv.cl3off.as<int64_t>();
+        a.cl3 = dv.cl3.as<Real>();
+        a.sp1 = dv.sp1.as<double>();
+        a.sp3_off = dv.sp3off.as<int64_t>();
+        a.sp3 = dv.sp3.as<Real>();
+        a.term_off = dv.termoff.as<int64_t>();
+        a.cterm = dv.cterm.as<double>();
+        a.sterm = dv.sterm.as<double>();
+        a.twiddle = dTwiddle.as<cpx<Real>>();
+        a.tw_off = dTwOff.as<int64_t>();
+        a.plan = dPlan.as<uint32_t>();
+        a.slice_pos = dPosTab.as<uint16_t>();
+        a.pos_off = dPosOff.as<int64_t>();
+        return a;
+    }
+    // the Clayton and sponge terms of this step, from the displacement and
+    // velocity it starts from (before any point kernel of the step)
+    void abcKernel() {
+        if (mAbcSD.n > 0) launch_abc<Real, 3>(mStream, abcArgs(mAbcSD));
+        if (mAbcFD.n > 0) launch_abc<Real, 1>(mStream, abcArgs(mAbcFD));
+    }
+    // the per-slice parts in the working precision, in record order (the
+    // offsets abcUp gave them)
+    void abcReal(const AbcHost &h, AbcDev &dv) {
+        if (dv.n == 0) return;
+        std::vector<Real> c3, s3;
+        for (size_t r = 0; r < h.point.size(); r++) {
+            for (double x : h.cl3[r]) c3.push_back((Real)x);
+            for (double x : h.sp3[r]) s3.push_back((Real)x);
+        }
+        dv.cl3.alloc(std::max<size_t>(c3.size(), 1) * sizeof(Real));
+        if (!c3.empty())
+            dv.cl3.upload(c3.data(), c3.size() * sizeof(Real), mStream);
+        dv.sp3.alloc(std::max<size_t>(s3.size(), 1) * sizeof(Real));
+        if (!s3.empty())
+            dv.sp3.upload(s3.data(), s3.size() * sizeof(Real), mStream);
     }
     Mass3DArgs<Real> mass3dArgs() const {
         Mass3DArgs<Real> m;
@@ -1351,6 +1733,10 @@ public:
         m.nr_max = mMass3DNrMax;
         m.point = dMass3DPts.as<int32_t>();
         m.inv_off = dMass3DOff.as<int64_t>();
+        if (!hMass3DK.empty()) {
+            m.k_off = dMass3DKOff.as<int64_t>();
+            m.ocean_k = dMass3DK.as<Real>();
+        }
         m.inv_mass = dInvMass3D.as<Real>();
         m.point_off = dPointOff.as<uint32_t>();
         m.point_nr = dPointNr.as<int32_t>();
@@ -1364,6 +1750,10 @@ public:
         m.accel = dAccel.as<double>();
         m.dt = mDt;
         m.flag = dFlag.as<int>();
+        m.abc = abcGeom(mAbcSD);
+        m.scheme = mScheme;
+        m.sym_pi = mSymPi;
+        m.sym_kappa = mSymKappa;
         return m;
     }
     PointGeom fluidGeom() const {
@@ -1376,6 +1766,10 @@ public:
         g.inv_mass = dFInvMass.as<double>();
         g.mass3d = mNumFDefer > 0 ? dFDeferFlag.as<uint8_t>() : nullptr;
         g.adj = dFAdj.as<uint32_t>();
+        g.abc = abcGeom(mAbcFD);
+        g.scheme = mScheme;
+        g.sym_pi = mSymPi;
+        g.sym_kappa = mSymKappa;
         return g;
     }
     FluidDeferArgs<Real> fluidDeferArgs() const {
@@ -1409,6 +1803,10 @@ public:
         f.plan = dPlan.as<uint32_t>();
         f.slice_pos = dPosTab.as<uint16_t>();
         f.pos_off = dPosOff.as<int64_t>();
+        f.abc = abcGeom(mAbcFD);
+        f.scheme = mScheme;
+        f.sym_pi = mSymPi;
+        f.sym_kappa = mSymKappa;
         return f;
     }
     CouplingGeom couplingGeom() const {
@@ -1608,11 +2006,23 @@ public:
         tock(AXS3D_KERNEL_FLUID_POINT);
     }
     // SolidFluidBoundary::apply (Domain.cpp:313-318)
+    // AbsorbingBoundary::applyClayton, then SolidFluidBoundary::apply
+    // (Domain.cpp:313-318)
     void applyBC_BeforeAssemblingStiff() override {
         requireFinalized("Domain::applyBC_BeforeAssemblingStiff");
-        if (mNumFElems <= 0 || mNumPairs <= 0) return;
+        const bool abc = mAbcSD.n > 0 || mAbcFD.n > 0;
+        if (abc) abcKernel();
+        if (!abc && (mNumFElems <= 0 || mNumPairs <= 0)) return;
         fluidPoint(PHASE_COUPLE);
         point(PHASE_COUPLE);
+    }
+    // AbsorbingBoundary::applySponge (Domain.cpp:330-332), on the
+    // accelerations computeStiffToAccel left, with the terms of this
+    // step's applyBC_BeforeAssemblingStiff
+    void applyBC_AfterComputingAccel() override {
+        requireFinalized("Domain::applyBC_AfterComputingAccel");
+        point(PHASE_SPONGE);
+        fluidPoint(PHASE_SPONGE);
     }
     void applyBC_AfterAssemblingStiff() override {
         requireFinalized("Domain::applyBC_AfterAssemblingStiff");
@@ -1626,6 +2036,51 @@ public:
     }
     void newmark() override {
         requireFinalized("NewmarkTimeScheme::update");
+        if (mScheme != 0)
+            throw Error("NewmarkTimeScheme::update || The domain marches "
+                        "with the symplectic scheme.");
+        point(PHASE_MARCH);
+        fluidPoint(PHASE_MARCH);
+    }
+
+    // SymplecticTimeScheme (SymplecticTimeScheme.hpp:81-100): the
+    // position-extended Forest-Ruth-like coefficients of Omelyan, Mryglod
+    // and Folk (2002), kappa_0..4 for the positions and pi_1..4 for the
+    // velocities
+    static constexpr double kSymA = 0.1786178958448091;
+    static constexpr double kSymB = -0.2123418310626054;
+    static constexpr double kSymG = -0.06626458266981849;
+    static double symKappa(int i) {
+        const double k[5] = {kSymA, kSymG, 1. - 2. * (kSymG + kSymA), kSymG,
+                             kSymA};
+        return k[i];
+    }
+    static double symPi(int i) {
+        const double p[5] = {0., 0.5 - kSymB, kSymB, kSymB, 0.5 - kSymB};
+        return p[i];
+    }
+    void symplecticLaunch() override {
+        requireFinalized("SymplecticTimeScheme::launch");
+        if (mScheme != 1)
+            throw Error("SymplecticTimeScheme::launch || The domain marches "
+                        "with the Newmark scheme.");
+        const double kdt = symKappa(0) * mDt;
+        launch_sym_launch(mStream, dDispl.as<double>(), dVeloc.as<double>(),
+                          mNumModes * 6, kdt);
+        if (mNumFElems > 0)
+            launch_sym_launch(mStream, dChi.as<double>(), dChiV.as<double>(),
+                              mNumFModes * 2, kdt);
+    }
+    void symplecticUpdate(int isub) override {
+        requireFinalized("SymplecticTimeScheme::update");
+        if (mScheme != 1)
+            throw Error("SymplecticTimeScheme::update || The domain marches "
+                        "with the Newmark scheme.");
+        if (isub < 1 || isub > 4)
+            throw Error("SymplecticTimeScheme::update || Sub-step out of "
+                        "range (1..4).");
+        mSymPi = symPi(isub);
+        mSymKappa = symKappa(isub);
         point(PHASE_MARCH);
         fluidPoint(PHASE_MARCH);
     }
@@ -1635,12 +2090,31 @@ public:
     // receivers sample u(t), then one fluid and one solid point kernel
     // gather, couple, mask, divide by the mass and march.
     void step(int tstep, int isample, bool dev) override {
-        elementKernel();
-        sourceKernel(tstep, true, dev);
-        if (dev || isample >= 0) recordOutput(isample, dev);
         const int all = PHASE_GATHER | PHASE_AXIAL | PHASE_MASS | PHASE_MARCH;
-        fluidPoint(all | PHASE_COUPLE);
-        point(all | PHASE_COUPLE);
+        if (mScheme == 1) {
+            // SymplecticTimeScheme::solve (SymplecticTimeScheme.cpp:30-
+            // 103): record, source (into the first sub-step's forces),
+            // launch, then four sub-steps of stiffness, boundaries, mass
+            // and update
+            if (dev || isample >= 0) recordOutput(isample, dev);
+            symplecticLaunch();
+            for (int isub = 1; isub <= 4; isub++) {
+                elementKernel();
+                if (isub == 1) sourceKernel(tstep, true, dev);
+                abcKernel();
+                mSymPi = symPi(isub);
+                mSymKappa = symKappa(isub);
+                fluidPoint(all | PHASE_COUPLE);
+                point(all | PHASE_COUPLE);
+            }
+        } else {
+            elementKernel();
+            sourceKernel(tstep, true, dev);
+            if (dev || isample >= 0) recordOutput(isample, dev);
+            abcKernel();
+            fluidPoint(all | PHASE_COUPLE);
+            point(all | PHASE_COUPLE);
+        }
         if (dev)
             launch_step_advance(mStream, dStepCtr.as<int>(), dFlag.as<int>());
         else
@@ -2057,20 +2531,66 @@ private:
         if (env && env[0] == '0') return;
         mCardTuneMs = timeCardinalPass();
         mTuneTrials = 1;
-        if (mScratchFixed) return;
-        double best_mib = mScratchMiB;
+        if (!mScratchFixed) {
+            double best_mib = mScratchMiB;
+            float best = (float)mCardTuneMs;
+            for (const size_t mib : {(size_t)1024, (size_t)256}) {
+                buildCardBatches(mib << 20);
+                const float ms = timeCardinalPass();
+                mTuneTrials++;
+                if (ms < best) {
+                    best = ms;
+                    best_mib = (double)mib;
+                }
+            }
+            if (best_mib != mScratchMiB)
+                buildCardBatches((size_t)best_mib * ((size_t)1 << 20));
+            mCardTuneMs = best;
+        }
+        tuneFftRadix();
+    }
+
+    // The FFT plans' largest pass radix: one timed cardinal pass with the
+    // plans of each of 16, 8 and 4 (the radices change the transforms'
+    // instruction mix and register demand, not their results beyond
+    // rounding), keeping the fastest, unless AXS3D_FFT_MAX_RADIX fixes it.
+    void uploadFftTables(int maxRadix) {
+        std::vector<uint32_t> plan;
+        std::vector<double> tw;
+        std::vector<uint16_t> pos;
+        buildFftTables(maxRadix, plan, tw, pos);
+        dPlan.upload(plan.data(), plan.size() * 4, mStream);
+        dPosTab.upload(pos.data(), pos.size() * 2, mStream);
+        std::vector<Real> twr(tw.begin(), tw.end());
+        if (!twr.empty())
+            dTwiddle.upload(twr.data(), twr.size() * sizeof(Real), mStream);
+        hip_check(hipStreamSynchronize(mStream), "Domain::autotune");
+    }
+    void tuneFftRadix() {
+        mRadixMs.clear();
+        // a pass under 5 ms is too short to rank plans reliably, and small
+        // domains keep the default plans so that their results do not
+        // depend on timing noise
+        if (mFftRadixFixed || mFftNrs.empty() || mCardTuneMs < 5.0) return;
+        int best_r = mFftMaxRadix;
         float best = (float)mCardTuneMs;
-        for (const size_t mib : {(size_t)1024, (size_t)256}) {
-            buildCardBatches(mib << 20);
+        mRadixMs.push_back(mFftMaxRadix);
+        mRadixMs.push_back(best);
+        for (const int r : {8, 4}) {
+            if (r == mFftMaxRadix) continue;
+            uploadFftTables(r);
             const float ms = timeCardinalPass();
-            mTuneTrials++;
+            mRadixMs.push_back(r);
+            mRadixMs.push_back(ms);
             if (ms < best) {
                 best = ms;
-                best_mib = (double)mib;
+                best_r = r;
             }
         }
-        if (best_mib != mScratchMiB)
-            buildCardBatches((size_t)best_mib * ((size_t)1 << 20));
+        if (best_r != mFftMaxRadix || mRadixMs.size() > 2) {
+            mFftMaxRadix = best_r;
+            uploadFftTables(best_r);
+        }
         mCardTuneMs = best;
     }
 
@@ -2245,12 +2765,16 @@ private:
             dTwiddle.upload(tw.data(), tw.size() * sizeof(Real), st);
         }
         if (mNumMass3D > 0) {
-            // Mass3D keeps 1 / m in the working precision (Mass3D.hpp:19-22)
-            std::vector<Real> inv(hMass3D.size());
-            for (size_t k = 0; k < inv.size(); k++)
-                inv[k] = (Real)(1.0 / hMass3D[k]);
+            // Mass3D keeps 1 / m in the working precision (Mass3D.hpp:19-22),
+            // MassOceanLoad3D also k (MassOceanLoad3D.cpp:13-20)
+            std::vector<Real> inv(hMass3D.begin(), hMass3D.end());
             dInvMass3D.alloc(inv.size() * sizeof(Real));
             dInvMass3D.upload(inv.data(), inv.size() * sizeof(Real), st);
+            if (!hMass3DK.empty()) {
+                std::vector<Real> kk(hMass3DK.begin(), hMass3DK.end());
+                dMass3DK.alloc(kk.size() * sizeof(Real));
+                dMass3DK.upload(kk.data(), kk.size() * sizeof(Real), st);
+            }
         }
         if (!hFMass3D.empty()) {
             std::vector<Real> inv(hFMass3D.size());
@@ -2334,6 +2858,8 @@ private:
             }
         }
         planStaticBatches();
+        abcReal(mAbcS, mAbcSD);
+        abcReal(mAbcF, mAbcFD);
         if (hSrcElem >= 0) {
             std::vector<Real> pat(hSrcPattern.begin(), hSrcPattern.end());
             dSrcPattern.alloc(pat.size() * sizeof(Real));

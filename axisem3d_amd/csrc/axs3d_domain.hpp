@@ -96,6 +96,8 @@ public:
     void setAttenuation1D(const double *dlam_dmu);
     void setAxialPoints(int n, const int32_t *points);
     void setSolidMass3D(int n, const int32_t *points, const double *mass);
+    void setOceanLoad(int n, const int32_t *points, const int32_t *nval,
+                      const double *values);
     void setMomentSource(int elem, int nu1, const double *pattern, int nsteps,
                          const double *stf);
     void addMomentSource(int elem, int nu1, const double *pattern, int nsteps,
@@ -103,6 +105,26 @@ public:
     void setInstances(int n, const int32_t *point_inst, const double *dt);
     void setReceivers(int n, const int32_t *elem, const double *w,
                       const double *phi, int max_samples);
+    // station frames: per receiver the wavefield coordinate system (0 spz,
+    // 1 RTZ, 2 ENZ, 3 xyz), theta and back azimuth (Station.hpp:24-29)
+    void setReceiverFrames(int n, const int32_t *wcs, const double *theta,
+                           const double *baz, int cartesian);
+    // absorbing boundaries (AbsorbingBoundary): entries as the reference
+    // constructs them, merged per point at finalize
+    void addClaytonSolid(int n, const int32_t *points, const double *rho_vp,
+                         const double *rho_vs, const double *area,
+                         const double *theta);
+    void addClaytonSolid3D(int n, const int32_t *points,
+                           const double *rho_vp, const double *rho_vs,
+                           const double *area, const double *unit_normal);
+    void addClaytonFluid(int n, const int32_t *points, int per_slice,
+                         const double *rho_vp, const double *area);
+    void addSponge(int fluid, int n, const int32_t *points, int per_slice,
+                   const double *gamma);
+    // time scheme: 0 Newmark (NewmarkTimeScheme), 1 symplectic
+    // (SymplecticTimeScheme, four sub-steps per step)
+    void setTimeScheme(int scheme);
+    int timeScheme() const { return mScheme; }
     void setDt(double dt) {
         mDt = dt;
         dropGraph();  // dt is a kernel argument of the captured step
@@ -122,8 +144,14 @@ public:
     // isample < 0 with dev: the sample follows the device step counter
     void recordOutput(int isample, bool dev = false);
     virtual void applyBC_AfterAssemblingStiff() = 0;
+    // Domain::applyBC_AfterComputingAccel (Domain.cpp:330-332): sponge
+    virtual void applyBC_AfterComputingAccel() = 0;
     virtual void computeStiffToAccel() = 0;
     virtual void newmark() = 0;
+    // SymplecticTimeScheme::launch (u += kappa_0 dt v) and ::update of
+    // sub-step isub (1..4) on all points (SymplecticTimeScheme.hpp:37-70)
+    virtual void symplecticLaunch() = 0;
+    virtual void symplecticUpdate(int isub) = 0;
     // one bulk Newmark step (sources, stiffness, receivers, points); with
     // dev, the step and sample come from the device step counter, which
     // the step advances (the form captured into a HIP graph)
@@ -169,6 +197,18 @@ public:
         ms[2] = mTuneTrials;
     }
     int precision() const { return mPrecision; }
+    // every autotuner choice and timing: [0] cardinal pass ms, [1] scratch
+    // MiB, [2] scratch sizes timed, [3] FFT largest radix, [4] radices
+    // timed, then (radix, ms) per radix timed; returns the count written
+    int autotuneReport(double *out, int n) const {
+        std::vector<double> v = {mCardTuneMs, mScratchMiB,
+                                 (double)mTuneTrials, (double)mFftMaxRadix,
+                                 (double)(mRadixMs.size() / 2)};
+        v.insert(v.end(), mRadixMs.begin(), mRadixMs.end());
+        const int k = std::min(n, (int)v.size());
+        for (int i = 0; i < k; i++) out[i] = v[i];
+        return k;
+    }
 
 protected:
     explicit DomainBase(int device, int precision);
@@ -188,6 +228,9 @@ protected:
     void dropGraph();
     bool mFinalized = false;
     double mDt = 0.;
+    int mScheme = 0;
+    // the march of the point kernels' next launch (PointGeom::scheme)
+    double mSymPi = 0., mSymKappa = 0.;
     int mRecordStep0 = 0;
 
     // host staging until finalize
@@ -256,9 +299,25 @@ protected:
     int mFDeferNrMax = 0;               // and of the deferred fluid points
     // largest FFT pass radix (2..16); AXS3D_FFT_MAX_RADIX to change it
     int mFftMaxRadix = 16;
+    bool mFftRadixFixed = false;        // set by AXS3D_FFT_MAX_RADIX
+    std::vector<int> mFftNrs;           // distinct Nr with FFT tables
+    int mFftNrMax = 0;
+    std::vector<double> mRadixMs;       // cardinal pass per radix timed
+    void buildFftTables(int maxRadix, std::vector<uint32_t> &plan,
+                        std::vector<double> &tw,
+                        std::vector<uint16_t> &pos) const;
     std::vector<int64_t> hMass3DOff;
-    std::vector<double> hMass3D;
+    std::vector<double> hMass3D;        // 1 / m per slice
     int mNumMass3D = 0;
+    // ocean load (MassOceanLoad1D/3D): the 1D entries' four factors per
+    // point for the point kernel; the 3D entries (1 / m and k per slice)
+    // join the 3D-mass list at finalize, with the offsets of their k
+    std::vector<int32_t> hOcean1DPts, hOcean3DPts;
+    std::vector<double> hOcean1D, hOcean3D;
+    size_t mMass3DUserPts = 0, mMass3DUserVals = 0;
+    std::vector<int64_t> hMass3DKOff;
+    std::vector<double> hMass3DK;
+    void mergeOceanLoad();
     // fluid points with a 3D mass; 3D solid-fluid pairs (normals [Nr][3]
     // per pair, unassembled and assembled); the deferred fluid points
     std::vector<int32_t> hFMass3DPts;
@@ -302,6 +361,7 @@ protected:
     // device: cardinal path
     DeviceBuffer dElemNr, dSliceOff, dPropOff, dList3D, dScrOff, dEM3D,
         dPosOff, dPosTab, dMass3DPts, dMass3DOff, dMass3DFlag, dInvMass3D,
+        dOceanIdx, dOcean1D, dMass3DKOff, dMass3DK,
         dEM3DAx,
         dFList3D, dFElemNr, dFK3D, dFK3DOff, dFScrOff, dFEM3D,
         dProp3D,
@@ -327,6 +387,29 @@ protected:
         dCpl3DCOff, dCplSolidPair3D, dCSolid3D;
     // device: source and receivers
     DeviceBuffer dSrcPattern, dStf, dRecElem, dRecW, dRecPhi, dSeismo;
+    // station frames: [receiver][6] two rotations (axis, cos, sin)
+    std::vector<double> hRecRot;
+    DeviceBuffer dRecRot;
+    // absorbing boundaries of the solid (3 components) and fluid (1)
+    // points: host records merged per point, their device form
+    struct AbcHost {
+        std::vector<int32_t> rec;              // [point] record or -1
+        std::vector<int32_t> point;            // [record] point
+        std::vector<double> cl1;               // [record][4 or 1]
+        std::vector<std::vector<double>> cl3;  // [record] per slice 6 or 1
+        std::vector<double> sp1;               // [record][2]
+        std::vector<std::vector<double>> sp3;  // [record] per slice 2
+    };
+    struct AbcDev {
+        int n = 0, nr_max = 0;
+        DeviceBuffer idx, point, cl1, cl3off, cl3, sp1, sp3off, sp3,
+            termoff, cterm, sterm;
+    };
+    AbcHost mAbcS, mAbcF;
+    AbcDev mAbcSD, mAbcFD;
+    // record of point p (created on first use) in h, for points of Nr
+    // nrOf(p); w1 = 1D Clayton width
+    int abcRecord(AbcHost &h, int npoints, int p, int w1);
     DeviceBuffer dPointDt;
     DeviceBuffer dFlag;
     DeviceBuffer dStepCtr;              // [0] step, [1] first recorded step

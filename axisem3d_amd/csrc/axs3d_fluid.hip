@@ -161,21 +161,28 @@ struct FluidPointArgs {
     CouplingGeom cpl;
     const double *solid_displ;
     const uint8_t *defer;          // [point] 1: finished by k_fluid_defer
+    AbcGeom abc;                   // Clayton and sponge terms (k_abc)
+    int scheme;                    // 0 Newmark, 1 symplectic sub-step
+    double sym_pi, sym_kappa;
 };
 
 // Fluid point update, one thread per (point, order), phases as for the
 // solid points; COUPLE adds the normal solid displacement of the coupled
 // solid point (coupleSolidToFluid) and leaves the result in cstiff for
-// the solid points' coupleFluidToSolid; AXIAL applies the axis (orders
-// >= 1 zero) and the free surface (all zero).
+// the solid points' coupleFluidToSolid, after the Clayton boundaries
+// (ClaytonFluid1D/3D: s -= the k_abc term); AXIAL applies the axis (orders
+// >= 1 zero) and the free surface (all zero); SPONGE alone subtracts the
+// sponge term from the acceleration held in stiff (the fused MASS | MARCH
+// form does so between the mass and the march).
 template <typename Real, bool GATHER, bool AXIAL, bool MASS, bool MARCH,
-          bool COUPLE>
+          bool COUPLE, bool SPONGE>
 __global__ __launch_bounds__(256) void k_fluid_point(
     FluidPointArgs a, const Real *__restrict__ eb, Real *__restrict__ stiff) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= a.n_modes) return;
     const int p = a.mode_point[k];
     const int alpha = (int)(k - (int64_t)a.point_off[p]);
+    const int ai = a.abc.idx != nullptr ? a.abc.idx[p] : -1;
     double s[2];
     if (GATHER) {
         Real f[2];
@@ -186,7 +193,22 @@ __global__ __launch_bounds__(256) void k_fluid_point(
         s[0] = (double)stiff[(size_t)k * 2];
         s[1] = (double)stiff[(size_t)k * 2 + 1];
     }
-    if (COUPLE) {
+    if (SPONGE) {
+        if (ai >= 0) {
+            const int nr = a.point_nr[p];
+            sponge_sub<1>(a.abc, ai, alpha, (nr & 1) ? -1 : nr / 2, s);
+            stiff[(size_t)k * 2] = (Real)s[0];
+            stiff[(size_t)k * 2 + 1] = (Real)s[1];
+        }
+        return;
+    }
+    if (COUPLE && ai >= 0) {
+        const double *c =
+            a.abc.cterm + ((size_t)a.abc.term_off[ai] + alpha) * 2;
+        s[0] -= c[0];
+        s[1] -= c[1];
+    }
+    if (COUPLE && a.cpl.n_pairs > 0) {
         const int pi = a.cpl.fluid_pair[p];
         if (pi >= 0) {
             const double *u = a.solid_displ +
@@ -217,16 +239,32 @@ __global__ __launch_bounds__(256) void k_fluid_point(
         const double im = a.inv_mass[p];
         s[0] *= im;
         s[1] *= im;
+        if (MARCH && ai >= 0)
+            sponge_sub<1>(a.abc, ai, alpha, (nr & 1) ? -1 : nr / 2, s);
     }
     if (!MARCH) {
         stiff[(size_t)k * 2] = (Real)s[0];
         stiff[(size_t)k * 2 + 1] = (Real)s[1];
         return;
     }
-    // all three loaded (16-byte vectors) before anything is stored
     double2 *up = reinterpret_cast<double2 *>(a.chi + (size_t)k * 2);
     double2 *vp = reinterpret_cast<double2 *>(a.chi_v + (size_t)k * 2);
     double2 *ap = reinterpret_cast<double2 *>(a.chi_a + (size_t)k * 2);
+    if (a.scheme == 1) {
+        // SymplecticTimeScheme::update: v += pi dt a, u += kappa dt v
+        const double2 u0 = *up, v0 = *vp;
+        const double pdt = a.sym_pi * a.dt, kdt = a.sym_kappa * a.dt;
+        const double vr = v0.x + pdt * s[0], vi = v0.y + pdt * s[1];
+        const double ur = u0.x + kdt * vr, ui = u0.y + kdt * vi;
+        *vp = make_double2(vr, vi);
+        *up = make_double2(ur, ui);
+        const bool bad = !isfinite(ur) || !isfinite(ui);
+        const bool bad0 = !isfinite(u0.x) || !isfinite(u0.y);
+        if (!GATHER) stiff[(size_t)k * 2] = stiff[(size_t)k * 2 + 1] = 0;
+        if (bad0 || bad) atomicOr(a.flag, (bad0 ? 1 : 0) | (bad ? 2 : 0));
+        return;
+    }
+    // all three loaded (16-byte vectors) before anything is stored
     const double2 u0 = *up, v0 = *vp, a0 = *ap;
     const double vr = v0.x + a.half_dt * (a0.x + s[0]);
     const double vi = v0.y + a.half_dt * (a0.y + s[1]);
@@ -285,19 +323,27 @@ void launch_fluid_point(hipStream_t st, int phases, const PointGeom &g,
     a.cpl = cpl;
     a.solid_displ = solid_displ;
     a.defer = g.mass3d;
-    if (cpl.n_pairs <= 0) phases &= ~PHASE_COUPLE;
-#define AXS3D_FPOINT_CASE(G, A, M, T, C)                                      \
-    case (A) | ((M) << 1) | ((T) << 2) | ((G) << 3) | ((C) << 4):            \
-        hipLaunchKernelGGL((k_fluid_point<Real, G, A, M, T, C>), dim3(grid),   \
-                           dim3(256), 0, st, a, eb, stiff);                   \
+    a.abc = g.abc;
+    a.scheme = g.scheme;
+    a.sym_pi = g.sym_pi;
+    a.sym_kappa = g.sym_kappa;
+    if (cpl.n_pairs <= 0 && g.abc.idx == nullptr) phases &= ~PHASE_COUPLE;
+    if (g.abc.idx == nullptr) phases &= ~PHASE_SPONGE;
+    if (phases == 0) return;
+#define AXS3D_FPOINT_CASE(G, A, M, T, C, S)                                   \
+    case (A) | ((M) << 1) | ((T) << 2) | ((G) << 3) | ((C) << 4) |           \
+        ((S) << 5):                                                          \
+        hipLaunchKernelGGL((k_fluid_point<Real, G, A, M, T, C, S>),            \
+                           dim3(grid), dim3(256), 0, st, a, eb, stiff);       \
         break;
     switch (phases) {
-        AXS3D_FPOINT_CASE(0, 1, 0, 0, 0)
-        AXS3D_FPOINT_CASE(0, 0, 1, 0, 0)
-        AXS3D_FPOINT_CASE(0, 0, 0, 1, 0)
-        AXS3D_FPOINT_CASE(0, 0, 0, 0, 1)
-        AXS3D_FPOINT_CASE(1, 1, 1, 1, 0)
-        AXS3D_FPOINT_CASE(1, 1, 1, 1, 1)
+        AXS3D_FPOINT_CASE(0, 1, 0, 0, 0, 0)
+        AXS3D_FPOINT_CASE(0, 0, 1, 0, 0, 0)
+        AXS3D_FPOINT_CASE(0, 0, 0, 1, 0, 0)
+        AXS3D_FPOINT_CASE(0, 0, 0, 0, 1, 0)
+        AXS3D_FPOINT_CASE(0, 0, 0, 0, 0, 1)
+        AXS3D_FPOINT_CASE(1, 1, 1, 1, 0, 0)
+        AXS3D_FPOINT_CASE(1, 1, 1, 1, 1, 0)
         default:
             break;
     }

@@ -307,6 +307,21 @@ __global__ __launch_bounds__(256) void k_assemble(
     for (int q = 0; q < 6; q++) sp[q] += f[q];
 }
 
+// MassOceanLoad1D::computeAccel (MassOceanLoad1D.hpp:48-62) on one mode's
+// (s, phi, z) re/im, FP64: (s, z) by the symmetric 2x2 factors o[0], o[1]
+// (diagonal) and o[2], phi by o[3] = 1/m.
+template <typename T>
+__device__ __forceinline__ void ocean_load_1d(const double *o, const T *s,
+                                              double *out) {
+#pragma unroll
+    for (int ri = 0; ri < 2; ri++) {
+        const double fs = (double)s[ri], fz = (double)s[4 + ri];
+        out[ri] = o[0] * fs + o[2] * fz;
+        out[4 + ri] = o[2] * fs + o[1] * fz;
+        out[2 + ri] = (double)s[2 + ri] * o[3];
+    }
+}
+
 // Point update, one thread per (point, mode).  Flags select the phases:
 //   GATHER assemble the element-local forces (bulk step; no point
 //          stiffness buffer is read or written)
@@ -323,19 +338,29 @@ struct PointArgs {
     const uint8_t *point_axial;
     const double *inv_mass;
     const uint8_t *mass3d;        // [point] 1: 3D mass (or null: none)
+    const int32_t *ocean_idx;     // [point] MassOceanLoad1D entry, or null
+    const double *ocean1d;        // [entry][4]
     const uint32_t *adj;
     double *displ, *veloc, *accel;
     double dt, half_dt, half_dt_dt;
     const double *point_dt;       // per-point time steps, or null
     int *flag;
     CouplingGeom cpl;
+    AbcGeom abc;
+    int scheme;                   // 0 Newmark, 1 symplectic sub-step
+    double sym_pi, sym_kappa;
 };
 
-//   COUPLE SolidFluidCoupling1D::coupleFluidToSolid (SolidFluidCoupling1D
+//   COUPLE the Clayton-Enquist boundaries (AbsorbingBoundary::
+//          applyClayton: s -= the k_abc force term), then
+//          SolidFluidCoupling1D::coupleFluidToSolid (SolidFluidCoupling1D
 //          .hpp:33-37): s -= (ns, nz) / m_fluid * fluid stiffness, after
 //          the fluid points have added the solid displacement
+//   SPONGE the sponge on the acceleration held in stiff
+//          (AbsorbingBoundary::applySponge), alone; the fused MASS | MARCH
+//          form applies it between the mass and the march
 template <typename Real, bool GATHER, bool AXIAL, bool MASS, bool MARCH,
-          bool COUPLE>
+          bool COUPLE, bool SPONGE>
 __global__ __launch_bounds__(256) void k_point(PointArgs a,
                                                const Real *__restrict__ eb,
                                                Real *__restrict__ stiff) {
@@ -343,6 +368,7 @@ __global__ __launch_bounds__(256) void k_point(PointArgs a,
     if (k >= a.n_modes) return;
     const int p = a.mode_point[k];
     const int alpha = (int)(k - (int64_t)a.point_off[p]);
+    const int ai = a.abc.idx != nullptr ? a.abc.idx[p] : -1;
     Real s[6];
     if (GATHER) {
         gather_forces(a.adj, eb, p, alpha, s);
@@ -350,7 +376,22 @@ __global__ __launch_bounds__(256) void k_point(PointArgs a,
 #pragma unroll
         for (int q = 0; q < 6; q++) s[q] = stiff[(size_t)k * 6 + q];
     }
-    if (COUPLE) {
+    if (SPONGE) {
+        if (ai >= 0) {
+            const int nr = a.point_nr[p];
+            sponge_sub<3>(a.abc, ai, alpha, (nr & 1) ? -1 : nr / 2, s);
+#pragma unroll
+            for (int q = 0; q < 6; q++) stiff[(size_t)k * 6 + q] = s[q];
+        }
+        return;
+    }
+    if (COUPLE && ai >= 0) {
+        const double *c =
+            a.abc.cterm + ((size_t)a.abc.term_off[ai] + alpha) * 6;
+#pragma unroll
+        for (int q = 0; q < 6; q++) s[q] = (Real)((double)s[q] - c[q]);
+    }
+    if (COUPLE && a.cpl.n_pairs > 0) {
         const int pi = a.cpl.solid_pair[p];
         if (pi >= 0) {
             const double f_re = a.cpl.cstiff[2 * ((size_t)a.cpl.foff[pi] +
@@ -404,10 +445,18 @@ __global__ __launch_bounds__(256) void k_point(PointArgs a,
             return;
         }
     }
+    const int oi = MASS && a.ocean_idx != nullptr ? a.ocean_idx[p] : -1;
     if (MASS && !MARCH) {
-        const Real im = (Real)a.inv_mass[p];
+        if (oi >= 0) {
+            double o[6];
+            ocean_load_1d(a.ocean1d + 4 * (size_t)oi, s, o);
 #pragma unroll
-        for (int q = 0; q < 6; q++) s[q] *= im;
+            for (int q = 0; q < 6; q++) s[q] = (Real)o[q];
+        } else {
+            const Real im = (Real)a.inv_mass[p];
+#pragma unroll
+            for (int q = 0; q < 6; q++) s[q] *= im;
+        }
     }
     if (!MARCH) {
 #pragma unroll
@@ -416,9 +465,17 @@ __global__ __launch_bounds__(256) void k_point(PointArgs a,
     }
     double a_new[6];
     if (MASS) {
-        const double im = a.inv_mass[p];
+        if (oi >= 0) {
+            ocean_load_1d(a.ocean1d + 4 * (size_t)oi, s, a_new);
+        } else {
+            const double im = a.inv_mass[p];
 #pragma unroll
-        for (int q = 0; q < 6; q++) a_new[q] = (double)s[q] * im;
+            for (int q = 0; q < 6; q++) a_new[q] = (double)s[q] * im;
+        }
+        if (ai >= 0) {
+            const int nr = a.point_nr[p];
+            sponge_sub<3>(a.abc, ai, alpha, (nr & 1) ? -1 : nr / 2, a_new);
+        }
     } else {
 #pragma unroll
         for (int q = 0; q < 6; q++) a_new[q] = (double)s[q];
@@ -429,18 +486,45 @@ __global__ __launch_bounds__(256) void k_point(PointArgs a,
     double2 *vp = reinterpret_cast<double2 *>(a.veloc + (size_t)k * 6);
     double2 *ap = reinterpret_cast<double2 *>(a.accel + (size_t)k * 6);
     double2 u2[3], v2[3], a2[3];
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-        u2[c] = up[c];
-        v2[c] = vp[c];
-        a2[c] = ap[c];
-    }
     bool bad = false, bad0 = false;
     double dt = a.dt, half_dt = a.half_dt, half_dt_dt = a.half_dt_dt;
     if (a.point_dt != nullptr) {
         dt = a.point_dt[p];
         half_dt = .5 * dt;
         half_dt_dt = .5 * dt * dt;
+    }
+    if (a.scheme == 1) {
+        // SymplecticTimeScheme::update (SymplecticTimeScheme.hpp:37-46):
+        // v += pi dt a, u += kappa dt v; no acceleration field
+        const double pdt = a.sym_pi * dt, kdt = a.sym_kappa * dt;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            u2[c] = up[c];
+            v2[c] = vp[c];
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            bad0 |= !isfinite(u2[c].x) || !isfinite(u2[c].y);
+            const double vr = v2[c].x + pdt * a_new[2 * c];
+            const double vi = v2[c].y + pdt * a_new[2 * c + 1];
+            const double ur = u2[c].x + kdt * vr;
+            const double ui = u2[c].y + kdt * vi;
+            vp[c] = make_double2(vr, vi);
+            up[c] = make_double2(ur, ui);
+            bad |= !isfinite(ur) || !isfinite(ui);
+        }
+        if (!GATHER) {
+#pragma unroll
+            for (int q = 0; q < 6; q++) stiff[(size_t)k * 6 + q] = 0;
+        }
+        if (bad0 || bad) atomicOr(a.flag, (bad0 ? 1 : 0) | (bad ? 2 : 0));
+        return;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        u2[c] = up[c];
+        v2[c] = vp[c];
+        a2[c] = ap[c];
     }
 #pragma unroll
     for (int c = 0; c < 3; c++) {
@@ -472,6 +556,22 @@ __global__ __launch_bounds__(256) void k_point(PointArgs a,
 }
 
 
+// SymplecticTimeScheme::launch (SymplecticTimeScheme.hpp:51-58): u +=
+// kappa_0 dt v over n doubles (the fields' complex components).
+__global__ __launch_bounds__(256) void k_sym_launch(double *__restrict__ u,
+                                                    const double *__restrict__ v,
+                                                    int64_t n, double kdt) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) u[i] += kdt * v[i];
+}
+
+void launch_sym_launch(hipStream_t st, double *u, const double *v, int64_t n,
+                       double kdt) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_sym_launch, dim3((unsigned)((n + 255) / 256)),
+                       dim3(256), 0, st, u, v, n, kdt);
+}
+
 // Receivers: one wave64 per station.  In-plane weighted sum over the
 // non-zero weights, then Re sum_alpha c_alpha x_alpha e^{i alpha phi}
 // with c_0 = 1 and c_alpha = 2 (Station.hpp:53-73, eigen_tools.hpp:79-106).
@@ -483,7 +583,7 @@ __global__ __launch_bounds__(64) void k_record(
     const uint32_t *__restrict__ point_off,
     const int32_t *__restrict__ point_nu1, const double *__restrict__ displ,
     double *__restrict__ out, const int *__restrict__ step_ctr,
-    int max_samples) {
+    int max_samples, const double *__restrict__ rec_rot) {
     const int r = blockIdx.x;
     if (r >= nrec) return;
     // a replayed step: sample = step - first recorded step (counter words
@@ -525,6 +625,20 @@ __global__ __launch_bounds__(64) void k_record(
         acc[c] = v;
     }
     if (lane == 0) {
+        // Station::rotateField (Station.hpp:79-118) as Station::rotate
+        // (Station.cpp:41-55): about axis k, v_i' = v_i c + v_j s, v_j' =
+        // -v_i s + v_j c with i = k + 1, j = k + 2 (mod 3)
+        if (rec_rot != nullptr) {
+            for (int slot = 0; slot < 2; slot++) {
+                const double *q = rec_rot + (size_t)r * 6 + 3 * slot;
+                const int k = (int)q[0];
+                if (k < 0) continue;
+                const int i = (k + 1) % 3, j = (k + 2) % 3;
+                const double vi = acc[i], vj = acc[j];
+                acc[i] = vi * q[1] + vj * q[2];
+                acc[j] = -vi * q[2] + vj * q[1];
+            }
+        }
 #pragma unroll
         for (int c = 0; c < 3; c++) out[r * 3 + c] = acc[c];
     }
@@ -614,6 +728,8 @@ void launch_point(hipStream_t st, int phases, const PointGeom &g,
     a.point_axial = g.point_axial;
     a.inv_mass = g.inv_mass;
     a.mass3d = g.mass3d;
+    a.ocean_idx = g.ocean_idx;
+    a.ocean1d = g.ocean1d;
     a.adj = g.adj;
     a.displ = displ;
     a.veloc = veloc;
@@ -624,20 +740,28 @@ void launch_point(hipStream_t st, int phases, const PointGeom &g,
     a.point_dt = g.point_dt;
     a.flag = flag;
     a.cpl = cpl;
-    if (cpl.n_pairs <= 0) phases &= ~PHASE_COUPLE;
-#define AXS3D_POINT_CASE(G, A, M, T, C)                                       \
-    case (A) | ((M) << 1) | ((T) << 2) | ((G) << 3) | ((C) << 4):            \
-        hipLaunchKernelGGL((k_point<Real, G, A, M, T, C>), dim3(grid),         \
+    a.abc = g.abc;
+    a.scheme = g.scheme;
+    a.sym_pi = g.sym_pi;
+    a.sym_kappa = g.sym_kappa;
+    if (cpl.n_pairs <= 0 && g.abc.idx == nullptr) phases &= ~PHASE_COUPLE;
+    if (g.abc.idx == nullptr) phases &= ~PHASE_SPONGE;
+    if (phases == 0) return;
+#define AXS3D_POINT_CASE(G, A, M, T, C, S)                                    \
+    case (A) | ((M) << 1) | ((T) << 2) | ((G) << 3) | ((C) << 4) |           \
+        ((S) << 5):                                                          \
+        hipLaunchKernelGGL((k_point<Real, G, A, M, T, C, S>), dim3(grid),      \
                            dim3(block), 0, st, a, eb, stiff);                 \
         break;
     switch (phases) {
-        AXS3D_POINT_CASE(0, 1, 0, 0, 0)
-        AXS3D_POINT_CASE(0, 0, 1, 0, 0)
-        AXS3D_POINT_CASE(0, 0, 0, 1, 0)
-        AXS3D_POINT_CASE(0, 0, 0, 0, 1)
-        AXS3D_POINT_CASE(0, 1, 1, 1, 0)
-        AXS3D_POINT_CASE(1, 1, 1, 1, 0)
-        AXS3D_POINT_CASE(1, 1, 1, 1, 1)
+        AXS3D_POINT_CASE(0, 1, 0, 0, 0, 0)
+        AXS3D_POINT_CASE(0, 0, 1, 0, 0, 0)
+        AXS3D_POINT_CASE(0, 0, 0, 1, 0, 0)
+        AXS3D_POINT_CASE(0, 0, 0, 0, 1, 0)
+        AXS3D_POINT_CASE(0, 0, 0, 0, 0, 1)
+        AXS3D_POINT_CASE(0, 1, 1, 1, 0, 0)
+        AXS3D_POINT_CASE(1, 1, 1, 1, 0, 0)
+        AXS3D_POINT_CASE(1, 1, 1, 1, 1, 0)
         default:
             break;
     }
@@ -649,11 +773,12 @@ void launch_record(hipStream_t st, int nrec, const int32_t *rec_elem,
                    const int32_t *elem_points, const int32_t *elem_nu1,
                    const uint32_t *point_off, const int32_t *point_nu1,
                    const double *displ, double *out, const int *step_ctr,
-                   int max_samples) {
+                   int max_samples, const double *rec_rot) {
     if (nrec <= 0) return;
     hipLaunchKernelGGL(k_record, dim3(nrec), dim3(64), 0, st, nrec, rec_elem,
                        rec_w, rec_phi, elem_points, elem_nu1, point_off,
-                       point_nu1, displ, out, step_ctr, max_samples);
+                       point_nu1, displ, out, step_ctr, max_samples,
+                       rec_rot);
 }
 
 // The step counter of replayed steps: [0] the step, [1] the first

@@ -15,7 +15,9 @@ enum PointPhase {
     PHASE_MASS = 2,
     PHASE_MARCH = 4,
     PHASE_GATHER = 8,
-    PHASE_COUPLE = 16    // solid-fluid coupling
+    PHASE_COUPLE = 16,   // Clayton boundaries, then solid-fluid coupling
+    PHASE_SPONGE = 32    // sponge on the acceleration (per-phase form; the
+                         // fused MASS | MARCH form applies it itself)
 };
 
 // Solid-fluid coupling pairs (one solid and one fluid point at the same
@@ -154,8 +156,38 @@ struct CardArgs {
     Real *dstress;                 // [6][npa]
 };
 
+// What the point kernels read of the absorbing boundaries (AbcArgs): per
+// point the index of its boundary record (-1: none), the record's term
+// offset, and whether its sponge is 3D (an acceleration that goes through
+// C2R and R2C loses the imaginary part of its order 0, Sponge.hpp:84-92).
+struct AbcGeom {
+    const int32_t *idx = nullptr;        // [point] record or -1; null: none
+    const int64_t *term_off = nullptr;
+    const int64_t *sp3_off = nullptr;
+    const double *cterm = nullptr;       // [term][nc] complex
+    const double *sterm = nullptr;
+};
+
+// Sponge term of mode alpha of boundary record ai subtracted from an
+// acceleration (Sponge::apply, Sponge.hpp:74-96; the term from k_abc); a
+// 3D sponge's C2R/R2C round trip drops the imaginary parts of order 0 and
+// of the Nyquist order nyq (nr / 2 for an even Nr, else -1).
+template <int NC, typename T>
+__device__ __forceinline__ void sponge_sub(const AbcGeom &g, int ai,
+                                           int alpha, int nyq, T *acc) {
+    if ((alpha == 0 || alpha == nyq) && g.sp3_off[ai] >= 0)
+#pragma unroll
+        for (int c = 0; c < NC; c++) acc[2 * c + 1] = (T)0;
+    const double *t =
+        g.sterm + ((size_t)g.term_off[ai] + alpha) * (2 * NC);
+#pragma unroll
+    for (int q = 0; q < 2 * NC; q++) acc[q] = (T)((double)acc[q] - t[q]);
+}
+
 // Device point description used by the gather and point kernels.
 struct PointGeom {
+    // absorbing boundaries of these points (idx null: none)
+    AbcGeom abc;
     int64_t n_modes;
     const int32_t *mode_point;
     const uint32_t *point_off;
@@ -163,12 +195,61 @@ struct PointGeom {
     const uint8_t *point_axial;
     const double *inv_mass;
     const uint8_t *mass3d;         // [point] 1: 3D mass, or null
+    // MassOceanLoad1D: [point] entry in ocean1d (-1: none), or null;
+    // ocean1d [entry][4] FP64 factors (MassOceanLoad1D.hpp:20-30)
+    const int32_t *ocean_idx = nullptr;
+    const double *ocean1d = nullptr;
     const uint32_t *adj;           // [point][4] element-local offsets,
                                    // ascending element, padded with ~0u
     // independent instances batched in one domain: the time step of each
     // point's instance (null: one time step for all)
     const double *point_dt = nullptr;
+    // the march: 0 Newmark; 1 a symplectic sub-step, v += sym_pi dt a, u
+    // += sym_kappa dt v (SymplecticTimeScheme::update)
+    int scheme = 0;
+    double sym_pi = 0., sym_kappa = 0.;
 };
+
+// Absorbing boundaries (AbsorbingBoundary, src/core/boundary/ABC) of the
+// points of one kind (solid: 3 components per mode, fluid: 1), merged per
+// point on the host: the Clayton-Enquist force term and the Kosloff sponge
+// acceleration term of a step are linear in the velocity and displacement
+// the step starts from, so k_abc forms both before the point kernels,
+// which subtract them in the reference's phases (Clayton before the
+// solid-fluid coupling, the sponge after the mass).
+//   cl1   [i][4] solid 1D Clayton sums (c0 s-s, c1 z-z, c2 s-z, c3 phi) /
+//         [i][1] fluid area / (rho vp); zeros without a 1D Clayton
+//   cl3   per slice, from cl3_off[i] (-1: none): solid the symmetric
+//         operator rsa I + k k^T summed over the point's entries, 6 values
+//         (ss, pp, zz, sp, sz, pz); fluid area / (rho vp)
+//   sp1   [i][2] 1D sponge (2 sum gamma, sum gamma^2); zeros without one
+//   sp3   per slice (2 gamma, gamma^2) from sp3_off[i] (-1: none)
+//   cterm, sterm  [term_off[i] + alpha][nc] complex FP64 terms
+template <typename Real>
+struct AbcArgs {
+    int n;                         // boundary points
+    int nr_max;                    // their largest Nr with a 3D part
+    const int32_t *point;          // [i] point
+    const uint32_t *point_off;
+    const int32_t *point_nr;
+    const double *displ, *veloc;   // the fields the step starts from
+    const double *cl1;
+    const int64_t *cl3_off;
+    const Real *cl3;
+    const double *sp1;
+    const int64_t *sp3_off;
+    const Real *sp3;
+    const int64_t *term_off;
+    double *cterm, *sterm;
+    const cpx<Real> *twiddle;
+    const int64_t *tw_off;
+    const uint32_t *plan;
+    const uint16_t *slice_pos;
+    const int64_t *pos_off;
+};
+
+template <typename Real, int NC>
+void launch_abc(hipStream_t st, const AbcArgs<Real> &a);
 
 // Solid points with a 3D mass (Mass3D): inverse mass per slice in Real,
 // the FFT tables of their Nr and the fields the march updates.
@@ -179,6 +260,10 @@ struct Mass3DArgs {
     const int32_t *point;          // [i] point index
     const int64_t *inv_off;        // [i] offset of its Nr inverse masses
     const Real *inv_mass;          // 1 / m per slice
+    // MassOceanLoad3D: [i] offset of the point's k per slice ([slice][3]
+    // in ocean_k; -1: plain Mass3D), or null for none
+    const int64_t *k_off = nullptr;
+    const Real *ocean_k = nullptr;
     const uint32_t *point_off;
     const int32_t *point_nr;
     const cpx<Real> *twiddle;
@@ -189,6 +274,9 @@ struct Mass3DArgs {
     double *displ, *veloc, *accel;
     double dt;
     int *flag;
+    AbcGeom abc;                   // sponge terms of the march
+    int scheme = 0;                // the march as PointGeom's
+    double sym_pi = 0., sym_kappa = 0.;
 };
 
 // Fluid points whose update needs transforms over the azimuth: a 3D mass
@@ -228,6 +316,9 @@ struct FluidDeferArgs {
     const uint32_t *plan;
     const uint16_t *slice_pos;
     const int64_t *pos_off;
+    AbcGeom abc;                   // sponge terms of the march
+    int scheme = 0;                // the march as PointGeom's
+    double sym_pi = 0., sym_kappa = 0.;
 };
 template <typename Real>
 void launch_fluid_defer(hipStream_t st, int phases,
@@ -308,12 +399,16 @@ void launch_fluid_point(hipStream_t st, int phases, const PointGeom &g,
                         double *chi_v, double *chi_a, double dt, int *flag,
                         const CouplingGeom &cpl, const double *solid_displ);
 
+// u += kdt v over n doubles (SymplecticTimeScheme::launch)
+void launch_sym_launch(hipStream_t st, double *u, const double *v, int64_t n,
+                       double kdt);
+
 void launch_record(hipStream_t st, int nrec, const int32_t *rec_elem,
                    const double *rec_w, const double *rec_phi,
                    const int32_t *elem_points, const int32_t *elem_nu1,
                    const uint32_t *point_off, const int32_t *point_nu1,
                    const double *displ, double *out, const int *step_ctr,
-                   int max_samples);
+                   int max_samples, const double *rec_rot = nullptr);
 
 // adjoint: sensitivity densities of one step, kern[e][k][3] += scale *
 // (lambda, mu, rho) densities of (psi, u, a); FP64 throughout

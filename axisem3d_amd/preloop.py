@@ -450,6 +450,7 @@ def build_domain(mesh: StructuredMesh,
         axial_points=axial_points.astype(np.int32), dt=dt)
     spec._mesh = mesh
     spec._gq = gq
+    spec._gq_mesh = gq
     spec._vp1d = vp
     spec._rho1d = rho
     spec._courant = courant
@@ -590,9 +591,418 @@ def make_fluid(spec, fluid_elem):
                     else v) for k, v in gq.items()}
     spec._vp1d = vp[se]
     spec._rho1d = rho[se]
+    spec._fvp1d = vp[fe]
+    spec._frho1d = rho[fe]
     m = -np.ones(E, dtype=np.int64)
     m[se] = np.arange(len(se))
     spec._elem_of_mesh = m
+    mf = -np.ones(E, dtype=np.int64)
+    mf[fe] = np.arange(len(fe))
+    spec._felem_of_mesh = mf
+    return spec
+
+
+# ------------------------------------------------ absorbing boundaries
+def gamma_solid_default(vp, vs, rho, span, t0):
+    """Kosloff sponge strength in solids, the template's gamma_expr_solid
+    (inparam.model.yaml: 1.1 / T0 * (VS / VP)^2 * exp(-0.04 * SPAN / (VP *
+    T0)), Haindl et al. 2020)."""
+    return 1.1 / t0 * (vs / vp) ** 2 * np.exp(-0.04 * span / (vp * t0))
+
+
+def gamma_fluid_default(vp, rho, span, t0):
+    """The template's gamma_expr_fluid: 0.88 / T0 * exp(-0.04 * SPAN / (VP
+    * T0))."""
+    return 0.88 / t0 * np.exp(-0.04 * span / (vp * t0))
+
+
+_EDGE_OF_KEY = {"BOTTOM": 0, "RIGHT": 1, "TOP": 2}
+
+
+def _solid_rho_vp_vs(spec, e):
+    """rho, vp, vs at the GLL points of solid element e
+    (Material::getPointwiseRhoVpVs, Material.cpp:250-300): (25,) arrays,
+    or (Nr, 25) for an element with a 3D medium; TI and anisotropic media
+    through the Voigt averages of kappa and mu."""
+    rho = np.asarray(spec._rho1d[e], dtype=np.float64)
+    mtype = getattr(spec, "elem_mtype", None)
+    kind = getattr(spec, "elem_kind", None)
+    mt = 0 if mtype is None else int(mtype[e])
+    if kind is not None and int(kind[e]) == 1:
+        nr = int(spec.elem_nr[e])
+        npar = MAT_NPARAMS[mt]
+        if mtype is None:
+            s0 = int(spec.elem_slice_off[e])
+            par = np.asarray(spec.prop3d)[s0:s0 + nr].reshape(nr, 2, NPEM)
+        else:
+            o = int(spec.prop_off[e])
+            par = np.asarray(spec.prop3d)[o:o + nr * npar * NPEM].reshape(
+                nr, npar, NPEM)
+        kp, mu = _voigt_kappa_mu(mt, np.transpose(par, (1, 0, 2)))
+    else:
+        if mtype is None:
+            par = spec.elem_iso[e]
+        else:
+            o = int(spec.mat1d_off[e])
+            par = np.asarray(spec.mat1d)[o:o + MAT_NPARAMS[mt] * NPEM]
+            par = par.reshape(MAT_NPARAMS[mt], NPEM)
+        kp, mu = _voigt_kappa_mu(mt, par)
+    vp = np.sqrt((kp + mu * (4.0 / 3.0)) / rho)
+    vs = np.sqrt(np.maximum(mu, 0.0) / rho)
+    return rho, vp, vs
+
+
+def _fluid_rho_vp(spec, f):
+    """rho, vp at the GLL points of fluid element f: (25,), or (Nr, 25)
+    with a 3D medium (rho = 1 / K per slice, vp of the 1D model)."""
+    vp = np.asarray(spec._fvp1d[f], dtype=np.float64)
+    kind = getattr(spec, "fluid_elem_kind", None)
+    if kind is not None and int(kind[f]) == 1:
+        nr = int(spec.fluid_elem_nr[f])
+        off = 0
+        for g in range(f):
+            if kind[g]:
+                off += int(spec.fluid_elem_nr[g]) * NPEM
+        k3 = np.asarray(spec.fluid_k3d)[off:off + nr * NPEM].reshape(nr, NPEM)
+        return 1.0 / k3, np.broadcast_to(vp[None, :], (nr, NPEM))
+    return np.asarray(spec._frho1d[f], dtype=np.float64), vp
+
+
+def _at_point(x, ipnt, nr_point):
+    """Values of GLL point ipnt at the point's own Nr slices: a 1D value
+    stays one value; an element's Nr slices are sampled at the nearest
+    slice of each of the point's (slice k of Nr_p at phi = 2 pi k / Nr_p)."""
+    x = np.asarray(x)
+    if x.ndim == 1:
+        return np.array([x[ipnt]])
+    nre = x.shape[0]
+    if nre == nr_point:
+        return x[:, ipnt].copy()
+    k = np.arange(nr_point)
+    src = np.rint(k * nre / nr_point).astype(np.int64) % nre
+    return x[src, ipnt]
+
+
+def add_ocean_load(spec, sum_rho_depth):
+    """Ocean load on the solid points of the mesh top (Quad::setupGLL,
+    Quad.cpp:235-246; GLLPoint::addOceanLoad and release, GLLPoint.hpp:
+    88-93, GLLPoint.cpp:54-87).  Per top point: nTop, the integrated
+    outward normals of the TOP edges of the solid elements that share it,
+    summed (Quad::computeNormal), and sum_rho_depth(s, z, phi) (kg/m^2,
+    the ocean's integrated density above the point, a scalar or one value
+    per slice phi_k = 2 pi k / Nr).  With a 1D mass and one sum_rho_depth
+    for all slices the point gets MassOceanLoad1D (m0 = sum_rho_depth
+    |nTop|, t = acos(nz / |nTop|) as geodesy::sz2rtheta gives it), else
+    MassOceanLoad3D (per slice 1/m and k = sqrt(m0 / (m (m + m0))) nTop /
+    |nTop|, m its 3D mass or its 1D mass on every slice); a point's 3D
+    mass moves into its ocean-load entry.  Call after the medium and the
+    3D masses are set.  Fills spec.ocean_load = dict(point, nval, off,
+    data): 4 numbers per value, as the oracle and axs3d_set_ocean_load
+    take them."""
+    mesh = spec._mesh
+    gq = spec._gq_mesh
+    ix, iz = mesh.element_ix_iz()
+    axial = mesh.is_axial()
+    w_gll = spectral.weights_1d(False)
+    w_glj = spectral.weights_1d(True)
+    s_of_mesh = np.asarray(spec._elem_of_mesh)
+    ntop = {}
+    for m in np.nonzero(iz == mesh.nz - 1)[0]:
+        e = int(s_of_mesh[m])
+        if e < 0:
+            continue
+        pts, n = _edge_normals(gq["J"][m], bool(axial[m]), 2, w_gll, w_glj,
+                               gq["sz"][m])
+        for k, ipnt in enumerate(pts):
+            p = int(spec.elem_points[e][ipnt])
+            ntop[p] = ntop.get(p, np.zeros(2)) + n[k]
+    m3p = getattr(spec, "mass3d_points", None)
+    m3 = {}
+    if m3p is not None and len(m3p) > 0:
+        nr_all = np.asarray(spec.solid_nr, dtype=np.int64)
+        cum = 0
+        for p in np.asarray(m3p, dtype=np.int64):
+            m3[int(p)] = np.asarray(spec.mass3d[cum:cum + nr_all[p]])
+            cum += nr_all[p]
+    points, nval, off, data = [], [], [], []
+    cum = 0
+    for p in sorted(ntop):
+        nr = int(spec.solid_nr[p])
+        phi = 2 * np.pi * np.arange(nr) / nr
+        s, z = spec.solid_sz[p]
+        srd = np.broadcast_to(np.asarray(
+            sum_rho_depth(s, z, phi), dtype=np.float64), (nr,)).copy()
+        n = ntop[p]
+        area = float(np.hypot(n[0], n[1]))
+        if p not in m3 and np.all(srd == srd[0]):
+            m = float(spec.solid_mass[p])
+            m0 = float(srd[0]) * area
+            t = 0.0 if area < np.finfo(float).eps else math.acos(n[1] / area)
+            c, si = math.cos(t), math.sin(t)
+            ih, iv = 1.0 / m, 1.0 / (m + m0)
+            vals = np.array([[ih * c * c + iv * si * si,
+                              ih * si * si + iv * c * c,
+                              (iv - ih) * c * si, ih]])
+        else:
+            m = m3.get(p, np.full(nr, float(spec.solid_mass[p])))
+            m0 = srd * area
+            unit = np.array([n[0], 0.0, n[1]]) / area
+            kk = np.sqrt(m0 / (m * (m + m0)))
+            vals = np.concatenate([(1.0 / m)[:, None],
+                                   kk[:, None] * unit[None, :]], axis=1)
+        points.append(p)
+        nval.append(len(vals))
+        off.append(cum)
+        data.append(vals.reshape(-1))
+        cum += len(vals)
+    if m3:
+        # the 3D masses of ocean-load points move into their entries
+        keep = [p for p in np.asarray(m3p, dtype=np.int64)
+                if int(p) not in ntop]
+        spec.mass3d_points = np.array(keep, dtype=np.int32)
+        spec.mass3d = (np.concatenate([m3[int(p)] for p in keep])
+                       if keep else np.zeros(0))
+    spec.ocean_load = {
+        "point": np.array(points, dtype=np.int32),
+        "nval": np.array(nval, dtype=np.int32),
+        "off": np.array(off, dtype=np.int64),
+        "data": np.concatenate(data) if data else np.zeros(0)}
+    return spec
+
+
+def add_absorbing(spec, keys=("RIGHT", "BOTTOM"), clayton=True,
+                  sponge=False, rel_spans=None, t0=None,
+                  gamma_solid=gamma_solid_default,
+                  gamma_fluid=gamma_fluid_default):
+    """Absorbing boundaries on the mesh sides `keys` (RIGHT: s max or
+    theta max; BOTTOM: z min or r min; TOP), after the medium (1D or 3D)
+    is set and before sources and receivers (dt changes):
+
+    * Clayton-Enquist (Quad::setupGLL, Quad.cpp:142-163; GLLPoint::
+      release, GLLPoint.cpp:152-192): one entry per boundary element edge
+      point, with the element's integrated edge normal nABC
+      (Quad::computeNormal) and its rho vp, rho vs; 1D elements give
+      ClaytonSolid1D (rho vp, rho vs, area = |nABC|, theta = acos(nz /
+      |nABC|) as geodesy::sz2rtheta gives it) or ClaytonFluid1D, 3D media
+      the per-slice forms (rho vp, rho vs, area, unit normal per slice of
+      the point);
+    * Kosloff sponge (Quad.cpp:165-231, GLLPoint.cpp:193-215): at every
+      element GLL point within rel_span * mesh span of a boundary, gamma =
+      gamma_expr(vp, vs, rho, |span|, T0) cos^2(pi/2 d), d the relative
+      distance to the boundary (the nearest boundary wins); a point's
+      gamma is the mean over the elements that give it one;
+    * dt: elements with a Clayton edge take half the time step
+      (factorDtForBC, Quad.cpp:330-349); the attenuation coefficients are
+      recomputed for the new dt.
+
+    Fills spec.clayton_solid / clayton_fluid / sponge_solid /
+    sponge_fluid: dicts of point (entries), nval (1 or Nr), off (value
+    offset) and data (4 or 1 numbers per value for the solid Clayton:
+    1D coefficients or (rsa, k_s, k_phi, k_z) per slice; 1 number per
+    value otherwise: area / (rho vp), or gamma), as the oracle takes them,
+    plus the raw per-entry inputs of the C ABI under 'abi'."""
+    if getattr(spec, "stf", None) is not None:
+        raise ValueError("add_absorbing: set the boundaries before the "
+                         "source (dt changes)")
+    mesh = spec._mesh
+    gq = spec._gq_mesh
+    keys = list(keys)
+    if rel_spans is None:
+        rel_spans = [0.05] * len(keys)
+    E = mesh.nelem
+    ix, iz = mesh.element_ix_iz()
+    axial = mesh.is_axial()
+    w_gll = spectral.weights_1d(False)
+    w_glj = spectral.weights_1d(True)
+    s_of_mesh = np.asarray(spec._elem_of_mesh)
+    f_of_mesh = getattr(spec, "_felem_of_mesh", -np.ones(E, np.int64))
+    mat_s = {}
+    mat_f = {}
+
+    def material(m):
+        if s_of_mesh[m] >= 0:
+            e = int(s_of_mesh[m])
+            if e not in mat_s:
+                mat_s[e] = _solid_rho_vp_vs(spec, e)
+            return False, e, mat_s[e]
+        f = int(f_of_mesh[m])
+        if f not in mat_f:
+            r, v = _fluid_rho_vp(spec, f)
+            mat_f[f] = (r, v, np.zeros_like(v))
+        return True, f, mat_f[f]
+
+    def point_of(fluid, e, ipnt):
+        if fluid:
+            p = int(spec.fluid_elem_points[e][ipnt])
+            return p, int(spec.fluid_nr[p])
+        p = int(spec.elem_points[e][ipnt])
+        return p, int(spec.solid_nr[p])
+
+    lists = {k: {"point": [], "nval": [], "vals": []} for k in
+             ("clayton_solid", "clayton_fluid", "sponge_solid",
+              "sponge_fluid")}
+    abi = {"clayton_solid": [], "clayton_solid3d": [],
+           "clayton_fluid": [], "clayton_fluid3d": [],
+           "sponge_solid": {}, "sponge_fluid": {}}
+    edge_ok = {"BOTTOM": iz == 0, "RIGHT": ix == mesh.nx - 1,
+               "TOP": iz == mesh.nz - 1}
+    factor = np.ones(E)
+    if clayton:
+        for key in keys:
+            edge = _EDGE_OF_KEY[key]
+            for m in np.nonzero(edge_ok[key])[0]:
+                fluid, e, (rho, vp, vs) = material(m)
+                factor[m] = 0.5
+                pts, nabc = _edge_normals(gq["J"][m], bool(axial[m]), edge,
+                                          w_gll, w_glj, gq["sz"][m])
+                for k, ipnt in enumerate(pts):
+                    p, nrp = point_of(fluid, e, ipnt)
+                    n2 = nabc[k]
+                    area = float(np.hypot(n2[0], n2[1]))
+                    rp = _at_point(rho * vp, ipnt, nrp)
+                    rs = _at_point(rho * vs, ipnt, nrp)
+                    three = len(rp) > 1
+                    if fluid:
+                        vals = area / rp
+                        lists["clayton_fluid"]["point"].append(p)
+                        lists["clayton_fluid"]["nval"].append(len(rp))
+                        lists["clayton_fluid"]["vals"].append(vals)
+                        abi["clayton_fluid3d" if three else
+                            "clayton_fluid"].append((p, rp, np.full(len(rp),
+                                                                    area)))
+                        continue
+                    if not three:
+                        th = math.acos(n2[1] / area) if area > 0 else 0.0
+                        c, s = math.cos(th), math.sin(th)
+                        rsa, rpa = rs[0] * area, rp[0] * area
+                        vals = np.array([rsa * c * c + rpa * s * s,
+                                         rsa * s * s + rpa * c * c,
+                                         (rpa - rsa) * c * s, rsa])
+                        abi["clayton_solid"].append((p, rp[0], rs[0], area,
+                                                     th))
+                    else:
+                        un = np.array([n2[0], 0.0, n2[1]]) / area
+                        kk = np.sqrt((rp - rs) * area)[:, None] * un[None, :]
+                        vals = np.concatenate([(rs * area)[:, None], kk],
+                                              axis=1).reshape(-1)
+                        abi["clayton_solid3d"].append(
+                            (p, rp, rs, np.full(nrp, area),
+                             np.broadcast_to(un, (nrp, 3)).copy()))
+                    lists["clayton_solid"]["point"].append(p)
+                    lists["clayton_solid"]["nval"].append(len(rp))
+                    lists["clayton_solid"]["vals"].append(vals)
+    if sponge:
+        if t0 is None:
+            raise ValueError("add_absorbing: the sponge needs the mesh "
+                             "period t0")
+        info = {}
+        sz_m = gq["sz"]
+        for key, rel in zip(keys, rel_spans):
+            if mesh.spherical:
+                r = np.hypot(sz_m[:, 0], sz_m[:, 1])
+                th = np.arccos(np.clip(sz_m[:, 1] / np.maximum(r, 1e-300),
+                                       -1, 1))
+                if key == "RIGHT":
+                    outer = float(th.max())
+                    span = rel * outer
+                elif key == "BOTTOM":
+                    outer = float(r.min())
+                    span = rel * (float(r.min()) - float(r.max()))
+                else:
+                    outer = float(r.max())
+                    span = rel * (float(r.max()) - float(r.min()))
+            else:
+                s, z = sz_m[:, 0], sz_m[:, 1]
+                if key == "RIGHT":
+                    outer = float(s.max())
+                    span = rel * outer
+                elif key == "BOTTOM":
+                    outer = float(z.min())
+                    span = rel * (float(z.min()) - float(z.max()))
+                else:
+                    outer = float(z.max())
+                    span = rel * (float(z.max()) - float(z.min()))
+            info[key] = (outer, span)
+        acc_s, acc_f = {}, {}
+        for m in range(E):
+            fluid, e, (rho, vp, vs) = material(m)
+            for ipnt in range(NPEM):
+                s_, z_ = gq["sz"][m, 0, ipnt], gq["sz"][m, 1, ipnt]
+                if mesh.spherical:
+                    rr = math.hypot(s_, z_)
+                    tt = math.acos(max(-1.0, min(1.0, z_ / rr))) \
+                        if rr > 0 else 0.0
+                best = 1.0
+                gam = None
+                for key in keys:
+                    outer, span = info[key]
+                    if mesh.spherical:
+                        coord = tt if key == "RIGHT" else rr
+                        radius = rr
+                    else:
+                        coord = s_ if key == "RIGHT" else z_
+                        radius = z_
+                    d = (outer - coord) / span
+                    if d > best:
+                        continue
+                    best = d
+                    pat = math.cos(0.5 * math.pi * d) ** 2
+                    sp = span * radius if (mesh.spherical and
+                                           key == "RIGHT") else span
+                    rv = rho[..., ipnt] if np.ndim(rho) == 2 else rho[ipnt]
+                    pv = vp[..., ipnt] if np.ndim(vp) == 2 else vp[ipnt]
+                    sv = vs[..., ipnt] if np.ndim(vs) == 2 else vs[ipnt]
+                    if fluid:
+                        g = gamma_fluid(pv, rv, abs(sp), t0)
+                    else:
+                        g = gamma_solid(pv, sv, rv, abs(sp), t0)
+                    gam = np.atleast_1d(np.asarray(g, dtype=np.float64)) * pat
+                if gam is None or np.linalg.norm(gam) <= 2.220446049250313e-16:
+                    continue
+                p, nrp = point_of(fluid, e, ipnt)
+                if len(gam) > 1 and len(gam) != nrp:
+                    gam = _at_point(gam[:, None], 0, nrp)
+                acc = acc_f if fluid else acc_s
+                tot, cnt = acc.get(p, (np.zeros(1), 0))
+                if len(gam) > 1 and len(tot) == 1:
+                    tot = np.full(nrp, tot[0])
+                if len(gam) == 1 and len(tot) > 1:
+                    gam = np.full(len(tot), gam[0])
+                acc[p] = (tot + gam, cnt + 1)
+        for fluid, acc in ((False, acc_s), (True, acc_f)):
+            name = "sponge_fluid" if fluid else "sponge_solid"
+            for p in sorted(acc):
+                tot, cnt = acc[p]
+                g = tot / cnt
+                lists[name]["point"].append(p)
+                lists[name]["nval"].append(len(g))
+                lists[name]["vals"].append(g)
+                abi[name][p] = g
+    for name, d in lists.items():
+        nval = np.asarray(d["nval"], dtype=np.int32)
+        width = 4 if name == "clayton_solid" else 1
+        off = np.zeros(len(nval), dtype=np.int64)
+        if len(nval):
+            off[1:] = np.cumsum(nval)[:-1]
+        data = (np.concatenate([np.asarray(v, np.float64).reshape(-1)
+                                for v in d["vals"]])
+                if d["vals"] else np.zeros(0))
+        assert len(data) == width * int(nval.sum())
+        setattr(spec, name, {"point": np.asarray(d["point"], np.int32),
+                             "nval": nval, "off": off, "data": data})
+    spec._abc_abi = abi
+    # dt: half the step for the elements with a Clayton edge
+    if clayton:
+        vmax = np.zeros(E)
+        for m in range(E):
+            fluid, e, (rho, vp, vs) = material(m)
+            vmax[m] = float(np.max(vp))
+        old = spec.dt
+        spec.dt = compute_dt(gq["sz"], vmax, spec._courant, factor)
+        att = getattr(spec, "att", None)
+        if att is not None and spec.dt != old:
+            a, b, g = att_alpha_beta_gamma(att["w"], att["y"], spec.dt)
+            att.update(alpha=a, beta=b, gamma=g)
     return spec
 
 

@@ -172,6 +172,14 @@ class GpuSolver:
             m3 = _c(s.mass3d, np.float64)
             self._chk(L.axs3d_set_solid_mass3d(ctx, len(m3p), m3p.ctypes.data,
                                                m3.ctypes.data), "set_mass3d")
+        oc = getattr(s, "ocean_load", None)
+        if oc is not None and len(oc["point"]) > 0:
+            op = _c(oc["point"], np.int32)
+            onv = _c(oc["nval"], np.int32)
+            ov = _c(oc["data"], np.float64)
+            self._chk(L.axs3d_set_ocean_load(ctx, len(op), op.ctypes.data,
+                                             onv.ctypes.data, ov.ctypes.data),
+                      "set_ocean_load")
         if s.src_elem >= 0 and s.src_pattern is not None:
             pat = _c(s.src_pattern, np.complex128)
             stf = _c(s.stf, np.float64)
@@ -202,8 +210,69 @@ class GpuSolver:
             self._chk(L.axs3d_set_receivers(ctx, self.nrec, re.ctypes.data,
                                             rw.ctypes.data, rp.ctypes.data,
                                             ms), "set_receivers")
+            fr = getattr(s, "rec_frames", None)
+            if fr is not None:
+                w = _c(fr["wcs"], np.int32)
+                th = _c(fr["theta"], np.float64)
+                bz = _c(fr["baz"], np.float64)
+                self._chk(L.axs3d_set_receiver_frames(
+                    ctx, self.nrec, w.ctypes.data, th.ctypes.data,
+                    bz.ctypes.data, int(fr["cartesian"])),
+                    "set_receiver_frames")
+        self._upload_abc(getattr(s, "_abc_abi", None))
+        if getattr(s, "time_scheme", "newmark") == "symplectic":
+            self._chk(L.axs3d_set_time_scheme(ctx, 1), "set_time_scheme")
         self._chk(L.axs3d_set_dt(ctx, float(s.dt)), "set_dt")
         self._chk(L.axs3d_finalize(ctx), "finalize")
+
+    def _upload_abc(self, abi):
+        """Absorbing-boundary entries as the reference constructs them
+        (preloop.add_absorbing): Clayton 1D/3D on solid and fluid points,
+        sponges."""
+        if not abi:
+            return
+        L, ctx = self.lib, self.ctx
+        cs = abi.get("clayton_solid") or []
+        if cs:
+            pts = _c([e[0] for e in cs], np.int32)
+            a = [_c([e[k] for e in cs], np.float64) for k in (1, 2, 3, 4)]
+            self._chk(L.axs3d_add_clayton_solid(
+                ctx, len(cs), pts.ctypes.data, *[x.ctypes.data for x in a]),
+                "add_clayton_solid")
+        c3 = abi.get("clayton_solid3d") or []
+        if c3:
+            pts = _c([e[0] for e in c3], np.int32)
+            a = [_c(np.concatenate([np.asarray(e[k]).reshape(-1)
+                                    for e in c3]), np.float64)
+                 for k in (1, 2, 3, 4)]
+            self._chk(L.axs3d_add_clayton_solid3d(
+                ctx, len(c3), pts.ctypes.data, *[x.ctypes.data for x in a]),
+                "add_clayton_solid3d")
+        for key, per in (("clayton_fluid", 0), ("clayton_fluid3d", 1)):
+            cf = abi.get(key) or []
+            if not cf:
+                continue
+            pts = _c([e[0] for e in cf], np.int32)
+            rp = _c(np.concatenate([np.atleast_1d(e[1]) for e in cf]),
+                    np.float64)
+            ar = _c(np.concatenate([np.atleast_1d(e[2]) for e in cf]),
+                    np.float64)
+            self._chk(L.axs3d_add_clayton_fluid(
+                ctx, len(cf), pts.ctypes.data, per, rp.ctypes.data,
+                ar.ctypes.data), "add_clayton_fluid")
+        for fluid, key in ((0, "sponge_solid"), (1, "sponge_fluid")):
+            sp = abi.get(key) or {}
+            for per in (0, 1):
+                ent = [(p, g) for p, g in sp.items()
+                       if (len(g) > 1) == bool(per)]
+                if not ent:
+                    continue
+                pts = _c([p for p, _ in ent], np.int32)
+                g = _c(np.concatenate([np.asarray(x).reshape(-1)
+                                       for _, x in ent]), np.float64)
+                self._chk(L.axs3d_add_sponge(ctx, fluid, len(ent),
+                                             pts.ctypes.data, per,
+                                             g.ctypes.data), "add_sponge")
 
     def close(self):
         if getattr(self, "ctx", None) is not None and self.ctx.value:
@@ -251,11 +320,20 @@ class GpuSolver:
     def applyBC_AfterAssemblingStiff(self):
         self._chk(self.lib.axs3d_apply_bc_after(self.ctx))
 
+    def applyBC_AfterComputingAccel(self):
+        self._chk(self.lib.axs3d_apply_bc_after_accel(self.ctx))
+
     def computeStiffToAccel(self):
         self._chk(self.lib.axs3d_stiff_to_accel(self.ctx))
 
     def newmark(self):
         self._chk(self.lib.axs3d_newmark(self.ctx))
+
+    def symplecticLaunch(self):
+        self._chk(self.lib.axs3d_symplectic_launch(self.ctx))
+
+    def symplecticUpdate(self, isub):
+        self._chk(self.lib.axs3d_symplectic_update(self.ctx, int(isub)))
 
     def checkStability(self, tstep=0):
         self._chk(self.lib.axs3d_check_stability(self.ctx, tstep))
@@ -339,8 +417,16 @@ class GpuSolver:
         ms = (C.c_double * 3)()
         self._chk(self.lib.axs3d_autotune_info(self.ctx, C.byref(occ),
                                                C.byref(blocks), ms))
+        rep = (C.c_double * 32)()
+        nw = C.c_int()
+        self._chk(self.lib.axs3d_autotune_report(self.ctx, rep, 32,
+                                                 C.byref(nw)))
+        r = list(rep[:nw.value])
         return {"cardinal_occupancy": occ.value,
                 "cardinal_blocks": blocks.value,
                 "cardinal_pass_ms": ms[0],
                 "cardinal_scratch_mib": ms[1],
-                "scratch_sizes_timed": int(ms[2])}
+                "scratch_sizes_timed": int(ms[2]),
+                "fft_max_radix": int(r[3]),
+                "fft_radices_timed": {int(r[5 + 2 * i]): r[6 + 2 * i]
+                                      for i in range(int(r[4]))}}

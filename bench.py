@@ -182,37 +182,64 @@ def bytes_per_step(spec, real):
             "cd_quad": int(quad), "point": int(point)}
 
 
+def source_fingerprint():
+    """sha256 (first 12 hex digits) of the native sources and the build
+    recipe: what a counter summary in profiles/ was measured on.  The
+    repository's history is not needed (a bare source tree has none)."""
+    import hashlib
+    h = hashlib.sha256()
+    base = os.path.join(ROOT, "axisem3d_amd")
+    files = [os.path.join("csrc", f) for f in
+             sorted(os.listdir(os.path.join(base, "csrc")))
+             if f.endswith((".hip", ".hpp", ".cpp", ".h"))] + ["build.py"]
+    for f in files:
+        h.update(f.encode())
+        with open(os.path.join(base, f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:12]
+
+
+def _profile_row(table, kernel, workload, nfields):
+    """The fields of `kernel` on `workload` in profiles/<table>, and the
+    source fingerprint its header records; None for the fields when the
+    table is absent, has no such row, or was measured on other sources
+    (the counters then describe kernels that no longer exist)."""
+    path = os.path.join(ROOT, "profiles", table)
+    if not os.path.exists(path):
+        return None, None
+    src = None
+    for line in open(path):
+        f = line.split()
+        if line.startswith("#"):
+            if "source" in f and f.index("source") + 1 < len(f):
+                src = f[f.index("source") + 1]
+            continue
+        if len(f) >= nfields and f[0] == kernel and f[1] == workload:
+            if src != source_fingerprint():
+                return None, src
+            return f, src
+    return None, src
+
+
 def pmc_valu_busy(kernel, workload):
     """Fraction of the VALU issue slots `kernel` keeps busy on `workload`
     (SQ_INSTS_VALU over the CU cycles, from the committed rocprofv3
-    counter summary), or None."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)),
-                        "profiles", "valu_busy.txt")
-    if not os.path.exists(path):
-        return None
-    for line in open(path):
-        f = line.split()
-        if len(f) >= 3 and f[0] == kernel and f[1] == workload:
-            return float(f[2])
-    return None
+    counter summary), or None (no summary, or one of other sources)."""
+    f, _ = _profile_row("valu_busy.txt", kernel, workload, 3)
+    return float(f[2]) if f else None
 
 
 def pmc_traffic(kernel, workload):
     """HBM bytes per launch of `kernel` on `workload` from the committed
     rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE per dispatch, each
     scaled by what the counter reads for a copy kernel of known size), or
-    None when no summary covers it."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)),
-                        "profiles", "hbm_traffic.txt")
-    if not os.path.exists(path):
+    None when no summary of these sources covers it."""
+    f, _ = _profile_row("hbm_traffic.txt", kernel, workload, 6)
+    if not f:
         return None
-    for line in open(path):
-        f = line.split()
-        if len(f) >= 6 and f[0] == kernel and f[1] == workload:
-            fetch_kib, fetch_fac = float(f[2]), float(f[3])
-            write_kib, write_fac = float(f[4]), float(f[5])
-            return (fetch_kib * fetch_fac + write_kib * write_fac) * 1024.0
-    return None
+    fetch_kib, fetch_fac = float(f[2]), float(f[3])
+    write_kib, write_fac = float(f[4]), float(f[5])
+    return (fetch_kib * fetch_fac + write_kib * write_fac) * 1024.0
 
 
 def time_gpu(spec, prec, steps, warmup, graph=False, total=0):
@@ -554,7 +581,11 @@ def main():
                      "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic,
-                     "valu_busy": pmc_valu_busy(names[dom], a.workload)},
+                     "valu_busy": pmc_valu_busy(names[dom], a.workload),
+                     "counters_source": {
+                         "profiles": _profile_row("hbm_traffic.txt", "", "",
+                                                  99)[1],
+                         "this_tree": source_fingerprint()}},
         "step_roofline": {"algorithmic_bytes_per_step": step_bytes,
                           "achieved_GBs": step_bytes / (t32 / a.steps) / 1e9,
                           "frac": step_bytes / (t32 / a.steps) / 1e9 /

@@ -267,6 +267,20 @@ int axs3d_set_axial_points(axs3d_ctx *ctx, int n, const int32_t *points);
 int axs3d_set_solid_mass3d(axs3d_ctx *ctx, int n, const int32_t *points,
                            const double *mass);
 
+/* Ocean load on solid points (after axs3d_set_solid_points; replaces the
+ * listed points' mass).  Entry i acts on points[i] with nval[i] values of
+ * 4 numbers each: nval 1 gives MassOceanLoad1D, the factors 1/m cos^2 t +
+ * 1/(m + m0) sin^2 t, 1/m sin^2 t + 1/(m + m0) cos^2 t, (1/(m + m0) -
+ * 1/m) cos t sin t and 1/m (FP64, MassOceanLoad1D.hpp:20-30); nval = the
+ * point's Nr gives MassOceanLoad3D, per slice 1/m and k = sqrt(m0 / (m (m
+ * + m0))) n (MassOceanLoad3D.cpp:13-20, in the working precision), m the
+ * point's mass, m0 its ocean mass, t the colatitude of its surface normal
+ * n.  Replaces the ocean-load branches of GLLPoint::release (src/preloop/
+ * se_model/gll/GLLPoint.cpp:54-87).  A listed point must not also have a
+ * 3D mass from axs3d_set_solid_mass3d. */
+int axs3d_set_ocean_load(axs3d_ctx *ctx, int n, const int32_t *points,
+                         const int32_t *nval, const double *values);
+
 /* Moment-tensor source: element, Fourier pattern [nu1][6][25] complex and
  * the source-time function sampled at every step (FP64).  Replaces
  * SolidMoment (src/core/source/elem_src/solid/SolidMoment.cpp:15-33) with
@@ -296,11 +310,58 @@ int axs3d_set_instances(axs3d_ctx *ctx, int n, const int32_t *point_inst,
 int axs3d_set_receivers(axs3d_ctx *ctx, int n, const int32_t *elem,
                         const double *weights, const double *phi,
                         int max_samples);
+/* Station frames: per receiver the wavefield coordinate system wcs (0
+ * spz, 1 RTZ, 2 ENZ, 3 xyz; channel::WavefieldCS, src/core/output/
+ * channel.hpp:24), theta (radians) and back azimuth (radians), the
+ * arguments of the Station constructor (Station.hpp:24-29, built at
+ * StationOutput.cpp:429-447); cartesian != 0 for a Cartesian mesh.  The
+ * receiver kernel then records the displacement rotated as
+ * Station::rotateField does (Station.hpp:79-118).  Without this call the
+ * seismograms stay in (s, phi, z). */
+int axs3d_set_receiver_frames(axs3d_ctx *ctx, int n, const int32_t *wcs,
+                              const double *theta,
+                              const double *back_azimuth, int cartesian);
+
+/* ---------------- absorbing boundaries (src/core/boundary/ABC) ------- */
+/* Entries are added (repeated points allowed, as a point on two boundary
+ * element edges gets two); they are merged per point at finalize.
+ * Clayton-Enquist on solid points, 1D form: per entry rho vp, rho vs, the
+ * area (|n| of the element's integrated edge normal) and theta (its angle
+ * to z), the ClaytonSolid1D constructor (ClaytonSolid1D.hpp:19-33, built at
+ * GLLPoint.cpp:172-178). */
+int axs3d_add_clayton_solid(axs3d_ctx *ctx, int n, const int32_t *points,
+                            const double *rho_vp, const double *rho_vs,
+                            const double *area, const double *theta);
+/* 3D form, ClaytonSolid3D (ClaytonSolid3D.hpp:19-27, GLLPoint.cpp:
+ * 180-191): per entry Nr (of its point) values of rho vp, rho vs, area and
+ * unit normal [Nr][3] (s, phi, z), concatenated in entry order. */
+int axs3d_add_clayton_solid3d(axs3d_ctx *ctx, int n, const int32_t *points,
+                              const double *rho_vp, const double *rho_vs,
+                              const double *area,
+                              const double *unit_normal);
+/* Clayton-Enquist on fluid points, ClaytonFluid1D (per_slice = 0: one
+ * rho vp and area per entry) or ClaytonFluid3D (Nr of each), GLLPoint.cpp:
+ * 156-170. */
+int axs3d_add_clayton_fluid(axs3d_ctx *ctx, int n, const int32_t *points,
+                            int per_slice, const double *rho_vp,
+                            const double *area);
+/* Kosloff sponge, Sponge<SolidPoint> (fluid = 0) or Sponge<FluidPoint>
+ * (Sponge.hpp:30-45, GLLPoint.cpp:193-215): gamma per entry (per_slice =
+ * 0) or Nr values per entry. */
+int axs3d_add_sponge(axs3d_ctx *ctx, int fluid, int n, const int32_t *points,
+                     int per_slice, const double *gamma);
 
 /* Time step.  Replaces TimeScheme::setTime (src/core/time/TimeScheme.hpp)
  * as used by NewmarkTimeScheme::update's dt constants
  * (NewmarkTimeScheme.hpp:65-67). */
 int axs3d_set_dt(axs3d_ctx *ctx, double dt);
+
+/* Time scheme: 0 = NewmarkTimeScheme (default), 1 = SymplecticTimeScheme
+ * (src/core/time/SymplecticTimeScheme.cpp:18-142: per step the receivers,
+ * the source, u += kappa_0 dt v, then four sub-steps of stiffness,
+ * boundaries, mass and v += pi_k dt a, u += kappa_k dt v).  axs3d_run
+ * follows the chosen scheme; one time step for the whole domain. */
+int axs3d_set_time_scheme(axs3d_ctx *ctx, int scheme);
 
 /* Ends set-up: checks sizes and compatibility (the reference's
  * checkCompatibility calls, e.g. src/core/element/common/Property.hpp:53-80)
@@ -324,19 +385,31 @@ int axs3d_compute_stiffness(axs3d_ctx *ctx);
  * one device (contexts on different streams) race.  Nothing in the array
  * needs to start at zero: each batch writes every row it reads. */
 int axs3d_compute_stiffness_static(axs3d_ctx *ctx);
-/* Domain::applyBC_BeforeAssemblingStiff (Domain.cpp:313-317). */
+/* Domain::applyBC_BeforeAssemblingStiff (Domain.cpp:313-317): the
+ * Clayton-Enquist boundaries, then the solid-fluid coupling. */
 int axs3d_apply_bc_before(axs3d_ctx *ctx);
 /* Domain::recordOutput for station displacement (Domain.cpp:385-409),
  * stored as sample `isample`. */
 int axs3d_record(axs3d_ctx *ctx, int isample);
 /* Domain::applyBC_AfterAssemblingStiff (Domain.cpp:322-326). */
 int axs3d_apply_bc_after(axs3d_ctx *ctx);
+/* Domain::applyBC_AfterComputingAccel (Domain.cpp:330-332): the sponge on
+ * the acceleration that axs3d_stiff_to_accel left, with the terms of this
+ * step's axs3d_apply_bc_before. */
+int axs3d_apply_bc_after_accel(axs3d_ctx *ctx);
 /* Domain::computeStiffToAccel (Domain.cpp:335-345): Nyquist masking and
  * mass division, in place in the stiffness field. */
 int axs3d_stiff_to_accel(axs3d_ctx *ctx);
 /* NewmarkTimeScheme::update on all points (NewmarkTimeScheme.hpp:48-71);
  * expects the stiffness field to hold the acceleration. */
 int axs3d_newmark(axs3d_ctx *ctx);
+/* SymplecticTimeScheme::launch (SymplecticTimeScheme.hpp:51-58): u +=
+ * kappa_0 dt v on all points, once per step after the source. */
+int axs3d_symplectic_launch(axs3d_ctx *ctx);
+/* SymplecticTimeScheme::update of sub-step isub = 1..4 (Symplectic-
+ * TimeScheme.hpp:37-46, 61-70): v += pi_isub dt a, u += kappa_isub dt v
+ * with a the acceleration in the stiffness field, which is then zeroed. */
+int axs3d_symplectic_update(axs3d_ctx *ctx, int isub);
 /* Domain::checkStability (Domain.cpp:574-612): raises when a displacement
  * is non-finite.  tstep is the step number the message names, counted
  * from 1 as NewmarkTimeScheme::solve passes it (tstep + 1 of its loop);
@@ -430,6 +503,13 @@ int axs3d_profile_reset(axs3d_ctx *ctx);
  * candidate size) and the number of sizes timed. */
 int axs3d_autotune_info(const axs3d_ctx *ctx, int *cardinal_occupancy,
                         int *cardinal_blocks, double *trial_ms3);
+/* Every choice the autotuner made at axs3d_finalize (no reference
+ * counterpart): out[0] the chosen cardinal pass (ms), [1] the scratch
+ * size (MiB), [2] scratch sizes timed, [3] the FFT plans' largest pass
+ * radix, [4] radices timed, then (radix, pass ms) for each radix timed;
+ * at most n values, their count in *written. */
+int axs3d_autotune_report(const axs3d_ctx *ctx, double *out, int n,
+                          int *written);
 
 #ifdef __cplusplus
 }

@@ -66,6 +66,17 @@ class OracleDomain(C.Structure):
         ("felem_kind", _p), ("fk3d", _p), ("fk3d_off", _p),
         ("fmass3d_off", _p), ("fmass3d", _p),
         ("sf3d_off", _p), ("sf3d_normal", _p),
+        ("n_clayton_s", C.c_int), ("clayton_s_point", _p),
+        ("clayton_s_nval", _p), ("clayton_s_off", _p),
+        ("clayton_s_data", _p),
+        ("n_clayton_f", C.c_int), ("clayton_f_point", _p),
+        ("clayton_f_nval", _p), ("clayton_f_off", _p),
+        ("clayton_f_data", _p),
+        ("n_sponge_s", C.c_int), ("sponge_s_point", _p),
+        ("sponge_s_nval", _p), ("sponge_s_off", _p), ("sponge_s_data", _p),
+        ("n_sponge_f", C.c_int), ("sponge_f_point", _p),
+        ("sponge_f_nval", _p), ("sponge_f_off", _p), ("sponge_f_data", _p),
+        ("ocean_off", _p), ("ocean_nval", _p), ("ocean_data", _p),
     ]
 
 
@@ -108,6 +119,11 @@ def _setup(L):
     L.oracle_material_stress.argtypes = [C.c_int, _p, _p, _p]
     L.oracle_rtz_to_spz6.argtypes = [_p, _p]
     L.oracle_set_threads.argtypes = [C.c_int]
+    L.oracle_clayton.argtypes = [C.POINTER(OracleDomain), _p, _p, _p, _p]
+    L.oracle_run_symplectic.argtypes = [C.POINTER(OracleDomain), C.c_int,
+                                        C.c_int] + [_p] * 7
+    L.oracle_run_symplectic.restype = C.c_int
+    L.oracle_sponge.argtypes = [C.POINTER(OracleDomain)] + [_p] * 6
     # serial unless a caller asks for threads (set_threads)
     L.oracle_set_threads(1)
     from axisem3d_amd import spectral
@@ -223,6 +239,39 @@ def c2r(c, nr):
 
 
 # ------------------------------------------------------------ domain level
+def rotate_station(seis, wcs, theta, baz, phi, cartesian):
+    """Station::rotateField for the displacement (Station.hpp:79-118),
+    which arrives in (s, phi, z) (SolidElement::displInRTZ is false), with
+    Station::rotate (Station.cpp:41-55): about axis k by angle a, v_i' =
+    v_i cos a + v_j sin a, v_j' = -v_i sin a + v_j cos a, i = k + 1, j = k +
+    2 (mod 3).  seis [sample][station][3]; per station wcs (0 spz, 1 RTZ,
+    2 ENZ, 3 xyz), theta, back azimuth baz and azimuth phi."""
+    out = np.array(seis, dtype=np.float64, copy=True)
+
+    def rot(r, k, a):
+        i, j = (k + 1) % 3, (k + 2) % 3
+        vi, vj = out[:, r, i].copy(), out[:, r, j].copy()
+        out[:, r, i] = vi * np.cos(a) + vj * np.sin(a)
+        out[:, r, j] = -vi * np.sin(a) + vj * np.cos(a)
+
+    for r in range(out.shape[1]):
+        w, bz90 = int(wcs[r]), float(baz[r]) + np.pi * 0.5
+        if cartesian:
+            if w == 2:
+                rot(r, 2, bz90)
+            elif w == 3:
+                rot(r, 2, -float(phi[r]))
+        else:
+            if w == 1:
+                rot(r, 1, float(theta[r]))
+            elif w == 2:
+                rot(r, 1, float(theta[r]))
+                rot(r, 2, bz90)
+            elif w == 3:
+                rot(r, 2, -float(phi[r]))
+    return out
+
+
 class Oracle:
     """FP64 restatement of the time loop on a ``DomainSpec``."""
 
@@ -387,6 +436,34 @@ class Oracle:
             d.sf3d_off = self._k(off)
             d.sf3d_normal = self._k(np.ascontiguousarray(
                 spec.sf3d_normal, dtype=np.float64))
+        # absorbing boundaries (preloop.add_absorbing)
+        for key, cname in (("clayton_solid", "clayton_s"),
+                           ("clayton_fluid", "clayton_f"),
+                           ("sponge_solid", "sponge_s"),
+                           ("sponge_fluid", "sponge_f")):
+            b = getattr(spec, key, None)
+            if b is None or len(b["point"]) == 0:
+                continue
+            setattr(d, "n_" + cname, len(b["point"]))
+            setattr(d, cname + "_point", self._k(np.ascontiguousarray(
+                b["point"], dtype=np.int32)))
+            setattr(d, cname + "_nval", self._k(np.ascontiguousarray(
+                b["nval"], dtype=np.int32)))
+            setattr(d, cname + "_off", self._k(np.ascontiguousarray(
+                b["off"], dtype=np.int64)))
+            setattr(d, cname + "_data", self._k(np.ascontiguousarray(
+                b["data"], dtype=np.float64)))
+        # ocean load (preloop.add_ocean_load)
+        oc = getattr(spec, "ocean_load", None)
+        if oc is not None and len(oc["point"]) > 0:
+            off = np.full(len(spec.solid_nr), -1, dtype=np.int64)
+            nv = np.zeros(len(spec.solid_nr), dtype=np.int32)
+            off[oc["point"]] = oc["off"]
+            nv[oc["point"]] = oc["nval"]
+            d.ocean_off = self._k(off)
+            d.ocean_nval = self._k(nv)
+            d.ocean_data = self._k(np.ascontiguousarray(oc["data"],
+                                                        dtype=np.float64))
         d.dt = float(spec.dt)
         self.d = d
         self._att_state = None
@@ -415,7 +492,15 @@ class Oracle:
     def run(self, nsteps, record=True):
         nrec = self.d.n_rec
         seis = np.zeros((nsteps, max(nrec, 1), 3)) if record else None
-        if self.fnmodes > 0:
+        if getattr(self.spec, "time_scheme", "newmark") == "symplectic":
+            f = self.fnmodes > 0
+            rc = lib().oracle_run_symplectic(
+                C.byref(self.d), self.step, nsteps, self.displ.ctypes.data,
+                self.veloc.ctypes.data, self.stiff.ctypes.data,
+                self.fdispl.ctypes.data if f else None,
+                self.fveloc.ctypes.data if f else None,
+                self.fstiff.ctypes.data if f else None, _ptr(seis))
+        elif self.fnmodes > 0:
             rc = lib().oracle_run_coupled(
                 C.byref(self.d), self.step, nsteps, self.displ.ctypes.data,
                 self.veloc.ctypes.data, self.accel.ctypes.data,
@@ -492,6 +577,33 @@ class Oracle:
         fstiff = np.ascontiguousarray(fstiff, dtype=np.complex128).copy()
         lib().oracle_fluid_stiff_to_accel(C.byref(self.d), fstiff.ctypes.data)
         return fstiff
+
+    def clayton(self, veloc, stiff, fveloc=None, fstiff=None):
+        """Clayton-Enquist boundaries on copies of (solid stiff, fluid
+        stiff) from the velocities."""
+        veloc = np.ascontiguousarray(veloc, dtype=np.complex128)
+        stiff = np.ascontiguousarray(stiff, dtype=np.complex128).copy()
+        fv = None if fveloc is None else np.ascontiguousarray(
+            fveloc, dtype=np.complex128)
+        fs = None if fstiff is None else np.ascontiguousarray(
+            fstiff, dtype=np.complex128).copy()
+        lib().oracle_clayton(C.byref(self.d), veloc.ctypes.data,
+                             stiff.ctypes.data, _ptr(fv), _ptr(fs))
+        return stiff, fs
+
+    def sponge(self, displ, veloc, accel, fdispl=None, fveloc=None,
+               faccel=None):
+        """Kosloff sponge on copies of (solid accel, fluid accel)."""
+        c = lambda a: None if a is None else np.ascontiguousarray(
+            a, dtype=np.complex128)
+        displ, veloc = c(displ), c(veloc)
+        accel = c(accel).copy()
+        fd, fv = c(fdispl), c(fveloc)
+        fa = None if faccel is None else c(faccel).copy()
+        lib().oracle_sponge(C.byref(self.d), displ.ctypes.data,
+                            veloc.ctypes.data, accel.ctypes.data, _ptr(fd),
+                            _ptr(fv), _ptr(fa))
+        return accel, fa
 
     def newmark(self, stiff, displ, veloc, accel):
         """Returns updated (stiff, displ, veloc, accel) copies."""

@@ -839,6 +839,46 @@ void oracle_stiff_to_accel(const oracle_domain *d, cplx *stiff) {
         if (nr % 2 == 0)
             for (int c = 0; c < 3; c++)
                 s[(nu1 - 1) * 3 + c] = creal(s[(nu1 - 1) * 3 + c]);
+        if (d->ocean_off && d->ocean_off[ip] >= 0) {
+            const double *o = d->ocean_data + d->ocean_off[ip] * 4;
+            if (d->ocean_nval[ip] == 1) {
+                /* MassOceanLoad1D::computeAccel (MassOceanLoad1D.hpp:
+                 * 48-62): (s, z) by the 2x2 factors, phi by 1/m */
+                for (int a = 0; a < nu1; a++) {
+                    const cplx fs = s[a * 3], fz = s[a * 3 + 2];
+                    s[a * 3] = o[0] * fs + o[2] * fz;
+                    s[a * 3 + 2] = o[2] * fs + o[1] * fz;
+                    s[a * 3 + 1] *= o[3];
+                }
+                continue;
+            }
+            /* MassOceanLoad3D::computeAccel (MassOceanLoad3D.cpp:41-56):
+             * F = C2R(stiff) per component, a = F / m - (F . k) k per
+             * slice, stiff = R2C(a) */
+            cplx *col = malloc(sizeof(cplx) * nu1);
+            double *f = malloc(sizeof(double) * nr * 3);
+            for (int c = 0; c < 3; c++) {
+                for (int a = 0; a < nu1; a++) col[a] = s[a * 3 + c];
+                oracle_c2r(col, nr, f + (size_t)c * nr);
+            }
+            for (int k = 0; k < nr; k++) {
+                const double *q = o + (size_t)k * 4;
+                double fk[3], dot = 0.;
+                for (int c = 0; c < 3; c++) {
+                    fk[c] = f[(size_t)c * nr + k];
+                    dot += fk[c] * q[1 + c];
+                }
+                for (int c = 0; c < 3; c++)
+                    f[(size_t)c * nr + k] = q[0] * fk[c] - dot * q[1 + c];
+            }
+            for (int c = 0; c < 3; c++) {
+                oracle_r2c(f + (size_t)c * nr, nr, col);
+                for (int a = 0; a < nu1; a++) s[a * 3 + c] = col[a];
+            }
+            free(col);
+            free(f);
+            continue;
+        }
         if (d->mass3d_off && d->mass3d_off[ip] >= 0) {
             /* Mass3D::computeAccel (src/core/point/mass/Mass3D.cpp:56-68):
              * per component C2R, times 1 / m slice by slice, R2C */
@@ -871,6 +911,134 @@ void oracle_newmark(int64_t nmodes, double dt, cplx *stiff, cplx *displ,
         accel[k] = stiff[k];
         displ[k] += dt * veloc[k] + half_dt_dt * accel[k];
         stiff[k] = 0.;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Absorbing boundaries (src/core/boundary/ABC)                        */
+/* ------------------------------------------------------------------ */
+/* One Fourier column (nu1 modes, stride `st` complex) of a point field to
+ * its Nr slices (SolverFFTW::computeC2R) and back (computeR2C). */
+static void col_c2r(const cplx *x, int st, int nr, cplx *tmp, double *r) {
+    const int nu1 = nr / 2 + 1;
+    for (int a = 0; a < nu1; a++) tmp[a] = x[(int64_t)a * st];
+    oracle_c2r(tmp, nr, r);
+}
+
+/* AbsorbingBoundary::applyClayton (AbsorbingBoundary.hpp:48-56), solid
+ * entries then fluid entries, each in list order:
+ *  ClaytonSolid1D::apply (ClaytonSolid1D.cpp:16-29): s -= c0 v_s + c2 v_z,
+ *    phi -= c3 v_phi, z -= c2 v_s + c1 v_z;
+ *  ClaytonSolid3D::apply (ClaytonSolid3D.cpp:43-67): V = C2R(v), a = rsa V
+ *    + (V . k) k per slice, stiff -= R2C(a);
+ *  ClaytonFluid1D::apply (ClaytonFluid1D.cpp:15-22): stiff -= v c;
+ *  ClaytonFluid3D::apply (ClaytonFluid3D.cpp:33-51): stiff -= R2C(C2R(v)
+ *    c). */
+void oracle_clayton(const oracle_domain *d, const cplx *veloc, cplx *stiff,
+                    const cplx *fveloc, cplx *fstiff) {
+    for (int i = 0; i < d->n_clayton_s; i++) {
+        const int ip = d->clayton_s_point[i];
+        const int nr = d->point_nr[ip], nu1 = nr / 2 + 1;
+        const double *c = d->clayton_s_data + 4 * d->clayton_s_off[i];
+        const cplx *v = veloc + d->offsets[ip] * 3;
+        cplx *s = stiff + d->offsets[ip] * 3;
+        if (d->clayton_s_nval[i] == 1) {
+            for (int a = 0; a < nu1; a++) {
+                const cplx vs = v[a * 3], vp = v[a * 3 + 1], vz = v[a * 3 + 2];
+                s[a * 3 + 0] -= c[0] * vs + c[2] * vz;
+                s[a * 3 + 2] -= c[2] * vs + c[1] * vz;
+                s[a * 3 + 1] -= c[3] * vp;
+            }
+            continue;
+        }
+        cplx *tmp = malloc(sizeof(cplx) * nu1);
+        double *V = malloc(sizeof(double) * nr * 3);
+        double *A = malloc(sizeof(double) * nr * 3);
+        for (int q = 0; q < 3; q++) col_c2r(v + q, 3, nr, tmp, V + q * nr);
+        for (int k = 0; k < nr; k++) {
+            const double *ck = c + 4 * k;
+            const double dot = V[k] * ck[1] + V[nr + k] * ck[2] +
+                               V[2 * nr + k] * ck[3];
+            for (int q = 0; q < 3; q++)
+                A[q * nr + k] = ck[0] * V[q * nr + k] + dot * ck[1 + q];
+        }
+        for (int q = 0; q < 3; q++) {
+            oracle_r2c(A + q * nr, nr, tmp);
+            for (int a = 0; a < nu1; a++) s[a * 3 + q] -= tmp[a];
+        }
+        free(tmp);
+        free(V);
+        free(A);
+    }
+    for (int i = 0; i < d->n_clayton_f; i++) {
+        const int ip = d->clayton_f_point[i];
+        const int nr = d->fpoint_nr[ip], nu1 = nr / 2 + 1;
+        const double *c = d->clayton_f_data + d->clayton_f_off[i];
+        const cplx *v = fveloc + d->foffsets[ip];
+        cplx *s = fstiff + d->foffsets[ip];
+        if (d->clayton_f_nval[i] == 1) {
+            for (int a = 0; a < nu1; a++) s[a] -= v[a] * c[0];
+            continue;
+        }
+        cplx *tmp = malloc(sizeof(cplx) * nu1);
+        double *V = malloc(sizeof(double) * nr);
+        col_c2r(v, 1, nr, tmp, V);
+        for (int k = 0; k < nr; k++) V[k] *= c[k];
+        oracle_r2c(V, nr, tmp);
+        for (int a = 0; a < nu1; a++) s[a] -= tmp[a];
+        free(tmp);
+        free(V);
+    }
+}
+
+/* Sponge::apply (Sponge.hpp:74-96) on one point of nc components, after
+ * the acceleration is formed: 1D a -= 2 gamma v + gamma^2 u; 3D a =
+ * R2C(C2R(a) - 2 gamma C2R(v) - gamma^2 C2R(u)) slice by slice. */
+static void sponge_point(int nc, int nr, int nval, const double *g,
+                         const cplx *u, const cplx *v, cplx *acc) {
+    const int nu1 = nr / 2 + 1;
+    if (nval == 1) {
+        const double g2 = 2. * g[0], gg = g[0] * g[0];
+        for (int k = 0; k < nu1 * nc; k++) acc[k] -= g2 * v[k] + gg * u[k];
+        return;
+    }
+    cplx *tmp = malloc(sizeof(cplx) * nu1);
+    double *A = malloc(sizeof(double) * nr);
+    double *U = malloc(sizeof(double) * nr);
+    double *V = malloc(sizeof(double) * nr);
+    for (int q = 0; q < nc; q++) {
+        col_c2r(acc + q, nc, nr, tmp, A);
+        col_c2r(u + q, nc, nr, tmp, U);
+        col_c2r(v + q, nc, nr, tmp, V);
+        for (int k = 0; k < nr; k++)
+            A[k] -= 2. * g[k] * V[k] + g[k] * g[k] * U[k];
+        oracle_r2c(A, nr, tmp);
+        for (int a = 0; a < nu1; a++) acc[(int64_t)a * nc + q] = tmp[a];
+    }
+    free(tmp);
+    free(A);
+    free(U);
+    free(V);
+}
+
+/* AbsorbingBoundary::applySponge (AbsorbingBoundary.hpp:59-66): solid
+ * sponges, then fluid sponges. */
+void oracle_sponge(const oracle_domain *d, const cplx *displ,
+                   const cplx *veloc, cplx *accel, const cplx *fdispl,
+                   const cplx *fveloc, cplx *faccel) {
+    for (int i = 0; i < d->n_sponge_s; i++) {
+        const int ip = d->sponge_s_point[i];
+        const int64_t o = d->offsets[ip] * 3;
+        sponge_point(3, d->point_nr[ip], d->sponge_s_nval[i],
+                     d->sponge_s_data + d->sponge_s_off[i], displ + o,
+                     veloc + o, accel + o);
+    }
+    for (int i = 0; i < d->n_sponge_f; i++) {
+        const int ip = d->sponge_f_point[i];
+        const int64_t o = d->foffsets[ip];
+        sponge_point(1, d->fpoint_nr[ip], d->sponge_f_nval[i],
+                     d->sponge_f_data + d->sponge_f_off[i], fdispl + o,
+                     fveloc + o, faccel + o);
     }
 }
 
@@ -1112,6 +1280,7 @@ int oracle_run_coupled(const oracle_domain *d, int step_begin, int nsteps,
         solid_stiffness(d, displ, stiff);
         for (int e = 0; e < d->n_felem; e++)
             oracle_fluid_element_stiff(d, e, fdispl, fstiff);
+        oracle_clayton(d, veloc, stiff, fveloc, fstiff);
         oracle_coupling(d, displ, stiff, fstiff);
         if (seismo && d->n_rec > 0)
             oracle_record(d, displ, seismo + (int64_t)(ts - step_begin) *
@@ -1123,6 +1292,7 @@ int oracle_run_coupled(const oracle_domain *d, int step_begin, int nsteps,
         oracle_fluid_bc(d, fstiff);
         oracle_stiff_to_accel(d, stiff);
         oracle_fluid_stiff_to_accel(d, fstiff);
+        oracle_sponge(d, displ, veloc, stiff, fdispl, fveloc, fstiff);
         oracle_newmark(d->offsets[d->n_points], d->dt, stiff, displ, veloc,
                        accel);
         /* one component per mode: the 3-component routine on nmf / 3
@@ -1144,6 +1314,7 @@ int oracle_run(const oracle_domain *d, int step_begin, int nsteps,
     for (int ts = step_begin; ts < step_begin + nsteps; ts++) {
         if (d->stf) oracle_apply_moment_source(d, d->stf[ts], stiff);
         solid_stiffness(d, displ, stiff);
+        oracle_clayton(d, veloc, stiff, NULL, NULL);
         if (seismo && d->n_rec > 0)
             oracle_record(d, displ, seismo + (int64_t)(ts - step_begin) *
                                                  d->n_rec * 3);
@@ -1152,8 +1323,67 @@ int oracle_run(const oracle_domain *d, int step_begin, int nsteps,
                 return 1;
         oracle_axial_solid(d, stiff);
         oracle_stiff_to_accel(d, stiff);
+        oracle_sponge(d, displ, veloc, stiff, NULL, NULL, NULL);
         oracle_newmark(d->offsets[d->n_points], d->dt, stiff, displ, veloc,
                        accel);
+    }
+    return 0;
+}
+
+/* SymplecticTimeScheme::solve (SymplecticTimeScheme.cpp:30-103) with
+ * SymplecticTimeScheme::launch/update (SymplecticTimeScheme.hpp:37-70):
+ * per step the receivers, the source, u += kappa_0 dt v, then four
+ * sub-steps of stiffness (solid, fluid), Clayton, coupling, axis and
+ * fluid surface, stiff to accel, sponge, v += pi_k dt a, u += kappa_k dt v.
+ * The coefficients are the position-extended Forest-Ruth-like ones of
+ * Omelyan, Mryglod and Folk (2002), SymplecticTimeScheme.hpp:81-100.
+ * fdispl .. fstiff may be NULL without fluid points. */
+int oracle_run_symplectic(const oracle_domain *d, int step_begin,
+                          int nsteps, cplx *displ, cplx *veloc, cplx *stiff,
+                          cplx *fdispl, cplx *fveloc, cplx *fstiff,
+                          double *seismo) {
+    const double A = 0.1786178958448091, B = -0.2123418310626054,
+                 G = -0.06626458266981849;
+    const double kap[5] = {A, G, 1. - 2. * (G + A), G, A};
+    const double pi_[5] = {0., .5 - B, B, B, .5 - B};
+    const int64_t nm = d->offsets[d->n_points] * 3;
+    const int64_t nmf = d->n_fpoints > 0 ? d->foffsets[d->n_fpoints] : 0;
+    for (int ts = step_begin; ts < step_begin + nsteps; ts++) {
+        if (seismo && d->n_rec > 0)
+            oracle_record(d, displ, seismo + (int64_t)(ts - step_begin) *
+                                                 d->n_rec * 3);
+        if (d->stf) oracle_apply_moment_source(d, d->stf[ts], stiff);
+        for (int64_t k = 0; k < nm; k++) displ[k] += kap[0] * d->dt * veloc[k];
+        for (int64_t k = 0; k < nmf; k++)
+            fdispl[k] += kap[0] * d->dt * fveloc[k];
+        for (int isub = 1; isub <= 4; isub++) {
+            solid_stiffness(d, displ, stiff);
+            for (int e = 0; e < d->n_felem; e++)
+                oracle_fluid_element_stiff(d, e, fdispl, fstiff);
+            oracle_clayton(d, veloc, stiff, fveloc, fstiff);
+            if (nmf > 0) oracle_coupling(d, displ, stiff, fstiff);
+            if (isub == 1)
+                for (int64_t k = 0; k < nm; k++)
+                    if (!isfinite(creal(displ[k])) ||
+                        !isfinite(cimag(displ[k])))
+                        return 1;
+            oracle_axial_solid(d, stiff);
+            if (nmf > 0) oracle_fluid_bc(d, fstiff);
+            oracle_stiff_to_accel(d, stiff);
+            if (nmf > 0) oracle_fluid_stiff_to_accel(d, fstiff);
+            oracle_sponge(d, displ, veloc, stiff, fdispl, fveloc, fstiff);
+            const double pdt = pi_[isub] * d->dt, kdt = kap[isub] * d->dt;
+            for (int64_t k = 0; k < nm; k++) {
+                veloc[k] += pdt * stiff[k];
+                displ[k] += kdt * veloc[k];
+                stiff[k] = 0.;
+            }
+            for (int64_t k = 0; k < nmf; k++) {
+                fveloc[k] += pdt * fstiff[k];
+                fdispl[k] += kdt * fveloc[k];
+                fstiff[k] = 0.;
+            }
+        }
     }
     return 0;
 }

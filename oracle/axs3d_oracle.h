@@ -96,6 +96,41 @@ typedef struct {
      * Nr normals (s, phi, z) in sf3d_normal, or -1 for the 1D normal */
     const int64_t *sf3d_off;
     const double *sf3d_normal;
+    /* absorbing boundaries, one list per kind: entry i acts on point
+     * *_point[i] with nval[i] values (1: the 1D form, Nr: one per slice,
+     * the 3D form) starting at value *_off[i] of *_data.
+     *  clayton_s (ClaytonSolid1D/3D): 4 numbers per value, 1D the
+     *    constructor's RSA cos^2 + RPA sin^2, RSA sin^2 + RPA cos^2,
+     *    (RPA - RSA) cos sin, RSA (ClaytonSolid1D.hpp:22-33); 3D per slice
+     *    rsa and k = sqrt((rpa - rsa) area) n (ClaytonSolid3D.hpp:20-27);
+     *  clayton_f (ClaytonFluid1D/3D): area / (rho vp);
+     *  sponge_s, sponge_f (Sponge): gamma. */
+    int n_clayton_s;
+    const int32_t *clayton_s_point, *clayton_s_nval;
+    const int64_t *clayton_s_off;
+    const double *clayton_s_data;
+    int n_clayton_f;
+    const int32_t *clayton_f_point, *clayton_f_nval;
+    const int64_t *clayton_f_off;
+    const double *clayton_f_data;
+    int n_sponge_s;
+    const int32_t *sponge_s_point, *sponge_s_nval;
+    const int64_t *sponge_s_off;
+    const double *sponge_s_data;
+    int n_sponge_f;
+    const int32_t *sponge_f_point, *sponge_f_nval;
+    const int64_t *sponge_f_off;
+    const double *sponge_f_data;
+    /* ocean load on solid points (NULL: none): per solid point the offset
+     * of its values in ocean_data, or -1, and their count in ocean_nval,
+     * 4 numbers per value.  1D (1 value): the MassOceanLoad1D constructor's
+     * 1/m cos^2 t + 1/(m + m0) sin^2 t, 1/m sin^2 t + 1/(m + m0) cos^2 t,
+     * (1/(m + m0) - 1/m) cos t sin t and 1/m (MassOceanLoad1D.hpp:20-30);
+     * 3D (Nr values, one per slice): 1/m and k = sqrt(m0 / (m (m + m0))) n
+     * (MassOceanLoad3D.cpp:13-20).  The point's own mass is not used. */
+    const int64_t *ocean_off;
+    const int32_t *ocean_nval;
+    const double *ocean_data;
 } oracle_domain;
 
 void oracle_set_gmat(const double *g_gll, const double *g_glj);
@@ -142,9 +177,24 @@ void oracle_newmark(int64_t nmodes, double dt, double complex *stiff,
                     double complex *accel);
 void oracle_record(const oracle_domain *d, const double complex *displ,
                    double *out);
+/* Clayton-Enquist boundaries on the solid and fluid stiffness
+ * (AbsorbingBoundary::applyClayton), and the Kosloff sponge on the solid
+ * and fluid accelerations (AbsorbingBoundary::applySponge). */
+void oracle_clayton(const oracle_domain *d, const double complex *veloc,
+                    double complex *stiff, const double complex *fveloc,
+                    double complex *fstiff);
+void oracle_sponge(const oracle_domain *d, const double complex *displ,
+                   const double complex *veloc, double complex *accel,
+                   const double complex *fdispl,
+                   const double complex *fveloc, double complex *faccel);
 int oracle_run(const oracle_domain *d, int step_begin, int nsteps,
                double complex *displ, double complex *veloc,
                double complex *accel, double complex *stiff, double *seismo);
+int oracle_run_symplectic(const oracle_domain *d, int step_begin,
+                          int nsteps, double complex *displ,
+                          double complex *veloc, double complex *stiff,
+                          double complex *fdispl, double complex *fveloc,
+                          double complex *fstiff, double *seismo);
 int64_t *oracle_make_offsets(int n, const int32_t *nr);
 void oracle_r2c(const double *r, int nr, double complex *c);
 void oracle_c2r(const double complex *c, int nr, double *r);

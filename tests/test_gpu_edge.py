@@ -31,7 +31,7 @@ def _small_nr_spec(nsteps, cardinal):
     return spec
 
 
-@pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("fp32", 1e-4)])
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 1e-4)])
 @pytest.mark.parametrize("cardinal", [False, True])
 def test_smallest_nr_matches_oracle(prec, tol, cardinal):
     from axisem3d_amd.solver import GpuSolver
@@ -50,7 +50,7 @@ def test_smallest_nr_matches_oracle(prec, tol, cardinal):
     assert err < tol
 
 
-@pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("fp32", 1e-4)])
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 1e-4)])
 def test_free_oscillation_without_source_or_receivers(prec, tol):
     from axisem3d_amd.solver import GpuSolver
     spec = _small_nr_spec(200, True)

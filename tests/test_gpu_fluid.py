@@ -4,8 +4,8 @@ oracle through the C ABI: a fluid layer inside a solid (two coupled
 interfaces, fluid points on the axis) and an ocean on top of a solid (free
 surface).
 
-Tolerances: FP64 mode agrees with the oracle to rounding (relative 1e-11
-per stage, 1e-10 seismogram L2); FP32+FP64 mode relative 1e-5 per stage
+Tolerances: FP64 mode agrees with the oracle to rounding (relative 1e-13
+per stage, 1e-12 seismogram L2); FP32+FP64 mode relative 1e-5 per stage
 and 1e-4 seismogram L2."""
 import math
 
@@ -53,7 +53,7 @@ def _solver(spec, prec):
     return GpuSolver(spec, prec)
 
 
-@pytest.mark.parametrize("prec,tol", [("fp64", 1e-11), ("fp32", 1e-5)])
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-13), ("fp32", 1e-5)])
 @pytest.mark.parametrize("layer", ["middle", "ocean"])
 def test_fluid_stiffness_and_coupling_match_oracle(prec, tol, layer):
     spec = _spec(layer)
@@ -85,7 +85,7 @@ def test_fluid_stiffness_and_coupling_match_oracle(prec, tol, layer):
     g.close()
 
 
-@pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("fp32", 1e-4)])
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 1e-4)])
 @pytest.mark.parametrize("layer", ["middle", "ocean"])
 def test_fluid_seismograms_match_oracle(prec, tol, layer):
     nsteps = 300
@@ -112,7 +112,7 @@ def _spec3d(layer, nsteps=300):
     return preloop.make_fluid3d(sp, np.arange(0, nfe, 2), amp=0.3, seed=5)
 
 
-@pytest.mark.parametrize("prec,tol", [("fp64", 1e-11), ("fp32", 1e-5)])
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-13), ("fp32", 1e-5)])
 @pytest.mark.parametrize("layer", ["middle", "ocean"])
 def test_fluid3d_stiffness_matches_oracle(prec, tol, layer):
     spec = _spec3d(layer)
@@ -132,7 +132,7 @@ def test_fluid3d_stiffness_matches_oracle(prec, tol, layer):
     g.close()
 
 
-@pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("fp32", 1e-4)])
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 1e-4)])
 def test_fluid3d_seismograms_match_oracle(prec, tol):
     nsteps = 300
     spec = _spec3d("middle", nsteps)
@@ -159,7 +159,7 @@ def _spec_c3d(layer, nsteps=300):
     return preloop.make_coupling3d(sp, amp=0.15, tilt=0.15, seed=8)
 
 
-@pytest.mark.parametrize("prec,tol", [("fp64", 1e-11), ("fp32", 1e-5)])
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-13), ("fp32", 1e-5)])
 @pytest.mark.parametrize("layer", ["middle", "ocean"])
 def test_coupling3d_and_fluid_mass3d_stages_match_oracle(prec, tol, layer):
     spec = _spec_c3d(layer)
@@ -189,7 +189,7 @@ def test_coupling3d_and_fluid_mass3d_stages_match_oracle(prec, tol, layer):
     g.close()
 
 
-@pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("fp32", 1e-4)])
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 1e-4)])
 @pytest.mark.parametrize("layer", ["middle", "ocean"])
 def test_coupling3d_seismograms_match_oracle(prec, tol, layer):
     nsteps = 300

@@ -2,7 +2,7 @@
 FP64 oracle, through the C ABI: the stage form (axs3d_stiff_to_accel) on a
 random stiffness, then whole runs whose seismograms cross the 3D-mass
 points.  Tolerances as for the other point phases: 1e-12 per stage in
-FP64, 1e-5 in FP32+FP64; seismograms 1e-10 and 1e-4."""
+FP64, 1e-5 in FP32+FP64; seismograms 1e-12 and 1e-4."""
 import numpy as np
 import pytest
 
@@ -44,7 +44,7 @@ def test_mass3d_stage_matches_oracle(prec):
     assert _rel(g.get_field("stiff"), ref) < TOL_STAGE[prec]
 
 
-@pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("fp32", 1e-4)])
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 1e-4)])
 def test_mass3d_seismograms_match_oracle(prec, tol):
     from axisem3d_amd.solver import GpuSolver
     nsteps = 300

@@ -2,7 +2,7 @@
 oracle, through the C ABI (``include/axs3d.h``).
 
 Tolerances: FP64 mode agrees with the oracle to rounding (relative
-1e-12 per stage, relative seismogram L2 1e-10 after hundreds of steps);
+1e-12 per stage, relative seismogram L2 1e-12 after hundreds of steps);
 FP32+FP64 mode is bounded by the north-star seismogram bar, relative L2
 <= 1e-4 against FP64, and 1e-5 per stage.
 """
@@ -100,7 +100,7 @@ def test_point_phases_match_oracle(prec):
     assert np.all(g.get_field("stiff") == 0)
 
 
-@pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("fp32", 1e-4)])
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 1e-4)])
 def test_seismograms_match_oracle(prec, tol):
     nsteps = 400
     spec = _varnr_spec(nsteps=nsteps)
@@ -136,7 +136,7 @@ def test_cartesian_8000_elements_fp32_vs_fp64():
     ref = o.run(60)
     g = _solver(spec, "fp64")
     g.run(60)
-    assert _rel(g.get_field("displ"), o.displ) < 1e-10
+    assert _rel(g.get_field("displ"), o.displ) < 1e-12
 
 
 def test_stability_flag_raises():

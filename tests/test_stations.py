@@ -106,3 +106,55 @@ def test_channel_centric_files(tmp_path):
     assert vals.shape == (4, 2)
     ref = data[:, :, 2].astype(np.float32).astype(np.float64)
     assert np.allclose(vals, ref, rtol=1e-7, atol=0)
+
+
+# ---- station frames: the oracle's restatement of Station::rotateField,
+# pinned by geometry (a station at colatitude theta from the source on a
+# spherical mesh; back azimuth baz = direction from the station to the
+# source, clockwise from north)
+def _one(v):
+    return np.asarray(v, dtype=np.float64).reshape(1, 1, 3)
+
+
+def test_rtz_maps_the_local_vertical_and_the_great_circle_direction():
+    import oracle
+    th = 0.7
+    up = _one([np.sin(th), 0.0, np.cos(th)])            # radial outward
+    away = _one([np.cos(th), 0.0, -np.sin(th)])         # increasing theta
+    phi_hat = _one([0.0, 1.0, 0.0])
+    rot = lambda v: oracle.rotate_station(v, [1], [th], [0.0], [0.0], False)
+    assert np.allclose(rot(up), _one([0, 0, 1]), atol=1e-15)
+    assert np.allclose(rot(away), _one([1, 0, 0]), atol=1e-15)
+    assert np.allclose(rot(phi_hat), _one([0, 1, 0]), atol=1e-15)
+
+
+def test_enz_points_north_towards_the_source():
+    import oracle
+    th = 0.4
+    away = _one([np.cos(th), 0.0, -np.sin(th)])
+    # back azimuth 0: the source lies due north, so "away" is south
+    enz = oracle.rotate_station(away, [2], [th], [0.0], [0.0], False)
+    assert np.allclose(enz, _one([0, -1, 0]), atol=1e-15)
+    # back azimuth 90 deg: the source lies due east, "away" is west
+    enz = oracle.rotate_station(away, [2], [th], [np.pi / 2], [0.0], False)
+    assert np.allclose(enz, _one([-1, 0, 0]), atol=1e-15)
+
+
+def test_xyz_undoes_the_azimuth_and_spz_is_the_identity():
+    import oracle
+    phi = 1.1
+    # the Cartesian x axis seen from azimuth phi: (cos phi, -sin phi, 0)
+    x_spz = _one([np.cos(phi), -np.sin(phi), 0.0])
+    for cart in (True, False):
+        xyz = oracle.rotate_station(x_spz, [3], [0.3], [0.2], [phi], cart)
+        assert np.allclose(xyz, _one([1, 0, 0]), atol=1e-15)
+    v = np.random.default_rng(0).standard_normal((4, 2, 3))
+    assert np.array_equal(oracle.rotate_station(v, [0, 0], [1, 2], [3, 4],
+                                                [5, 6], False), v)
+    # Cartesian meshes: RTZ is SPZ
+    assert np.array_equal(oracle.rotate_station(v, [1, 1], [1, 2], [3, 4],
+                                                [5, 6], True), v)
+    # every frame keeps lengths
+    w = oracle.rotate_station(v, [2, 3], [0.3, 1.2], [0.1, 2.0],
+                              [0.5, 4.0], False)
+    assert np.allclose(np.linalg.norm(w, axis=2), np.linalg.norm(v, axis=2))
