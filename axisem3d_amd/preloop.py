@@ -26,6 +26,7 @@ from __future__ import annotations
 
 import dataclasses
 import math
+import os
 from typing import Callable, Optional
 
 import numpy as np
@@ -160,10 +161,177 @@ class StructuredMesh:
     def npoint_grid(self):
         return (4 * self.nx + 1) * (4 * self.nz + 1)
 
+    def corner_nodes(self):
+        """(E, 4) node ids of the element corners (nodes 0..3)."""
+        ix, iz = self.element_ix_iz()
+        na = self.nx + 1
+        return np.stack([iz * na + ix, iz * na + ix + 1,
+                         (iz + 1) * na + ix + 1, (iz + 1) * na + ix], axis=1)
+
+    @property
+    def nnodes(self):
+        return (self.nx + 1) * (self.nz + 1)
+
+    def boundary_edges(self, key):
+        """(element, edge) of the mesh side `key` (LEFT, RIGHT, BOTTOM,
+        TOP)."""
+        ix, iz = self.element_ix_iz()
+        sel = {"LEFT": ix == 0, "RIGHT": ix == self.nx - 1,
+               "BOTTOM": iz == 0, "TOP": iz == self.nz - 1}[key]
+        edge = {"LEFT": 3, "RIGHT": 1, "BOTTOM": 0, "TOP": 2}[key]
+        return [(int(m), edge) for m in np.nonzero(sel)[0]]
+
+    def interface_edges(self, fluid):
+        """(element, edge) of every element edge whose neighbour across it
+        has the other medium (fluid: bool per element)."""
+        ix, iz = self.element_ix_iz()
+        cell = -np.ones((self.nx, self.nz), dtype=np.int64)
+        cell[ix, iz] = np.arange(self.nelem)
+        out = []
+        for e in range(self.nelem):
+            for edge, (dx, dz) in ((0, (0, -1)), (1, (1, 0)), (2, (0, 1)),
+                                   (3, (-1, 0))):
+                jx, jz = ix[e] + dx, iz[e] + dz
+                if 0 <= jx < self.nx and 0 <= jz < self.nz and \
+                        fluid[cell[jx, jz]] != fluid[e]:
+                    out.append((e, edge))
+        return out
+
     def edge_points(self, edge):
         """Element point indices on an edge (``vicinity.hpp:52-56``)."""
         return {0: [0, 5, 10, 15, 20], 1: [20, 21, 22, 23, 24],
                 2: [24, 19, 14, 9, 4], 3: [4, 3, 2, 1, 0]}[edge]
+
+
+# element-local points of the corners (nodes 0..3) and, per edge u -> v
+# (edge k joins node k and node k+1 mod 4), its three interior points in
+# order from u
+_CORNER_IPNT = (0, 20, 24, 4)
+_EDGE_INNER = {0: (5, 10, 15), 1: (21, 22, 23), 2: (19, 14, 9),
+               3: (3, 2, 1)}
+_INNER_IPNT = tuple(i * NPED + j for i in range(1, 4) for j in range(1, 4))
+
+
+class QuadMesh:
+    """An unstructured mesh of 4-node quads with linear geometry, as an
+    AxiSEM3D Exodus mesh in Cartesian mode gives it (ExodusMesh.cpp:
+    205-619; MappingLinear.hpp): corner nodes and coordinates, axial
+    elements (a side in the LEFT side set, which must be element side 3,
+    xi = -1), side sets for the boundaries, the solid-fluid interface and
+    the medium per element.  GLL points are numbered through the
+    connectivity (corner nodes, edges by their two nodes, interiors), which
+    is exact for a conforming mesh."""
+    spherical = False
+
+    def __init__(self, coords, connectivity, side_sets, fluid=None):
+        self.coords = np.asarray(coords, dtype=np.float64)
+        self.connectivity = np.asarray(connectivity, dtype=np.int64)
+        self.side_sets = side_sets
+        E = len(self.connectivity)
+        self.fluid = (np.zeros(E, bool) if fluid is None else
+                      np.asarray(fluid, dtype=bool))
+        self._axial = np.zeros(E, dtype=bool)
+        for e, side in side_sets.get("x0", {}).items():
+            if side != 3:
+                raise ValueError("QuadMesh: an axial element must have the "
+                                 "axis on its side 3 (xi = -1)")
+            self._axial[e] = True
+        self._number_points()
+
+    @classmethod
+    def from_exodus(cls, ex):
+        if not ex.cartesian or np.any(ex.geometry_type != 1):
+            raise NotImplementedError(
+                "QuadMesh: Cartesian meshes of linear elements only (the "
+                "spherical and semi-spherical mappings are not built)")
+        return cls(ex.coords, ex.connectivity, ex.side_sets, ex.fluid)
+
+    @property
+    def nelem(self):
+        return len(self.connectivity)
+
+    def is_axial(self):
+        return self._axial
+
+    def corner_nodes(self):
+        return self.connectivity
+
+    @property
+    def nnodes(self):
+        return len(self.coords)
+
+    def nodal_sz(self):
+        return self.coords[self.connectivity].transpose(0, 2, 1).copy()
+
+    def map_points(self, xieta, axial):
+        """MappingLinear: bilinear shape functions on the 4 nodes; sz (E, 2,
+        25) and J (E, 25, 2, 2) = d(s,z)/d(xi,eta)."""
+        xi = xieta[0][None, :]
+        eta = xieta[1][None, :]
+        nsz = self.nodal_sz()
+        xip, xim, etp, etm = 1 + xi, 1 - xi, 1 + eta, 1 - eta
+        shp = np.stack([xim * etm, xip * etm, xip * etp, xim * etp],
+                       axis=1) * 0.25
+        sz = np.einsum('eij,xjp->eip', nsz, shp)
+        d_xi = np.stack([-etm, etm, etp, -etp], axis=1) * 0.25
+        d_eta = np.stack([-xim, -xip, xip, xim], axis=1) * 0.25
+        J = np.empty((self.nelem, NPEM, 2, 2))
+        dxi = np.einsum('eij,xjp->eip', nsz, d_xi)
+        deta = np.einsum('eij,xjp->eip', nsz, d_eta)
+        J[:, :, 0, 0] = dxi[:, 0]
+        J[:, :, 1, 0] = dxi[:, 1]
+        J[:, :, 0, 1] = deta[:, 0]
+        J[:, :, 1, 1] = deta[:, 1]
+        return sz, J
+
+    def _number_points(self):
+        E = self.nelem
+        ep = np.empty((E, NPEM), dtype=np.int64)
+        keys = {}
+
+        def pid(key):
+            if key not in keys:
+                keys[key] = len(keys)
+            return keys[key]
+        for e in range(E):
+            nd = self.connectivity[e]
+            for k in range(4):
+                ep[e, _CORNER_IPNT[k]] = pid(("n", int(nd[k])))
+            for k in range(4):
+                u, v = int(nd[k]), int(nd[(k + 1) % 4])
+                for i, ip in enumerate(_EDGE_INNER[k]):
+                    pos = i if u < v else 2 - i
+                    ep[e, ip] = pid(("e", min(u, v), max(u, v), pos))
+            for i, ip in enumerate(_INNER_IPNT):
+                ep[e, ip] = pid(("i", e, i))
+        self._ep = ep
+        self._npoints = len(keys)
+
+    def element_points(self):
+        return self._ep.copy()
+
+    @property
+    def npoint_grid(self):
+        return self._npoints
+
+    def boundary_edges(self, key):
+        name = {"LEFT": "x0", "RIGHT": "x1", "BOTTOM": "y0", "TOP": "y1"}[key]
+        return sorted((int(e), int(s)) for e, s in
+                      self.side_sets.get(name, {}).items())
+
+    def interface_edges(self, fluid):
+        fluid = np.asarray(fluid, dtype=bool)
+        owner = {}
+        for e in range(self.nelem):
+            nd = self.connectivity[e]
+            for k in range(4):
+                u, v = int(nd[k]), int(nd[(k + 1) % 4])
+                owner.setdefault((min(u, v), max(u, v)), []).append((e, k))
+        out = []
+        for lst in owner.values():
+            if len(lst) == 2 and fluid[lst[0][0]] != fluid[lst[1][0]]:
+                out.extend(lst)
+        return sorted(out)
 
 
 # ------------------------------------------------------------ GradQuad ops
@@ -255,11 +423,34 @@ def compute_dt(sz, vmax, courant, factor=None):
 # ------------------------------------------------------------------ source
 def ricker_stf(t, half_duration, decay_factor, time_shift=0.0):
     """GaussianSTF, derivative order 2 (``GaussianSTF.hpp:35-58``), FP64."""
+    return gaussian_stf(t, half_duration, decay_factor, time_shift, 2)
+
+
+# use_derivative_integral of inparam.source.yaml -> derivative order
+# (STF.cpp:35-39)
+STF_ORDER = {"ERF": -1, "GAUSSIAN": 0, "FIRST_DERIVATIVE": 1, "RICKER": 2}
+
+
+def gaussian_stf(t, half_duration, decay_factor, time_shift=0.0, order=2):
+    """GaussianSTF::getValue (``GaussianSTF.hpp:35-58``), FP64, with f =
+    decay / half_duration and t' = t - time_shift: order -1 erf(f t') / 2 +
+    1/2; 0 f exp(-(f t')^2) / sqrt(pi); 1 -2 t' f^3 exp(-(f t')^2) /
+    sqrt(pi) (t', not f t', as the reference writes it); 2 (4 (f t')^2 -
+    2) f^3 exp(-(f t')^2) / sqrt(pi)."""
     f = decay_factor / half_duration
     tt = np.asarray(t, dtype=np.float64) - time_shift
     ft = f * tt
-    return (1.0 / math.sqrt(math.pi)) * f ** 3 * np.exp(-ft * ft) * (
-        4.0 * ft * ft - 2.0)
+    g = np.exp(-ft * ft) / math.sqrt(math.pi)
+    if order == -1:
+        from scipy.special import erf
+        return erf(ft) * 0.5 + 0.5
+    if order == 0:
+        return f * g
+    if order == 1:
+        return f ** 3 * g * (-2.0 * tt)
+    if order == 2:
+        return f ** 3 * g * (4.0 * ft * ft - 2.0)
+    raise ValueError("gaussian_stf: derivative order must be -1, 0, 1 or 2")
 
 
 def moment_pattern_axial(mt6, inplane, nu1_elem):
@@ -356,24 +547,22 @@ def inplane_nr_bound(mesh: StructuredMesh):
     for a, b in ((0, 1), (1, 2), (2, 3), (3, 0)):
         edges += np.hypot(nsz[:, 0, a] - nsz[:, 0, b],
                           nsz[:, 1, a] - nsz[:, 1, b])
-    ix, iz = mesh.element_ix_iz()
-    na, nb = mesh.nx + 1, mesh.nz + 1
-    corner = [(0, 0), (1, 0), (1, 1), (0, 1)]
-    tot = np.zeros((nb, na))
-    cnt = np.zeros((nb, na))
-    for c, (da, db) in enumerate(corner):
-        np.add.at(tot, (iz + db, ix + da), edges)
-        np.add.at(cnt, (iz + db, ix + da), 1.0)
-    spacing = tot / (cnt * 4 * spectral.NPOL)
+    nodes = mesh.corner_nodes()
+    tot = np.zeros(mesh.nnodes)
+    cnt = np.zeros(mesh.nnodes)
+    for c in range(4):
+        np.add.at(tot, nodes[:, c], edges)
+        np.add.at(cnt, nodes[:, c], 1.0)
+    spacing = tot / np.maximum(cnt * 4 * spectral.NPOL, 1e-300)
+    if mesh.spherical:
+        tol = 1e-6 * float(np.max(mesh.node_b))
+    else:
+        tol = 1e-6 * max(float(np.ptp(nsz[:, 0])), float(np.ptp(nsz[:, 1])))
     out = np.empty((mesh.nelem, 4))
-    for c, (da, db) in enumerate(corner):
-        h = spacing[iz + db, ix + da]
+    for c in range(4):
+        h = spacing[nodes[:, c]]
         s = nsz[:, 0, c]
         up = np.maximum(round_half_away(2.0 * math.pi * s / h), 1.0)
-        tol = 1e-6 * max(abs(mesh.node_a[-1] - mesh.node_a[0]),
-                         abs(mesh.node_b[-1] - mesh.node_b[0]))
-        if mesh.spherical:
-            tol = 1e-6 * float(np.max(mesh.node_b))
         out[:, c] = np.where(s < tol, 5.0, up)
     return out
 
@@ -387,8 +576,9 @@ def build_domain(mesh: StructuredMesh,
     """Points, solid elements, 1D isotropic material, mass, dt.
 
     rho_vp_vs(r) -> (rho, vp, vs) arrays evaluated at the element corners'
-    radial coordinate (z for Cartesian, |(s,z)| for spherical);
-    nodal_nr(s, z) -> Nr at element corners.
+    radial coordinate (z for Cartesian, |(s,z)| for spherical), or the
+    tuple of those (E, 4) arrays themselves; nodal_nr(s, z) -> Nr at
+    element corners.
     """
     axial = mesh.is_axial()
     E = mesh.nelem
@@ -396,11 +586,17 @@ def build_domain(mesh: StructuredMesh,
     nsz = mesh.nodal_sz()
     rcorner = (np.hypot(nsz[:, 0], nsz[:, 1]) if mesh.spherical
                else nsz[:, 1])
-    try:
-        rho_n, vp_n, vs_n = rho_vp_vs(rcorner, rcorner.mean(axis=1,
-                                                            keepdims=True))
-    except TypeError:
-        rho_n, vp_n, vs_n = rho_vp_vs(rcorner)
+    if not callable(rho_vp_vs):
+        # values at the element corners, (E, 4) each (an Exodus mesh's
+        # element-node storage)
+        rho_n, vp_n, vs_n = (np.asarray(x, dtype=np.float64)
+                             for x in rho_vp_vs)
+    else:
+        try:
+            rho_n, vp_n, vs_n = rho_vp_vs(rcorner, rcorner.mean(
+                axis=1, keepdims=True))
+        except TypeError:
+            rho_n, vp_n, vs_n = rho_vp_vs(rcorner)
     rho = interpolate_gll(rho_n, axial)
     vp = interpolate_gll(vp_n, axial)
     vs = interpolate_gll(vs_n, axial)
@@ -456,6 +652,11 @@ def build_domain(mesh: StructuredMesh,
     spec._courant = courant
     spec._elem_of_mesh = np.arange(E)
     return spec
+
+
+# element-local points of each edge, in the order of _edge_normals
+_EDGE_IPNT = {0: [0, 5, 10, 15, 20], 1: [20, 21, 22, 23, 24],
+              2: [24, 19, 14, 9, 4], 3: [4, 3, 2, 1, 0]}
 
 
 def _edge_normals(J, axial, edge, w_gll, w_glj, sz):
@@ -536,35 +737,26 @@ def make_fluid(spec, fluid_elem):
         if np.any(axial[fe]) else np.zeros(0, np.int64)
     # interface normals, accumulated per grid point (fluid side + n/2,
     # solid side - n/2, both pointing from fluid to solid)
-    ix, iz = mesh.element_ix_iz()
-    cell = -np.ones((mesh.nx, mesh.nz), dtype=np.int64)
-    cell[ix, iz] = np.arange(E)
     w_gll = spectral.weights_1d(False)
     w_glj = spectral.weights_1d(True)
     nsum = {}
-    for e in range(E):
-        for edge, (dx, dz) in ((0, (0, -1)), (1, (1, 0)), (2, (0, 1)),
-                               (3, (-1, 0))):
-            jx, jz = ix[e] + dx, iz[e] + dz
-            if not (0 <= jx < mesh.nx and 0 <= jz < mesh.nz):
-                continue
-            if fl[cell[jx, jz]] == fl[e]:
-                continue
-            pts, n = _edge_normals(gq["J"][e], bool(axial[e]), edge, w_gll,
-                                   w_glj, gq["sz"][e])
-            sign = 0.5 if fl[e] else -0.5
-            for k, p in enumerate(pts):
-                g = int(ep_grid[e, p])
-                nsum[g] = nsum.get(g, np.zeros(2)) + sign * n[k]
+    for e, edge in mesh.interface_edges(fl):
+        pts, n = _edge_normals(gq["J"][e], bool(axial[e]), edge, w_gll,
+                               w_glj, gq["sz"][e])
+        sign = 0.5 if fl[e] else -0.5
+        for k, p in enumerate(pts):
+            g = int(ep_grid[e, p])
+            nsum[g] = nsum.get(g, np.zeros(2)) + sign * n[k]
     sf_grid = np.array(sorted(nsum.keys()), dtype=np.int64)
     spec.sf_solid = rs[sf_grid].astype(np.int32)
     spec.sf_fluid = rf[sf_grid].astype(np.int32)
     spec.sf_normal = np.array([nsum[g] for g in sf_grid]).reshape(-1, 2)
     assert (spec.sf_solid >= 0).all() and (spec.sf_fluid >= 0).all()
-    # fluid free surface: fluid points on the top row of the mesh
-    top = ep_grid[fe][:, [4, 9, 14, 19, 24]][iz[fe] == mesh.nz - 1]
-    spec.fluid_surface_points = np.unique(rf[top.reshape(-1)]).astype(
-        np.int32)
+    # fluid free surface: fluid points on the top edges of the mesh
+    top = [ep_grid[e, p] for e, edge in mesh.boundary_edges("TOP") if fl[e]
+           for p in _EDGE_IPNT[edge]]
+    spec.fluid_surface_points = np.unique(
+        rf[np.asarray(top, dtype=np.int64)]).astype(np.int32)
     # fluid arrays
     spec.fluid_nr = nr_grid[flu_ids].astype(np.int32)
     spec.fluid_mass = f_mass
@@ -616,7 +808,6 @@ def gamma_fluid_default(vp, rho, span, t0):
     return 0.88 / t0 * np.exp(-0.04 * span / (vp * t0))
 
 
-_EDGE_OF_KEY = {"BOTTOM": 0, "RIGHT": 1, "TOP": 2}
 
 
 def _solid_rho_vp_vs(spec, e):
@@ -701,17 +892,16 @@ def add_ocean_load(spec, sum_rho_depth):
     take them."""
     mesh = spec._mesh
     gq = spec._gq_mesh
-    ix, iz = mesh.element_ix_iz()
     axial = mesh.is_axial()
     w_gll = spectral.weights_1d(False)
     w_glj = spectral.weights_1d(True)
     s_of_mesh = np.asarray(spec._elem_of_mesh)
     ntop = {}
-    for m in np.nonzero(iz == mesh.nz - 1)[0]:
+    for m, edge in mesh.boundary_edges("TOP"):
         e = int(s_of_mesh[m])
         if e < 0:
             continue
-        pts, n = _edge_normals(gq["J"][m], bool(axial[m]), 2, w_gll, w_glj,
+        pts, n = _edge_normals(gq["J"][m], bool(axial[m]), edge, w_gll, w_glj,
                                gq["sz"][m])
         for k, ipnt in enumerate(pts):
             p = int(spec.elem_points[e][ipnt])
@@ -810,7 +1000,6 @@ def add_absorbing(spec, keys=("RIGHT", "BOTTOM"), clayton=True,
     if rel_spans is None:
         rel_spans = [0.05] * len(keys)
     E = mesh.nelem
-    ix, iz = mesh.element_ix_iz()
     axial = mesh.is_axial()
     w_gll = spectral.weights_1d(False)
     w_glj = spectral.weights_1d(True)
@@ -844,13 +1033,10 @@ def add_absorbing(spec, keys=("RIGHT", "BOTTOM"), clayton=True,
     abi = {"clayton_solid": [], "clayton_solid3d": [],
            "clayton_fluid": [], "clayton_fluid3d": [],
            "sponge_solid": {}, "sponge_fluid": {}}
-    edge_ok = {"BOTTOM": iz == 0, "RIGHT": ix == mesh.nx - 1,
-               "TOP": iz == mesh.nz - 1}
     factor = np.ones(E)
     if clayton:
         for key in keys:
-            edge = _EDGE_OF_KEY[key]
-            for m in np.nonzero(edge_ok[key])[0]:
+            for m, edge in mesh.boundary_edges(key):
                 fluid, e, (rho, vp, vs) = material(m)
                 factor[m] = 0.5
                 pts, nabc = _edge_normals(gq["J"][m], bool(axial[m]), edge,
@@ -1006,9 +1192,41 @@ def add_absorbing(spec, keys=("RIGHT", "BOTTOM"), clayton=True,
     return spec
 
 
+def _locate_quads(mesh, s, z):
+    """locate() on an unstructured linear mesh: the first element in index
+    order whose closed quad contains (s, z), with (xi, eta) from Newton's
+    method on the bilinear map."""
+    nsz = mesh.nodal_sz()
+    lo, hi = nsz.min(axis=2), nsz.max(axis=2)
+    tol = 1e-9 * max(float(np.ptp(nsz[:, 0])), float(np.ptp(nsz[:, 1])))
+    cand = np.nonzero((lo[:, 0] - tol <= s) & (s <= hi[:, 0] + tol) &
+                      (lo[:, 1] - tol <= z) & (z <= hi[:, 1] + tol))[0]
+    for e in cand:
+        x = nsz[e]
+        xi = eta = 0.0
+        for _ in range(50):
+            xip, xim, etp, etm = 1 + xi, 1 - xi, 1 + eta, 1 - eta
+            shp = 0.25 * np.array([xim * etm, xip * etm, xip * etp,
+                                   xim * etp])
+            r = x @ shp - np.array([s, z])
+            dxi = 0.25 * np.array([-etm, etm, etp, -etp])
+            det_ = 0.25 * np.array([-xim, -xip, xip, xim])
+            Jm = np.stack([x @ dxi, x @ det_], axis=1)
+            step = np.linalg.solve(Jm, r)
+            xi, eta = xi - step[0], eta - step[1]
+            if abs(step).max() < 1e-14:
+                break
+        if abs(xi) <= 1 + 1e-9 and abs(eta) <= 1 + 1e-9:
+            return int(e), float(np.clip(xi, -1, 1)), float(np.clip(eta, -1,
+                                                                    1))
+    raise ValueError(f"locate: ({s}, {z}) is outside the mesh")
+
+
 def locate(mesh: StructuredMesh, s, z):
     """Element and (xi, eta) of a point (s, z); first element in index order
     whose closed cell contains it."""
+    if isinstance(mesh, QuadMesh):
+        return _locate_quads(mesh, s, z)
     if mesh.spherical:
         a = math.atan2(s, z)
         b = math.hypot(s, z)
@@ -1049,10 +1267,10 @@ def time_axis(t0, dt, nsteps):
 
 def add_axial_moment_source(spec: DomainSpec, depth_z: float, mt6,
                             half_duration, decay_factor, nsteps,
-                            time_shift=0.0):
-    """Axial moment source at (s=0, z) with a Ricker STF; sets the time axis
-    t0 = floor(t_start / dt) dt (``main.cpp:574-602``); hdur >= 5 dt
-    (``STF.cpp:32``)."""
+                            time_shift=0.0, stf_order=2):
+    """Axial moment source at (s=0, z) with a GaussianSTF of derivative
+    order stf_order (2: Ricker); sets the time axis t0 = floor(t_start /
+    dt) dt (``main.cpp:574-602``); hdur >= 5 dt (``STF.cpp:32``)."""
     mesh = spec._mesh
     elem_m, xi, eta = locate(mesh, 0.0, depth_z)
     w = inplane_factor(mesh, elem_m, xi, eta)
@@ -1067,8 +1285,8 @@ def add_axial_moment_source(spec: DomainSpec, depth_z: float, mt6,
     step0 = int(math.floor(t_start / spec.dt))
     spec.t0 = step0 * spec.dt
     spec.nsteps = int(nsteps)
-    spec.stf = ricker_stf(time_axis(spec.t0, spec.dt, nsteps), hdur,
-                          decay_factor, time_shift)
+    spec.stf = gaussian_stf(time_axis(spec.t0, spec.dt, nsteps), hdur,
+                            decay_factor, time_shift, stf_order)
     spec._stf_params = (hdur, decay_factor, time_shift)
     return spec
 
@@ -1174,6 +1392,140 @@ def cartesian_homogeneous(nx=80, nz=100, width=80e3, depth=80e3, nu=2,
     spec.name = f"cartesian_iso1d_E{mesh.nelem}_nu{nu}"
     spec._mt = mt
     return spec
+
+
+# --------------------------------------------------- Exodus + inparam
+def from_exodus(ex, nodal_nr, courant=0.6, bound_nr=True, lucky=True):
+    """A domain from an AxiSEM3D Exodus mesh (exodus.ExodusMesh) in
+    Cartesian mode with linear elements and an isotropic model stored per
+    element node (RHO, VP, VS; Material.cpp reads the same variables,
+    ExodusMesh::readBcastRadial, ExodusMesh.cpp:403-619): the points,
+    elements, mass and dt of build_domain on the mesh's own quads, then the
+    mesh's fluid elements through make_fluid (solid-fluid pairs from the
+    edges whose neighbour has the other medium, the fluid free surface from
+    the TOP side set)."""
+    if any(k.startswith(("VPV", "ETA")) for k in ex.elem_vars):
+        raise NotImplementedError("from_exodus: transversely isotropic "
+                                  "Exodus models are not read")
+    qm = QuadMesh.from_exodus(ex)
+    mat = tuple(ex.nodal_variable(v) for v in ("RHO", "VP", "VS"))
+    sp = build_domain(qm, mat, nodal_nr, courant=courant, lucky=lucky,
+                      bound_nr=bound_nr)
+    if ex.fluid.any():
+        make_fluid(sp, ex.fluid)
+    sp._exodus = ex
+    return sp
+
+
+def load_inparam(input_dir):
+    """The keys of a run's inparam.model/nr/source.yaml that the GPU path
+    takes (a subset; the reference reads them in main.cpp, NrField.cpp,
+    ABC.cpp and Source/STF builders), as a plain dict."""
+    import yaml
+
+    def rd(name):
+        with open(os.path.join(input_dir, name)) as f:
+            return yaml.safe_load(f)
+    model, nr, src = (rd("inparam.model.yaml"), rd("inparam.nr.yaml"),
+                      rd("inparam.source.yaml"))
+    abc = model.get("absorbing_boundary") or {}
+    kk = abc.get("Kosloff_Kosloff") or {}
+    first = list(src["list_of_sources"][0].values())[0]
+    loc, mech, stf = (first["location"], first["mechanism"],
+                      first["source_time_function"])
+    return {
+        "exodus_mesh": model["model1D"]["exodus_mesh"],
+        "attenuation": model.get("attenuation", "NONE"),
+        "abc": {"boundaries": list(abc.get("boundaries") or []),
+                "clayton": bool(abc.get("enable_Clayton_Enquist", False)),
+                "sponge": bool(kk.get("enable", False)),
+                "relative_spans": list(kk.get("relative_spans") or []),
+                "gamma_expr_solid": kk.get("gamma_expr_solid"),
+                "gamma_expr_fluid": kk.get("gamma_expr_fluid")},
+        "nr": {"type": nr["type_Nr"], "constant": nr.get("constant"),
+               "bound_by_inplane": bool(nr.get("bound_Nr_by_inplane",
+                                                True))},
+        "time_axis": {k: src["time_axis"].get(k) for k in
+                      ("record_length", "enforced_dt", "Courant_number",
+                       "integrator")},
+        "source": {"latitude_longitude": list(loc["latitude_longitude"]),
+                   "depth": float(loc["depth"]),
+                   "mechanism": mech["type"],
+                   "data": [float(x) for x in mech["data"]],
+                   "unit": float(mech["unit"]),
+                   "stf_class": stf["class_name"],
+                   "half_duration": float(stf["half_duration"]),
+                   "decay_factor": float(stf["decay_factor"]),
+                   "time_shift": float(stf["time_shift"]),
+                   "derivative": stf["use_derivative_integral"]}}
+
+
+# the template's Kosloff gamma expressions (inparam.model.yaml), which
+# gamma_solid_default / gamma_fluid_default restate
+_GAMMA_SOLID_EXPR = "1.1 / T0 * (VS / VP)^2 * exp(-0.04 * SPAN / (VP * T0))"
+_GAMMA_FLUID_EXPR = "0.88 / T0 * exp(-0.04 * SPAN / (VP * T0))"
+
+
+def from_inparam(inp, ex, nsteps, receivers_s=None, receivers_z=None,
+                 receivers_phi=None):
+    """A run set up as the reference sets it up from its inputs (inp:
+    load_inparam's dict; ex: the exodus.ExodusMesh it names): constant Nr
+    (bounded by the in-plane resolution if asked), the Courant number,
+    Clayton-Enquist and Kosloff boundaries with the template's gamma
+    expressions and the mesh period T0, Newmark or the symplectic scheme,
+    one moment-tensor source on the axis with a GaussianSTF (depth below
+    the mesh top, tensor data times unit), and receivers at (s, z, phi)
+    (the inputs' station groups are not read).  Raises NotImplementedError
+    for what the GPU path does not build from inputs (other Nr types,
+    attenuation, sources off the axis)."""
+    nr = inp["nr"]
+    if nr["type"] != "CONSTANT":
+        raise NotImplementedError("from_inparam: type_Nr other than "
+                                  "CONSTANT")
+    if str(inp["attenuation"]).upper() != "NONE":
+        raise NotImplementedError("from_inparam: attenuation from an "
+                                  "Exodus mesh")
+    const = float(nr["constant"])
+    ta = inp["time_axis"]
+    sp = from_exodus(ex, lambda s, z: np.full(np.shape(s), const),
+                     courant=float(ta["Courant_number"]),
+                     bound_nr=nr["bound_by_inplane"])
+    abc = inp["abc"]
+    if abc["boundaries"]:
+        if abc["sponge"] and (abc["gamma_expr_solid"] != _GAMMA_SOLID_EXPR
+                              or abc["gamma_expr_fluid"] !=
+                              _GAMMA_FLUID_EXPR):
+            raise NotImplementedError("from_inparam: Kosloff gamma "
+                                      "expressions other than the "
+                                      "template's")
+        add_absorbing(sp, keys=tuple(abc["boundaries"]),
+                      clayton=abc["clayton"], sponge=abc["sponge"],
+                      rel_spans=abc["relative_spans"] or None,
+                      t0=ex.global_variables["minimum_period"])
+    if ta.get("enforced_dt") not in (None, "NONE"):
+        # main.cpp:431-480: an enforced dt replaces the computed one
+        sp.dt = float(ta["enforced_dt"])
+    if str(ta.get("integrator", "NEWMARK")).upper() == "SYMPLECTIC":
+        sp.time_scheme = "symplectic"
+    src = inp["source"]
+    if src["mechanism"] != "MOMENT_TENSOR" or src["stf_class"] != \
+            "GaussianSTF":
+        raise NotImplementedError("from_inparam: moment tensors with a "
+                                  "GaussianSTF only")
+    if abs(float(src["latitude_longitude"][0]) - 90.0) > 1e-12:
+        raise NotImplementedError("from_inparam: sources on the axis "
+                                  "(latitude 90) only")
+    ztop = float(ex.coords[:, 1].max())
+    mt = np.asarray(src["data"]) * src["unit"]
+    add_axial_moment_source(sp, ztop - src["depth"], mt,
+                            src["half_duration"], src["decay_factor"],
+                            nsteps, time_shift=src["time_shift"],
+                            stf_order=STF_ORDER[src["derivative"]])
+    if receivers_s is not None:
+        add_receivers(sp, np.asarray(receivers_s, dtype=np.float64),
+                      np.asarray(receivers_z, dtype=np.float64),
+                      np.asarray(receivers_phi, dtype=np.float64))
+    return sp
 
 
 # ------------------------------------------------------------- 3D media
