@@ -71,20 +71,19 @@ __device__ __forceinline__ void pack_pair(cpx<Real> *buf,
                                           const cpx<Real> *__restrict__ Y,
                                           int nr, int nu1, bool even,
                                           int lane) {
-    // branch-free: slice k takes row entry kk = k (k < nu1) or nr - k
-    // (conjugated); the mean and an even Nr's Nyquist entry are real,
-    // the latter doubled
+    // each row entry k < nu1 is read once and gives slice k and, conjugated,
+    // its mirror nr - k; the mean and an even Nr's Nyquist entry are real
+    // (no mirror), the latter doubled
     const int nyq = even ? nu1 - 1 : -1;
 #pragma unroll 4
-    for (int k = lane; k < nr; k += 64) {
-        const bool lo = k < nu1;
-        const int kk = lo ? k : nr - k;
-        const cpx<Real> x = X[kk], y = Y[kk];
+    for (int k = lane; k < nu1; k += 64) {
+        const cpx<Real> x = X[k], y = Y[k];
         const bool real_only = k == 0 || k == nyq;
         const Real sc = k == nyq ? (Real)2 : (Real)1;
-        const Real sim = real_only ? (Real)0 : (lo ? (Real)1 : (Real)-1);
+        const Real sim = real_only ? (Real)0 : (Real)1;
         // xf = (sc x.re, sim x.im), yf likewise; buf = xf + i yf
         buf[pad(k)] = {sc * x.re - sim * y.im, sim * x.im + sc * y.re};
+        if (!real_only) buf[pad(nr - k)] = {x.re + y.im, y.re - x.im};
     }
     wave_sync();
 }
@@ -401,7 +400,8 @@ __global__ __launch_bounds__(CD_PT, CardOcc<Real>::value) void k_cd_stress(
         const int c = t / a.nsls, l = t - c * a.nsls;
         abg[c * CD_MAX_SLS + l] = a.abg[t];
     }
-    __syncthreads();
+    // the strain pair goes to the wave's own buffer while the tables load:
+    // both sets of global reads are in flight before the barrier
     const bool even = (nr & 1) == 0;
     const size_t rs = (size_t)NPEM * scr_row(nu1);
     cpx<Real> *r0 = reinterpret_cast<cpx<Real> *>(a.scratch) + a.scr_off[e] +
@@ -409,6 +409,7 @@ __global__ __launch_bounds__(CD_PT, CardOcc<Real>::value) void k_cd_stress(
     cpx<Real> *b = L.buf(q);
     pack_pair<Real>(b, r0 + 2 * q * rs, r0 + (2 * q + 1) * rs, nr, nu1, even,
                     lane);
+    __syncthreads();
     // both transforms from one call site (the inverse, the stress, then
     // the forward transform), which halves the FFT's code
 #pragma unroll 1

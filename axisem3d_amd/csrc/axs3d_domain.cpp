@@ -749,6 +749,37 @@ void DomainBase::buildFftTables(int maxRadix, std::vector<uint32_t> &plan,
     }
 }
 
+// Point -> element-local force offsets in ascending element order: four
+// slots per point (a conforming quad mesh has at most four elements at a
+// point); a point shared by more (the doubling layers of unstructured
+// meshes) keeps its first three in slots 0..2 and in slot 3 the marker
+// 0x80000000 | j of the rest of its list, stored from index j of the same
+// array (after the 4 n slots) and closed by 0xffffffff.
+template <typename OffsetFn>
+std::vector<uint32_t> buildAdjacency(int npoints, int nelems,
+                                     const int32_t *elem_points,
+                                     OffsetFn offset) {
+    std::vector<std::vector<uint32_t>> lists(npoints);
+    for (int e = 0; e < nelems; e++)
+        for (int k = 0; k < NPEM; k++)
+            lists[elem_points[(size_t)e * NPEM + k]].push_back(offset(e, k));
+    std::vector<uint32_t> adj((size_t)npoints * 4, 0xffffffffu);
+    for (int p = 0; p < npoints; p++) {
+        const std::vector<uint32_t> &l = lists[p];
+        if (l.size() <= 4) {
+            for (size_t i = 0; i < l.size(); i++) adj[(size_t)p * 4 + i] = l[i];
+            continue;
+        }
+        for (int i = 0; i < 3; i++) adj[(size_t)p * 4 + i] = l[i];
+        if (adj.size() >= 0x7fffffffu)
+            throw Error("Domain::finalize || Point adjacency too large.");
+        adj[(size_t)p * 4 + 3] = 0x80000000u | (uint32_t)adj.size();
+        adj.insert(adj.end(), l.begin() + 3, l.end());
+        adj.push_back(0xffffffffu);
+    }
+    return adj;
+}
+
 void DomainBase::finalize() {
     hip_check(hipSetDevice(mDevice), "Domain::finalize");
     if (const char *env = std::getenv("AXS3D_FFT_MAX_RADIX")) {
@@ -933,21 +964,11 @@ void DomainBase::finalize() {
         throw Error("Domain::finalize || Too many element modes.");
     mNumEBTriples = ebTotal;
     // point -> (element, ipnt) adjacency in ascending element order, the
-    // order in which Domain::computeStiffness visits the elements; four
-    // slots per point (conforming quad meshes)
-    std::vector<uint32_t> adj((size_t)mNumPoints * 4, 0xffffffffu);
-    {
-        std::vector<uint8_t> fill(mNumPoints, 0);
-        for (int e = 0; e < mNumElems; e++)
-            for (int k = 0; k < NPEM; k++) {
-                int p = hElemPoints[(size_t)e * NPEM + k];
-                if (fill[p] >= 4)
-                    throw Error("Domain::finalize || A point is shared by "
-                                "more than four elements.");
-                adj[(size_t)p * 4 + fill[p]++] =
-                    ebOff[e] + (uint32_t)k * eNu1[e];
-            }
-    }
+    // order in which Domain::computeStiffness visits the elements
+    std::vector<uint32_t> adj = buildAdjacency(
+        mNumPoints, mNumElems, hElemPoints.data(), [&](int e, int k) {
+            return ebOff[e] + (uint32_t)k * eNu1[e];
+        });
     // sources
     mSrcNu1c.clear();
     for (size_t i = 0; i < hSrcElems.size(); i++) {
@@ -1045,17 +1066,10 @@ void DomainBase::finalize() {
             throw Error("Domain::finalize || Too many fluid element modes.");
         mNumFEBModes = eb;
         mNumFEM = (int)fem.size();
-        fAdj.assign((size_t)mNumFPoints * 4, 0xffffffffu);
-        std::vector<uint8_t> fill(mNumFPoints, 0);
-        for (int e = 0; e < mNumFElems; e++)
-            for (int k = 0; k < NPEM; k++) {
-                const int p = hFElemPoints[(size_t)e * NPEM + k];
-                if (fill[p] >= 4)
-                    throw Error("Domain::finalize || A fluid point is shared "
-                                "by more than four elements.");
-                fAdj[(size_t)p * 4 + fill[p]++] =
-                    fEbOff[e] + (uint32_t)k * fElemNu1[e];
-            }
+        fAdj = buildAdjacency(
+            mNumFPoints, mNumFElems, hFElemPoints.data(), [&](int e, int k) {
+                return fEbOff[e] + (uint32_t)k * fElemNu1[e];
+            });
         for (int32_t p : hFAxial) {
             if (p < 0 || p >= mNumFPoints)
                 throw Error("Domain::finalize || Fluid axial point out of "

@@ -113,6 +113,8 @@ __global__ __launch_bounds__(STIFF_BLOCK) void k_fluid_stiff_fr(
 }
 
 constexpr uint32_t FADJ_NONE = 0xffffffffu;
+// slot 3 of a fluid point shared by more than four elements (as ADJ_MORE)
+constexpr uint32_t FADJ_MORE = 0x80000000u;
 
 template <typename Real>
 __device__ __forceinline__ void gather_fluid(const uint32_t *__restrict__ adj,
@@ -121,12 +123,20 @@ __device__ __forceinline__ void gather_fluid(const uint32_t *__restrict__ adj,
     f[0] = f[1] = 0;
     const uint4 a4 = *reinterpret_cast<const uint4 *>(adj + (size_t)p * 4);
     const uint32_t base[4] = {a4.x, a4.y, a4.z, a4.w};
+    auto add = [&](uint32_t b) {
+        const Real *src = eb + ((size_t)b + alpha) * 2;
+        f[0] -= src[0];
+        f[1] -= src[1];
+    };
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-        if (base[k] != FADJ_NONE) {
-            const Real *src = eb + ((size_t)base[k] + alpha) * 2;
-            f[0] -= src[0];
-            f[1] -= src[1];
+    for (int k = 0; k < 3; k++)
+        if (base[k] != FADJ_NONE) add(base[k]);
+    if (base[3] != FADJ_NONE) {
+        if (base[3] & FADJ_MORE) {
+            for (uint32_t j = base[3] & ~FADJ_MORE; adj[j] != FADJ_NONE; j++)
+                add(adj[j]);
+        } else {
+            add(base[3]);
         }
     }
 }

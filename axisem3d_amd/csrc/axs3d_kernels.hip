@@ -268,7 +268,9 @@ __global__ __launch_bounds__(STIFF_BLOCK) void k_moment_source(
 // most four elements, so MAXADJ = 4 covers the structured meshes and the
 // fixed width lets every load of the point issue at once.
 constexpr uint32_t ADJ_NONE = 0xffffffffu;
-constexpr int MAXADJ = 4;
+// slot 3 of a point shared by more than four elements: ADJ_MORE | j, the
+// rest of its list at adj[j], adj[j + 1], ... up to an ADJ_NONE
+constexpr uint32_t ADJ_MORE = 0x80000000u;
 
 template <typename Real>
 __device__ __forceinline__ void gather_forces(
@@ -278,12 +280,21 @@ __device__ __forceinline__ void gather_forces(
     for (int q = 0; q < 6; q++) f[q] = 0;
     const uint4 a4 = *reinterpret_cast<const uint4 *>(adj + (size_t)p * 4);
     const uint32_t base[4] = {a4.x, a4.y, a4.z, a4.w};
+    auto add = [&](uint32_t b) {
+        const Real *src = eb + ((size_t)b + alpha) * 6;
 #pragma unroll
-    for (int k = 0; k < MAXADJ; k++) {
-        if (base[k] != ADJ_NONE) {
-            const Real *src = eb + ((size_t)base[k] + alpha) * 6;
+        for (int q = 0; q < 6; q++) f[q] -= src[q];
+    };
 #pragma unroll
-            for (int q = 0; q < 6; q++) f[q] -= src[q];
+    for (int k = 0; k < 3; k++)
+        if (base[k] != ADJ_NONE) add(base[k]);
+    if (base[3] != ADJ_NONE) {
+        if (base[3] & ADJ_MORE) {
+            // element order kept: the overflow list follows slots 0..2
+            for (uint32_t j = base[3] & ~ADJ_MORE; adj[j] != ADJ_NONE; j++)
+                add(adj[j]);
+        } else {
+            add(base[3]);
         }
     }
 }

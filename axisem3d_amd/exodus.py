@@ -153,9 +153,30 @@ class ExodusMesh:
         self.cartesian = self.records.get("crdsys", "").lower() == \
             "cartesian"
         left = "x0" if self.cartesian else "t0"
+        self._orient_axial(left)
         self.left = np.full(self.nelem, -1, dtype=np.int64)
         for e, s in self.side_sets.get(left, {}).items():
             self.left[e] = s
+
+    def _orient_axial(self, left):
+        """Cycles the nodes of every axial element so that its side on the
+        axis becomes side 3 (xi = -1), with its element-node variables and
+        side-set sides (ExodusMesh.cpp:585-610); a no-op on an oriented
+        mesh."""
+        node_vars = sorted({k[:-2] for k in self.elem_vars
+                            if k[-2:] == "_0"})
+        for e, a in list(self.side_sets.get(left, {}).items()):
+            if a == 3:
+                continue
+            src = [(k + a - 3) % 4 for k in range(4)]
+            self.connectivity[e] = self.connectivity[e][src]
+            for v in node_vars:
+                vals = [self.elem_vars[f"{v}_{k}"][e] for k in range(4)]
+                for k in range(4):
+                    self.elem_vars[f"{v}_{k}"][e] = vals[src[k]]
+            for m in self.side_sets.values():
+                if e in m:
+                    m[e] = (m[e] - a + 3) % 4
 
     def _read(self, path):
         f = ExodusFile(path)

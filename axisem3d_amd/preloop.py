@@ -212,26 +212,124 @@ _EDGE_INNER = {0: (5, 10, 15), 1: (21, 22, 23), 2: (19, 14, 9),
 _INNER_IPNT = tuple(i * NPED + j for i in range(1, 4) for j in range(1, 4))
 
 
-class QuadMesh:
-    """An unstructured mesh of 4-node quads with linear geometry, as an
-    AxiSEM3D Exodus mesh in Cartesian mode gives it (ExodusMesh.cpp:
-    205-619; MappingLinear.hpp): corner nodes and coordinates, axial
-    elements (a side in the LEFT side set, which must be element side 3,
-    xi = -1), side sets for the boundaries, the solid-fluid interface and
-    the medium per element.  GLL points are numbered through the
-    connectivity (corner nodes, edges by their two nodes, interiors), which
-    is exact for a conforming mesh."""
-    spherical = False
+# coordinate rotations that bring an element's curved outer edge k to edge
+# 2 (Mapping::sOrthogQ2)
+_Q2 = (np.array([[-1.0, 0.0], [0.0, -1.0]]), np.array([[0.0, -1.0],
+                                                         [1.0, 0.0]]),
+       np.eye(2), np.array([[0.0, 1.0], [-1.0, 0.0]]))
 
-    def __init__(self, coords, connectivity, side_sets, fluid=None):
+
+def _angles_close(a, b):
+    """Mapping::makeAnglesClose: shift one of two angles by 2 pi so that
+    they differ by at most pi."""
+    if a - b > math.pi:
+        b += 2.0 * math.pi
+    if b - a > math.pi:
+        a += 2.0 * math.pi
+    return a, b
+
+
+def _curved_outer(nsz, semi):
+    """The curved outer edge of a spherical (semi=False: edges 0 and 2 or
+    1 and 3 on circles) or semi-spherical element (one edge on a circle),
+    as the MappingSpherical / MappingSemiSpherical constructors find it."""
+    r = np.hypot(nsz[0], nsz[1])
+    edge = min(np.hypot(*(nsz[:, k] - nsz[:, (k + 1) % 4])) for k in range(4))
+    tol = edge * 1e-3
+    if not semi:
+        if abs(r[0] - r[1]) < tol and abs(r[2] - r[3]) < tol:
+            return 2 if r[2] > r[0] else 0
+        if abs(r[1] - r[2]) < tol and abs(r[3] - r[0]) < tol:
+            return 3 if r[3] > r[1] else 1
+    else:
+        for c in range(4):
+            if abs(r[c] - r[(c + 1) % 4]) < tol and r[c] > r[(c + 2) % 4]:
+                return c
+    raise ValueError("QuadMesh: cannot identify the curved outer edge of a "
+                     "(semi-)spherical element")
+
+
+def _map_curved(nsz, xieta, semi):
+    """MappingSpherical / MappingSemiSpherical of one element: rotate so
+    the curved outer edge is edge 2, interpolate the angle along the outer
+    edge (and along the inner edge for the spherical form, the straight
+    inner edge for the semi-spherical one), rotate back.  Returns sz (2,
+    25) and J (25, 2, 2) = d(s,z)/d(xi,eta)."""
+    c = _curved_outer(nsz, semi)
+    Q = _Q2[c]
+    szq = Q @ nsz
+    xq = Q @ xieta
+    rr = np.hypot(szq[0], szq[1])
+    th = 0.5 * math.pi - np.arctan2(szq[1], szq[0])
+    k = lambda d: (c + d) % 4
+    xi, eta = xq[0], xq[1]
+    t2, t3 = float(th[k(0)]), float(th[k(1)])
+    t3, t2 = _angles_close(t3, t2)
+    r2 = float(rr[k(0)])
+    t32 = ((1.0 - xi) * t3 + (1.0 + xi) * t2) * 0.5
+    t32_xi = (t2 - t3) * 0.5
+    r2p = (1.0 + eta) * r2 * 0.5
+    Jq = np.empty((xieta.shape[1], 2, 2))
+    if not semi:
+        r0 = float(rr[k(-2)])
+        t0, t1 = float(th[k(-2)]), float(th[k(-1)])
+        t0, t1 = _angles_close(t0, t1)
+        t01 = ((1.0 - xi) * t0 + (1.0 + xi) * t1) * 0.5
+        t01_xi = (t1 - t0) * 0.5
+        r0m = (1.0 - eta) * r0 * 0.5
+        s = r0m * np.sin(t01) + r2p * np.sin(t32)
+        z = r0m * np.cos(t01) + r2p * np.cos(t32)
+        Jq[:, 0, 0] = r0m * np.cos(t01) * t01_xi + r2p * np.cos(t32) * t32_xi
+        Jq[:, 1, 0] = -(r0m * np.sin(t01) * t01_xi +
+                        r2p * np.sin(t32) * t32_xi)
+        Jq[:, 0, 1] = -0.5 * r0 * np.sin(t01) + 0.5 * r2 * np.sin(t32)
+        Jq[:, 1, 1] = -0.5 * r0 * np.cos(t01) + 0.5 * r2 * np.cos(t32)
+    else:
+        s0, z0 = szq[0, k(-2)], szq[1, k(-2)]
+        s1, z1 = szq[0, k(-1)], szq[1, k(-1)]
+        s01 = ((1.0 - xi) * s0 + (1.0 + xi) * s1) * 0.5
+        z01 = ((1.0 - xi) * z0 + (1.0 + xi) * z1) * 0.5
+        etam = (1.0 - eta) * 0.5
+        s = etam * s01 + r2p * np.sin(t32)
+        z = etam * z01 + r2p * np.cos(t32)
+        Jq[:, 0, 0] = etam * (s1 - s0) * 0.5 + r2p * np.cos(t32) * t32_xi
+        Jq[:, 1, 0] = etam * (z1 - z0) * 0.5 - r2p * np.sin(t32) * t32_xi
+        Jq[:, 0, 1] = -0.5 * s01 + 0.5 * r2 * np.sin(t32)
+        Jq[:, 1, 1] = -0.5 * z01 + 0.5 * r2 * np.cos(t32)
+    sz = Q.T @ np.stack([s, z])
+    J = np.einsum("ji,pjk,kl->pil", Q, Jq, Q)
+    return sz, J
+
+
+class QuadMesh:
+    """An unstructured mesh of 4-node quads, as an AxiSEM3D Exodus mesh
+    gives it (ExodusMesh.cpp:205-619): corner nodes and coordinates, the
+    geometry type per element (0 spherical, 1 linear, 2 semi-spherical;
+    MappingSpherical/Linear/SemiSpherical.hpp), axial elements (a side in
+    the LEFT side set, x0 or t0, which must be element side 3, xi = -1),
+    side sets for the boundaries (x0/x1/y0/y1 in Cartesian meshes,
+    t0/t1/r0/r1 in spherical ones), the solid-fluid interface and the
+    medium per element.  GLL points are numbered through the connectivity
+    (corner nodes, edges by their two nodes, interiors), which is exact for
+    a conforming mesh."""
+
+    def __init__(self, coords, connectivity, side_sets, fluid=None,
+                 geometry_type=None, spherical=False):
         self.coords = np.asarray(coords, dtype=np.float64)
         self.connectivity = np.asarray(connectivity, dtype=np.int64)
         self.side_sets = side_sets
+        self.spherical = bool(spherical)
         E = len(self.connectivity)
+        self.geometry_type = (np.ones(E, np.int64) if geometry_type is None
+                              else np.asarray(geometry_type, np.int64))
         self.fluid = (np.zeros(E, bool) if fluid is None else
                       np.asarray(fluid, dtype=bool))
+        self._keys = ({"LEFT": "t0", "RIGHT": "t1", "BOTTOM": "r0",
+                       "TOP": "r1"} if self.spherical else
+                      {"LEFT": "x0", "RIGHT": "x1", "BOTTOM": "y0",
+                       "TOP": "y1"})
         self._axial = np.zeros(E, dtype=bool)
-        for e, side in side_sets.get("x0", {}).items():
+        for e, side in side_sets.get(self._keys["LEFT"], {}).items():
             if side != 3:
                 raise ValueError("QuadMesh: an axial element must have the "
                                  "axis on its side 3 (xi = -1)")
@@ -240,11 +338,8 @@ class QuadMesh:
 
     @classmethod
     def from_exodus(cls, ex):
-        if not ex.cartesian or np.any(ex.geometry_type != 1):
-            raise NotImplementedError(
-                "QuadMesh: Cartesian meshes of linear elements only (the "
-                "spherical and semi-spherical mappings are not built)")
-        return cls(ex.coords, ex.connectivity, ex.side_sets, ex.fluid)
+        return cls(ex.coords, ex.connectivity, ex.side_sets, ex.fluid,
+                   ex.geometry_type, spherical=not ex.cartesian)
 
     @property
     def nelem(self):
@@ -264,8 +359,18 @@ class QuadMesh:
         return self.coords[self.connectivity].transpose(0, 2, 1).copy()
 
     def map_points(self, xieta, axial):
-        """MappingLinear: bilinear shape functions on the 4 nodes; sz (E, 2,
-        25) and J (E, 25, 2, 2) = d(s,z)/d(xi,eta)."""
+        """The element mappings at reference coords (2, 25): sz (E, 2, 25)
+        and J (E, 25, 2, 2) = d(s,z)/d(xi,eta); MappingLinear (bilinear
+        shape functions on the 4 nodes) for geometry type 1, the curved
+        forms for 0 and 2."""
+        sz, J = self._map_linear(xieta)
+        nsz = self.nodal_sz()
+        for e in np.nonzero(self.geometry_type != 1)[0]:
+            sz[e], J[e] = _map_curved(nsz[e], xieta,
+                                      self.geometry_type[e] == 2)
+        return sz, J
+
+    def _map_linear(self, xieta):
         xi = xieta[0][None, :]
         eta = xieta[1][None, :]
         nsz = self.nodal_sz()
@@ -275,7 +380,7 @@ class QuadMesh:
         sz = np.einsum('eij,xjp->eip', nsz, shp)
         d_xi = np.stack([-etm, etm, etp, -etp], axis=1) * 0.25
         d_eta = np.stack([-xim, -xip, xip, xim], axis=1) * 0.25
-        J = np.empty((self.nelem, NPEM, 2, 2))
+        J = np.empty((self.nelem, xieta.shape[1], 2, 2))
         dxi = np.einsum('eij,xjp->eip', nsz, d_xi)
         deta = np.einsum('eij,xjp->eip', nsz, d_eta)
         J[:, :, 0, 0] = dxi[:, 0]
@@ -315,7 +420,7 @@ class QuadMesh:
         return self._npoints
 
     def boundary_edges(self, key):
-        name = {"LEFT": "x0", "RIGHT": "x1", "BOTTOM": "y0", "TOP": "y1"}[key]
+        name = self._keys[key]
         return sorted((int(e), int(s)) for e, s in
                       self.side_sets.get(name, {}).items())
 
@@ -555,7 +660,7 @@ def inplane_nr_bound(mesh: StructuredMesh):
         np.add.at(cnt, nodes[:, c], 1.0)
     spacing = tot / np.maximum(cnt * 4 * spectral.NPOL, 1e-300)
     if mesh.spherical:
-        tol = 1e-6 * float(np.max(mesh.node_b))
+        tol = 1e-6 * float(np.max(np.hypot(nsz[:, 0], nsz[:, 1])))
     else:
         tol = 1e-6 * max(float(np.ptp(nsz[:, 0])), float(np.ptp(nsz[:, 1])))
     out = np.empty((mesh.nelem, 4))
@@ -1395,24 +1500,82 @@ def cartesian_homogeneous(nx=80, nz=100, width=80e3, depth=80e3, nu=2,
 
 
 # --------------------------------------------------- Exodus + inparam
-def from_exodus(ex, nodal_nr, courant=0.6, bound_nr=True, lucky=True):
-    """A domain from an AxiSEM3D Exodus mesh (exodus.ExodusMesh) in
-    Cartesian mode with linear elements and an isotropic model stored per
-    element node (RHO, VP, VS; Material.cpp reads the same variables,
-    ExodusMesh::readBcastRadial, ExodusMesh.cpp:403-619): the points,
-    elements, mass and dt of build_domain on the mesh's own quads, then the
-    mesh's fluid elements through make_fluid (solid-fluid pairs from the
-    edges whose neighbour has the other medium, the fluid free surface from
-    the TOP side set)."""
-    if any(k.startswith(("VPV", "ETA")) for k in ex.elem_vars):
-        raise NotImplementedError("from_exodus: transversely isotropic "
-                                  "Exodus models are not read")
+def from_exodus(ex, nodal_nr, courant=0.6, bound_nr=True, lucky=True,
+                attenuation="NONE"):
+    """A domain from an AxiSEM3D Exodus mesh (exodus.ExodusMesh): its quads
+    with their linear, spherical or semi-spherical mappings, the model
+    stored per element node (RHO with VP, VS or the transversely isotropic
+    VPV, VPH, VSV, VSH, ETA; QKAPPA, QMU for attenuation), the points,
+    elements, mass and dt of build_domain, the fluid elements through
+    make_fluid (solid-fluid pairs from the edges whose neighbour has the
+    other medium, the free surface from the TOP side set).  A transversely
+    isotropic element stays one unless its velocities reduce to isotropic
+    (Material.cpp:80-101): A = rho vph^2, C = rho vpv^2, L = rho vsv^2, N =
+    rho vsh^2, F = eta (A - 2 L) at the GLL points (Material::createTISO,
+    Material.cpp:498-520).  attenuation CG4 or FULL takes the mesh's SLS
+    (w_l, y_l, nr_lin_solids) and band (f_min, f_max, f_ref) and corrects
+    every medium through its Voigt averages (make_media)."""
+    ti = any(k in ex.elem_vars for k in ("VPV_0", "VPV"))
+    nv = ex.nodal_variable
+    rho = nv("RHO")
+    if ti:
+        vpv, vph, vsv, vsh, eta = (nv(k) for k in
+                                   ("VPV", "VPH", "VSV", "VSH", "ETA"))
+        vp_b, vs_b = np.maximum(vpv, vph), np.maximum(vsv, vsh)
+    else:
+        vp_b, vs_b = nv("VP"), nv("VS")
     qm = QuadMesh.from_exodus(ex)
-    mat = tuple(ex.nodal_variable(v) for v in ("RHO", "VP", "VS"))
-    sp = build_domain(qm, mat, nodal_nr, courant=courant, lucky=lucky,
-                      bound_nr=bound_nr)
+    sp = build_domain(qm, (rho, vp_b, vs_b), nodal_nr, courant=courant,
+                      lucky=lucky, bound_nr=bound_nr)
     if ex.fluid.any():
         make_fluid(sp, ex.fluid)
+    att_kind = str(attenuation).upper()
+    if ti or att_kind != "NONE":
+        ax = qm.is_axial()
+        g = lambda a: interpolate_gll(a, ax)
+        e_of_m = np.asarray(sp._elem_of_mesh)
+        solid = np.nonzero(e_of_m >= 0)[0]
+        mesh_of = solid[np.argsort(e_of_m[solid])]    # solid elem -> mesh
+        rg = g(rho)[mesh_of]
+        mtype = np.zeros(len(mesh_of), dtype=np.uint8)
+        params = []
+        if ti:
+            gv = {k: g(a)[mesh_of] for k, a in
+                  (("vpv", vpv), ("vph", vph), ("vsv", vsv), ("vsh", vsh),
+                   ("eta", eta))}
+            eps = np.finfo(float).eps
+            for i, m in enumerate(mesh_of):
+                iso = (np.linalg.norm(vpv[m] - vph[m]) <
+                       np.linalg.norm(vpv[m]) * eps and
+                       np.linalg.norm(vsv[m] - vsh[m]) <
+                       np.linalg.norm(vsv[m]) * eps and
+                       np.linalg.norm(eta[m] - 1.0) < 2.0 * eps)
+                if iso:
+                    vp2, vs2 = gv["vpv"][i] ** 2, gv["vsv"][i] ** 2
+                    params.append(np.array([rg[i] * (vp2 - 2.0 * vs2),
+                                            rg[i] * vs2]))
+                    continue
+                mtype[i] = 1
+                A = rg[i] * gv["vph"][i] ** 2
+                C = rg[i] * gv["vpv"][i] ** 2
+                L = rg[i] * gv["vsv"][i] ** 2
+                N = rg[i] * gv["vsh"][i] ** 2
+                params.append(np.array([A, C, gv["eta"][i] * (A - 2.0 * L),
+                                        L, N]))
+        else:
+            params = [sp.elem_iso[i] for i in range(len(mesh_of))]
+        att = None
+        if att_kind != "NONE":
+            gl = ex.global_variables
+            nsls = int(round(gl["nr_lin_solids"]))
+            att = {"nsls": nsls, "cg4": att_kind == "CG4",
+                   "f_min": gl["f_min"], "f_max": gl["f_max"],
+                   "f_ref": gl["f_ref"],
+                   "w": [gl[f"w_{k}"] for k in range(nsls)],
+                   "y": [gl[f"y_{k}"] for k in range(nsls)],
+                   "qkp": g(nv("QKAPPA"))[mesh_of],
+                   "qmu": g(nv("QMU"))[mesh_of]}
+        make_media(sp, mtype, att=att, params1d=params)
     sp._exodus = ex
     return sp
 
@@ -1433,9 +1596,11 @@ def load_inparam(input_dir):
     first = list(src["list_of_sources"][0].values())[0]
     loc, mech, stf = (first["location"], first["mechanism"],
                       first["source_time_function"])
+    geo = model.get("geodesy") or {}
     return {
         "exodus_mesh": model["model1D"]["exodus_mesh"],
         "attenuation": model.get("attenuation", "NONE"),
+        "pole": str(geo.get("lat_lon_north_pole_mesh", "SOURCE")),
         "abc": {"boundaries": list(abc.get("boundaries") or []),
                 "clayton": bool(abc.get("enable_Clayton_Enquist", False)),
                 "sponge": bool(kk.get("enable", False)),
@@ -1450,6 +1615,9 @@ def load_inparam(input_dir):
                        "integrator")},
         "source": {"latitude_longitude": list(loc["latitude_longitude"]),
                    "depth": float(loc["depth"]),
+                   "ellipticity": bool(loc.get("ellipticity", False)),
+                   "undulated_geometry": bool(loc.get("undulated_geometry",
+                                                      False)),
                    "mechanism": mech["type"],
                    "data": [float(x) for x in mech["data"]],
                    "unit": float(mech["unit"]),
@@ -1475,21 +1643,20 @@ def from_inparam(inp, ex, nsteps, receivers_s=None, receivers_z=None,
     expressions and the mesh period T0, Newmark or the symplectic scheme,
     one moment-tensor source on the axis with a GaussianSTF (depth below
     the mesh top, tensor data times unit), and receivers at (s, z, phi)
-    (the inputs' station groups are not read).  Raises NotImplementedError
-    for what the GPU path does not build from inputs (other Nr types,
-    attenuation, sources off the axis)."""
+    (the inputs' station groups are not read); attenuation NONE, CG4 or
+    FULL as from_exodus takes it.  Raises NotImplementedError for what the
+    GPU path does not build from inputs (other Nr types, sources off the
+    axis, ellipticity, undulated geometry)."""
     nr = inp["nr"]
     if nr["type"] != "CONSTANT":
         raise NotImplementedError("from_inparam: type_Nr other than "
                                   "CONSTANT")
-    if str(inp["attenuation"]).upper() != "NONE":
-        raise NotImplementedError("from_inparam: attenuation from an "
-                                  "Exodus mesh")
     const = float(nr["constant"])
     ta = inp["time_axis"]
     sp = from_exodus(ex, lambda s, z: np.full(np.shape(s), const),
                      courant=float(ta["Courant_number"]),
-                     bound_nr=nr["bound_by_inplane"])
+                     bound_nr=nr["bound_by_inplane"],
+                     attenuation=inp["attenuation"])
     abc = inp["abc"]
     if abc["boundaries"]:
         if abc["sponge"] and (abc["gamma_expr_solid"] != _GAMMA_SOLID_EXPR
@@ -1512,9 +1679,15 @@ def from_inparam(inp, ex, nsteps, receivers_s=None, receivers_z=None,
             "GaussianSTF":
         raise NotImplementedError("from_inparam: moment tensors with a "
                                   "GaussianSTF only")
-    if abs(float(src["latitude_longitude"][0]) - 90.0) > 1e-12:
+    # the mesh's north pole at the source puts it on the axis
+    if (abs(float(src["latitude_longitude"][0]) - 90.0) > 1e-12 and
+            inp.get("pole", "SOURCE").upper() != "SOURCE"):
         raise NotImplementedError("from_inparam: sources on the axis "
-                                  "(latitude 90) only")
+                                  "(latitude 90, or the mesh's pole at "
+                                  "the source) only")
+    if src.get("ellipticity") or src.get("undulated_geometry"):
+        raise NotImplementedError("from_inparam: ellipticity and "
+                                  "undulated geometry")
     ztop = float(ex.coords[:, 1].max())
     mt = np.asarray(src["data"]) * src["unit"]
     add_axial_moment_source(sp, ztop - src["depth"], mt,
@@ -1803,7 +1976,7 @@ def gll_colatitude(sz):
 
 
 def make_media(spec, mtype, field=None, mask3d=None, att=None, seed=0,
-               perturb=0.05):
+               perturb=0.05, params1d=None):
     """General solid media: per element material type mtype (0 isotropic, 1
     TI with vertical (radial) axis, 2 anisotropic), 1D or, where mask3d
     holds, 3D (the isotropic 1D model scaled per slice by (1 + d)^2 with
@@ -1811,8 +1984,12 @@ def make_media(spec, mtype, field=None, mask3d=None, att=None, seed=0,
     are the isotropic ones perturbed by +-perturb (seeded, per GLL point
     and slice).  Attenuation corrects every medium through its Voigt
     averages.  Spherical meshes get the GLL colatitudes for the RTZ
-    rotation of TI and anisotropic elements.  Fills spec.elem_mtype,
-    mat1d/mat1d_off, prop3d/prop_off, elem_kind/elem_slice_off, att."""
+    rotation of TI and anisotropic elements.  params1d, if given, holds
+    each 1D element's own parameters ((np, 25) per element: lambda, mu or
+    A, C, F, L, N or the 21 Cij) in place of the perturbed isotropic ones;
+    att may bring its SLS (w, y) (keys "w", "y") in place of the fit over
+    its band.  Fills spec.elem_mtype, mat1d/mat1d_off, prop3d/prop_off,
+    elem_kind/elem_slice_off, att."""
     E = spec.n_solid_elements
     mtype = np.asarray(mtype, dtype=np.uint8)
     if mask3d is None:
@@ -1832,7 +2009,11 @@ def make_media(spec, mtype, field=None, mask3d=None, att=None, seed=0,
     iso = spec.elem_iso.copy()
     sz = spec._gq["sz"]
     if att is not None:
-        w, y = att_sls_fit(att["nsls"], att["f_min"], att["f_max"])
+        if att.get("w") is not None:
+            w = np.asarray(att["w"], dtype=np.float64)
+            y = np.asarray(att["y"], dtype=np.float64)
+        else:
+            w, y = att_sls_fit(att["nsls"], att["f_min"], att["f_max"])
         consts = att_constants(w, y, att["f_min"], att["f_max"],
                                att["f_ref"])
         npa = 4 if att["cg4"] else NPEM
@@ -1845,15 +2026,19 @@ def make_media(spec, mtype, field=None, mask3d=None, att=None, seed=0,
         mt = int(mtype[e])
         lam, mu = iso[e, 0], iso[e, 1]
         if not mask3d[e]:
-            u = rng.uniform(-perturb, perturb, (npar[e], NPEM))
-            par = _iso_to_type(mt, lam, mu, u)
+            if params1d is not None:
+                par = np.array(params1d[e], dtype=np.float64)
+            else:
+                u = rng.uniform(-perturb, perturb, (npar[e], NPEM))
+                par = _iso_to_type(mt, lam, mu, u)
             if att is not None:
                 par, dl, dm = _att_correct(mt, par, att["qkp"][e],
                                            att["qmu"][e], consts,
                                            att["cg4"], wcg4[e])
                 d1d[e, 0], d1d[e, 1] = dl, dm
             mat1d[mat1d_off[e]:mat1d_off[e + 1]] = par.reshape(-1)
-            vmax[e] = spec._vp1d[e].max() * (grow if mt else 1.0)
+            vmax[e] = spec._vp1d[e].max() * (grow if mt and params1d is None
+                                             else 1.0)
             continue
         k = np.arange(nr[e])
         phi = 2 * math.pi * k / nr[e]

@@ -12,6 +12,7 @@ a structured Cartesian mesh read as unstructured quads gives the same
 domain, bit for bit up to the point numbering, and the same oracle
 seismograms."""
 import json
+import math
 import os
 
 import numpy as np
@@ -89,7 +90,8 @@ def test_salt_domain_from_inputs():
     assert len(sp.sponge_solid["point"]) > 0
     assert sp.src_elem >= 0 and np.abs(sp.stf).max() > 0
     with pytest.raises(NotImplementedError):
-        preloop.from_inparam(dict(inp, attenuation="CG4"), ex, 10)
+        preloop.from_inparam(dict(inp, nr=dict(inp["nr"], type="ANALYTICAL")),
+                             ex, 10)
 
 
 def _as_quads(mesh):
@@ -133,3 +135,161 @@ def test_quadmesh_equals_structured_mesh():
     sb = oracle.Oracle(b).run(80)
     assert np.abs(sa).max() > 0
     np.testing.assert_allclose(sb, sa, rtol=0, atol=1e-13 * np.abs(sa).max())
+
+
+def _structured_as_quads(mesh, perm=(0, 1, 2, 3)):
+    """A structured mesh's quads as an unstructured QuadMesh (spherical
+    elements of geometry type 0); perm reorders each element's corner
+    nodes cyclically (its side 0 moves to side perm[0])."""
+    na = mesh.nx + 1
+    ids = np.arange(mesh.nnodes)
+    a, b = mesh.node_a[ids % na], mesh.node_b[ids // na]
+    if mesh.spherical:
+        coords = np.stack([b * np.sin(a), b * np.cos(a)], axis=1)
+    else:
+        coords = np.stack([a, b], axis=1)
+    nodes = mesh.corner_nodes()[:, list(perm)]
+    ix, iz = mesh.element_ix_iz()
+    names = ("t0", "t1", "r0", "r1") if mesh.spherical else \
+        ("x0", "x1", "y0", "y1")
+    sides = {names[0]: {int(e): 3 for e in np.nonzero(ix == 0)[0]},
+             names[1]: {int(e): 1 for e in np.nonzero(ix == mesh.nx - 1)[0]},
+             names[2]: {int(e): 0 for e in np.nonzero(iz == 0)[0]},
+             names[3]: {int(e): 2 for e in np.nonzero(iz == mesh.nz - 1)[0]}}
+    gt = np.zeros(mesh.nelem, np.int64) if mesh.spherical else None
+    return preloop.QuadMesh(coords, nodes, sides, geometry_type=gt,
+                            spherical=mesh.spherical)
+
+
+def test_spherical_quads_equal_structured_sector():
+    mesh = preloop.StructuredMesh(np.linspace(0, 0.05, 4),
+                                  np.linspace(6311e3, 6371e3, 4),
+                                  spherical=True)
+    qm = _structured_as_quads(mesh)
+    for ax in (False, True):
+        xe = preloop.spectral.xieta_element(ax)
+        sa, Ja = mesh.map_points(xe, ax)
+        sb, Jb = qm.map_points(xe, ax)
+        np.testing.assert_allclose(sb, sa, rtol=1e-13, atol=1e-6)
+        np.testing.assert_allclose(Jb, Ja, rtol=1e-12, atol=1e-6)
+    # the sector's volume per radian, int r^2 sin(theta) dr dtheta
+    gq = preloop.gradquad_operands(qm)
+    r0, r1, t1 = 6311e3, 6371e3, 0.05
+    vol = (r1 ** 3 - r0 ** 3) / 3 * (1 - math.cos(t1))
+    assert gq["ifact"].sum() == pytest.approx(vol, rel=1e-10)
+
+
+def test_curved_mappings_are_rotation_invariant():
+    # one spherical element with its corners listed from each of its four
+    # corners: the curved outer edge moves to sides 2, 1, 0, 3 and the
+    # mapped GLL points stay the same set
+    mesh = preloop.StructuredMesh(np.array([0.1, 0.14]),
+                                  np.array([6000e3, 6100e3]), spherical=True)
+    xe = preloop.spectral.xieta_element(False)
+    ref = None
+    for shift in range(4):
+        perm = [(k + shift) % 4 for k in range(4)]
+        qm = _structured_as_quads(mesh, perm)
+        sz, J = qm.map_points(xe, False)
+        pts = np.sort(np.round(sz[0].T, 3), axis=0)
+        if ref is None:
+            ref = pts
+        np.testing.assert_allclose(pts, ref, atol=2e-3)
+        det = J[0, :, 0, 0] * J[0, :, 1, 1] - J[0, :, 0, 1] * J[0, :, 1, 0]
+        assert np.all(det > 0)
+
+
+def test_semi_spherical_mapping_edges():
+    # straight inner edge, circular outer edge of radius r2
+    r2 = 6371e3
+    t2, t3 = 0.30, 0.25
+    nsz = np.array([[3000e3, 3400e3, r2 * math.sin(t2), r2 * math.sin(t3)],
+                    [5000e3, 4700e3, r2 * math.cos(t2), r2 * math.cos(t3)]])
+    qm = preloop.QuadMesh(nsz.T, np.array([[0, 1, 2, 3]]), {},
+                          geometry_type=np.array([2]), spherical=True)
+    xi = np.linspace(-1, 1, 7)
+    for eta, check in ((1.0, "outer"), (-1.0, "inner")):
+        sz, _ = qm.map_points(np.stack([xi, np.full_like(xi, eta)]), False)
+        if check == "outer":
+            np.testing.assert_allclose(np.hypot(sz[0, 0], sz[0, 1]), r2,
+                                       rtol=1e-14)
+        else:
+            w = (1 + xi) / 2
+            np.testing.assert_allclose(sz[0, 0], nsz[0, 0] * (1 - w) +
+                                       nsz[0, 1] * w, rtol=1e-14)
+            np.testing.assert_allclose(sz[0, 1], nsz[1, 0] * (1 - w) +
+                                       nsz[1, 1] * w, rtol=1e-14)
+    corners = np.array([[-1, 1, 1, -1], [-1, -1, 1, 1]], dtype=float)
+    sz, _ = qm.map_points(corners, False)
+    np.testing.assert_allclose(sz[0], nsz, rtol=1e-14)
+
+
+PREM_REGIONAL = ("/root/reference/examples/02_3d_crust_S362ANI_regional/"
+                 "input_share/regional_mesh__prem_ani__50s.e")
+PREM_GLOBAL = ("/root/reference/examples/01_S362ANI_EMC_global/input1D/"
+               "global_mesh__prem_ani__50s.e")
+
+
+def prem_global_mesh():
+    with np.load(os.path.join(GOLD, "prem_global_mesh.npz")) as a:
+        return exodus.ExodusMesh(arrays=dict(a))
+
+
+def prem_regional_mesh():
+    with np.load(os.path.join(GOLD, "prem_regional_mesh.npz")) as a:
+        return exodus.ExodusMesh(arrays=dict(a))
+
+
+@pytest.mark.skipif(not os.path.isfile(PREM_REGIONAL),
+                    reason="the reference tree is not on this machine")
+def test_prem_regional_reader_matches_fixture():
+    ex, fx = exodus.ExodusMesh(PREM_REGIONAL), prem_regional_mesh()
+    assert np.array_equal(ex.connectivity, fx.connectivity)
+    assert np.array_equal(ex.coords, fx.coords)
+    assert ex.side_sets == fx.side_sets
+    for k, v in fx.elem_vars.items():
+        assert np.array_equal(ex.elem_vars[k], v)
+
+
+def test_prem_regional_geometry_and_media():
+    ex = prem_regional_mesh()
+    assert not ex.cartesian and set(np.unique(ex.geometry_type)) == {0, 2}
+    qm = preloop.QuadMesh.from_exodus(ex)
+    gq = preloop.gradquad_operands(qm)
+    r = np.hypot(ex.coords[:, 0], ex.coords[:, 1])
+    tmax = float(np.arctan2(ex.coords[:, 0], ex.coords[:, 1]).max())
+    vol = (r.max() ** 3 - r.min() ** 3) / 3 * (1 - math.cos(tmax))
+    assert gq["ifact"].sum() == pytest.approx(vol, rel=1e-11)
+    sp = preloop.from_exodus(ex, lambda s, z: np.full(np.shape(s), 5.0),
+                             attenuation="CG4")
+    gl = ex.global_variables
+    # the mesh's own SLS, not a fit
+    assert np.array_equal(sp.att["w"], [gl[f"w_{k}"] for k in range(5)])
+    assert np.array_equal(sp.att["y"], [gl[f"y_{k}"] for k in range(5)])
+    # transversely isotropic elements: the ones whose velocities differ
+    vpv, vph = ex.nodal_variable("VPV"), ex.nodal_variable("VPH")
+    ti = np.abs(vpv - vph).max(axis=1) > 0
+    assert sp.elem_mtype.sum() == ti.sum() > 0
+    assert sp.elem_theta is not None
+
+
+@pytest.mark.skipif(not os.path.isfile(PREM_GLOBAL),
+                    reason="the reference tree is not on this machine")
+def test_prem_global_mesh_geometry():
+    # the whole sphere per radian of azimuth is 2 R^3 / 3; the solid-fluid
+    # normals (fluid to solid) integrate 2 r^2 outward on the core-mantle
+    # boundary and inward on the inner-core boundary
+    ex = exodus.ExodusMesh(PREM_GLOBAL)
+    qm = preloop.QuadMesh.from_exodus(ex)
+    gq = preloop.gradquad_operands(qm)
+    R = float(np.hypot(ex.coords[:, 0], ex.coords[:, 1]).max())
+    assert gq["ifact"].sum() == pytest.approx(2 * R ** 3 / 3, rel=1e-11)
+    sp = preloop.from_exodus(ex, lambda s, z: np.full(np.shape(s), 5.0))
+    sz = sp.solid_sz[sp.sf_solid]
+    rr = np.hypot(sz[:, 0], sz[:, 1])
+    radial = np.sum(sp.sf_normal * sz, axis=1) / rr
+    r_cmb, r_icb = rr.max(), rr.min()
+    assert np.allclose(np.sort(np.unique(np.round(rr, -2))),
+                       [round(r_icb, -2), round(r_cmb, -2)])
+    assert radial.sum() == pytest.approx(2 * (r_cmb ** 2 - r_icb ** 2),
+                                         rel=1e-9)
