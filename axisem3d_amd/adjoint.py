@@ -68,6 +68,50 @@ def with_model(spec, rho, vp, vs, vp3d=None, vs3d=None):
     return sp
 
 
+def with_media(spec, mat1d, rho, prop3d=None):
+    """A copy of a general-media spec (make_media: spec.elem_mtype,
+    spec.mat1d, spec.prop3d) with the material parameters mat1d (packed as
+    spec.mat1d: per element its type's parameters x 25), prop3d (packed as
+    spec.prop3d) and the point masses of the density rho (n_elements, 25)
+    at every element GLL point (GLLPoint.cpp:54-77).  The attenuation
+    state (dlambda, dmu) and the time step stay those of the spec."""
+    sp = copy.copy(spec)
+    sp.mat1d = np.ascontiguousarray(mat1d, dtype=np.float64)
+    if prop3d is not None:
+        sp.prop3d = np.ascontiguousarray(prop3d, dtype=np.float64)
+    rho = np.asarray(rho, dtype=np.float64)
+    ifac = np.asarray(spec.elem_gq)[:, 5, :]
+    mass = np.zeros(len(spec.solid_nr))
+    np.add.at(mass, np.asarray(spec.elem_points).reshape(-1),
+              (rho * ifac).reshape(-1))
+    sp.solid_mass = mass
+    sp._rho1d = rho
+    return sp
+
+
+def forward_seismograms_media(spec, media, nsteps, precision="fp64"):
+    """Seismograms of the general media (mat1d, rho, prop3d) from rest."""
+    g = GpuSolver(with_media(spec, *media), precision, max_samples=nsteps)
+    try:
+        g.run(nsteps)
+        return g.seismograms(nsteps)
+    finally:
+        g.close()
+
+
+def gradient_media(spec, media, djdr, precision="fp64",
+                   checkpoint_interval=None):
+    """dJ/d(mat1d) (packed as spec.mat1d), dJ/drho (n_elements, 25) at
+    fixed moduli and dJ/d(prop3d) (packed as spec.prop3d, or None) of the
+    general media (mat1d, rho, prop3d) for the misfit derivative djdr
+    (axs3d_adjoint_run_media)."""
+    g = GpuSolver(with_media(spec, *media), precision, max_samples=len(djdr))
+    try:
+        return g.adjoint_run_media(djdr, checkpoint_interval)
+    finally:
+        g.close()
+
+
 def _slice_rho(spec, rho):
     """rho of each slice's GLL points, [slice][25] (3D elements)."""
     kind = np.asarray(spec.elem_kind)

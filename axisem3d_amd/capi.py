@@ -39,6 +39,7 @@ SIGNATURES = {
     "axs3d_set_axial_points": (_i, [_p, _i, _p]),
     "axs3d_set_solid_mass3d": (_i, [_p, _i, _p, _p]),
     "axs3d_set_ocean_load": (_i, [_p, _i, _p, _p, _p]),
+    "axs3d_set_solid_prt": (_i, [_p, _i, _p, _p]),
     "axs3d_set_fluid_kinds": (_i, [_p, _p]),
     "axs3d_set_fluid_k3d": (_i, [_p, _i, _i, _p]),
     "axs3d_set_element_kinds": (_i, [_p, _p]),
@@ -87,6 +88,7 @@ SIGNATURES = {
     "axs3d_add_moment_source": (_i, [_p, _i, _i, _p, _i, _p]),
     "axs3d_set_instances": (_i, [_p, _i, _p, _p]),
     "axs3d_adjoint_run": (_i, [_p, _i, _i, _p, _p, _p]),
+    "axs3d_adjoint_run_media": (_i, [_p, _i, _i, _p, _p, _p, _p]),
     "axs3d_adjoint_density": (_i, [_p, _p, _p, _p, _p]),
     "axs3d_read_seismograms": (_i, [_p, _p, _i]),
     "axs3d_num_modes": (_i, [_p, C.POINTER(C.c_int64)]),

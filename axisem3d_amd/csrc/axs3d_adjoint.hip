@@ -40,16 +40,17 @@ __device__ __forceinline__ double rdot(cpx<double> x, cpx<double> y) {
 }
 
 // One block per element, 10 Fourier orders x 25 GLL points per pass over
-// the element's orders; the three densities of each GLL point are summed
-// over the orders in a fixed order and added to kern[e][k][0..2].
-__global__ __launch_bounds__(STIFF_BLOCK) void k_sensitivity(SensArgs a) {
-    __shared__ cpx<double> sU[EMB][3][NPEM];
-    __shared__ cpx<double> sP[EMB][3][NPEM];
-    __shared__ double sG[2][NPEM];
-    __shared__ double red[EMB][NPEM][3];
+// the element's orders; the densities of each GLL point are summed over
+// the orders in a fixed order and added to the kernels: lambda, mu, rho
+// to kern[e][k][0..2] (isotropic domains), or the parameters of the
+// element's material type (mat_density) to kernm and rho to kernr.
+template <int MT>
+__device__ __forceinline__ void sens_element(
+    const SensArgs &a, int e, cpx<double> (*sU)[3][NPEM],
+    cpx<double> (*sP)[3][NPEM], const double (*sG)[NPEM],
+    double (*red)[NPEM]) {
+    constexpr int NP = MT == 0 ? 2 : MT == 1 ? 5 : 21;
     const int t = threadIdx.x;
-    if (t < 2 * NPEM) sG[t / NPEM][t % NPEM] = a.gtab[t];
-    const int e = blockIdx.x;
     const int slot = t / NPEM;
     const int ipnt = t - slot * NPEM;
     const bool lane_ok = slot < EMB;
@@ -57,13 +58,15 @@ __global__ __launch_bounds__(STIFF_BLOCK) void k_sensitivity(SensArgs a) {
     const bool axial = a.elem_axial[e] != 0;
     const bool axial_row0 = axial && (i == 0);
     const int nu1e = a.elem_nu1[e];
-    // 3D elements: only the mass density here (k_cd_sens does lambda, mu)
+    // 3D elements: only the mass density here (k_cd_sens does the moduli)
     const bool kind3d = a.elem_kind != nullptr && a.elem_kind[e] == 1;
     const int p = lane_ok ? a.elem_points[e * NPEM + ipnt] : 0;
     const int nu1p = lane_ok ? a.point_nu1[p] : 0;
     const size_t poff = lane_ok ? a.point_off[p] : 0;
     PointOps<double> g{};
     double ifac = 0.;
+    double rs1 = 0., rc1 = 1., rs2 = 0., rc2 = 1.;
+    const bool rtz = MT != 0 && a.rot != nullptr;
     if (lane_ok) {
         const double *gq = a.gq + (size_t)e * 6 * NPEM + ipnt;
         g.dsdxii = gq[0];
@@ -72,11 +75,20 @@ __global__ __launch_bounds__(STIFF_BLOCK) void k_sensitivity(SensArgs a) {
         g.dzdeta = gq[3 * NPEM];
         g.invs = gq[4 * NPEM];
         ifac = gq[5 * NPEM];
+        if (rtz) {
+            const double *r = a.rot + (size_t)e * 4 * NPEM + ipnt;
+            rs1 = r[0];
+            rc1 = r[NPEM];
+            rs2 = r[2 * NPEM];
+            rc2 = r[3 * NPEM];
+        }
     }
-    __syncthreads();
     PointG<double> G;
     load_point_g<double>(sG, axial, i, j, G);
-    double kl = 0., km = 0., kr = 0.;
+    double acc[NP];
+#pragma unroll
+    for (int c = 0; c < NP; c++) acc[c] = 0.;
+    double kr = 0.;
     for (int a0 = 0; a0 < nu1e; a0 += EMB) {
         const int alpha = a0 + slot;
         const bool act = lane_ok && alpha < nu1e;
@@ -107,38 +119,56 @@ __global__ __launch_bounds__(STIFF_BLOCK) void k_sensitivity(SensArgs a) {
                                 G, g, eu);
             grad6_point<double>(sP[slot], ps, (double)alpha, axial_row0, i,
                                 j, G, g, ep);
-            const cpx<double> tu = cadd(cadd(eu[0], eu[1]), eu[2]);
-            const cpx<double> tp = cadd(cadd(ep[0], ep[1]), ep[2]);
             if (!kind3d) {
-                kl += w * rdot(tp, tu);
-                km += w * (2. * (rdot(ep[0], eu[0]) + rdot(ep[1], eu[1]) +
-                                 rdot(ep[2], eu[2])) +
-                           rdot(ep[3], eu[3]) + rdot(ep[4], eu[4]) +
-                           rdot(ep[5], eu[5]));
+                if (rtz) {
+                    rot_spz_rtz6<cpx<double>, double>(eu, rs1, rc1, rs2, rc2);
+                    rot_spz_rtz6<cpx<double>, double>(ep, rs1, rc1, rs2, rc2);
+                }
+                mat_density<MT>(ep, eu, rdot,
+                                [&](int c, double v) { acc[c] += w * v; });
             }
             kr += w * (rdot(ps[0], ac[0]) + rdot(ps[1], ac[1]) +
                        rdot(ps[2], ac[2]));
         }
         __syncthreads();
     }
-    if (lane_ok) {
-        red[slot][ipnt][0] = kl * ifac;
-        red[slot][ipnt][1] = km * ifac;
-        red[slot][ipnt][2] = kr * ifac;
-    }
-    __syncthreads();
-    if (t < NPEM) {
-        double s0 = 0., s1 = 0., s2 = 0.;
-        for (int s = 0; s < EMB; s++) {
-            s0 += red[s][t][0];
-            s1 += red[s][t][1];
-            s2 += red[s][t][2];
+    // sum over the order slots, one value at a time: the moduli, then rho
+#pragma unroll
+    for (int c = 0; c <= NP; c++) {
+        if (lane_ok) red[slot][ipnt] = (c < NP ? acc[c] : kr) * ifac;
+        __syncthreads();
+        if (t < NPEM) {
+            double sum = 0.;
+            for (int s = 0; s < EMB; s++) sum += red[s][t];
+            double *dst;
+            if (a.kernm == nullptr)
+                dst = a.kern + ((size_t)e * NPEM + t) * 3 + (c < NP ? c : 2);
+            else if (c < NP)
+                dst = a.kernm + a.mat_off[e] + (size_t)c * NPEM + t;
+            else
+                dst = a.kernr + (size_t)e * NPEM + t;
+            *dst += a.scale * sum;
         }
-        double *k = a.kern + ((size_t)e * NPEM + t) * 3;
-        k[0] += a.scale * s0;
-        k[1] += a.scale * s1;
-        k[2] += a.scale * s2;
+        __syncthreads();
     }
+}
+
+__global__ __launch_bounds__(STIFF_BLOCK) void k_sensitivity(SensArgs a) {
+    __shared__ cpx<double> sU[EMB][3][NPEM];
+    __shared__ cpx<double> sP[EMB][3][NPEM];
+    __shared__ double sG[2][NPEM];
+    __shared__ double red[EMB][NPEM];
+    const int t = threadIdx.x;
+    if (t < 2 * NPEM) sG[t / NPEM][t % NPEM] = a.gtab[t];
+    __syncthreads();
+    const int e = blockIdx.x;
+    const int mt = a.elem_mtype != nullptr ? a.elem_mtype[e] : 0;
+    if (mt == 0)
+        sens_element<0>(a, e, sU, sP, sG, red);
+    else if (mt == 1)
+        sens_element<1>(a, e, sU, sP, sG, red);
+    else
+        sens_element<2>(a, e, sU, sP, sG, red);
 }
 
 // The adjoint force of one step, R^* G / dt^2, into the receivers'

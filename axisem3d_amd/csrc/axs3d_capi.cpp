@@ -240,6 +240,13 @@ int axs3d_set_ocean_load(axs3d_ctx *ctx, int n, const int32_t *points,
     });
 }
 
+int axs3d_set_solid_prt(axs3d_ctx *ctx, int n, const int32_t *elems,
+                        const double *x) {
+    return setup(ctx, "Domain::set_solid_prt", [&](axs3d::DomainBase &d) {
+        d.setSolidPRT(n, elems, x);
+    });
+}
+
 int axs3d_set_moment_source(axs3d_ctx *ctx, int elem, int nu1,
                             const double *pattern, int nsteps,
                             const double *stf) {
@@ -426,6 +433,18 @@ int axs3d_adjoint_run(axs3d_ctx *ctx, int nsteps, int checkpoint_interval,
     return guarded(ctx, [&](axs3d::DomainBase &d) {
         if (!kernels) throw axs3d::Error("Domain::adjoint || Null output.");
         d.adjointRun(nsteps, checkpoint_interval, djdr, kernels, kernels3d);
+    });
+}
+
+int axs3d_adjoint_run_media(axs3d_ctx *ctx, int nsteps,
+                            int checkpoint_interval, const double *djdr,
+                            double *kernels_par, double *kernels_rho,
+                            double *kernels_par3d) {
+    return guarded(ctx, [&](axs3d::DomainBase &d) {
+        if (!kernels_par || !kernels_rho)
+            throw axs3d::Error("Domain::adjoint || Null output.");
+        d.adjointRunMedia(nsteps, checkpoint_interval, djdr, kernels_par,
+                          kernels_rho, kernels_par3d);
     });
 }
 

@@ -71,19 +71,21 @@ __device__ __forceinline__ void pack_pair(cpx<Real> *buf,
                                           const cpx<Real> *__restrict__ Y,
                                           int nr, int nu1, bool even,
                                           int lane) {
-    // each row entry k < nu1 is read once and gives slice k and, conjugated,
-    // its mirror nr - k; the mean and an even Nr's Nyquist entry are real
-    // (no mirror), the latter doubled
+    // branch-free: slice k takes row entry kk = k (k < nu1) or nr - k
+    // (conjugated); the mean and an even Nr's Nyquist entry are real,
+    // the latter doubled.  (Reading each entry once and writing its mirror
+    // too measured 2% slower on the regional workload.)
     const int nyq = even ? nu1 - 1 : -1;
 #pragma unroll 4
-    for (int k = lane; k < nu1; k += 64) {
-        const cpx<Real> x = X[k], y = Y[k];
+    for (int k = lane; k < nr; k += 64) {
+        const bool lo = k < nu1;
+        const int kk = lo ? k : nr - k;
+        const cpx<Real> x = X[kk], y = Y[kk];
         const bool real_only = k == 0 || k == nyq;
         const Real sc = k == nyq ? (Real)2 : (Real)1;
-        const Real sim = real_only ? (Real)0 : (Real)1;
+        const Real sim = real_only ? (Real)0 : (lo ? (Real)1 : (Real)-1);
         // xf = (sc x.re, sim x.im), yf likewise; buf = xf + i yf
         buf[pad(k)] = {sc * x.re - sim * y.im, sim * x.im + sc * y.re};
-        if (!real_only) buf[pad(nr - k)] = {x.re + y.im, y.re - x.im};
     }
     wave_sync();
 }
@@ -192,7 +194,13 @@ __device__ __forceinline__ void att_update(double *mvk, Real *dsk, int cs,
 // launch over all element-modes leaves those points out, which keeps the
 // axis code and its registers out of the kernel that does nearly all the
 // work.
-template <typename Real, bool ROT, bool AX>
+// PRT: elements with a particle relabelling transformation (an
+// axisymmetric undulation, one operand set for all slices) write the
+// undulated strain: the 9-component gradient, rotated to RTZ on spherical
+// meshes, through the inverse Jacobian.  The operands do not depend on
+// phi, so applying them to the Fourier coefficients equals applying them
+// slice by slice after the C2R (SolidElement.cpp:139-141).
+template <typename Real, bool ROT, bool AX, bool PRT>
 __global__ __launch_bounds__(CD_EMB_THREADS) void k_cd_grad(CardArgs<Real> a,
                                                          int em0, int em1) {
     __shared__ cpx<Real> sU[CD_EMB][3][NPEM];
@@ -230,11 +238,23 @@ __global__ __launch_bounds__(CD_EMB_THREADS) void k_cd_grad(CardArgs<Real> a,
     PointOps<Real> g{}, q{};
     load_point_ops(s.elem_ops, e, ipnt, g, q);
     cpx<Real> eps[6];
-    grad6_point<Real>(sU[eml], u, (Real)alpha, AX, i, j, G, g, eps);
-    if (ROT) {
-        const Real *r = s.rot + (size_t)e * 4 * NPEM + ipnt;
-        rot_spz_rtz6<cpx<Real>, Real>(eps, r[0], r[NPEM], r[2 * NPEM],
-                                      r[3 * NPEM]);
+    if (PRT && s.prt_idx[e] >= 0) {
+        cpx<Real> g9[9];
+        grad9_point<Real>(sU[eml], u, (Real)alpha, AX, i, j, G, g, g9);
+        if (s.rot != nullptr) {
+            const Real *r = s.rot + (size_t)e * 4 * NPEM + ipnt;
+            rot9<cpx<Real>, Real>(g9, r[0], r[NPEM], r[2 * NPEM],
+                                  r[3 * NPEM]);
+        }
+        prt_sph_to_und6<cpx<Real>, Real>(
+            g9, s.prt + (size_t)s.prt_idx[e] * 8 * NPEM + ipnt, eps);
+    } else {
+        grad6_point<Real>(sU[eml], u, (Real)alpha, AX, i, j, G, g, eps);
+        if (ROT) {
+            const Real *r = s.rot + (size_t)e * 4 * NPEM + ipnt;
+            rot_spz_rtz6<cpx<Real>, Real>(eps, r[0], r[NPEM], r[2 * NPEM],
+                                          r[3 * NPEM]);
+        }
     }
     const int nu1 = s.elem_nu1[e];
     cpx<Real> *dst = reinterpret_cast<cpx<Real> *>(a.scratch) +
@@ -246,7 +266,7 @@ __global__ __launch_bounds__(CD_EMB_THREADS) void k_cd_grad(CardArgs<Real> a,
 
 // C: quadrature of the element-modes [em0, em1) from the scratch stress
 // rows (rotated back from RTZ) into the element-local force buffer.
-template <typename Real, bool ROT>
+template <typename Real, bool ROT, bool PRT>
 __global__ __launch_bounds__(CD_EMB_THREADS) void k_cd_quad(CardArgs<Real> a,
                                                          int em0, int em1) {
     __shared__ cpx<Real> sX[CD_EMB][3][NPEM];
@@ -265,6 +285,7 @@ __global__ __launch_bounds__(CD_EMB_THREADS) void k_cd_quad(CardArgs<Real> a,
     bool axial = false;
     cpx<Real> sig[6];
     PointOps<Real> g{}, q{};
+    const Real *xp = nullptr;
     if (active) {
         const EMItem it = s.em_list[em];
         e = it.elem;
@@ -277,7 +298,9 @@ __global__ __launch_bounds__(CD_EMB_THREADS) void k_cd_quad(CardArgs<Real> a,
 #pragma unroll
         for (int d = 0; d < 6; d++)
             sig[d] = src[(size_t)d * NPEM * scr_row(nu1)];
-        if (ROT) {
+        if (PRT && s.prt_idx[e] >= 0)
+            xp = s.prt + (size_t)s.prt_idx[e] * 8 * NPEM + ipnt;
+        if (ROT && xp == nullptr) {
             const Real *r = s.rot + (size_t)e * 4 * NPEM + ipnt;
             rot_rtz_spz6<cpx<Real>, Real>(sig, r[0], r[NPEM], r[2 * NPEM],
                                           r[3 * NPEM]);
@@ -287,8 +310,30 @@ __global__ __launch_bounds__(CD_EMB_THREADS) void k_cd_quad(CardArgs<Real> a,
     PointG<Real> G;
     load_point_g<Real>(sG, axial, i, j, G);
     cpx<Real> F[3];
-    quad6_point<Real>(sig, (Real)alpha, axial && i == 0, q, sX[eml], sY[eml],
-                      ipnt, i, j, G, active, F);
+    if (PRT) {
+        cpx<Real> L[3];
+        if (active && xp != nullptr) {
+            // back through the transpose, from RTZ, and computeQuad9
+            // (SolidElement.cpp:145-159)
+            cpx<Real> f9[9];
+            prt_und_to_sph9<cpx<Real>, Real>(sig, xp + 4 * NPEM, f9);
+            if (s.rot != nullptr) {
+                const Real *r = s.rot + (size_t)e * 4 * NPEM + ipnt;
+                rot9<cpx<Real>, Real>(f9, -r[0], r[NPEM], -r[2 * NPEM],
+                                      r[3 * NPEM]);
+            }
+            quad9_prep<Real>(f9, (Real)alpha, axial && i == 0, q, sX[eml],
+                             sY[eml], ipnt, L);
+        } else if (active) {
+            quad6_prep<Real>(sig, (Real)alpha, axial && i == 0, q, sX[eml],
+                             sY[eml], ipnt, L);
+        }
+        __syncthreads();
+        if (active) quad6_finish<Real>(sX[eml], sY[eml], i, j, G, L, F);
+    } else {
+        quad6_point<Real>(sig, (Real)alpha, axial && i == 0, q, sX[eml],
+                          sY[eml], ipnt, i, j, G, active, F);
+    }
     if (active && alpha < s.point_nu1[s.elem_points[e * NPEM + ipnt]]) {
         Real *dst = s.eb + ((size_t)s.elem_eb_off[e] + (size_t)ipnt * nu1 +
                             alpha) * 6;
@@ -400,8 +445,7 @@ __global__ __launch_bounds__(CD_PT, CardOcc<Real>::value) void k_cd_stress(
         const int c = t / a.nsls, l = t - c * a.nsls;
         abg[c * CD_MAX_SLS + l] = a.abg[t];
     }
-    // the strain pair goes to the wave's own buffer while the tables load:
-    // both sets of global reads are in flight before the barrier
+    __syncthreads();
     const bool even = (nr & 1) == 0;
     const size_t rs = (size_t)NPEM * scr_row(nu1);
     cpx<Real> *r0 = reinterpret_cast<cpx<Real> *>(a.scratch) + a.scr_off[e] +
@@ -409,7 +453,6 @@ __global__ __launch_bounds__(CD_PT, CardOcc<Real>::value) void k_cd_stress(
     cpx<Real> *b = L.buf(q);
     pack_pair<Real>(b, r0 + 2 * q * rs, r0 + (2 * q + 1) * rs, nr, nu1, even,
                     lane);
-    __syncthreads();
     // both transforms from one call site (the inverse, the stress, then
     // the forward transform), which halves the FFT's code
 #pragma unroll 1
@@ -467,24 +510,23 @@ void launch_cd_grad(hipStream_t st, const CardArgs<Real> &a,
     const int gem = (bt.em_end - bt.em_begin + CD_EMB - 1) / CD_EMB;
     if (gem <= 0) return;
     const bool rot = bt.mtype != 0 && a.s.rot != nullptr;
-    if (rot)
-        hipLaunchKernelGGL((k_cd_grad<Real, true, false>), dim3(gem),
-                           dim3(CD_EMB_THREADS), 0, st, a, bt.em_begin,
-                           bt.em_end);
-    else
-        hipLaunchKernelGGL((k_cd_grad<Real, false, false>), dim3(gem),
-                           dim3(CD_EMB_THREADS), 0, st, a, bt.em_begin,
-                           bt.em_end);
+    const bool prt = a.s.prt_idx != nullptr;
+#define AXS3D_CDG(ROT_, AX_, G_, B_, E_)                                      \
+    do {                                                                      \
+        if (prt)                                                              \
+            hipLaunchKernelGGL((k_cd_grad<Real, ROT_, AX_, true>), dim3(G_),  \
+                               dim3(CD_EMB_THREADS), 0, st, a, B_, E_);       \
+        else                                                                  \
+            hipLaunchKernelGGL((k_cd_grad<Real, ROT_, AX_, false>), dim3(G_), \
+                               dim3(CD_EMB_THREADS), 0, st, a, B_, E_);       \
+    } while (0)
+    if (rot) AXS3D_CDG(true, false, gem, bt.em_begin, bt.em_end);
+    else AXS3D_CDG(false, false, gem, bt.em_begin, bt.em_end);
     const int gax = (bt.ax_end - bt.ax_begin + CD_EMB - 1) / CD_EMB;
     if (gax <= 0) return;
-    if (rot)
-        hipLaunchKernelGGL((k_cd_grad<Real, true, true>), dim3(gax),
-                           dim3(CD_EMB_THREADS), 0, st, a, bt.ax_begin,
-                           bt.ax_end);
-    else
-        hipLaunchKernelGGL((k_cd_grad<Real, false, true>), dim3(gax),
-                           dim3(CD_EMB_THREADS), 0, st, a, bt.ax_begin,
-                           bt.ax_end);
+    if (rot) AXS3D_CDG(true, true, gax, bt.ax_begin, bt.ax_end);
+    else AXS3D_CDG(false, true, gax, bt.ax_begin, bt.ax_end);
+#undef AXS3D_CDG
 }
 
 template <typename Real>
@@ -506,12 +548,22 @@ void launch_cd_quad(hipStream_t st, const CardArgs<Real> &a,
                     const CardBatch &bt) {
     const int gem = (bt.em_end - bt.em_begin + CD_EMB - 1) / CD_EMB;
     if (gem <= 0) return;
-    if (bt.mtype != 0 && a.s.rot != nullptr)
-        hipLaunchKernelGGL((k_cd_quad<Real, true>), dim3(gem),
+    const bool rot = bt.mtype != 0 && a.s.rot != nullptr;
+    const bool prt = a.s.prt_idx != nullptr;
+    if (rot && prt)
+        hipLaunchKernelGGL((k_cd_quad<Real, true, true>), dim3(gem),
+                           dim3(CD_EMB_THREADS), 0, st, a, bt.em_begin,
+                           bt.em_end);
+    else if (rot)
+        hipLaunchKernelGGL((k_cd_quad<Real, true, false>), dim3(gem),
+                           dim3(CD_EMB_THREADS), 0, st, a, bt.em_begin,
+                           bt.em_end);
+    else if (prt)
+        hipLaunchKernelGGL((k_cd_quad<Real, false, true>), dim3(gem),
                            dim3(CD_EMB_THREADS), 0, st, a, bt.em_begin,
                            bt.em_end);
     else
-        hipLaunchKernelGGL((k_cd_quad<Real, false>), dim3(gem),
+        hipLaunchKernelGGL((k_cd_quad<Real, false, false>), dim3(gem),
                            dim3(CD_EMB_THREADS), 0, st, a, bt.em_begin,
                            bt.em_end);
 }
@@ -1293,12 +1345,13 @@ void launch_fluid_defer(hipStream_t st, int phases,
 // (Parseval: the Fourier-space pairing of strain and stress equals the
 // mean over the slices of their product), in the layout of the host's
 // per-slice lambda, mu ([slice][2][25], slices at slice_off[e]).
-template <typename Real>
+template <typename Real, int MT>
 __global__ __launch_bounds__(CD_PT) void k_cd_sens(CardArgs<Real> a,
                                                    const Real *scrb, int w0,
                                                    const double *ifac,
                                                    double *kern3d,
                                                    double scale) {
+    constexpr int NP = MT == 0 ? 2 : MT == 1 ? 5 : 21;
     const int t = threadIdx.x;
     const int lane = t & 63;
     const int q = __builtin_amdgcn_readfirstlane(t >> 6);
@@ -1330,7 +1383,10 @@ __global__ __launch_bounds__(CD_PT) void k_cd_sens(CardArgs<Real> a,
     __syncthreads();
     const double f = scale * ifac[(size_t)e * 6 * NPEM + 5 * NPEM + pp] /
                      (double)nr;
-    double *K = kern3d + (size_t)a.slice_off[e] * 2 * NPEM;
+    // [slice][param][25] from the element's property offset (the strains
+    // of TI and anisotropic batches are in RTZ already, as k_cd_grad
+    // formed them)
+    double *K = kern3d + a.prop_off[e];
     for (int k = t; k < nr; k += CD_PT) {
         const int pk = pad(pos[k]);
         double ep[6], eu[6];
@@ -1342,12 +1398,10 @@ __global__ __launch_bounds__(CD_PT) void k_cd_sens(CardArgs<Real> a,
             eu[2 * c] = (double)zu.re;
             eu[2 * c + 1] = (double)zu.im;
         }
-        const double tp = ep[0] + ep[1] + ep[2], tu = eu[0] + eu[1] + eu[2];
-        const double dm = 2. * (ep[0] * eu[0] + ep[1] * eu[1] +
-                                ep[2] * eu[2]) +
-                          ep[3] * eu[3] + ep[4] * eu[4] + ep[5] * eu[5];
-        K[((size_t)k * 2) * NPEM + pp] += f * tp * tu;
-        K[((size_t)k * 2 + 1) * NPEM + pp] += f * dm;
+        mat_density<MT>(ep, eu, [](double x, double y) { return x * y; },
+                        [&](int c, double v) {
+                            K[((size_t)k * NP + c) * NPEM + pp] += f * v;
+                        });
     }
 }
 
@@ -1357,9 +1411,16 @@ void launch_cd_sens(hipStream_t st, const CardArgs<Real> &a, const Real *scrb,
                     double scale) {
     const int n = bt.end - bt.begin;
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_cd_sens<Real>, dim3(n * NPEM), dim3(CD_PT),
-                       fft_lds_bytes<Real>(bt.nr_max, 2 * CD_PW), st, a, scrb,
-                       bt.begin, ifac, kern3d, scale);
+    const size_t lds = fft_lds_bytes<Real>(bt.nr_max, 2 * CD_PW);
+    if (bt.mtype == 0)
+        hipLaunchKernelGGL((k_cd_sens<Real, 0>), dim3(n * NPEM), dim3(CD_PT),
+                           lds, st, a, scrb, bt.begin, ifac, kern3d, scale);
+    else if (bt.mtype == 1)
+        hipLaunchKernelGGL((k_cd_sens<Real, 1>), dim3(n * NPEM), dim3(CD_PT),
+                           lds, st, a, scrb, bt.begin, ifac, kern3d, scale);
+    else
+        hipLaunchKernelGGL((k_cd_sens<Real, 2>), dim3(n * NPEM), dim3(CD_PT),
+                           lds, st, a, scrb, bt.begin, ifac, kern3d, scale);
 }
 
 template <typename Real>

@@ -445,6 +445,13 @@ void DomainBase::setSolidMass3D(int n, const int32_t *points,
     mMass3DUserVals = hMass3D.size();
 }
 
+void DomainBase::setSolidPRT(int n, const int32_t *elems, const double *x) {
+    if (n < 0 || (n > 0 && (elems == nullptr || x == nullptr)))
+        throw Error("Domain::setSolidPRT || Invalid arguments.");
+    hPrtElems.assign(elems, elems + n);
+    hPrtX.assign(x, x + (size_t)n * 5 * NPEM);
+}
+
 void DomainBase::setOceanLoad(int n, const int32_t *points,
                               const int32_t *nval, const double *values) {
     if (hPointNr.empty())
@@ -979,6 +986,32 @@ void DomainBase::finalize() {
     }
     if (!hSrcElems.empty())
         mStfSteps = (int)(hStf.size() / hSrcElems.size());
+    // particle relabelling: elements in range, listed once, |J| > 0, and
+    // no moment source in an undulated element (its pattern would need
+    // undulatedToSpherical6_NoIntegration, SolidElement.cpp:311-331)
+    if (!hPrtElems.empty()) {
+        std::vector<uint8_t> seen(mNumElems, 0);
+        for (size_t i = 0; i < hPrtElems.size(); i++) {
+            const int e = hPrtElems[i];
+            if (e < 0 || e >= mNumElems || seen[e])
+                throw Error("Domain::finalize || PRT element out of range or "
+                            "listed twice.");
+            seen[e] = 1;
+            for (int k = 0; k < 5 * NPEM; k++)
+                if (!std::isfinite(hPrtX[i * 5 * NPEM + k]))
+                    throw Error("Domain::finalize || PRT operands must be "
+                                "finite.");
+            for (int k = 0; k < NPEM; k++)
+                if (!(hPrtX[(i * 5 + 4) * NPEM + k] > 0.))
+                    throw Error("Domain::finalize || Negative Jacobian or "
+                                "distorted element occurred (PRT |J| <= 0).");
+        }
+        for (int32_t e : hSrcElems)
+            if (seen[e])
+                throw Error("Domain::finalize || A moment source in an "
+                            "element with undulated geometry (PRT) is not "
+                            "supported.");
+    }
     // batched instances
     if (!hInstDt.empty()) {
         if ((int)hPointInst.size() != mNumPoints)
@@ -1434,21 +1467,24 @@ void DomainBase::captureStep() {
 }
 
 // ------------------------------------------------------------ adjoint
-void DomainBase::adjointCheck() const {
+void DomainBase::adjointCheck(bool media) const {
     requireFinalized("Domain::adjoint");
     bool iso = true;
     for (uint8_t t : hMType) iso = iso && t == 0;
+    // the media form takes transversely isotropic and anisotropic solids
+    if (media) iso = true;
     bool rotated = false;
     for (size_t i = 0; i < hRecRot.size(); i += 3)
         rotated = rotated || hRecRot[i] >= 0.;
     if (mNumFElems > 0 || mNumMass3D > 0 || !iso || !hInstDt.empty() ||
         !mAbcS.point.empty() || rotated || mScheme != 0 ||
-        !hOcean1DPts.empty())
+        !hOcean1DPts.empty() || !hPrtElems.empty())
         throw Error("Domain::adjoint || The adjoint run supports solid "
                     "elements with an isotropic elastic medium (1D or 3D, "
                     "with or without attenuation) and no fluid, 3D mass, "
                     "ocean load, batched instances, absorbing boundaries, "
-                    "rotated station frames or the symplectic scheme.");
+                    "rotated station frames, undulated geometry (PRT) or "
+                    "the symplectic scheme.");
 }
 
 void DomainBase::adjointOperands() {
@@ -1465,6 +1501,34 @@ void DomainBase::adjointOperands() {
     if (mNum3D > 0) {
         dAdjKind.alloc(hElemKind.size());
         dAdjKind.upload(hElemKind.data(), hElemKind.size(), mStream);
+    }
+    // the media form: per element type and parameter offset; the FP64
+    // sin, cos of theta and 2 theta for TI and anisotropic elements on
+    // spherical meshes
+    hAdjMatOff.assign(mNumElems, 0);
+    int64_t off = 0;
+    for (int e = 0; e < mNumElems; e++) {
+        hAdjMatOff[e] = off;
+        off += (int64_t)mat_nparams(hMType[e]) * NPEM;
+    }
+    mAdjMatTotal = off;
+    dAdjMType.alloc(hMType.size());
+    dAdjMType.upload(hMType.data(), hMType.size(), mStream);
+    dAdjMatOff.alloc(hAdjMatOff.size() * 8);
+    dAdjMatOff.upload(hAdjMatOff.data(), hAdjMatOff.size() * 8, mStream);
+    if (!hTheta.empty()) {
+        std::vector<double> rot((size_t)mNumElems * 4 * NPEM);
+        for (int e = 0; e < mNumElems; e++)
+            for (int k = 0; k < NPEM; k++) {
+                const double t = hTheta[(size_t)e * NPEM + k];
+                double *r = rot.data() + (size_t)e * 4 * NPEM + k;
+                r[0] = std::sin(t);
+                r[NPEM] = std::cos(t);
+                r[2 * NPEM] = std::sin(2. * t);
+                r[3 * NPEM] = std::cos(2. * t);
+            }
+        dAdjRot.alloc(rot.size() * sizeof(double));
+        dAdjRot.upload(rot.data(), rot.size() * sizeof(double), mStream);
     }
 }
 
@@ -1617,6 +1681,8 @@ public:
         a.elem_mat = dElemMat.as<Real>();
         a.mat_off = dMatOff.as<uint32_t>();
         a.rot = hTheta.empty() ? nullptr : dRot.as<Real>();
+        a.prt_idx = hPrtElems.empty() ? nullptr : dPrtIdx.as<int32_t>();
+        a.prt = hPrtElems.empty() ? nullptr : dPrt.as<Real>();
         for (int k = 0; k < 4; k++) a.em_type_begin[k] = mEMTypeBegin[k];
         for (int k = 0; k < 3; k++) a.em_ax_begin[k] = mEMAxBegin[k];
         a.point_off = dPointOff.as<uint32_t>();
@@ -2145,10 +2211,25 @@ public:
     // then one step with the adjoint force instead of the source.
     void adjointRun(int nsteps, int seg, const double *djdr, double *kern,
                     double *kern3d) override {
-        adjointCheck();
+        adjointImpl(nsteps, seg, djdr, false, kern, nullptr, kern3d);
+    }
+    void adjointRunMedia(int nsteps, int seg, const double *djdr,
+                         double *kern_par, double *kern_rho,
+                         double *kern_par3d) override {
+        adjointImpl(nsteps, seg, djdr, true, kern_par, kern_rho, kern_par3d);
+    }
+    // media = false: kern [e][25][3] (lambda, mu, rho), kern3d [slice][2]
+    // [25]; media = true: kern the packed parameters, kern_rho [e][25],
+    // kern3d the packed per-slice parameters
+    void adjointImpl(int nsteps, int seg, const double *djdr, bool media,
+                     double *kern, double *kern_rho, double *kern3d) {
+        adjointCheck(media);
         if (mNum3D > 0 && !kern3d)
             throw Error("Domain::adjoint || A domain with 3D elements needs "
                         "the per-slice output.");
+        if (media && (!kern || !kern_rho))
+            throw Error("Domain::adjoint || The media form needs the "
+                        "parameter and density outputs.");
         if (nsteps <= 0 || seg <= 0)
             throw Error("Domain::adjoint || The number of steps and the "
                         "checkpoint interval must be positive.");
@@ -2169,7 +2250,7 @@ public:
         const double need =
             (double)ckBytes * nck + (double)nf * (2.0 * seglen + 3.0) +
             (double)attBytes + (double)mCardScrMax * sizeof(cpx<Real>) +
-            (double)mNumSlices * 2 * NPEM * sizeof(double);
+            (double)mPropTotal * sizeof(double);
         if (need > 0.9 * (double)freeb) {
             std::ostringstream ss;
             ss << "Domain::adjoint || Checkpoints and segment fields need "
@@ -2188,10 +2269,16 @@ public:
         const size_t ng = (size_t)nsteps * mNumRec * 3;
         dAdjG.alloc(std::max<size_t>(ng, 1) * sizeof(double));
         if (ng > 0) dAdjG.upload(djdr, ng * sizeof(double), mStream);
-        const size_t nk = (size_t)mNumElems * NPEM * 3;
+        const size_t nk = media ? (size_t)mAdjMatTotal
+                                : (size_t)mNumElems * NPEM * 3;
         dAdjKern.alloc(nk * sizeof(double));
         dAdjKern.zero(mStream);
-        const size_t nk3 = (size_t)mNumSlices * 2 * NPEM;
+        const size_t nkr = (size_t)mNumElems * NPEM;
+        if (media) {
+            dAdjKernR.alloc(nkr * sizeof(double));
+            dAdjKernR.zero(mStream);
+        }
+        const size_t nk3 = (size_t)mPropTotal;
         if (mNum3D > 0) {
             dAdjKern3D.alloc(nk3 * sizeof(double));
             dAdjKern3D.zero(mStream);
@@ -2266,11 +2353,19 @@ public:
             }
             swapAdjointFields();
             for (int n = n1 - 1; n >= n0; n--) {
-                launch_sensitivity(
-                    mStream, sensArgs(dDispl.as<double>(),
-                                      su + (size_t)(n - n0) * nd,
-                                      sa + (size_t)(n - n0) * nd,
-                                      dAdjKern.as<double>(), -dt2));
+                SensArgs sa_n = sensArgs(dDispl.as<double>(),
+                                         su + (size_t)(n - n0) * nd,
+                                         sa + (size_t)(n - n0) * nd,
+                                         dAdjKern.as<double>(), -dt2);
+                if (media) {
+                    sa_n.elem_mtype = dAdjMType.as<uint8_t>();
+                    sa_n.mat_off = dAdjMatOff.as<int64_t>();
+                    sa_n.kernm = dAdjKern.as<double>();
+                    sa_n.kernr = dAdjKernR.as<double>();
+                    sa_n.rot = hTheta.empty() ? nullptr
+                                              : dAdjRot.as<double>();
+                }
+                launch_sensitivity(mStream, sa_n);
                 if (mNum3D > 0) {
                     // per-slice densities: strain rows of psi and u_n,
                     // batch by batch
@@ -2302,6 +2397,11 @@ public:
                                  nk * sizeof(double), hipMemcpyDeviceToHost,
                                  mStream),
                   "Domain::adjoint");
+        if (media)
+            hip_check(hipMemcpyAsync(kern_rho, dAdjKernR.as<double>(),
+                                     nkr * sizeof(double),
+                                     hipMemcpyDeviceToHost, mStream),
+                      "Domain::adjoint");
         if (mNum3D > 0)
             hip_check(hipMemcpyAsync(kern3d, dAdjKern3D.as<double>(),
                                      nk3 * sizeof(double),
@@ -2750,6 +2850,27 @@ private:
                 }
             dRot.alloc(rot.size() * sizeof(Real));
             dRot.upload(rot.data(), rot.size() * sizeof(Real), st);
+        }
+        if (!hPrtElems.empty()) {
+            // X0..X3 and X0..X3 times |J| in FP64, then cast (the PRT
+            // constructor, PRT.hpp:57-80)
+            const size_t n = hPrtElems.size();
+            std::vector<int32_t> idx(mNumElems, -1);
+            std::vector<Real> px(n * 8 * NPEM);
+            for (size_t i = 0; i < n; i++) {
+                idx[hPrtElems[i]] = (int32_t)i;
+                const double *x = hPrtX.data() + i * 5 * NPEM;
+                for (int c = 0; c < 4; c++)
+                    for (int k = 0; k < NPEM; k++) {
+                        px[(i * 8 + c) * NPEM + k] = (Real)x[c * NPEM + k];
+                        px[(i * 8 + 4 + c) * NPEM + k] =
+                            (Real)(x[c * NPEM + k] * x[4 * NPEM + k]);
+                    }
+            }
+            dPrtIdx.alloc(idx.size() * 4);
+            dPrtIdx.upload(idx.data(), idx.size() * 4, st);
+            dPrt.alloc(px.size() * sizeof(Real));
+            dPrt.upload(px.data(), px.size() * sizeof(Real), st);
         }
         std::vector<Real> gt(2 * NPEM);
         for (int k = 0; k < NPEM; k++) {

@@ -96,6 +96,11 @@ public:
     void setAttenuation1D(const double *dlam_dmu);
     void setAxialPoints(int n, const int32_t *points);
     void setSolidMass3D(int n, const int32_t *points, const double *mass);
+    // Particle relabelling transformation of undulated geometry on solid
+    // elements elems[n]: x[n][5][25] = X0, X1, X2, X3, XJ per GLL point,
+    // one set for all slices (Undulation::createPRT with an axisymmetric
+    // undulation, Undulation.cpp:124-168).
+    void setSolidPRT(int n, const int32_t *elems, const double *x);
     void setOceanLoad(int n, const int32_t *points, const int32_t *nval,
                       const double *values);
     void setMomentSource(int elem, int nu1, const double *pattern, int nsteps,
@@ -172,6 +177,13 @@ public:
     // dJ/d(lambda, mu, rho) at GLL point k of element e
     virtual void adjointRun(int nsteps, int seg, const double *djdr,
                             double *kern, double *kern3d) = 0;
+    // the adjoint run for any solid media: dJ/d(material parameters) packed
+    // as axs3d_set_solid_material1d takes them (every element, its type's
+    // parameters x 25), dJ/drho [e][25], and per slice of the 3D elements
+    // as axs3d_set_solid_material3d takes them
+    virtual void adjointRunMedia(int nsteps, int seg, const double *djdr,
+                                 double *kern_par, double *kern_rho,
+                                 double *kern_par3d) = 0;
     // sensitivity densities of one (psi, u, a) triple, FP64, kern[e][k][3]
     void adjointDensity(const double *psi, const double *u, const double *a,
                         double *kern);
@@ -245,6 +257,8 @@ protected:
     std::vector<int64_t> hMatOff;
     std::vector<uint8_t> hMType;        // 0 isotropic, 1 TI, 2 anisotropic
     std::vector<double> hTheta;         // [e][25] colatitude, or empty
+    std::vector<int32_t> hPrtElems;     // elements with a PRT
+    std::vector<double> hPrtX;          // [n][5][25] X0..X3, XJ
     int mEMTypeBegin[4] = {0, 0, 0, 0};
     int mEMAxBegin[3] = {0, 0, 0};
     // fluid
@@ -357,7 +371,7 @@ protected:
     DeviceBuffer dDispl, dVeloc, dAccel, dStiff;
     // device: elements
     DeviceBuffer dElemPoints, dElemAxial, dElemNu1, dElemOps, dElemMat,
-        dMatOff, dRot, dEMList, dGTab, dElemEBOff, dEB, dAdj;
+        dMatOff, dRot, dEMList, dGTab, dElemEBOff, dEB, dAdj, dPrtIdx, dPrt;
     // device: cardinal path
     DeviceBuffer dElemNr, dSliceOff, dPropOff, dList3D, dScrOff, dEM3D,
         dPosOff, dPosTab, dMass3DPts, dMass3DOff, dMass3DFlag, dInvMass3D,
@@ -425,7 +439,12 @@ protected:
         return b[i];
     }
     int mRecNu1Max = 0;
-    void adjointCheck() const;
+    void adjointCheck(bool media = false) const;
+    // media form of the adjoint: offsets of each element's parameters in
+    // the packed output, their total, the types and FP64 rotations
+    std::vector<int64_t> hAdjMatOff;
+    int64_t mAdjMatTotal = 0;
+    DeviceBuffer dAdjMType, dAdjMatOff, dAdjRot, dAdjKernR;
     void adjointOperands();
     SensArgs sensArgs(const double *psi, const double *u, const double *a,
                       double *kern, double scale) const;

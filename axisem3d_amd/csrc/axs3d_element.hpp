@@ -124,6 +124,111 @@ __device__ __forceinline__ void grad6_point(const cpx<Real> (*sU)[NPEM],
     }
 }
 
+// computeGrad9 at point (i, j) (GradientQuadrature.hpp:181-201): the full
+// gradient g[3 c + d] = d u_c / d x_d, x = (s, phi, z), for the particle
+// relabelling path; the same contractions as grad6_point with the s-z and
+// phi pairs kept apart.
+template <typename Real>
+__device__ __forceinline__ void grad9_point(const cpx<Real> (*sU)[NPEM],
+                                            const cpx<Real> (&u)[3],
+                                            Real ralpha, bool axial_row0,
+                                            int i, int j,
+                                            const PointG<Real> &G,
+                                            const PointOps<Real> &g,
+                                            cpx<Real> (&g9)[9]) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        cpx<Real> x = {0, 0}, y = {0, 0};
+#pragma unroll
+        for (int k = 0; k < NPED; k++) {
+            x = cfma(sU[c][k * NPED + j], G.gxi_col[k], x);
+            y = cfma(sU[c][i * NPED + k], G.g_col[k], y);
+        }
+        g9[3 * c] = cadd(cmul(x, g.dzdeta), cmul(y, g.dzdxii));
+        g9[3 * c + 2] = cadd(cmul(x, g.dsdeta), cmul(y, g.dsdxii));
+    }
+    if (!axial_row0) {
+        g9[1] = cmul(csub(ialpha(ralpha, u[0]), u[1]), g.invs);
+        g9[4] = cmul(cadd(ialpha(ralpha, u[1]), u[0]), g.invs);
+        g9[7] = cmul(ialpha(ralpha, u[2]), g.invs);
+    } else {
+        cpx<Real> xa[3] = {{0, 0}, {0, 0}, {0, 0}};
+        cpx<Real> ya[3] = {{0, 0}, {0, 0}, {0, 0}};
+#pragma unroll
+        for (int k = 0; k < NPED; k++) {
+            const cpx<Real> uc0 = sU[0][k * NPED + j];
+            const cpx<Real> uc1 = sU[1][k * NPED + j];
+            const cpx<Real> uc2 = sU[2][k * NPED + j];
+            const cpx<Real> ur0 = sU[0][k];
+            const cpx<Real> ur1 = sU[1][k];
+            const cpx<Real> ur2 = sU[2][k];
+            xa[0] = cfma(csub(ialpha(ralpha, uc0), uc1), G.gxi_col[k], xa[0]);
+            xa[1] = cfma(cadd(ialpha(ralpha, uc1), uc0), G.gxi_col[k], xa[1]);
+            xa[2] = cfma(ialpha(ralpha, uc2), G.gxi_col[k], xa[2]);
+            ya[0] = cfma(csub(ialpha(ralpha, ur0), ur1), G.g_col[k], ya[0]);
+            ya[1] = cfma(cadd(ialpha(ralpha, ur1), ur0), G.g_col[k], ya[1]);
+            ya[2] = cfma(ialpha(ralpha, ur2), G.g_col[k], ya[2]);
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            g9[3 * c + 1] = cadd(cmul(xa[c], g.dzdeta), cmul(ya[c], g.dzdxii));
+    }
+}
+
+// A gradient-shaped tensor g -> Q g Q^T, Q the rotation by the colatitude
+// in the (s, z) plane (CoordTransformSpherical::transformSPZ_RTZ9,
+// CoordTransformSpherical.hpp:102-141); with -sin it is the way back
+// (transformRTZ_SPZ9, :144-184).
+template <typename T, typename Real>
+__device__ __forceinline__ void rot9(T (&g)[9], Real s1, Real c1, Real s2,
+                                     Real c2) {
+    const T s08 = g[0] + g[8], d08 = g[0] - g[8];
+    const T s26 = g[2] + g[6], d26 = g[2] - g[6];
+    g[0] = (s08 + d08 * c2 - s26 * s2) * (Real)0.5;
+    g[2] = (d26 + s26 * c2 + d08 * s2) * (Real)0.5;
+    g[6] = g[2] - d26;
+    g[8] = s08 - g[0];
+    const T g1 = g[1], g3 = g[3];
+    g[1] = g1 * c1 - g[7] * s1;
+    g[7] = g[7] * c1 + g1 * s1;
+    g[3] = g3 * c1 - g[5] * s1;
+    g[5] = g[5] * c1 + g3 * s1;
+}
+
+// Particle relabelling (PRT::sphericalToUndulated6, PRT.hpp:327-340): the
+// undulated Voigt strain (engineering shears) of the spherical gradient
+// g with the inverse Jacobian X0..X3 at x[0], x[NPEM], ...
+template <typename T, typename Real>
+__device__ __forceinline__ void prt_sph_to_und6(const T (&g)[9],
+                                                const Real *x, T (&e)[6]) {
+    const Real X0 = x[0], X1 = x[NPEM], X2 = x[2 * NPEM], X3 = x[3 * NPEM];
+    e[0] = g[0] * X0 + g[2] * X1;
+    e[1] = g[4] * X0 + g[5] * X2;
+    e[2] = g[8] * X3;
+    e[3] = g[7] * X0 + g[8] * X2 + g[5] * X3;
+    e[4] = g[6] * X0 + g[8] * X1 + g[2] * X3;
+    e[5] = (g[3] + g[1]) * X0 + g[5] * X1 + g[2] * X2;
+}
+
+// PRT::undulatedToSpherical6 (PRT.hpp:374-389): the spherical tensor of
+// the undulated stress s with the scaled inverse Jacobian X_k |J| at
+// xj[0], xj[NPEM], ...; the transpose of prt_sph_to_und6.
+template <typename T, typename Real>
+__device__ __forceinline__ void prt_und_to_sph9(const T (&s)[6],
+                                                const Real *xj, T (&f)[9]) {
+    const Real Y0 = xj[0], Y1 = xj[NPEM], Y2 = xj[2 * NPEM];
+    const Real Y3 = xj[3 * NPEM];
+    f[0] = s[0] * Y0;
+    f[1] = s[5] * Y0;
+    f[2] = s[0] * Y1 + s[4] * Y3 + s[5] * Y2;
+    f[3] = f[1];
+    f[4] = s[1] * Y0;
+    f[5] = s[1] * Y2 + s[3] * Y3 + s[5] * Y1;
+    f[6] = s[4] * Y0;
+    f[7] = s[3] * Y0;
+    f[8] = s[2] * Y3 + s[3] * Y2 + s[4] * Y1;
+}
+
 // Isotropic Hooke's law (Isotropic::strainToStress, src/core/element/
 // material/elastic/anisotropy/Isotropic.hpp:155-172).
 template <typename Real>
@@ -176,6 +281,38 @@ __device__ __forceinline__ void aniso_hooke(const T (&e)[6], Get c,
     }
 }
 
+// Densities of the bilinear form psi . sigma_theta(u) with respect to the
+// parameters theta of material type MT (0: lambda, mu; 1: A, C, F, L, N;
+// 2: the 21 Cij in the order of aniso_hooke), for Voigt strains with
+// engineering shears: d(psi . C u)/d theta.  pr(x, y) is the pairing of
+// two strain components (x y for slices, Re conj(x) y for orders);
+// add(c, v) receives the density of parameter c.
+template <int MT, typename T, typename Pair, typename Add>
+__device__ __forceinline__ void mat_density(const T (&p)[6], const T (&u)[6],
+                                            Pair pr, Add add) {
+    if constexpr (MT == 0) {
+        const T tp = p[0] + p[1] + p[2], tu = u[0] + u[1] + u[2];
+        add(0, pr(tp, tu));
+        add(1, 2. * (pr(p[0], u[0]) + pr(p[1], u[1]) + pr(p[2], u[2])) +
+                   pr(p[3], u[3]) + pr(p[4], u[4]) + pr(p[5], u[5]));
+    } else if constexpr (MT == 1) {
+        const T p01 = p[0] + p[1], u01 = u[0] + u[1];
+        add(0, pr(p01, u01));
+        add(1, pr(p[2], u[2]));
+        add(2, pr(p01, u[2]) + pr(p[2], u01));
+        add(3, pr(p[3], u[3]) + pr(p[4], u[4]));
+        add(4, -2. * (pr(p[0], u[1]) + pr(p[1], u[0])) + pr(p[5], u[5]));
+    } else {
+        int c = 0;
+#pragma unroll
+        for (int i = 0; i < 6; i++)
+#pragma unroll
+            for (int j = i; j < 6; j++, c++)
+                add(c, i == j ? pr(p[i], u[i])
+                              : pr(p[i], u[j]) + pr(p[j], u[i]));
+    }
+}
+
 // (s, phi, z) -> (R, T, Z) for a Voigt strain with engineering shears and
 // back for a stress (CoordTransformSpherical::transformSPZ_RTZ6 /
 // transformRTZ_SPZ6, src/core/element/crd/CoordTransformSpherical.hpp:
@@ -222,6 +359,12 @@ __device__ __forceinline__ void quad_xy(cpx<Real> dfds, cpx<Real> dfdz,
 // them in the LDS tiles sX, sY, and returns the local 1/s term L
 // (GradientQuadrature.hpp:311-318).
 template <typename Real>
+__device__ __forceinline__ void quad_prep_core(
+    const cpx<Real> (&X)[3], const cpx<Real> (&Y)[3], const cpx<Real> (&b)[3],
+    bool axial_row0, const PointOps<Real> &q, cpx<Real> (*sX)[NPEM],
+    cpx<Real> (*sY)[NPEM], int ipnt, cpx<Real> (&L)[3]);
+
+template <typename Real>
 __device__ __forceinline__ void quad6_prep(
     const cpx<Real> (&sig)[6], Real alpha, bool axial_row0,
     const PointOps<Real> &q, cpx<Real> (*sX)[NPEM], cpx<Real> (*sY)[NPEM],
@@ -235,6 +378,36 @@ __device__ __forceinline__ void quad6_prep(
     b[0] = csub(ialpha(alpha, sig[5]), sig[1]);
     b[1] = cadd(ialpha(alpha, sig[1]), sig[5]);
     b[2] = ialpha(alpha, sig[3]);
+    quad_prep_core<Real>(X, Y, b, axial_row0, q, sX, sY, ipnt, L);
+}
+
+// The same first half for a 9-component tensor f (computeQuad9,
+// GradientQuadrature.hpp:258-288): (dFdS, dFdZ) = (f0, f2), (f3, f5),
+// (f6, f8); lower-order numerators i alpha f1 - f4, i alpha f4 + f1,
+// i alpha f7.
+template <typename Real>
+__device__ __forceinline__ void quad9_prep(
+    const cpx<Real> (&f)[9], Real alpha, bool axial_row0,
+    const PointOps<Real> &q, cpx<Real> (*sX)[NPEM], cpx<Real> (*sY)[NPEM],
+    int ipnt, cpx<Real> (&L)[3]) {
+    cpx<Real> X[3], Y[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) quad_xy(f[3 * c], f[3 * c + 2], q, X[c], Y[c]);
+    cpx<Real> b[3];
+    b[0] = csub(ialpha(alpha, f[1]), f[4]);
+    b[1] = cadd(ialpha(alpha, f[4]), f[1]);
+    b[2] = ialpha(alpha, f[7]);
+    quad_prep_core<Real>(X, Y, b, axial_row0, q, sX, sY, ipnt, L);
+}
+
+template <typename Real>
+__device__ __forceinline__ void quad_prep_core(
+    const cpx<Real> (&Xin)[3], const cpx<Real> (&Yin)[3],
+    const cpx<Real> (&b)[3], bool axial_row0, const PointOps<Real> &q,
+    cpx<Real> (*sX)[NPEM], cpx<Real> (*sY)[NPEM], int ipnt,
+    cpx<Real> (&L)[3]) {
+    cpx<Real> X[3] = {Xin[0], Xin[1], Xin[2]};
+    cpx<Real> Y[3] = {Yin[0], Yin[1], Yin[2]};
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         L[c] = cmul(b[c], -q.invs);

@@ -42,7 +42,13 @@ __device__ __forceinline__ int att_slot_fr(int p, int npa) {
 // element-mode list keeps the axial elements of each type at its end, so
 // that the axis code and its registers stay out of the launch that does
 // nearly all the work.
-template <typename Real, bool ATT, int MT, bool AX>
+// PRT: some elements carry a particle relabelling transformation
+// (SolidElement::displToStiff, SolidElement.cpp:115-159): theirs take the
+// 9-component gradient, rotated to RTZ on spherical meshes whatever the
+// material, to the undulated strain (one operand set for all orders: the
+// undulation is axisymmetric), and the stress back through the transpose
+// and computeQuad9.
+template <typename Real, bool ATT, int MT, bool AX, bool PRT>
 __global__ __launch_bounds__(STIFF_BLOCK) void k_solid_stiff_fr(
     StiffArgs<Real> a, int em0, int em1) {
     __shared__ cpx<Real> sU[EMB][3][NPEM];
@@ -88,22 +94,33 @@ __global__ __launch_bounds__(STIFF_BLOCK) void k_solid_stiff_fr(
     PointOps<Real> g{}, q{};
     const Real ralpha = (Real)alpha;
     const bool axial_row0 = axial && (i == 0);
+    const Real *xp = nullptr;
+    if (PRT && active && a.prt_idx[e] >= 0)
+        xp = a.prt + (size_t)a.prt_idx[e] * 8 * NPEM + ipnt;
+    Real rs1 = 0, rc1 = 1, rs2 = 0, rc2 = 1;
     if (active) {
         load_point_ops(a.elem_ops, e, ipnt, g, q);
         cpx<Real> eps[6];
-        grad6_point<Real>(sU[eml], u, ralpha, axial_row0, i, j, G, g, eps);
         const Real *mat = a.elem_mat + a.mat_off[e] + ipnt;
         // TI and anisotropic media work in RTZ on spherical meshes
         // (SolidElement.cpp:177-193)
-        const bool rtz = MT != 0 && a.rot != nullptr;
-        Real rs1 = 0, rc1 = 1, rs2 = 0, rc2 = 1;
-        if (rtz) {
+        const bool rtz = MT != 0 && a.rot != nullptr && xp == nullptr;
+        if ((rtz || xp != nullptr) && a.rot != nullptr) {
             const Real *r = a.rot + (size_t)e * 4 * NPEM + ipnt;
             rs1 = r[0];
             rc1 = r[NPEM];
             rs2 = r[2 * NPEM];
             rc2 = r[3 * NPEM];
-            rot_spz_rtz6<cpx<Real>, Real>(eps, rs1, rc1, rs2, rc2);
+        }
+        if (PRT && xp != nullptr) {
+            cpx<Real> g9[9];
+            grad9_point<Real>(sU[eml], u, ralpha, axial_row0, i, j, G, g, g9);
+            if (a.rot != nullptr)
+                rot9<cpx<Real>, Real>(g9, rs1, rc1, rs2, rc2);
+            prt_sph_to_und6<cpx<Real>, Real>(g9, xp, eps);
+        } else {
+            grad6_point<Real>(sU[eml], u, ralpha, axial_row0, i, j, G, g, eps);
+            if (rtz) rot_spz_rtz6<cpx<Real>, Real>(eps, rs1, rc1, rs2, rc2);
         }
         if constexpr (MT == 0) {
             iso_hooke<Real>(eps, mat[0], mat[NPEM], sig);
@@ -167,8 +184,25 @@ __global__ __launch_bounds__(STIFF_BLOCK) void k_solid_stiff_fr(
         if (rtz) rot_rtz_spz6<cpx<Real>, Real>(sig, rs1, rc1, rs2, rc2);
     }
     cpx<Real> F[3];
-    quad6_point<Real>(sig, ralpha, axial_row0, q, sX[emx], sY[emx], ipnt, i,
-                      j, G, active, F);
+    if (PRT) {
+        cpx<Real> L[3];
+        if (active && xp != nullptr) {
+            cpx<Real> f9[9];
+            prt_und_to_sph9<cpx<Real>, Real>(sig, xp + 4 * NPEM, f9);
+            if (a.rot != nullptr)
+                rot9<cpx<Real>, Real>(f9, -rs1, rc1, -rs2, rc2);
+            quad9_prep<Real>(f9, ralpha, axial_row0, q, sX[emx], sY[emx],
+                             ipnt, L);
+        } else if (active) {
+            quad6_prep<Real>(sig, ralpha, axial_row0, q, sX[emx], sY[emx],
+                             ipnt, L);
+        }
+        __syncthreads();
+        if (active) quad6_finish<Real>(sX[emx], sY[emx], i, j, G, L, F);
+    } else {
+        quad6_point<Real>(sig, ralpha, axial_row0, q, sX[emx], sY[emx], ipnt,
+                          i, j, G, active, F);
+    }
     if (active && alpha < nu1p) {
         Real *dst = a.eb + ((size_t)a.elem_eb_off[e] +
                             (size_t)ipnt * a.elem_nu1[e] + alpha) * 6;
@@ -668,16 +702,23 @@ void launch_solid_stiff_fr(hipStream_t st, const StiffArgs<Real> &a) {
             const int em0 = ax ? ea : eb, em1 = ax ? ee : ea;
             if (em1 <= em0) continue;
             const int grid = (em1 - em0 + EMB - 1) / EMB;
-#define AXS3D_FR(ATT_, MT_)                                                   \
+#define AXS3D_FR2(ATT_, MT_, PRT_)                                            \
     do {                                                                      \
         if (ax)                                                               \
-            hipLaunchKernelGGL((k_solid_stiff_fr<Real, ATT_, MT_, true>),     \
+            hipLaunchKernelGGL((k_solid_stiff_fr<Real, ATT_, MT_, true,       \
+                                                 PRT_>),                      \
                                dim3(grid), dim3(STIFF_BLOCK), 0, st, a, em0,  \
                                em1);                                          \
         else                                                                  \
-            hipLaunchKernelGGL((k_solid_stiff_fr<Real, ATT_, MT_, false>),    \
+            hipLaunchKernelGGL((k_solid_stiff_fr<Real, ATT_, MT_, false,      \
+                                                 PRT_>),                      \
                                dim3(grid), dim3(STIFF_BLOCK), 0, st, a, em0,  \
                                em1);                                          \
+    } while (0)
+#define AXS3D_FR(ATT_, MT_)                                                   \
+    do {                                                                      \
+        if (a.prt_idx != nullptr) AXS3D_FR2(ATT_, MT_, true);                 \
+        else AXS3D_FR2(ATT_, MT_, false);                                     \
     } while (0)
             if (mt == 0) {
                 if (att) AXS3D_FR(true, 0); else AXS3D_FR(false, 0);
@@ -687,6 +728,7 @@ void launch_solid_stiff_fr(hipStream_t st, const StiffArgs<Real> &a) {
                 if (att) AXS3D_FR(true, 2); else AXS3D_FR(false, 2);
             }
 #undef AXS3D_FR
+#undef AXS3D_FR2
         }
     }
 }

@@ -87,6 +87,10 @@ struct StiffArgs {
     const uint32_t *mat_off;       // offset of element e in elem_mat
     const Real *rot;               // [e][4][25] sin, cos of theta, 2 theta
                                    // (spherical RTZ), or null
+    // particle relabelling of undulated geometry: per element its operand
+    // set in prt ([8][25]: X0..X3, X0..X3 times |J|) or -1; null: none
+    const int32_t *prt_idx = nullptr;
+    const Real *prt = nullptr;
     int em_type_begin[4];          // em_list grouped by material type
     int em_ax_begin[3];            // within type t, the axial elements'
                                    // element-modes from em_ax_begin[t]
@@ -422,14 +426,28 @@ struct SensArgs {
     const int32_t *point_nu1;
     const double *gtab;            // [2][25] G_GLL, G_GLJ
     const double *psi, *u, *a;     // [mode][3] complex
-    double *kern;
+    double *kern;                  // [e][25][3] lambda, mu, rho (kernm
+                                   // null: isotropic domains)
     double scale;
-    const uint8_t *elem_kind;      // 1: 3D element (lambda, mu per slice
+    const uint8_t *elem_kind;      // 1: 3D element (its moduli per slice
                                    // elsewhere), or null
+    // any media (kernm non-null): the densities of the parameters of each
+    // element's material type (elem_mtype; null: all isotropic) into kernm
+    // + mat_off[e] as [param][25], the mass density into kernr[e][25];
+    // TI and anisotropic elements on spherical meshes form both strains in
+    // RTZ with rot ([e][4][25] sin, cos of theta and of 2 theta; null:
+    // Cartesian)
+    const uint8_t *elem_mtype = nullptr;
+    const int64_t *mat_off = nullptr;
+    double *kernm = nullptr;
+    double *kernr = nullptr;
+    const double *rot = nullptr;
 };
 void launch_sensitivity(hipStream_t st, const SensArgs &a);
 // per-slice lambda, mu densities of the 3D elements of one batch, from the
 // strain rows of psi (a.scratch) and of u (scrb); see k_cd_sens
+// (the moduli of the batch's material type: kern3d + prop_off[e] as
+// [slice][param][25], the layout of axs3d_set_solid_material3d)
 template <typename Real>
 void launch_cd_sens(hipStream_t st, const CardArgs<Real> &a, const Real *scrb,
                     const CardBatch &bt, const double *ifac, double *kern3d,

@@ -979,6 +979,87 @@ def _at_point(x, ipnt, nr_point):
     return x[src, ipnt]
 
 
+def add_undulation(spec, dz, elems=None):
+    """Undulated geometry by particle relabelling (Undulation,
+    src/preloop/se_model/quad/physical/Undulation.cpp): every GLL point of
+    the solid elements `elems` (default: those where dz is not zero) moves
+    by dz along the vertical (Cartesian z, or the radius on spherical
+    meshes), axisymmetric (the same for all slices).  dz is an array
+    (n_elements, 25) at the element GLL points or a function of (s, z).
+
+    Per element: the gradient of dz (computeGrad3 of order 0, Undulation::
+    finished3D, :37-81), rotated to (R, Z) on spherical meshes; J1 = dZ/dR,
+    J2 = 0, J3 = 1 + dZ/dZ, J0 = 1 (Cartesian) or 1 + dz / r (r > 0, else
+    J3); X0 = 1 / J0, X1 = -J1 / (J0 J3), X2 = 0, X3 = 1 / J3, XJ = J0^2 J3
+    (createPRT, :124-168).  The point masses are multiplied by XJ
+    (getMassJacobian, :84-121).  Fills spec.prt = dict(elems, x) with x
+    (n, 5, 25) = X0, X1, X2, X3, XJ, and on spherical meshes the GLL
+    colatitudes spec.elem_theta for the RTZ rotation.  Solid-only domains
+    without absorbing boundaries, ocean load or 3D mass (their normals and
+    masses would need the undulation too)."""
+    if (len(getattr(spec, "fluid_nr", ())) > 0 or
+            getattr(spec, "_abc_abi", None) is not None or
+            getattr(spec, "ocean_load", None) is not None or
+            len(getattr(spec, "mass3d_points", ())) > 0):
+        raise ValueError("add_undulation: solid domains without absorbing "
+                         "boundaries, ocean load or 3D mass")
+    gq = spec._gq
+    sz = gq["sz"]
+    E = spec.n_solid_elements
+    if callable(dz):
+        dzv = np.asarray(dz(sz[:, 0], sz[:, 1]), dtype=np.float64)
+    else:
+        dzv = np.asarray(dz, dtype=np.float64)
+    dzv = np.broadcast_to(dzv, (E, NPEM))
+    if elems is None:
+        elems = np.nonzero(np.any(dzv != 0.0, axis=1))[0]
+    elems = np.asarray(elems, dtype=np.int64)
+    t = spectral.tables()
+    g_gll, g_glj = t[4], t[5]
+    spherical = bool(getattr(spec._mesh, "spherical", False))
+    theta = gll_colatitude(sz) if spherical else None
+    eg = np.asarray(spec.elem_gq)
+    x = np.zeros((len(elems), 5, NPEM))
+    jac = np.ones((E, NPEM))
+    for n, e in enumerate(elems):
+        F = dzv[e].reshape(NPED, NPED)
+        gxi = g_glj if spec.elem_axial[e] else g_gll
+        xx = (gxi.T @ F).reshape(-1)
+        yy = (F @ g_gll).reshape(-1)
+        dsdxii, dsdeta, dzdxii, dzdeta = eg[e, 0], eg[e, 1], eg[e, 2], eg[e, 3]
+        d_s = xx * dzdeta + yy * dzdxii
+        d_z = xx * dsdeta + yy * dsdxii
+        if spherical:
+            c, s = np.cos(theta[e]), np.sin(theta[e])
+            j1 = d_s * c - d_z * s
+            j3 = 1.0 + (d_z * c + d_s * s)
+            r = np.hypot(sz[e, 0], sz[e, 1])
+            j0 = np.where(r > np.finfo(float).eps,
+                          1.0 + dzv[e] / np.where(r > 0, r, 1.0), j3)
+        else:
+            j1 = d_s
+            j3 = 1.0 + d_z
+            j0 = np.ones(NPEM)
+        xj = j0 ** 2 * j3
+        if xj.min() < np.finfo(float).eps:
+            raise ValueError("add_undulation: negative Jacobian or distorted "
+                             "element")
+        x[n, 0] = 1.0 / j0
+        x[n, 1] = -j1 / (j0 * j3)
+        x[n, 3] = 1.0 / j3
+        x[n, 4] = xj
+        jac[e] = xj
+    rho = np.asarray(spec._rho1d, dtype=np.float64)
+    mass = np.zeros(len(spec.solid_nr))
+    np.add.at(mass, np.asarray(spec.elem_points).reshape(-1),
+              (rho * eg[:, 5, :] * jac).reshape(-1))
+    spec.solid_mass = mass
+    spec.prt = {"elems": elems.astype(np.int32), "x": x}
+    if spherical and getattr(spec, "elem_theta", None) is None:
+        spec.elem_theta = theta
+    return spec
+
+
 def add_ocean_load(spec, sum_rho_depth):
     """Ocean load on the solid points of the mesh top (Quad::setupGLL,
     Quad.cpp:235-246; GLLPoint::addOceanLoad and release, GLLPoint.hpp:

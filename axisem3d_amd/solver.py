@@ -180,6 +180,12 @@ class GpuSolver:
             self._chk(L.axs3d_set_ocean_load(ctx, len(op), op.ctypes.data,
                                              onv.ctypes.data, ov.ctypes.data),
                       "set_ocean_load")
+        prt = getattr(s, "prt", None)
+        if prt is not None and len(prt["elems"]) > 0:
+            pe = _c(prt["elems"], np.int32)
+            px = _c(prt["x"], np.float64)
+            self._chk(L.axs3d_set_solid_prt(ctx, len(pe), pe.ctypes.data,
+                                            px.ctypes.data), "set_solid_prt")
         if s.src_elem >= 0 and s.src_pattern is not None:
             pat = _c(s.src_pattern, np.complex128)
             stf = _c(s.stf, np.float64)
@@ -387,6 +393,32 @@ class GpuSolver:
             kern.ctypes.data, None if k3 is None else k3.ctypes.data),
             "adjoint_run")
         return kern, k3
+
+    def adjoint_run_media(self, djdr, checkpoint_interval=None):
+        """The adjoint run for any solid media (axs3d_adjoint_run_media):
+        dJ/d(material parameters) packed as spec.mat1d, dJ/drho per element
+        GLL point (n_elements, 25), and with 3D elements the per-slice
+        parameters packed as spec.prop3d (else None)."""
+        g = _c(djdr, np.float64)
+        nsteps = g.shape[0]
+        if checkpoint_interval is None:
+            checkpoint_interval = max(1, int(round(np.sqrt(nsteps))))
+        s = self.spec
+        mtype = getattr(s, "elem_mtype", None)
+        ne = len(s.elem_points)
+        npar = (np.full(ne, 2) if mtype is None else
+                np.array([2, 5, 21])[np.asarray(mtype)])
+        kp = np.zeros(int(npar.sum()) * 25)
+        kr = np.zeros((ne, 25))
+        kind = getattr(s, "elem_kind", None)
+        k3 = None
+        if kind is not None and np.any(np.asarray(kind) == 1):
+            k3 = np.zeros(np.asarray(s.prop3d).size)
+        self._chk(self.lib.axs3d_adjoint_run_media(
+            self.ctx, int(nsteps), int(checkpoint_interval), g.ctypes.data,
+            kp.ctypes.data, kr.ctypes.data,
+            None if k3 is None else k3.ctypes.data), "adjoint_run_media")
+        return kp, kr, k3
 
     def adjoint_density(self, psi, u, a):
         """Unscaled per-step sensitivity densities (axs3d_adjoint_density)

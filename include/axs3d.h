@@ -281,6 +281,20 @@ int axs3d_set_solid_mass3d(axs3d_ctx *ctx, int n, const int32_t *points,
 int axs3d_set_ocean_load(axs3d_ctx *ctx, int n, const int32_t *points,
                          const int32_t *nval, const double *values);
 
+/* Undulated geometry (topography, ellipticity, undulated interfaces) of
+ * solid elements elems[n] by the particle relabelling transformation:
+ * x[n][5][25] = X0, X1, X2, X3 (the inverse Jacobian of the relabelling,
+ * eq. 13 of Leng et al. 2019) and XJ (its |J|) at each GLL point, one set
+ * for all slices (an axisymmetric undulation).  Replaces the element's PRT
+ * (src/core/element/PRT/PRT.hpp:57-80, built by Undulation::createPRT,
+ * src/preloop/se_model/quad/physical/Undulation.cpp:124-168) and the PRT
+ * branch of SolidElement::displToStiff (SolidElement.cpp:115-159); the
+ * point masses passed to axs3d_set_solid_points already include |J|
+ * (Undulation::getMassJacobian).  Not with a moment source in such an
+ * element, nor with the adjoint run. */
+int axs3d_set_solid_prt(axs3d_ctx *ctx, int n, const int32_t *elems,
+                        const double *x);
+
 /* Moment-tensor source: element, Fourier pattern [nu1][6][25] complex and
  * the source-time function sampled at every step (FP64).  Replaces
  * SolidMoment (src/core/source/elem_src/solid/SolidMoment.cpp:15-33) with
@@ -457,6 +471,22 @@ int axs3d_set_graph(axs3d_ctx *ctx, int enable);
 int axs3d_adjoint_run(axs3d_ctx *ctx, int nsteps, int checkpoint_interval,
                       const double *djdr, double *kernels,
                       double *kernels3d);
+/* The same adjoint run for any solid media (no reference counterpart):
+ * isotropic, transversely isotropic and anisotropic elements, 1D or 3D,
+ * with or without attenuation.  kernels_par receives dJ/d(material
+ * parameters) of every element packed as axs3d_set_solid_material1d takes
+ * them (per element its type's parameters x 25: lambda, mu; A, C, F, L,
+ * N; or the 21 Cij), zero for 3D elements; kernels_rho ([n_elements][25])
+ * dJ/drho through the assembled mass; kernels_par3d (NULL without 3D
+ * elements) the per-slice parameters of the 3D elements packed as
+ * axs3d_set_solid_material3d takes them.  TI and anisotropic elements on
+ * spherical meshes take the derivatives in the RTZ frame their moduli are
+ * given in.  With attenuation the derivatives are those of the unrelaxed
+ * moduli at fixed (dlambda, dmu). */
+int axs3d_adjoint_run_media(axs3d_ctx *ctx, int nsteps,
+                            int checkpoint_interval, const double *djdr,
+                            double *kernels_par, double *kernels_rho,
+                            double *kernels_par3d);
 /* The per-step densities of axs3d_adjoint_run, unscaled, for one adjoint
  * field psi, forward displacement u and forward acceleration a (each laid
  * out as axs3d_get_field): kernels[e][k] = (<psi, dK/dlambda u>,

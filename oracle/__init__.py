@@ -77,6 +77,7 @@ class OracleDomain(C.Structure):
         ("n_sponge_f", C.c_int), ("sponge_f_point", _p),
         ("sponge_f_nval", _p), ("sponge_f_off", _p), ("sponge_f_data", _p),
         ("ocean_off", _p), ("ocean_nval", _p), ("ocean_data", _p),
+        ("prt_off", _p), ("prt_data", _p),
     ]
 
 
@@ -106,6 +107,11 @@ def _setup(L):
     L.oracle_att_apply_fr.argtypes = [C.POINTER(OracleDomain), C.c_int,
                                       C.c_int, _p, _p]
     L.oracle_spz_to_rtz6.argtypes = [_p, _p]
+    for name in ("oracle_grad9", "oracle_quad9"):
+        getattr(L, name).argtypes = [_p, C.c_int, C.c_int, _p, _p]
+    for name in ("oracle_spz_to_rtz9", "oracle_rtz_to_spz9",
+                 "oracle_prt_sph_to_und6", "oracle_prt_und_to_sph9"):
+        getattr(L, name).argtypes = [_p, _p, _p][:2 if "rtz" in name else 3]
     for name in ("oracle_grad3", "oracle_quad3"):
         getattr(L, name).argtypes = [_p, C.c_int, C.c_int, _p, _p]
     L.oracle_fluid_element_stiff.argtypes = [C.POINTER(OracleDomain),
@@ -200,6 +206,54 @@ def rtz_to_spz6(theta, sig):
     th = np.ascontiguousarray(theta, dtype=np.float64)
     lib().oracle_rtz_to_spz6(th.ctypes.data, s.ctypes.data)
     return s
+
+
+def grad9(gq_e, axial, alpha, u):
+    """computeGrad9: u (3, 25) complex -> (9, 25), g[3 i + j] = du_i/dx_j."""
+    u = np.ascontiguousarray(u, dtype=np.complex128)
+    g = np.zeros((9, 25), dtype=np.complex128)
+    lib().oracle_grad9(_ptr(gq_e), int(axial), int(alpha), _ptr(u), _ptr(g))
+    return g
+
+
+def quad9(gq_e, axial, alpha, f):
+    """computeQuad9: f (9, 25) complex -> (3, 25)."""
+    f = np.ascontiguousarray(f, dtype=np.complex128)
+    fi = np.zeros((3, 25), dtype=np.complex128)
+    lib().oracle_quad9(_ptr(gq_e), int(axial), int(alpha), _ptr(f), _ptr(fi))
+    return fi
+
+
+def spz_to_rtz9(theta, g):
+    g = np.array(g, dtype=np.complex128)
+    lib().oracle_spz_to_rtz9(_ptr(np.ascontiguousarray(theta, np.float64)),
+                             _ptr(g))
+    return g
+
+
+def rtz_to_spz9(theta, g):
+    g = np.array(g, dtype=np.complex128)
+    lib().oracle_rtz_to_spz9(_ptr(np.ascontiguousarray(theta, np.float64)),
+                             _ptr(g))
+    return g
+
+
+def prt_sph_to_und6(x, g9):
+    """PRT::sphericalToUndulated6 with x = (X0, X1, X2, X3) (4, 25)."""
+    g9 = np.ascontiguousarray(g9, dtype=np.complex128)
+    e6 = np.zeros((6, 25), dtype=np.complex128)
+    lib().oracle_prt_sph_to_und6(_ptr(np.ascontiguousarray(x, np.float64)),
+                                 _ptr(g9), _ptr(e6))
+    return e6
+
+
+def prt_und_to_sph9(xj, s6):
+    """PRT::undulatedToSpherical6 with xj = (X0 XJ, .., X3 XJ) (4, 25)."""
+    s6 = np.ascontiguousarray(s6, dtype=np.complex128)
+    f9 = np.zeros((9, 25), dtype=np.complex128)
+    lib().oracle_prt_und_to_sph9(_ptr(np.ascontiguousarray(xj, np.float64)),
+                                 _ptr(s6), _ptr(f9))
+    return f9
 
 
 def grad3(gq_e, axial, alpha, p):
@@ -464,6 +518,14 @@ class Oracle:
             d.ocean_nval = self._k(nv)
             d.ocean_data = self._k(np.ascontiguousarray(oc["data"],
                                                         dtype=np.float64))
+        # undulated geometry (preloop.add_undulation)
+        prt = getattr(spec, "prt", None)
+        if prt is not None and len(prt["elems"]) > 0:
+            off = np.full(len(spec.elem_points), -1, dtype=np.int64)
+            off[np.asarray(prt["elems"])] = np.arange(len(prt["elems"])) * 125
+            d.prt_off = self._k(off)
+            d.prt_data = self._k(np.ascontiguousarray(prt["x"],
+                                                      dtype=np.float64))
         d.dt = float(spec.dt)
         self.d = d
         self._att_state = None

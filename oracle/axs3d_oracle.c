@@ -123,6 +123,23 @@ void oracle_grad6(const double *gq_e, int axial, int alpha, const cplx *u,
     grad_1overs(&g, sA[2], e + 3 * N, 1);
 }
 
+/* computeGrad9, GradientQuadrature.hpp:181-201.  u[3][25] -> g[9][25],
+ * g[3 i + j] = d u_i / d x_j for x = (s, phi, z). */
+void oracle_grad9(const double *gq_e, int axial, int alpha, const cplx *u,
+                  cplx *g9) {
+    gq_t g = gq_of(gq_e, axial);
+    for (int c = 0; c < 3; c++)
+        grad_dsdz(&g, u + c * N, g9 + (3 * c) * N, g9 + (3 * c + 2) * N);
+    cplx sA[3][N];
+    cplx ai = (double)alpha * I;
+    for (int p = 0; p < N; p++) {
+        sA[0][p] = ai * u[0 * N + p] - u[1 * N + p];
+        sA[1][p] = ai * u[1 * N + p] + u[0 * N + p];
+        sA[2][p] = ai * u[2 * N + p];
+    }
+    for (int c = 0; c < 3; c++) grad_1overs(&g, sA[c], g9 + (3 * c + 1) * N, 0);
+}
+
 /* computeGrad3, GradientQuadrature.hpp:163-175: p[25] -> p_i[3][25] =
  * (dp/ds, i alpha p / s, dp/dz). */
 void oracle_grad3(const double *gq_e, int axial, int alpha, const cplx *p,
@@ -215,6 +232,32 @@ void oracle_quad6(const double *gq_e, int axial, int alpha, const cplx *sij,
         o[p] = g.inv_s[p] * g.ifact[p];
     }
     quad6_ops(axial, alpha, sij, fi, a, b, c, d, o);
+}
+
+/* computeQuad9, GradientQuadrature.hpp:258-288: f[9][25] -> fi[3][25],
+ * operators scaled by the integral factor. */
+void oracle_quad9(const double *gq_e, int axial, int alpha, const cplx *f,
+                  cplx *fi) {
+    gq_t g = gq_of(gq_e, axial);
+    double a[N], b[N], c[N], d[N], o[N];
+    for (int p = 0; p < N; p++) {
+        a[p] = g.dsdxii[p] * g.ifact[p];
+        b[p] = g.dsdeta[p] * g.ifact[p];
+        c[p] = g.dzdxii[p] * g.ifact[p];
+        d[p] = g.dzdeta[p] * g.ifact[p];
+        o[p] = g.inv_s[p] * g.ifact[p];
+    }
+    for (int k = 0; k < 3; k++)
+        quad_dsdz(axial, f + (3 * k) * N, f + (3 * k + 2) * N, fi + k * N,
+                  a, b, c, d);
+    cplx sA[3][N];
+    cplx ai = (double)alpha * I;
+    for (int p = 0; p < N; p++) {
+        sA[0][p] = ai * f[1 * N + p] - f[4 * N + p];
+        sA[1][p] = ai * f[4 * N + p] + f[1 * N + p];
+        sA[2][p] = ai * f[7 * N + p];
+    }
+    for (int k = 0; k < 3; k++) quad_1overs(axial, sA[k], fi + k * N, o, c, d);
 }
 
 /* computeQuad3, GradientQuadrature.hpp:234-252: q_i[3][25] -> q[25]. */
@@ -372,6 +415,84 @@ void oracle_rtz_to_spz6(const double *theta, cplx *s) {
         s[3 * N + p] = s3 * c1 - s[5 * N + p] * s1;
         s[5 * N + p] = s[5 * N + p] * c1 + s3 * s1;
     }
+}
+
+/* transformSPZ_RTZ9 / transformRTZ_SPZ9 of a gradient-shaped tensor g[9]
+ * (CoordTransformSpherical.hpp:102-184): g -> Q g Q^T with Q the rotation
+ * by the colatitude in the (s, z) plane, and back. */
+static void rot9(const double *theta, cplx *g, double sgn) {
+    for (int p = 0; p < N; p++) {
+        double s1 = sgn * sin(theta[p]), c1 = cos(theta[p]);
+        double s2 = sgn * sin(2. * theta[p]), c2 = cos(2. * theta[p]);
+        cplx *g0 = g + 0 * N + p, *g2 = g + 2 * N + p;
+        cplx *g6 = g + 6 * N + p, *g8 = g + 8 * N + p;
+        cplx s08 = *g0 + *g8, d08 = *g0 - *g8;
+        cplx s26 = *g2 + *g6, d26 = *g2 - *g6;
+        *g0 = (s08 + d08 * c2 - s26 * s2) * .5;
+        *g2 = (d26 + s26 * c2 + d08 * s2) * .5;
+        *g6 = *g2 - d26;
+        *g8 = s08 - *g0;
+        cplx g1 = g[1 * N + p], g3 = g[3 * N + p];
+        g[1 * N + p] = g1 * c1 - g[7 * N + p] * s1;
+        g[7 * N + p] = g[7 * N + p] * c1 + g1 * s1;
+        g[3 * N + p] = g3 * c1 - g[5 * N + p] * s1;
+        g[5 * N + p] = g[5 * N + p] * c1 + g3 * s1;
+    }
+}
+
+void oracle_spz_to_rtz9(const double *theta, cplx *g) { rot9(theta, g, 1.); }
+void oracle_rtz_to_spz9(const double *theta, cplx *g) { rot9(theta, g, -1.); }
+
+/* PRT::sphericalToUndulated6 (PRT.hpp:327-340): the undulated Voigt strain
+ * (engineering shears) from the spherical gradient g9 with the inverse
+ * Jacobian x = [X0, X1, X2, X3] (each [25]). */
+#define PRT_S2U(T, x, g, e)                                                   \
+    for (int p = 0; p < N; p++) {                                             \
+        double X0 = x[p], X1 = x[N + p], X2 = x[2 * N + p], X3 = x[3 * N + p];\
+        T G0 = g[0 * N + p], G1 = g[1 * N + p], G2 = g[2 * N + p];            \
+        T G3 = g[3 * N + p], G4 = g[4 * N + p], G5 = g[5 * N + p];            \
+        T G6 = g[6 * N + p], G7 = g[7 * N + p], G8 = g[8 * N + p];            \
+        e[0 * N + p] = G0 * X0 + G2 * X1;                                     \
+        e[1 * N + p] = G4 * X0 + G5 * X2;                                     \
+        e[2 * N + p] = G8 * X3;                                               \
+        e[3 * N + p] = G7 * X0 + G8 * X2 + G5 * X3;                           \
+        e[4 * N + p] = G6 * X0 + G8 * X1 + G2 * X3;                           \
+        e[5 * N + p] = (G3 + G1) * X0 + G5 * X1 + G2 * X2;                    \
+    }
+
+/* PRT::undulatedToSpherical6 (PRT.hpp:374-389): the spherical tensor f9
+ * of the undulated stress s6 with the scaled inverse Jacobian
+ * xj = [X0 XJ, X1 XJ, X2 XJ, X3 XJ]; f[3] = f[1]. */
+#define PRT_U2S(T, xj, s, f)                                                  \
+    for (int p = 0; p < N; p++) {                                             \
+        double Y0 = xj[p], Y1 = xj[N + p], Y2 = xj[2 * N + p];                \
+        double Y3 = xj[3 * N + p];                                            \
+        T S0 = s[0 * N + p], S1 = s[1 * N + p], S2 = s[2 * N + p];            \
+        T S3 = s[3 * N + p], S4 = s[4 * N + p], S5 = s[5 * N + p];            \
+        f[0 * N + p] = S0 * Y0;                                               \
+        f[1 * N + p] = S5 * Y0;                                               \
+        f[2 * N + p] = S0 * Y1 + S4 * Y3 + S5 * Y2;                           \
+        f[3 * N + p] = f[1 * N + p];                                          \
+        f[4 * N + p] = S1 * Y0;                                               \
+        f[5 * N + p] = S1 * Y2 + S3 * Y3 + S5 * Y1;                           \
+        f[6 * N + p] = S4 * Y0;                                               \
+        f[7 * N + p] = S3 * Y0;                                               \
+        f[8 * N + p] = S2 * Y3 + S3 * Y2 + S4 * Y1;                           \
+    }
+
+void oracle_prt_sph_to_und6(const double *x, const cplx *g9, cplx *e6) {
+    PRT_S2U(cplx, x, g9, e6)
+}
+
+void oracle_prt_und_to_sph9(const double *x, const cplx *s6, cplx *f9) {
+    PRT_U2S(cplx, x, s6, f9)
+}
+
+/* The scaled inverse Jacobian X_k XJ of a PRT operand set [5][25]
+ * (the PRT constructor, PRT.hpp:57-80). */
+static void prt_scaled(const double *x5, double *xj) {
+    for (int k = 0; k < 4; k++)
+        for (int p = 0; p < N; p++) xj[k * N + p] = x5[k * N + p] * x5[4 * N + p];
 }
 
 /* Material type of element e (Material::createElastic, src/preloop/
@@ -653,9 +774,97 @@ void oracle_solid_element_stiff(const oracle_domain *d, int e,
     free(f);
 }
 
+/* The forces of an element with a particle relabelling transformation
+ * (SolidElement::displToStiff, SolidElement.cpp:115-159): the 9-component
+ * spherical gradient, rotated to RTZ on spherical meshes, to the
+ * undulated strain per Fourier order (1D material) or per slice after
+ * the C2R of all nine series (3D material), the stress, back to the
+ * spherical tensor and its R2C, rotated back, and computeQuad9. */
+static void element_forces_prt(const oracle_domain *d, int e,
+                               const cplx *displ, cplx *fout) {
+    const int32_t *pts = d->elem_points + (int64_t)e * N;
+    int nr = d->elem_nr[e];
+    int nu1 = nr / 2 + 1;
+    const double *gq_e = d->elem_gq + (int64_t)e * 6 * N;
+    int axial = d->elem_axial[e];
+    int is3d = elem_is3d(d, e);
+    int mt = elem_mt(d, e);
+    const double *x5 = d->prt_data + d->prt_off[e];
+    const double *theta = d->elem_theta ? d->elem_theta + (int64_t)e * N
+                                        : NULL;
+    double xj[4 * N];
+    prt_scaled(x5, xj);
+    cplx *g9 = (cplx *)malloc(sizeof(cplx) * nu1 * 9 * N);
+    cplx *f9 = (cplx *)malloc(sizeof(cplx) * nu1 * 9 * N);
+    cplx u[3 * N];
+    for (int alpha = 0; alpha < nu1; alpha++) {
+        for (int p = 0; p < N; p++) {
+            int ip = pts[p];
+            int pnu1 = d->point_nr[ip] / 2 + 1;
+            for (int c = 0; c < 3; c++)
+                u[c * N + p] = alpha < pnu1
+                    ? displ[(d->offsets[ip] + alpha) * 3 + c] : 0.;
+        }
+        cplx *ga = g9 + (int64_t)alpha * 9 * N;
+        oracle_grad9(gq_e, axial, alpha, u, ga);
+        if (theta) oracle_spz_to_rtz9(theta, ga);
+    }
+    if (!is3d) {
+        const double *par = elem_par1d(d, e);
+        cplx e6[6 * N], s6[6 * N];
+        for (int alpha = 0; alpha < nu1; alpha++) {
+            const cplx *ga = g9 + (int64_t)alpha * 9 * N;
+            PRT_S2U(cplx, x5, ga, e6)
+            stress_fr(mt, par, e6, s6);
+            if (d->nsls > 0 && d->att1d) att_fr(d, e, alpha, e6, s6);
+            cplx *fa = f9 + (int64_t)alpha * 9 * N;
+            PRT_U2S(cplx, xj, s6, fa)
+        }
+    } else {
+        double *gcd = (double *)malloc(sizeof(double) * nr * 9 * N);
+        double *fcd = (double *)malloc(sizeof(double) * nr * 9 * N);
+        cplx *ser = (cplx *)malloc(sizeof(cplx) * nu1);
+        double *rser = (double *)malloc(sizeof(double) * nr);
+        for (int col = 0; col < 9 * N; col++) {
+            for (int a = 0; a < nu1; a++) ser[a] = g9[(int64_t)a * 9 * N + col];
+            oracle_c2r(ser, nr, rser);
+            for (int k = 0; k < nr; k++) gcd[(int64_t)k * 9 * N + col] = rser[k];
+        }
+        double e6[6 * N], s6[6 * N];
+        for (int k = 0; k < nr; k++) {
+            const double *gk = gcd + (int64_t)k * 9 * N;
+            double *fk = fcd + (int64_t)k * 9 * N;
+            PRT_S2U(double, x5, gk, e6)
+            stress_cd(mt, elem_par3d(d, e, k), e6, s6);
+            if (d->nsls > 0 && d->att3d) att_cd(d, e, k, e6, s6);
+            PRT_U2S(double, xj, s6, fk)
+        }
+        for (int col = 0; col < 9 * N; col++) {
+            for (int k = 0; k < nr; k++) rser[k] = fcd[(int64_t)k * 9 * N + col];
+            oracle_r2c(rser, nr, ser);
+            for (int a = 0; a < nu1; a++) f9[(int64_t)a * 9 * N + col] = ser[a];
+        }
+        free(gcd);
+        free(fcd);
+        free(ser);
+        free(rser);
+    }
+    for (int alpha = 0; alpha < nu1; alpha++) {
+        cplx *fa = f9 + (int64_t)alpha * 9 * N;
+        if (theta) oracle_rtz_to_spz9(theta, fa);
+        oracle_quad9(gq_e, axial, alpha, fa, fout + (int64_t)alpha * 3 * N);
+    }
+    free(g9);
+    free(f9);
+}
+
 /* The element's forces, fout[alpha][c][p] (before the subtraction). */
 static void element_forces(const oracle_domain *d, int e, const cplx *displ,
                            cplx *fout) {
+    if (d->prt_off && d->prt_off[e] >= 0) {
+        element_forces_prt(d, e, displ, fout);
+        return;
+    }
     const int32_t *pts = d->elem_points + (int64_t)e * N;
     int nr = d->elem_nr[e];
     int nu1 = nr / 2 + 1;

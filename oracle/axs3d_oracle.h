@@ -131,6 +131,12 @@ typedef struct {
     const int64_t *ocean_off;
     const int32_t *ocean_nval;
     const double *ocean_data;
+    /* particle relabelling transformation of undulated geometry (NULL:
+     * none): per solid element the offset of its operands in prt_data, or
+     * -1; one set for all slices (1D undulation), [5][25] X0, X1, X2, X3,
+     * XJ (PRT.hpp:57-80, Undulation::createPRT). */
+    const int64_t *prt_off;
+    const double *prt_data;
 } oracle_domain;
 
 void oracle_set_gmat(const double *g_gll, const double *g_glj);
@@ -146,6 +152,16 @@ int oracle_mat_nparams(int mt);
 void oracle_material_stress(int mt, const double *par,
                             const double complex *e, double complex *s);
 void oracle_spz_to_rtz6(const double *theta, double complex *e);
+void oracle_grad9(const double *gq_e, int axial, int alpha,
+                  const double complex *u, double complex *g);
+void oracle_quad9(const double *gq_e, int axial, int alpha,
+                  const double complex *f, double complex *fi);
+void oracle_spz_to_rtz9(const double *theta, double complex *g);
+void oracle_rtz_to_spz9(const double *theta, double complex *g);
+void oracle_prt_sph_to_und6(const double *x, const double complex *g9,
+                            double complex *e6);
+void oracle_prt_und_to_sph9(const double *x, const double complex *s6,
+                            double complex *f9);
 void oracle_rtz_to_spz6(const double *theta, double complex *s);
 void oracle_grad3(const double *gq_e, int axial, int alpha,
                   const double complex *p, double complex *pi);
