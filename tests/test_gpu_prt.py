@@ -121,3 +121,21 @@ def test_prt_seismograms_match_oracle(name, prec, tol):
     assert np.abs(ref).max() > 0
     assert err < tol
     g.close()
+
+
+def test_prt_refusals():
+    """A moment source inside an undulated element and the adjoint run of
+    an undulated domain are refused at the C ABI (the library raises
+    before any kernel runs)."""
+    from axisem3d_amd import capi
+    from axisem3d_amd.solver import GpuSolver
+    sp = _box()
+    src = int(sp.src_elem)
+    sp.prt = {"elems": np.concatenate([sp.prt["elems"], [src]]).astype(
+        np.int32), "x": np.concatenate([sp.prt["x"], sp.prt["x"][:1]])}
+    with pytest.raises(capi.Axs3dError, match="undulated"):
+        GpuSolver(sp, "fp64")
+    g = GpuSolver(_box(), "fp64")
+    with pytest.raises(capi.Axs3dError, match="PRT"):
+        g.adjoint_run(np.zeros((10, 4, 3)))
+    g.close()

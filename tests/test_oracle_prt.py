@@ -187,3 +187,17 @@ def test_undulation_refuses_a_distorted_element():
         assert "Jacobian" in str(exc)
     else:
         raise AssertionError("a folded element was accepted")
+
+
+def test_undulation_refuses_fluids_and_boundaries():
+    mesh = preloop.StructuredMesh(np.linspace(0, 4e3, 5),
+                                  np.linspace(0, 4.8e3, 5))
+    sp = preloop.build_domain(mesh, _mat, _nr, courant=0.4)
+    ix, iz = mesh.element_ix_iz()
+    preloop.make_fluid(sp, iz == 3)
+    with pytest.raises(ValueError, match="solid domains"):
+        preloop.add_undulation(sp, lambda s, z: 0.01 * z)
+    sp = _box()
+    preloop.add_absorbing(sp, keys=("RIGHT",))
+    with pytest.raises(ValueError, match="solid domains"):
+        preloop.add_undulation(sp, lambda s, z: 0.01 * z)
