@@ -986,9 +986,7 @@ void DomainBase::finalize() {
     }
     if (!hSrcElems.empty())
         mStfSteps = (int)(hStf.size() / hSrcElems.size());
-    // particle relabelling: elements in range, listed once, |J| > 0, and
-    // no moment source in an undulated element (its pattern would need
-    // undulatedToSpherical6_NoIntegration, SolidElement.cpp:311-331)
+    // particle relabelling: elements in range, listed once, |J| > 0
     if (!hPrtElems.empty()) {
         std::vector<uint8_t> seen(mNumElems, 0);
         for (size_t i = 0; i < hPrtElems.size(); i++) {
@@ -1006,11 +1004,6 @@ void DomainBase::finalize() {
                     throw Error("Domain::finalize || Negative Jacobian or "
                                 "distorted element occurred (PRT |J| <= 0).");
         }
-        for (int32_t e : hSrcElems)
-            if (seen[e])
-                throw Error("Domain::finalize || A moment source in an "
-                            "element with undulated geometry (PRT) is not "
-                            "supported.");
     }
     // batched instances
     if (!hInstDt.empty()) {
@@ -1955,12 +1948,18 @@ public:
             throw Error("Domain::applySources || Step outside the STF.");
         tick(AXS3D_KERNEL_SOURCE);
         const StiffArgs<Real> sa = stiffArgs();
-        for (size_t i = 0; i < hSrcElems.size(); i++)
+        for (size_t i = 0; i < hSrcElems.size(); i++) {
+            // the source element's PRT operands (X0..X3 unscaled), if any
+            const Real *prt_e = nullptr;
+            for (size_t k = 0; k < hPrtElems.size(); k++)
+                if (hPrtElems[k] == hSrcElems[i])
+                    prt_e = dPrt.as<Real>() + k * 8 * NPEM;
             launch_moment_source<Real>(
                 mStream, toElem, sa, hSrcElems[i], mSrcNu1c[i],
                 dSrcPattern.as<Real>() + 2 * hSrcPatOff[i],
                 dStf.as<double>() + i * (size_t)mStfSteps, tstep, mStfSteps,
-                dev ? dStepCtr.as<int>() : nullptr, dStiff.as<Real>());
+                dev ? dStepCtr.as<int>() : nullptr, dStiff.as<Real>(), prt_e);
+        }
         tock(AXS3D_KERNEL_SOURCE);
     }
 

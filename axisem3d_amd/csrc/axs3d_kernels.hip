@@ -231,7 +231,8 @@ __global__ __launch_bounds__(STIFF_BLOCK) void k_moment_source(
     const int32_t *__restrict__ point_nu1, Real *__restrict__ stiff,
     const uint32_t *__restrict__ elem_eb_off,
     const int32_t *__restrict__ elem_nu1, Real *__restrict__ eb,
-    const Real *__restrict__ gtab) {
+    const Real *__restrict__ gtab, const Real *__restrict__ prt_e,
+    const Real *__restrict__ rot_e) {
     __shared__ cpx<Real> sX[EMB][3][NPEM];
     __shared__ cpx<Real> sY[EMB][3][NPEM];
     __shared__ Real sG[2][NPEM];
@@ -268,10 +269,37 @@ __global__ __launch_bounds__(STIFF_BLOCK) void k_moment_source(
             sig[k] = {-re, -im};
         }
     }
-    cpx<Real> F[3];
-    quad6_point<Real>(sig, (Real)alpha, axial && i == 0, q,
-                      sX[active ? eml : 0], sY[active ? eml : 0], ipnt, i, j,
-                      G, active, F);
+    cpx<Real> F[3], L[3];
+    const int emx = active ? eml : 0;
+    if (active && prt_e != nullptr) {
+        // with a PRT (SolidElement.cpp:293-331): the stress to RTZ by the
+        // strain rotation with doubled shears, to the spherical tensor with
+        // the unscaled inverse Jacobian, back to SPZ, computeQuad9_
+        // NoIntegration
+        Real rs1 = 0, rc1 = 1, rs2 = 0, rc2 = 1;
+        if (rot_e != nullptr) {
+            rs1 = rot_e[ipnt];
+            rc1 = rot_e[NPEM + ipnt];
+            rs2 = rot_e[2 * NPEM + ipnt];
+            rc2 = rot_e[3 * NPEM + ipnt];
+#pragma unroll
+            for (int k = 3; k < 6; k++) sig[k] = sig[k] * (Real)2;
+            rot_spz_rtz6<cpx<Real>, Real>(sig, rs1, rc1, rs2, rc2);
+#pragma unroll
+            for (int k = 3; k < 6; k++) sig[k] = sig[k] * (Real)0.5;
+        }
+        cpx<Real> f9[9];
+        prt_und_to_sph9<cpx<Real>, Real>(sig, prt_e + ipnt, f9);
+        if (rot_e != nullptr)
+            rot9<cpx<Real>, Real>(f9, -rs1, rc1, -rs2, rc2);
+        quad9_prep<Real>(f9, (Real)alpha, axial && i == 0, q, sX[emx],
+                         sY[emx], ipnt, L);
+    } else if (active) {
+        quad6_prep<Real>(sig, (Real)alpha, axial && i == 0, q, sX[emx],
+                         sY[emx], ipnt, L);
+    }
+    __syncthreads();
+    if (active) quad6_finish<Real>(sX[emx], sY[emx], i, j, G, L, F);
     if (active) {
         const int p = elem_points[elem * NPEM + ipnt];
         if (alpha < point_nu1[p]) {
@@ -737,21 +765,23 @@ template <typename Real>
 void launch_moment_source(hipStream_t st, bool to_elem,
                           const StiffArgs<Real> &a, int elem, int nu1c,
                           const Real *pattern, const double *stf, int tstep,
-                          int nstf, const int *step_ctr, Real *stiff) {
+                          int nstf, const int *step_ctr, Real *stiff,
+                          const Real *prt_e) {
     if (nu1c <= 0) return;
     const int grid = (nu1c + EMB - 1) / EMB;
+    const Real *rot_e = a.rot ? a.rot + (size_t)elem * 4 * NPEM : nullptr;
     if (to_elem)
         hipLaunchKernelGGL((k_moment_source<Real, true>), dim3(grid),
                            dim3(STIFF_BLOCK), 0, st, elem, nu1c, pattern, stf,
                            tstep, nstf, step_ctr, a.elem_points, a.elem_axial, a.elem_ops,
                            a.point_off, a.point_nu1, stiff, a.elem_eb_off,
-                           a.elem_nu1, a.eb, a.gtab);
+                           a.elem_nu1, a.eb, a.gtab, prt_e, rot_e);
     else
         hipLaunchKernelGGL((k_moment_source<Real, false>), dim3(grid),
                            dim3(STIFF_BLOCK), 0, st, elem, nu1c, pattern, stf,
                            tstep, nstf, step_ctr, a.elem_points, a.elem_axial, a.elem_ops,
                            a.point_off, a.point_nu1, stiff, a.elem_eb_off,
-                           a.elem_nu1, a.eb, a.gtab);
+                           a.elem_nu1, a.eb, a.gtab, prt_e, rot_e);
 }
 
 template <typename Real>
@@ -877,7 +907,7 @@ void launch_flag_note(hipStream_t st, int *flag, int tstep) {
                                               const StiffArgs<Real> &);      \
     template void launch_moment_source<Real>(                                 \
         hipStream_t, bool, const StiffArgs<Real> &, int, int, const Real *,   \
-        const double *, int, int, const int *, Real *);                       \
+        const double *, int, int, const int *, Real *, const Real *);         \
     template void launch_assemble<Real>(hipStream_t, const PointGeom &,       \
                                         const Real *, Real *);                \
     template void launch_point<Real>(hipStream_t, int, const PointGeom &,     \

@@ -377,7 +377,8 @@ template <typename Real>
 void launch_moment_source(hipStream_t st, bool to_elem,
                           const StiffArgs<Real> &a, int elem, int nu1c,
                           const Real *pattern, const double *stf, int tstep,
-                          int nstf, const int *step_ctr, Real *stiff);
+                          int nstf, const int *step_ctr, Real *stiff,
+                          const Real *prt_e = nullptr);
 
 template <typename Real>
 void launch_assemble(hipStream_t st, const PointGeom &g, const Real *eb,

@@ -290,8 +290,9 @@ int axs3d_set_ocean_load(axs3d_ctx *ctx, int n, const int32_t *points,
  * src/preloop/se_model/quad/physical/Undulation.cpp:124-168) and the PRT
  * branch of SolidElement::displToStiff (SolidElement.cpp:115-159); the
  * point masses passed to axs3d_set_solid_points already include |J|
- * (Undulation::getMassJacobian).  Not with a moment source in such an
- * element, nor with the adjoint run. */
+ * (Undulation::getMassJacobian).  A moment source in such an element
+ * takes the PRT branch of SolidElement::addMomentSource (:293-331).  Not
+ * with the adjoint run. */
 int axs3d_set_solid_prt(axs3d_ctx *ctx, int n, const int32_t *elems,
                         const double *x);
 

@@ -236,17 +236,9 @@ void oracle_quad6(const double *gq_e, int axial, int alpha, const cplx *sij,
 
 /* computeQuad9, GradientQuadrature.hpp:258-288: f[9][25] -> fi[3][25],
  * operators scaled by the integral factor. */
-void oracle_quad9(const double *gq_e, int axial, int alpha, const cplx *f,
-                  cplx *fi) {
-    gq_t g = gq_of(gq_e, axial);
-    double a[N], b[N], c[N], d[N], o[N];
-    for (int p = 0; p < N; p++) {
-        a[p] = g.dsdxii[p] * g.ifact[p];
-        b[p] = g.dsdeta[p] * g.ifact[p];
-        c[p] = g.dzdxii[p] * g.ifact[p];
-        d[p] = g.dzdeta[p] * g.ifact[p];
-        o[p] = g.inv_s[p] * g.ifact[p];
-    }
+static void quad9_ops(int axial, int alpha, const cplx *f, cplx *fi,
+                      const double *a, const double *b, const double *c,
+                      const double *d, const double *o) {
     for (int k = 0; k < 3; k++)
         quad_dsdz(axial, f + (3 * k) * N, f + (3 * k + 2) * N, fi + k * N,
                   a, b, c, d);
@@ -258,6 +250,28 @@ void oracle_quad9(const double *gq_e, int axial, int alpha, const cplx *f,
         sA[2][p] = ai * f[7 * N + p];
     }
     for (int k = 0; k < 3; k++) quad_1overs(axial, sA[k], fi + k * N, o, c, d);
+}
+
+void oracle_quad9(const double *gq_e, int axial, int alpha, const cplx *f,
+                  cplx *fi) {
+    gq_t g = gq_of(gq_e, axial);
+    double a[N], b[N], c[N], d[N], o[N];
+    for (int p = 0; p < N; p++) {
+        a[p] = g.dsdxii[p] * g.ifact[p];
+        b[p] = g.dsdeta[p] * g.ifact[p];
+        c[p] = g.dzdxii[p] * g.ifact[p];
+        d[p] = g.dzdeta[p] * g.ifact[p];
+        o[p] = g.inv_s[p] * g.ifact[p];
+    }
+    quad9_ops(axial, alpha, f, fi, a, b, c, d, o);
+}
+
+/* computeQuad9_NoIntegration, GradientQuadrature.hpp:330-360 */
+static void quad9_noint(const double *gq_e, int axial, int alpha,
+                        const cplx *f, cplx *fi) {
+    gq_t g = gq_of(gq_e, axial);
+    quad9_ops(axial, alpha, f, fi, g.dsdxii, g.dsdeta, g.dzdxii, g.dzdeta,
+              g.inv_s);
 }
 
 /* computeQuad3, GradientQuadrature.hpp:234-252: q_i[3][25] -> q[25]. */
@@ -1006,7 +1020,25 @@ void oracle_apply_moment_source(const oracle_domain *d, double stf_value,
                 cplx pv = d->src_pattern[(int64_t)alpha * 6 * N + k];
                 sig[k] = -(pv * stf_value);
             }
-            oracle_quad6_noint(gq_e, axial, alpha, sig, f);
+            if (d->prt_off && d->prt_off[e] >= 0) {
+                /* SolidElement::addMomentSource with a PRT
+                 * (SolidElement.cpp:293-331): the stress to RTZ by the
+                 * strain rotation with doubled shears, to the spherical
+                 * tensor with the unscaled inverse Jacobian, back to SPZ,
+                 * computeQuad9_NoIntegration */
+                const double *theta = d->elem_theta
+                    ? d->elem_theta + (int64_t)e * N : NULL;
+                const double *x5 = d->prt_data + d->prt_off[e];
+                cplx f9[9 * N];
+                for (int k = 3 * N; k < 6 * N; k++) sig[k] *= 2.;
+                if (theta) oracle_spz_to_rtz6(theta, sig);
+                for (int k = 3 * N; k < 6 * N; k++) sig[k] *= .5;
+                PRT_U2S(cplx, x5, sig, f9)
+                if (theta) oracle_rtz_to_spz9(theta, f9);
+                quad9_noint(gq_e, axial, alpha, f9, f);
+            } else {
+                oracle_quad6_noint(gq_e, axial, alpha, sig, f);
+            }
         } else {
             for (int k = 0; k < 3 * N; k++) f[k] = 0.;
         }

@@ -4,7 +4,7 @@ against the FP64 oracle through the C ABI.
 
 Domains: a Cartesian box whose top two element rows are undulated
 (dz = a (z - z0)^2 cos(s / L), zero below z0) with 1D and 3D elements and
-5-SLS attenuation, and a spherical sector (isotropic, TI and anisotropic
+5-SLS attenuation, the same box undulated down past its moment source, and a spherical sector (isotropic, TI and anisotropic
 elements in turn, half of them 3D, CG4 attenuation) whose top row is
 undulated with an axisymmetric topography.  The GPU path applies the
 operands to the Fourier coefficients before the transform (they do not
@@ -26,7 +26,7 @@ pytestmark = pytest.mark.gpu
 NSTEPS = 300
 
 
-def _box():
+def _box(z0=4.0e3, a=0.08 / 1.6e3):
     mesh = preloop.StructuredMesh(np.linspace(0, 6e3, 7),
                                   np.linspace(0, 5.6e3, 8))
     sp = preloop.build_domain(
@@ -38,7 +38,6 @@ def _box():
     preloop.make_3d_isotropic(
         sp, lambda x, y, z: 0.03 * np.sin(x / 900.0) * np.cos(y / 700.0),
         att, (np.arange(E) % 2) == 1)
-    z0, a = 4.0e3, 0.08 / 1.6e3
     preloop.add_undulation(
         sp, lambda s, z: np.where(z > z0, a * (z - z0) ** 2, 0.0) *
         np.cos(s / 2.5e3))
@@ -74,7 +73,10 @@ def _sector():
     return sp
 
 
-SPECS = {"box": _box, "sector": _sector}
+SPECS = {"box": _box, "sector": _sector,
+         # the undulation reaches down past the source: the moment source
+         # takes the PRT branch (SolidElement.cpp:293-331)
+         "box_src": lambda: _box(1.6e3, 0.04 / 1.6e3)}
 
 
 def _rel(a, b):
@@ -110,6 +112,8 @@ def test_prt_stiffness_matches_oracle(name, prec, tol):
 def test_prt_seismograms_match_oracle(name, prec, tol):
     from axisem3d_amd.solver import GpuSolver
     sp = SPECS[name]()
+    assert (int(sp.src_elem) in set(sp.prt["elems"].tolist())) == (
+        name == "box_src")
     o = oracle.Oracle(sp)
     ref = o.run(NSTEPS)
     g = GpuSolver(sp, prec)
@@ -123,18 +127,11 @@ def test_prt_seismograms_match_oracle(name, prec, tol):
     g.close()
 
 
-def test_prt_refusals():
-    """A moment source inside an undulated element and the adjoint run of
-    an undulated domain are refused at the C ABI (the library raises
-    before any kernel runs)."""
+def test_prt_adjoint_is_refused():
+    """The adjoint run of an undulated domain is refused at the C ABI (the
+    library raises before any kernel runs)."""
     from axisem3d_amd import capi
     from axisem3d_amd.solver import GpuSolver
-    sp = _box()
-    src = int(sp.src_elem)
-    sp.prt = {"elems": np.concatenate([sp.prt["elems"], [src]]).astype(
-        np.int32), "x": np.concatenate([sp.prt["x"], sp.prt["x"][:1]])}
-    with pytest.raises(capi.Axs3dError, match="undulated"):
-        GpuSolver(sp, "fp64")
     g = GpuSolver(_box(), "fp64")
     with pytest.raises(capi.Axs3dError, match="PRT"):
         g.adjoint_run(np.zeros((10, 4, 3)))

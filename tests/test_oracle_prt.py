@@ -201,3 +201,19 @@ def test_undulation_refuses_fluids_and_boundaries():
     preloop.add_absorbing(sp, keys=("RIGHT",))
     with pytest.raises(ValueError, match="solid domains"):
         preloop.add_undulation(sp, lambda s, z: 0.01 * z)
+
+
+def test_identity_prt_leaves_the_moment_source_bitwise():
+    # the source's PRT branch (SolidElement.cpp:293-331) with X = 1, no
+    # rotation: the 9-component quadrature of the mirrored stress
+    sp = _box()
+    mt = np.array([1.0, 0.7, -0.4, 0.3, 0.2, -0.1]) * 1e15
+    preloop.add_axial_moment_source(sp, 2.0e3, mt, 0.26, 1.628, 50)
+    o0 = oracle.Oracle(sp)
+    f0 = o0.apply_source(np.zeros((o0.nmodes, 3), complex), 20)
+    preloop.add_undulation(sp, np.zeros((sp.n_solid_elements, 25)),
+                           elems=np.arange(sp.n_solid_elements))
+    o1 = oracle.Oracle(sp)
+    f1 = o1.apply_source(np.zeros((o1.nmodes, 3), complex), 20)
+    assert np.abs(f0).max() > 0
+    assert np.array_equal(f0, f1)
