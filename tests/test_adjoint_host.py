@@ -124,3 +124,26 @@ def test_with_model_rewrites_general_material_arrays():
         pass
     else:
         raise AssertionError("TI media must be refused")
+
+
+def test_with_media_round_trip_and_mass():
+    """with_media on a general-media spec's own (mat1d, rho, prop3d)
+    reproduces its material arrays and point masses; doubling rho doubles
+    every mass (GLLPoint.cpp:54-77)."""
+    import numpy as np
+    from axisem3d_amd import adjoint, preloop
+    sp = preloop.cartesian_homogeneous(
+        nx=4, nz=4, width=4e3, depth=4e3, nsteps=10, src_depth=2e3,
+        n_receivers=2, nodal_nr=lambda s, z: np.full(s.shape, 5.0))
+    E = sp.n_solid_elements
+    preloop.make_media(sp, (np.arange(E) % 3).astype(np.uint8), field=None,
+                       mask3d=(np.arange(E) % 2) == 1, att=None, seed=3)
+    media = (np.array(sp.mat1d), np.array(sp._rho1d),
+             np.array(sp.prop3d).reshape(-1))
+    s2 = adjoint.with_media(sp, *media)
+    assert np.array_equal(s2.mat1d, sp.mat1d)
+    assert np.array_equal(np.asarray(s2.prop3d).reshape(-1), media[2])
+    assert np.allclose(s2.solid_mass, sp.solid_mass, rtol=1e-14)
+    s3 = adjoint.with_media(sp, media[0], 2.0 * media[1], media[2])
+    assert np.allclose(s3.solid_mass, 2.0 * np.asarray(sp.solid_mass),
+                       rtol=1e-14)
